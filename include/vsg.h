@@ -1,0 +1,210 @@
+/*
+ * vsg.h -- C ABI of the MI355X (gfx950) VoxelSG depth-fusion library.
+ *
+ * The library replaces the hot path of the reference's
+ *   VoxelScene::integrateFromScanData   (VoxelSG/include/VoxelScene.h:89-91)
+ *   RayCastRender::initialize / render  (VoxelSG/include/RayCastRender.h:56,66-70)
+ * and the launch wrappers under them (VoxelSG/include/VoxelScene.h:13-42,
+ * VoxelSG/src/RayCastRender.cpp:28-45).  Those wrappers are extern "C" in the
+ * reference but take C++ references to CUDAHashRef/Params; this header is a true
+ * C ABI: plain pointers, sizes and POD structs, usable from C, C++, ctypes or cgo.
+ * include/vsg_voxelscene.hpp keeps the reference's C++ class API on top of it.
+ *
+ * Conventions (identical to the reference):
+ *   - pose: row-major float[16] camera->world [R|t; 0 0 0 1]; camera position = t.
+ *   - pointmap: float3 per pixel in camera coordinates (x right, y down, z ahead),
+ *     row-major W*H; a pixel is invalid when its point is (0,0,0).
+ *   - colormap: uchar3 per pixel; normalmap: float3 per pixel (not used by the
+ *     update, as in the reference).
+ *   - frame buffers are DEVICE pointers owned by the caller; the scene owns the
+ *     hash table and voxel store; a renderer owns its output buffers.
+ *
+ * Every call is asynchronous on the object's HIP stream (default: a stream the
+ * library creates) except the ones documented as synchronising.  One scene per
+ * host thread; calls on one scene must not overlap.
+ */
+#ifndef VSG_H
+#define VSG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum vsg_status {
+  VSG_OK = 0,
+  VSG_ERR_INVALID_ARG = 1, /* null pointer, bad size, or parameters out of range   */
+  VSG_ERR_OOM = 2,         /* device allocation failed                              */
+  VSG_ERR_HIP = 3,         /* a HIP runtime call or kernel launch failed            */
+  VSG_ERR_NOT_READY = 4    /* object not initialised (RayCastRender.cpp:128-131)    */
+} vsg_status;
+
+typedef enum vsg_store {
+  VSG_STORE_FP32 = 0, /* f32 SDF + u16 weight per voxel (8 B) + uchar4 colour     */
+  VSG_STORE_FP16 = 1  /* f16 SDF + u16 weight per voxel (4 B) + uchar4 colour     */
+} vsg_store;
+
+/* Params (VoxelSG/include/CUDAHashData.h:9-42).  Block size 8 and bucket size 10
+ * are fixed (VoxelSG/include/globalPramasConfig.h:8-9). */
+typedef struct vsg_params {
+  int32_t hash_slot_num; /* buckets B (Params::hashSlotNum)                        */
+  int32_t sdf_block_num; /* voxel blocks N (Params::SDFBlockNum)                   */
+  float voxel_size;      /* metres (Params::voxelSize)                             */
+  int32_t store;         /* vsg_store                                              */
+  int32_t reserved[4];   /* must be zero                                           */
+} vsg_params;
+
+typedef struct vsg_intrinsics {
+  float fx, fy, cx, cy;
+} vsg_intrinsics;
+
+typedef struct vsg_frame {
+  const void* points;  /* device float3[W*H]                                     */
+  const void* colors;  /* device uchar3[W*H]                                     */
+  const void* normals; /* device float3[W*H], may be NULL                        */
+  int32_t width, height;
+} vsg_frame;
+
+/* Counters of the last integrate / render call (reference prints these:
+ * VoxelSG/src/VoxelScene.cpp:487-532,708-714, VoxelSG/src/RayCastRender.cu:599-616). */
+typedef struct vsg_counters {
+  uint32_t valid_pixels;   /* pixels with z > 1e-6 (allocation kernel's counter)  */
+  uint32_t new_blocks;     /* blocks inserted by the last integrate               */
+  uint32_t heap_fail;      /* bucket winners that found the block heap empty      */
+  uint32_t num_allocated;  /* live blocks after the last integrate                */
+  uint32_t visible_blocks; /* blocks the last integrate visited                   */
+  uint32_t updated_voxels; /* voxels the last integrate updated                   */
+  uint32_t claimed_buckets;/* buckets with at least one insertion request         */
+  uint32_t out_of_range;   /* requests for blocks beyond +-2^20 block indices     */
+  uint32_t frames;         /* integrate calls since reset                         */
+  uint32_t reserved[7];
+} vsg_counters;
+
+typedef struct vsg_scene vsg_scene;
+typedef struct vsg_render vsg_render;
+
+/* ---- library ------------------------------------------------------------------ */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int32_t vsg_version(void);
+
+/* Last error message of this host thread (static storage, never NULL). */
+const char* vsg_last_error(void);
+
+/* ---- scene: VoxelScene (VoxelSG/include/VoxelScene.h:52-209) -------------------- */
+
+/* VoxelScene::VoxelScene(const Params&) + allocate() + reset()
+ * (VoxelSG/src/VoxelScene.cpp:67-84,97-127,276-302).  Synchronising. */
+vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out);
+
+/* VoxelScene::~VoxelScene (VoxelSG/src/VoxelScene.cpp:89-92).  Synchronising. */
+vsg_status vsg_scene_destroy(vsg_scene* scene);
+
+/* VoxelScene::reset (VoxelSG/src/VoxelScene.cpp:276-302): empty hash table, full
+ * block heap, zero voxels, frame count 0.  Asynchronous. */
+vsg_status vsg_scene_reset(vsg_scene* scene);
+
+/* Stream the scene's work is queued on (hipStream_t).  The reference uses the
+ * legacy default stream and synchronises after every kernel; this library never
+ * synchronises inside integrate/render. */
+vsg_status vsg_scene_set_stream(vsg_scene* scene, void* hip_stream);
+vsg_status vsg_scene_get_stream(const vsg_scene* scene, void** hip_stream);
+
+/* VoxelScene::integrateFromScanData (VoxelSG/include/VoxelScene.h:89-91,
+ * VoxelSG/src/VoxelScene.cpp:404-833): allocate the blocks along every valid
+ * pixel's truncation band, list the blocks the frame can reach, integrate the
+ * frame into them.  pose: HOST float[16] row-major camera->world (the reference
+ * takes a cv::Mat and uploads it, VoxelScene.cpp:431-453).  Asynchronous. */
+vsg_status vsg_integrate(vsg_scene* scene, const vsg_frame* frame, float truncation, const float pose[16],
+                         const vsg_intrinsics* intrinsics);
+
+/* Counters of the last integrate (synchronises the scene's stream). */
+vsg_status vsg_scene_counters(vsg_scene* scene, vsg_counters* out);
+
+/* VoxelScene::getHeapFreeCount (VoxelSG/src/VoxelScene.cpp:370-375): heap counter
+ * + 1, i.e. N - allocated blocks.  Synchronising. */
+vsg_status vsg_heap_free_count(vsg_scene* scene, uint32_t* out);
+
+/* VoxelScene::getNumIntegratedFrames (VoxelSG/include/VoxelScene.h:104-106). */
+vsg_status vsg_num_integrated_frames(const vsg_scene* scene, uint32_t* out);
+
+/* Params the scene was created with (VoxelScene::getParams, VoxelScene.h:133-153). */
+vsg_status vsg_scene_params(const vsg_scene* scene, vsg_params* out);
+
+/* Allocated blocks in pointer order (block k has pointer k), copied to HOST
+ * memory: coords[3*k..3*k+2] = block coordinate, slots[k] = hash entry index
+ * (bucket*10 + slot).  The reference downloads its compacted list the same way
+ * (VoxelSG/src/VoxelScene.cpp:540-560).  *count receives the number of
+ * allocated blocks; at most `capacity` are written.  Synchronising. */
+vsg_status vsg_export_blocks(vsg_scene* scene, int32_t* coords, int32_t* slots, uint32_t capacity,
+                             uint32_t* count);
+
+/* Voxels of blocks [first, first+nblocks) to HOST memory, 512 per block in the
+ * reference's order (x fastest, VoxelScene.cu:955-957): sdf as float (FP16
+ * store widened), weight, colour rgb + flags byte (bit0 isUpdated, bit1
+ * isZeroCrossing, CUDAHashRef.h:23-45).  Any output may be NULL.  Synchronising. */
+vsg_status vsg_export_voxels(vsg_scene* scene, uint32_t first, uint32_t nblocks, float* sdf, uint16_t* weight,
+                             uint8_t* rgbf);
+
+/* ---- renderer: RayCastRender (VoxelSG/include/RayCastRender.h:20-123) ----------- */
+
+/* RayCastRender::initialize(w, h) (VoxelSG/src/RayCastRender.cpp:88-118):
+ * allocates depth (float), position, colour and normal (float4) outputs. */
+vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out);
+vsg_status vsg_render_destroy(vsg_render* render);
+vsg_status vsg_render_set_stream(vsg_render* render, void* hip_stream);
+
+/* RayCastRender::render (VoxelSG/src/RayCastRender.cpp:123-183,
+ * VoxelSG/src/RayCastRender.cu:407-620): one ray per pixel, marched in half-voxel
+ * steps over [min_depth, max_depth] (z-depth), zero crossing refined by three
+ * regula-falsi steps.  cam_pos and pose are HOST arrays (the reference passes the
+ * pose as a device pointer; the C++ shim copies it).  voxel_size <= 0 uses the
+ * scene's.  Work is queued on the renderer's stream after the scene's pending
+ * work.  When the scene holds no block the outputs are left untouched
+ * (RayCastRender.cu:553-556).  Asynchronous. */
+vsg_status vsg_render_frame(vsg_render* render, vsg_scene* scene, const float cam_pos[3], const float pose[16],
+                            const vsg_intrinsics* intrinsics, float min_depth, float max_depth, float voxel_size);
+
+/* Device output buffers (RayCastRender::getOutputDepth/Color/Normal/Positions,
+ * RayCastRender.h:110-113): depth is float[W*H] (z-depth, 1e10 = no hit);
+ * positions, colours and normals are float4[W*H]. */
+vsg_status vsg_render_buffers(const vsg_render* render, float** depth, float** positions, float** colors,
+                              float** normals);
+
+/* downloadDepthFloat / downloadResults (RayCastRender.cpp:188-221) to HOST
+ * memory; any output may be NULL.  Synchronising. */
+vsg_status vsg_render_download(vsg_render* render, float* depth, float* positions, float* colors, float* normals);
+
+/* Hit / miss counts of the last render (RayCastRender.cu:599-616).  Synchronising. */
+vsg_status vsg_render_counters(vsg_render* render, uint32_t* hits, uint32_t* misses);
+
+/* ---- kernel timing ---------------------------------------------------------- */
+
+/* Kernel ids for vsg_timing_read. */
+enum {
+  VSG_K_ALLOC_CLAIM = 0,
+  VSG_K_ALLOC_RESOLVE = 1,
+  VSG_K_ALLOC_SCAN = 2,
+  VSG_K_ALLOC_INSERT = 3,
+  VSG_K_COMPACT = 4,
+  VSG_K_INTEGRATE = 5,
+  VSG_K_RAYCAST = 6,
+  VSG_K_RESET = 7,
+  VSG_K_COUNT = 8
+};
+
+/* When enabled, every kernel launch is bracketed by HIP events on the stream it
+ * runs on (no host synchronisation).  vsg_timing_read synchronises, folds the
+ * recorded events into per-kernel totals and returns, per kernel id, the total
+ * milliseconds and the launch count since the last vsg_timing_reset. */
+vsg_status vsg_timing_enable(int32_t on);
+vsg_status vsg_timing_reset(void);
+vsg_status vsg_timing_read(double ms[VSG_K_COUNT], uint32_t launches[VSG_K_COUNT]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VSG_H */

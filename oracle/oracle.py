@@ -1,0 +1,211 @@
+"""ctypes binding of the VoxelSG oracle (oracle/libvsg_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker and the CPU baseline; the product path never
+imports it.  The oracle restates the reference's kernels on the CPU
+(oracle/vsg_oracle.hpp); float arithmetic for parity, double for precision.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+ORACLE_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(ORACLE_DIR, "libvsg_oracle.so")
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+    return LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run make in {ORACLE_DIR}")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.vsgo_create.restype = vp
+        L.vsgo_create.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_int, C.c_int]
+        L.vsgo_destroy.argtypes = [vp]
+        L.vsgo_reset.argtypes = [vp]
+        L.vsgo_integrate.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_int, vp]
+        L.vsgo_alloc.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double, vp]
+        L.vsgo_render.restype = C.c_int
+        L.vsgo_render.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, vp, vp, vp, vp, vp, vp, C.c_long]
+        L.vsgo_num_allocated.restype = C.c_uint64
+        L.vsgo_num_allocated.argtypes = [vp]
+        L.vsgo_heap_counter.restype = C.c_uint32
+        L.vsgo_heap_counter.argtypes = [vp]
+        L.vsgo_export_blocks.restype = C.c_uint64
+        L.vsgo_export_blocks.argtypes = [vp, vp, vp, vp, C.c_uint64]
+        L.vsgo_export_voxels.argtypes = [vp, C.c_uint64, vp, vp, vp, vp]
+        L.vsgo_hash.restype = C.c_uint32
+        L.vsgo_hash.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
+        L.vsgo_alloc_world_to_block.argtypes = [vp, C.c_float, vp, vp]
+        L.vsgo_ray_world_to_voxel.argtypes = [vp, C.c_float, vp, vp, vp]
+        L.vsgo_dda.restype = C.c_int
+        L.vsgo_dda.argtypes = [vp, vp, C.c_float, vp, C.c_int]
+        L.vsgo_f32_to_f16.restype = C.c_uint16
+        L.vsgo_f32_to_f16.argtypes = [C.c_float]
+        L.vsgo_f16_to_f32.restype = C.c_float
+        L.vsgo_f16_to_f32.argtypes = [C.c_uint16]
+        L.vsgo_trilinear.restype = C.c_int
+        L.vsgo_trilinear.argtypes = [vp, vp, C.c_float, vp, vp]
+        L.vsgo_depth_range.argtypes = [vp, C.c_long, C.c_long, vp]
+        _lib = L
+    return _lib
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pose(T) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(T, dtype=np.float64).reshape(16))
+
+
+COUNTER_NAMES = ["valid_pixels", "new_blocks", "heap_fail", "num_allocated", "visible", "updated",
+                 "heap_counter", "frames"]
+
+
+class OracleScene:
+    """The reference's hash/voxel state on the CPU (float or double arithmetic)."""
+
+    def __init__(self, num_buckets: int, num_blocks: int, voxel: float, store_fp16: bool = False,
+                 fp64: bool = False):
+        self.voxel = voxel
+        self.fp64 = fp64
+        self._h = C.c_void_p(lib().vsgo_create(num_buckets, num_blocks, voxel, int(store_fp16), int(fp64)))
+
+    def __del__(self):
+        try:
+            if self._h.value:
+                lib().vsgo_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    def reset(self) -> None:
+        lib().vsgo_reset(self._h)
+
+    def integrate(self, points, colors, trunc: float, pose, intr, visible_only: bool = False) -> dict:
+        pts = _f32(points)
+        H, W = pts.shape[0], pts.shape[1]
+        rgb = np.ascontiguousarray(colors, dtype=np.uint8)
+        ctr = np.zeros(8, np.uint32)
+        fx, fy, cx, cy = intr
+        lib().vsgo_integrate(self._h, pts.ctypes.data, rgb.ctypes.data, W, H, float(trunc), _pose(pose).ctypes.data,
+                             fx, fy, cx, cy, int(visible_only), ctr.ctypes.data)
+        return dict(zip(COUNTER_NAMES, (int(v) for v in ctr)))
+
+    def alloc(self, points, trunc: float, pose) -> None:
+        pts = _f32(points)
+        H, W = pts.shape[0], pts.shape[1]
+        lib().vsgo_alloc(self._h, pts.ctypes.data, W, H, float(trunc), _pose(pose).ctypes.data)
+
+    def render(self, W: int, H: int, cam, pose, intr, minD: float, maxD: float, voxel: float | None = None,
+               init: float = np.nan, pixels=None) -> dict:
+        """Ray casts every pixel, or only `pixels` (int indices; the rest keep `init`)."""
+        depth = np.full(W * H, init, np.float32)
+        pos = np.full((W * H, 4), init, np.float32)
+        col = np.full((W * H, 4), init, np.float32)
+        nrm = np.full((W * H, 4), init, np.float32)
+        hm = np.zeros(2, np.uint32)
+        camd = np.ascontiguousarray(np.asarray(cam, np.float64).reshape(3))
+        fx, fy, cx, cy = intr
+        plist = None if pixels is None else np.ascontiguousarray(pixels, dtype=np.int32)
+        ran = lib().vsgo_render(self._h, W, H, camd.ctypes.data, _pose(pose).ctypes.data, fx, fy, cx, cy, minD, maxD,
+                                self.voxel if voxel is None else voxel, depth.ctypes.data, pos.ctypes.data,
+                                col.ctypes.data, nrm.ctypes.data, hm.ctypes.data,
+                                None if plist is None else plist.ctypes.data, 0 if plist is None else plist.size)
+        return {"depth": depth, "pos": pos, "col": col, "nrm": nrm, "hits": int(hm[0]), "misses": int(hm[1]),
+                "ran": bool(ran)}
+
+    def num_allocated(self) -> int:
+        return int(lib().vsgo_num_allocated(self._h))
+
+    def heap_counter(self) -> int:
+        return int(lib().vsgo_heap_counter(self._h))
+
+    def export_blocks(self):
+        """(coords int32 [n,3], ptrs int32 [n], hash entry indices int32 [n]) in pointer order."""
+        n = int(lib().vsgo_export_blocks(self._h, None, None, None, 0))
+        coords = np.zeros((n, 3), np.int32)
+        ptrs = np.zeros(n, np.int32)
+        slots = np.zeros(n, np.int32)
+        if n:
+            lib().vsgo_export_blocks(self._h, coords.ctypes.data, ptrs.ctypes.data, slots.ctypes.data, n)
+        return coords, ptrs, slots
+
+    def export_voxels(self, nblocks: int | None = None):
+        """(sdf float64 [n*512], weight uint8, colour uint8 [n*512,3], flags uint8) for pointers [0, n)."""
+        if nblocks is None:
+            nblocks = self.num_allocated()
+        nv = nblocks * 512
+        sdf = np.zeros(nv, np.float64)
+        w = np.zeros(nv, np.uint8)
+        c = np.zeros((nv, 3), np.uint8)
+        f = np.zeros(nv, np.uint8)
+        if nblocks:
+            lib().vsgo_export_voxels(self._h, nblocks, sdf.ctypes.data, w.ctypes.data, c.ctypes.data, f.ctypes.data)
+        return sdf, w, c, f
+
+    def trilinear(self, p, voxel: float | None = None):
+        p3 = _f32(np.asarray(p).reshape(3))
+        d = C.c_float()
+        col = np.zeros(3, np.uint8)
+        ok = lib().vsgo_trilinear(self._h, p3.ctypes.data, self.voxel if voxel is None else voxel, C.byref(d),
+                                  col.ctypes.data)
+        return bool(ok), float(d.value), col
+
+
+def hash_bucket(x: int, y: int, z: int, num_buckets: int) -> int:
+    return int(lib().vsgo_hash(x, y, z, num_buckets))
+
+
+def alloc_world_to_block(p, v: float):
+    p3 = _f32(np.asarray(p).reshape(3))
+    vox = np.zeros(3, np.int32)
+    blk = np.zeros(3, np.int32)
+    lib().vsgo_alloc_world_to_block(p3.ctypes.data, v, vox.ctypes.data, blk.ctypes.data)
+    return vox, blk
+
+
+def ray_world_to_voxel(p, v: float):
+    p3 = _f32(np.asarray(p).reshape(3))
+    vox = np.zeros(3, np.int32)
+    blk = np.zeros(3, np.int32)
+    loc = np.zeros(3, np.int32)
+    lib().vsgo_ray_world_to_voxel(p3.ctypes.data, v, vox.ctypes.data, blk.ctypes.data, loc.ctypes.data)
+    return vox, blk, loc
+
+
+def dda(a, b, v: float, cap: int = 2048) -> np.ndarray:
+    a3, b3 = _f32(np.asarray(a).reshape(3)), _f32(np.asarray(b).reshape(3))
+    out = np.zeros((cap, 3), np.int32)
+    n = lib().vsgo_dda(a3.ctypes.data, b3.ctypes.data, v, out.ctypes.data, cap)
+    return out[:min(n, cap)].copy()
+
+
+def f32_to_f16(x: float) -> int:
+    return int(lib().vsgo_f32_to_f16(x))
+
+
+def f16_to_f32(h: int) -> float:
+    return float(lib().vsgo_f16_to_f32(h))
+
+
+def depth_range(depth) -> tuple[float, float]:
+    d = _f32(np.asarray(depth).reshape(-1))
+    out = np.zeros(2, np.float32)
+    lib().vsgo_depth_range(d.ctypes.data, d.size, 1, out.ctypes.data)
+    return float(out[0]), float(out[1])

@@ -1,0 +1,235 @@
+// C API over the VoxelSG oracle (vsg_oracle.hpp) for the Python test harness.
+//
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
+// bench.py's cpu_baseline leg as the checker / CPU baseline, never by the
+// product path.  Poses and intrinsics cross this API as double; the FP32
+// oracle rounds them to float (exact for values that came from float).
+#include <algorithm>
+#include <cstdint>
+#include <utility>
+#include <vector>
+
+#include "vsg_oracle.hpp"
+
+using namespace vsgo;
+
+namespace {
+
+struct Handle {
+  int fp64 = 0;
+  Scene<float> sf;
+  Scene<double> sd;
+};
+
+template <class R> void to_pose(const double* T, R* M) {
+  for (int i = 0; i < 16; ++i) M[i] = (R)T[i];
+}
+
+template <class R>
+void do_integrate(Scene<R>& s, const float* pts, const uint8_t* rgb, int W, int H, double trunc, const double* T,
+                  double fx, double fy, double cx, double cy, int visible_only, uint32_t* ctr) {
+  R M[16];
+  to_pose(T, M);
+  integrate_frame(s, pts, rgb, W, H, (R)trunc, M, (R)fx, (R)fy, (R)cx, (R)cy, visible_only);
+  if (ctr) {
+    ctr[0] = s.c_valid_pixels;
+    ctr[1] = s.c_new_blocks;
+    ctr[2] = s.c_heap_fail;
+    ctr[3] = (uint32_t)s.num_allocated();
+    ctr[4] = s.c_visible;
+    ctr[5] = s.c_updated;
+    ctr[6] = s.heap_counter;
+    ctr[7] = s.frames;
+  }
+}
+
+template <class R>
+int do_render(const Scene<R>& s, int W, int H, const double* cam3, const double* T, double fx, double fy, double cx,
+              double cy, double minD, double maxD, double voxel, float* depth, float* pos, float* col, float* nrm,
+              uint32_t* hm, const int32_t* pix_list, long n_list) {
+  R M[16];
+  to_pose(T, M);
+  V3<R> cam{(R)cam3[0], (R)cam3[1], (R)cam3[2]};
+  return render(s, W, H, cam, M, (R)fx, (R)fy, (R)cx, (R)cy, (R)minD, (R)maxD, (R)voxel, depth, pos, col, nrm, &hm[0],
+                &hm[1], pix_list, n_list);
+}
+
+template <class R> uint64_t do_export_blocks(const Scene<R>& s, int32_t* coords, int32_t* ptrs, int32_t* slots,
+                                             uint64_t cap) {
+  std::vector<std::pair<int32_t, size_t>> live;
+  for (size_t i = 0; i < s.slot_ptr.size(); ++i)
+    if (s.slot_ptr[i] != -1) live.push_back({s.slot_ptr[i], i});
+  std::sort(live.begin(), live.end());
+  uint64_t n = std::min<uint64_t>(cap, live.size());
+  for (uint64_t k = 0; k < n; ++k) {
+    const I3& p = s.slot_pos[live[k].second];
+    if (coords) {
+      coords[3 * k] = p.x;
+      coords[3 * k + 1] = p.y;
+      coords[3 * k + 2] = p.z;
+    }
+    if (ptrs) ptrs[k] = live[k].first;
+    if (slots) slots[k] = (int32_t)live[k].second;
+  }
+  return live.size();
+}
+
+template <class R> void do_export_voxels(const Scene<R>& s, uint64_t nblocks, double* sdf, uint8_t* w, uint8_t* c3,
+                                         uint8_t* fl) {
+  size_t have = s.weight.size();
+  for (size_t i = 0; i < (size_t)nblocks * kVox; ++i) {
+    bool ok = i < have;
+    if (sdf) sdf[i] = ok ? (double)s.load_sdf(i) : 0.0;
+    if (w) w[i] = ok ? s.weight[i] : 0;
+    if (c3) {
+      for (int ch = 0; ch < 3; ++ch) c3[3 * i + ch] = ok ? s.color[3 * i + ch] : 0;
+    }
+    if (fl) fl[i] = ok ? s.flags[i] : 0;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void* vsgo_create(uint32_t num_buckets, uint32_t num_blocks, double voxel, int store_fp16, int fp64) {
+  Handle* h = new Handle();
+  h->fp64 = fp64;
+  if (fp64) h->sd.init(num_buckets, num_blocks, voxel, store_fp16);
+  else h->sf.init(num_buckets, num_blocks, (float)voxel, store_fp16);
+  return h;
+}
+
+void vsgo_destroy(void* p) { delete (Handle*)p; }
+
+void vsgo_reset(void* p) {
+  Handle* h = (Handle*)p;
+  if (h->fp64) h->sd.reset(); else h->sf.reset();
+}
+
+// counters: valid_pixels, new_blocks, heap_fail, num_allocated, visible, updated, heap_counter, frames
+void vsgo_integrate(void* p, const float* pts, const uint8_t* rgb, int W, int H, double trunc, const double* T16,
+                    double fx, double fy, double cx, double cy, int visible_only, uint32_t* counters) {
+  Handle* h = (Handle*)p;
+  if (h->fp64) do_integrate(h->sd, pts, rgb, W, H, trunc, T16, fx, fy, cx, cy, visible_only, counters);
+  else do_integrate(h->sf, pts, rgb, W, H, trunc, T16, fx, fy, cx, cy, visible_only, counters);
+}
+
+// Allocation step only (for allocation-level tests).
+void vsgo_alloc(void* p, const float* pts, int W, int H, double trunc, const double* T16) {
+  Handle* h = (Handle*)p;
+  if (h->fp64) {
+    double M[16];
+    to_pose(T16, M);
+    alloc_frame(h->sd, pts, W, H, trunc, M);
+  } else {
+    float M[16];
+    to_pose(T16, M);
+    alloc_frame(h->sf, pts, W, H, (float)trunc, M);
+  }
+}
+
+// pix_list (optional, n_list entries): trace only these pixel indices.
+int vsgo_render(void* p, int W, int H, const double* cam3, const double* T16, double fx, double fy, double cx,
+                double cy, double minD, double maxD, double voxel, float* depth, float* pos, float* col, float* nrm,
+                uint32_t* hits_misses, const int32_t* pix_list, long n_list) {
+  Handle* h = (Handle*)p;
+  if (h->fp64)
+    return do_render(h->sd, W, H, cam3, T16, fx, fy, cx, cy, minD, maxD, voxel, depth, pos, col, nrm, hits_misses,
+                     pix_list, n_list);
+  return do_render(h->sf, W, H, cam3, T16, fx, fy, cx, cy, minD, maxD, voxel, depth, pos, col, nrm, hits_misses,
+                   pix_list, n_list);
+}
+
+uint64_t vsgo_num_allocated(void* p) {
+  Handle* h = (Handle*)p;
+  return h->fp64 ? h->sd.num_allocated() : h->sf.num_allocated();
+}
+
+uint32_t vsgo_heap_counter(void* p) {
+  Handle* h = (Handle*)p;
+  return h->fp64 ? h->sd.heap_counter : h->sf.heap_counter;
+}
+
+uint64_t vsgo_export_blocks(void* p, int32_t* coords, int32_t* ptrs, int32_t* slots, uint64_t cap) {
+  Handle* h = (Handle*)p;
+  return h->fp64 ? do_export_blocks(h->sd, coords, ptrs, slots, cap) : do_export_blocks(h->sf, coords, ptrs, slots, cap);
+}
+
+void vsgo_export_voxels(void* p, uint64_t nblocks, double* sdf, uint8_t* weight, uint8_t* color3, uint8_t* flags) {
+  Handle* h = (Handle*)p;
+  if (h->fp64) do_export_voxels(h->sd, nblocks, sdf, weight, color3, flags);
+  else do_export_voxels(h->sf, nblocks, sdf, weight, color3, flags);
+}
+
+// ---- unit-level entry points (float arithmetic, as the reference kernels)
+uint32_t vsgo_hash(int32_t x, int32_t y, int32_t z, uint32_t num_buckets) {
+  return hash_bucket(I3{x, y, z}, num_buckets);
+}
+
+void vsgo_alloc_world_to_block(const float* p3, float v, int32_t* voxel3, int32_t* block3) {
+  V3<float> p{p3[0], p3[1], p3[2]};
+  voxel3[0] = alloc_voxel_axis(p.x, v);
+  voxel3[1] = alloc_voxel_axis(p.y, v);
+  voxel3[2] = alloc_voxel_axis(p.z, v);
+  I3 b = alloc_world_to_block(p, v);
+  block3[0] = b.x;
+  block3[1] = b.y;
+  block3[2] = b.z;
+}
+
+void vsgo_ray_world_to_voxel(const float* p3, float v, int32_t* voxel3, int32_t* block3, int32_t* local3) {
+  int32_t vv[3] = {ray_voxel_axis(p3[0], v), ray_voxel_axis(p3[1], v), ray_voxel_axis(p3[2], v)};
+  for (int k = 0; k < 3; ++k) {
+    voxel3[k] = vv[k];
+    block3[k] = voxel_to_block_axis(vv[k]);
+    local3[k] = voxel_local_axis(vv[k]);
+  }
+}
+
+int vsgo_dda(const float* a3, const float* b3, float v, int32_t* out, int cap) {
+  int n = 0;
+  dda_band(V3<float>{a3[0], a3[1], a3[2]}, V3<float>{b3[0], b3[1], b3[2]}, v, [&](I3 c, int) {
+    if (n < cap) {
+      out[3 * n] = c.x;
+      out[3 * n + 1] = c.y;
+      out[3 * n + 2] = c.z;
+    }
+    n++;
+  });
+  return n;
+}
+
+uint16_t vsgo_f32_to_f16(float f) { return f32_to_f16(f); }
+float vsgo_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
+
+// Trilinear sample of the current store at world point p3 (float oracle only).
+int vsgo_trilinear(void* p, const float* p3, float v, float* dist, uint8_t* col3) {
+  Handle* h = (Handle*)p;
+  if (h->fp64) {
+    double d = 0;
+    int ok = trilinear(h->sd, V3<double>{p3[0], p3[1], p3[2]}, (double)v, &d, col3) ? 1 : 0;
+    *dist = (float)d;
+    return ok;
+  }
+  return trilinear(h->sf, V3<float>{p3[0], p3[1], p3[2]}, v, dist, col3) ? 1 : 0;
+}
+
+// Driver's per-frame depth range (VoxelSG/src/voxelSG.cpp:594-608), float arithmetic.
+void vsgo_depth_range(const float* depth, long n, long stride, float* out2) {
+  float mn = 1e10f, mx = -1e10f;
+  for (long i = 0; i < n; ++i) {
+    float d = depth[i * stride];
+    if (d > 0.0f) {
+      mn = std::fmin(mn, d);
+      mx = std::fmax(mx, d);
+    }
+  }
+  float margin = (mx - mn) * 0.1f;
+  mn = std::fmax(0.01f, mn - margin);
+  mx += margin;
+  out2[0] = mn;
+  out2[1] = mx;
+}
+
+}  // extern "C"

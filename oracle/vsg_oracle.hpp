@@ -1,0 +1,637 @@
+// VoxelSG depth-fusion oracle: a CPU restatement of the reference algorithm.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product path links, loads or calls
+// this code; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg use it, and only as the checker (or as the timed CPU baseline).
+//
+// It restates, operation by operation, what the reference's CUDA kernels do
+// (all citations are relative to the reference repository root):
+//   hash ........................ VoxelSG/src/VoxelScene.cu:61-67, RayCastRender.cu:47-53
+//   world->block (allocator) .... VoxelSG/src/VoxelScene.cu:73-96
+//   bucket insert (allocBlock) .. VoxelSG/src/VoxelScene.cu:137-226
+//   heap / table init ........... VoxelSG/src/VoxelScene.cu:250-284,1166-1236
+//   band DDA .................... VoxelSG/src/VoxelScene.cu:295-481
+//   per-pixel allocation ........ VoxelSG/src/VoxelScene.cu:487-633
+//   compaction .................. VoxelSG/src/VoxelScene.cu:1275-1300
+//   voxel integration ........... VoxelSG/src/VoxelScene.cu:918-1146
+//   raycast ..................... VoxelSG/src/RayCastRender.cu:13-517
+//   depth-range rule ............ VoxelSG/src/voxelSG.cpp:594-608
+//
+// Conventions this oracle fixes where the reference leaves them open:
+//   * Allocation runs as a sequential loop over pixels in row-major order and,
+//     per pixel, over DDA steps in order.  In the reference the winner of a
+//     contended bucket is whichever GPU thread takes the bucket lock first; a
+//     sequential loop makes the first request in (pixel, step) order the
+//     winner.  Heap pops happen in the same order, so block pointers are
+//     deterministic too.
+//   * No floating-point contraction: every a*b+c is a rounded product and a
+//     rounded sum (the build compiles this file with -ffp-contract=off).
+//   * float->int conversions saturate and map NaN to 0 (what the GPU's
+//     conversion instruction does; C++'s own cast is undefined there).
+//
+// Templated on the arithmetic type R: R=float is the parity oracle (bit-exact
+// with the GPU path), R=double is the precision oracle.  The voxel store can
+// hold the SDF as FP32 or as FP16 (with FP32/FP64 arithmetic around it).
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+namespace vsgo {
+
+constexpr int kBlock = 8;     // SDF_BLOCK_SIZE (VoxelSG/include/globalPramasConfig.h:8)
+constexpr int kBucket = 10;   // HASH_BUCKET_SIZE (VoxelSG/include/globalPramasConfig.h:9)
+constexpr int kVox = kBlock * kBlock * kBlock;
+
+template <class R> struct V3 { R x, y, z; };
+struct I3 {
+  int32_t x, y, z;
+  bool operator==(const I3& o) const { return x == o.x && y == o.y && z == o.z; }
+};
+
+// Saturating conversion with NaN -> 0, as the GPU's v_cvt_i32_f32 does.
+template <class R> inline int32_t cvt_i32(R x) {
+  if (x != x) return 0;
+  if (x >= R(2147483648.0)) return INT32_MAX;
+  if (x <= R(-2147483648.0)) return INT32_MIN;
+  return (int32_t)x;
+}
+
+template <class R> inline R sgn(R q) { return q > R(0) ? R(1) : (q < R(0) ? R(-1) : R(0)); }
+
+// ---- IEEE binary16 conversion (round to nearest even), used by the FP16 store.
+inline uint16_t f32_to_f16(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t ax = x & 0x7fffffffu;
+  if (ax >= 0x7f800000u) {  // inf or NaN
+    return (uint16_t)(sign | 0x7c00u | (ax > 0x7f800000u ? 0x200u | ((ax >> 13) & 0x3ffu) : 0u));
+  }
+  if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);  // rounds to >= 65520 -> inf
+  if (ax < 0x38800000u) {                                     // result is subnormal or zero
+    if (ax < 0x33000000u) return (uint16_t)sign;              // < 2^-25 rounds to 0
+    // value = m * 2^(e-150) with the implicit bit in m; the half subnormal unit
+    // is 2^-24, so the quotient is m >> (126 - e), rounded to nearest even.
+    uint32_t e = ax >> 23;
+    uint32_t m = (ax & 0x7fffffu) | 0x800000u;
+    uint32_t shift = 126u - e;
+    uint32_t q = m >> shift;
+    uint32_t rem = m & ((1u << shift) - 1u);
+    uint32_t half = 1u << (shift - 1u);
+    if (rem > half || (rem == half && (q & 1u))) q++;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t e = (ax >> 23) - 112u;  // rebias 127 -> 15
+  uint32_t m = ax & 0x7fffffu;
+  uint32_t q = (e << 10) | (m >> 13);
+  uint32_t rem = m & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
+  return (uint16_t)(sign | q);
+}
+
+inline float f16_to_f32(uint16_t h) {
+  uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+  uint32_t e = (h >> 10) & 0x1fu;
+  uint32_t m = h & 0x3ffu;
+  uint32_t x;
+  if (e == 0) {
+    if (m == 0) {
+      x = sign;
+    } else {  // subnormal: normalise
+      int k = -1;
+      do { k++; m <<= 1; } while (!(m & 0x400u));
+      x = sign | ((uint32_t)(112 - k) << 23) | ((m & 0x3ffu) << 13);
+    }
+  } else if (e == 31) {
+    x = sign | 0x7f800000u | (m << 13);
+  } else {
+    x = sign | ((e + 112u) << 23) | (m << 13);
+  }
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+// ---- A2: bucket hash, 32-bit wrap-around (VoxelScene.cu:61-67 == RayCastRender.cu:47-53)
+inline uint32_t hash_bucket(I3 b, uint32_t num_buckets) {
+  uint32_t h = ((uint32_t)b.x * 73856093u) ^ ((uint32_t)b.y * 19349663u) ^ ((uint32_t)b.z * 83492791u);
+  return h % num_buckets;
+}
+
+// ---- Q1: allocator world->voxel->block (VoxelScene.cu:73-96)
+template <class R> inline int32_t alloc_voxel_axis(R p, R v) {
+  R q = p / v;
+  return cvt_i32(q + sgn(q) * R(0.5f));  // (int) truncates toward zero
+}
+inline int32_t voxel_to_block_axis(int32_t vi) {
+  if (vi < 0) vi = (int32_t)((uint32_t)vi - (uint32_t)(kBlock - 1));
+  return vi / kBlock;  // C division truncates toward zero
+}
+template <class R> inline I3 alloc_world_to_block(V3<R> p, R v) {
+  return I3{voxel_to_block_axis(alloc_voxel_axis(p.x, v)), voxel_to_block_axis(alloc_voxel_axis(p.y, v)),
+            voxel_to_block_axis(alloc_voxel_axis(p.z, v))};
+}
+
+// ---- Q2: raycaster world->voxel (RayCastRender.cu:17-25), block (:28-34), local (:37-42)
+template <class R> inline int32_t ray_voxel_axis(R p, R v) {
+  R q = p / v;
+  return cvt_i32(std::floor(q + R(0.5f) * sgn(q)));
+}
+inline int32_t voxel_local_axis(int32_t vi) { return ((vi % kBlock) + kBlock) % kBlock; }
+
+// rigid transform with a row-major 4x4 camera->world matrix, each row summed left to right
+template <class R> inline V3<R> xform_point(const R* M, V3<R> p) {
+  return V3<R>{M[0] * p.x + M[1] * p.y + M[2] * p.z + M[3], M[4] * p.x + M[5] * p.y + M[6] * p.z + M[7],
+               M[8] * p.x + M[9] * p.y + M[10] * p.z + M[11]};
+}
+
+// ---- Voxel hash state (VoxelSG/include/CUDAHashRef.h:11-79, CUDAHashRef.cu:40-152)
+template <class R> struct Scene {
+  uint32_t num_buckets = 0;  // Params::hashSlotNum
+  uint32_t num_blocks = 0;   // Params::SDFBlockNum
+  R voxel = R(0);            // Params::voxelSize
+  int store_fp16 = 0;        // 0: FP32 SDF, 1: FP16 SDF
+
+  std::vector<I3> slot_pos;      // T = B*10 (HashSlot::pos)
+  std::vector<int32_t> slot_ptr; // T (HashSlot::ptr, -1 = free)
+  std::vector<uint32_t> heap;    // N+1
+  uint32_t heap_counter = 0;
+  std::vector<int32_t> mutex;    // B
+  std::vector<uint8_t> heap_failed;  // B: bucket saw an insertion fail for lack of blocks this frame
+
+  // voxel store, indexed ptr*512 + voxel, grown on demand (ptr < num_blocks)
+  std::vector<R> sdf;
+  std::vector<uint16_t> sdf_h;
+  std::vector<uint8_t> weight;
+  std::vector<uint8_t> color;  // 3 per voxel
+  std::vector<uint8_t> flags;  // bit0 isUpdated, bit1 isZeroCrossing
+  uint32_t frames = 0;
+
+  // per-frame counters
+  // c_heap_fail: buckets whose insertion failed for lack of blocks in the last frame
+  uint32_t c_valid_pixels = 0, c_new_blocks = 0, c_heap_fail = 0, c_updated = 0, c_visible = 0;
+
+  void init(uint32_t B, uint32_t N, R v, int fp16) {
+    num_buckets = B;
+    num_blocks = N;
+    voxel = v;
+    store_fp16 = fp16;
+    reset();
+  }
+
+  // resetHashDataCUDA (VoxelScene.cu:1166-1236): heap[i] = N-1-i, counter = N-1,
+  // table ptr=-1 / pos=0, mutexes 0; voxel memory zero (CUDAHashRef.cu:86-93).
+  void reset() {
+    slot_pos.assign((size_t)num_buckets * kBucket, I3{0, 0, 0});
+    slot_ptr.assign((size_t)num_buckets * kBucket, -1);
+    heap.assign((size_t)num_blocks + 1, 0u);
+    for (uint32_t i = 0; i < num_blocks; ++i) heap[i] = num_blocks - i - 1;
+    heap_counter = num_blocks - 1;
+    mutex.assign(num_buckets, 0);
+    heap_failed.assign(num_buckets, 0);
+    sdf.clear();
+    sdf_h.clear();
+    weight.clear();
+    color.clear();
+    flags.clear();
+    frames = 0;
+  }
+
+  void ensure_block(int32_t ptr) {
+    size_t need = ((size_t)ptr + 1) * kVox;
+    if (weight.size() >= need) return;
+    if (store_fp16) sdf_h.resize(need, 0); else sdf.resize(need, R(0));
+    weight.resize(need, 0);
+    color.resize(need * 3, 0);
+    flags.resize(need, 0);
+  }
+
+  R load_sdf(size_t i) const { return store_fp16 ? R(f16_to_f32(sdf_h[i])) : sdf[i]; }
+  void store_sdf(size_t i, R s) {
+    if (store_fp16) sdf_h[i] = f32_to_f16((float)s); else sdf[i] = s;
+  }
+
+  // ---- A6: allocBlock (VoxelScene.cu:137-226), executed sequentially
+  int32_t alloc_block(I3 c) {
+    uint32_t h = hash_bucket(c, num_buckets);
+    size_t hp = (size_t)h * kBucket;
+    long first_empty = -1;
+    for (int j = 0; j < kBucket; ++j) {
+      size_t i = hp + j;
+      if (slot_pos[i] == c && slot_ptr[i] != -1) return slot_ptr[i];
+      if (first_empty == -1 && slot_ptr[i] == -1) first_empty = (long)i;
+    }
+    if (first_empty != -1) {
+      int prev = mutex[h];
+      mutex[h] = 1;
+      if (prev != 1) {
+        uint32_t heap_index = heap_counter;  // atomicSub returns the old value
+        heap_counter -= 1u;
+        if (heap_index <= num_blocks && heap_index != 0xFFFFFFFFu) {
+          uint32_t block_index = heap[heap_index];
+          slot_pos[first_empty] = c;
+          slot_ptr[first_empty] = (int32_t)block_index;
+          ensure_block((int32_t)block_index);
+          c_new_blocks++;
+          return (int32_t)block_index;  // the bucket lock stays taken (VoxelScene.cu:201-203)
+        }
+        heap_counter += 1u;  // out of blocks: restore, release the lock
+        mutex[h] = 0;
+        if (!heap_failed[h]) {  // counted once per bucket and frame
+          heap_failed[h] = 1;
+          c_heap_fail++;
+        }
+        return -1;
+      }
+      return -1;
+    }
+    return -1;
+  }
+
+  size_t num_allocated() const { return (size_t)(num_blocks - 1u - heap_counter); }
+};
+
+// ---- A5: block DDA from block(rayStart) to block(rayEnd) (VoxelScene.cu:295-481).
+// Calls visit(blockCoord, iteration) for every visited block, in order.
+template <class R, class F> inline void dda_band(V3<R> a, V3<R> b, R v, F&& visit) {
+  I3 cur = alloc_world_to_block(a, v);
+  I3 end = alloc_world_to_block(b, v);
+  if (cur == end) return;  // Q6: start == end allocates nothing (VoxelScene.cu:345-348)
+  V3<R> d{b.x - a.x, b.y - a.y, b.z - a.z};
+  R len = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+  if (len < R(1e-6f)) return;
+  d.x /= len;
+  d.y /= len;
+  d.z /= len;
+  V3<R> step{d.x > R(0) ? R(1) : R(-1), d.y > R(0) ? R(1) : R(-1), d.z > R(0) ? R(1) : R(-1)};
+  R ext = R(kBlock) * v;
+  V3<R> sc{step.x > R(0) ? R(1) : R(0), step.y > R(0) ? R(1) : R(0), step.z > R(0) ? R(1) : R(0)};
+  R hv = R(0.5f) * v;
+  V3<R> bnd{((R)cur.x + sc.x) * ext - hv, ((R)cur.y + sc.y) * ext - hv, ((R)cur.z + sc.z) * ext - hv};
+  V3<R> tmax{(bnd.x - a.x) / d.x, (bnd.y - a.y) / d.y, (bnd.z - a.z) / d.z};
+  V3<R> tdel{(step.x * ext) / d.x, (step.y * ext) / d.y, (step.z * ext) / d.z};
+  if (d.x == R(0)) { tmax.x = R(1e6f); tdel.x = R(1e6f); }
+  if (d.y == R(0)) { tmax.y = R(1e6f); tdel.y = R(1e6f); }
+  if (d.z == R(0)) { tmax.z = R(1e6f); tdel.z = R(1e6f); }
+  const int32_t sx = (int32_t)step.x, sy = (int32_t)step.y, sz = (int32_t)step.z;
+  for (int iter = 0; iter < 1024; ++iter) {
+    visit(cur, iter);
+    if (cur == end) break;
+    if (tmax.x < tmax.y && tmax.x < tmax.z) {
+      tmax.x += tdel.x;
+      cur.x += sx;
+    } else if (tmax.y < tmax.z) {
+      tmax.y += tdel.y;
+      cur.y += sy;
+    } else {
+      tmax.z += tdel.z;
+      cur.z += sz;
+    }
+  }
+}
+
+// Truncation band of one pixel in world space (VoxelScene.cu:527-605).  Returns
+// false when the reference's kernel returns before walking the band.
+template <class R> inline bool pixel_band(const R* M, V3<R> cam, V3<R> P, R trunc, V3<R>* lo, V3<R>* hi) {
+  if (P.z <= R(1e-6f)) return false;
+  V3<R> w = xform_point(M, P);
+  V3<R> rd{w.x - cam.x, w.y - cam.y, w.z - cam.z};
+  R len = std::sqrt(rd.x * rd.x + rd.y * rd.y + rd.z * rd.z);
+  if (len < R(1e-6f)) return false;
+  *lo = xform_point(M, V3<R>{P.x, P.y, P.z - trunc});
+  *hi = xform_point(M, V3<R>{P.x, P.y, P.z + trunc});
+  return true;
+}
+
+// ---- A4: allocation for one frame, sequential canonical order.
+template <class R> inline void alloc_frame(Scene<R>& s, const float* pts, int W, int H, R trunc, const R* M) {
+  // resetHashBucketMutexKernel before every allocation (VoxelScene.cu:1444-1449)
+  std::fill(s.mutex.begin(), s.mutex.end(), 0);
+  std::fill(s.heap_failed.begin(), s.heap_failed.end(), 0);
+  s.c_new_blocks = 0;
+  s.c_heap_fail = 0;
+  s.c_valid_pixels = 0;
+  V3<R> cam{M[3], M[7], M[11]};  // cameraPos = translation (VoxelScene.cpp:462-466)
+  for (int pix = 0; pix < W * H; ++pix) {
+    V3<R> P{(R)pts[3 * pix], (R)pts[3 * pix + 1], (R)pts[3 * pix + 2]};
+    if (P.z <= R(1e-6f)) continue;
+    s.c_valid_pixels++;
+    V3<R> lo, hi;
+    if (!pixel_band(M, cam, P, trunc, &lo, &hi)) continue;
+    dda_band(lo, hi, s.voxel, [&](I3 c, int) { s.alloc_block(c); });
+  }
+}
+
+// Conservative per-block test: can any voxel of block c be updated by this frame?
+// False only when provably no voxel projects onto an in-image pixel or lies within
+// the truncation band's depth slab.  zmin/zmax: camera-z range of valid pixel points.
+template <class R>
+inline bool block_maybe_visible(const R* M, I3 c, R v, int W, int H, R fx, R fy, R cx, R cy, R trunc, R zmin,
+                                R zmax) {
+  double lo_z = 1e300, hi_z = -1e300, lo_u = 1e300, hi_u = -1e300, lo_v = 1e300, hi_v = -1e300;
+  bool all_front = true;
+  for (int k = 0; k < 8; ++k) {
+    double wx = ((double)c.x * kBlock + ((k & 1) ? 7 : 0)) * (double)v;
+    double wy = ((double)c.y * kBlock + ((k & 2) ? 7 : 0)) * (double)v;
+    double wz = ((double)c.z * kBlock + ((k & 4) ? 7 : 0)) * (double)v;
+    double rx = wx - (double)M[3], ry = wy - (double)M[7], rz = wz - (double)M[11];
+    double X = (double)M[0] * rx + (double)M[4] * ry + (double)M[8] * rz;
+    double Y = (double)M[1] * rx + (double)M[5] * ry + (double)M[9] * rz;
+    double Z = (double)M[2] * rx + (double)M[6] * ry + (double)M[10] * rz;
+    lo_z = Z < lo_z ? Z : lo_z;
+    hi_z = Z > hi_z ? Z : hi_z;
+    if (Z > 1e-4) {
+      double u = X * (double)fx / Z + (double)cx, vv = Y * (double)fy / Z + (double)cy;
+      lo_u = u < lo_u ? u : lo_u;
+      hi_u = u > hi_u ? u : hi_u;
+      lo_v = vv < lo_v ? vv : lo_v;
+      hi_v = vv > hi_v ? vv : hi_v;
+    } else {
+      all_front = false;
+    }
+  }
+  double tz = (double)trunc * 1.001 + 1e-4;
+  if (hi_z < (double)zmin - tz || lo_z > (double)zmax + tz) return false;
+  if (all_front) {
+    if (hi_u < -2.5 || lo_u > (double)W + 0.5 || hi_v < -2.5 || lo_v > (double)H + 0.5) return false;
+  }
+  return true;
+}
+
+// ---- A8: integrate one frame into the listed blocks (VoxelScene.cu:918-1146).
+// The normal map is read by the reference but its value never reaches the store
+// (VoxelScene.cu:1075-1094), so it is not an input here.
+template <class R>
+inline void integrate_block(Scene<R>& s, I3 bc, int32_t ptr, const float* pts, const uint8_t* rgb, int W, int H,
+                            R trunc, const R* M, R fx, R fy, R cx, R cy, uint32_t* updated) {
+  const R v = s.voxel;
+  V3<R> cam{M[3], M[7], M[11]};
+  uint32_t upd = 0;
+  for (int idx = 0; idx < kVox; ++idx) {
+    int lx = idx % kBlock, ly = (idx / kBlock) % kBlock, lz = idx / (kBlock * kBlock);
+    V3<R> vw{(R)(bc.x * kBlock + lx) * v, (R)(bc.y * kBlock + ly) * v, (R)(bc.z * kBlock + lz) * v};
+    V3<R> rel{vw.x - M[3], vw.y - M[7], vw.z - M[11]};
+    V3<R> vc{M[0] * rel.x + M[4] * rel.y + M[8] * rel.z, M[1] * rel.x + M[5] * rel.y + M[9] * rel.z,
+             M[2] * rel.x + M[6] * rel.y + M[10] * rel.z};
+    R sxf = vc.x * fx / vc.z + cx;
+    R syf = vc.y * fy / vc.z + cy;
+    int32_t px = cvt_i32(sxf + R(0.5f)), py = cvt_i32(syf + R(0.5f));
+    if (!(px >= 0 && px < W && py >= 0 && py < H)) continue;
+    int pix = py * W + px;
+    V3<R> P{(R)pts[3 * pix], (R)pts[3 * pix + 1], (R)pts[3 * pix + 2]};
+    if (!(P.x != R(0) || P.y != R(0) || P.z != R(0))) continue;
+    V3<R> D = xform_point(M, P);
+    V3<R> sd{vw.x - D.x, vw.y - D.y, vw.z - D.z};
+    R sdfv = std::sqrt(sd.x * sd.x + sd.y * sd.y + sd.z * sd.z);
+    V3<R> cd{D.x - cam.x, D.y - cam.y, D.z - cam.z};
+    R cam_dist = std::sqrt(cd.x * cd.x + cd.y * cd.y + cd.z * cd.z);
+    R vox_dist = std::sqrt((vw.x - cam.x) * (vw.x - cam.x) + (vw.y - cam.y) * (vw.y - cam.y) +
+                           (vw.z - cam.z) * (vw.z - cam.z));
+    if (vox_dist < cam_dist) sdfv = -sdfv;
+    if (sdfv < -trunc || sdfv > trunc) continue;
+    sdfv = std::fmax(-trunc, std::fmin(trunc, sdfv));
+    size_t vi = (size_t)ptr * kVox + idx;
+    R old_sdf = s.load_sdf(vi);
+    R old_w = (R)s.weight[vi];
+    R new_w = old_w + R(1);
+    if (new_w > R(0)) {
+      R nsdf = (old_sdf * old_w + sdfv * R(1)) / new_w;
+      s.store_sdf(vi, nsdf);
+      R stored = s.load_sdf(vi);  // isZeroCrossing compares against the stored value
+      s.weight[vi] = (uint8_t)std::fmin(R(255), new_w);
+      for (int ch = 0; ch < 3; ++ch) {
+        R oc = (R)s.color[3 * vi + ch];
+        R nc = (oc * old_w + (R)rgb[3 * pix + ch] * R(1)) / new_w;
+        s.color[3 * vi + ch] = (uint8_t)nc;
+      }
+      uint8_t fl = s.flags[vi] | 1u;
+      if ((old_sdf > R(0) && stored <= R(0)) || (old_sdf <= R(0) && stored > R(0))) fl |= 2u;
+      s.flags[vi] = fl;
+      upd++;
+    }
+  }
+  *updated += upd;
+}
+
+// One call of VoxelScene::integrateFromScanData (VoxelScene.cpp:404-833): allocate,
+// compact (every allocated block, VoxelScene.cu:1275-1300), integrate.
+// visible_only != 0 integrates only blocks passing block_maybe_visible; the
+// result must be identical (that is what the test of the culling checks).
+template <class R>
+inline void integrate_frame(Scene<R>& s, const float* pts, const uint8_t* rgb, int W, int H, R trunc, const R* M,
+                            R fx, R fy, R cx, R cy, int visible_only) {
+  alloc_frame(s, pts, W, H, trunc, M);
+  // compaction: every live entry, in table order
+  std::vector<std::pair<I3, int32_t>> list;
+  for (size_t i = 0; i < s.slot_ptr.size(); ++i)
+    if (s.slot_ptr[i] != -1) list.push_back({s.slot_pos[i], s.slot_ptr[i]});
+  R zmin = R(1e30f), zmax = R(-1e30f);
+  bool finite = true;
+  if (visible_only) {
+    for (int pix = 0; pix < W * H; ++pix) {
+      R X = (R)pts[3 * pix], Y = (R)pts[3 * pix + 1], Z = (R)pts[3 * pix + 2];
+      if (!(X != R(0) || Y != R(0) || Z != R(0))) continue;
+      if (!std::isfinite(X) || !std::isfinite(Y) || !std::isfinite(Z)) finite = false;
+      zmin = Z < zmin ? Z : zmin;
+      zmax = Z > zmax ? Z : zmax;
+    }
+  }
+  std::vector<char> keep(list.size(), 1);
+  if (visible_only && finite) {
+    for (size_t i = 0; i < list.size(); ++i)
+      keep[i] = block_maybe_visible(M, list[i].first, s.voxel, W, H, fx, fy, cx, cy, trunc, zmin, zmax) ? 1 : 0;
+  }
+  uint32_t updated = 0, visible = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : updated, visible)
+  for (long i = 0; i < (long)list.size(); ++i) {
+    if (!keep[i]) continue;
+    visible++;
+    integrate_block(s, list[i].first, list[i].second, pts, rgb, W, H, trunc, M, fx, fy, cx, cy, &updated);
+  }
+  s.c_updated = updated;
+  s.c_visible = visible;
+  s.frames++;
+}
+
+// ---- A13: getVoxel (RayCastRender.cu:55-121): weight and sdf of the voxel at p.
+template <class R> struct VoxelRead {
+  R sdf;
+  uint8_t w;
+  uint8_t c[3];
+};
+
+template <class R> inline VoxelRead<R> get_voxel(const Scene<R>& s, V3<R> p, R v) {
+  int32_t vx = ray_voxel_axis(p.x, v), vy = ray_voxel_axis(p.y, v), vz = ray_voxel_axis(p.z, v);
+  I3 b{voxel_to_block_axis(vx), voxel_to_block_axis(vy), voxel_to_block_axis(vz)};
+  int lx = voxel_local_axis(vx), ly = voxel_local_axis(vy), lz = voxel_local_axis(vz);
+  uint32_t h = hash_bucket(b, s.num_buckets);
+  size_t hp = (size_t)h * kBucket;
+  for (int j = 0; j < kBucket; ++j) {
+    if (s.slot_ptr[hp + j] == -1) continue;
+    if (s.slot_pos[hp + j] == b) {
+      size_t vi = (size_t)s.slot_ptr[hp + j] * kVox + (size_t)(lx + ly * kBlock + lz * kBlock * kBlock);
+      VoxelRead<R> r;
+      r.sdf = s.load_sdf(vi);
+      r.w = s.weight[vi];
+      r.c[0] = s.color[3 * vi];
+      r.c[1] = s.color[3 * vi + 1];
+      r.c[2] = s.color[3 * vi + 2];
+      return r;
+    }
+  }
+  return VoxelRead<R>{R(0), 0, {0, 0, 0}};
+}
+
+template <class R> inline R frac01(R x) { return x - std::floor(x); }
+
+// ---- A12: trilinearInterpolationSimpleFastFast (RayCastRender.cu:127-215)
+template <class R> inline bool trilinear(const Scene<R>& s, V3<R> p, R v, R* dist, uint8_t col[3]) {
+  const R off = v;
+  V3<R> pd{p.x - off / R(2), p.y - off / R(2), p.z - off / R(2)};
+  V3<R> vp{pd.x / v, pd.y / v, pd.z / v};
+  V3<R> w{frac01(vp.x), frac01(vp.y), frac01(vp.z)};
+  R d = R(0), cr = R(0), cg = R(0), cb = R(0), wsum = R(0);
+  for (int i = 0; i < 8; ++i) {
+    R ax = (i & 1) ? off : R(0), ay = (i & 2) ? off : R(0), az = (i & 4) ? off : R(0);
+    VoxelRead<R> vox = get_voxel(s, V3<R>{pd.x + ax, pd.y + ay, pd.z + az}, v);
+    if (vox.w == 0) continue;
+    R wi = ((i & 1) ? w.x : (R(1) - w.x)) * ((i & 2) ? w.y : (R(1) - w.y)) * ((i & 4) ? w.z : (R(1) - w.z));
+    d += wi * vox.sdf;
+    cr += wi * (R)vox.c[0];
+    cg += wi * (R)vox.c[1];
+    cb += wi * (R)vox.c[2];
+    wsum += wi;
+  }
+  if (wsum <= R(1e-6f)) return false;
+  R inv = R(1) / wsum;
+  *dist = d * inv;
+  cr *= inv;
+  cg *= inv;
+  cb *= inv;
+  col[0] = (uint8_t)cr;
+  col[1] = (uint8_t)cg;
+  col[2] = (uint8_t)cb;
+  return true;
+}
+
+// ---- A14: three regula-falsi steps (RayCastRender.cu:220-271)
+template <class R>
+inline bool refine(const Scene<R>& s, V3<R> cam, V3<R> dir, R v, R d0, R r0, R d1, R r1, R* alpha, uint8_t col[3]) {
+  R a = r0, ad = d0, b = r1, bd = d1, c = R(0);
+  for (int i = 0; i < 3; ++i) {
+    c = a + (ad / (ad - bd)) * (b - a);
+    R cd;
+    if (!trilinear(s, V3<R>{cam.x + c * dir.x, cam.y + c * dir.y, cam.z + c * dir.z}, v, &cd, col)) return false;
+    if (ad * cd > R(0)) {
+      a = c;
+      ad = cd;
+    } else {
+      b = c;
+      bd = cd;
+    }
+  }
+  *alpha = c;
+  return true;
+}
+
+// ---- A10/A11: renderKernel + traverseCoarseGridSimpleSampleAll (RayCastRender.cu:276-517).
+// Outputs: depth[W*H], pos[4*W*H], col[4*W*H], nrm[4*W*H] (normals stay 1e10).
+// Returns 0 if the kernel would not run (no allocated blocks: outputs untouched).
+template <class R>
+inline int render(const Scene<R>& s, int W, int H, V3<R> cam, const R* M, R fx, R fy, R cx, R cy, R minD, R maxD,
+                  R voxel_size, float* depth, float* pos, float* col, float* nrm, uint32_t* hits, uint32_t* misses,
+                  const int32_t* pix_list = nullptr, long n_list = 0) {
+  *hits = 0;
+  *misses = 0;
+  if (s.num_allocated() == 0) return 0;  // RayCastRender.cu:553-556
+  const R MINF = R(1e10f);
+  const R v = voxel_size;
+  uint32_t nh = 0, nm = 0;
+  // Pixels to trace: all (row-major), or the listed ones (others left untouched).
+  const long nwork = pix_list ? n_list : (long)W * H;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : nh, nm)
+  for (long wi = 0; wi < nwork; ++wi) {
+    {
+      const int pix = pix_list ? pix_list[wi] : (int)wi;
+      const int x = pix % W, y = pix / W;
+      depth[pix] = (float)MINF;
+      for (int k = 0; k < 4; ++k) {
+        pos[4 * pix + k] = (float)MINF;
+        col[4 * pix + k] = (float)MINF;
+        nrm[4 * pix + k] = (float)MINF;
+      }
+      V3<R> cd{((R)x - cx) / fx, ((R)y - cy) / fy, R(1)};
+      R cl = std::sqrt(cd.x * cd.x + cd.y * cd.y + cd.z * cd.z);
+      if (cl > R(1e-6f)) {
+        cd.x /= cl;
+        cd.y /= cl;
+        cd.z /= cl;
+      }
+      V3<R> wd{M[0] * cd.x + M[1] * cd.y + M[2] * cd.z, M[4] * cd.x + M[5] * cd.y + M[6] * cd.z,
+               M[8] * cd.x + M[9] * cd.y + M[10] * cd.z};
+      R wl = std::sqrt(wd.x * wd.x + wd.y * wd.y + wd.z * wd.z);
+      if (wl > R(1e-6f)) {
+        wd.x /= wl;
+        wd.y /= wl;
+        wd.z /= wl;
+      }
+      if (minD == R(0) || minD == MINF) continue;
+      if (maxD == R(0) || maxD == MINF) continue;
+      R last_sdf = R(0), last_alpha = R(0);
+      unsigned last_w = 0;
+      const R d2r = R(1) / cd.z;
+      R ray = d2r * std::fmax(minD, minD);
+      R ray_end = d2r * std::fmin(maxD, maxD);
+      const R th = v * R(3);
+      const R inc = v * R(0.5f);
+      bool hit = false;
+      while (ray < ray_end) {
+        V3<R> p{cam.x + ray * wd.x, cam.y + ray * wd.y, cam.z + ray * wd.z};
+        R dist;
+        uint8_t c1[3];
+        if (trilinear(s, p, v, &dist, c1)) {
+          if (last_w > 0 && last_sdf < R(0) && dist > R(0)) {
+            R alpha = R(0);
+            uint8_t c2[3] = {0, 0, 0};
+            bool b = refine(s, cam, wd, v, last_sdf, last_alpha, dist, ray, &alpha, c2);
+            if (b && std::fabs(last_sdf - dist) < th) {
+              if (std::fabs(dist) < th) {
+                V3<R> iso{cam.x + alpha * wd.x, cam.y + alpha * wd.y, cam.z + alpha * wd.z};
+                depth[pix] = (float)(alpha / d2r);
+                pos[4 * pix] = (float)iso.x;
+                pos[4 * pix + 1] = (float)iso.y;
+                pos[4 * pix + 2] = (float)iso.z;
+                pos[4 * pix + 3] = 1.0f;
+                col[4 * pix] = (float)((R)c2[0] / R(255));
+                col[4 * pix + 1] = (float)((R)c2[1] / R(255));
+                col[4 * pix + 2] = (float)((R)c2[2] / R(255));
+                col[4 * pix + 3] = 1.0f;
+                hit = true;
+                break;
+              }
+            }
+          }
+          last_sdf = dist;
+          last_alpha = ray;
+          last_w = 1;
+        } else {
+          last_w = 0;
+        }
+        // rayCurrent += voxelSize * 0.5f.  When the step is absorbed by rounding the
+        // reference's loop never ends; this restatement stops there (a miss).
+        R next = ray + inc;
+        if (next == ray) break;
+        ray = next;
+      }
+      if (hit) nh++; else nm++;
+    }
+  }
+  *hits = nh;
+  *misses = nm;
+  return 1;
+}
+
+}  // namespace vsgo

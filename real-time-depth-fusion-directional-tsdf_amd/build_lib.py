@@ -1,0 +1,80 @@
+"""Builds the in-tree shared libraries of the VoxelSG MI355X package.
+
+    lib/libvsg.so        HIP kernels + C ABI (include/vsg.h), gfx950 code objects
+    lib/libvsg_synth.so  synthetic frame generator (host C++)
+
+Run as ``python build_lib.py`` or call :func:`build`.  Every call compiles.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+REPO_DIR = os.path.dirname(PKG_DIR)
+CSRC = os.path.join(PKG_DIR, "csrc")
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+INCLUDE = os.path.join(REPO_DIR, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+
+# -ffp-contract=off: products and sums round separately, in the reference's
+# operation order (bit-exact with the oracle).  Packed FP32 ops are disabled.
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-shared",
+    "-ffp-contract=off",
+    "-fno-fast-math",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd: list[str], verbose: bool) -> None:
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"build step failed ({res.returncode}): {' '.join(cmd)}")
+    if verbose and (res.stdout or res.stderr):
+        msg = "\n".join(
+            ln for ln in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in ln
+        )
+        if msg.strip():
+            print(msg)
+
+
+def build_vsg(verbose: bool = False) -> str:
+    os.makedirs(LIB_DIR, exist_ok=True)
+    out = os.path.join(LIB_DIR, "libvsg.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, "vsg_kernels.hip"), os.path.join(CSRC, "vsg_capi.hip")]
+    _run([HIPCC, *HIP_FLAGS, "-I", INCLUDE, "-I", CSRC, "-o", tmp, *srcs], verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def build_synth(verbose: bool = False) -> str:
+    os.makedirs(LIB_DIR, exist_ok=True)
+    out = os.path.join(LIB_DIR, "libvsg_synth.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off",
+          "-o", tmp, os.path.join(CSRC, "synth.cpp")], verbose)
+    os.replace(tmp, out)
+    return out
+
+
+def build(verbose: bool = False) -> None:
+    build_synth(verbose)
+    build_vsg(verbose)
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,665 @@
+// C ABI of the VoxelSG depth-fusion library (declared in include/vsg.h).
+//
+// Owns the device memory of scenes and renderers, validates arguments, and queues
+// the kernels of vsg_kernels.hip on the object's stream.  No call synchronises
+// except the ones vsg.h documents as synchronising.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "vsg.h"
+#include "vsg_device.hpp"
+#include "vsg_internal.hpp"
+
+using namespace vsg;
+
+namespace {
+
+thread_local std::string g_err;
+
+vsg_status fail(vsg_status st, const std::string& msg) {
+  g_err = msg;
+  return st;
+}
+
+vsg_status hip_fail(hipError_t e, const char* what) {
+  return fail(VSG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define VSG_HIP(call)                                  \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+// ---- kernel timing ---------------------------------------------------------------
+struct TimingRec {
+  int kid;
+  hipEvent_t a, b;
+};
+
+struct Timing {
+  std::mutex mu;
+  bool on = false;
+  std::vector<TimingRec> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[VSG_K_COUNT] = {};
+  uint32_t n[VSG_K_COUNT] = {};
+
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+};
+Timing g_timing;
+
+// Brackets one launch with events when timing is on.
+struct TimedLaunch {
+  hipStream_t s;
+  int kid;
+  hipEvent_t a = nullptr, b = nullptr;
+  bool on;
+  TimedLaunch(hipStream_t s_, int kid_) : s(s_), kid(kid_) {
+    std::lock_guard<std::mutex> g(g_timing.mu);
+    on = g_timing.on;
+    if (on) {
+      a = g_timing.get();
+      b = g_timing.get();
+      if (!a || !b) on = false;
+      else (void)hipEventRecord(a, s);
+    }
+  }
+  ~TimedLaunch() {
+    if (!on) return;
+    (void)hipEventRecord(b, s);
+    std::lock_guard<std::mutex> g(g_timing.mu);
+    g_timing.pending.push_back(TimingRec{kid, a, b});
+  }
+};
+
+vsg_status check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, what);
+  return VSG_OK;
+}
+
+}  // namespace
+
+// ---- objects ---------------------------------------------------------------------
+
+struct vsg_scene {
+  vsg_params p{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint32_t word_bytes = 8;
+  TableView t{};
+  void* vox = nullptr;
+  uint32_t* col = nullptr;
+  uint32_t* vis = nullptr;
+  DevState* st = nullptr;
+  uint32_t* pixcnt = nullptr;
+  uint32_t* wgsum = nullptr;
+  size_t pix_cap = 0;
+  uint32_t frames = 0;  // integrate calls since reset (host mirror)
+};
+
+struct vsg_render {
+  int32_t W = 0, H = 0;
+  hipStream_t stream = nullptr;  // used when has_stream, else the scene's stream
+  bool has_stream = false;
+  hipStream_t last = nullptr;    // stream of the last render (valid when has_last)
+  bool has_last = false;
+  float* depth = nullptr;
+  float4* pos = nullptr;
+  float4* col = nullptr;
+  float4* nrm = nullptr;
+  uint32_t* rc = nullptr;  // 2 sets x {hits, misses}
+  uint32_t calls = 0;
+  hipEvent_t ev = nullptr;
+};
+
+namespace {
+
+void free_scene(vsg_scene* s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(s->t.rec);
+  (void)hipFree(s->t.cnt);
+  (void)hipFree(s->t.claim);
+  (void)hipFree(s->t.blk);
+  (void)hipFree(s->vox);
+  (void)hipFree(s->col);
+  (void)hipFree(s->vis);
+  (void)hipFree(s->st);
+  (void)hipFree(s->pixcnt);
+  (void)hipFree(s->wgsum);
+  if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+template <class T> vsg_status dmalloc(T** p, size_t bytes, const char* what) {
+  hipError_t e = hipMalloc((void**)p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    return fail(VSG_ERR_OOM, std::string("device allocation of ") + what + " (" + std::to_string(bytes) +
+                                 " bytes) failed: " + hipGetErrorString(e));
+  }
+  return VSG_OK;
+}
+
+vsg_status queue_reset(vsg_scene* s) {
+  {
+    TimedLaunch tl(s->stream, VSG_K_RESET);
+    launch_reset_voxels(s->stream, s->st, s->vox, s->word_bytes, s->col);
+  }
+  vsg_status r = check_launch("reset voxels");
+  if (r != VSG_OK) return r;
+  {
+    TimedLaunch tl(s->stream, VSG_K_RESET);
+    launch_reset_table(s->stream, s->t, s->st);
+  }
+  r = check_launch("reset table");
+  if (r != VSG_OK) return r;
+  s->frames = 0;
+  return VSG_OK;
+}
+
+// Conservative culling needs the pose's rotation to be a rotation: R^T R = I.
+bool pose_orthonormal(const float* M) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double d = 0;
+      for (int k = 0; k < 3; ++k) d += (double)M[4 * k + i] * (double)M[4 * k + j];
+      if (std::fabs(d - (i == j ? 1.0 : 0.0)) > 1e-5) return false;
+    }
+  return true;
+}
+
+bool all_finite(const float* v, int n) {
+  for (int i = 0; i < n; ++i)
+    if (!std::isfinite(v[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vsg_version(void) { return 100; }
+
+const char* vsg_last_error(void) { return g_err.c_str(); }
+
+vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
+  if (!params || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: null argument");
+  *out = nullptr;
+  const vsg_params& p = *params;
+  if (p.hash_slot_num <= 0 || p.hash_slot_num > (1 << 28))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: hash_slot_num must be in [1, 2^28]");
+  if (p.sdf_block_num <= 0 || p.sdf_block_num > (1 << 26))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: sdf_block_num must be in [1, 2^26]");
+  if (!(p.voxel_size > 0.0f) || !std::isfinite(p.voxel_size))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: voxel_size must be a positive finite number");
+  if (p.store != VSG_STORE_FP32 && p.store != VSG_STORE_FP16)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: store must be VSG_STORE_FP32 or VSG_STORE_FP16");
+  for (int i = 0; i < 4; ++i)
+    if (p.reserved[i] != 0) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: reserved fields must be zero");
+
+  vsg_scene* s = new vsg_scene();
+  s->p = p;
+  s->word_bytes = p.store == VSG_STORE_FP16 ? 4u : 8u;
+  const size_t B = (size_t)p.hash_slot_num, N = (size_t)p.sdf_block_num;
+  s->t.num_buckets = (uint32_t)B;
+  s->t.num_blocks = (uint32_t)N;
+  hipError_t e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete s;
+    return hip_fail(e, "hipStreamCreateWithFlags");
+  }
+  s->own_stream = true;
+  vsg_status r = VSG_OK;
+  if (r == VSG_OK) r = dmalloc(&s->t.rec, B * kRecWords * sizeof(uint64_t), "hash table");
+  if (r == VSG_OK) r = dmalloc(&s->t.cnt, B, "bucket counts");
+  if (r == VSG_OK) r = dmalloc(&s->t.claim, B * sizeof(uint32_t), "bucket claims");
+  if (r == VSG_OK) r = dmalloc(&s->t.blk, N * sizeof(int4), "block list");
+  if (r == VSG_OK) r = dmalloc(&s->vox, N * kVox * s->word_bytes, "voxel store");
+  if (r == VSG_OK) r = dmalloc(&s->col, N * kVox * sizeof(uint32_t), "colour store");
+  if (r == VSG_OK) r = dmalloc(&s->vis, N * sizeof(uint32_t), "visible list");
+  if (r == VSG_OK) r = dmalloc(&s->st, sizeof(DevState), "scene state");
+  if (r != VSG_OK) {
+    free_scene(s);
+    return r;
+  }
+  // new scene: voxel memory zero (CUDAHashRef.cu:86-93), then reset
+  e = hipMemsetAsync(s->vox, 0, N * kVox * s->word_bytes, s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->col, 0, N * kVox * sizeof(uint32_t), s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->t.blk, 0, N * sizeof(int4), s->stream);
+  if (e != hipSuccess) {
+    free_scene(s);
+    return hip_fail(e, "hipMemsetAsync");
+  }
+  r = queue_reset(s);
+  if (r == VSG_OK) {
+    e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) r = hip_fail(e, "hipStreamSynchronize");
+  }
+  if (r != VSG_OK) {
+    free_scene(s);
+    return r;
+  }
+  *out = s;
+  return VSG_OK;
+}
+
+vsg_status vsg_scene_destroy(vsg_scene* scene) {
+  if (!scene) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_destroy: null scene");
+  free_scene(scene);
+  return VSG_OK;
+}
+
+vsg_status vsg_scene_reset(vsg_scene* scene) {
+  if (!scene) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_reset: null scene");
+  return queue_reset(scene);
+}
+
+vsg_status vsg_scene_set_stream(vsg_scene* scene, void* hip_stream) {
+  if (!scene) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_set_stream: null scene");
+  VSG_HIP(hipStreamSynchronize(scene->stream));
+  if (scene->own_stream) VSG_HIP(hipStreamDestroy(scene->stream));
+  scene->stream = (hipStream_t)hip_stream;
+  scene->own_stream = false;
+  return VSG_OK;
+}
+
+vsg_status vsg_scene_get_stream(const vsg_scene* scene, void** hip_stream) {
+  if (!scene || !hip_stream) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_get_stream: null argument");
+  *hip_stream = (void*)scene->stream;
+  return VSG_OK;
+}
+
+vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, const float pose[16],
+                         const vsg_intrinsics* k) {
+  if (!s || !f || !pose || !k) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: null argument");
+  if (!f->points || !f->colors) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: null point or colour map");
+  if (f->width <= 0 || f->height <= 0) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: empty frame");
+  const size_t npix = (size_t)f->width * (size_t)f->height;
+  if (npix > (1u << 22))  // request keys are (pixel << 10 | DDA step) in 32 bits
+    return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: frames are limited to 2^22 pixels");
+  if (!std::isfinite(truncation)) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: truncation must be finite");
+  if (!all_finite(pose, 16)) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: pose must be finite");
+  const float kk[4] = {k->fx, k->fy, k->cx, k->cy};
+  if (!all_finite(kk, 4)) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: intrinsics must be finite");
+
+  const uint32_t nwg = (uint32_t)((npix + 255) / 256);
+  if (npix > s->pix_cap) {
+    VSG_HIP(hipStreamSynchronize(s->stream));
+    (void)hipFree(s->pixcnt);
+    (void)hipFree(s->wgsum);
+    s->pixcnt = nullptr;
+    s->wgsum = nullptr;
+    s->pix_cap = 0;
+    vsg_status r = dmalloc(&s->pixcnt, npix * sizeof(uint32_t), "pixel scratch");
+    if (r == VSG_OK) r = dmalloc(&s->wgsum, (size_t)nwg * sizeof(uint32_t), "workgroup scratch");
+    if (r != VSG_OK) return r;
+    s->pix_cap = npix;
+  }
+
+  FrameArgs fa;
+  std::memset(&fa, 0, sizeof(fa));
+  for (int i = 0; i < 16; ++i) {
+    fa.M[i] = pose[i];
+    fa.Md[i] = (double)pose[i];
+  }
+  fa.fx = k->fx;
+  fa.fy = k->fy;
+  fa.cx = k->cx;
+  fa.cy = k->cy;
+  fa.trunc = truncation;
+  fa.voxel = s->p.voxel_size;
+  fa.W = f->width;
+  fa.H = f->height;
+  fa.cset = (int32_t)(s->frames & 1u);
+  fa.cull_ok = pose_orthonormal(pose) ? 1 : 0;
+
+  const float* pts = (const float*)f->points;
+  const uint8_t* rgb = (const uint8_t*)f->colors;
+  const bool fp16 = s->p.store == VSG_STORE_FP16;
+  vsg_status r;
+  {
+    TimedLaunch tl(s->stream, VSG_K_ALLOC_CLAIM);
+    launch_alloc_claim(s->stream, s->t, s->st, fa, pts, s->pixcnt);
+  }
+  if ((r = check_launch("alloc claim")) != VSG_OK) return r;
+  {
+    TimedLaunch tl(s->stream, VSG_K_ALLOC_RESOLVE);
+    launch_alloc_resolve(s->stream, s->t, fa, pts, s->pixcnt, s->wgsum);
+  }
+  if ((r = check_launch("alloc resolve")) != VSG_OK) return r;
+  {
+    TimedLaunch tl(s->stream, VSG_K_ALLOC_SCAN);
+    launch_alloc_scan(s->stream, s->st, s->wgsum, nwg, fa.cset, s->t.num_blocks);
+  }
+  if ((r = check_launch("alloc scan")) != VSG_OK) return r;
+  {
+    TimedLaunch tl(s->stream, VSG_K_ALLOC_INSERT);
+    launch_alloc_insert(s->stream, s->t, s->st, fa, pts, s->pixcnt, s->wgsum);
+  }
+  if ((r = check_launch("alloc insert")) != VSG_OK) return r;
+  {
+    TimedLaunch tl(s->stream, VSG_K_COMPACT);
+    launch_compact(s->stream, s->t, s->st, fa, s->vis, 1024);
+  }
+  if ((r = check_launch("compact")) != VSG_OK) return r;
+  {
+    TimedLaunch tl(s->stream, VSG_K_INTEGRATE);
+    launch_integrate(s->stream, fp16, s->t, s->st, fa, pts, rgb, s->vis, s->vox, s->col, 2048);
+  }
+  if ((r = check_launch("integrate")) != VSG_OK) return r;
+  s->frames++;
+  return VSG_OK;
+}
+
+vsg_status vsg_scene_counters(vsg_scene* s, vsg_counters* out) {
+  if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_counters: null argument");
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  DevState h;
+  VSG_HIP(hipMemcpy(&h, s->st, sizeof(h), hipMemcpyDeviceToHost));
+  std::memset(out, 0, sizeof(*out));
+  out->num_allocated = h.n_alloc;
+  out->frames = h.frames;
+  if (s->frames > 0) {
+    const FrameCounters& f = h.set[(s->frames - 1u) & 1u];
+    out->valid_pixels = f.valid_pixels;
+    out->new_blocks = h.n_alloc - h.n_alloc_prev;
+    out->heap_fail = f.heap_fail;
+    out->visible_blocks = f.visible;
+    out->updated_voxels = f.updated;
+    out->claimed_buckets = f.claimed_buckets;
+    out->out_of_range = f.out_of_range;
+  }
+  return VSG_OK;
+}
+
+vsg_status vsg_heap_free_count(vsg_scene* s, uint32_t* out) {
+  if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_heap_free_count: null argument");
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  uint32_t n = 0;
+  VSG_HIP(hipMemcpy(&n, &s->st->n_alloc, sizeof(n), hipMemcpyDeviceToHost));
+  *out = s->t.num_blocks - n;
+  return VSG_OK;
+}
+
+vsg_status vsg_num_integrated_frames(const vsg_scene* s, uint32_t* out) {
+  if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_num_integrated_frames: null argument");
+  *out = s->frames;
+  return VSG_OK;
+}
+
+vsg_status vsg_scene_params(const vsg_scene* s, vsg_params* out) {
+  if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_params: null argument");
+  *out = s->p;
+  return VSG_OK;
+}
+
+vsg_status vsg_export_blocks(vsg_scene* s, int32_t* coords, int32_t* slots, uint32_t capacity, uint32_t* count) {
+  if (!s || !count) return fail(VSG_ERR_INVALID_ARG, "vsg_export_blocks: null argument");
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  uint32_t n = 0;
+  VSG_HIP(hipMemcpy(&n, &s->st->n_alloc, sizeof(n), hipMemcpyDeviceToHost));
+  *count = n;
+  const uint32_t m = n < capacity ? n : capacity;
+  if (m == 0 || (!coords && !slots)) return VSG_OK;
+  std::vector<int4> h(m);
+  VSG_HIP(hipMemcpy(h.data(), s->t.blk, m * sizeof(int4), hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < m; ++i) {
+    if (coords) {
+      coords[3 * i] = h[i].x;
+      coords[3 * i + 1] = h[i].y;
+      coords[3 * i + 2] = h[i].z;
+    }
+    if (slots) slots[i] = h[i].w;
+  }
+  return VSG_OK;
+}
+
+vsg_status vsg_export_voxels(vsg_scene* s, uint32_t first, uint32_t nblocks, float* sdf, uint16_t* weight,
+                             uint8_t* rgbf) {
+  if (!s) return fail(VSG_ERR_INVALID_ARG, "vsg_export_voxels: null scene");
+  if ((uint64_t)first + nblocks > s->t.num_blocks)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_export_voxels: block range beyond the store");
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  const size_t nv = (size_t)nblocks * kVox, off = (size_t)first * kVox;
+  if (nv == 0) return VSG_OK;
+  if (sdf || weight) {
+    std::vector<uint8_t> w(nv * s->word_bytes);
+    VSG_HIP(hipMemcpy(w.data(), (uint8_t*)s->vox + off * s->word_bytes, w.size(), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < nv; ++i) {
+      if (s->word_bytes == 8) {
+        uint32_t a, b;
+        std::memcpy(&a, &w[8 * i], 4);
+        std::memcpy(&b, &w[8 * i + 4], 4);
+        if (sdf) std::memcpy(&sdf[i], &a, 4);
+        if (weight) weight[i] = (uint16_t)(b & 0xFFFFu);
+      } else {
+        uint32_t a;
+        std::memcpy(&a, &w[4 * i], 4);
+        if (sdf) {
+          // binary16 -> binary32 (exact)
+          uint32_t h = a & 0xFFFFu, sign = (h & 0x8000u) << 16, e = (h >> 10) & 0x1Fu, m = h & 0x3FFu, x;
+          if (e == 0) {
+            if (m == 0) {
+              x = sign;
+            } else {
+              int kk = -1;
+              do {
+                kk++;
+                m <<= 1;
+              } while (!(m & 0x400u));
+              x = sign | ((uint32_t)(112 - kk) << 23) | ((m & 0x3FFu) << 13);
+            }
+          } else if (e == 31) {
+            x = sign | 0x7F800000u | (m << 13);
+          } else {
+            x = sign | ((e + 112u) << 23) | (m << 13);
+          }
+          std::memcpy(&sdf[i], &x, 4);
+        }
+        if (weight) weight[i] = (uint16_t)(a >> 16);
+      }
+    }
+  }
+  if (rgbf) VSG_HIP(hipMemcpy(rgbf, s->col + off, nv * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return VSG_OK;
+}
+
+// ---- renderer ----------------------------------------------------------------------
+
+vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out) {
+  if (!out) return fail(VSG_ERR_INVALID_ARG, "vsg_render_create: null argument");
+  *out = nullptr;
+  if (width <= 0 || height <= 0 || (int64_t)width * height > (1 << 26))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_create: width and height must be positive, at most 2^26 pixels");
+  vsg_render* r = new vsg_render();
+  r->W = width;
+  r->H = height;
+  const size_t n = (size_t)width * height;
+  vsg_status st = dmalloc(&r->depth, n * sizeof(float), "depth output");
+  if (st == VSG_OK) st = dmalloc(&r->pos, n * sizeof(float4), "position output");
+  if (st == VSG_OK) st = dmalloc(&r->col, n * sizeof(float4), "colour output");
+  if (st == VSG_OK) st = dmalloc(&r->nrm, n * sizeof(float4), "normal output");
+  if (st == VSG_OK) st = dmalloc(&r->rc, 4 * sizeof(uint32_t), "render counters");
+  if (st == VSG_OK && hipMemset(r->rc, 0, 4 * sizeof(uint32_t)) != hipSuccess) st = fail(VSG_ERR_HIP, "hipMemset");
+  if (st == VSG_OK && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
+    st = fail(VSG_ERR_HIP, "hipEventCreate");
+  if (st != VSG_OK) {
+    vsg_render_destroy(r);
+    return st;
+  }
+  *out = r;
+  return VSG_OK;
+}
+
+vsg_status vsg_render_destroy(vsg_render* r) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_destroy: null renderer");
+  if (r->has_last) (void)hipStreamSynchronize(r->last);
+  (void)hipFree(r->depth);
+  (void)hipFree(r->pos);
+  (void)hipFree(r->col);
+  (void)hipFree(r->nrm);
+  (void)hipFree(r->rc);
+  if (r->ev) (void)hipEventDestroy(r->ev);
+  delete r;
+  return VSG_OK;
+}
+
+vsg_status vsg_render_set_stream(vsg_render* r, void* hip_stream) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_stream: null renderer");
+  if (r->has_last) VSG_HIP(hipStreamSynchronize(r->last));
+  r->stream = (hipStream_t)hip_stream;
+  r->has_stream = true;
+  return VSG_OK;
+}
+
+vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3], const float pose[16],
+                            const vsg_intrinsics* k, float min_depth, float max_depth, float voxel_size) {
+  if (!r || !s || !cam_pos || !pose || !k) return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: null argument");
+  if (!all_finite(cam_pos, 3) || !all_finite(pose, 16))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: camera position and pose must be finite");
+  const float kk[4] = {k->fx, k->fy, k->cx, k->cy};
+  if (!all_finite(kk, 4)) return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: intrinsics must be finite");
+  if (std::isnan(min_depth) || std::isnan(max_depth) || std::isinf(min_depth) || std::isinf(max_depth))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: depth range must be finite");
+  RenderArgs ra;
+  std::memset(&ra, 0, sizeof(ra));
+  for (int i = 0; i < 16; ++i) ra.M[i] = pose[i];
+  for (int i = 0; i < 3; ++i) ra.cam[i] = cam_pos[i];
+  ra.fx = k->fx;
+  ra.fy = k->fy;
+  ra.cx = k->cx;
+  ra.cy = k->cy;
+  ra.minD = min_depth;
+  ra.maxD = max_depth;
+  ra.voxel = voxel_size > 0.0f ? voxel_size : s->p.voxel_size;
+  ra.W = r->W;
+  ra.H = r->H;
+  hipStream_t strm = r->has_stream ? r->stream : s->stream;
+  if (strm != s->stream) {  // order after the scene's pending work
+    VSG_HIP(hipEventRecord(r->ev, s->stream));
+    VSG_HIP(hipStreamWaitEvent(strm, r->ev, 0));
+  }
+  uint32_t* rc = r->rc + 2 * (r->calls & 1u);
+  VSG_HIP(hipMemsetAsync(rc, 0, 2 * sizeof(uint32_t), strm));
+  {
+    TimedLaunch tl(strm, VSG_K_RAYCAST);
+    launch_render(strm, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->depth, r->pos, r->col,
+                  r->nrm, rc);
+  }
+  vsg_status st = check_launch("render");
+  if (st != VSG_OK) return st;
+  if (strm != s->stream) {  // the scene's next integrate waits for this render
+    VSG_HIP(hipEventRecord(r->ev, strm));
+    VSG_HIP(hipStreamWaitEvent(s->stream, r->ev, 0));
+  }
+  r->last = strm;
+  r->has_last = true;
+  r->calls++;
+  return VSG_OK;
+}
+
+vsg_status vsg_render_buffers(const vsg_render* r, float** depth, float** positions, float** colors, float** normals) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_buffers: null renderer");
+  if (depth) *depth = r->depth;
+  if (positions) *positions = (float*)r->pos;
+  if (colors) *colors = (float*)r->col;
+  if (normals) *normals = (float*)r->nrm;
+  return VSG_OK;
+}
+
+vsg_status vsg_render_download(vsg_render* r, float* depth, float* positions, float* colors, float* normals) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_download: null renderer");
+  if (r->has_last) VSG_HIP(hipStreamSynchronize(r->last));
+  const size_t n = (size_t)r->W * r->H;
+  if (depth) VSG_HIP(hipMemcpy(depth, r->depth, n * sizeof(float), hipMemcpyDeviceToHost));
+  if (positions) VSG_HIP(hipMemcpy(positions, r->pos, n * sizeof(float4), hipMemcpyDeviceToHost));
+  if (colors) VSG_HIP(hipMemcpy(colors, r->col, n * sizeof(float4), hipMemcpyDeviceToHost));
+  if (normals) VSG_HIP(hipMemcpy(normals, r->nrm, n * sizeof(float4), hipMemcpyDeviceToHost));
+  return VSG_OK;
+}
+
+vsg_status vsg_render_counters(vsg_render* r, uint32_t* hits, uint32_t* misses) {
+  if (!r || !hits || !misses) return fail(VSG_ERR_INVALID_ARG, "vsg_render_counters: null argument");
+  *hits = *misses = 0;
+  if (r->calls == 0) return VSG_OK;
+  if (r->has_last) VSG_HIP(hipStreamSynchronize(r->last));
+  uint32_t h[2];
+  VSG_HIP(hipMemcpy(h, r->rc + 2 * ((r->calls - 1u) & 1u), sizeof(h), hipMemcpyDeviceToHost));
+  *hits = h[0];
+  *misses = h[1];
+  return VSG_OK;
+}
+
+// ---- timing -------------------------------------------------------------------------
+
+vsg_status vsg_timing_enable(int32_t on) {
+  std::lock_guard<std::mutex> g(g_timing.mu);
+  g_timing.on = on != 0;
+  return VSG_OK;
+}
+
+vsg_status vsg_timing_reset(void) {
+  std::vector<TimingRec> pend;
+  {
+    std::lock_guard<std::mutex> g(g_timing.mu);
+    pend.swap(g_timing.pending);
+  }
+  for (auto& t : pend) (void)hipEventSynchronize(t.b);
+  std::lock_guard<std::mutex> g(g_timing.mu);
+  for (auto& t : pend) {
+    g_timing.pool.push_back(t.a);
+    g_timing.pool.push_back(t.b);
+  }
+  for (int i = 0; i < VSG_K_COUNT; ++i) {
+    g_timing.ms[i] = 0;
+    g_timing.n[i] = 0;
+  }
+  return VSG_OK;
+}
+
+vsg_status vsg_timing_read(double ms[VSG_K_COUNT], uint32_t launches[VSG_K_COUNT]) {
+  std::vector<TimingRec> pend;
+  {
+    std::lock_guard<std::mutex> g(g_timing.mu);
+    pend.swap(g_timing.pending);
+  }
+  std::vector<float> el(pend.size(), 0.0f);
+  for (size_t i = 0; i < pend.size(); ++i) {
+    VSG_HIP(hipEventSynchronize(pend[i].b));
+    VSG_HIP(hipEventElapsedTime(&el[i], pend[i].a, pend[i].b));
+  }
+  std::lock_guard<std::mutex> g(g_timing.mu);
+  for (size_t i = 0; i < pend.size(); ++i) {
+    g_timing.ms[pend[i].kid] += el[i];
+    g_timing.n[pend[i].kid] += 1;
+    g_timing.pool.push_back(pend[i].a);
+    g_timing.pool.push_back(pend[i].b);
+  }
+  for (int i = 0; i < VSG_K_COUNT; ++i) {
+    if (ms) ms[i] = g_timing.ms[i];
+    if (launches) launches[i] = g_timing.n[i];
+  }
+  return VSG_OK;
+}
+
+}  // extern "C"

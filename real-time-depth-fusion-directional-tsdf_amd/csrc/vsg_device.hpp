@@ -1,0 +1,207 @@
+// Device helpers shared by the VoxelSG kernels (gfx950, wave64).
+//
+// Every floating-point expression here is written in the reference's operation
+// order and the library is compiled with -ffp-contract=off, so products and sums
+// round separately exactly as the reference's kernels state them; divisions and
+// square roots are IEEE correctly rounded (HIP's default).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+namespace vsg {
+
+constexpr int kBlock = 8;                              // SDF_BLOCK_SIZE (globalPramasConfig.h:8)
+constexpr int kBucket = 10;                            // HASH_BUCKET_SIZE (globalPramasConfig.h:9)
+constexpr int kVox = kBlock * kBlock * kBlock;         // 512 voxels per block
+constexpr int kRecWords = 16;                          // bucket record: 16 x u64 = 128 B
+constexpr int kKeyBits = 21;                           // per block coordinate in a packed key
+constexpr int32_t kKeyMin = -(1 << (kKeyBits - 1));
+constexpr int32_t kKeyMax = (1 << (kKeyBits - 1)) - 1;
+constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
+
+// ---- views passed to kernels by value -------------------------------------------
+
+// Hash table: bucket b is a 128-byte record at rec + 16*b holding 10 packed keys
+// (u64 words 0..9) and 10 block pointers (i32, bytes 80..119).  cnt[b] is the
+// number of used slots; slots fill in order and are never freed, so the used slots
+// are always slots [0, cnt[b]).  claim[b] is the per-frame insertion claim word.
+// blk[p] = (x, y, z, hash entry index) of the block with pointer p.
+struct TableView {
+  uint64_t* rec;
+  uint8_t* cnt;
+  uint32_t* claim;
+  int4* blk;
+  uint32_t num_buckets;
+  uint32_t num_blocks;
+};
+
+// Per-frame counter set (two sets alternate between frames so that each frame's
+// scan kernel can clear the next frame's set without an extra launch).
+struct FrameCounters {
+  uint32_t valid_pixels;
+  uint32_t claim_pixels;
+  uint32_t winners;
+  uint32_t heap_fail;
+  uint32_t out_of_range;
+  uint32_t visible;
+  uint32_t updated;
+  uint32_t nonfinite;
+  uint32_t zmin_key;  // ordered-uint encoding of the smallest / largest camera z
+  uint32_t zmax_key;  // of the frame's valid points
+  uint32_t claimed_buckets;
+  uint32_t pad[5];
+};
+
+struct DevState {
+  uint32_t n_alloc;       // live blocks (block pointers are 0..n_alloc-1)
+  uint32_t n_alloc_prev;  // live blocks before the current frame's insertions
+  uint32_t frames;
+  uint32_t pad;
+  FrameCounters set[2];
+};
+
+struct FrameArgs {
+  float M[16];  // row-major camera->world
+  float fx, fy, cx, cy;
+  float trunc;
+  float voxel;
+  int32_t W, H;
+  int32_t cset;     // counter set of this frame
+  int32_t cull_ok;  // pose rotation orthonormal enough for conservative culling
+  double Md[16];    // the same pose in double for the culling test
+};
+
+// ---- exact float helpers (mirroring the reference's expressions) -------------------
+
+// float->int like the hardware conversion: saturating, NaN -> 0.
+__device__ __forceinline__ int32_t cvt_i32(float x) {
+  if (x != x) return 0;
+  if (x >= 2147483648.0f) return INT32_MAX;
+  if (x <= -2147483648.0f) return INT32_MIN;
+  return (int32_t)x;
+}
+
+__device__ __forceinline__ float sgnf(float q) { return q > 0.0f ? 1.0f : (q < 0.0f ? -1.0f : 0.0f); }
+
+// Allocator world->voxel (VoxelScene.cu:78-89): (int)(q + 0.5*sign(q)), q = p/v.
+__device__ __forceinline__ int32_t alloc_voxel_axis(float p, float v) {
+  float q = p / v;
+  return cvt_i32(q + sgnf(q) * 0.5f);
+}
+
+// Raycaster world->voxel (RayCastRender.cu:17-25): floor(q + 0.5*sign(q)).
+__device__ __forceinline__ int32_t ray_voxel_of_q(float q) { return cvt_i32(floorf(q + 0.5f * sgnf(q))); }
+
+// virtualVoxelPosToSDFBlock (VoxelScene.cu:91-95, RayCastRender.cu:28-34).
+__device__ __forceinline__ int32_t voxel_to_block_axis(int32_t vi) {
+  if (vi < 0) vi = (int32_t)((uint32_t)vi - (uint32_t)(kBlock - 1));
+  return vi / kBlock;
+}
+
+// virtualVoxelPosToLocal (RayCastRender.cu:37-42).
+__device__ __forceinline__ int32_t voxel_local_axis(int32_t vi) { return ((vi % kBlock) + kBlock) % kBlock; }
+
+// hashBlockCoordinate (VoxelScene.cu:61-67): 32-bit wrap-around.
+__device__ __forceinline__ uint32_t hash_bucket(int32_t x, int32_t y, int32_t z, uint32_t nb) {
+  uint32_t h = ((uint32_t)x * 73856093u) ^ ((uint32_t)y * 19349663u) ^ ((uint32_t)z * 83492791u);
+  return h % nb;
+}
+
+__device__ __forceinline__ bool key_in_range(int32_t x, int32_t y, int32_t z) {
+  return x >= kKeyMin && x <= kKeyMax && y >= kKeyMin && y <= kKeyMax && z >= kKeyMin && z <= kKeyMax;
+}
+
+__device__ __forceinline__ uint64_t pack_key(int32_t x, int32_t y, int32_t z) {
+  const uint64_t m = (1ull << kKeyBits) - 1ull;
+  return (((uint64_t)(uint32_t)x & m) << (2 * kKeyBits)) | (((uint64_t)(uint32_t)y & m) << kKeyBits) |
+         ((uint64_t)(uint32_t)z & m);
+}
+
+// Row-major rigid transform, each row summed left to right (VoxelScene.cu:545-549).
+__device__ __forceinline__ float3 xform_point(const float* M, float x, float y, float z) {
+  return make_float3(M[0] * x + M[1] * y + M[2] * z + M[3], M[4] * x + M[5] * y + M[6] * z + M[7],
+                     M[8] * x + M[9] * y + M[10] * z + M[11]);
+}
+
+// Pointer of block (x,y,z), or -1.  Probes only the used slots of the home bucket:
+// keys are unique per bucket, so this is the reference's 10-slot scan
+// (RayCastRender.cu:81-111) without reading free slots.
+__device__ __forceinline__ int32_t table_lookup(const TableView& t, int32_t x, int32_t y, int32_t z) {
+  if (!key_in_range(x, y, z)) return -1;
+  const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
+  const int c = t.cnt[h];
+  if (c == 0) return -1;
+  const uint64_t key = pack_key(x, y, z);
+  const uint64_t* r = t.rec + (size_t)h * kRecWords;
+  int hit = -1;
+#pragma unroll
+  for (int j = 0; j < kBucket; ++j) {
+    if (j < c && r[j] == key) hit = j;
+  }
+  if (hit < 0) return -1;
+  return reinterpret_cast<const int32_t*>(r + kBucket)[hit];
+}
+
+// ---- ordered float encoding for atomicMin/atomicMax ----------------------------------
+__device__ __forceinline__ uint32_t float_to_ordered(float f) {
+  uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// ---- wave64 reductions (call with the whole wave converged) -------------------------
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_min(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = __shfl_xor(v, o, 64);
+    v = w < v ? w : v;
+  }
+  return v;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    uint32_t w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t w = __shfl_up(v, o, 64);
+    if (lane >= o) v += w;
+  }
+  return v;
+}
+
+// Exclusive prefix sum over a workgroup of NT threads (NT multiple of 64); also
+// returns the workgroup total.  Every thread of the workgroup must call it.
+template <int NT> __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t* total) {
+  __shared__ uint32_t wsum[NT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  uint32_t inc = wave_incl_scan(v);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) {
+    uint32_t s = wsum[w];
+    if (w < wid) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + inc - v;
+}
+
+}  // namespace vsg

@@ -1,0 +1,38 @@
+// Internal interface between the C ABI (vsg_capi.hip) and the kernels (vsg_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "vsg_device.hpp"
+
+namespace vsg {
+
+struct RenderArgs {
+  float M[16];
+  float cam[3];
+  float fx, fy, cx, cy;
+  float minD, maxD;
+  float voxel;
+  int32_t W, H;
+};
+
+void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col);
+void launch_reset_table(hipStream_t s, const TableView& t, DevState* st);
+void launch_alloc_claim(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
+                        uint32_t* pixcnt);
+void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
+                          uint32_t* wgsum);
+void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, uint32_t nwg, int32_t cset, uint32_t num_blocks);
+void launch_alloc_insert(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
+                         const uint32_t* pixcnt, const uint32_t* wgoff);
+void launch_compact(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, uint32_t* vis,
+                    uint32_t grid);
+void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
+                      const float* pts, const uint8_t* rgb, const uint32_t* vis, void* vox, uint32_t* col,
+                      uint32_t grid);
+void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
+                   const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,
+                   uint32_t* rc);
+
+}  // namespace vsg

@@ -1,0 +1,685 @@
+// VoxelSG depth-fusion kernels for gfx950 (wave64).
+//
+// Per frame, on one stream, with no host synchronisation:
+//   k_alloc_claim    one thread per pixel: truncation band -> block DDA -> for every
+//                    block absent from its bucket, atomicMin of the request key
+//                    (pixel << 10 | DDA step) on the bucket's claim word
+//   k_alloc_resolve  pixels that claimed re-walk their band and count the steps
+//                    whose key won; workgroup sums
+//   k_alloc_scan     one workgroup: exclusive scan of the workgroup sums, new
+//                    block count, heap-exhaustion count, next frame's counters
+//   k_alloc_insert   winners insert their block into the first free slot of the
+//                    bucket with pointer = blocks before + rank in key order
+//   k_compact        one thread per live block: conservative "can this frame
+//                    update it" test, wave ballot + prefix-sum stream compaction
+//   k_integrate      one 512-thread workgroup per listed block, one voxel per
+//                    thread, grid-stride over the list
+// and for rendering:
+//   k_render         one thread per pixel (8x8 tiles): ray march in half-voxel
+//                    steps, trilinear samples, three regula-falsi steps
+//
+// The reference lets GPU scheduling pick the one insertion per bucket per frame
+// (bucket lock taken and never released on success, VoxelScene.cu:176-217).
+// Here the request with the smallest (pixel, step) key wins: the outcome of a
+// sequential loop over the same requests, so the block set and the block
+// pointers are deterministic.
+#include "vsg_device.hpp"
+#include "vsg_internal.hpp"
+
+namespace vsg {
+
+// ---------------------------------------------------------------------------------
+// Band and DDA (VoxelScene.cu:295-481, 527-605)
+// ---------------------------------------------------------------------------------
+
+// Band ends of one pixel; false when the reference's kernel returns early.
+__device__ __forceinline__ bool pixel_band(const FrameArgs& fa, float Px, float Py, float Pz, float3* lo,
+                                           float3* hi) {
+  if (Pz <= 1e-6f) return false;
+  const float3 w = xform_point(fa.M, Px, Py, Pz);
+  const float rx = w.x - fa.M[3], ry = w.y - fa.M[7], rz = w.z - fa.M[11];
+  const float len = sqrtf(rx * rx + ry * ry + rz * rz);
+  if (len < 1e-6f) return false;
+  *lo = xform_point(fa.M, Px, Py, Pz - fa.trunc);
+  *hi = xform_point(fa.M, Px, Py, Pz + fa.trunc);
+  return true;
+}
+
+template <class F> __device__ __forceinline__ void dda_band(float3 a, float3 b, float v, F&& visit) {
+  int32_t cx = voxel_to_block_axis(alloc_voxel_axis(a.x, v));
+  int32_t cy = voxel_to_block_axis(alloc_voxel_axis(a.y, v));
+  int32_t cz = voxel_to_block_axis(alloc_voxel_axis(a.z, v));
+  const int32_t ex = voxel_to_block_axis(alloc_voxel_axis(b.x, v));
+  const int32_t ey = voxel_to_block_axis(alloc_voxel_axis(b.y, v));
+  const int32_t ez = voxel_to_block_axis(alloc_voxel_axis(b.z, v));
+  if (cx == ex && cy == ey && cz == ez) return;
+  float dx = b.x - a.x, dy = b.y - a.y, dz = b.z - a.z;
+  const float len = sqrtf(dx * dx + dy * dy + dz * dz);
+  if (len < 1e-6f) return;
+  dx /= len;
+  dy /= len;
+  dz /= len;
+  const float stx = dx > 0.0f ? 1.0f : -1.0f, sty = dy > 0.0f ? 1.0f : -1.0f, stz = dz > 0.0f ? 1.0f : -1.0f;
+  const float ext = (float)kBlock * v;
+  const float scx = stx > 0.0f ? 1.0f : 0.0f, scy = sty > 0.0f ? 1.0f : 0.0f, scz = stz > 0.0f ? 1.0f : 0.0f;
+  const float hv = 0.5f * v;
+  const float bx = ((float)cx + scx) * ext - hv, by = ((float)cy + scy) * ext - hv, bz = ((float)cz + scz) * ext - hv;
+  float tmx = (bx - a.x) / dx, tmy = (by - a.y) / dy, tmz = (bz - a.z) / dz;
+  float tdx = (stx * ext) / dx, tdy = (sty * ext) / dy, tdz = (stz * ext) / dz;
+  if (dx == 0.0f) { tmx = 1e6f; tdx = 1e6f; }
+  if (dy == 0.0f) { tmy = 1e6f; tdy = 1e6f; }
+  if (dz == 0.0f) { tmz = 1e6f; tdz = 1e6f; }
+  const int32_t sx = (int32_t)stx, sy = (int32_t)sty, sz = (int32_t)stz;
+  for (int iter = 0; iter < 1024; ++iter) {
+    visit(cx, cy, cz, iter);
+    if (cx == ex && cy == ey && cz == ez) break;
+    if (tmx < tmy && tmx < tmz) {
+      tmx += tdx;
+      cx += sx;
+    } else if (tmy < tmz) {
+      tmy += tdy;
+      cy += sy;
+    } else {
+      tmz += tdz;
+      cz += sz;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Allocation
+// ---------------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_alloc_claim(TableView t, DevState* st, FrameArgs fa,
+                                                     const float* __restrict__ pts, uint32_t* __restrict__ pixflag) {
+  const int pix = blockIdx.x * 256 + threadIdx.x;
+  const int npix = fa.W * fa.H;
+  uint32_t valid = 0, claimed = 0, oor = 0, nonfinite = 0;
+  uint32_t zmin = 0xFFFFFFFFu, zmax = 0u;
+  if (pix < npix) {
+    const float Px = pts[3 * pix], Py = pts[3 * pix + 1], Pz = pts[3 * pix + 2];
+    if (Px != 0.0f || Py != 0.0f || Pz != 0.0f) {  // integrate-side validity (VoxelScene.cu:1037-1043)
+      if (!(isfinite(Px) && isfinite(Py) && isfinite(Pz))) {
+        nonfinite = 1;
+      } else {
+        zmin = zmax = float_to_ordered(Pz);
+      }
+    }
+    if (!(Pz <= 1e-6f)) valid = 1;
+    float3 lo, hi;
+    if (valid && pixel_band(fa, Px, Py, Pz, &lo, &hi)) {
+      dda_band(lo, hi, fa.voxel, [&](int32_t x, int32_t y, int32_t z, int iter) {
+        if (!key_in_range(x, y, z)) {
+          oor++;
+          return;
+        }
+        const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
+        const int c = t.cnt[h];
+        if (c >= kBucket) return;  // full: present (no-op) or refused (no-op)
+        const uint64_t key = pack_key(x, y, z);
+        const uint64_t* r = t.rec + (size_t)h * kRecWords;
+        bool present = false;
+        for (int j = 0; j < c; ++j) present |= (r[j] == key);
+        if (present) return;
+        atomicMin(&t.claim[h], ((uint32_t)pix << 10) | (uint32_t)iter);
+        claimed = 1;
+      });
+    }
+    pixflag[pix] = claimed;
+  }
+  FrameCounters& fc = st->set[fa.cset];
+  valid = wave_sum(valid);
+  claimed = wave_sum(claimed);
+  oor = wave_sum(oor);
+  nonfinite = wave_sum(nonfinite);
+  zmin = wave_min(zmin);
+  zmax = wave_max(zmax);
+  if ((threadIdx.x & 63) == 0) {
+    if (valid) atomicAdd(&fc.valid_pixels, valid);
+    if (claimed) atomicAdd(&fc.claim_pixels, claimed);
+    if (oor) atomicAdd(&fc.out_of_range, oor);
+    if (nonfinite) atomicAdd(&fc.nonfinite, nonfinite);
+    if (zmin != 0xFFFFFFFFu) atomicMin(&fc.zmin_key, zmin);
+    if (zmax != 0u) atomicMax(&fc.zmax_key, zmax);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_alloc_resolve(TableView t, FrameArgs fa, const float* __restrict__ pts,
+                                                       uint32_t* __restrict__ pixcnt, uint32_t* __restrict__ wgsum) {
+  const int pix = blockIdx.x * 256 + threadIdx.x;
+  const int npix = fa.W * fa.H;
+  uint32_t n = 0;
+  if (pix < npix && pixcnt[pix] != 0u) {
+    const float Px = pts[3 * pix], Py = pts[3 * pix + 1], Pz = pts[3 * pix + 2];
+    float3 lo, hi;
+    if (pixel_band(fa, Px, Py, Pz, &lo, &hi)) {
+      dda_band(lo, hi, fa.voxel, [&](int32_t x, int32_t y, int32_t z, int iter) {
+        if (!key_in_range(x, y, z)) return;
+        const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
+        if (t.claim[h] == (((uint32_t)pix << 10) | (uint32_t)iter)) n++;
+      });
+    }
+    pixcnt[pix] = n;
+  }
+  __shared__ uint32_t ws[4];
+  const uint32_t s = wave_sum(n);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) wgsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+__global__ __launch_bounds__(1024) void k_alloc_scan(DevState* st, uint32_t* __restrict__ wg, uint32_t nwg,
+                                                     int32_t cset, uint32_t num_blocks) {
+  const uint32_t per = (nwg + 1023u) / 1024u;
+  const uint32_t b0 = threadIdx.x * per;
+  uint32_t local = 0;
+  for (uint32_t k = 0; k < per; ++k)
+    if (b0 + k < nwg) local += wg[b0 + k];
+  uint32_t total = 0;
+  uint32_t run = block_excl_scan<1024>(local, &total);
+  for (uint32_t k = 0; k < per; ++k) {
+    if (b0 + k < nwg) {
+      const uint32_t x = wg[b0 + k];
+      wg[b0 + k] = run;
+      run += x;
+    }
+  }
+  if (threadIdx.x == 0) {
+    const uint32_t prev = st->n_alloc;
+    const uint32_t room = num_blocks - prev;
+    const uint32_t ok = total < room ? total : room;
+    st->n_alloc_prev = prev;
+    st->n_alloc = prev + ok;
+    st->frames += 1u;
+    FrameCounters& fc = st->set[cset];
+    fc.winners = total;
+    fc.claimed_buckets = total;
+    fc.heap_fail = total - ok;
+    FrameCounters& nx = st->set[cset ^ 1];
+    nx.valid_pixels = 0;
+    nx.claim_pixels = 0;
+    nx.winners = 0;
+    nx.heap_fail = 0;
+    nx.out_of_range = 0;
+    nx.visible = 0;
+    nx.updated = 0;
+    nx.nonfinite = 0;
+    nx.zmin_key = 0xFFFFFFFFu;
+    nx.zmax_key = 0u;
+    nx.claimed_buckets = 0;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_alloc_insert(TableView t, DevState* st, FrameArgs fa,
+                                                      const float* __restrict__ pts, const uint32_t* __restrict__ pixcnt,
+                                                      const uint32_t* __restrict__ wgoff) {
+  const int pix = blockIdx.x * 256 + threadIdx.x;
+  const int npix = fa.W * fa.H;
+  const uint32_t n = pix < npix ? pixcnt[pix] : 0u;
+  uint32_t tot;
+  uint32_t rank = block_excl_scan<256>(n, &tot);
+  if (n == 0u) return;
+  rank += wgoff[blockIdx.x];
+  const uint32_t prev = st->n_alloc_prev;
+  const float Px = pts[3 * pix], Py = pts[3 * pix + 1], Pz = pts[3 * pix + 2];
+  float3 lo, hi;
+  if (!pixel_band(fa, Px, Py, Pz, &lo, &hi)) return;
+  dda_band(lo, hi, fa.voxel, [&](int32_t x, int32_t y, int32_t z, int iter) {
+    if (!key_in_range(x, y, z)) return;
+    const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
+    if (t.claim[h] != (((uint32_t)pix << 10) | (uint32_t)iter)) return;
+    t.claim[h] = kNoClaim;  // this thread is the bucket's only winner
+    const uint32_t p = prev + rank;
+    rank++;
+    if (p >= t.num_blocks) return;  // heap exhausted (VoxelScene.cu:204-214)
+    const int slot = t.cnt[h];
+    uint64_t* r = t.rec + (size_t)h * kRecWords;
+    r[slot] = pack_key(x, y, z);
+    reinterpret_cast<int32_t*>(r + kBucket)[slot] = (int32_t)p;
+    t.cnt[h] = (uint8_t)(slot + 1);
+    t.blk[p] = make_int4(x, y, z, (int32_t)(h * kBucket + slot));
+  });
+}
+
+// ---------------------------------------------------------------------------------
+// Visible-block compaction
+// ---------------------------------------------------------------------------------
+
+__device__ __forceinline__ float ordered_to_float(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// False only when no voxel of block (x,y,z) can be updated by this frame: every
+// updated voxel lies within trunc of a valid point along the camera z axis, and
+// (for blocks fully in front of the camera) projects inside the image.  Margins
+// cover float rounding of the per-voxel arithmetic many times over.
+__device__ __forceinline__ bool block_maybe_visible(const FrameArgs& fa, int4 b, double zmin, double zmax) {
+  const double* M = fa.Md;
+  const double v = (double)fa.voxel;
+  double lo_z = 1e300, hi_z = -1e300, lo_u = 1e300, hi_u = -1e300, lo_v = 1e300, hi_v = -1e300;
+  bool all_front = true;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const double wx = ((double)b.x * kBlock + ((k & 1) ? 7 : 0)) * v;
+    const double wy = ((double)b.y * kBlock + ((k & 2) ? 7 : 0)) * v;
+    const double wz = ((double)b.z * kBlock + ((k & 4) ? 7 : 0)) * v;
+    const double rx = wx - M[3], ry = wy - M[7], rz = wz - M[11];
+    const double X = M[0] * rx + M[4] * ry + M[8] * rz;
+    const double Y = M[1] * rx + M[5] * ry + M[9] * rz;
+    const double Z = M[2] * rx + M[6] * ry + M[10] * rz;
+    lo_z = Z < lo_z ? Z : lo_z;
+    hi_z = Z > hi_z ? Z : hi_z;
+    if (Z > 1e-4) {
+      const double u = X * (double)fa.fx / Z + (double)fa.cx, vv = Y * (double)fa.fy / Z + (double)fa.cy;
+      lo_u = u < lo_u ? u : lo_u;
+      hi_u = u > hi_u ? u : hi_u;
+      lo_v = vv < lo_v ? vv : lo_v;
+      hi_v = vv > hi_v ? vv : hi_v;
+    } else {
+      all_front = false;
+    }
+  }
+  const double tz = (double)fa.trunc * 1.001 + 1e-4;
+  if (hi_z < zmin - tz || lo_z > zmax + tz) return false;
+  if (all_front) {
+    if (hi_u < -2.5 || lo_u > (double)fa.W + 0.5 || hi_v < -2.5 || lo_v > (double)fa.H + 0.5) return false;
+  }
+  return true;
+}
+
+__global__ __launch_bounds__(256) void k_compact(TableView t, DevState* st, FrameArgs fa, uint32_t* __restrict__ vis) {
+  const uint32_t n = st->n_alloc;
+  FrameCounters& fc = st->set[fa.cset];
+  const bool any_valid = fc.zmin_key != 0xFFFFFFFFu || fc.nonfinite != 0u;
+  const bool cull = fa.cull_ok != 0 && fc.nonfinite == 0u;
+  const double zmin = any_valid && cull ? (double)ordered_to_float(fc.zmin_key) : 0.0;
+  const double zmax = any_valid && cull ? (double)ordered_to_float(fc.zmax_key) : 0.0;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+    const uint32_t i = base + threadIdx.x;
+    bool keep = false;
+    if (i < n && any_valid) keep = !cull || block_maybe_visible(fa, t.blk[i], zmin, zmax);
+    const uint64_t m = __ballot(keep);
+    const uint32_t cnt = (uint32_t)__popcll(m);
+    uint32_t off = 0;
+    if (lane == 0 && cnt) off = atomicAdd(&fc.visible, cnt);
+    off = __shfl(off, 0, 64);
+    if (keep) vis[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Integration (VoxelScene.cu:918-1146)
+// ---------------------------------------------------------------------------------
+
+// FP32 store: uint2 {sdf bits, weight}; FP16 store: uint32 {half sdf | weight << 16}.
+// Colour plane: uint32 {r | g << 8 | b << 16 | flags << 24}.
+template <bool F16> struct Store;
+template <> struct Store<false> {
+  using word = uint2;
+  __device__ static float sdf(word w) { return __uint_as_float(w.x); }
+  __device__ static uint32_t weight(word w) { return w.y & 0xFFFFu; }
+  __device__ static word make(float s, uint32_t wt) { return make_uint2(__float_as_uint(s), wt); }
+};
+template <> struct Store<true> {
+  using word = uint32_t;
+  __device__ static float sdf(word w) { return __half2float(__ushort_as_half((unsigned short)(w & 0xFFFFu))); }
+  __device__ static uint32_t weight(word w) { return w >> 16; }
+  __device__ static word make(float s, uint32_t wt) {
+    return (uint32_t)__half_as_ushort(__float2half_rn(s)) | (wt << 16);
+  }
+};
+
+template <bool F16>
+__global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, FrameArgs fa,
+                                                   const float* __restrict__ pts, const uint8_t* __restrict__ rgb,
+                                                   const uint32_t* __restrict__ vis, void* __restrict__ vox_,
+                                                   uint32_t* __restrict__ col) {
+  using S = Store<F16>;
+  typename S::word* vox = reinterpret_cast<typename S::word*>(vox_);
+  FrameCounters& fc = st->set[fa.cset];
+  const uint32_t nvis = fc.visible;
+  const int idx = threadIdx.x;
+  const int lx = idx % kBlock, ly = (idx / kBlock) % kBlock, lz = idx / (kBlock * kBlock);
+  const float* M = fa.M;
+  const float v = fa.voxel, tr = fa.trunc;
+  const float camx = M[3], camy = M[7], camz = M[11];
+  uint32_t upd = 0;
+  for (uint32_t i = blockIdx.x; i < nvis; i += gridDim.x) {
+    const uint32_t ptr = vis[i];
+    const int4 b = t.blk[ptr];
+    const float vwx = (float)(b.x * kBlock + lx) * v;
+    const float vwy = (float)(b.y * kBlock + ly) * v;
+    const float vwz = (float)(b.z * kBlock + lz) * v;
+    const float rx = vwx - M[3], ry = vwy - M[7], rz = vwz - M[11];
+    const float vcx = M[0] * rx + M[4] * ry + M[8] * rz;
+    const float vcy = M[1] * rx + M[5] * ry + M[9] * rz;
+    const float vcz = M[2] * rx + M[6] * ry + M[10] * rz;
+    const float sxf = vcx * fa.fx / vcz + fa.cx;
+    const float syf = vcy * fa.fy / vcz + fa.cy;
+    const int32_t px = cvt_i32(sxf + 0.5f), py = cvt_i32(syf + 0.5f);
+    if (!(px >= 0 && px < fa.W && py >= 0 && py < fa.H)) continue;
+    const int pix = py * fa.W + px;
+    const float Px = pts[3 * pix], Py = pts[3 * pix + 1], Pz = pts[3 * pix + 2];
+    if (!(Px != 0.0f || Py != 0.0f || Pz != 0.0f)) continue;
+    const float3 D = xform_point(M, Px, Py, Pz);
+    const float sdx = vwx - D.x, sdy = vwy - D.y, sdz = vwz - D.z;
+    float sdf = sqrtf(sdx * sdx + sdy * sdy + sdz * sdz);
+    const float cdx = D.x - camx, cdy = D.y - camy, cdz = D.z - camz;
+    const float cam_dist = sqrtf(cdx * cdx + cdy * cdy + cdz * cdz);
+    const float vox_dist =
+        sqrtf((vwx - camx) * (vwx - camx) + (vwy - camy) * (vwy - camy) + (vwz - camz) * (vwz - camz));
+    if (vox_dist < cam_dist) sdf = -sdf;
+    if (sdf < -tr || sdf > tr) continue;
+    sdf = fmaxf(-tr, fminf(tr, sdf));
+    const size_t vi = (size_t)ptr * kVox + (size_t)idx;
+    const typename S::word w = vox[vi];
+    const uint32_t c = col[vi];
+    const float old_sdf = S::sdf(w);
+    const float old_w = (float)S::weight(w);
+    const float new_w = old_w + 1.0f;
+    if (new_w > 0.0f) {
+      const float nsdf = (old_sdf * old_w + sdf * 1.0f) / new_w;
+      const uint32_t nw = (uint32_t)fminf(255.0f, new_w);
+      const typename S::word nwd = S::make(nsdf, nw);
+      const float stored = S::sdf(nwd);
+      const uint8_t* cp = rgb + 3 * pix;
+      uint32_t nc = 0;
+#pragma unroll
+      for (int ch = 0; ch < 3; ++ch) {
+        const float oc = (float)((c >> (8 * ch)) & 0xFFu);
+        const float ncf = (oc * old_w + (float)cp[ch] * 1.0f) / new_w;
+        nc |= ((uint32_t)ncf & 0xFFu) << (8 * ch);
+      }
+      uint32_t fl = (c >> 24) | 1u;
+      if ((old_sdf > 0.0f && stored <= 0.0f) || (old_sdf <= 0.0f && stored > 0.0f)) fl |= 2u;
+      vox[vi] = nwd;
+      col[vi] = nc | (fl << 24);
+      upd++;
+    }
+  }
+  upd = wave_sum(upd);
+  if ((threadIdx.x & 63) == 0 && upd) atomicAdd(&fc.updated, upd);
+}
+
+// ---------------------------------------------------------------------------------
+// Raycast (RayCastRender.cu:127-517)
+// ---------------------------------------------------------------------------------
+
+// Two most recent block lookups of this ray.
+struct BlockCache {
+  int32_t x0, y0, z0, p0;
+  int32_t x1, y1, z1, p1;
+  __device__ void init() {
+    x0 = y0 = z0 = INT32_MIN;
+    x1 = y1 = z1 = INT32_MIN;
+    p0 = p1 = -1;
+  }
+  __device__ __forceinline__ int32_t get(const TableView& t, int32_t x, int32_t y, int32_t z) {
+    if (x == x0 && y == y0 && z == z0) return p0;
+    if (x == x1 && y == y1 && z == z1) return p1;
+    const int32_t p = table_lookup(t, x, y, z);
+    x1 = x0; y1 = y0; z1 = z0; p1 = p0;
+    x0 = x; y0 = y; z0 = z; p0 = p;
+    return p;
+  }
+};
+
+// trilinearInterpolationSimpleFastFast (RayCastRender.cu:127-215).  The eight
+// corners' voxel indices come from two quotients per axis (posDual/v and
+// (posDual+v)/v): the reference computes the same quotient once per corner.
+template <bool F16, bool COLOR>
+__device__ __forceinline__ bool trilinear(const TableView& t, BlockCache& bc, const void* __restrict__ vox_,
+                                          const uint32_t* __restrict__ col, float px, float py, float pz, float v,
+                                          float* dist, uint32_t* rgb) {
+  using S = Store<F16>;
+  const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
+  const float off = v;
+  const float pdx = px - off / 2.0f, pdy = py - off / 2.0f, pdz = pz - off / 2.0f;
+  const float qx0 = pdx / v, qy0 = pdy / v, qz0 = pdz / v;
+  const float wx = qx0 - floorf(qx0), wy = qy0 - floorf(qy0), wz = qz0 - floorf(qz0);
+  const float qx1 = (pdx + off) / v, qy1 = (pdy + off) / v, qz1 = (pdz + off) / v;
+  const int32_t vx[2] = {ray_voxel_of_q(qx0), ray_voxel_of_q(qx1)};
+  const int32_t vy[2] = {ray_voxel_of_q(qy0), ray_voxel_of_q(qy1)};
+  const int32_t vz[2] = {ray_voxel_of_q(qz0), ray_voxel_of_q(qz1)};
+  float d = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, wsum = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const int32_t ix = vx[i & 1], iy = vy[(i >> 1) & 1], iz = vz[(i >> 2) & 1];
+    const int32_t p = bc.get(t, voxel_to_block_axis(ix), voxel_to_block_axis(iy), voxel_to_block_axis(iz));
+    if (p < 0) continue;
+    const size_t vi = (size_t)p * kVox +
+                      (size_t)(voxel_local_axis(ix) + voxel_local_axis(iy) * kBlock + voxel_local_axis(iz) * kBlock * kBlock);
+    const typename S::word w = vox[vi];
+    if (S::weight(w) == 0u) continue;
+    const float wi = ((i & 1) ? wx : (1.0f - wx)) * ((i & 2) ? wy : (1.0f - wy)) * ((i & 4) ? wz : (1.0f - wz));
+    d += wi * S::sdf(w);
+    if (COLOR) {
+      const uint32_t c = col[vi];
+      cr += wi * (float)(c & 0xFFu);
+      cg += wi * (float)((c >> 8) & 0xFFu);
+      cb += wi * (float)((c >> 16) & 0xFFu);
+    }
+    wsum += wi;
+  }
+  if (wsum <= 1e-6f) return false;
+  const float inv = 1.0f / wsum;
+  *dist = d * inv;
+  if (COLOR) {
+    cr *= inv;
+    cg *= inv;
+    cb *= inv;
+    *rgb = ((uint32_t)cr & 0xFFu) | (((uint32_t)cg & 0xFFu) << 8) | (((uint32_t)cb & 0xFFu) << 16);
+  }
+  return true;
+}
+
+template <bool F16>
+__global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __restrict__ st, RenderArgs ra,
+                                               const void* __restrict__ vox, const uint32_t* __restrict__ col,
+                                               float* __restrict__ out_depth, float4* __restrict__ out_pos,
+                                               float4* __restrict__ out_col, float4* __restrict__ out_nrm,
+                                               uint32_t* __restrict__ rc) {
+  if (st->n_alloc == 0u) return;  // RayCastRender.cu:553-556: nothing to render, outputs untouched
+  const uint32_t x = blockIdx.x * 8u + (threadIdx.x & 7u);
+  const uint32_t y = blockIdx.y * 8u + (threadIdx.x >> 3);
+  uint32_t hit = 0, miss = 0;
+  if (x < (uint32_t)ra.W && y < (uint32_t)ra.H) {
+    const float MINF = 1e10f;
+    const int pix = (int)y * ra.W + (int)x;
+    float o_depth = MINF;
+    float4 o_pos = make_float4(MINF, MINF, MINF, MINF), o_col = o_pos;
+    float cdx = ((float)x - ra.cx) / ra.fx, cdy = ((float)y - ra.cy) / ra.fy, cdz = 1.0f;
+    const float cl = sqrtf(cdx * cdx + cdy * cdy + cdz * cdz);
+    if (cl > 1e-6f) {
+      cdx /= cl;
+      cdy /= cl;
+      cdz /= cl;
+    }
+    const float* M = ra.M;
+    float wdx = M[0] * cdx + M[1] * cdy + M[2] * cdz;
+    float wdy = M[4] * cdx + M[5] * cdy + M[6] * cdz;
+    float wdz = M[8] * cdx + M[9] * cdy + M[10] * cdz;
+    const float wl = sqrtf(wdx * wdx + wdy * wdy + wdz * wdz);
+    if (wl > 1e-6f) {
+      wdx /= wl;
+      wdy /= wl;
+      wdz /= wl;
+    }
+    const float minD = ra.minD, maxD = ra.maxD;
+    if (!(minD == 0.0f || minD == MINF) && !(maxD == 0.0f || maxD == MINF)) {
+      const float v = ra.voxel;
+      const float camx = ra.cam[0], camy = ra.cam[1], camz = ra.cam[2];
+      const float d2r = 1.0f / cdz;
+      float ray = d2r * fmaxf(minD, minD);
+      const float ray_end = d2r * fminf(maxD, maxD);
+      const float th = v * 3.0f;
+      const float inc = v * 0.5f;
+      float last_sdf = 0.0f, last_alpha = 0.0f;
+      uint32_t last_w = 0;
+      BlockCache bc;
+      bc.init();
+      bool found = false;
+      while (ray < ray_end) {
+        const float ppx = camx + ray * wdx, ppy = camy + ray * wdy, ppz = camz + ray * wdz;
+        float dist;
+        if (trilinear<F16, false>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr)) {
+          if (last_w > 0u && last_sdf < 0.0f && dist > 0.0f) {
+            // findIntersectionBisection (RayCastRender.cu:229-271)
+            float a = last_alpha, ad = last_sdf, bb = ray, bd = dist, c = 0.0f;
+            uint32_t rgb = 0;
+            bool ok = true;
+            for (int it = 0; it < 3; ++it) {
+              c = a + (ad / (ad - bd)) * (bb - a);
+              float cd;
+              if (!trilinear<F16, true>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd,
+                                        &rgb)) {
+                ok = false;
+                break;
+              }
+              if (ad * cd > 0.0f) {
+                a = c;
+                ad = cd;
+              } else {
+                bb = c;
+                bd = cd;
+              }
+            }
+            if (ok && fabsf(last_sdf - dist) < th && fabsf(dist) < th) {
+              const float alpha = c;
+              o_depth = alpha / d2r;
+              o_pos = make_float4(camx + alpha * wdx, camy + alpha * wdy, camz + alpha * wdz, 1.0f);
+              o_col = make_float4((float)(rgb & 0xFFu) / 255.0f, (float)((rgb >> 8) & 0xFFu) / 255.0f,
+                                  (float)((rgb >> 16) & 0xFFu) / 255.0f, 1.0f);
+              found = true;
+              break;
+            }
+          }
+          last_sdf = dist;
+          last_alpha = ray;
+          last_w = 1u;
+        } else {
+          last_w = 0u;
+        }
+        const float next = ray + inc;
+        if (next == ray) break;  // step absorbed by rounding: the reference would never end
+        ray = next;
+      }
+      hit = found ? 1u : 0u;
+      miss = found ? 0u : 1u;
+    }
+    out_depth[pix] = o_depth;
+    out_pos[pix] = o_pos;
+    out_col[pix] = o_col;
+    out_nrm[pix] = make_float4(MINF, MINF, MINF, MINF);
+  }
+  hit = wave_sum(hit);
+  miss = wave_sum(miss);
+  if (threadIdx.x == 0) {
+    if (hit) atomicAdd(&rc[0], hit);
+    if (miss) atomicAdd(&rc[1], miss);
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Reset
+// ---------------------------------------------------------------------------------
+
+// Zeroes the voxels of the blocks that are live.  The reference's reset
+// (VoxelScene.cu:1166-1236) leaves voxel memory as it was, so a block handed out
+// again after a reset would start from stale values; zeroing them keeps a reset
+// scene equal to a new one.  Runs before k_reset_table, which clears n_alloc.
+__global__ __launch_bounds__(256) void k_reset_voxels(const DevState* st, void* vox, uint32_t word_bytes,
+                                                      uint32_t* col) {
+  const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * 256;
+  const size_t nvox = (size_t)st->n_alloc * kVox;
+  for (size_t i = tid; i < nvox; i += stride) {
+    if (word_bytes == 8) reinterpret_cast<uint2*>(vox)[i] = make_uint2(0u, 0u);
+    else reinterpret_cast<uint32_t*>(vox)[i] = 0u;
+    col[i] = 0u;
+  }
+}
+
+// Empties the table (no used slot, no claim) and initialises the scene state and
+// both counter sets (VoxelScene.cu:250-284: heap full, table free, mutexes open).
+__global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) {
+  const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * 256;
+  for (size_t b = tid; b < t.num_buckets; b += stride) {
+    t.cnt[b] = 0;
+    t.claim[b] = kNoClaim;
+  }
+  if (tid == 0) {
+    st->n_alloc = 0;
+    st->n_alloc_prev = 0;
+    st->frames = 0;
+    for (int s = 0; s < 2; ++s) {
+      FrameCounters& f = st->set[s];
+      f.valid_pixels = f.claim_pixels = f.winners = f.heap_fail = f.out_of_range = 0;
+      f.visible = f.updated = f.nonfinite = f.claimed_buckets = 0;
+      f.zmin_key = 0xFFFFFFFFu;
+      f.zmax_key = 0u;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------------
+
+void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col) {
+  hipLaunchKernelGGL(k_reset_voxels, dim3(2048), dim3(256), 0, s, st, vox, word_bytes, col);
+}
+
+void launch_reset_table(hipStream_t s, const TableView& t, DevState* st) {
+  hipLaunchKernelGGL(k_reset_table, dim3(1024), dim3(256), 0, s, t, st);
+}
+
+void launch_alloc_claim(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
+                        uint32_t* pixcnt) {
+  const uint32_t nwg = (uint32_t)((fa.W * fa.H + 255) / 256);
+  hipLaunchKernelGGL(k_alloc_claim, dim3(nwg), dim3(256), 0, s, t, st, fa, pts, pixcnt);
+}
+
+void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
+                          uint32_t* wgsum) {
+  const uint32_t nwg = (uint32_t)((fa.W * fa.H + 255) / 256);
+  hipLaunchKernelGGL(k_alloc_resolve, dim3(nwg), dim3(256), 0, s, t, fa, pts, pixcnt, wgsum);
+}
+
+void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, uint32_t nwg, int32_t cset, uint32_t num_blocks) {
+  hipLaunchKernelGGL(k_alloc_scan, dim3(1), dim3(1024), 0, s, st, wgsum, nwg, cset, num_blocks);
+}
+
+void launch_alloc_insert(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
+                         const uint32_t* pixcnt, const uint32_t* wgoff) {
+  const uint32_t nwg = (uint32_t)((fa.W * fa.H + 255) / 256);
+  hipLaunchKernelGGL(k_alloc_insert, dim3(nwg), dim3(256), 0, s, t, st, fa, pts, pixcnt, wgoff);
+}
+
+void launch_compact(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, uint32_t* vis,
+                    uint32_t grid) {
+  hipLaunchKernelGGL(k_compact, dim3(grid), dim3(256), 0, s, t, st, fa, vis);
+}
+
+void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
+                      const float* pts, const uint8_t* rgb, const uint32_t* vis, void* vox, uint32_t* col,
+                      uint32_t grid) {
+  if (fp16)
+    hipLaunchKernelGGL(k_integrate<true>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col);
+  else
+    hipLaunchKernelGGL(k_integrate<false>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col);
+}
+
+void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
+                   const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,
+                   uint32_t* rc) {
+  const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
+  if (fp16)
+    hipLaunchKernelGGL(k_render<true>, grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+  else
+    hipLaunchKernelGGL(k_render<false>, grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+}
+
+}  // namespace vsg

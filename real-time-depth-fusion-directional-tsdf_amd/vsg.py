@@ -1,0 +1,463 @@
+"""Host mirror of the reference's VoxelScene / RayCastRender API over the C ABI.
+
+Class and method names, argument order and meaning follow
+VoxelSG/include/VoxelScene.h:52-209 and VoxelSG/include/RayCastRender.h:20-123;
+the work runs in lib/libvsg.so (hand-written HIP kernels for gfx950) through the
+C ABI of include/vsg.h.  Device buffers are passed as torch CUDA tensors (or raw
+device addresses); PyTorch only provides device memory and streams here.
+
+Error behaviour mirrors the reference where it has one: ``initialize`` returns
+False on failure (RayCastRender.cpp:105-111); calls on an unallocated object
+report and return.  Everything else raises :class:`VSGError` with the library's
+message, and loading fails loudly when lib/libvsg.so is missing: there is no
+CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "lib", "libvsg.so")
+
+VSG_OK = 0
+STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "NOT_READY"}
+STORE_FP32 = 0
+STORE_FP16 = 1
+KERNELS = ["alloc_claim", "alloc_resolve", "alloc_scan", "alloc_insert", "compact", "integrate", "raycast",
+           "reset"]
+
+SDF_BLOCK_SIZE = 8      # VoxelSG/include/globalPramasConfig.h:8
+HASH_BUCKET_SIZE = 10   # VoxelSG/include/globalPramasConfig.h:9
+
+
+class VSGError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"vsg status {STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class vsg_params(C.Structure):
+    _fields_ = [("hash_slot_num", C.c_int32), ("sdf_block_num", C.c_int32), ("voxel_size", C.c_float),
+                ("store", C.c_int32), ("reserved", C.c_int32 * 4)]
+
+
+class vsg_intrinsics(C.Structure):
+    _fields_ = [("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float)]
+
+
+class vsg_frame(C.Structure):
+    _fields_ = [("points", C.c_void_p), ("colors", C.c_void_p), ("normals", C.c_void_p),
+                ("width", C.c_int32), ("height", C.c_int32)]
+
+
+class vsg_counters(C.Structure):
+    _fields_ = [("valid_pixels", C.c_uint32), ("new_blocks", C.c_uint32), ("heap_fail", C.c_uint32),
+                ("num_allocated", C.c_uint32), ("visible_blocks", C.c_uint32), ("updated_voxels", C.c_uint32),
+                ("claimed_buckets", C.c_uint32), ("out_of_range", C.c_uint32), ("frames", C.c_uint32),
+                ("reserved", C.c_uint32 * 7)]
+
+
+# Every symbol include/vsg.h declares, with its ctypes signature.
+_SIGNATURES = {
+    "vsg_version": (C.c_int32, []),
+    "vsg_last_error": (C.c_char_p, []),
+    "vsg_scene_create": (C.c_int, [C.POINTER(vsg_params), C.POINTER(C.c_void_p)]),
+    "vsg_scene_destroy": (C.c_int, [C.c_void_p]),
+    "vsg_scene_reset": (C.c_int, [C.c_void_p]),
+    "vsg_scene_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vsg_scene_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "vsg_integrate": (C.c_int, [C.c_void_p, C.POINTER(vsg_frame), C.c_float, C.POINTER(C.c_float),
+                                C.POINTER(vsg_intrinsics)]),
+    "vsg_scene_counters": (C.c_int, [C.c_void_p, C.POINTER(vsg_counters)]),
+    "vsg_heap_free_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "vsg_num_integrated_frames": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "vsg_scene_params": (C.c_int, [C.c_void_p, C.POINTER(vsg_params)]),
+    "vsg_export_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "vsg_export_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vsg_render_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "vsg_render_destroy": (C.c_int, [C.c_void_p]),
+    "vsg_render_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vsg_render_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                   C.POINTER(vsg_intrinsics), C.c_float, C.c_float, C.c_float]),
+    "vsg_render_buffers": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4),
+    "vsg_render_download": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
+    "vsg_render_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vsg_timing_enable": (C.c_int, [C.c_int32]),
+    "vsg_timing_reset": (C.c_int, []),
+    "vsg_timing_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Loads lib/libvsg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with build_lib.py (no CPU fallback exists)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols() -> list[str]:
+    return list(_SIGNATURES)
+
+
+def _check(status: int) -> None:
+    if status != VSG_OK:
+        raise VSGError(status, lib().vsg_last_error().decode(errors="replace"))
+
+
+def _dev_ptr(x) -> int:
+    """Device address of a torch CUDA tensor (contiguous) or an int address."""
+    if x is None:
+        return 0
+    if isinstance(x, int):
+        return x
+    if hasattr(x, "data_ptr"):
+        if not x.is_cuda:
+            raise ValueError("device buffers must be CUDA tensors")
+        if not x.is_contiguous():
+            raise ValueError("device buffers must be contiguous")
+        return int(x.data_ptr())
+    raise TypeError(f"cannot take a device address of {type(x)!r}")
+
+
+def _pose16(T) -> C.Array:
+    a = np.ascontiguousarray(np.asarray(T, dtype=np.float32).reshape(16))
+    return (C.c_float * 16)(*a.tolist())
+
+
+class Params:
+    """Params (VoxelSG/include/CUDAHashData.h:9-42).
+
+    ``truncationScale``/``truncation`` keep the reference's names; the reference
+    swaps them when filling from the config (CUDAHashData.h:27-28) and its
+    integrate call takes the truncation as an argument instead (voxelSG.cpp:501).
+    """
+
+    def __init__(self, hashSlotNum: int, SDFBlockNum: int, voxelSize: float, truncation: float = 0.02,
+                 truncationScale: float = 0.0):
+        self.hashSlotNum = int(hashSlotNum)
+        self.SDFBlockNum = int(SDFBlockNum)
+        self.SDFBlockSize = SDF_BLOCK_SIZE
+        self.slotSize = HASH_BUCKET_SIZE
+        self.totalHashSize = self.hashSlotNum * HASH_BUCKET_SIZE
+        self.totalBlockSize = self.SDFBlockNum * SDF_BLOCK_SIZE ** 3
+        self.voxelSize = float(voxelSize)
+        self.truncation = float(truncation)
+        self.truncationScale = float(truncationScale)
+
+    def __repr__(self) -> str:
+        return (f"Params(hashSlotNum={self.hashSlotNum}, SDFBlockNum={self.SDFBlockNum}, "
+                f"voxelSize={self.voxelSize}, truncation={self.truncation})")
+
+
+class VoxelScene:
+    """VoxelScene (VoxelSG/include/VoxelScene.h:52-209).
+
+    store: "fp32" (f32 SDF + u16 weight) or "fp16" (f16 SDF + u16 weight).
+    """
+
+    def __init__(self, params: Params, store: str = "fp32"):
+        self.m_params = params
+        self.m_store = store
+        p = vsg_params()
+        p.hash_slot_num = params.hashSlotNum
+        p.sdf_block_num = params.SDFBlockNum
+        p.voxel_size = params.voxelSize
+        p.store = {"fp32": STORE_FP32, "fp16": STORE_FP16}[store]
+        h = C.c_void_p()
+        _check(lib().vsg_scene_create(C.byref(p), C.byref(h)))
+        self._h = h
+        self.m_isAllocated = True
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_h", None) and self._h.value:
+            _check(lib().vsg_scene_destroy(self._h))
+            self._h = C.c_void_p()
+            self.m_isAllocated = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    # -- reference API --------------------------------------------------------
+    def reset(self) -> None:
+        """VoxelScene::reset (VoxelSG/src/VoxelScene.cpp:276-302)."""
+        if not self.m_isAllocated:
+            print("VoxelScene: Not allocated yet!", file=sys.stderr)
+            return
+        _check(lib().vsg_scene_reset(self._h))
+
+    def integrateFromScanData(self, depthmap, colormap, normalmap, width: int, height: int,
+                              truncationDistance: float, cameraTransform, fx: float, fy: float, cx: float,
+                              cy: float) -> None:
+        """VoxelScene::integrateFromScanData (VoxelSG/src/VoxelScene.cpp:404-833).
+
+        depthmap: device float3[H*W] camera-space points; colormap: device
+        uchar3[H*W]; normalmap: device float3[H*W] (may be None);
+        cameraTransform: host 4x4 row-major camera->world.
+        """
+        if depthmap is None or colormap is None:
+            print("VoxelScene: Invalid pointers (nullptr)!", file=sys.stderr)
+            return
+        if not self.m_isAllocated:
+            print("VoxelScene: Not allocated!", file=sys.stderr)
+            return
+        f = vsg_frame(_dev_ptr(depthmap), _dev_ptr(colormap), _dev_ptr(normalmap), int(width), int(height))
+        k = vsg_intrinsics(fx, fy, cx, cy)
+        _check(lib().vsg_integrate(self._h, C.byref(f), float(truncationDistance), _pose16(cameraTransform),
+                                   C.byref(k)))
+
+    def compactifyHashEntries(self) -> None:
+        """Compaction runs inside every integrate call; nothing to do here."""
+
+    def getHeapFreeCount(self) -> int:
+        """VoxelScene::getHeapFreeCount (VoxelSG/src/VoxelScene.cpp:370-375)."""
+        n = C.c_uint32()
+        _check(lib().vsg_heap_free_count(self._h, C.byref(n)))
+        return n.value
+
+    def getNumIntegratedFrames(self) -> int:
+        n = C.c_uint32()
+        _check(lib().vsg_num_integrated_frames(self._h, C.byref(n)))
+        return n.value
+
+    def isAllocated(self) -> bool:
+        return self.m_isAllocated
+
+    def getParams(self) -> Params:
+        return self.m_params
+
+    def printStats(self) -> None:
+        """VoxelScene::printStats (VoxelSG/src/VoxelScene.cpp:380-399)."""
+        p = self.m_params
+        print("\n=== VoxelScene Stats ===")
+        print(f"Hash Slots: {p.hashSlotNum}")
+        print(f"Bucket Size: {p.slotSize}")
+        print(f"Total Hash Entries: {p.totalHashSize}")
+        print(f"SDF Blocks: {p.SDFBlockNum}")
+        print(f"Block Size: {p.SDFBlockSize}x{p.SDFBlockSize}x{p.SDFBlockSize}")
+        print(f"Total Voxels: {p.totalBlockSize}")
+        print(f"Integrated Frames: {self.getNumIntegratedFrames()}")
+        if self.m_isAllocated:
+            free = self.getHeapFreeCount()
+            print(f"Free Blocks: {free} / {p.SDFBlockNum} ({free * 100.0 / p.SDFBlockNum}%)")
+        print("========================\n")
+
+    def saveToMesh(self, filename: str) -> bool:
+        """Stub in the reference (VoxelSG/src/VoxelScene.cpp:838-845): returns False."""
+        print("VoxelScene::saveToMesh: Not implemented yet", file=sys.stderr)
+        return False
+
+    # -- extensions -----------------------------------------------------------
+    def set_stream(self, hip_stream: int) -> None:
+        _check(lib().vsg_scene_set_stream(self._h, C.c_void_p(hip_stream)))
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        _check(lib().vsg_scene_get_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    def counters(self) -> dict:
+        c = vsg_counters()
+        _check(lib().vsg_scene_counters(self._h, C.byref(c)))
+        return {name: getattr(c, name) for name, _ in vsg_counters._fields_ if name != "reserved"}
+
+    def export_blocks(self):
+        """(coords int32 [n,3], hash entry indices int32 [n]) in pointer order."""
+        n = C.c_uint32()
+        _check(lib().vsg_export_blocks(self._h, None, None, 0, C.byref(n)))
+        coords = np.zeros((n.value, 3), np.int32)
+        slots = np.zeros(n.value, np.int32)
+        if n.value:
+            _check(lib().vsg_export_blocks(self._h, coords.ctypes.data, slots.ctypes.data, n.value, C.byref(n)))
+        return coords, slots
+
+    def export_voxels(self, first: int = 0, nblocks: int | None = None):
+        """(sdf float32 [n*512], weight uint16 [n*512], rgbf uint8 [n*512,4]) of blocks [first, first+n)."""
+        if nblocks is None:
+            nblocks = self.m_params.SDFBlockNum - self.getHeapFreeCount() - first
+        nv = nblocks * 512
+        sdf = np.zeros(nv, np.float32)
+        w = np.zeros(nv, np.uint16)
+        rgbf = np.zeros((nv, 4), np.uint8)
+        if nblocks:
+            _check(lib().vsg_export_voxels(self._h, first, nblocks, sdf.ctypes.data, w.ctypes.data,
+                                           rgbf.ctypes.data))
+        return sdf, w, rgbf
+
+
+class RayCastRender:
+    """RayCastRender (VoxelSG/include/RayCastRender.h:20-123)."""
+
+    def __init__(self):
+        self.m_isAllocated = False
+        self.m_width = 0
+        self.m_height = 0
+        self._h = C.c_void_p()
+
+    def initialize(self, width: int, height: int) -> bool:
+        """RayCastRender::initialize (VoxelSG/src/RayCastRender.cpp:88-118)."""
+        if self.m_isAllocated:
+            print("RayCastRender: Already initialized!", file=sys.stderr)
+            return False
+        h = C.c_void_p()
+        st = lib().vsg_render_create(int(width), int(height), C.byref(h))
+        if st != VSG_OK:
+            print("RayCastRender: Failed to allocate GPU memory! "
+                  + lib().vsg_last_error().decode(errors="replace"), file=sys.stderr)
+            return False
+        self._h = h
+        self.m_width, self.m_height = int(width), int(height)
+        self.m_isAllocated = True
+        return True
+
+    def close(self) -> None:
+        if self._h.value:
+            _check(lib().vsg_render_destroy(self._h))
+            self._h = C.c_void_p()
+            self.m_isAllocated = False
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def render(self, voxelScene: VoxelScene, cameraPos, cameraTransform, fx: float, fy: float, cx: float,
+               cy: float, minDepth: float, maxDepth: float, voxelSize: float = 0.0) -> None:
+        """RayCastRender::render (VoxelSG/src/RayCastRender.cpp:123-183).
+
+        cameraPos: 3 floats; cameraTransform: host 4x4 row-major camera->world.
+        voxelSize <= 0 uses the scene's (the reference reads the config's, which
+        equals Params::voxelSize).
+        """
+        if not self.m_isAllocated:
+            print("RayCastRender: Not initialized!", file=sys.stderr)
+            return
+        if voxelScene is None or not voxelScene.isAllocated():
+            print("RayCastRender: Invalid voxel scene!", file=sys.stderr)
+            return
+        cp = np.asarray(cameraPos, dtype=np.float32).reshape(3)
+        k = vsg_intrinsics(fx, fy, cx, cy)
+        _check(lib().vsg_render_frame(self._h, voxelScene.handle, (C.c_float * 3)(*cp.tolist()),
+                                      _pose16(cameraTransform), C.byref(k), float(minDepth), float(maxDepth),
+                                      float(voxelSize)))
+
+    def _download(self, depth=False, pos=False, col=False, nrm=False):
+        n = self.m_width * self.m_height
+        out = {}
+        bufs = []
+        for flag, name, shape in ((depth, "depth", (n,)), (pos, "pos", (n, 4)), (col, "col", (n, 4)),
+                                  (nrm, "nrm", (n, 4))):
+            if flag:
+                out[name] = np.zeros(shape, np.float32)
+                bufs.append(out[name].ctypes.data)
+            else:
+                bufs.append(None)
+        _check(lib().vsg_render_download(self._h, *bufs))
+        return out
+
+    def downloadResults(self):
+        """(depth float4 [n], colour float4 [n], normal float4 [n]) as the reference fills them
+        (RayCastRender.cpp:188-214): depth replicated into xyz, w = 1."""
+        r = self._download(depth=True, col=True, nrm=True)
+        d = r["depth"]
+        depth4 = np.stack([d, d, d, np.ones_like(d)], axis=1)
+        return depth4, r["col"], r["nrm"]
+
+    def downloadDepthFloat(self) -> np.ndarray:
+        return self._download(depth=True)["depth"]
+
+    def downloadAll(self) -> dict:
+        """depth [n], positions [n,4], colours [n,4], normals [n,4]."""
+        return self._download(True, True, True, True)
+
+    def savePointCloudPLY(self, filePath: str) -> bool:
+        """RayCastRender::savePointCloudPLY (VoxelSG/src/RayCastRender.cpp:221-302): ASCII PLY of
+        the hit pixels' world positions and colours."""
+        if not self.m_isAllocated:
+            print("RayCastRender: Not initialized!", file=sys.stderr)
+            return False
+        r = self._download(depth=True, pos=True, col=True)
+        ok = (r["depth"] < 1e9) & np.isfinite(r["pos"][:, :3]).all(axis=1)
+        pts, cols = r["pos"][ok, :3], np.clip(r["col"][ok, :3] * 255.0, 0, 255).astype(np.uint8)
+        with open(filePath, "w") as fh:
+            fh.write("ply\nformat ascii 1.0\n")
+            fh.write(f"element vertex {len(pts)}\n")
+            fh.write("property float x\nproperty float y\nproperty float z\n")
+            fh.write("property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n")
+            for p, c in zip(pts, cols):
+                fh.write(f"{p[0]} {p[1]} {p[2]} {c[0]} {c[1]} {c[2]}\n")
+        return True
+
+    def counters(self):
+        h, m = C.c_uint32(), C.c_uint32()
+        _check(lib().vsg_render_counters(self._h, C.byref(h), C.byref(m)))
+        return h.value, m.value
+
+    def buffers(self) -> dict:
+        ptrs = [C.c_void_p() for _ in range(4)]
+        _check(lib().vsg_render_buffers(self._h, *[C.byref(p) for p in ptrs]))
+        return dict(zip(("depth", "pos", "col", "nrm"), (p.value for p in ptrs)))
+
+    def getOutputDepth(self) -> int:
+        return self.buffers()["depth"]
+
+    def getOutputColor(self) -> int:
+        return self.buffers()["col"]
+
+    def getOutputNormal(self) -> int:
+        return self.buffers()["nrm"]
+
+    def getOutputPositions(self) -> int:
+        return self.buffers()["pos"]
+
+    def getWidth(self) -> int:
+        return self.m_width
+
+    def getHeight(self) -> int:
+        return self.m_height
+
+    def isAllocated(self) -> bool:
+        return self.m_isAllocated
+
+
+def timing_enable(on: bool = True) -> None:
+    _check(lib().vsg_timing_enable(1 if on else 0))
+
+
+def timing_reset() -> None:
+    _check(lib().vsg_timing_reset())
+
+
+def timing_read() -> dict:
+    ms = (C.c_double * len(KERNELS))()
+    n = (C.c_uint32 * len(KERNELS))()
+    _check(lib().vsg_timing_read(ms, n))
+    return {k: (ms[i], n[i]) for i, k in enumerate(KERNELS)}
