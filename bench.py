@@ -1,0 +1,280 @@
+#!/usr/bin/env python3
+"""Fused integrate + raycast throughput of the VoxelSG path on one MI355X.
+
+Workload: the 640x480 room sequence (300 float32 depth frames, f=525, 4x4x3 m room
+with 20 primitives, random-walk camera, voxel 4 mm, truncation 0.02 m, 2^20
+buckets, 2^19 blocks).  One step = one pass over the whole sequence from an empty
+scene: per frame VoxelScene::integrateFromScanData, then RayCastRender::render at
+the frame's pose over the driver's depth range.  All frames are resident in HBM
+before the timed region starts; nothing is copied inside it.
+
+Prints one JSON line: frames/s (value), the roofline of the dominant kernel (HIP
+events on the scene's stream around every launch in the timed region, algorithmic
+bytes from per-frame counters of an untimed instrumented pass), the fused-frame
+roofline, and the oracle's CPU throughput on a bounded sample of the same frames.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(REPO, "real-time-depth-fusion-directional-tsdf_amd")
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def load_pkg():
+    spec = importlib.util.spec_from_file_location("vsg_amd", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["vsg_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def kernel_bytes(st: dict, npix: int, b_sw: int) -> dict:
+    """Algorithmic HBM bytes of one frame, per kernel (compulsory traffic of the data flow)."""
+    return {
+        # pointmap read, claim flag written, buckets with live entries probed (10 x 16 B each)
+        "alloc_claim": 12 * npix + 4 * npix + 160 * st["alloc_buckets"],
+        # live-block list (16 B per block) read, visible list (4 B) written
+        "compact": 16 * st["num_allocated"] + 4 * st["visible_blocks"],
+        # pointmap + colour read once, list + block coordinate per block, voxel word and colour
+        # read-modify-write per updated voxel
+        "integrate": 15 * npix + 20 * st["visible_blocks"] + st["updated_voxels"] * (2 * b_sw + 8),
+        # outputs (depth 4 + position 16 + colour 16 + normal 16), probed buckets, voxel words of
+        # the blocks the march samples
+        "raycast": 52 * npix + 160 * st["ray_buckets"] + 512 * b_sw * st["ray_blocks"],
+    }
+
+
+def frame_bytes(st: dict, npix: int, b_sw: int) -> int:
+    """Bytes(frame) of the fused path: inputs, outputs, live entries, probed buckets, voxel
+    read-modify-writes, sampled voxel blocks."""
+    return (27 * npix + 52 * npix + 16 * st["num_allocated"] + 160 * st["any_buckets"]
+            + st["updated_voxels"] * (2 * b_sw + 8) + 512 * b_sw * st["ray_blocks"])
+
+
+def cpu_baseline(sc, frames: list[dict], wl, nframes: int, stride: int) -> dict:
+    """The oracle (float, literal reference algorithm: every allocated block integrated,
+    10-slot bucket probes, half-voxel ray steps) on the first `nframes` frames: full
+    integrate, raycast of every `stride`-th row and column scaled by stride^2."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    o = O.OracleScene(wl.buckets, wl.blocks, wl.voxel)
+    W, H = sc.W, sc.H
+    ys, xs = np.meshgrid(np.arange(0, H, stride), np.arange(0, W, stride), indexing="ij")
+    pix = (ys * W + xs).reshape(-1).astype(np.int32)
+    t_int = t_ray = 0.0
+    for i in range(nframes):
+        f = frames[i]
+        t0 = time.perf_counter()
+        o.integrate(f["points"], f["colors"], wl.trunc, f["pose"], sc.intrinsics)
+        t1 = time.perf_counter()
+        mn, mx = f["range"]
+        T = f["pose"].astype(np.float64)
+        o.render(W, H, T[:3, 3], T, sc.intrinsics, mn, mx, pixels=pix)
+        t2 = time.perf_counter()
+        t_int += t1 - t0
+        t_ray += t2 - t1
+    scale = (W * H) / len(pix)
+    per_frame = (t_int + t_ray * scale) / nframes
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(1.0 / per_frame, 4), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": (f"oracle (float) on frames 0-{nframes - 1} of the 300-frame room 640x480 sequence: "
+                       f"integrate in full ({t_int / nframes * 1e3:.0f} ms/frame), raycast of every "
+                       f"{stride}th row and column scaled x{scale:.0f} ({t_ray * scale / nframes * 1e3:.0f} ms/frame)"),
+            "integrate_ms": round(t_int / nframes * 1e3, 2), "raycast_ms": round(t_ray * scale / nframes * 1e3, 2)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3, help="timed passes over the sequence")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed passes before timing")
+    ap.add_argument("--frames", type=int, default=300)
+    ap.add_argument("--store", choices=["fp32", "fp16"], default="fp32")
+    ap.add_argument("--compare-fp16", action="store_true", default=True,
+                    help="also time the FP16 store (reported beside the FP32 headline)")
+    ap.add_argument("--no-compare-fp16", dest="compare_fp16", action="store_false")
+    ap.add_argument("--cpu-frames", type=int, default=40)
+    ap.add_argument("--cpu-stride", type=int, default=8)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU: "
+              f"each GPU would own a share of the hash buckets, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import torch
+    load_pkg()
+    from vsg_amd import synth, vsg
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    wl = synth.WORKLOADS[1]
+    sc = synth.SynthScene(1, nframes=args.frames)
+    W, H, nf = sc.W, sc.H, sc.nframes
+    npix = W * H
+    fx, fy, cx, cy = sc.intrinsics
+
+    log(f"rendering {nf} synthetic frames {W}x{H}")
+    t0 = time.perf_counter()
+    frames = []
+    for i in range(nf):
+        f = sc.render(i)
+        f["range"] = synth.depth_range(f["depth"])
+        frames.append(f)
+    log(f"frames ready in {time.perf_counter() - t0:.1f} s; uploading to HBM")
+    dev = torch.device("cuda:0")
+    pts = torch.from_numpy(np.stack([f["points"] for f in frames])).to(dev)
+    rgb = torch.from_numpy(np.stack([f["colors"] for f in frames])).to(dev)
+    nrm = torch.from_numpy(np.stack([f["normals"] for f in frames])).to(dev)
+    poses = [f["pose"] for f in frames]
+    ranges = [f["range"] for f in frames]
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.Stream()
+
+    def make(store):
+        scene = vsg.VoxelScene(vsg.Params(wl.buckets, wl.blocks, wl.voxel, wl.trunc), store)
+        scene.set_stream(stream.cuda_stream)
+        rr = vsg.RayCastRender()
+        assert rr.initialize(W, H)
+        return scene, rr
+
+    def run_pass(scene, rr, per_frame=None):
+        scene.reset()
+        for i in range(nf):
+            T = poses[i]
+            scene.integrateFromScanData(pts[i], rgb[i], nrm[i], W, H, wl.trunc, T, fx, fy, cx, cy)
+            rr.render(scene, T[:3, 3], T, fx, fy, cx, cy, ranges[i][0], ranges[i][1])
+            if per_frame is not None:
+                per_frame(i)
+
+    def measure(store):
+        scene, rr = make(store)
+        b_sw = 8 if store == "fp32" else 4
+        # untimed instrumented pass: per-frame counters of the byte model
+        log(f"[{store}] instrumented pass for the byte model")
+        scene.diag_enable(True)
+        stats = []
+
+        def collect(i):
+            c = scene.counters()
+            d = scene.diag_read(clear=True)
+            h, m = rr.counters()
+            stats.append({**c, **d, "hits": h, "misses": m})
+
+        run_pass(scene, rr, collect)
+        scene.diag_enable(False)
+        for w in range(args.warmup):
+            log(f"[{store}] warmup pass {w + 1}/{args.warmup}")
+            run_pass(scene, rr)
+        torch.cuda.synchronize()
+        vsg.timing_enable(True)
+        vsg.timing_reset()
+        log(f"[{store}] timing {args.steps} passes x {nf} frames")
+        torch.cuda.synchronize()
+        t_start = time.perf_counter()
+        for _ in range(args.steps):
+            run_pass(scene, rr)
+        torch.cuda.synchronize()
+        t_total = time.perf_counter() - t_start
+        ktimes = vsg.timing_read()
+        vsg.timing_enable(False)
+        scene.close()
+        rr.close()
+        kb = {k: 0 for k in ("alloc_claim", "compact", "integrate", "raycast")}
+        fb = 0
+        for st in stats:
+            for k, v in kernel_bytes(st, npix, b_sw).items():
+                kb[k] += v
+            fb += frame_bytes(st, npix, b_sw)
+        log(f"[{store}] {t_total:.3f} s for {args.steps * nf} frames")
+        return {"t_total": t_total, "ktimes": ktimes, "kbytes": kb, "fbytes": fb, "stats": stats}
+
+    res = measure(args.store)
+    t_total, ktimes = res["t_total"], res["ktimes"]
+    frames_done = args.steps * nf
+    fps = frames_done / t_total
+    # dominant kernel: most total time in the timed region
+    kern_ms = {k: v[0] for k, v in ktimes.items() if v[1] > 0}
+    dom = max(kern_ms, key=kern_ms.get)
+    launches = ktimes[dom][1]
+    avg_ms = ktimes[dom][0] / launches
+    bytes_per_launch = res["kbytes"].get(dom, 0) / nf
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    traffic = None
+    tr_path = os.path.join(REPO, "profiles", f"{dom}_room640_traffic.json")
+    if os.path.exists(tr_path):
+        traffic = json.load(open(tr_path)).get("bytes_per_launch")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
+                "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 4)}
+    fused_gbs = res["fbytes"] * args.steps / t_total / 1e9
+    per_kernel = {}
+    for k, (ms, n) in ktimes.items():
+        if n == 0:
+            continue
+        e = {"avg_ms": round(ms / n, 4), "launches": n, "share": round(ms / sum(kern_ms.values()), 4)}
+        if k in res["kbytes"]:
+            gbs = res["kbytes"][k] / nf / (ms / n * 1e-3) / 1e9
+            e["gbs"] = round(gbs, 1)
+            e["frac"] = round(gbs / HBM_PEAK_GBS, 5)
+        per_kernel[k] = e
+    st = res["stats"]
+    extra = {
+        "ms_per_frame": round(t_total / frames_done * 1e3, 4),
+        "fused_roofline": {"achieved": round(fused_gbs, 2), "frac": round(fused_gbs / HBM_PEAK_GBS, 5),
+                           "bytes_per_frame": int(res["fbytes"] / nf)},
+        "kernels": per_kernel,
+        "scene_end": {"blocks": st[-1]["num_allocated"], "visible_mean": int(np.mean([s["visible_blocks"] for s in st])),
+                      "updated_mean": int(np.mean([s["updated_voxels"] for s in st])),
+                      "ray_blocks_mean": int(np.mean([s["ray_blocks"] for s in st])),
+                      "hit_rate": round(float(np.mean([s["hits"] / npix for s in st])), 4)},
+    }
+    if args.compare_fp16 and args.store == "fp32":
+        r16 = measure("fp16")
+        extra["fp16_store"] = {"value": round(frames_done / r16["t_total"], 3),
+                               "speedup_vs_fp32": round(t_total / r16["t_total"], 4),
+                               "ms_per_frame": round(r16["t_total"] / frames_done * 1e3, 4)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        log(f"cpu baseline: oracle on {args.cpu_frames} frames")
+        cpu = cpu_baseline(sc, frames, wl, args.cpu_frames, args.cpu_stride)
+    out = {
+        "metric": "frames/s 640x480 integrate+raycast, % HBM roofline; fp16 vs fp32 at ≤0.5mm err",
+        "value": round(fps, 3),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(t_total / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.store == "fp32" else "f32 math, f16 store",
+        "data": "synthetic",
+        "config": {"workload": f"room 640x480, {nf} frames, voxel 4 mm, trunc 0.02 m, 2^20 buckets, "
+                               f"{args.store} store; step = one pass over the sequence from an empty scene",
+                   "frames_per_step": nf, "store": args.store},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        **extra,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

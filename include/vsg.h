@@ -180,6 +180,25 @@ vsg_status vsg_render_download(vsg_render* render, float* depth, float* position
 /* Hit / miss counts of the last render (RayCastRender.cu:599-616).  Synchronising. */
 vsg_status vsg_render_counters(vsg_render* render, uint32_t* hits, uint32_t* misses);
 
+/* ---- diagnostics (byte-model counters; no counterpart in the reference) ------------ */
+
+typedef struct vsg_diag {
+  uint32_t alloc_buckets; /* distinct buckets with live entries the allocation probed */
+  uint32_t ray_buckets;   /* distinct buckets with live entries the raycast probed    */
+  uint32_t any_buckets;   /* union of the two                                         */
+  uint32_t ray_blocks;    /* distinct live blocks a raycast trilinear corner touched  */
+  uint32_t reserved[4];
+} vsg_diag;
+
+/* When on, integrate and render run instrumented kernel variants that mark the
+ * buckets and blocks they touch (the production kernels are not instrumented).
+ * Turning it on allocates B + B + N bytes of marks and clears them. */
+vsg_status vsg_diag_enable(vsg_scene* scene, int32_t on);
+
+/* Counts the marks recorded since the last clear (synchronising); clear != 0
+ * clears them afterwards. */
+vsg_status vsg_diag_read(vsg_scene* scene, vsg_diag* out, int32_t clear);
+
 /* ---- kernel timing ---------------------------------------------------------- */
 
 /* Kernel ids for vsg_timing_read. */

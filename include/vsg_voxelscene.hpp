@@ -1,0 +1,287 @@
+// vsg_voxelscene.hpp -- the reference's C++ API (VoxelScene, RayCastRender, Params)
+// over the C ABI of vsg.h, header-only.
+//
+// Same class names, method names, argument order and meaning as
+//   VoxelSG/include/VoxelScene.h:52-209
+//   VoxelSG/include/RayCastRender.h:20-123
+//   VoxelSG/include/CUDAHashData.h:9-42
+// so host code written against the reference compiles against this header and
+// libvsg.so.  Differences, all at the edges of the API:
+//   * the pose of integrateFromScanData is a host float[16] row-major
+//     camera->world; the cv::Mat overload exists when VSG_HAVE_OPENCV is defined
+//     (OpenCV is used by the reference only as a pose container);
+//   * Params is filled directly instead of from GlobalParamsConfig;
+//   * getHashData() is not provided: the device hash layout is this library's own
+//     (vsg_export_blocks / vsg_export_voxels copy it out in the reference's terms);
+//   * errors print and return, as the reference does (VoxelScene.cpp:420-428).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "vsg.h"
+
+#ifdef VSG_HAVE_OPENCV
+#include <opencv2/core.hpp>
+#endif
+
+#define SDF_BLOCK_SIZE 8    // VoxelSG/include/globalPramasConfig.h:8
+#define HASH_BUCKET_SIZE 10 // VoxelSG/include/globalPramasConfig.h:9
+
+// Params (VoxelSG/include/CUDAHashData.h:9-42), field order kept.
+struct Params {
+  Params() = default;
+  Params(int hashSlotNum_, int SDFBlockNum_, float voxelSize_, float truncation_ = 0.02f)
+      : hashSlotNum(hashSlotNum_),
+        SDFBlockNum(SDFBlockNum_),
+        SDFBlockSize(SDF_BLOCK_SIZE),
+        totalHashSize(hashSlotNum_ * HASH_BUCKET_SIZE),
+        slotSize(HASH_BUCKET_SIZE),
+        totalBlockSize(SDFBlockNum_ * SDF_BLOCK_SIZE * SDF_BLOCK_SIZE * SDF_BLOCK_SIZE),
+        voxelSize(voxelSize_),
+        truncationScale(0.0f),
+        truncation(truncation_) {}
+
+  int hashSlotNum = 0;
+  int SDFBlockNum = 0;
+  int SDFBlockSize = SDF_BLOCK_SIZE;
+  int totalHashSize = 0;
+  int slotSize = HASH_BUCKET_SIZE;
+  int totalBlockSize = 0;
+  float voxelSize = 0.0f;
+  float truncationScale = 0.0f;
+  float truncation = 0.0f;
+};
+
+namespace vsg_detail {
+inline bool report(vsg_status st, const char* where) {
+  if (st == VSG_OK) return true;
+  std::fprintf(stderr, "%s: %s\n", where, vsg_last_error());
+  return false;
+}
+}  // namespace vsg_detail
+
+// VoxelScene (VoxelSG/include/VoxelScene.h:52-209)
+class VoxelScene {
+ public:
+  explicit VoxelScene(const Params& params, vsg_store store = VSG_STORE_FP32) : m_params(params) {
+    vsg_params p{};
+    p.hash_slot_num = params.hashSlotNum;
+    p.sdf_block_num = params.SDFBlockNum;
+    p.voxel_size = params.voxelSize;
+    p.store = store;
+    m_isAllocated = vsg_detail::report(vsg_scene_create(&p, &m_scene), "VoxelScene");
+  }
+  ~VoxelScene() {
+    if (m_scene) vsg_scene_destroy(m_scene);
+  }
+  VoxelScene(const VoxelScene&) = delete;
+  VoxelScene& operator=(const VoxelScene&) = delete;
+
+  void reset() {
+    if (!m_isAllocated) {
+      std::fprintf(stderr, "VoxelScene: Not allocated yet!\n");
+      return;
+    }
+    vsg_detail::report(vsg_scene_reset(m_scene), "VoxelScene::reset");
+  }
+
+  // VoxelScene::integrateFromScanData (VoxelSG/include/VoxelScene.h:89-91): device
+  // pointmap / colour map / normal map, host pose.
+  void integrateFromScanData(const float3* depthmap, const uchar3* colormap, const float3* normalmap, int width,
+                             int height, float truncationDistance, const float cameraTransform[16], float fx, float fy,
+                             float cx, float cy) {
+    if (depthmap == nullptr || colormap == nullptr || normalmap == nullptr) {
+      std::fprintf(stderr, "VoxelScene: Invalid pointers (nullptr)!\n");
+      return;
+    }
+    if (!m_isAllocated) {
+      std::fprintf(stderr, "VoxelScene: Not allocated!\n");
+      return;
+    }
+    vsg_frame f{depthmap, colormap, normalmap, width, height};
+    vsg_intrinsics k{fx, fy, cx, cy};
+    vsg_detail::report(vsg_integrate(m_scene, &f, truncationDistance, cameraTransform, &k),
+                       "VoxelScene::integrateFromScanData");
+  }
+
+#ifdef VSG_HAVE_OPENCV
+  void integrateFromScanData(const float3* depthmap, const uchar3* colormap, const float3* normalmap, int width,
+                             int height, float truncationDistance, const cv::Mat& cameraTransform, float fx, float fy,
+                             float cx, float cy) {
+    float T[16];
+    for (int i = 0; i < 4; ++i)
+      for (int j = 0; j < 4; ++j) T[4 * i + j] = cameraTransform.at<float>(i, j);
+    integrateFromScanData(depthmap, colormap, normalmap, width, height, truncationDistance, T, fx, fy, cx, cy);
+  }
+#endif
+
+  // Compaction runs inside every integrate call.
+  void compactifyHashEntries() {}
+
+  unsigned int getHeapFreeCount() const {
+    uint32_t n = 0;
+    vsg_detail::report(vsg_heap_free_count(m_scene, &n), "VoxelScene::getHeapFreeCount");
+    return n;
+  }
+  unsigned int getNumIntegratedFrames() const {
+    uint32_t n = 0;
+    vsg_num_integrated_frames(m_scene, &n);
+    return n;
+  }
+  bool isAllocated() const { return m_isAllocated; }
+  Params& getParams() { return m_params; }
+  const Params& getParams() const { return m_params; }
+
+  void printStats() const {
+    std::printf("\n=== VoxelScene Stats ===\nHash Slots: %d\nBucket Size: %d\nTotal Hash Entries: %d\n",
+                m_params.hashSlotNum, m_params.slotSize, m_params.totalHashSize);
+    std::printf("SDF Blocks: %d\nBlock Size: %dx%dx%d\nTotal Voxels: %d\nIntegrated Frames: %u\n",
+                m_params.SDFBlockNum, m_params.SDFBlockSize, m_params.SDFBlockSize, m_params.SDFBlockSize,
+                m_params.totalBlockSize, getNumIntegratedFrames());
+    if (m_isAllocated) {
+      unsigned int freeCount = getHeapFreeCount();
+      std::printf("Free Blocks: %u / %d (%g%%)\n", freeCount, m_params.SDFBlockNum,
+                  freeCount * 100.0 / m_params.SDFBlockNum);
+    }
+    std::printf("========================\n\n");
+  }
+
+  // Stub in the reference (VoxelSG/src/VoxelScene.cpp:838-845).
+  bool saveToMesh(const std::string&) {
+    std::fprintf(stderr, "VoxelScene::saveToMesh: Not implemented yet\n");
+    return false;
+  }
+
+  vsg_scene* handle() const { return m_scene; }
+
+ private:
+  Params m_params;
+  vsg_scene* m_scene = nullptr;
+  bool m_isAllocated = false;
+};
+
+// RayCastRender (VoxelSG/include/RayCastRender.h:20-123)
+class RayCastRender {
+ public:
+  RayCastRender() = default;
+  ~RayCastRender() {
+    if (m_render) vsg_render_destroy(m_render);
+  }
+  RayCastRender(const RayCastRender&) = delete;
+  RayCastRender& operator=(const RayCastRender&) = delete;
+
+  bool initialize(int width, int height) {
+    if (m_isAllocated) {
+      std::fprintf(stderr, "RayCastRender: Already initialized!\n");
+      return false;
+    }
+    if (!vsg_detail::report(vsg_render_create(width, height, &m_render), "RayCastRender: Failed to allocate GPU memory!"))
+      return false;
+    m_width = width;
+    m_height = height;
+    m_isAllocated = true;
+    return true;
+  }
+
+  // RayCastRender::render (VoxelSG/include/RayCastRender.h:66-70): the pose is a
+  // DEVICE float[16] row-major camera->world, as the reference's driver passes it.
+  void render(const VoxelScene* voxelScene, float3 cameraPos, float* cameraTransform, float fx, float fy, float cx,
+              float cy, float minDepth, float maxDepth) {
+    float T[16];
+    if (hipMemcpy(T, cameraTransform, sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) {
+      std::fprintf(stderr, "RayCastRender: cannot read the camera transform\n");
+      return;
+    }
+    renderHostPose(voxelScene, cameraPos, T, fx, fy, cx, cy, minDepth, maxDepth);
+  }
+
+  // Same, with the pose in host memory (no copy).
+  void renderHostPose(const VoxelScene* voxelScene, float3 cameraPos, const float cameraTransform[16], float fx,
+                      float fy, float cx, float cy, float minDepth, float maxDepth) {
+    if (!m_isAllocated) {
+      std::fprintf(stderr, "RayCastRender: Not initialized!\n");
+      return;
+    }
+    if (!voxelScene || !voxelScene->isAllocated()) {
+      std::fprintf(stderr, "RayCastRender: Invalid voxel scene!\n");
+      return;
+    }
+    const float cam[3] = {cameraPos.x, cameraPos.y, cameraPos.z};
+    vsg_intrinsics k{fx, fy, cx, cy};
+    vsg_detail::report(vsg_render_frame(m_render, voxelScene->handle(), cam, cameraTransform, &k, minDepth, maxDepth,
+                                        voxelScene->getParams().voxelSize),
+                       "RayCastRender::render");
+  }
+
+  // RayCastRender::downloadResults (RayCastRender.cpp:188-214): depth replicated
+  // into xyz with w = 1, colours and normals as rendered.
+  void downloadResults(float4* h_outputDepth, float4* h_outputColor, float4* h_outputNormal) {
+    if (!m_isAllocated) {
+      std::fprintf(stderr, "RayCastRender: Not initialized!\n");
+      return;
+    }
+    std::vector<float> d((size_t)m_width * m_height);
+    vsg_detail::report(vsg_render_download(m_render, d.data(), nullptr, (float*)h_outputColor, (float*)h_outputNormal),
+                       "RayCastRender::downloadResults");
+    for (size_t i = 0; i < d.size(); ++i) h_outputDepth[i] = make_float4(d[i], d[i], d[i], 1.0f);
+  }
+
+  void downloadDepthFloat(float* h_outputDepthFloat) {
+    if (!m_isAllocated || h_outputDepthFloat == nullptr) return;
+    vsg_render_download(m_render, h_outputDepthFloat, nullptr, nullptr, nullptr);
+  }
+
+  // RayCastRender::savePointCloudPLY (RayCastRender.cpp:221-302): ASCII PLY of the
+  // hit pixels' world positions and colours.
+  bool savePointCloudPLY(const std::string& filePath) {
+    if (!m_isAllocated) {
+      std::fprintf(stderr, "RayCastRender: Not initialized!\n");
+      return false;
+    }
+    const size_t n = (size_t)m_width * m_height;
+    std::vector<float> d(n);
+    std::vector<float4> p(n), c(n);
+    if (!vsg_detail::report(vsg_render_download(m_render, d.data(), (float*)p.data(), (float*)c.data(), nullptr),
+                            "RayCastRender::savePointCloudPLY"))
+      return false;
+    size_t cnt = 0;
+    for (size_t i = 0; i < n; ++i) cnt += d[i] < 1e9f ? 1 : 0;
+    FILE* fp = std::fopen(filePath.c_str(), "w");
+    if (!fp) return false;
+    std::fprintf(fp, "ply\nformat ascii 1.0\nelement vertex %zu\n", cnt);
+    std::fprintf(fp, "property float x\nproperty float y\nproperty float z\n");
+    std::fprintf(fp, "property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n");
+    for (size_t i = 0; i < n; ++i) {
+      if (!(d[i] < 1e9f)) continue;
+      auto u8 = [](float v) { return (int)(v < 0.f ? 0.f : (v > 1.f ? 255.f : v * 255.f)); };
+      std::fprintf(fp, "%g %g %g %d %d %d\n", p[i].x, p[i].y, p[i].z, u8(c[i].x), u8(c[i].y), u8(c[i].z));
+    }
+    std::fclose(fp);
+    return true;
+  }
+
+  float4* getOutputDepth() const { return (float4*)buf(0); }  // a float[W*H], as in the reference
+  float4* getOutputColor() const { return (float4*)buf(2); }
+  float4* getOutputNormal() const { return (float4*)buf(3); }
+  float4* getOutputPositions() const { return (float4*)buf(1); }
+  int getWidth() const { return m_width; }
+  int getHeight() const { return m_height; }
+  bool isAllocated() const { return m_isAllocated; }
+
+  // Hit / miss counts of the last render (printed by the reference, RayCastRender.cu:599-616).
+  void getCounters(unsigned int* hits, unsigned int* misses) const { vsg_render_counters(m_render, hits, misses); }
+
+ private:
+  float* buf(int i) const {
+    float* b[4] = {nullptr, nullptr, nullptr, nullptr};
+    if (m_render) vsg_render_buffers(m_render, &b[0], &b[1], &b[2], &b[3]);
+    return b[i];
+  }
+  vsg_render* m_render = nullptr;
+  int m_width = 0, m_height = 0;
+  bool m_isAllocated = false;
+};

@@ -71,9 +71,20 @@ def build_synth(verbose: bool = False) -> str:
     return out
 
 
+def build_shim_example(verbose: bool = False) -> str:
+    """C++ driver over include/vsg_voxelscene.hpp (the reference's class API)."""
+    out = os.path.join(LIB_DIR, "vsg_shim_example")
+    tmp = out + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", "-I", INCLUDE, os.path.join(CSRC, "shim_example.cpp"), "-L", LIB_DIR,
+          "-lvsg", "-lvsg_synth", "-Wl,-rpath,$ORIGIN", "-o", tmp], verbose)
+    os.replace(tmp, out)
+    return out
+
+
 def build(verbose: bool = False) -> None:
     build_synth(verbose)
     build_vsg(verbose)
+    build_shim_example(verbose)
 
 
 if __name__ == "__main__":

@@ -1,0 +1,89 @@
+// Host driver written against the reference's C++ API (include/vsg_voxelscene.hpp),
+// following the per-frame order of VoxelSG/src/voxelSG.cpp:392-690: upload the
+// frame, VoxelScene::integrateFromScanData, create a RayCastRender, upload the pose,
+// compute the depth range from the host depth map (voxelSG.cpp:594-608), render,
+// download.  Frames come from the synthetic generator (libvsg_synth.so).
+//
+// usage: vsg_shim_example <workload 0|1> <frames>
+// prints: frames, allocated blocks, hits, misses, and a checksum of the last depth map.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "vsg_voxelscene.hpp"
+
+extern "C" {
+void* vsgs_create(int cfg, uint64_t seed, int nframes);
+void vsgs_destroy(void* p);
+void vsgs_info(void* p, int* whn, double* intr);
+void vsgs_pose(void* p, int frame, double* T16);
+void vsgs_render(void* p, int frame, const double* T16, int noise, float* pts, uint8_t* rgb, float* nrm, float* depth);
+void vsgs_depth_range(const float* depth, long n, float* out2);
+}
+
+int main(int argc, char** argv) {
+  const int cfg = argc > 1 ? std::atoi(argv[1]) : 0;
+  const int nframes = argc > 2 ? std::atoi(argv[2]) : 5;
+  void* sc = vsgs_create(cfg, 0, 0);
+  int whn[3];
+  double intr[4];
+  vsgs_info(sc, whn, intr);
+  const int W = whn[0], H = whn[1];
+  const float fx = (float)intr[0], fy = (float)intr[1], cx = (float)intr[2], cy = (float)intr[3];
+  const size_t n = (size_t)W * H;
+
+  Params params = cfg == 0 ? Params(200000, 1 << 17, 0.004f) : Params(1 << 20, 1 << 19, 0.004f);
+  const float truncation = 0.02f;
+  VoxelScene scene(params);
+  if (!scene.isAllocated()) return 1;
+
+  std::vector<float> pts(3 * n), nrm(3 * n), depth(n);
+  std::vector<uint8_t> rgb(3 * n);
+  float3 *d_pts = nullptr, *d_nrm = nullptr;
+  uchar3* d_rgb = nullptr;
+  float* d_T = nullptr;
+  if (hipMalloc(&d_pts, 12 * n) != hipSuccess || hipMalloc(&d_nrm, 12 * n) != hipSuccess ||
+      hipMalloc(&d_rgb, 3 * n) != hipSuccess || hipMalloc(&d_T, 64) != hipSuccess)
+    return 2;
+
+  unsigned int hits = 0, misses = 0;
+  double checksum = 0.0;
+  for (int i = 0; i < nframes; ++i) {
+    double Td[16];
+    vsgs_pose(sc, i, Td);
+    vsgs_render(sc, i, Td, 1, pts.data(), rgb.data(), nrm.data(), depth.data());
+    float T[16];
+    for (int k = 0; k < 16; ++k) T[k] = (float)Td[k];
+    if (hipMemcpy(d_pts, pts.data(), 12 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_rgb, rgb.data(), 3 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_nrm, nrm.data(), 12 * n, hipMemcpyHostToDevice) != hipSuccess)
+      return 3;
+    scene.integrateFromScanData(d_pts, d_rgb, d_nrm, W, H, truncation, T, fx, fy, cx, cy);
+
+    RayCastRender renderer;  // created per frame, as the driver does (voxelSG.cpp:578-579)
+    if (!renderer.initialize(W, H)) return 4;
+    float3 camPos = make_float3(T[3], T[7], T[11]);
+    if (hipMemcpy(d_T, T, 64, hipMemcpyHostToDevice) != hipSuccess) return 5;
+    float range[2];
+    vsgs_depth_range(depth.data(), (long)n, range);
+    renderer.render(&scene, camPos, d_T, fx, fy, cx, cy, range[0], range[1]);
+    std::vector<float> out(n);
+    renderer.downloadDepthFloat(out.data());
+    renderer.getCounters(&hits, &misses);
+    checksum = 0.0;
+    for (size_t k = 0; k < n; ++k)
+      if (out[k] < 1e9f) checksum += (double)out[k] * (double)((k % 97) + 1);
+  }
+  std::printf("frames %d blocks %u hits %u misses %u checksum %.9e\n", scene.getNumIntegratedFrames(),
+              (unsigned)(params.SDFBlockNum - scene.getHeapFreeCount()), hits, misses, checksum);
+  (void)hipFree(d_pts);
+  (void)hipFree(d_nrm);
+  (void)hipFree(d_rgb);
+  (void)hipFree(d_T);
+  vsgs_destroy(sc);
+  return 0;
+}

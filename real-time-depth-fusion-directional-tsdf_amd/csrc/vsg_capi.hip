@@ -112,6 +112,10 @@ struct vsg_scene {
   uint32_t* wgsum = nullptr;
   size_t pix_cap = 0;
   uint32_t frames = 0;  // integrate calls since reset (host mirror)
+  bool diag = false;
+  uint8_t* mark_abkt = nullptr;  // diagnostics marks (B, B, N bytes)
+  uint8_t* mark_rbkt = nullptr;
+  uint8_t* mark_blk = nullptr;
 };
 
 struct vsg_render {
@@ -138,12 +142,16 @@ void free_scene(vsg_scene* s) {
   (void)hipFree(s->t.cnt);
   (void)hipFree(s->t.claim);
   (void)hipFree(s->t.blk);
+  (void)hipFree(s->t.sb);
   (void)hipFree(s->vox);
   (void)hipFree(s->col);
   (void)hipFree(s->vis);
   (void)hipFree(s->st);
   (void)hipFree(s->pixcnt);
   (void)hipFree(s->wgsum);
+  (void)hipFree(s->mark_abkt);
+  (void)hipFree(s->mark_rbkt);
+  (void)hipFree(s->mark_blk);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -233,6 +241,7 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   if (r == VSG_OK) r = dmalloc(&s->t.cnt, B, "bucket counts");
   if (r == VSG_OK) r = dmalloc(&s->t.claim, B * sizeof(uint32_t), "bucket claims");
   if (r == VSG_OK) r = dmalloc(&s->t.blk, N * sizeof(int4), "block list");
+  if (r == VSG_OK) r = dmalloc(&s->t.sb, kSuperBits / 8, "super-block bitmap");
   if (r == VSG_OK) r = dmalloc(&s->vox, N * kVox * s->word_bytes, "voxel store");
   if (r == VSG_OK) r = dmalloc(&s->col, N * kVox * sizeof(uint32_t), "colour store");
   if (r == VSG_OK) r = dmalloc(&s->vis, N * sizeof(uint32_t), "visible list");
@@ -339,7 +348,7 @@ vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, con
   vsg_status r;
   {
     TimedLaunch tl(s->stream, VSG_K_ALLOC_CLAIM);
-    launch_alloc_claim(s->stream, s->t, s->st, fa, pts, s->pixcnt);
+    launch_alloc_claim(s->stream, s->t, s->st, fa, pts, s->pixcnt, s->diag ? s->mark_abkt : nullptr);
   }
   if ((r = check_launch("alloc claim")) != VSG_OK) return r;
   {
@@ -554,6 +563,8 @@ vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3],
   ra.voxel = voxel_size > 0.0f ? voxel_size : s->p.voxel_size;
   ra.W = r->W;
   ra.H = r->H;
+  ra.mark_bkt = s->diag ? s->mark_rbkt : nullptr;
+  ra.mark_blk = s->diag ? s->mark_blk : nullptr;
   hipStream_t strm = r->has_stream ? r->stream : s->stream;
   if (strm != s->stream) {  // order after the scene's pending work
     VSG_HIP(hipEventRecord(r->ev, s->stream));
@@ -607,6 +618,49 @@ vsg_status vsg_render_counters(vsg_render* r, uint32_t* hits, uint32_t* misses) 
   VSG_HIP(hipMemcpy(h, r->rc + 2 * ((r->calls - 1u) & 1u), sizeof(h), hipMemcpyDeviceToHost));
   *hits = h[0];
   *misses = h[1];
+  return VSG_OK;
+}
+
+// ---- diagnostics ----------------------------------------------------------------------
+
+vsg_status vsg_diag_enable(vsg_scene* s, int32_t on) {
+  if (!s) return fail(VSG_ERR_INVALID_ARG, "vsg_diag_enable: null scene");
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  if (on && !s->mark_blk) {
+    vsg_status r = dmalloc(&s->mark_abkt, s->t.num_buckets, "diagnostic marks");
+    if (r == VSG_OK) r = dmalloc(&s->mark_rbkt, s->t.num_buckets, "diagnostic marks");
+    if (r == VSG_OK) r = dmalloc(&s->mark_blk, s->t.num_blocks, "diagnostic marks");
+    if (r != VSG_OK) return r;
+  }
+  if (on) {
+    VSG_HIP(hipMemset(s->mark_abkt, 0, s->t.num_buckets));
+    VSG_HIP(hipMemset(s->mark_rbkt, 0, s->t.num_buckets));
+    VSG_HIP(hipMemset(s->mark_blk, 0, s->t.num_blocks));
+  }
+  s->diag = on != 0;
+  return VSG_OK;
+}
+
+vsg_status vsg_diag_read(vsg_scene* s, vsg_diag* out, int32_t clear) {
+  if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_diag_read: null argument");
+  std::memset(out, 0, sizeof(*out));
+  if (!s->mark_blk) return VSG_OK;
+  VSG_HIP(hipDeviceSynchronize());
+  std::vector<uint8_t> a(s->t.num_buckets), r(s->t.num_buckets), b(s->t.num_blocks);
+  VSG_HIP(hipMemcpy(a.data(), s->mark_abkt, a.size(), hipMemcpyDeviceToHost));
+  VSG_HIP(hipMemcpy(r.data(), s->mark_rbkt, r.size(), hipMemcpyDeviceToHost));
+  VSG_HIP(hipMemcpy(b.data(), s->mark_blk, b.size(), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < a.size(); ++i) {
+    out->alloc_buckets += a[i] ? 1u : 0u;
+    out->ray_buckets += r[i] ? 1u : 0u;
+    out->any_buckets += (a[i] | r[i]) ? 1u : 0u;
+  }
+  for (size_t i = 0; i < b.size(); ++i) out->ray_blocks += b[i] ? 1u : 0u;
+  if (clear) {
+    VSG_HIP(hipMemset(s->mark_abkt, 0, s->t.num_buckets));
+    VSG_HIP(hipMemset(s->mark_rbkt, 0, s->t.num_buckets));
+    VSG_HIP(hipMemset(s->mark_blk, 0, s->t.num_blocks));
+  }
   return VSG_OK;
 }
 

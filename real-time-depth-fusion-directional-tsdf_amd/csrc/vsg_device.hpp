@@ -20,6 +20,7 @@ constexpr int kKeyBits = 21;                           // per block coordinate i
 constexpr int32_t kKeyMin = -(1 << (kKeyBits - 1));
 constexpr int32_t kKeyMax = (1 << (kKeyBits - 1)) - 1;
 constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
+constexpr uint32_t kSuperBits = 1u << 24;              // super-block occupancy bitmap: 2 MiB
 
 // ---- views passed to kernels by value -------------------------------------------
 
@@ -28,11 +29,14 @@ constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
 // number of used slots; slots fill in order and are never freed, so the used slots
 // are always slots [0, cnt[b]).  claim[b] is the per-frame insertion claim word.
 // blk[p] = (x, y, z, hash entry index) of the block with pointer p.
+// sb is a hashed occupancy bitmap of 4x4x4-block super-blocks (bit set when any
+// block of the super-block is allocated; collisions only set extra bits).
 struct TableView {
   uint64_t* rec;
   uint8_t* cnt;
   uint32_t* claim;
   int4* blk;
+  uint32_t* sb;
   uint32_t num_buckets;
   uint32_t num_blocks;
 };
@@ -142,6 +146,15 @@ __device__ __forceinline__ int32_t table_lookup(const TableView& t, int32_t x, i
   }
   if (hit < 0) return -1;
   return reinterpret_cast<const int32_t*>(r + kBucket)[hit];
+}
+
+// Bit of super-block (sx, sy, sz) (4x4x4 blocks) in the occupancy bitmap.
+__device__ __forceinline__ uint32_t sb_bit(int32_t sx, int32_t sy, int32_t sz) {
+  uint32_t k = (uint32_t)sx * 0x9E3779B1u ^ (uint32_t)sy * 0x85EBCA77u ^ (uint32_t)sz * 0xC2B2AE3Du;
+  k ^= k >> 15;
+  k *= 0x2C1B3C6Du;
+  k ^= k >> 12;
+  return k & (kSuperBits - 1u);
 }
 
 // ---- ordered float encoding for atomicMin/atomicMax ----------------------------------

@@ -15,12 +15,14 @@ struct RenderArgs {
   float minD, maxD;
   float voxel;
   int32_t W, H;
+  uint8_t* mark_bkt;  // diagnostics (null when off): buckets with live entries probed
+  uint8_t* mark_blk;  // diagnostics (null when off): blocks a trilinear corner touched
 };
 
 void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col);
 void launch_reset_table(hipStream_t s, const TableView& t, DevState* st);
 void launch_alloc_claim(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
-                        uint32_t* pixcnt);
+                        uint32_t* pixcnt, uint8_t* mark_bkt);
 void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
                           uint32_t* wgsum);
 void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, uint32_t nwg, int32_t cset, uint32_t num_blocks);

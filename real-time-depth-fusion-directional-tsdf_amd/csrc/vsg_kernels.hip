@@ -91,7 +91,8 @@ template <class F> __device__ __forceinline__ void dda_band(float3 a, float3 b, 
 // ---------------------------------------------------------------------------------
 
 __global__ __launch_bounds__(256) void k_alloc_claim(TableView t, DevState* st, FrameArgs fa,
-                                                     const float* __restrict__ pts, uint32_t* __restrict__ pixflag) {
+                                                     const float* __restrict__ pts, uint32_t* __restrict__ pixflag,
+                                                     uint8_t* __restrict__ mark_bkt) {
   const int pix = blockIdx.x * 256 + threadIdx.x;
   const int npix = fa.W * fa.H;
   uint32_t valid = 0, claimed = 0, oor = 0, nonfinite = 0;
@@ -115,6 +116,7 @@ __global__ __launch_bounds__(256) void k_alloc_claim(TableView t, DevState* st, 
         }
         const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
         const int c = t.cnt[h];
+        if (mark_bkt && c > 0) mark_bkt[h] = 1;
         if (c >= kBucket) return;  // full: present (no-op) or refused (no-op)
         const uint64_t key = pack_key(x, y, z);
         const uint64_t* r = t.rec + (size_t)h * kRecWords;
@@ -238,6 +240,8 @@ __global__ __launch_bounds__(256) void k_alloc_insert(TableView t, DevState* st,
     reinterpret_cast<int32_t*>(r + kBucket)[slot] = (int32_t)p;
     t.cnt[h] = (uint8_t)(slot + 1);
     t.blk[p] = make_int4(x, y, z, (int32_t)(h * kBucket + slot));
+    const uint32_t sbit = sb_bit(x >> 2, y >> 2, z >> 2);  // super-block of 4x4x4 blocks
+    atomicOr(&t.sb[sbit >> 5], 1u << (sbit & 31u));
   });
 }
 
@@ -406,32 +410,55 @@ __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, Fr
 // Raycast (RayCastRender.cu:127-517)
 // ---------------------------------------------------------------------------------
 
-// Two most recent block lookups of this ray.
-struct BlockCache {
+// Two most recent block lookups of this ray.  DIAG marks every bucket with live
+// entries that a lookup probes and every block it finds (byte-model counters).
+template <bool DIAG> struct BlockCache {
   int32_t x0, y0, z0, p0;
   int32_t x1, y1, z1, p1;
-  __device__ void init() {
+  uint8_t* mark_bkt;
+  uint8_t* mark_blk;
+  __device__ void init(uint8_t* mb, uint8_t* mk) {
     x0 = y0 = z0 = INT32_MIN;
     x1 = y1 = z1 = INT32_MIN;
     p0 = p1 = -1;
+    mark_bkt = mb;
+    mark_blk = mk;
   }
   __device__ __forceinline__ int32_t get(const TableView& t, int32_t x, int32_t y, int32_t z) {
     if (x == x0 && y == y0 && z == z0) return p0;
     if (x == x1 && y == y1 && z == z1) return p1;
     const int32_t p = table_lookup(t, x, y, z);
+    if (DIAG) {
+      if (key_in_range(x, y, z)) {
+        const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
+        if (t.cnt[h] > 0) mark_bkt[h] = 1;
+      }
+      if (p >= 0) mark_blk[p] = 1;
+    }
     x1 = x0; y1 = y0; z1 = z0; p1 = p0;
     x0 = x; y0 = y; z0 = z; p0 = p;
     return p;
   }
 };
 
+// Sample status of a trilinear evaluation.
+enum : int { kSampleEmpty = 0, kSampleInvalid = 1, kSampleValid = 2 };
+
+// Block range covered by the eight corners of the last sample (per axis lo/hi).
+struct CornerBlocks {
+  int32_t x0, x1, y0, y1, z0, z1;
+};
+
 // trilinearInterpolationSimpleFastFast (RayCastRender.cu:127-215).  The eight
 // corners' voxel indices come from two quotients per axis (posDual/v and
 // (posDual+v)/v): the reference computes the same quotient once per corner.
-template <bool F16, bool COLOR>
-__device__ __forceinline__ bool trilinear(const TableView& t, BlockCache& bc, const void* __restrict__ vox_,
-                                          const uint32_t* __restrict__ col, float px, float py, float pz, float v,
-                                          float* dist, uint32_t* rgb) {
+// Returns kSampleValid with *dist (and *rgb), kSampleInvalid when a corner block
+// exists but every corner voxel has weight 0, kSampleEmpty when no corner block
+// exists (then *cb holds the corner blocks' range).
+template <bool F16, bool COLOR, class Cache>
+__device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const void* __restrict__ vox_,
+                                         const uint32_t* __restrict__ col, float px, float py, float pz, float v,
+                                         float* dist, uint32_t* rgb, CornerBlocks* cb) {
   using S = Store<F16>;
   const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
   const float off = v;
@@ -442,12 +469,17 @@ __device__ __forceinline__ bool trilinear(const TableView& t, BlockCache& bc, co
   const int32_t vx[2] = {ray_voxel_of_q(qx0), ray_voxel_of_q(qx1)};
   const int32_t vy[2] = {ray_voxel_of_q(qy0), ray_voxel_of_q(qy1)};
   const int32_t vz[2] = {ray_voxel_of_q(qz0), ray_voxel_of_q(qz1)};
-  float d = 0.0f, cr = 0.0f, cg = 0.0f, cb = 0.0f, wsum = 0.0f;
+  const int32_t bx[2] = {voxel_to_block_axis(vx[0]), voxel_to_block_axis(vx[1])};
+  const int32_t by[2] = {voxel_to_block_axis(vy[0]), voxel_to_block_axis(vy[1])};
+  const int32_t bz[2] = {voxel_to_block_axis(vz[0]), voxel_to_block_axis(vz[1])};
+  float d = 0.0f, cr = 0.0f, cg = 0.0f, cbl = 0.0f, wsum = 0.0f;
+  bool any_block = false;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
     const int32_t ix = vx[i & 1], iy = vy[(i >> 1) & 1], iz = vz[(i >> 2) & 1];
-    const int32_t p = bc.get(t, voxel_to_block_axis(ix), voxel_to_block_axis(iy), voxel_to_block_axis(iz));
+    const int32_t p = bc.get(t, bx[i & 1], by[(i >> 1) & 1], bz[(i >> 2) & 1]);
     if (p < 0) continue;
+    any_block = true;
     const size_t vi = (size_t)p * kVox +
                       (size_t)(voxel_local_axis(ix) + voxel_local_axis(iy) * kBlock + voxel_local_axis(iz) * kBlock * kBlock);
     const typename S::word w = vox[vi];
@@ -458,23 +490,77 @@ __device__ __forceinline__ bool trilinear(const TableView& t, BlockCache& bc, co
       const uint32_t c = col[vi];
       cr += wi * (float)(c & 0xFFu);
       cg += wi * (float)((c >> 8) & 0xFFu);
-      cb += wi * (float)((c >> 16) & 0xFFu);
+      cbl += wi * (float)((c >> 16) & 0xFFu);
     }
     wsum += wi;
   }
-  if (wsum <= 1e-6f) return false;
+  if (wsum <= 1e-6f) {
+    if (any_block) return kSampleInvalid;
+    cb->x0 = bx[0]; cb->x1 = bx[1];
+    cb->y0 = by[0]; cb->y1 = by[1];
+    cb->z0 = bz[0]; cb->z1 = bz[1];
+    return kSampleEmpty;
+  }
   const float inv = 1.0f / wsum;
   *dist = d * inv;
   if (COLOR) {
     cr *= inv;
     cg *= inv;
-    cb *= inv;
-    *rgb = ((uint32_t)cr & 0xFFu) | (((uint32_t)cg & 0xFFu) << 8) | (((uint32_t)cb & 0xFFu) << 16);
+    cbl *= inv;
+    *rgb = ((uint32_t)cr & 0xFFu) | (((uint32_t)cg & 0xFFu) << 8) | (((uint32_t)cbl & 0xFFu) << 16);
   }
-  return true;
+  return kSampleValid;
 }
 
-template <bool F16>
+// ---- exact empty-space skipping ---------------------------------------------------
+//
+// A sample is invalid whenever none of its eight corner voxels lies in an allocated
+// block.  The corner voxel indices of a sample at p are r(q) and r(q+1) per axis
+// with q = (p - v/2)/v and r(q) in (q - 1.5, q + 0.5], so they lie in
+// (p/v - 2, p/v + 1].  While p/v stays inside [8*b0 + 2, 8*b1 + 6] on every axis,
+// all corners fall in blocks b0..b1; if those blocks are all unallocated, every
+// such sample is invalid and the march only has to replay its step additions
+// (rayCurrent += v/2) to reach the same sample positions as the reference.
+// The same holds for a 4x4x4-block super-block with no allocated block, read
+// from a hashed occupancy bitmap (a hash collision only makes it look occupied).
+// Boxes are shrunk by a margin covering the float rounding of p and q.
+
+struct RaySkip {
+  float camx, camy, camz;
+  float ix, iy, iz;  // 1 / direction (inf for a zero component)
+  float v, inv_v;
+  float eps;         // safety margin in voxels
+};
+
+// Parameter interval [t0, t1) of the ray inside the box [lo, hi] (voxel units,
+// shrunk by the margin); t1 <= t0 when the ray misses it.
+__device__ __forceinline__ void box_interval(const RaySkip& k, float lox, float hix, float loy, float hiy, float loz,
+                                             float hiz, float* t0, float* t1) {
+  float tn = -INFINITY, tf = INFINITY;
+  const float lo[3] = {lox, loy, loz}, hi[3] = {hix, hiy, hiz};
+  const float c[3] = {k.camx, k.camy, k.camz}, id[3] = {k.ix, k.iy, k.iz};
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float l = (lo[a] + k.eps) * k.v - c[a], h = (hi[a] - k.eps) * k.v - c[a];
+    if (isinf(id[a])) {  // parallel to this slab: inside for every t, or never
+      if (!(l <= 0.0f && 0.0f <= h)) { tf = -INFINITY; }
+      continue;
+    }
+    const float ta = l * id[a], tb = h * id[a];
+    tn = fmaxf(tn, fminf(ta, tb));
+    tf = fminf(tf, fmaxf(ta, tb));
+  }
+  // shrink by a relative margin against the rounding of the products above
+  *t0 = tn + fabsf(tn) * 1e-5f + 1e-7f;
+  *t1 = tf - fabsf(tf) * 1e-5f - 1e-7f;
+}
+
+__device__ __forceinline__ bool sb_occupied(const uint32_t* __restrict__ sb, int32_t sx, int32_t sy, int32_t sz) {
+  const uint32_t b = sb_bit(sx, sy, sz);
+  return (sb[b >> 5] >> (b & 31u)) & 1u;
+}
+
+template <bool F16, bool DIAG>
 __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                float* __restrict__ out_depth, float4* __restrict__ out_pos,
@@ -517,13 +603,50 @@ __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __re
       const float inc = v * 0.5f;
       float last_sdf = 0.0f, last_alpha = 0.0f;
       uint32_t last_w = 0;
-      BlockCache bc;
-      bc.init();
-      bool found = false;
+      BlockCache<DIAG> bc;
+      bc.init(ra.mark_bkt, ra.mark_blk);
+      RaySkip sk;
+      sk.camx = camx;
+      sk.camy = camy;
+      sk.camz = camz;
+      sk.ix = wdx != 0.0f ? 1.0f / wdx : INFINITY;
+      sk.iy = wdy != 0.0f ? 1.0f / wdy : INFINITY;
+      sk.iz = wdz != 0.0f ? 1.0f / wdz : INFINITY;
+      sk.v = v;
+      sk.inv_v = 1.0f / v;
+      {
+        const float mag = fmaxf(fmaxf(fabsf(camx), fabsf(camy)), fabsf(camz)) + fabsf(ray_end);
+        sk.eps = 0.02f + 8e-6f * mag * sk.inv_v;
+      }
+      bool found = false, try_super = true, stalled = false;
       while (ray < ray_end) {
         const float ppx = camx + ray * wdx, ppy = camy + ray * wdy, ppz = camz + ray * wdz;
+        if (try_super) {
+          // skip the interior of an unoccupied super-block without evaluating samples
+          const float qx = ppx * sk.inv_v, qy = ppy * sk.inv_v, qz = ppz * sk.inv_v;
+          const int32_t sx = (int32_t)floorf(qx * (1.0f / 32.0f)), sy = (int32_t)floorf(qy * (1.0f / 32.0f)),
+                        sz = (int32_t)floorf(qz * (1.0f / 32.0f));
+          if (!sb_occupied(t.sb, sx, sy, sz)) {
+            float t0, t1;
+            box_interval(sk, 32.0f * sx + 2.0f, 32.0f * sx + 30.0f, 32.0f * sy + 2.0f, 32.0f * sy + 30.0f,
+                         32.0f * sz + 2.0f, 32.0f * sz + 30.0f, &t0, &t1);
+            if (t0 <= ray && ray < t1) {
+              last_w = 0u;
+              while (ray < t1 && ray < ray_end) {
+                const float next = ray + inc;
+                if (next == ray) { stalled = true; break; }
+                ray = next;
+              }
+              if (stalled) break;
+              continue;
+            }
+          }
+        }
         float dist;
-        if (trilinear<F16, false>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr)) {
+        CornerBlocks cb;
+        const int stat = trilinear<F16, false>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr, &cb);
+        if (stat == kSampleValid) {
+          try_super = false;
           if (last_w > 0u && last_sdf < 0.0f && dist > 0.0f) {
             // findIntersectionBisection (RayCastRender.cu:229-271)
             float a = last_alpha, ad = last_sdf, bb = ray, bd = dist, c = 0.0f;
@@ -532,8 +655,9 @@ __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __re
             for (int it = 0; it < 3; ++it) {
               c = a + (ad / (ad - bd)) * (bb - a);
               float cd;
-              if (!trilinear<F16, true>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd,
-                                        &rgb)) {
+              CornerBlocks cb2;
+              if (trilinear<F16, true>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd, &rgb,
+                                       &cb2) != kSampleValid) {
                 ok = false;
                 break;
               }
@@ -564,6 +688,22 @@ __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __re
         const float next = ray + inc;
         if (next == ray) break;  // step absorbed by rounding: the reference would never end
         ray = next;
+        if (stat == kSampleEmpty) {
+          // every corner block of this sample is unallocated: skip the samples whose
+          // corners stay inside those blocks
+          try_super = true;
+          float t0, t1;
+          box_interval(sk, 8.0f * cb.x0 + 2.0f, 8.0f * cb.x1 + 6.0f, 8.0f * cb.y0 + 2.0f, 8.0f * cb.y1 + 6.0f,
+                       8.0f * cb.z0 + 2.0f, 8.0f * cb.z1 + 6.0f, &t0, &t1);
+          if (t0 <= ray) {
+            while (ray < t1 && ray < ray_end) {
+              const float nx = ray + inc;
+              if (nx == ray) { stalled = true; break; }
+              ray = nx;
+            }
+            if (stalled) break;
+          }
+        }
       }
       hit = found ? 1u : 0u;
       miss = found ? 0u : 1u;
@@ -610,6 +750,7 @@ __global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) 
     t.cnt[b] = 0;
     t.claim[b] = kNoClaim;
   }
+  for (size_t w = tid; w < kSuperBits / 32u; w += stride) t.sb[w] = 0u;
   if (tid == 0) {
     st->n_alloc = 0;
     st->n_alloc_prev = 0;
@@ -637,9 +778,9 @@ void launch_reset_table(hipStream_t s, const TableView& t, DevState* st) {
 }
 
 void launch_alloc_claim(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
-                        uint32_t* pixcnt) {
+                        uint32_t* pixcnt, uint8_t* mark_bkt) {
   const uint32_t nwg = (uint32_t)((fa.W * fa.H + 255) / 256);
-  hipLaunchKernelGGL(k_alloc_claim, dim3(nwg), dim3(256), 0, s, t, st, fa, pts, pixcnt);
+  hipLaunchKernelGGL(k_alloc_claim, dim3(nwg), dim3(256), 0, s, t, st, fa, pts, pixcnt, mark_bkt);
 }
 
 void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
@@ -676,10 +817,18 @@ void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState*
                    const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,
                    uint32_t* rc) {
   const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
-  if (fp16)
-    hipLaunchKernelGGL(k_render<true>, grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
-  else
-    hipLaunchKernelGGL(k_render<false>, grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+  const bool diag = ra.mark_blk != nullptr;
+  if (fp16) {
+    if (diag)
+      hipLaunchKernelGGL((k_render<true, true>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+    else
+      hipLaunchKernelGGL((k_render<true, false>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+  } else {
+    if (diag)
+      hipLaunchKernelGGL((k_render<false, true>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+    else
+      hipLaunchKernelGGL((k_render<false, false>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+  }
 }
 
 }  // namespace vsg

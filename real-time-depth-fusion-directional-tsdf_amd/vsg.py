@@ -61,6 +61,11 @@ class vsg_counters(C.Structure):
                 ("reserved", C.c_uint32 * 7)]
 
 
+class vsg_diag(C.Structure):
+    _fields_ = [("alloc_buckets", C.c_uint32), ("ray_buckets", C.c_uint32), ("any_buckets", C.c_uint32),
+                ("ray_blocks", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
 # Every symbol include/vsg.h declares, with its ctypes signature.
 _SIGNATURES = {
     "vsg_version": (C.c_int32, []),
@@ -86,6 +91,8 @@ _SIGNATURES = {
     "vsg_render_buffers": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4),
     "vsg_render_download": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "vsg_render_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vsg_diag_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vsg_diag_read": (C.c_int, [C.c_void_p, C.POINTER(vsg_diag), C.c_int32]),
     "vsg_timing_enable": (C.c_int, [C.c_int32]),
     "vsg_timing_reset": (C.c_int, []),
     "vsg_timing_read": (C.c_int, [C.POINTER(C.c_double), C.POINTER(C.c_uint32)]),
@@ -287,6 +294,14 @@ class VoxelScene:
         c = vsg_counters()
         _check(lib().vsg_scene_counters(self._h, C.byref(c)))
         return {name: getattr(c, name) for name, _ in vsg_counters._fields_ if name != "reserved"}
+
+    def diag_enable(self, on: bool = True) -> None:
+        _check(lib().vsg_diag_enable(self._h, 1 if on else 0))
+
+    def diag_read(self, clear: bool = True) -> dict:
+        d = vsg_diag()
+        _check(lib().vsg_diag_read(self._h, C.byref(d), 1 if clear else 0))
+        return {name: getattr(d, name) for name, _ in vsg_diag._fields_ if name != "reserved"}
 
     def export_blocks(self):
         """(coords int32 [n,3], hash entry indices int32 [n]) in pointer order."""
