@@ -242,6 +242,8 @@ def main() -> int:
                       "updated_mean": int(np.mean([s["updated_voxels"] for s in st])),
                       "ray_blocks_mean": int(np.mean([s["ray_blocks"] for s in st])),
                       "hit_rate": round(float(np.mean([s["hits"] / npix for s in st])), 4)},
+        "march_per_ray": {k[6:]: round(float(np.mean([s[k] for s in st])) / npix, 2)
+                          for k in ("march_evals", "march_super_skips", "march_block_skips", "march_replayed")},
     }
     if args.compare_fp16 and args.store == "fp32":
         r16 = measure("fp16")

@@ -180,6 +180,40 @@ vsg_status vsg_render_download(vsg_render* render, float* depth, float* position
 /* Hit / miss counts of the last render (RayCastRender.cu:599-616).  Synchronising. */
 vsg_status vsg_render_counters(vsg_render* render, uint32_t* hits, uint32_t* misses);
 
+/* ---- differentiable raycast and pose refinement (no counterpart in the reference;
+ *      the reference's renderer is forward only) ---------------------------------------- */
+
+/* Adjoint of the last vsg_render_frame of this renderer.  Recomputes every pixel's
+ * march and differentiates the rendered depth through the three regula-falsi steps
+ * and the trilinear samples (the computation the forward performed; the sampled
+ * field's jumps at half-voxel planes are not differentiated):
+ *   jac        (device float[6*W*H] or NULL): d depth / d xi for the camera-frame
+ *              perturbation pose * exp(xi^), xi = (omega, upsilon); 0 without a hit;
+ *   grad_depth (device float[W*H]) with grad_sdf (device float[N*512]):
+ *              grad_sdf[ptr*512 + voxel] += sum_pixels grad_depth * d depth / d sdf.
+ * Asynchronous. */
+vsg_status vsg_render_adjoint(vsg_render* render, vsg_scene* scene, float* jac, const float* grad_depth,
+                              float* grad_sdf);
+
+/* One Gauss-Newton step of frame-to-model pose refinement: renders at pose (camera
+ * position = its translation), takes the adjoint's Jacobian, accumulates in FP64
+ * H = sum J J^T and g = sum J r over pixels with a rendered and an observed depth
+ * (device float[W*H], 0 = invalid) and |r| < max_residual, r = rendered - observed;
+ * solves (H + lambda diag H) delta = -g by FP64 Cholesky and composes
+ * pose <- pose * exp(delta^) in FP64 on the device.  pose: HOST double[16] row-major
+ * camera->world, updated in place.  stats (HOST double[8], may be NULL): cost before
+ * the step (sum r^2), residual count, solved (1/0), |omega| (rad), |upsilon| (m).
+ * Synchronising (the new pose is read back). */
+vsg_status vsg_pose_gn_step(vsg_render* render, vsg_scene* scene, const float* observed_depth,
+                            const vsg_intrinsics* intrinsics, float min_depth, float max_depth, float max_residual,
+                            double lambda, double pose[16], double stats[8]);
+
+/* Overwrites the voxels of blocks [first, first+nblocks) from HOST memory (the
+ * layout vsg_export_voxels writes; rgbf may be NULL to keep colours and flags).
+ * Synchronising. */
+vsg_status vsg_import_voxels(vsg_scene* scene, uint32_t first, uint32_t nblocks, const float* sdf,
+                             const uint16_t* weight, const uint8_t* rgbf);
+
 /* ---- diagnostics (byte-model counters; no counterpart in the reference) ------------ */
 
 typedef struct vsg_diag {
@@ -187,7 +221,8 @@ typedef struct vsg_diag {
   uint32_t ray_buckets;   /* distinct buckets with live entries the raycast probed    */
   uint32_t any_buckets;   /* union of the two                                         */
   uint32_t ray_blocks;    /* distinct live blocks a raycast trilinear corner touched  */
-  uint32_t reserved[4];
+  uint32_t march[4];      /* ray march: samples evaluated, super-block skips, block   */
+                          /* skips, samples skipped by replaying the step additions   */
 } vsg_diag;
 
 /* When on, integrate and render run instrumented kernel variants that mark the
@@ -211,7 +246,9 @@ enum {
   VSG_K_INTEGRATE = 5,
   VSG_K_RAYCAST = 6,
   VSG_K_RESET = 7,
-  VSG_K_COUNT = 8
+  VSG_K_ADJOINT = 8,
+  VSG_K_GN = 9,
+  VSG_K_COUNT = 10
 };
 
 /* When enabled, every kernel launch is bracketed by HIP events on the stream it

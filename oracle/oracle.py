@@ -61,6 +61,12 @@ def lib() -> C.CDLL:
         L.vsgo_trilinear.restype = C.c_int
         L.vsgo_trilinear.argtypes = [vp, vp, C.c_float, vp, vp]
         L.vsgo_depth_range.argtypes = [vp, C.c_long, C.c_long, vp]
+        L.vsgo_render_adjoint.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, C.c_double, vp, C.c_long, vp, vp, vp,
+                                          vp, vp]
+        L.vsgo_get_sdf.restype = C.c_double
+        L.vsgo_get_sdf.argtypes = [vp, C.c_int64]
+        L.vsgo_set_sdf.argtypes = [vp, C.c_int64, C.c_double]
         _lib = L
     return _lib
 
@@ -129,6 +135,40 @@ class OracleScene:
                                 None if plist is None else plist.ctypes.data, 0 if plist is None else plist.size)
         return {"depth": depth, "pos": pos, "col": col, "nrm": nrm, "hits": int(hm[0]), "misses": int(hm[1]),
                 "ran": bool(ran)}
+
+    def render_adjoint(self, W: int, H: int, cam, pose, intr, minD: float, maxD: float, pixels,
+                       grad_depth=None, voxel: float | None = None):
+        """(J [n,6] = d depth / d xi for pose * exp(xi^), hit [n] bool, grad_sdf or None).
+        self.last_depth receives the depth of the listed pixels in the oracle's arithmetic."""
+        pix = np.ascontiguousarray(pixels, dtype=np.int32)
+        n = pix.size
+        J = np.zeros((n, 6), np.float64)
+        hit = np.zeros(n, np.uint8)
+        depth = np.zeros(n, np.float64)
+        camd = np.ascontiguousarray(np.asarray(cam, np.float64).reshape(3))
+        gd = gs = None
+        if grad_depth is not None:
+            gd = np.ascontiguousarray(grad_depth, dtype=np.float64)
+            gs = np.zeros(self.num_allocated() * 512, np.float64)
+        fx, fy, cx, cy = intr
+        lib().vsgo_render_adjoint(self._h, W, H, camd.ctypes.data, _pose(pose).ctypes.data, fx, fy, cx, cy, minD, maxD,
+                                  self.voxel if voxel is None else voxel, pix.ctypes.data, n, J.ctypes.data,
+                                  hit.ctypes.data, depth.ctypes.data, None if gd is None else gd.ctypes.data,
+                                  None if gs is None else gs.ctypes.data)
+        self.last_depth = depth
+        return J, hit.astype(bool), gs
+
+    def depth_at(self, W, H, pose, intr, minD, maxD, pixels) -> np.ndarray:
+        """Rendered depth of listed pixels in the oracle's arithmetic (float64 output)."""
+        T = np.asarray(pose, np.float64)
+        self.render_adjoint(W, H, T[:3, 3], T, intr, minD, maxD, pixels)
+        return self.last_depth.copy()
+
+    def get_sdf(self, idx: int) -> float:
+        return float(lib().vsgo_get_sdf(self._h, int(idx)))
+
+    def set_sdf(self, idx: int, value: float) -> None:
+        lib().vsgo_set_sdf(self._h, int(idx), float(value))
 
     def num_allocated(self) -> int:
         return int(lib().vsgo_num_allocated(self._h))
@@ -209,3 +249,24 @@ def depth_range(depth) -> tuple[float, float]:
     out = np.zeros(2, np.float32)
     lib().vsgo_depth_range(d.ctypes.data, d.size, 1, out.ctypes.data)
     return float(out[0]), float(out[1])
+
+
+def se3_exp(xi) -> np.ndarray:
+    """4x4 exp of xi = (omega, upsilon) (rotation vector, translation), float64."""
+    xi = np.asarray(xi, np.float64)
+    w, u = xi[:3], xi[3:]
+    th = np.linalg.norm(w)
+    K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    if th < 1e-12:
+        R = np.eye(3) + K
+        V = np.eye(3) + 0.5 * K
+    else:
+        A = np.sin(th) / th
+        B = (1 - np.cos(th)) / th ** 2
+        Cc = (th - np.sin(th)) / th ** 3
+        R = np.eye(3) + A * K + B * K @ K
+        V = np.eye(3) + B * K + Cc * K @ K
+    T = np.eye(4)
+    T[:3, :3] = R
+    T[:3, 3] = V @ u
+    return T

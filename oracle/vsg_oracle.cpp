@@ -215,6 +215,51 @@ int vsgo_trilinear(void* p, const float* p3, float v, float* dist, uint8_t* col3
   return trilinear(h->sf, V3<float>{p3[0], p3[1], p3[2]}, v, dist, col3) ? 1 : 0;
 }
 
+// Raycast adjoint for listed pixels: J (6 per pixel, d depth / d xi for T * exp(xi^)),
+// depth (in the oracle's arithmetic; 1e10 without a hit),
+// and, when grad_depth is given, grad_sdf[ptr*512 + voxel] += grad_depth[k] * d depth / d sdf.
+// hit[k] = 1 where the pixel has a hit.  grad_sdf covers num_allocated() blocks.
+void vsgo_render_adjoint(void* p, int W, int H, const double* cam3, const double* T16, double fx, double fy, double cx,
+                         double cy, double minD, double maxD, double voxel, const int32_t* pix, long n, double* J,
+                         uint8_t* hit, double* depth, const double* grad_depth, double* grad_sdf) {
+  Handle* h = (Handle*)p;
+  (void)H;
+  auto run = [&](auto& s, auto tag) {
+    using R = decltype(tag);
+    R M[16];
+    to_pose(T16, M);
+    V3<R> cam{(R)cam3[0], (R)cam3[1], (R)cam3[2]};
+    std::vector<int64_t> vx((size_t)n * 32);
+    std::vector<double> dd((size_t)n * 32);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long k = 0; k < n; ++k) {
+      double dk = 0;
+      bool ok = adjoint_pixel(s, pix[k] % W, pix[k] / W, cam, M, (R)fx, (R)fy, (R)cx, (R)cy, (R)minD, (R)maxD,
+                              (R)voxel, &J[6 * k], &vx[32 * k], &dd[32 * k], &dk);
+      if (depth) depth[k] = dk;
+      if (hit) hit[k] = ok ? 1 : 0;
+    }
+    if (grad_depth && grad_sdf) {
+      for (long k = 0; k < n; ++k)
+        for (int j = 0; j < 32; ++j)
+          if (vx[32 * k + j] >= 0) grad_sdf[vx[32 * k + j]] += grad_depth[k] * dd[32 * k + j];
+    }
+  };
+  if (h->fp64) run(h->sd, 0.0);
+  else run(h->sf, 0.0f);
+}
+
+// Reads / overwrites the stored SDF of one voxel (global index ptr*512 + voxel).
+double vsgo_get_sdf(void* p, int64_t idx) {
+  Handle* h = (Handle*)p;
+  return h->fp64 ? h->sd.load_sdf((size_t)idx) : (double)h->sf.load_sdf((size_t)idx);
+}
+void vsgo_set_sdf(void* p, int64_t idx, double value) {
+  Handle* h = (Handle*)p;
+  if (h->fp64) h->sd.store_sdf((size_t)idx, value);
+  else h->sf.store_sdf((size_t)idx, (float)value);
+}
+
 // Driver's per-frame depth range (VoxelSG/src/voxelSG.cpp:594-608), float arithmetic.
 void vsgo_depth_range(const float* depth, long n, long stride, float* out2) {
   float mn = 1e10f, mx = -1e10f;

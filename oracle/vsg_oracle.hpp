@@ -537,7 +537,90 @@ inline bool refine(const Scene<R>& s, V3<R> cam, V3<R> dir, R v, R d0, R r0, R d
   return true;
 }
 
-// ---- A10/A11: renderKernel + traverseCoarseGridSimpleSampleAll (RayCastRender.cu:276-517).
+// ---- A10/A11: one pixel of renderKernel + traverseCoarseGridSimpleSampleAll
+// (RayCastRender.cu:276-517).  Also returns the bracket [a, b] (ray parameters and
+// sdf values) at which the accepted crossing was refined; the adjoint restarts there.
+template <class R> struct MarchResult {
+  bool marched = false;  // depth range valid (RayCastRender.cu:461-462)
+  bool hit = false;
+  R alpha = R(0), d2r = R(1);
+  V3<R> cd{R(0), R(0), R(0)}, wd{R(0), R(0), R(0)};
+  R a = R(0), ad = R(0), b = R(0), bd = R(0);
+  uint8_t rgb[3] = {0, 0, 0};
+};
+
+template <class R>
+inline MarchResult<R> march_pixel(const Scene<R>& s, int x, int y, V3<R> cam, const R* M, R fx, R fy, R cx, R cy,
+                                  R minD, R maxD, R v) {
+  const R MINF = R(1e10f);
+  MarchResult<R> res;
+  V3<R> cd{((R)x - cx) / fx, ((R)y - cy) / fy, R(1)};
+  R cl = std::sqrt(cd.x * cd.x + cd.y * cd.y + cd.z * cd.z);
+  if (cl > R(1e-6f)) {
+    cd.x /= cl;
+    cd.y /= cl;
+    cd.z /= cl;
+  }
+  V3<R> wd{M[0] * cd.x + M[1] * cd.y + M[2] * cd.z, M[4] * cd.x + M[5] * cd.y + M[6] * cd.z,
+           M[8] * cd.x + M[9] * cd.y + M[10] * cd.z};
+  R wl = std::sqrt(wd.x * wd.x + wd.y * wd.y + wd.z * wd.z);
+  if (wl > R(1e-6f)) {
+    wd.x /= wl;
+    wd.y /= wl;
+    wd.z /= wl;
+  }
+  res.cd = cd;
+  res.wd = wd;
+  if (minD == R(0) || minD == MINF) return res;
+  if (maxD == R(0) || maxD == MINF) return res;
+  res.marched = true;
+  R last_sdf = R(0), last_alpha = R(0);
+  unsigned last_w = 0;
+  const R d2r = R(1) / cd.z;
+  res.d2r = d2r;
+  R ray = d2r * std::fmax(minD, minD);
+  R ray_end = d2r * std::fmin(maxD, maxD);
+  const R th = v * R(3);
+  const R inc = v * R(0.5f);
+  while (ray < ray_end) {
+    V3<R> p{cam.x + ray * wd.x, cam.y + ray * wd.y, cam.z + ray * wd.z};
+    R dist;
+    uint8_t c1[3];
+    if (trilinear(s, p, v, &dist, c1)) {
+      if (last_w > 0 && last_sdf < R(0) && dist > R(0)) {
+        R alpha = R(0);
+        uint8_t c2[3] = {0, 0, 0};
+        bool b = refine(s, cam, wd, v, last_sdf, last_alpha, dist, ray, &alpha, c2);
+        if (b && std::fabs(last_sdf - dist) < th) {
+          if (std::fabs(dist) < th) {
+            res.hit = true;
+            res.alpha = alpha;
+            res.a = last_alpha;
+            res.ad = last_sdf;
+            res.b = ray;
+            res.bd = dist;
+            res.rgb[0] = c2[0];
+            res.rgb[1] = c2[1];
+            res.rgb[2] = c2[2];
+            return res;
+          }
+        }
+      }
+      last_sdf = dist;
+      last_alpha = ray;
+      last_w = 1;
+    } else {
+      last_w = 0;
+    }
+    // rayCurrent += voxelSize * 0.5f.  When the step is absorbed by rounding the
+    // reference's loop never ends; this restatement stops there (a miss).
+    R next = ray + inc;
+    if (next == ray) break;
+    ray = next;
+  }
+  return res;
+}
+
 // Outputs: depth[W*H], pos[4*W*H], col[4*W*H], nrm[4*W*H] (normals stay 1e10).
 // Returns 0 if the kernel would not run (no allocated blocks: outputs untouched).
 template <class R>
@@ -548,90 +631,212 @@ inline int render(const Scene<R>& s, int W, int H, V3<R> cam, const R* M, R fx, 
   *misses = 0;
   if (s.num_allocated() == 0) return 0;  // RayCastRender.cu:553-556
   const R MINF = R(1e10f);
-  const R v = voxel_size;
   uint32_t nh = 0, nm = 0;
   // Pixels to trace: all (row-major), or the listed ones (others left untouched).
   const long nwork = pix_list ? n_list : (long)W * H;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : nh, nm)
   for (long wi = 0; wi < nwork; ++wi) {
-    {
-      const int pix = pix_list ? pix_list[wi] : (int)wi;
-      const int x = pix % W, y = pix / W;
-      depth[pix] = (float)MINF;
-      for (int k = 0; k < 4; ++k) {
-        pos[4 * pix + k] = (float)MINF;
-        col[4 * pix + k] = (float)MINF;
-        nrm[4 * pix + k] = (float)MINF;
-      }
-      V3<R> cd{((R)x - cx) / fx, ((R)y - cy) / fy, R(1)};
-      R cl = std::sqrt(cd.x * cd.x + cd.y * cd.y + cd.z * cd.z);
-      if (cl > R(1e-6f)) {
-        cd.x /= cl;
-        cd.y /= cl;
-        cd.z /= cl;
-      }
-      V3<R> wd{M[0] * cd.x + M[1] * cd.y + M[2] * cd.z, M[4] * cd.x + M[5] * cd.y + M[6] * cd.z,
-               M[8] * cd.x + M[9] * cd.y + M[10] * cd.z};
-      R wl = std::sqrt(wd.x * wd.x + wd.y * wd.y + wd.z * wd.z);
-      if (wl > R(1e-6f)) {
-        wd.x /= wl;
-        wd.y /= wl;
-        wd.z /= wl;
-      }
-      if (minD == R(0) || minD == MINF) continue;
-      if (maxD == R(0) || maxD == MINF) continue;
-      R last_sdf = R(0), last_alpha = R(0);
-      unsigned last_w = 0;
-      const R d2r = R(1) / cd.z;
-      R ray = d2r * std::fmax(minD, minD);
-      R ray_end = d2r * std::fmin(maxD, maxD);
-      const R th = v * R(3);
-      const R inc = v * R(0.5f);
-      bool hit = false;
-      while (ray < ray_end) {
-        V3<R> p{cam.x + ray * wd.x, cam.y + ray * wd.y, cam.z + ray * wd.z};
-        R dist;
-        uint8_t c1[3];
-        if (trilinear(s, p, v, &dist, c1)) {
-          if (last_w > 0 && last_sdf < R(0) && dist > R(0)) {
-            R alpha = R(0);
-            uint8_t c2[3] = {0, 0, 0};
-            bool b = refine(s, cam, wd, v, last_sdf, last_alpha, dist, ray, &alpha, c2);
-            if (b && std::fabs(last_sdf - dist) < th) {
-              if (std::fabs(dist) < th) {
-                V3<R> iso{cam.x + alpha * wd.x, cam.y + alpha * wd.y, cam.z + alpha * wd.z};
-                depth[pix] = (float)(alpha / d2r);
-                pos[4 * pix] = (float)iso.x;
-                pos[4 * pix + 1] = (float)iso.y;
-                pos[4 * pix + 2] = (float)iso.z;
-                pos[4 * pix + 3] = 1.0f;
-                col[4 * pix] = (float)((R)c2[0] / R(255));
-                col[4 * pix + 1] = (float)((R)c2[1] / R(255));
-                col[4 * pix + 2] = (float)((R)c2[2] / R(255));
-                col[4 * pix + 3] = 1.0f;
-                hit = true;
-                break;
-              }
-            }
-          }
-          last_sdf = dist;
-          last_alpha = ray;
-          last_w = 1;
-        } else {
-          last_w = 0;
-        }
-        // rayCurrent += voxelSize * 0.5f.  When the step is absorbed by rounding the
-        // reference's loop never ends; this restatement stops there (a miss).
-        R next = ray + inc;
-        if (next == ray) break;
-        ray = next;
-      }
-      if (hit) nh++; else nm++;
+    const int pix = pix_list ? pix_list[wi] : (int)wi;
+    depth[pix] = (float)MINF;
+    for (int k = 0; k < 4; ++k) {
+      pos[4 * pix + k] = (float)MINF;
+      col[4 * pix + k] = (float)MINF;
+      nrm[4 * pix + k] = (float)MINF;
+    }
+    MarchResult<R> m = march_pixel(s, pix % W, pix / W, cam, M, fx, fy, cx, cy, minD, maxD, voxel_size);
+    if (!m.marched) continue;
+    if (m.hit) {
+      const R alpha = m.alpha;
+      V3<R> iso{cam.x + alpha * m.wd.x, cam.y + alpha * m.wd.y, cam.z + alpha * m.wd.z};
+      depth[pix] = (float)(alpha / m.d2r);
+      pos[4 * pix] = (float)iso.x;
+      pos[4 * pix + 1] = (float)iso.y;
+      pos[4 * pix + 2] = (float)iso.z;
+      pos[4 * pix + 3] = 1.0f;
+      col[4 * pix] = (float)((R)m.rgb[0] / R(255));
+      col[4 * pix + 1] = (float)((R)m.rgb[1] / R(255));
+      col[4 * pix + 2] = (float)((R)m.rgb[2] / R(255));
+      col[4 * pix + 3] = 1.0f;
+      nh++;
+    } else {
+      nm++;
     }
   }
   *hits = nh;
   *misses = nm;
   return 1;
+}
+
+// ---- Raycast adjoint (no counterpart in the reference) -----------------------------
+//
+// Rendered depth = alpha * cd.z, alpha the third regula-falsi point.  Differentiated
+// along the computation the forward performed (fixed bracket choices, fixed trilinear
+// corner sets; the field's jumps at half-voxel planes are not differentiated), with
+// respect to
+//   * a camera-frame SE(3) perturbation T(xi) = T * exp(xi^), xi = (omega, upsilon):
+//     a point at ray parameter r moves as dp = R (upsilon + omega x r*cd) + wd dr;
+//   * the SDF of every voxel whose value entered alpha (up to 8 corners of each of the
+//     four trilinear evaluations alpha depends on).
+// Derivatives are carried in forward mode in double: 6 pose components and 4 seeds, one
+// per evaluation (its value perturbed directly); d alpha / d sdf_j = sum_k seed_k *
+// (w_j / sum w) of evaluation k.
+
+struct Dual10 {
+  double v;
+  double d[10];
+};
+
+// Trilinear value (bit-identical to trilinear()) plus its gradient in world
+// coordinates and the corners' partials d f / d sdf_j = w_j / sum(w).
+template <class R> struct TriGrad {
+  bool ok = false;
+  R val = R(0);
+  double g[3] = {0, 0, 0};
+  int64_t vox[8];
+  double dfds[8];
+};
+
+template <class R> inline TriGrad<R> trilinear_grad(const Scene<R>& s, V3<R> p, R v) {
+  TriGrad<R> t;
+  uint8_t col[3];
+  R dist;
+  t.ok = trilinear(s, p, v, &dist, col);
+  for (int i = 0; i < 8; ++i) {
+    t.vox[i] = -1;
+    t.dfds[i] = 0.0;
+  }
+  if (!t.ok) return t;
+  t.val = dist;
+  const R off = v;
+  V3<R> pd{p.x - off / R(2), p.y - off / R(2), p.z - off / R(2)};
+  V3<R> q{pd.x / v, pd.y / v, pd.z / v};
+  const double fr[3] = {(double)frac01(q.x), (double)frac01(q.y), (double)frac01(q.z)};
+  double W = 0, N = 0, dW[3] = {0, 0, 0}, dN[3] = {0, 0, 0}, w[8];
+  double sv[8];
+  for (int i = 0; i < 8; ++i) {
+    w[i] = 0;
+    R ax = (i & 1) ? off : R(0), ay = (i & 2) ? off : R(0), az = (i & 4) ? off : R(0);
+    V3<R> c{pd.x + ax, pd.y + ay, pd.z + az};
+    int32_t vx = ray_voxel_axis(c.x, v), vy = ray_voxel_axis(c.y, v), vz = ray_voxel_axis(c.z, v);
+    I3 b{voxel_to_block_axis(vx), voxel_to_block_axis(vy), voxel_to_block_axis(vz)};
+    uint32_t h = hash_bucket(b, s.num_buckets);
+    int64_t found = -1;
+    for (int j = 0; j < kBucket; ++j) {
+      size_t e = (size_t)h * kBucket + j;
+      if (s.slot_ptr[e] != -1 && s.slot_pos[e] == b)
+        found = (int64_t)s.slot_ptr[e] * kVox + voxel_local_axis(vx) + voxel_local_axis(vy) * kBlock +
+                voxel_local_axis(vz) * kBlock * kBlock;
+    }
+    if (found < 0 || s.weight[(size_t)found] == 0) continue;
+    double phi[3], dphi[3];
+    for (int a = 0; a < 3; ++a) {
+      bool hi = (i >> a) & 1;
+      phi[a] = hi ? fr[a] : 1.0 - fr[a];
+      dphi[a] = (hi ? 1.0 : -1.0) / (double)v;
+    }
+    w[i] = phi[0] * phi[1] * phi[2];
+    sv[i] = (double)s.load_sdf((size_t)found);
+    t.vox[i] = found;
+    W += w[i];
+    N += w[i] * sv[i];
+    const double dwi[3] = {dphi[0] * phi[1] * phi[2], phi[0] * dphi[1] * phi[2], phi[0] * phi[1] * dphi[2]};
+    for (int a = 0; a < 3; ++a) {
+      dW[a] += dwi[a];
+      dN[a] += dwi[a] * sv[i];
+    }
+  }
+  const double f = N / W;
+  for (int a = 0; a < 3; ++a) t.g[a] = (dN[a] - f * dW[a]) / W;
+  for (int i = 0; i < 8; ++i)
+    if (t.vox[i] >= 0) t.dfds[i] = w[i] / W;
+  return t;
+}
+
+// Dual of one evaluation f(p(r)) at ray parameter r (dual: its derivative), seed k.
+template <class R>
+inline Dual10 eval_dual(const TriGrad<R>& tg, const Dual10& r, const double* cdd, const double* wdd,
+                        const double* Rm, int seed) {
+  // camera-frame point and the world gradient rotated into the camera frame
+  const double q[3] = {r.v * cdd[0], r.v * cdd[1], r.v * cdd[2]};
+  const double gc[3] = {Rm[0] * tg.g[0] + Rm[3] * tg.g[1] + Rm[6] * tg.g[2],
+                        Rm[1] * tg.g[0] + Rm[4] * tg.g[1] + Rm[7] * tg.g[2],
+                        Rm[2] * tg.g[0] + Rm[5] * tg.g[1] + Rm[8] * tg.g[2]};
+  const double gw = tg.g[0] * wdd[0] + tg.g[1] * wdd[1] + tg.g[2] * wdd[2];  // d f / d r
+  Dual10 e;
+  e.v = (double)tg.val;
+  // omega: q x gc ; upsilon: gc
+  const double dom[3] = {q[1] * gc[2] - q[2] * gc[1], q[2] * gc[0] - q[0] * gc[2], q[0] * gc[1] - q[1] * gc[0]};
+  for (int k = 0; k < 10; ++k) e.d[k] = gw * r.d[k];
+  for (int k = 0; k < 3; ++k) {
+    e.d[k] += dom[k];
+    e.d[3 + k] += gc[k];
+  }
+  e.d[6 + seed] += 1.0;
+  return e;
+}
+
+// d depth / d xi (6) and the SDF partials of one pixel.  Returns false without a hit.
+template <class R>
+inline bool adjoint_pixel(const Scene<R>& s, int x, int y, V3<R> cam, const R* M, R fx, R fy, R cx, R cy, R minD,
+                          R maxD, R v, double J[6], int64_t vox[32], double dds[32], double* depth) {
+  MarchResult<R> m = march_pixel(s, x, y, cam, M, fx, fy, cx, cy, minD, maxD, v);
+  *depth = m.hit ? (double)(m.alpha / m.d2r) : 1e10;
+  for (int k = 0; k < 6; ++k) J[k] = 0;
+  for (int k = 0; k < 32; ++k) {
+    vox[k] = -1;
+    dds[k] = 0;
+  }
+  if (!m.hit) return false;
+  const double Rm[9] = {(double)M[0], (double)M[1], (double)M[2], (double)M[4], (double)M[5],
+                        (double)M[6], (double)M[8], (double)M[9], (double)M[10]};
+  const double cdd[3] = {(double)m.cd.x, (double)m.cd.y, (double)m.cd.z};
+  const double wdd[3] = {(double)m.wd.x, (double)m.wd.y, (double)m.wd.z};
+  TriGrad<R> tg[4];
+  auto pos = [&](R r) { return V3<R>{cam.x + r * m.wd.x, cam.y + r * m.wd.y, cam.z + r * m.wd.z}; };
+  // bracket: constants a, b; their evaluations are seeds 0 and 1
+  Dual10 a{}, b{};
+  a.v = (double)m.a;
+  b.v = (double)m.b;
+  tg[0] = trilinear_grad(s, pos(m.a), v);
+  tg[1] = trilinear_grad(s, pos(m.b), v);
+  Dual10 ad = eval_dual(tg[0], a, cdd, wdd, Rm, 0);
+  Dual10 bd = eval_dual(tg[1], b, cdd, wdd, Rm, 1);
+  // float state that reproduces the forward's sample positions and branch choices
+  R fa = m.a, fad = m.ad, fb = m.b, fbd = m.bd, fc = R(0);
+  Dual10 c{};
+  for (int it = 0; it < 3; ++it) {
+    fc = fa + (fad / (fad - fbd)) * (fb - fa);
+    const double den = ad.v - bd.v, u = ad.v / den;
+    c.v = a.v + u * (b.v - a.v);
+    for (int k = 0; k < 10; ++k) {
+      const double du = (-ad.d[k] * bd.v + ad.v * bd.d[k]) / (den * den);
+      c.d[k] = a.d[k] + du * (b.v - a.v) + u * (b.d[k] - a.d[k]);
+    }
+    if (it == 2) break;
+    tg[2 + it] = trilinear_grad(s, pos(fc), v);
+    Dual10 cdual = c;
+    cdual.v = (double)fc;
+    Dual10 cd = eval_dual(tg[2 + it], cdual, cdd, wdd, Rm, 2 + it);
+    const R fcd = tg[2 + it].val;
+    if (fad * fcd > R(0)) {
+      fa = fc;
+      fad = fcd;
+      a = cdual;
+      ad = cd;
+    } else {
+      fb = fc;
+      fbd = fcd;
+      b = cdual;
+      bd = cd;
+    }
+  }
+  for (int k = 0; k < 6; ++k) J[k] = c.d[k] * cdd[2];
+  for (int e = 0; e < 4; ++e)
+    for (int i = 0; i < 8; ++i) {
+      vox[8 * e + i] = tg[e].vox[i];
+      dds[8 * e + i] = c.d[6 + e] * tg[e].dfds[i] * cdd[2];
+    }
+  return true;
 }
 
 }  // namespace vsgo

@@ -116,6 +116,7 @@ struct vsg_scene {
   uint8_t* mark_abkt = nullptr;  // diagnostics marks (B, B, N bytes)
   uint8_t* mark_rbkt = nullptr;
   uint8_t* mark_blk = nullptr;
+  uint32_t* march_stats = nullptr;  // 4 counters
 };
 
 struct vsg_render {
@@ -131,6 +132,11 @@ struct vsg_render {
   uint32_t* rc = nullptr;  // 2 sets x {hits, misses}
   uint32_t calls = 0;
   hipEvent_t ev = nullptr;
+  RenderArgs last_args{};  // arguments of the last render (the adjoint's forward)
+  bool has_args = false;
+  float* jac = nullptr;       // GN scratch: Jacobian per pixel
+  double* partials = nullptr; // GN scratch: per-workgroup sums
+  double* gn_out = nullptr;   // GN result (25 doubles)
 };
 
 namespace {
@@ -152,6 +158,7 @@ void free_scene(vsg_scene* s) {
   (void)hipFree(s->mark_abkt);
   (void)hipFree(s->mark_rbkt);
   (void)hipFree(s->mark_blk);
+  (void)hipFree(s->march_stats);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -528,6 +535,9 @@ vsg_status vsg_render_destroy(vsg_render* r) {
   (void)hipFree(r->col);
   (void)hipFree(r->nrm);
   (void)hipFree(r->rc);
+  (void)hipFree(r->jac);
+  (void)hipFree(r->partials);
+  (void)hipFree(r->gn_out);
   if (r->ev) (void)hipEventDestroy(r->ev);
   delete r;
   return VSG_OK;
@@ -565,6 +575,7 @@ vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3],
   ra.H = r->H;
   ra.mark_bkt = s->diag ? s->mark_rbkt : nullptr;
   ra.mark_blk = s->diag ? s->mark_blk : nullptr;
+  ra.march_stats = s->diag ? s->march_stats : nullptr;
   hipStream_t strm = r->has_stream ? r->stream : s->stream;
   if (strm != s->stream) {  // order after the scene's pending work
     VSG_HIP(hipEventRecord(r->ev, s->stream));
@@ -586,6 +597,8 @@ vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3],
   r->last = strm;
   r->has_last = true;
   r->calls++;
+  r->last_args = ra;
+  r->has_args = true;
   return VSG_OK;
 }
 
@@ -621,6 +634,113 @@ vsg_status vsg_render_counters(vsg_render* r, uint32_t* hits, uint32_t* misses) 
   return VSG_OK;
 }
 
+// ---- adjoint and pose refinement ---------------------------------------------------------
+
+vsg_status vsg_render_adjoint(vsg_render* r, vsg_scene* s, float* jac, const float* grad_depth, float* grad_sdf) {
+  if (!r || !s) return fail(VSG_ERR_INVALID_ARG, "vsg_render_adjoint: null argument");
+  if (!r->has_args) return fail(VSG_ERR_NOT_READY, "vsg_render_adjoint: no render to differentiate");
+  if ((grad_depth == nullptr) != (grad_sdf == nullptr))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_adjoint: grad_depth and grad_sdf go together");
+  RenderArgs ra = r->last_args;
+  ra.mark_bkt = ra.mark_blk = nullptr;
+  {
+    TimedLaunch tl(r->last, VSG_K_ADJOINT);
+    launch_render_adjoint(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, jac, grad_depth,
+                          grad_sdf);
+  }
+  return check_launch("render adjoint");
+}
+
+vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_depth, const vsg_intrinsics* k,
+                            float min_depth, float max_depth, float max_residual, double lambda, double pose[16],
+                            double stats[8]) {
+  if (!r || !s || !observed_depth || !k || !pose) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_step: null argument");
+  for (int i = 0; i < 16; ++i)
+    if (!std::isfinite(pose[i])) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_step: pose must be finite");
+  const int npix = r->W * r->H;
+  if (!r->jac) {
+    vsg_status st = dmalloc(&r->jac, (size_t)npix * 6 * sizeof(float), "Jacobian scratch");
+    if (st == VSG_OK) st = dmalloc(&r->partials, (size_t)gn_partial_count(npix) * 29 * sizeof(double), "GN scratch");
+    if (st == VSG_OK) st = dmalloc(&r->gn_out, 32 * sizeof(double), "GN result");
+    if (st != VSG_OK) return st;
+  }
+  float pf[16], cam[3];
+  for (int i = 0; i < 16; ++i) pf[i] = (float)pose[i];
+  cam[0] = pf[3];
+  cam[1] = pf[7];
+  cam[2] = pf[11];
+  vsg_status st = vsg_render_frame(r, s, cam, pf, k, min_depth, max_depth, 0.0f);
+  if (st != VSG_OK) return st;
+  st = vsg_render_adjoint(r, s, r->jac, nullptr, nullptr);
+  if (st != VSG_OK) return st;
+  {
+    TimedLaunch tl(r->last, VSG_K_GN);
+    launch_gn(r->last, r->depth, observed_depth, r->jac, npix, max_residual, lambda, pose, r->partials, r->gn_out);
+  }
+  st = check_launch("gauss-newton");
+  if (st != VSG_OK) return st;
+  double out[25];
+  VSG_HIP(hipStreamSynchronize(r->last));
+  VSG_HIP(hipMemcpy(out, r->gn_out, sizeof(out), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 16; ++i) pose[i] = out[i];
+  if (stats) {
+    stats[0] = out[22];
+    stats[1] = out[23];
+    stats[2] = out[24];
+    stats[3] = std::sqrt(out[16] * out[16] + out[17] * out[17] + out[18] * out[18]);
+    stats[4] = std::sqrt(out[19] * out[19] + out[20] * out[20] + out[21] * out[21]);
+    stats[5] = stats[6] = stats[7] = 0.0;
+  }
+  return VSG_OK;
+}
+
+vsg_status vsg_import_voxels(vsg_scene* s, uint32_t first, uint32_t nblocks, const float* sdf, const uint16_t* weight,
+                             const uint8_t* rgbf) {
+  if (!s || !sdf || !weight) return fail(VSG_ERR_INVALID_ARG, "vsg_import_voxels: null argument");
+  if ((uint64_t)first + nblocks > s->t.num_blocks)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_import_voxels: block range beyond the store");
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  const size_t nv = (size_t)nblocks * kVox, off = (size_t)first * kVox;
+  if (nv == 0) return VSG_OK;
+  std::vector<uint8_t> w(nv * s->word_bytes);
+  for (size_t i = 0; i < nv; ++i) {
+    if (s->word_bytes == 8) {
+      uint32_t a, b = weight[i];
+      std::memcpy(&a, &sdf[i], 4);
+      std::memcpy(&w[8 * i], &a, 4);
+      std::memcpy(&w[8 * i + 4], &b, 4);
+    } else {
+      // binary32 -> binary16, round to nearest even
+      uint32_t x;
+      std::memcpy(&x, &sdf[i], 4);
+      const uint32_t sign = (x >> 16) & 0x8000u, ax = x & 0x7FFFFFFFu;
+      uint32_t h;
+      if (ax >= 0x7F800000u) h = sign | 0x7C00u | (ax > 0x7F800000u ? 0x200u : 0u);
+      else if (ax >= 0x477FF000u) h = sign | 0x7C00u;
+      else if (ax < 0x38800000u) {
+        if (ax < 0x33000000u) h = sign;
+        else {
+          const uint32_t e = ax >> 23, m = (ax & 0x7FFFFFu) | 0x800000u, sh = 126u - e;
+          uint32_t q = m >> sh;
+          const uint32_t rem = m & ((1u << sh) - 1u), half = 1u << (sh - 1u);
+          if (rem > half || (rem == half && (q & 1u))) q++;
+          h = sign | q;
+        }
+      } else {
+        uint32_t q = ((ax >> 23) - 112u) << 10 | ((ax & 0x7FFFFFu) >> 13);
+        const uint32_t rem = ax & 0x1FFFu;
+        if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
+        h = sign | q;
+      }
+      const uint32_t word = (h & 0xFFFFu) | ((uint32_t)weight[i] << 16);
+      std::memcpy(&w[4 * i], &word, 4);
+    }
+  }
+  VSG_HIP(hipMemcpy((uint8_t*)s->vox + off * s->word_bytes, w.data(), w.size(), hipMemcpyHostToDevice));
+  if (rgbf) VSG_HIP(hipMemcpy(s->col + off, rgbf, nv * sizeof(uint32_t), hipMemcpyHostToDevice));
+  return VSG_OK;
+}
+
 // ---- diagnostics ----------------------------------------------------------------------
 
 vsg_status vsg_diag_enable(vsg_scene* s, int32_t on) {
@@ -630,9 +750,11 @@ vsg_status vsg_diag_enable(vsg_scene* s, int32_t on) {
     vsg_status r = dmalloc(&s->mark_abkt, s->t.num_buckets, "diagnostic marks");
     if (r == VSG_OK) r = dmalloc(&s->mark_rbkt, s->t.num_buckets, "diagnostic marks");
     if (r == VSG_OK) r = dmalloc(&s->mark_blk, s->t.num_blocks, "diagnostic marks");
+    if (r == VSG_OK) r = dmalloc(&s->march_stats, 4 * sizeof(uint32_t), "diagnostic counters");
     if (r != VSG_OK) return r;
   }
   if (on) {
+    VSG_HIP(hipMemset(s->march_stats, 0, 4 * sizeof(uint32_t)));
     VSG_HIP(hipMemset(s->mark_abkt, 0, s->t.num_buckets));
     VSG_HIP(hipMemset(s->mark_rbkt, 0, s->t.num_buckets));
     VSG_HIP(hipMemset(s->mark_blk, 0, s->t.num_blocks));
@@ -656,7 +778,9 @@ vsg_status vsg_diag_read(vsg_scene* s, vsg_diag* out, int32_t clear) {
     out->any_buckets += (a[i] | r[i]) ? 1u : 0u;
   }
   for (size_t i = 0; i < b.size(); ++i) out->ray_blocks += b[i] ? 1u : 0u;
+  VSG_HIP(hipMemcpy(out->march, s->march_stats, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (clear) {
+    VSG_HIP(hipMemset(s->march_stats, 0, 4 * sizeof(uint32_t)));
     VSG_HIP(hipMemset(s->mark_abkt, 0, s->t.num_buckets));
     VSG_HIP(hipMemset(s->mark_rbkt, 0, s->t.num_buckets));
     VSG_HIP(hipMemset(s->mark_blk, 0, s->t.num_blocks));

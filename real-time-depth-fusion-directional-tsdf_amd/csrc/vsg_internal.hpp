@@ -17,6 +17,7 @@ struct RenderArgs {
   int32_t W, H;
   uint8_t* mark_bkt;  // diagnostics (null when off): buckets with live entries probed
   uint8_t* mark_blk;  // diagnostics (null when off): blocks a trilinear corner touched
+  uint32_t* march_stats;  // diagnostics (null when off): evaluations, super/block skips, replayed steps
 };
 
 void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col);
@@ -33,6 +34,13 @@ void launch_compact(hipStream_t s, const TableView& t, DevState* st, const Frame
 void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
                       const float* pts, const uint8_t* rgb, const uint32_t* vis, void* vox, uint32_t* col,
                       uint32_t grid);
+void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
+                           const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
+                           float* grad_sdf);
+uint32_t gn_partial_count(int npix);
+// out: new pose (16 doubles), delta (6), cost, count, solved flag (25 doubles)
+void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int npix, float max_res,
+               double lambda, const double pose[16], double* partials, double* out);
 void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                    const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,
                    uint32_t* rc);

@@ -417,12 +417,14 @@ template <bool DIAG> struct BlockCache {
   int32_t x1, y1, z1, p1;
   uint8_t* mark_bkt;
   uint8_t* mark_blk;
-  __device__ void init(uint8_t* mb, uint8_t* mk) {
+  uint32_t* stats;  // march statistics (evaluations, super-block skips, block skips, replayed steps)
+  __device__ void init(uint8_t* mb, uint8_t* mk, uint32_t* st = nullptr) {
     x0 = y0 = z0 = INT32_MIN;
     x1 = y1 = z1 = INT32_MIN;
     p0 = p1 = -1;
     mark_bkt = mb;
     mark_blk = mk;
+    stats = st;
   }
   __device__ __forceinline__ int32_t get(const TableView& t, int32_t x, int32_t y, int32_t z) {
     if (x == x0 && y == y0 && z == z0) return p0;
@@ -560,6 +562,190 @@ __device__ __forceinline__ bool sb_occupied(const uint32_t* __restrict__ sb, int
   return (sb[b >> 5] >> (b & 31u)) & 1u;
 }
 
+// One ray of renderKernel (RayCastRender.cu:407-517) with the march of
+// traverseCoarseGridSimpleSampleAll (RayCastRender.cu:276-398).  Also returns the
+// bracket [a, b] at which the accepted crossing was refined (the adjoint restarts
+// there).
+struct RayHit {
+  bool marched;  // depth range valid (RayCastRender.cu:461-462)
+  bool found;
+  float alpha, d2r;
+  float cdx, cdy, cdz, wdx, wdy, wdz;
+  float a, ad, b, bd;
+  uint32_t rgb;
+};
+
+template <bool F16, bool DIAG, class Cache>
+__device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
+                                            const uint32_t* __restrict__ col, uint32_t x, uint32_t y, Cache& bc) {
+  const float MINF = 1e10f;
+  RayHit h;
+  h.marched = false;
+  h.found = false;
+  h.alpha = 0.0f;
+  h.a = h.ad = h.b = h.bd = 0.0f;
+  h.rgb = 0;
+  float cdx = ((float)x - ra.cx) / ra.fx, cdy = ((float)y - ra.cy) / ra.fy, cdz = 1.0f;
+  const float cl = sqrtf(cdx * cdx + cdy * cdy + cdz * cdz);
+  if (cl > 1e-6f) {
+    cdx /= cl;
+    cdy /= cl;
+    cdz /= cl;
+  }
+  const float* M = ra.M;
+  float wdx = M[0] * cdx + M[1] * cdy + M[2] * cdz;
+  float wdy = M[4] * cdx + M[5] * cdy + M[6] * cdz;
+  float wdz = M[8] * cdx + M[9] * cdy + M[10] * cdz;
+  const float wl = sqrtf(wdx * wdx + wdy * wdy + wdz * wdz);
+  if (wl > 1e-6f) {
+    wdx /= wl;
+    wdy /= wl;
+    wdz /= wl;
+  }
+  h.cdx = cdx;
+  h.cdy = cdy;
+  h.cdz = cdz;
+  h.wdx = wdx;
+  h.wdy = wdy;
+  h.wdz = wdz;
+  h.d2r = 1.0f / cdz;
+  const float minD = ra.minD, maxD = ra.maxD;
+  if (minD == 0.0f || minD == MINF || maxD == 0.0f || maxD == MINF) return h;
+  h.marched = true;
+  const float v = ra.voxel;
+  const float camx = ra.cam[0], camy = ra.cam[1], camz = ra.cam[2];
+  const float d2r = h.d2r;
+  float ray = d2r * fmaxf(minD, minD);
+  const float ray_end = d2r * fminf(maxD, maxD);
+  const float th = v * 3.0f;
+  const float inc = v * 0.5f;
+  float last_sdf = 0.0f, last_alpha = 0.0f;
+  uint32_t last_w = 0;
+  RaySkip sk;
+  sk.camx = camx;
+  sk.camy = camy;
+  sk.camz = camz;
+  sk.ix = wdx != 0.0f ? 1.0f / wdx : INFINITY;
+  sk.iy = wdy != 0.0f ? 1.0f / wdy : INFINITY;
+  sk.iz = wdz != 0.0f ? 1.0f / wdz : INFINITY;
+  sk.v = v;
+  sk.inv_v = 1.0f / v;
+  {
+    const float mag = fmaxf(fmaxf(fabsf(camx), fabsf(camy)), fabsf(camz)) + fabsf(ray_end);
+    sk.eps = 0.02f + 8e-6f * mag * sk.inv_v;
+  }
+  // Each lane first takes every skip its super-block test allows, then the wave
+  // evaluates one sample per lane together (lanes stay in phase for the expensive part).
+  bool try_super = true, stalled = false;
+  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0;
+  while (true) {
+    while (try_super && ray < ray_end) {
+      // the corner support (q-2, q+1] of the sample at ray spans 1-8 super-blocks of
+      // 32 voxels; if none holds an allocated block, every sample whose support stays
+      // inside them is invalid
+      const float qx = (camx + ray * wdx) * sk.inv_v, qy = (camy + ray * wdy) * sk.inv_v,
+                  qz = (camz + ray * wdz) * sk.inv_v;
+      const float m = 2.0f + 0.25f;
+      const int32_t sx0 = (int32_t)floorf((qx - m) * (1.0f / 32.0f)), sx1 = (int32_t)floorf((qx + m) * (1.0f / 32.0f));
+      const int32_t sy0 = (int32_t)floorf((qy - m) * (1.0f / 32.0f)), sy1 = (int32_t)floorf((qy + m) * (1.0f / 32.0f));
+      const int32_t sz0 = (int32_t)floorf((qz - m) * (1.0f / 32.0f)), sz1 = (int32_t)floorf((qz + m) * (1.0f / 32.0f));
+      bool empty = (sx1 - sx0) <= 1 && (sy1 - sy0) <= 1 && (sz1 - sz0) <= 1;
+      for (int32_t a = sx0; empty && a <= sx1; ++a)
+        for (int32_t b = sy0; empty && b <= sy1; ++b)
+          for (int32_t c = sz0; empty && c <= sz1; ++c) empty = !sb_occupied(t.sb, a, b, c);
+      if (!empty) break;
+      float t0, t1;
+      box_interval(sk, 32.0f * sx0 + 2.0f, 32.0f * sx1 + 30.0f, 32.0f * sy0 + 2.0f, 32.0f * sy1 + 30.0f,
+                   32.0f * sz0 + 2.0f, 32.0f * sz1 + 30.0f, &t0, &t1);
+      if (!(t0 <= ray && ray < t1)) break;
+      last_w = 0u;
+      n_super++;
+      while (ray < t1 && ray < ray_end) {
+        const float next = ray + inc;
+        if (next == ray) { stalled = true; break; }
+        ray = next;
+        n_replay++;
+      }
+      if (stalled) break;
+    }
+    if (stalled || !(ray < ray_end)) break;
+    const float ppx = camx + ray * wdx, ppy = camy + ray * wdy, ppz = camz + ray * wdz;
+    float dist;
+    CornerBlocks cb;
+    const int stat = trilinear<F16, false>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr, &cb);
+    n_eval++;
+    if (stat == kSampleValid) {
+      try_super = false;
+      if (last_w > 0u && last_sdf < 0.0f && dist > 0.0f) {
+        // findIntersectionBisection (RayCastRender.cu:229-271)
+        float a = last_alpha, ad = last_sdf, bb = ray, bd = dist, c = 0.0f;
+        uint32_t rgb = 0;
+        bool ok = true;
+        for (int it = 0; it < 3; ++it) {
+          c = a + (ad / (ad - bd)) * (bb - a);
+          float cd;
+          CornerBlocks cb2;
+          if (trilinear<F16, true>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd, &rgb,
+                                   &cb2) != kSampleValid) {
+            ok = false;
+            break;
+          }
+          if (ad * cd > 0.0f) {
+            a = c;
+            ad = cd;
+          } else {
+            bb = c;
+            bd = cd;
+          }
+        }
+        if (ok && fabsf(last_sdf - dist) < th && fabsf(dist) < th) {
+          h.found = true;
+          h.alpha = c;
+          h.a = last_alpha;
+          h.ad = last_sdf;
+          h.b = ray;
+          h.bd = dist;
+          h.rgb = rgb;
+          break;
+        }
+      }
+      last_sdf = dist;
+      last_alpha = ray;
+      last_w = 1u;
+    } else {
+      last_w = 0u;
+    }
+    const float next = ray + inc;
+    if (next == ray) break;  // step absorbed by rounding: the reference would never end
+    ray = next;
+    if (stat == kSampleEmpty) {
+      // every corner block of this sample is unallocated: skip the samples whose
+      // corners stay inside those blocks
+      try_super = true;
+      float t0, t1;
+      box_interval(sk, 8.0f * cb.x0 + 2.0f, 8.0f * cb.x1 + 6.0f, 8.0f * cb.y0 + 2.0f, 8.0f * cb.y1 + 6.0f,
+                   8.0f * cb.z0 + 2.0f, 8.0f * cb.z1 + 6.0f, &t0, &t1);
+      if (t0 <= ray) {
+        n_block++;
+        while (ray < t1 && ray < ray_end) {
+          const float nx = ray + inc;
+          if (nx == ray) { stalled = true; break; }
+          ray = nx;
+          n_replay++;
+        }
+        if (stalled) break;
+      }
+    }
+  }
+  if (DIAG && bc.stats) {
+    atomicAdd(&bc.stats[0], n_eval);
+    atomicAdd(&bc.stats[1], n_super);
+    atomicAdd(&bc.stats[2], n_block);
+    atomicAdd(&bc.stats[3], n_replay);
+  }
+  return h;
+}
+
 template <bool F16, bool DIAG>
 __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                const void* __restrict__ vox, const uint32_t* __restrict__ col,
@@ -573,141 +759,20 @@ __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __re
   if (x < (uint32_t)ra.W && y < (uint32_t)ra.H) {
     const float MINF = 1e10f;
     const int pix = (int)y * ra.W + (int)x;
+    BlockCache<DIAG> bc;
+    bc.init(ra.mark_bkt, ra.mark_blk, ra.march_stats);
+    const RayHit h = march_ray<F16, DIAG>(t, ra, vox, col, x, y, bc);
     float o_depth = MINF;
     float4 o_pos = make_float4(MINF, MINF, MINF, MINF), o_col = o_pos;
-    float cdx = ((float)x - ra.cx) / ra.fx, cdy = ((float)y - ra.cy) / ra.fy, cdz = 1.0f;
-    const float cl = sqrtf(cdx * cdx + cdy * cdy + cdz * cdz);
-    if (cl > 1e-6f) {
-      cdx /= cl;
-      cdy /= cl;
-      cdz /= cl;
+    if (h.found) {
+      const float alpha = h.alpha;
+      o_depth = alpha / h.d2r;
+      o_pos = make_float4(ra.cam[0] + alpha * h.wdx, ra.cam[1] + alpha * h.wdy, ra.cam[2] + alpha * h.wdz, 1.0f);
+      o_col = make_float4((float)(h.rgb & 0xFFu) / 255.0f, (float)((h.rgb >> 8) & 0xFFu) / 255.0f,
+                          (float)((h.rgb >> 16) & 0xFFu) / 255.0f, 1.0f);
     }
-    const float* M = ra.M;
-    float wdx = M[0] * cdx + M[1] * cdy + M[2] * cdz;
-    float wdy = M[4] * cdx + M[5] * cdy + M[6] * cdz;
-    float wdz = M[8] * cdx + M[9] * cdy + M[10] * cdz;
-    const float wl = sqrtf(wdx * wdx + wdy * wdy + wdz * wdz);
-    if (wl > 1e-6f) {
-      wdx /= wl;
-      wdy /= wl;
-      wdz /= wl;
-    }
-    const float minD = ra.minD, maxD = ra.maxD;
-    if (!(minD == 0.0f || minD == MINF) && !(maxD == 0.0f || maxD == MINF)) {
-      const float v = ra.voxel;
-      const float camx = ra.cam[0], camy = ra.cam[1], camz = ra.cam[2];
-      const float d2r = 1.0f / cdz;
-      float ray = d2r * fmaxf(minD, minD);
-      const float ray_end = d2r * fminf(maxD, maxD);
-      const float th = v * 3.0f;
-      const float inc = v * 0.5f;
-      float last_sdf = 0.0f, last_alpha = 0.0f;
-      uint32_t last_w = 0;
-      BlockCache<DIAG> bc;
-      bc.init(ra.mark_bkt, ra.mark_blk);
-      RaySkip sk;
-      sk.camx = camx;
-      sk.camy = camy;
-      sk.camz = camz;
-      sk.ix = wdx != 0.0f ? 1.0f / wdx : INFINITY;
-      sk.iy = wdy != 0.0f ? 1.0f / wdy : INFINITY;
-      sk.iz = wdz != 0.0f ? 1.0f / wdz : INFINITY;
-      sk.v = v;
-      sk.inv_v = 1.0f / v;
-      {
-        const float mag = fmaxf(fmaxf(fabsf(camx), fabsf(camy)), fabsf(camz)) + fabsf(ray_end);
-        sk.eps = 0.02f + 8e-6f * mag * sk.inv_v;
-      }
-      bool found = false, try_super = true, stalled = false;
-      while (ray < ray_end) {
-        const float ppx = camx + ray * wdx, ppy = camy + ray * wdy, ppz = camz + ray * wdz;
-        if (try_super) {
-          // skip the interior of an unoccupied super-block without evaluating samples
-          const float qx = ppx * sk.inv_v, qy = ppy * sk.inv_v, qz = ppz * sk.inv_v;
-          const int32_t sx = (int32_t)floorf(qx * (1.0f / 32.0f)), sy = (int32_t)floorf(qy * (1.0f / 32.0f)),
-                        sz = (int32_t)floorf(qz * (1.0f / 32.0f));
-          if (!sb_occupied(t.sb, sx, sy, sz)) {
-            float t0, t1;
-            box_interval(sk, 32.0f * sx + 2.0f, 32.0f * sx + 30.0f, 32.0f * sy + 2.0f, 32.0f * sy + 30.0f,
-                         32.0f * sz + 2.0f, 32.0f * sz + 30.0f, &t0, &t1);
-            if (t0 <= ray && ray < t1) {
-              last_w = 0u;
-              while (ray < t1 && ray < ray_end) {
-                const float next = ray + inc;
-                if (next == ray) { stalled = true; break; }
-                ray = next;
-              }
-              if (stalled) break;
-              continue;
-            }
-          }
-        }
-        float dist;
-        CornerBlocks cb;
-        const int stat = trilinear<F16, false>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr, &cb);
-        if (stat == kSampleValid) {
-          try_super = false;
-          if (last_w > 0u && last_sdf < 0.0f && dist > 0.0f) {
-            // findIntersectionBisection (RayCastRender.cu:229-271)
-            float a = last_alpha, ad = last_sdf, bb = ray, bd = dist, c = 0.0f;
-            uint32_t rgb = 0;
-            bool ok = true;
-            for (int it = 0; it < 3; ++it) {
-              c = a + (ad / (ad - bd)) * (bb - a);
-              float cd;
-              CornerBlocks cb2;
-              if (trilinear<F16, true>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd, &rgb,
-                                       &cb2) != kSampleValid) {
-                ok = false;
-                break;
-              }
-              if (ad * cd > 0.0f) {
-                a = c;
-                ad = cd;
-              } else {
-                bb = c;
-                bd = cd;
-              }
-            }
-            if (ok && fabsf(last_sdf - dist) < th && fabsf(dist) < th) {
-              const float alpha = c;
-              o_depth = alpha / d2r;
-              o_pos = make_float4(camx + alpha * wdx, camy + alpha * wdy, camz + alpha * wdz, 1.0f);
-              o_col = make_float4((float)(rgb & 0xFFu) / 255.0f, (float)((rgb >> 8) & 0xFFu) / 255.0f,
-                                  (float)((rgb >> 16) & 0xFFu) / 255.0f, 1.0f);
-              found = true;
-              break;
-            }
-          }
-          last_sdf = dist;
-          last_alpha = ray;
-          last_w = 1u;
-        } else {
-          last_w = 0u;
-        }
-        const float next = ray + inc;
-        if (next == ray) break;  // step absorbed by rounding: the reference would never end
-        ray = next;
-        if (stat == kSampleEmpty) {
-          // every corner block of this sample is unallocated: skip the samples whose
-          // corners stay inside those blocks
-          try_super = true;
-          float t0, t1;
-          box_interval(sk, 8.0f * cb.x0 + 2.0f, 8.0f * cb.x1 + 6.0f, 8.0f * cb.y0 + 2.0f, 8.0f * cb.y1 + 6.0f,
-                       8.0f * cb.z0 + 2.0f, 8.0f * cb.z1 + 6.0f, &t0, &t1);
-          if (t0 <= ray) {
-            while (ray < t1 && ray < ray_end) {
-              const float nx = ray + inc;
-              if (nx == ray) { stalled = true; break; }
-              ray = nx;
-            }
-            if (stalled) break;
-          }
-        }
-      }
-      hit = found ? 1u : 0u;
-      miss = found ? 0u : 1u;
-    }
+    hit = (h.marched && h.found) ? 1u : 0u;
+    miss = (h.marched && !h.found) ? 1u : 0u;
     out_depth[pix] = o_depth;
     out_pos[pix] = o_pos;
     out_col[pix] = o_col;
@@ -719,6 +784,341 @@ __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __re
     if (hit) atomicAdd(&rc[0], hit);
     if (miss) atomicAdd(&rc[1], miss);
   }
+}
+
+// ---------------------------------------------------------------------------------
+// Raycast adjoint (no counterpart in the reference)
+// ---------------------------------------------------------------------------------
+//
+// Rendered depth = alpha * cd.z with alpha the third regula-falsi point.  Recomputes
+// the march (the same code as k_render, so the same bracket), then differentiates the
+// computation the forward performed (fixed bracket choices and trilinear corner sets)
+// in forward mode with 10 components: a camera-frame SE(3) perturbation
+// T(xi) = T exp(xi^), xi = (omega, upsilon), under which a point at ray parameter r
+// moves by R (upsilon + omega x r cd) + wd dr, and one seed per trilinear evaluation
+// alpha depends on (the two bracket ends and the first two refinement points), whose
+// product with that evaluation's d f / d sdf_j = w_j / sum(w) gives d alpha / d sdf_j.
+
+struct TriGradF {
+  float g[3];        // gradient of the sampled field in world coordinates
+  int64_t vox[8];    // corner voxel indices (ptr*512 + voxel), -1 when not used
+  float dfds[8];     // d f / d sdf of each used corner
+};
+
+// Gradient part of trilinear() at p (whose value the forward computed); the corner
+// set and weights are recomputed exactly as the forward forms them.
+template <bool F16, class Cache>
+__device__ __forceinline__ void trilinear_grad(const TableView& t, Cache& bc, const void* __restrict__ vox_, float px,
+                                               float py, float pz, float v, float f, TriGradF* out) {
+  using S = Store<F16>;
+  const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
+  const float off = v;
+  const float pdx = px - off / 2.0f, pdy = py - off / 2.0f, pdz = pz - off / 2.0f;
+  const float qx0 = pdx / v, qy0 = pdy / v, qz0 = pdz / v;
+  const float fr[3] = {qx0 - floorf(qx0), qy0 - floorf(qy0), qz0 - floorf(qz0)};
+  const float qx1 = (pdx + off) / v, qy1 = (pdy + off) / v, qz1 = (pdz + off) / v;
+  const int32_t vx[2] = {ray_voxel_of_q(qx0), ray_voxel_of_q(qx1)};
+  const int32_t vy[2] = {ray_voxel_of_q(qy0), ray_voxel_of_q(qy1)};
+  const int32_t vz[2] = {ray_voxel_of_q(qz0), ray_voxel_of_q(qz1)};
+  const float iv = 1.0f / v;
+  float W = 0.0f, dW[3] = {0.0f, 0.0f, 0.0f}, dN[3] = {0.0f, 0.0f, 0.0f}, w[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    w[i] = 0.0f;
+    out->vox[i] = -1;
+    out->dfds[i] = 0.0f;
+    const int32_t ix = vx[i & 1], iy = vy[(i >> 1) & 1], iz = vz[(i >> 2) & 1];
+    const int32_t p = bc.get(t, voxel_to_block_axis(ix), voxel_to_block_axis(iy), voxel_to_block_axis(iz));
+    if (p < 0) continue;
+    const int64_t vi = (int64_t)p * kVox +
+                       (voxel_local_axis(ix) + voxel_local_axis(iy) * kBlock + voxel_local_axis(iz) * kBlock * kBlock);
+    const typename S::word wd = vox[vi];
+    if (S::weight(wd) == 0u) continue;
+    float phi[3], dphi[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      const bool hi = (i >> a) & 1;
+      phi[a] = hi ? fr[a] : 1.0f - fr[a];
+      dphi[a] = hi ? iv : -iv;
+    }
+    w[i] = phi[0] * phi[1] * phi[2];
+    const float sv = S::sdf(wd);
+    out->vox[i] = vi;
+    W += w[i];
+    const float dwi[3] = {dphi[0] * phi[1] * phi[2], phi[0] * dphi[1] * phi[2], phi[0] * phi[1] * dphi[2]};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      dW[a] += dwi[a];
+      dN[a] += dwi[a] * sv;
+    }
+  }
+  const float invW = W > 0.0f ? 1.0f / W : 0.0f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) out->g[a] = (dN[a] - f * dW[a]) * invW;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) out->dfds[i] = out->vox[i] >= 0 ? w[i] * invW : 0.0f;
+}
+
+struct Dual10 {
+  float v;
+  float d[10];
+};
+
+// Dual of an evaluation f(p(r)): r carries its derivative, tg the field gradient.
+__device__ __forceinline__ Dual10 eval_dual(const TriGradF& tg, float fval, const Dual10& r, const RayHit& h,
+                                            const float* M, int seed) {
+  const float q[3] = {r.v * h.cdx, r.v * h.cdy, r.v * h.cdz};
+  const float gc[3] = {M[0] * tg.g[0] + M[4] * tg.g[1] + M[8] * tg.g[2],
+                       M[1] * tg.g[0] + M[5] * tg.g[1] + M[9] * tg.g[2],
+                       M[2] * tg.g[0] + M[6] * tg.g[1] + M[10] * tg.g[2]};
+  const float gw = tg.g[0] * h.wdx + tg.g[1] * h.wdy + tg.g[2] * h.wdz;
+  Dual10 e;
+  e.v = fval;
+#pragma unroll
+  for (int k = 0; k < 10; ++k) e.d[k] = gw * r.d[k];
+  e.d[0] += q[1] * gc[2] - q[2] * gc[1];
+  e.d[1] += q[2] * gc[0] - q[0] * gc[2];
+  e.d[2] += q[0] * gc[1] - q[1] * gc[0];
+  e.d[3] += gc[0];
+  e.d[4] += gc[1];
+  e.d[5] += gc[2];
+  e.d[6 + seed] += 1.0f;
+  return e;
+}
+
+template <bool F16>
+__global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevState* __restrict__ st, RenderArgs ra,
+                                                       const void* __restrict__ vox, const uint32_t* __restrict__ col,
+                                                       float* __restrict__ jac, const float* __restrict__ grad_depth,
+                                                       float* __restrict__ grad_sdf) {
+  const uint32_t x = blockIdx.x * 8u + (threadIdx.x & 7u);
+  const uint32_t y = blockIdx.y * 8u + (threadIdx.x >> 3);
+  if (x >= (uint32_t)ra.W || y >= (uint32_t)ra.H) return;
+  const int pix = (int)y * ra.W + (int)x;
+  float J[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  if (st->n_alloc != 0u) {
+    BlockCache<false> bc;
+    bc.init(nullptr, nullptr);
+    const RayHit h = march_ray<F16, false>(t, ra, vox, col, x, y, bc);
+    if (h.found) {
+      const float v = ra.voxel;
+      const float camx = ra.cam[0], camy = ra.cam[1], camz = ra.cam[2];
+      auto px = [&](float r) { return camx + r * h.wdx; };
+      auto py = [&](float r) { return camy + r * h.wdy; };
+      auto pz = [&](float r) { return camz + r * h.wdz; };
+      TriGradF tg[4];
+      Dual10 a, b, ad, bd, c;
+#pragma unroll
+      for (int k = 0; k < 10; ++k) a.d[k] = b.d[k] = c.d[k] = 0.0f;
+      a.v = h.a;
+      b.v = h.b;
+      trilinear_grad<F16>(t, bc, vox, px(h.a), py(h.a), pz(h.a), v, h.ad, &tg[0]);
+      trilinear_grad<F16>(t, bc, vox, px(h.b), py(h.b), pz(h.b), v, h.bd, &tg[1]);
+      ad = eval_dual(tg[0], h.ad, a, h, ra.M, 0);
+      bd = eval_dual(tg[1], h.bd, b, h, ra.M, 1);
+      float fa = h.a, fad = h.ad, fb = h.b, fbd = h.bd;
+      for (int it = 0; it < 3; ++it) {
+        const float fc = fa + (fad / (fad - fbd)) * (fb - fa);  // the forward's point
+        const float den = ad.v - bd.v, u = ad.v / den, den2 = den * den;
+        c.v = fc;
+#pragma unroll
+        for (int k = 0; k < 10; ++k) {
+          const float du = (bd.d[k] * ad.v - ad.d[k] * bd.v) / den2;
+          c.d[k] = a.d[k] + du * (b.v - a.v) + u * (b.d[k] - a.d[k]);
+        }
+        if (it == 2) break;
+        float fcd;
+        uint32_t rgb;
+        CornerBlocks cb;
+        trilinear<F16, false>(t, bc, vox, col, px(fc), py(fc), pz(fc), v, &fcd, &rgb, &cb);
+        trilinear_grad<F16>(t, bc, vox, px(fc), py(fc), pz(fc), v, fcd, &tg[2 + it]);
+        const Dual10 cd = eval_dual(tg[2 + it], fcd, c, h, ra.M, 2 + it);
+        if (fad * fcd > 0.0f) {
+          fa = fc;
+          fad = fcd;
+          a = c;
+          ad = cd;
+        } else {
+          fb = fc;
+          fbd = fcd;
+          b = c;
+          bd = cd;
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 6; ++k) J[k] = c.d[k] * h.cdz;
+      if (grad_sdf && grad_depth) {
+        const float g = grad_depth[pix] * h.cdz;
+        if (g != 0.0f) {
+          for (int e = 0; e < 4; ++e) {
+            const float ge = g * c.d[6 + e];
+            for (int i = 0; i < 8; ++i)
+              if (tg[e].vox[i] >= 0) atomicAdd(&grad_sdf[tg[e].vox[i]], ge * tg[e].dfds[i]);
+          }
+        }
+      }
+    }
+  }
+  if (jac) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) jac[6 * (size_t)pix + k] = J[k];
+  }
+}
+
+// ---------------------------------------------------------------------------------
+// Gauss-Newton pose refinement, FP64 (no counterpart in the reference)
+// ---------------------------------------------------------------------------------
+
+constexpr int kGnTerms = 29;  // 21 (upper triangle of H) + 6 (g) + cost + count
+
+__device__ __forceinline__ double shfl_xor_d(double v, int o) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __shfl_xor(lo, o, 64);
+  hi = __shfl_xor(hi, o, 64);
+  return __hiloint2double(hi, lo);
+}
+
+// Per-workgroup partial sums of J J^T, J r, r^2 and the count over pixels with a
+// rendered and an observed depth and |r| < max_res (r = rendered - observed).
+__global__ __launch_bounds__(256) void k_gn_accum(const float* __restrict__ depth, const float* __restrict__ obs,
+                                                  const float* __restrict__ jac, int npix, float max_res,
+                                                  double* __restrict__ partials) {
+  const int pix = blockIdx.x * 256 + threadIdx.x;
+  double acc[kGnTerms];
+#pragma unroll
+  for (int k = 0; k < kGnTerms; ++k) acc[k] = 0.0;
+  if (pix < npix) {
+    const float d = depth[pix], o = obs[pix];
+    const float r = d - o;
+    if (d < 1e9f && o > 0.0f && fabsf(r) < max_res) {
+      double Jd[6];
+#pragma unroll
+      for (int k = 0; k < 6; ++k) Jd[k] = (double)jac[6 * (size_t)pix + k];
+      int n = 0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = i; j < 6; ++j) acc[n++] = Jd[i] * Jd[j];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc[21 + i] = Jd[i] * (double)r;
+      acc[27] = (double)r * (double)r;
+      acc[28] = 1.0;
+    }
+  }
+  __shared__ double ws[4][kGnTerms];
+#pragma unroll
+  for (int k = 0; k < kGnTerms; ++k) {
+    double v = acc[k];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += shfl_xor_d(v, o);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < kGnTerms)
+    partials[(size_t)blockIdx.x * kGnTerms + threadIdx.x] =
+        ws[0][threadIdx.x] + ws[1][threadIdx.x] + ws[2][threadIdx.x] + ws[3][threadIdx.x];
+}
+
+struct Pose64 {
+  double T[16];
+};
+
+// Sums the partials in a fixed order, solves (H + lambda diag(H)) delta = -g by
+// Cholesky and composes pose * exp(delta^), all in FP64.  out: new pose (16),
+// delta (6), cost, count, solved flag.
+__global__ __launch_bounds__(64) void k_gn_solve(const double* __restrict__ partials, int nwg, double lambda,
+                                                 Pose64 pose, double* __restrict__ out) {
+  __shared__ double sum[kGnTerms];
+  if (threadIdx.x < kGnTerms) {
+    double s = 0.0;
+    for (int b = 0; b < nwg; ++b) s += partials[(size_t)b * kGnTerms + threadIdx.x];
+    sum[threadIdx.x] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double H[36], g[6];
+  int n = 0;
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) {
+      H[6 * i + j] = H[6 * j + i] = sum[n++];
+    }
+  for (int i = 0; i < 6; ++i) g[i] = sum[21 + i];
+  for (int i = 0; i < 6; ++i) H[7 * i] += lambda * H[7 * i] + 1e-12;
+  // Cholesky H = L L^T
+  double L[36];
+  for (int k = 0; k < 36; ++k) L[k] = 0.0;
+  bool ok = sum[28] >= 6.0;
+  for (int j = 0; j < 6 && ok; ++j) {
+    double d = H[7 * j];
+    for (int k = 0; k < j; ++k) d -= L[6 * j + k] * L[6 * j + k];
+    if (!(d > 0.0)) {
+      ok = false;
+      break;
+    }
+    L[7 * j] = sqrt(d);
+    for (int i = j + 1; i < 6; ++i) {
+      double s = H[6 * i + j];
+      for (int k = 0; k < j; ++k) s -= L[6 * i + k] * L[6 * j + k];
+      L[6 * i + j] = s / L[7 * j];
+    }
+  }
+  double delta[6] = {0, 0, 0, 0, 0, 0};
+  if (ok) {
+    double z[6];
+    for (int i = 0; i < 6; ++i) {
+      double s = -g[i];
+      for (int k = 0; k < i; ++k) s -= L[6 * i + k] * z[k];
+      z[i] = s / L[7 * i];
+    }
+    for (int i = 5; i >= 0; --i) {
+      double s = z[i];
+      for (int k = i + 1; k < 6; ++k) s -= L[6 * k + i] * delta[k];
+      delta[i] = s / L[7 * i];
+    }
+  }
+  // exp(delta^): rotation by Rodrigues, translation V upsilon
+  const double wx = delta[0], wy = delta[1], wz = delta[2];
+  const double th = sqrt(wx * wx + wy * wy + wz * wz);
+  double A, Bc, Cc;
+  if (th < 1e-8) {
+    A = 1.0 - th * th / 6.0;
+    Bc = 0.5 - th * th / 24.0;
+    Cc = 1.0 / 6.0 - th * th / 120.0;
+  } else {
+    A = sin(th) / th;
+    Bc = (1.0 - cos(th)) / (th * th);
+    Cc = (th - sin(th)) / (th * th * th);
+  }
+  const double K[9] = {0.0, -wz, wy, wz, 0.0, -wx, -wy, wx, 0.0};
+  double K2[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double s = 0.0;
+      for (int k = 0; k < 3; ++k) s += K[3 * i + k] * K[3 * k + j];
+      K2[3 * i + j] = s;
+    }
+  double Rd[9], V[9];
+  for (int i = 0; i < 9; ++i) {
+    const double I = (i % 4 == 0) ? 1.0 : 0.0;
+    Rd[i] = I + A * K[i] + Bc * K2[i];
+    V[i] = I + Bc * K[i] + Cc * K2[i];
+  }
+  double td[3];
+  for (int i = 0; i < 3; ++i) td[i] = V[3 * i] * delta[3] + V[3 * i + 1] * delta[4] + V[3 * i + 2] * delta[5];
+  // pose * [Rd td; 0 1]
+  double Tn[16];
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j)
+      Tn[4 * i + j] = pose.T[4 * i] * Rd[j] + pose.T[4 * i + 1] * Rd[3 + j] + pose.T[4 * i + 2] * Rd[6 + j];
+    Tn[4 * i + 3] = pose.T[4 * i] * td[0] + pose.T[4 * i + 1] * td[1] + pose.T[4 * i + 2] * td[2] + pose.T[4 * i + 3];
+  }
+  Tn[12] = Tn[13] = Tn[14] = 0.0;
+  Tn[15] = 1.0;
+  for (int i = 0; i < 16; ++i) out[i] = Tn[i];
+  for (int i = 0; i < 6; ++i) out[16 + i] = delta[i];
+  out[22] = sum[27];
+  out[23] = sum[28];
+  out[24] = ok ? 1.0 : 0.0;
 }
 
 // ---------------------------------------------------------------------------------
@@ -811,6 +1211,27 @@ void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st
     hipLaunchKernelGGL(k_integrate<true>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col);
   else
     hipLaunchKernelGGL(k_integrate<false>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col);
+}
+
+void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
+                           const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
+                           float* grad_sdf) {
+  const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
+  if (fp16)
+    hipLaunchKernelGGL(k_render_adjoint<true>, grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf);
+  else
+    hipLaunchKernelGGL(k_render_adjoint<false>, grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf);
+}
+
+uint32_t gn_partial_count(int npix) { return (uint32_t)((npix + 255) / 256); }
+
+void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int npix, float max_res,
+               double lambda, const double pose[16], double* partials, double* out) {
+  const uint32_t nwg = gn_partial_count(npix);
+  hipLaunchKernelGGL(k_gn_accum, dim3(nwg), dim3(256), 0, s, depth, obs, jac, npix, max_res, partials);
+  Pose64 p;
+  for (int i = 0; i < 16; ++i) p.T[i] = pose[i];
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, partials, (int)nwg, lambda, p, out);
 }
 
 void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,

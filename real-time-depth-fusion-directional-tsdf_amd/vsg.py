@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "NOT_READY"}
 STORE_FP32 = 0
 STORE_FP16 = 1
 KERNELS = ["alloc_claim", "alloc_resolve", "alloc_scan", "alloc_insert", "compact", "integrate", "raycast",
-           "reset"]
+           "reset", "adjoint", "gauss_newton"]
 
 SDF_BLOCK_SIZE = 8      # VoxelSG/include/globalPramasConfig.h:8
 HASH_BUCKET_SIZE = 10   # VoxelSG/include/globalPramasConfig.h:9
@@ -63,7 +63,7 @@ class vsg_counters(C.Structure):
 
 class vsg_diag(C.Structure):
     _fields_ = [("alloc_buckets", C.c_uint32), ("ray_buckets", C.c_uint32), ("any_buckets", C.c_uint32),
-                ("ray_blocks", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("ray_blocks", C.c_uint32), ("march", C.c_uint32 * 4)]
 
 
 # Every symbol include/vsg.h declares, with its ctypes signature.
@@ -91,6 +91,10 @@ _SIGNATURES = {
     "vsg_render_buffers": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4),
     "vsg_render_download": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "vsg_render_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vsg_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vsg_pose_gn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
+                                   C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "vsg_import_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_diag_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsg_diag_read": (C.c_int, [C.c_void_p, C.POINTER(vsg_diag), C.c_int32]),
     "vsg_timing_enable": (C.c_int, [C.c_int32]),
@@ -295,13 +299,24 @@ class VoxelScene:
         _check(lib().vsg_scene_counters(self._h, C.byref(c)))
         return {name: getattr(c, name) for name, _ in vsg_counters._fields_ if name != "reserved"}
 
+    def import_voxels(self, first: int, sdf: np.ndarray, weight: np.ndarray, rgbf: np.ndarray | None = None) -> None:
+        """Overwrites the voxels of blocks [first, first + len(sdf)/512)."""
+        sdf = np.ascontiguousarray(sdf, np.float32)
+        weight = np.ascontiguousarray(weight, np.uint16)
+        nb = sdf.size // 512
+        r = None if rgbf is None else np.ascontiguousarray(rgbf, np.uint8)
+        _check(lib().vsg_import_voxels(self._h, first, nb, sdf.ctypes.data, weight.ctypes.data,
+                                       None if r is None else r.ctypes.data))
+
     def diag_enable(self, on: bool = True) -> None:
         _check(lib().vsg_diag_enable(self._h, 1 if on else 0))
 
     def diag_read(self, clear: bool = True) -> dict:
         d = vsg_diag()
         _check(lib().vsg_diag_read(self._h, C.byref(d), 1 if clear else 0))
-        return {name: getattr(d, name) for name, _ in vsg_diag._fields_ if name != "reserved"}
+        out = {name: getattr(d, name) for name, _ in vsg_diag._fields_ if name != "march"}
+        out.update(zip(("march_evals", "march_super_skips", "march_block_skips", "march_replayed"), d.march))
+        return out
 
     def export_blocks(self):
         """(coords int32 [n,3], hash entry indices int32 [n]) in pointer order."""
@@ -430,6 +445,23 @@ class RayCastRender:
             for p, c in zip(pts, cols):
                 fh.write(f"{p[0]} {p[1]} {p[2]} {c[0]} {c[1]} {c[2]}\n")
         return True
+
+    def adjoint(self, voxelScene: VoxelScene, jac=None, grad_depth=None, grad_sdf=None) -> None:
+        """Adjoint of the last render: jac (device float[6*W*H]) receives d depth / d xi for
+        pose * exp(xi^); grad_sdf (device float[N*512]) accumulates grad_depth^T d depth / d sdf."""
+        _check(lib().vsg_render_adjoint(self._h, voxelScene.handle, _dev_ptr(jac), _dev_ptr(grad_depth),
+                                        _dev_ptr(grad_sdf)))
+
+    def gn_step(self, voxelScene: VoxelScene, observed_depth, fx, fy, cx, cy, minDepth, maxDepth, pose,
+                max_residual: float = 0.05, lam: float = 1e-4):
+        """One FP64 Gauss-Newton step on the device; returns (new pose float64 4x4, stats dict)."""
+        P = (C.c_double * 16)(*np.asarray(pose, np.float64).reshape(16).tolist())
+        st = (C.c_double * 8)()
+        k = vsg_intrinsics(fx, fy, cx, cy)
+        _check(lib().vsg_pose_gn_step(self._h, voxelScene.handle, _dev_ptr(observed_depth), C.byref(k),
+                                      float(minDepth), float(maxDepth), float(max_residual), float(lam), P, st))
+        return (np.array(P[:], np.float64).reshape(4, 4),
+                {"cost": st[0], "count": int(st[1]), "solved": bool(st[2]), "rot": st[3], "trans": st[4]})
 
     def counters(self):
         h, m = C.c_uint32(), C.c_uint32()

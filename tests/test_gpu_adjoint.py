@@ -1,0 +1,224 @@
+"""Raycast adjoint and pose refinement on the GPU (through the C ABI).
+
+d depth / d pose (camera-frame perturbation pose * exp(xi^)) and d depth / d voxel
+SDF are checked against the oracle's adjoint (same float forward, FP64 derivative
+arithmetic: the CPU reference gradient), against central finite differences
+(h = 1e-4) of the GPU's own renders, and through a torch.autograd.Function; FP64
+Gauss-Newton on the device recovers perturbed poses of held-out frames.
+
+The rendered depth is piecewise smooth: the sampled field jumps at half-voxel
+planes, so finite differences disagree with the derivative on the few pixels whose
+samples cross such a plane within +-h; criteria are fractions of the checked pixels.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from fusion_util import GpuRun, OracleRun
+from vsg_amd import synth, vsg
+
+pytestmark = pytest.mark.gpu
+
+
+def _fused(torch, workload=0, frames=(0, 3, 6, 9)):
+    wl = synth.WORKLOADS[workload]
+    sc = synth.SynthScene(workload)
+    g, o = GpuRun(torch, wl), OracleRun(wl)
+    for i in frames:
+        f = sc.render(i)
+        g.integrate(f, sc.intrinsics)
+        o.integrate(f, sc.intrinsics)
+    return wl, sc, g, o, f
+
+
+def _render_depth(g, sc, T, mn, mx):
+    fx, fy, cx, cy = sc.intrinsics
+    T = np.asarray(T, np.float64)
+    if g.render is None:
+        g.render = vsg.RayCastRender()
+        assert g.render.initialize(sc.W, sc.H)
+    g.render.render(g.scene, T[:3, 3].astype(np.float32), T.astype(np.float32), fx, fy, cx, cy, mn, mx)
+    return g.render.downloadDepthFloat()
+
+
+def _jac(torch, g, sc):
+    n = sc.W * sc.H
+    jac = torch.zeros(6 * n, dtype=torch.float32, device="cuda")
+    g.render.adjoint(g.scene, jac=jac)
+    torch.cuda.synchronize()
+    return jac.view(n, 6).cpu().numpy().astype(np.float64)
+
+
+def test_jacobian_matches_oracle_adjoint(torch_cuda):
+    wl, sc, g, o, f = _fused(torch_cuda)
+    T = f["pose"].astype(np.float64)
+    mn, mx = synth.depth_range(f["depth"])
+    d = _render_depth(g, sc, T, mn, mx)
+    J = _jac(torch_cuda, g, sc)
+    rng = np.random.default_rng(6)
+    pix = rng.choice(sc.W * sc.H, 4096, replace=False).astype(np.int32)
+    Jo, hit, _ = o.o.render_adjoint(sc.W, sc.H, T[:3, 3], T, sc.intrinsics, mn, mx, pix)
+    assert np.array_equal(hit, d[pix] < 1e9)
+    Jg = J[pix][hit]
+    Jo = Jo[hit]
+    err = np.abs(Jg - Jo).max(axis=1)
+    scale = np.abs(Jo).max(axis=1)
+    good = err <= 2e-3 * scale + 1e-5
+    print(f"jacobian vs oracle: {good.mean():.4f} of {hit.sum()} hit pixels")
+    assert good.mean() > 0.99
+
+
+def test_jacobian_central_differences(torch_cuda):
+    wl, sc, g, o, f = _fused(torch_cuda)
+    T = f["pose"].astype(np.float64)
+    mn, mx = synth.depth_range(f["depth"])
+    d0 = _render_depth(g, sc, T, mn, mx)
+    J = _jac(torch_cuda, g, sc)
+    rng = np.random.default_rng(8)
+    pix = rng.choice(np.flatnonzero(d0 < 1e9), 4096, replace=False)
+    h = 1e-4
+    fd = np.zeros((pix.size, 6))
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = h
+        dp = _render_depth(g, sc, T @ O.se3_exp(e), mn, mx)[pix].astype(np.float64)
+        dm = _render_depth(g, sc, T @ O.se3_exp(-e), mn, mx)[pix].astype(np.float64)
+        fd[:, k] = (dp - dm) / (2 * h)
+    ok = np.all(np.abs(fd) < 1e3, axis=1)
+    err = np.abs(J[pix][ok] - fd[ok]).max(axis=1)
+    scale = np.abs(fd[ok]).max(axis=1)
+    good = err <= 0.02 * scale + 5e-3
+    print(f"jacobian vs central differences: {good.mean():.4f} of {ok.sum()}")
+    assert ok.mean() > 0.9 and good.mean() > 0.9
+
+
+def test_sdf_gradient_matches_oracle(torch_cuda):
+    wl, sc, g, o, f = _fused(torch_cuda)
+    T = f["pose"].astype(np.float64)
+    mn, mx = synth.depth_range(f["depth"])
+    _render_depth(g, sc, T, mn, mx)
+    n = sc.W * sc.H
+    rng = np.random.default_rng(9)
+    pix = rng.choice(n, 4096, replace=False).astype(np.int32)
+    w = rng.uniform(-1, 1, pix.size)
+    gd = np.zeros(n, np.float32)
+    gd[pix] = w
+    nb = g.scene.counters()["num_allocated"]
+    gsdf = torch_cuda.zeros(wl.blocks * 512, dtype=torch_cuda.float32, device="cuda")
+    g.render.adjoint(g.scene, grad_depth=torch_cuda.from_numpy(gd).cuda(), grad_sdf=gsdf)
+    torch_cuda.cuda.synchronize()
+    got = gsdf.cpu().numpy()[:nb * 512].astype(np.float64)
+    _, _, ref = o.o.render_adjoint(sc.W, sc.H, T[:3, 3], T, sc.intrinsics, mn, mx, pix, grad_depth=w)
+    nz = np.abs(ref) > 1e-6
+    assert nz.sum() > 1000
+    rel = np.abs(got[nz] - ref[nz]) / (np.abs(ref[nz]) + 1e-3)
+    assert np.mean(rel < 1e-2) > 0.99
+    assert np.abs(got[~nz]).max() < 1e-3
+
+
+def test_sdf_gradient_central_differences(torch_cuda):
+    """Perturb single voxel SDFs in the store (import) and difference the GPU render."""
+    wl, sc, g, o, f = _fused(torch_cuda)
+    T = f["pose"].astype(np.float64)
+    mn, mx = synth.depth_range(f["depth"])
+    d0 = _render_depth(g, sc, T, mn, mx)
+    n = sc.W * sc.H
+    rng = np.random.default_rng(10)
+    checked = agree = 0
+    for p in rng.choice(np.flatnonzero(d0 < 1e9), 12, replace=False):
+        gd = np.zeros(n, np.float32)
+        gd[p] = 1.0
+        gsdf = torch_cuda.zeros(wl.blocks * 512, dtype=torch_cuda.float32, device="cuda")
+        _render_depth(g, sc, T, mn, mx)
+        g.render.adjoint(g.scene, grad_depth=torch_cuda.from_numpy(gd).cuda(), grad_sdf=gsdf)
+        grad = gsdf.cpu().numpy()
+        idx = np.flatnonzero(grad)
+        for vi in idx[np.argsort(-np.abs(grad[idx]))[:2]]:
+            blk = int(vi) // 512
+            sdf, w, rgbf = g.scene.export_voxels(blk, 1)
+            h = 2e-4
+            vals = []
+            for sgn in (1, -1):
+                s2 = sdf.copy()
+                s2[vi % 512] += sgn * h
+                g.scene.import_voxels(blk, s2, w, rgbf)
+                vals.append(float(_render_depth(g, sc, T, mn, mx)[p]))
+            g.scene.import_voxels(blk, sdf, w, rgbf)
+            fd = (vals[0] - vals[1]) / (2 * h)
+            checked += 1
+            agree += abs(fd - grad[vi]) <= 0.05 * abs(grad[vi]) + 2e-3
+    assert checked >= 20 and agree / checked >= 0.85, (agree, checked)
+
+
+def test_autograd_function(torch_cuda):
+    torch = torch_cuda
+    wl, sc, g, o, f = _fused(torch)
+    T0 = f["pose"].astype(np.float64)
+    mn, mx = synth.depth_range(f["depth"])
+    fx, fy, cx, cy = sc.intrinsics
+    n = sc.W * sc.H
+
+    class RaycastDepth(torch.autograd.Function):
+        """depth(xi) = render at T0 * exp(xi^); backward through the raycast adjoint."""
+
+        @staticmethod
+        def forward(ctx, xi):
+            T = T0 @ O.se3_exp(xi.detach().cpu().numpy())
+            d = _render_depth(g, sc, T, mn, mx)
+            return torch.from_numpy(d).cuda()
+
+        @staticmethod
+        def backward(ctx, gout):
+            jac = torch.zeros(6 * n, dtype=torch.float32, device="cuda")
+            g.render.adjoint(g.scene, jac=jac)
+            return (jac.view(n, 6).double() * gout.double().view(n, 1)).sum(0).to(gout.device).cpu()
+
+    xi = torch.zeros(6, dtype=torch.float64, requires_grad=True)
+    depth = RaycastDepth.apply(xi)
+    valid = depth < 1e9
+    w = torch.linspace(-1.0, 1.0, n, device="cuda") * valid
+    loss = (w * torch.where(valid, depth, torch.zeros_like(depth))).sum()
+    loss.backward()
+    J = _jac(torch, g, sc)
+    expect = (J * w.cpu().numpy()[:, None]).sum(0)
+    assert np.allclose(xi.grad.numpy(), expect, rtol=1e-5, atol=1e-6)
+    assert np.abs(expect).max() > 0
+
+
+def test_pose_refinement_recovers_perturbed_poses(torch_cuda):
+    """Room 640x480: 32 frames fused at their poses; 8 held-out frames perturbed by up
+    to 2 deg / 2 cm are refined by <= 10 FP64 Gauss-Newton steps on the device."""
+    torch = torch_cuda
+    wl = synth.WORKLOADS[1]
+    sc = synth.SynthScene(1)
+    g = GpuRun(torch, wl)
+    for i in range(32):
+        g.integrate(sc.render(i), sc.intrinsics)
+    g.render = vsg.RayCastRender()
+    assert g.render.initialize(sc.W, sc.H)
+    fx, fy, cx, cy = sc.intrinsics
+    rng = np.random.default_rng(5)
+    rot_err, trans_err = [], []
+    for i in range(34, 42):
+        f = sc.render(i)
+        Tgt = sc.pose(i)
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        ang = np.radians(rng.uniform(0, 2.0))
+        tdir = rng.normal(size=3)
+        tdir /= np.linalg.norm(tdir)
+        xi = np.concatenate([axis * ang, tdir * rng.uniform(0, 0.02)])
+        T = Tgt @ O.se3_exp(xi)
+        obs = torch.from_numpy(f["depth"].reshape(-1)).cuda()
+        mn, mx = synth.depth_range(f["depth"])
+        for it in range(10):
+            res = 0.05 if it < 3 else 0.02
+            T, st = g.render.gn_step(g.scene, obs, fx, fy, cx, cy, mn, mx, T, max_residual=res, lam=1e-3)
+            if st["rot"] < 1e-6 and st["trans"] < 1e-6:
+                break
+        E = np.linalg.inv(Tgt) @ T
+        rot_err.append(np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1))))
+        trans_err.append(np.linalg.norm(E[:3, 3]))
+    rot_err, trans_err = np.array(rot_err), np.array(trans_err)
+    print(f"pose refinement: rot deg {np.round(rot_err, 4)}, trans mm {np.round(trans_err * 1e3, 3)}")
+    assert np.median(rot_err) <= 0.1 and np.median(trans_err) <= 1e-3
