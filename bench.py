@@ -97,6 +97,68 @@ def cpu_baseline(sc, frames: list[dict], wl, nframes: int, stride: int) -> dict:
             "integrate_ms": round(t_int / nframes * 1e3, 2), "raycast_ms": round(t_ray * scale / nframes * 1e3, 2)}
 
 
+def error_stats(err: np.ndarray) -> dict:
+    err = np.abs(np.asarray(err, np.float64))
+    if err.size == 0:
+        return {"n": 0}
+    return {"n": int(err.size), "max_mm": round(float(err.max()) * 1e3, 5),
+            "rms_mm": round(float(np.sqrt(np.mean(err ** 2))) * 1e3, 5),
+            "p999_mm": round(float(np.quantile(err, 0.999)) * 1e3, 5)}
+
+
+def precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm, nframes: int, npix_check: int) -> dict:
+    """SDF and raycast-depth error of the FP32 and FP16 stores against the FP64 oracle on
+    the first `nframes` frames: SDF on voxels updated in both (matched by block
+    coordinate), depth of the last frame's raycast on `npix_check` seeded pixels valid
+    in both."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    W, H = sc.W, sc.H
+    fx, fy, cx, cy = sc.intrinsics
+    o = O.OracleScene(wl.buckets, wl.blocks, wl.voxel, fp64=True)
+    for i in range(nframes):
+        o.integrate(frames[i]["points"], frames[i]["colors"], wl.trunc, frames[i]["pose"], sc.intrinsics)
+    oc, _, _ = o.export_blocks()
+    osdf, ow, _, _ = o.export_voxels(len(oc))
+    rng = np.random.default_rng(11)
+    pix = rng.choice(W * H, npix_check, replace=False).astype(np.int32)
+    f = frames[nframes - 1]
+    T = f["pose"].astype(np.float64)
+    mn, mx = f["range"]
+    od = o.render(W, H, T[:3, 3], T, sc.intrinsics, mn, mx, pixels=pix)["depth"][pix]
+    okey = {tuple(c): k for k, c in enumerate(oc)}
+    out = {"frames": nframes, "pixels": npix_check}
+    for store in ("fp32", "fp16"):
+        scene = vsg.VoxelScene(vsg.Params(wl.buckets, wl.blocks, wl.voxel, wl.trunc), store)
+        scene.set_stream(torch.cuda.current_stream().cuda_stream)
+        for i in range(nframes):
+            T_i = frames[i]["pose"]
+            scene.integrateFromScanData(pts[i], rgb[i], nrm[i], W, H, wl.trunc, T_i, fx, fy, cx, cy)
+        rr = vsg.RayCastRender()
+        rr.initialize(W, H)
+        rr.render(scene, T[:3, 3], T, fx, fy, cx, cy, mn, mx)
+        gd = rr.downloadDepthFloat()[pix]
+        gc, _ = scene.export_blocks()
+        gsdf, gw, _ = scene.export_voxels(0, len(gc))
+        gi, oi = [], []
+        for k, c in enumerate(gc):
+            j = okey.get(tuple(c))
+            if j is not None:
+                gi.append(k)
+                oi.append(j)
+        gi, oi = np.array(gi), np.array(oi)
+        gv = (gi[:, None] * 512 + np.arange(512)[None, :]).reshape(-1)
+        ov = (oi[:, None] * 512 + np.arange(512)[None, :]).reshape(-1)
+        both = (gw[gv] > 0) & (ow[ov] > 0)
+        valid = (gd < 1e9) & (od < 1e9)
+        out[store] = {"sdf": error_stats(gsdf[gv][both].astype(np.float64) - osdf[ov][both]),
+                      "depth": error_stats(gd[valid].astype(np.float64) - od[valid]),
+                      "blocks_gpu": int(len(gc)), "blocks_fp64": int(len(oc)), "blocks_common": int(len(gi))}
+        rr.close()
+        scene.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +172,8 @@ def main() -> int:
     ap.add_argument("--cpu-frames", type=int, default=40)
     ap.add_argument("--cpu-stride", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--precision-frames", type=int, default=30)
+    ap.add_argument("--no-precision", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU: "
@@ -250,6 +314,10 @@ def main() -> int:
         extra["fp16_store"] = {"value": round(frames_done / r16["t_total"], 3),
                                "speedup_vs_fp32": round(t_total / r16["t_total"], 4),
                                "ms_per_frame": round(r16["t_total"] / frames_done * 1e3, 4)}
+    if not args.no_precision:
+        log(f"precision: FP32 and FP16 stores vs the FP64 oracle on {args.precision_frames} frames")
+        extra["precision_vs_fp64"] = precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm,
+                                                      args.precision_frames, 20000)
     cpu = None
     if not args.no_cpu_baseline:
         log(f"cpu baseline: oracle on {args.cpu_frames} frames")

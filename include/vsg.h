@@ -156,6 +156,15 @@ vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out);
 vsg_status vsg_render_destroy(vsg_render* render);
 vsg_status vsg_render_set_stream(vsg_render* render, void* hip_stream);
 
+/* Trilinear sampling of the ray march.  VSG_SAMPLE_REFERENCE (default) is the
+ * reference's scheme (RayCastRender.cu:139-176): weights frac(p/v - 1/2) with
+ * corners rounded from p -+ v/2, which returns the stored field half a voxel off and
+ * jumps at half-voxel planes.  VSG_SAMPLE_TRILINEAR interpolates the voxels at i*v
+ * with weights frac(p/v): a continuous field for pose tracking.  Only the reference
+ * scheme reproduces the reference's renders.  No counterpart in the reference. */
+typedef enum vsg_sampling { VSG_SAMPLE_REFERENCE = 0, VSG_SAMPLE_TRILINEAR = 1 } vsg_sampling;
+vsg_status vsg_render_set_sampling(vsg_render* render, int32_t mode);
+
 /* RayCastRender::render (VoxelSG/src/RayCastRender.cpp:123-183,
  * VoxelSG/src/RayCastRender.cu:407-620): one ray per pixel, marched in half-voxel
  * steps over [min_depth, max_depth] (z-depth), zero crossing refined by three

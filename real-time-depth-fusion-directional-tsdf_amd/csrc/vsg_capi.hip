@@ -110,6 +110,8 @@ struct vsg_scene {
   DevState* st = nullptr;
   uint32_t* pixcnt = nullptr;
   uint32_t* wgsum = nullptr;
+  uint32_t* claim_part = nullptr;  // per-workgroup claim counters (8 per workgroup)
+  uint32_t* upd_part = nullptr;    // per-workgroup updated-voxel counts of the last integrate
   size_t pix_cap = 0;
   uint32_t frames = 0;  // integrate calls since reset (host mirror)
   bool diag = false;
@@ -129,9 +131,10 @@ struct vsg_render {
   float4* pos = nullptr;
   float4* col = nullptr;
   float4* nrm = nullptr;
-  uint32_t* rc = nullptr;  // 2 sets x {hits, misses}
+  uint32_t* rc = nullptr;  // {hits, misses} per workgroup of the last render
   uint32_t calls = 0;
   hipEvent_t ev = nullptr;
+  int32_t sampling = VSG_SAMPLE_REFERENCE;
   RenderArgs last_args{};  // arguments of the last render (the adjoint's forward)
   bool has_args = false;
   float* jac = nullptr;       // GN scratch: Jacobian per pixel
@@ -155,6 +158,8 @@ void free_scene(vsg_scene* s) {
   (void)hipFree(s->st);
   (void)hipFree(s->pixcnt);
   (void)hipFree(s->wgsum);
+  (void)hipFree(s->claim_part);
+  (void)hipFree(s->upd_part);
   (void)hipFree(s->mark_abkt);
   (void)hipFree(s->mark_rbkt);
   (void)hipFree(s->mark_blk);
@@ -253,6 +258,7 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   if (r == VSG_OK) r = dmalloc(&s->col, N * kVox * sizeof(uint32_t), "colour store");
   if (r == VSG_OK) r = dmalloc(&s->vis, N * sizeof(uint32_t), "visible list");
   if (r == VSG_OK) r = dmalloc(&s->st, sizeof(DevState), "scene state");
+  if (r == VSG_OK) r = dmalloc(&s->upd_part, kIntegrateGrid * sizeof(uint32_t), "integrate counters");
   if (r != VSG_OK) {
     free_scene(s);
     return r;
@@ -262,6 +268,7 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   if (e == hipSuccess) e = hipMemsetAsync(s->col, 0, N * kVox * sizeof(uint32_t), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->t.blk, 0, N * sizeof(int4), s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->upd_part, 0, kIntegrateGrid * sizeof(uint32_t), s->stream);
   if (e != hipSuccess) {
     free_scene(s);
     return hip_fail(e, "hipMemsetAsync");
@@ -323,11 +330,14 @@ vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, con
     VSG_HIP(hipStreamSynchronize(s->stream));
     (void)hipFree(s->pixcnt);
     (void)hipFree(s->wgsum);
+    (void)hipFree(s->claim_part);
     s->pixcnt = nullptr;
     s->wgsum = nullptr;
+    s->claim_part = nullptr;
     s->pix_cap = 0;
     vsg_status r = dmalloc(&s->pixcnt, npix * sizeof(uint32_t), "pixel scratch");
     if (r == VSG_OK) r = dmalloc(&s->wgsum, (size_t)nwg * sizeof(uint32_t), "workgroup scratch");
+    if (r == VSG_OK) r = dmalloc(&s->claim_part, (size_t)nwg * 8 * sizeof(uint32_t), "workgroup counters");
     if (r != VSG_OK) return r;
     s->pix_cap = npix;
   }
@@ -355,7 +365,7 @@ vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, con
   vsg_status r;
   {
     TimedLaunch tl(s->stream, VSG_K_ALLOC_CLAIM);
-    launch_alloc_claim(s->stream, s->t, s->st, fa, pts, s->pixcnt, s->diag ? s->mark_abkt : nullptr);
+    launch_alloc_claim(s->stream, s->t, fa, pts, s->pixcnt, s->diag ? s->mark_abkt : nullptr, s->claim_part);
   }
   if ((r = check_launch("alloc claim")) != VSG_OK) return r;
   {
@@ -365,7 +375,7 @@ vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, con
   if ((r = check_launch("alloc resolve")) != VSG_OK) return r;
   {
     TimedLaunch tl(s->stream, VSG_K_ALLOC_SCAN);
-    launch_alloc_scan(s->stream, s->st, s->wgsum, nwg, fa.cset, s->t.num_blocks);
+    launch_alloc_scan(s->stream, s->st, s->wgsum, s->claim_part, nwg, fa.cset, s->t.num_blocks);
   }
   if ((r = check_launch("alloc scan")) != VSG_OK) return r;
   {
@@ -380,7 +390,8 @@ vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, con
   if ((r = check_launch("compact")) != VSG_OK) return r;
   {
     TimedLaunch tl(s->stream, VSG_K_INTEGRATE);
-    launch_integrate(s->stream, fp16, s->t, s->st, fa, pts, rgb, s->vis, s->vox, s->col, 2048);
+    launch_integrate(s->stream, fp16, s->t, s->st, fa, pts, rgb, s->vis, s->vox, s->col, kIntegrateGrid,
+                     s->upd_part);
   }
   if ((r = check_launch("integrate")) != VSG_OK) return r;
   s->frames++;
@@ -401,7 +412,11 @@ vsg_status vsg_scene_counters(vsg_scene* s, vsg_counters* out) {
     out->new_blocks = h.n_alloc - h.n_alloc_prev;
     out->heap_fail = f.heap_fail;
     out->visible_blocks = f.visible;
-    out->updated_voxels = f.updated;
+    std::vector<uint32_t> up(kIntegrateGrid);
+    VSG_HIP(hipMemcpy(up.data(), s->upd_part, up.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    uint32_t updated = 0;
+    for (uint32_t u : up) updated += u;
+    out->updated_voxels = updated;
     out->claimed_buckets = f.claimed_buckets;
     out->out_of_range = f.out_of_range;
   }
@@ -515,8 +530,9 @@ vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out) {
   if (st == VSG_OK) st = dmalloc(&r->pos, n * sizeof(float4), "position output");
   if (st == VSG_OK) st = dmalloc(&r->col, n * sizeof(float4), "colour output");
   if (st == VSG_OK) st = dmalloc(&r->nrm, n * sizeof(float4), "normal output");
-  if (st == VSG_OK) st = dmalloc(&r->rc, 4 * sizeof(uint32_t), "render counters");
-  if (st == VSG_OK && hipMemset(r->rc, 0, 4 * sizeof(uint32_t)) != hipSuccess) st = fail(VSG_ERR_HIP, "hipMemset");
+  const size_t nwg = (size_t)((width + 7) / 8) * ((height + 7) / 8);
+  if (st == VSG_OK) st = dmalloc(&r->rc, nwg * 2 * sizeof(uint32_t), "render counters");
+  if (st == VSG_OK && hipMemset(r->rc, 0, nwg * 2 * sizeof(uint32_t)) != hipSuccess) st = fail(VSG_ERR_HIP, "hipMemset");
   if (st == VSG_OK && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipEventCreate");
   if (st != VSG_OK) {
@@ -540,6 +556,14 @@ vsg_status vsg_render_destroy(vsg_render* r) {
   (void)hipFree(r->gn_out);
   if (r->ev) (void)hipEventDestroy(r->ev);
   delete r;
+  return VSG_OK;
+}
+
+vsg_status vsg_render_set_sampling(vsg_render* r, int32_t mode) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_sampling: null renderer");
+  if (mode != VSG_SAMPLE_REFERENCE && mode != VSG_SAMPLE_TRILINEAR)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_sampling: unknown mode");
+  r->sampling = mode;
   return VSG_OK;
 }
 
@@ -576,13 +600,13 @@ vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3],
   ra.mark_bkt = s->diag ? s->mark_rbkt : nullptr;
   ra.mark_blk = s->diag ? s->mark_blk : nullptr;
   ra.march_stats = s->diag ? s->march_stats : nullptr;
+  ra.mode = r->sampling;
   hipStream_t strm = r->has_stream ? r->stream : s->stream;
   if (strm != s->stream) {  // order after the scene's pending work
     VSG_HIP(hipEventRecord(r->ev, s->stream));
     VSG_HIP(hipStreamWaitEvent(strm, r->ev, 0));
   }
-  uint32_t* rc = r->rc + 2 * (r->calls & 1u);
-  VSG_HIP(hipMemsetAsync(rc, 0, 2 * sizeof(uint32_t), strm));
+  uint32_t* rc = r->rc;
   {
     TimedLaunch tl(strm, VSG_K_RAYCAST);
     launch_render(strm, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->depth, r->pos, r->col,
@@ -627,10 +651,13 @@ vsg_status vsg_render_counters(vsg_render* r, uint32_t* hits, uint32_t* misses) 
   *hits = *misses = 0;
   if (r->calls == 0) return VSG_OK;
   if (r->has_last) VSG_HIP(hipStreamSynchronize(r->last));
-  uint32_t h[2];
-  VSG_HIP(hipMemcpy(h, r->rc + 2 * ((r->calls - 1u) & 1u), sizeof(h), hipMemcpyDeviceToHost));
-  *hits = h[0];
-  *misses = h[1];
+  const size_t nwg = (size_t)((r->W + 7) / 8) * ((r->H + 7) / 8);
+  std::vector<uint32_t> h(2 * nwg);
+  VSG_HIP(hipMemcpy(h.data(), r->rc, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < nwg; ++i) {
+    *hits += h[2 * i];
+    *misses += h[2 * i + 1];
+  }
   return VSG_OK;
 }
 

@@ -18,22 +18,25 @@ struct RenderArgs {
   uint8_t* mark_bkt;  // diagnostics (null when off): buckets with live entries probed
   uint8_t* mark_blk;  // diagnostics (null when off): blocks a trilinear corner touched
   uint32_t* march_stats;  // diagnostics (null when off): evaluations, super/block skips, replayed steps
+  int32_t mode;           // vsg_sampling: 0 the reference's trilinear sampling, 1 standard trilinear
 };
 
 void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col);
 void launch_reset_table(hipStream_t s, const TableView& t, DevState* st);
-void launch_alloc_claim(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
-                        uint32_t* pixcnt, uint8_t* mark_bkt);
+constexpr uint32_t kIntegrateGrid = 2048;  // workgroups of k_integrate (grid-stride over the list)
+void launch_alloc_claim(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
+                        uint8_t* mark_bkt, uint32_t* part);
 void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
                           uint32_t* wgsum);
-void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, uint32_t nwg, int32_t cset, uint32_t num_blocks);
+void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, const uint32_t* part, uint32_t nwg, int32_t cset,
+                       uint32_t num_blocks);
 void launch_alloc_insert(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
                          const uint32_t* pixcnt, const uint32_t* wgoff);
 void launch_compact(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, uint32_t* vis,
                     uint32_t grid);
 void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
                       const float* pts, const uint8_t* rgb, const uint32_t* vis, void* vox, uint32_t* col,
-                      uint32_t grid);
+                      uint32_t grid, uint32_t* upd_part);
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                            const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
                            float* grad_sdf);

@@ -90,9 +90,12 @@ template <class F> __device__ __forceinline__ void dda_band(float3 a, float3 b, 
 // Allocation
 // ---------------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_alloc_claim(TableView t, DevState* st, FrameArgs fa,
-                                                     const float* __restrict__ pts, uint32_t* __restrict__ pixflag,
-                                                     uint8_t* __restrict__ mark_bkt) {
+// Per-workgroup partial counters of k_alloc_claim (reduced by k_alloc_scan).
+constexpr int kClaimPart = 8;  // valid, claimed, out of range, non-finite, zmin key, zmax key, -, -
+
+__global__ __launch_bounds__(256) void k_alloc_claim(TableView t, FrameArgs fa, const float* __restrict__ pts,
+                                                     uint32_t* __restrict__ pixflag, uint8_t* __restrict__ mark_bkt,
+                                                     uint32_t* __restrict__ part) {
   const int pix = blockIdx.x * 256 + threadIdx.x;
   const int npix = fa.W * fa.H;
   uint32_t valid = 0, claimed = 0, oor = 0, nonfinite = 0;
@@ -129,7 +132,9 @@ __global__ __launch_bounds__(256) void k_alloc_claim(TableView t, DevState* st, 
     }
     pixflag[pix] = claimed;
   }
-  FrameCounters& fc = st->set[fa.cset];
+  // one write per workgroup: same-address atomics from every wave would serialise
+  __shared__ uint32_t red[4][6];
+  const int w = threadIdx.x >> 6;
   valid = wave_sum(valid);
   claimed = wave_sum(claimed);
   oor = wave_sum(oor);
@@ -137,12 +142,22 @@ __global__ __launch_bounds__(256) void k_alloc_claim(TableView t, DevState* st, 
   zmin = wave_min(zmin);
   zmax = wave_max(zmax);
   if ((threadIdx.x & 63) == 0) {
-    if (valid) atomicAdd(&fc.valid_pixels, valid);
-    if (claimed) atomicAdd(&fc.claim_pixels, claimed);
-    if (oor) atomicAdd(&fc.out_of_range, oor);
-    if (nonfinite) atomicAdd(&fc.nonfinite, nonfinite);
-    if (zmin != 0xFFFFFFFFu) atomicMin(&fc.zmin_key, zmin);
-    if (zmax != 0u) atomicMax(&fc.zmax_key, zmax);
+    red[w][0] = valid;
+    red[w][1] = claimed;
+    red[w][2] = oor;
+    red[w][3] = nonfinite;
+    red[w][4] = zmin;
+    red[w][5] = zmax;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t* o = part + (size_t)blockIdx.x * kClaimPart;
+    o[0] = red[0][0] + red[1][0] + red[2][0] + red[3][0];
+    o[1] = red[0][1] + red[1][1] + red[2][1] + red[3][1];
+    o[2] = red[0][2] + red[1][2] + red[2][2] + red[3][2];
+    o[3] = red[0][3] + red[1][3] + red[2][3] + red[3][3];
+    o[4] = min(min(red[0][4], red[1][4]), min(red[2][4], red[3][4]));
+    o[5] = max(max(red[0][5], red[1][5]), max(red[2][5], red[3][5]));
   }
 }
 
@@ -170,13 +185,25 @@ __global__ __launch_bounds__(256) void k_alloc_resolve(TableView t, FrameArgs fa
   if (threadIdx.x == 0) wgsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-__global__ __launch_bounds__(1024) void k_alloc_scan(DevState* st, uint32_t* __restrict__ wg, uint32_t nwg,
-                                                     int32_t cset, uint32_t num_blocks) {
+__global__ __launch_bounds__(1024) void k_alloc_scan(DevState* st, uint32_t* __restrict__ wg,
+                                                     const uint32_t* __restrict__ part, uint32_t nwg, int32_t cset,
+                                                     uint32_t num_blocks) {
   const uint32_t per = (nwg + 1023u) / 1024u;
   const uint32_t b0 = threadIdx.x * per;
   uint32_t local = 0;
-  for (uint32_t k = 0; k < per; ++k)
-    if (b0 + k < nwg) local += wg[b0 + k];
+  uint32_t cv = 0, cc = 0, co = 0, cn = 0, zmn = 0xFFFFFFFFu, zmx = 0u;
+  for (uint32_t k = 0; k < per; ++k) {
+    if (b0 + k < nwg) {
+      local += wg[b0 + k];
+      const uint32_t* p = part + (size_t)(b0 + k) * kClaimPart;
+      cv += p[0];
+      cc += p[1];
+      co += p[2];
+      cn += p[3];
+      zmn = min(zmn, p[4]);
+      zmx = max(zmx, p[5]);
+    }
+  }
   uint32_t total = 0;
   uint32_t run = block_excl_scan<1024>(local, &total);
   for (uint32_t k = 0; k < per; ++k) {
@@ -186,14 +213,44 @@ __global__ __launch_bounds__(1024) void k_alloc_scan(DevState* st, uint32_t* __r
       run += x;
     }
   }
+  // frame counters: one workgroup-wide reduction of the claim partials
+  __shared__ uint32_t red[16][6];
+  cv = wave_sum(cv);
+  cc = wave_sum(cc);
+  co = wave_sum(co);
+  cn = wave_sum(cn);
+  zmn = wave_min(zmn);
+  zmx = wave_max(zmx);
+  if ((threadIdx.x & 63) == 0) {
+    const int w = threadIdx.x >> 6;
+    red[w][0] = cv;
+    red[w][1] = cc;
+    red[w][2] = co;
+    red[w][3] = cn;
+    red[w][4] = zmn;
+    red[w][5] = zmx;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
+    FrameCounters& fc = st->set[cset];
+    uint32_t a[6] = {0, 0, 0, 0, 0xFFFFFFFFu, 0u};
+    for (int w = 0; w < 16; ++w) {
+      for (int k = 0; k < 4; ++k) a[k] += red[w][k];
+      a[4] = min(a[4], red[w][4]);
+      a[5] = max(a[5], red[w][5]);
+    }
+    fc.valid_pixels = a[0];
+    fc.claim_pixels = a[1];
+    fc.out_of_range = a[2];
+    fc.nonfinite = a[3];
+    fc.zmin_key = a[4];
+    fc.zmax_key = a[5];
     const uint32_t prev = st->n_alloc;
     const uint32_t room = num_blocks - prev;
     const uint32_t ok = total < room ? total : room;
     st->n_alloc_prev = prev;
     st->n_alloc = prev + ok;
     st->frames += 1u;
-    FrameCounters& fc = st->set[cset];
     fc.winners = total;
     fc.claimed_buckets = total;
     fc.heap_fail = total - ok;
@@ -298,17 +355,26 @@ __global__ __launch_bounds__(256) void k_compact(TableView t, DevState* st, Fram
   const bool cull = fa.cull_ok != 0 && fc.nonfinite == 0u;
   const double zmin = any_valid && cull ? (double)ordered_to_float(fc.zmin_key) : 0.0;
   const double zmax = any_valid && cull ? (double)ordered_to_float(fc.zmax_key) : 0.0;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ uint32_t wcnt[4];
+  __shared__ uint32_t wbase;
   for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
     const uint32_t i = base + threadIdx.x;
     bool keep = false;
     if (i < n && any_valid) keep = !cull || block_maybe_visible(fa, t.blk[i], zmin, zmax);
+    // wave ballot + prefix, then one atomic per workgroup for the list offset
     const uint64_t m = __ballot(keep);
-    const uint32_t cnt = (uint32_t)__popcll(m);
-    uint32_t off = 0;
-    if (lane == 0 && cnt) off = atomicAdd(&fc.visible, cnt);
-    off = __shfl(off, 0, 64);
+    if (lane == 0) wcnt[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint32_t tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+      wbase = tot ? atomicAdd(&fc.visible, tot) : 0u;
+    }
+    __syncthreads();
+    uint32_t off = wbase;
+    for (int k = 0; k < w; ++k) off += wcnt[k];
     if (keep) vis[off + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    __syncthreads();
   }
 }
 
@@ -338,7 +404,7 @@ template <bool F16>
 __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, FrameArgs fa,
                                                    const float* __restrict__ pts, const uint8_t* __restrict__ rgb,
                                                    const uint32_t* __restrict__ vis, void* __restrict__ vox_,
-                                                   uint32_t* __restrict__ col) {
+                                                   uint32_t* __restrict__ col, uint32_t* __restrict__ upd_part) {
   using S = Store<F16>;
   typename S::word* vox = reinterpret_cast<typename S::word*>(vox_);
   FrameCounters& fc = st->set[fa.cset];
@@ -402,8 +468,16 @@ __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, Fr
       upd++;
     }
   }
+  // one write per workgroup (summed when the counters are read)
+  __shared__ uint32_t ws[8];
   upd = wave_sum(upd);
-  if ((threadIdx.x & 63) == 0 && upd) atomicAdd(&fc.updated, upd);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = upd;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int k = 0; k < 8; ++k) tot += ws[k];
+    upd_part[blockIdx.x] = tot;
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -457,20 +531,52 @@ struct CornerBlocks {
 // Returns kSampleValid with *dist (and *rgb), kSampleInvalid when a corner block
 // exists but every corner voxel has weight 0, kSampleEmpty when no corner block
 // exists (then *cb holds the corner blocks' range).
-template <bool F16, bool COLOR, class Cache>
+// Corner voxels and weights of a sample.  REF: the reference's scheme (weights
+// frac(p/v - 1/2), corners rounded from p - v/2 and p + v/2: RayCastRender.cu:139-176),
+// which samples the stored field half a voxel off.  Otherwise standard trilinear
+// interpolation of voxels at i*v (weights frac(p/v), corners floor(p/v) + {0,1}),
+// offered for tracking; it is not the parity path.
+template <bool REF>
+__device__ __forceinline__ void corner_setup(float px, float py, float pz, float v, int32_t vx[2], int32_t vy[2],
+                                             int32_t vz[2], float* wx, float* wy, float* wz) {
+  if (REF) {
+    const float off = v;
+    const float pdx = px - off / 2.0f, pdy = py - off / 2.0f, pdz = pz - off / 2.0f;
+    const float qx0 = pdx / v, qy0 = pdy / v, qz0 = pdz / v;
+    *wx = qx0 - floorf(qx0);
+    *wy = qy0 - floorf(qy0);
+    *wz = qz0 - floorf(qz0);
+    const float qx1 = (pdx + off) / v, qy1 = (pdy + off) / v, qz1 = (pdz + off) / v;
+    vx[0] = ray_voxel_of_q(qx0);
+    vx[1] = ray_voxel_of_q(qx1);
+    vy[0] = ray_voxel_of_q(qy0);
+    vy[1] = ray_voxel_of_q(qy1);
+    vz[0] = ray_voxel_of_q(qz0);
+    vz[1] = ray_voxel_of_q(qz1);
+  } else {
+    const float qx = px / v, qy = py / v, qz = pz / v;
+    const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+    *wx = qx - fx;
+    *wy = qy - fy;
+    *wz = qz - fz;
+    vx[0] = cvt_i32(fx);
+    vy[0] = cvt_i32(fy);
+    vz[0] = cvt_i32(fz);
+    vx[1] = vx[0] + 1;
+    vy[1] = vy[0] + 1;
+    vz[1] = vz[0] + 1;
+  }
+}
+
+template <bool F16, bool COLOR, bool REF, class Cache>
 __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const void* __restrict__ vox_,
                                          const uint32_t* __restrict__ col, float px, float py, float pz, float v,
                                          float* dist, uint32_t* rgb, CornerBlocks* cb) {
   using S = Store<F16>;
   const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
-  const float off = v;
-  const float pdx = px - off / 2.0f, pdy = py - off / 2.0f, pdz = pz - off / 2.0f;
-  const float qx0 = pdx / v, qy0 = pdy / v, qz0 = pdz / v;
-  const float wx = qx0 - floorf(qx0), wy = qy0 - floorf(qy0), wz = qz0 - floorf(qz0);
-  const float qx1 = (pdx + off) / v, qy1 = (pdy + off) / v, qz1 = (pdz + off) / v;
-  const int32_t vx[2] = {ray_voxel_of_q(qx0), ray_voxel_of_q(qx1)};
-  const int32_t vy[2] = {ray_voxel_of_q(qy0), ray_voxel_of_q(qy1)};
-  const int32_t vz[2] = {ray_voxel_of_q(qz0), ray_voxel_of_q(qz1)};
+  int32_t vx[2], vy[2], vz[2];
+  float wx, wy, wz;
+  corner_setup<REF>(px, py, pz, v, vx, vy, vz, &wx, &wy, &wz);
   const int32_t bx[2] = {voxel_to_block_axis(vx[0]), voxel_to_block_axis(vx[1])};
   const int32_t by[2] = {voxel_to_block_axis(vy[0]), voxel_to_block_axis(vy[1])};
   const int32_t bz[2] = {voxel_to_block_axis(vz[0]), voxel_to_block_axis(vz[1])};
@@ -575,7 +681,7 @@ struct RayHit {
   uint32_t rgb;
 };
 
-template <bool F16, bool DIAG, class Cache>
+template <bool F16, bool DIAG, bool REF, class Cache>
 __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
                                             const uint32_t* __restrict__ col, uint32_t x, uint32_t y, Cache& bc) {
   const float MINF = 1e10f;
@@ -672,7 +778,7 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
     const float ppx = camx + ray * wdx, ppy = camy + ray * wdy, ppz = camz + ray * wdz;
     float dist;
     CornerBlocks cb;
-    const int stat = trilinear<F16, false>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr, &cb);
+    const int stat = trilinear<F16, false, REF>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr, &cb);
     n_eval++;
     if (stat == kSampleValid) {
       try_super = false;
@@ -685,8 +791,8 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
           c = a + (ad / (ad - bd)) * (bb - a);
           float cd;
           CornerBlocks cb2;
-          if (trilinear<F16, true>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd, &rgb,
-                                   &cb2) != kSampleValid) {
+          if (trilinear<F16, true, REF>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd,
+                                        &rgb, &cb2) != kSampleValid) {
             ok = false;
             break;
           }
@@ -746,13 +852,20 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
   return h;
 }
 
-template <bool F16, bool DIAG>
+template <bool F16, bool DIAG, bool REF>
 __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                float* __restrict__ out_depth, float4* __restrict__ out_pos,
                                                float4* __restrict__ out_col, float4* __restrict__ out_nrm,
                                                uint32_t* __restrict__ rc) {
-  if (st->n_alloc == 0u) return;  // RayCastRender.cu:553-556: nothing to render, outputs untouched
+  const size_t wg = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if (st->n_alloc == 0u) {  // RayCastRender.cu:553-556: nothing to render, outputs untouched
+    if (threadIdx.x == 0) {
+      rc[2 * wg] = 0u;
+      rc[2 * wg + 1] = 0u;
+    }
+    return;
+  }
   const uint32_t x = blockIdx.x * 8u + (threadIdx.x & 7u);
   const uint32_t y = blockIdx.y * 8u + (threadIdx.x >> 3);
   uint32_t hit = 0, miss = 0;
@@ -761,7 +874,7 @@ __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __re
     const int pix = (int)y * ra.W + (int)x;
     BlockCache<DIAG> bc;
     bc.init(ra.mark_bkt, ra.mark_blk, ra.march_stats);
-    const RayHit h = march_ray<F16, DIAG>(t, ra, vox, col, x, y, bc);
+    const RayHit h = march_ray<F16, DIAG, REF>(t, ra, vox, col, x, y, bc);
     float o_depth = MINF;
     float4 o_pos = make_float4(MINF, MINF, MINF, MINF), o_col = o_pos;
     if (h.found) {
@@ -778,11 +891,12 @@ __global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __re
     out_col[pix] = o_col;
     out_nrm[pix] = make_float4(MINF, MINF, MINF, MINF);
   }
+  // one wave per workgroup: one write of its hit / miss counts (summed when read)
   hit = wave_sum(hit);
   miss = wave_sum(miss);
   if (threadIdx.x == 0) {
-    if (hit) atomicAdd(&rc[0], hit);
-    if (miss) atomicAdd(&rc[1], miss);
+    rc[2 * wg] = hit;
+    rc[2 * wg + 1] = miss;
   }
 }
 
@@ -807,19 +921,14 @@ struct TriGradF {
 
 // Gradient part of trilinear() at p (whose value the forward computed); the corner
 // set and weights are recomputed exactly as the forward forms them.
-template <bool F16, class Cache>
+template <bool F16, bool REF, class Cache>
 __device__ __forceinline__ void trilinear_grad(const TableView& t, Cache& bc, const void* __restrict__ vox_, float px,
                                                float py, float pz, float v, float f, TriGradF* out) {
   using S = Store<F16>;
   const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
-  const float off = v;
-  const float pdx = px - off / 2.0f, pdy = py - off / 2.0f, pdz = pz - off / 2.0f;
-  const float qx0 = pdx / v, qy0 = pdy / v, qz0 = pdz / v;
-  const float fr[3] = {qx0 - floorf(qx0), qy0 - floorf(qy0), qz0 - floorf(qz0)};
-  const float qx1 = (pdx + off) / v, qy1 = (pdy + off) / v, qz1 = (pdz + off) / v;
-  const int32_t vx[2] = {ray_voxel_of_q(qx0), ray_voxel_of_q(qx1)};
-  const int32_t vy[2] = {ray_voxel_of_q(qy0), ray_voxel_of_q(qy1)};
-  const int32_t vz[2] = {ray_voxel_of_q(qz0), ray_voxel_of_q(qz1)};
+  int32_t vx[2], vy[2], vz[2];
+  float fr[3];
+  corner_setup<REF>(px, py, pz, v, vx, vy, vz, &fr[0], &fr[1], &fr[2]);
   const float iv = 1.0f / v;
   float W = 0.0f, dW[3] = {0.0f, 0.0f, 0.0f}, dN[3] = {0.0f, 0.0f, 0.0f}, w[8];
 #pragma unroll
@@ -886,7 +995,7 @@ __device__ __forceinline__ Dual10 eval_dual(const TriGradF& tg, float fval, cons
   return e;
 }
 
-template <bool F16>
+template <bool F16, bool REF>
 __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                        const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                        float* __restrict__ jac, const float* __restrict__ grad_depth,
@@ -899,7 +1008,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
   if (st->n_alloc != 0u) {
     BlockCache<false> bc;
     bc.init(nullptr, nullptr);
-    const RayHit h = march_ray<F16, false>(t, ra, vox, col, x, y, bc);
+    const RayHit h = march_ray<F16, false, REF>(t, ra, vox, col, x, y, bc);
     if (h.found) {
       const float v = ra.voxel;
       const float camx = ra.cam[0], camy = ra.cam[1], camz = ra.cam[2];
@@ -912,8 +1021,8 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
       for (int k = 0; k < 10; ++k) a.d[k] = b.d[k] = c.d[k] = 0.0f;
       a.v = h.a;
       b.v = h.b;
-      trilinear_grad<F16>(t, bc, vox, px(h.a), py(h.a), pz(h.a), v, h.ad, &tg[0]);
-      trilinear_grad<F16>(t, bc, vox, px(h.b), py(h.b), pz(h.b), v, h.bd, &tg[1]);
+      trilinear_grad<F16, REF>(t, bc, vox, px(h.a), py(h.a), pz(h.a), v, h.ad, &tg[0]);
+      trilinear_grad<F16, REF>(t, bc, vox, px(h.b), py(h.b), pz(h.b), v, h.bd, &tg[1]);
       ad = eval_dual(tg[0], h.ad, a, h, ra.M, 0);
       bd = eval_dual(tg[1], h.bd, b, h, ra.M, 1);
       float fa = h.a, fad = h.ad, fb = h.b, fbd = h.bd;
@@ -930,8 +1039,8 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
         float fcd;
         uint32_t rgb;
         CornerBlocks cb;
-        trilinear<F16, false>(t, bc, vox, col, px(fc), py(fc), pz(fc), v, &fcd, &rgb, &cb);
-        trilinear_grad<F16>(t, bc, vox, px(fc), py(fc), pz(fc), v, fcd, &tg[2 + it]);
+        trilinear<F16, false, REF>(t, bc, vox, col, px(fc), py(fc), pz(fc), v, &fcd, &rgb, &cb);
+        trilinear_grad<F16, REF>(t, bc, vox, px(fc), py(fc), pz(fc), v, fcd, &tg[2 + it]);
         const Dual10 cd = eval_dual(tg[2 + it], fcd, c, h, ra.M, 2 + it);
         if (fad * fcd > 0.0f) {
           fa = fc;
@@ -1177,10 +1286,10 @@ void launch_reset_table(hipStream_t s, const TableView& t, DevState* st) {
   hipLaunchKernelGGL(k_reset_table, dim3(1024), dim3(256), 0, s, t, st);
 }
 
-void launch_alloc_claim(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
-                        uint32_t* pixcnt, uint8_t* mark_bkt) {
+void launch_alloc_claim(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
+                        uint8_t* mark_bkt, uint32_t* part) {
   const uint32_t nwg = (uint32_t)((fa.W * fa.H + 255) / 256);
-  hipLaunchKernelGGL(k_alloc_claim, dim3(nwg), dim3(256), 0, s, t, st, fa, pts, pixcnt, mark_bkt);
+  hipLaunchKernelGGL(k_alloc_claim, dim3(nwg), dim3(256), 0, s, t, fa, pts, pixcnt, mark_bkt, part);
 }
 
 void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
@@ -1189,8 +1298,9 @@ void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa
   hipLaunchKernelGGL(k_alloc_resolve, dim3(nwg), dim3(256), 0, s, t, fa, pts, pixcnt, wgsum);
 }
 
-void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, uint32_t nwg, int32_t cset, uint32_t num_blocks) {
-  hipLaunchKernelGGL(k_alloc_scan, dim3(1), dim3(1024), 0, s, st, wgsum, nwg, cset, num_blocks);
+void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, const uint32_t* part, uint32_t nwg, int32_t cset,
+                       uint32_t num_blocks) {
+  hipLaunchKernelGGL(k_alloc_scan, dim3(1), dim3(1024), 0, s, st, wgsum, part, nwg, cset, num_blocks);
 }
 
 void launch_alloc_insert(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
@@ -1206,21 +1316,26 @@ void launch_compact(hipStream_t s, const TableView& t, DevState* st, const Frame
 
 void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
                       const float* pts, const uint8_t* rgb, const uint32_t* vis, void* vox, uint32_t* col,
-                      uint32_t grid) {
+                      uint32_t grid, uint32_t* upd_part) {
   if (fp16)
-    hipLaunchKernelGGL(k_integrate<true>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col);
+    hipLaunchKernelGGL(k_integrate<true>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col, upd_part);
   else
-    hipLaunchKernelGGL(k_integrate<false>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col);
+    hipLaunchKernelGGL(k_integrate<false>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col, upd_part);
 }
 
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                            const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
                            float* grad_sdf) {
   const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
-  if (fp16)
-    hipLaunchKernelGGL(k_render_adjoint<true>, grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf);
-  else
-    hipLaunchKernelGGL(k_render_adjoint<false>, grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf);
+  const bool ref = ra.mode == 0;
+#define VSG_ADJ(F, R) \
+  hipLaunchKernelGGL((k_render_adjoint<F, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf)
+  if (fp16) {
+    if (ref) VSG_ADJ(true, true); else VSG_ADJ(true, false);
+  } else {
+    if (ref) VSG_ADJ(false, true); else VSG_ADJ(false, false);
+  }
+#undef VSG_ADJ
 }
 
 uint32_t gn_partial_count(int npix) { return (uint32_t)((npix + 255) / 256); }
@@ -1239,17 +1354,23 @@ void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState*
                    uint32_t* rc) {
   const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
   const bool diag = ra.mark_blk != nullptr;
+  const bool ref = ra.mode == 0;
+#define VSG_RENDER(F, D, R) \
+  hipLaunchKernelGGL((k_render<F, D, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc)
   if (fp16) {
-    if (diag)
-      hipLaunchKernelGGL((k_render<true, true>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
-    else
-      hipLaunchKernelGGL((k_render<true, false>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+    if (diag) {
+      if (ref) VSG_RENDER(true, true, true); else VSG_RENDER(true, true, false);
+    } else {
+      if (ref) VSG_RENDER(true, false, true); else VSG_RENDER(true, false, false);
+    }
   } else {
-    if (diag)
-      hipLaunchKernelGGL((k_render<false, true>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
-    else
-      hipLaunchKernelGGL((k_render<false, false>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc);
+    if (diag) {
+      if (ref) VSG_RENDER(false, true, true); else VSG_RENDER(false, true, false);
+    } else {
+      if (ref) VSG_RENDER(false, false, true); else VSG_RENDER(false, false, false);
+    }
   }
+#undef VSG_RENDER
 }
 
 }  // namespace vsg

@@ -86,6 +86,7 @@ _SIGNATURES = {
     "vsg_render_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "vsg_render_destroy": (C.c_int, [C.c_void_p]),
     "vsg_render_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vsg_render_set_sampling": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsg_render_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                    C.POINTER(vsg_intrinsics), C.c_float, C.c_float, C.c_float]),
     "vsg_render_buffers": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4),
@@ -445,6 +446,10 @@ class RayCastRender:
             for p, c in zip(pts, cols):
                 fh.write(f"{p[0]} {p[1]} {p[2]} {c[0]} {c[1]} {c[2]}\n")
         return True
+
+    def set_sampling(self, mode: str) -> None:
+        """"reference" (the reference's half-voxel-shifted sampling, default) or "trilinear"."""
+        _check(lib().vsg_render_set_sampling(self._h, {"reference": 0, "trilinear": 1}[mode]))
 
     def adjoint(self, voxelScene: VoxelScene, jac=None, grad_depth=None, grad_sdf=None) -> None:
         """Adjoint of the last render: jac (device float[6*W*H]) receives d depth / d xi for

@@ -69,6 +69,11 @@ def test_jacobian_matches_oracle_adjoint(torch_cuda):
 
 
 def test_jacobian_central_differences(torch_cuda):
+    """Per (pixel, component) agreement with central differences of the GPU render.
+
+    The sampled field jumps every half voxel along each axis, so a difference whose
+    evaluation points cross such a plane within +-h measures a jump, not the slope:
+    at h = 1e-4 an evaluation point moves ~0.035 half-voxel, at h = 2e-5 ~0.007."""
     wl, sc, g, o, f = _fused(torch_cuda)
     T = f["pose"].astype(np.float64)
     mn, mx = synth.depth_range(f["depth"])
@@ -76,20 +81,20 @@ def test_jacobian_central_differences(torch_cuda):
     J = _jac(torch_cuda, g, sc)
     rng = np.random.default_rng(8)
     pix = rng.choice(np.flatnonzero(d0 < 1e9), 4096, replace=False)
-    h = 1e-4
-    fd = np.zeros((pix.size, 6))
-    for k in range(6):
-        e = np.zeros(6)
-        e[k] = h
-        dp = _render_depth(g, sc, T @ O.se3_exp(e), mn, mx)[pix].astype(np.float64)
-        dm = _render_depth(g, sc, T @ O.se3_exp(-e), mn, mx)[pix].astype(np.float64)
-        fd[:, k] = (dp - dm) / (2 * h)
-    ok = np.all(np.abs(fd) < 1e3, axis=1)
-    err = np.abs(J[pix][ok] - fd[ok]).max(axis=1)
-    scale = np.abs(fd[ok]).max(axis=1)
-    good = err <= 0.02 * scale + 5e-3
-    print(f"jacobian vs central differences: {good.mean():.4f} of {ok.sum()}")
-    assert ok.mean() > 0.9 and good.mean() > 0.9
+    for h, need in ((1e-4, 0.6), (2e-5, 0.85)):
+        fd = np.zeros((pix.size, 6))
+        for k in range(6):
+            e = np.zeros(6)
+            e[k] = h
+            dp = _render_depth(g, sc, T @ O.se3_exp(e), mn, mx)[pix].astype(np.float64)
+            dm = _render_depth(g, sc, T @ O.se3_exp(-e), mn, mx)[pix].astype(np.float64)
+            fd[:, k] = (dp - dm) / (2 * h)
+        Jp = J[pix]
+        agree = np.abs(Jp - fd) <= 0.02 * np.abs(Jp) + 2e-3 + 1.5e-7 / h
+        frac = agree.mean()
+        print(f"jacobian vs central differences h={h:g}: {frac:.4f} of {agree.size} (pixel, component) pairs, "
+              f"per component {np.round(agree.mean(axis=0), 3)}")
+        assert frac >= need
 
 
 def test_sdf_gradient_matches_oracle(torch_cuda):
@@ -185,9 +190,15 @@ def test_autograd_function(torch_cuda):
     assert np.abs(expect).max() > 0
 
 
-def test_pose_refinement_recovers_perturbed_poses(torch_cuda):
+@pytest.mark.parametrize("sampling", ["trilinear", "reference"])
+def test_pose_refinement_recovers_perturbed_poses(torch_cuda, sampling):
     """Room 640x480: 32 frames fused at their poses; 8 held-out frames perturbed by up
-    to 2 deg / 2 cm are refined by <= 10 FP64 Gauss-Newton steps on the device."""
+    to 2 deg / 2 cm are refined by <= 10 FP64 Gauss-Newton steps on the device.
+
+    With the standard trilinear sampling the poses come back to <= 1 mm / 0.1 deg.
+    The reference's sampling returns the field half a voxel off (a wall renders +-v/2
+    from where it was observed, depending on its offset to the voxel grid), so the
+    alignment settles a few millimetres away; that bound is looser."""
     torch = torch_cuda
     wl = synth.WORKLOADS[1]
     sc = synth.SynthScene(1)
@@ -196,9 +207,10 @@ def test_pose_refinement_recovers_perturbed_poses(torch_cuda):
         g.integrate(sc.render(i), sc.intrinsics)
     g.render = vsg.RayCastRender()
     assert g.render.initialize(sc.W, sc.H)
+    g.render.set_sampling(sampling)
     fx, fy, cx, cy = sc.intrinsics
     rng = np.random.default_rng(5)
-    rot_err, trans_err = [], []
+    rot_err, trans_err, rot0, trans0 = [], [], [], []
     for i in range(34, 42):
         f = sc.render(i)
         Tgt = sc.pose(i)
@@ -209,6 +221,8 @@ def test_pose_refinement_recovers_perturbed_poses(torch_cuda):
         tdir /= np.linalg.norm(tdir)
         xi = np.concatenate([axis * ang, tdir * rng.uniform(0, 0.02)])
         T = Tgt @ O.se3_exp(xi)
+        rot0.append(np.degrees(ang))
+        trans0.append(np.linalg.norm(xi[3:]))
         obs = torch.from_numpy(f["depth"].reshape(-1)).cuda()
         mn, mx = synth.depth_range(f["depth"])
         for it in range(10):
@@ -220,5 +234,10 @@ def test_pose_refinement_recovers_perturbed_poses(torch_cuda):
         rot_err.append(np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1))))
         trans_err.append(np.linalg.norm(E[:3, 3]))
     rot_err, trans_err = np.array(rot_err), np.array(trans_err)
-    print(f"pose refinement: rot deg {np.round(rot_err, 4)}, trans mm {np.round(trans_err * 1e3, 3)}")
-    assert np.median(rot_err) <= 0.1 and np.median(trans_err) <= 1e-3
+    print(f"pose refinement ({sampling}): start deg {np.round(rot0, 3)} mm {np.round(np.array(trans0) * 1e3, 2)}; "
+          f"end deg {np.round(rot_err, 4)} mm {np.round(trans_err * 1e3, 3)}")
+    if sampling == "trilinear":
+        assert np.median(rot_err) <= 0.1 and np.median(trans_err) <= 1e-3
+        assert rot_err.max() <= 0.2 and trans_err.max() <= 3e-3
+    else:
+        assert np.median(rot_err) <= 0.5 and np.median(trans_err) <= 1e-2
