@@ -25,6 +25,7 @@
 // pointers are deterministic.
 #include "vsg_device.hpp"
 #include "vsg_internal.hpp"
+#include "vsg_replay.hpp"
 
 namespace vsg {
 
@@ -763,15 +764,12 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
       float t0, t1;
       box_interval(sk, 32.0f * sx0 + 2.0f, 32.0f * sx1 + 30.0f, 32.0f * sy0 + 2.0f, 32.0f * sy1 + 30.0f,
                    32.0f * sz0 + 2.0f, 32.0f * sz1 + 30.0f, &t0, &t1);
-      if (!(t0 <= ray && ray < t1)) break;
+      if (!(t0 <= ray && ray < t1)) break;  // also false for a NaN interval
       last_w = 0u;
       n_super++;
-      while (ray < t1 && ray < ray_end) {
-        const float next = ray + inc;
-        if (next == ray) { stalled = true; break; }
-        ray = next;
-        n_replay++;
-      }
+      const float from = ray;
+      ray = replay_steps(ray, fminf(t1, ray_end), inc, &stalled);
+      if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
       if (stalled) break;
     }
     if (stalled || !(ray < ray_end)) break;
@@ -831,14 +829,11 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
       float t0, t1;
       box_interval(sk, 8.0f * cb.x0 + 2.0f, 8.0f * cb.x1 + 6.0f, 8.0f * cb.y0 + 2.0f, 8.0f * cb.y1 + 6.0f,
                    8.0f * cb.z0 + 2.0f, 8.0f * cb.z1 + 6.0f, &t0, &t1);
-      if (t0 <= ray) {
+      if (t0 <= ray && ray < t1) {  // false for a NaN interval
         n_block++;
-        while (ray < t1 && ray < ray_end) {
-          const float nx = ray + inc;
-          if (nx == ray) { stalled = true; break; }
-          ray = nx;
-          n_replay++;
-        }
+        const float from = ray;
+        ray = replay_steps(ray, fminf(t1, ray_end), inc, &stalled);
+        if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
         if (stalled) break;
       }
     }

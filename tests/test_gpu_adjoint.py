@@ -193,9 +193,11 @@ def test_autograd_function(torch_cuda):
 @pytest.mark.parametrize("sampling", ["trilinear", "reference"])
 def test_pose_refinement_recovers_perturbed_poses(torch_cuda, sampling):
     """Room 640x480: 32 frames fused at their poses; 8 held-out frames perturbed by up
-    to 2 deg / 2 cm are refined by <= 10 FP64 Gauss-Newton steps on the device.
+    to 2 deg / 2 cm are refined by <= 30 FP64 Gauss-Newton steps on the device, with
+    the residual window narrowed from 10 cm to 2 cm (a 2 deg error moves the image by
+    ~18 pixels, so a narrow window at the start keeps too few correspondences).
 
-    With the standard trilinear sampling the poses come back to <= 1 mm / 0.1 deg.
+    With the standard trilinear sampling every pose comes back to <= 1 mm / 0.1 deg.
     The reference's sampling returns the field half a voxel off (a wall renders +-v/2
     from where it was observed, depending on its offset to the voxel grid), so the
     alignment settles a few millimetres away; that bound is looser."""
@@ -225,10 +227,10 @@ def test_pose_refinement_recovers_perturbed_poses(torch_cuda, sampling):
         trans0.append(np.linalg.norm(xi[3:]))
         obs = torch.from_numpy(f["depth"].reshape(-1)).cuda()
         mn, mx = synth.depth_range(f["depth"])
-        for it in range(10):
-            res = 0.05 if it < 3 else 0.02
+        for it in range(30):
+            res = 0.10 if it < 5 else (0.05 if it < 10 else 0.02)
             T, st = g.render.gn_step(g.scene, obs, fx, fy, cx, cy, mn, mx, T, max_residual=res, lam=1e-3)
-            if st["rot"] < 1e-6 and st["trans"] < 1e-6:
+            if it >= 10 and st["rot"] < 1e-6 and st["trans"] < 1e-6:
                 break
         E = np.linalg.inv(Tgt) @ T
         rot_err.append(np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1))))
@@ -237,7 +239,6 @@ def test_pose_refinement_recovers_perturbed_poses(torch_cuda, sampling):
     print(f"pose refinement ({sampling}): start deg {np.round(rot0, 3)} mm {np.round(np.array(trans0) * 1e3, 2)}; "
           f"end deg {np.round(rot_err, 4)} mm {np.round(trans_err * 1e3, 3)}")
     if sampling == "trilinear":
-        assert np.median(rot_err) <= 0.1 and np.median(trans_err) <= 1e-3
-        assert rot_err.max() <= 0.2 and trans_err.max() <= 3e-3
+        assert rot_err.max() <= 0.1 and trans_err.max() <= 1e-3
     else:
         assert np.median(rot_err) <= 0.5 and np.median(trans_err) <= 1e-2
