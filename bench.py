@@ -174,6 +174,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-frames", type=int, default=30)
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="first run a plain copy kernel (torch.neg, float32) over this many MiB, 4 times: "
+                         "its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc scale the counters of the "
+                         "fusion kernels to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU: "
@@ -206,6 +210,16 @@ def main() -> int:
     poses = [f["pose"] for f in frames]
     ranges = [f["range"] for f in frames]
     torch.cuda.synchronize()
+
+    if args.calibration_copy_mib > 0:
+        n = args.calibration_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        log(f"calibration copy: 4 x {args.calibration_copy_mib} MiB read + {args.calibration_copy_mib} MiB written")
+        del src, dst
 
     stream = torch.cuda.Stream()
 

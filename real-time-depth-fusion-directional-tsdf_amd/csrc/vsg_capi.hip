@@ -151,7 +151,8 @@ void free_scene(vsg_scene* s) {
   (void)hipFree(s->t.cnt);
   (void)hipFree(s->t.claim);
   (void)hipFree(s->t.blk);
-  (void)hipFree(s->t.sb);
+  (void)hipFree(s->t.occ);
+  (void)hipFree(s->t.idx);
   (void)hipFree(s->vox);
   (void)hipFree(s->col);
   (void)hipFree(s->vis);
@@ -253,7 +254,18 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   if (r == VSG_OK) r = dmalloc(&s->t.cnt, B, "bucket counts");
   if (r == VSG_OK) r = dmalloc(&s->t.claim, B * sizeof(uint32_t), "bucket claims");
   if (r == VSG_OK) r = dmalloc(&s->t.blk, N * sizeof(int4), "block list");
-  if (r == VSG_OK) r = dmalloc(&s->t.sb, kSuperBits / 8, "super-block bitmap");
+  // occupancy map: one word per 4x4x4-block super-block hashed into a power of two
+  // of at least 2^16 words and at least one word per block slot, so that fewer
+  // super-blocks than words are ever occupied
+  size_t occ_words = 1u << 16;
+  while (occ_words < N) occ_words <<= 1;
+  s->t.occ_mask = (uint32_t)(occ_words - 1);
+  if (r == VSG_OK) r = dmalloc(&s->t.occ, occ_words * sizeof(uint64_t), "block occupancy map");
+  // block index: a power of two of at least 4 entries per block slot
+  size_t idx_entries = 1u << 16;
+  while (idx_entries < 4 * N) idx_entries <<= 1;
+  s->t.idx_mask = (uint32_t)(idx_entries - 1);
+  if (r == VSG_OK) r = dmalloc(&s->t.idx, idx_entries * sizeof(uint4), "block index");
   if (r == VSG_OK) r = dmalloc(&s->vox, N * kVox * s->word_bytes, "voxel store");
   if (r == VSG_OK) r = dmalloc(&s->col, N * kVox * sizeof(uint32_t), "colour store");
   if (r == VSG_OK) r = dmalloc(&s->vis, N * sizeof(uint32_t), "visible list");

@@ -20,7 +20,6 @@ constexpr int kKeyBits = 21;                           // per block coordinate i
 constexpr int32_t kKeyMin = -(1 << (kKeyBits - 1));
 constexpr int32_t kKeyMax = (1 << (kKeyBits - 1)) - 1;
 constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
-constexpr uint32_t kSuperBits = 1u << 24;              // super-block occupancy bitmap: 2 MiB
 
 // ---- views passed to kernels by value -------------------------------------------
 
@@ -29,17 +28,29 @@ constexpr uint32_t kSuperBits = 1u << 24;              // super-block occupancy 
 // number of used slots; slots fill in order and are never freed, so the used slots
 // are always slots [0, cnt[b]).  claim[b] is the per-frame insertion claim word.
 // blk[p] = (x, y, z, hash entry index) of the block with pointer p.
-// sb is a hashed occupancy bitmap of 4x4x4-block super-blocks (bit set when any
-// block of the super-block is allocated; collisions only set extra bits).
+// occ is the block occupancy map: one 64-bit word per super-block of 4x4x4 blocks,
+// hashed into occ_mask + 1 words; bit occ_bit(x, y, z) of the word of block
+// (x, y, z)'s super-block is set once the block is allocated.  A hash collision
+// only sets extra bits, so a clear bit proves the block absent.
+// idx is the block index the raycaster reads: an open-addressing table of
+// idx_mask + 1 16-byte entries {packed key lo, hi, block pointer, 0} (linear
+// probing, load factor <= 1/4, empty key all ones), filled when a block is
+// inserted.  One probe usually answers a lookup in one memory round trip, where the
+// bucket records take the count, the keys and the pointer.
 struct TableView {
   uint64_t* rec;
   uint8_t* cnt;
   uint32_t* claim;
   int4* blk;
-  uint32_t* sb;
+  uint64_t* occ;
+  uint4* idx;
   uint32_t num_buckets;
   uint32_t num_blocks;
+  uint32_t occ_mask;
+  uint32_t idx_mask;
 };
+
+constexpr uint32_t kIdxEmpty = 0xFFFFFFFFu;  // both key words of an empty index entry
 
 // Per-frame counter set (two sets alternate between frames so that each frame's
 // scan kernel can clear the next frame's set without an extra launch).
@@ -129,32 +140,68 @@ __device__ __forceinline__ float3 xform_point(const float* M, float x, float y, 
                      M[8] * x + M[9] * y + M[10] * z + M[11]);
 }
 
-// Pointer of block (x,y,z), or -1.  Probes only the used slots of the home bucket:
+// Pointer of block (x,y,z), or -1.  Compares the used slots of the home bucket:
 // keys are unique per bucket, so this is the reference's 10-slot scan
-// (RayCastRender.cu:81-111) without reading free slots.
+// (RayCastRender.cu:81-111).  The slot count and the ten keys are read together
+// (then the matching pointer): two round trips to memory instead of three
+// dependent ones.  Slots at or beyond the count are ignored whatever they hold.
 __device__ __forceinline__ int32_t table_lookup(const TableView& t, int32_t x, int32_t y, int32_t z) {
   if (!key_in_range(x, y, z)) return -1;
   const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
-  const int c = t.cnt[h];
-  if (c == 0) return -1;
-  const uint64_t key = pack_key(x, y, z);
   const uint64_t* r = t.rec + (size_t)h * kRecWords;
+  const uint4* r4 = reinterpret_cast<const uint4*>(r);
+  uint4 w[5];
+#pragma unroll
+  for (int i = 0; i < 5; ++i) w[i] = r4[i];
+  const int c = t.cnt[h];
+  const uint64_t key = pack_key(x, y, z);
+  const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
   int hit = -1;
 #pragma unroll
   for (int j = 0; j < kBucket; ++j) {
-    if (j < c && r[j] == key) hit = j;
+    const uint32_t lo = (j & 1) ? w[j >> 1].z : w[j >> 1].x, hi = (j & 1) ? w[j >> 1].w : w[j >> 1].y;
+    if (j < c && lo == klo && hi == khi) hit = j;
   }
   if (hit < 0) return -1;
   return reinterpret_cast<const int32_t*>(r + kBucket)[hit];
 }
 
-// Bit of super-block (sx, sy, sz) (4x4x4 blocks) in the occupancy bitmap.
-__device__ __forceinline__ uint32_t sb_bit(int32_t sx, int32_t sy, int32_t sz) {
+// Occupancy map word of super-block (sx, sy, sz) and bit of block (x, y, z) in the
+// word of its super-block (x >> 2, y >> 2, z >> 2).
+__device__ __forceinline__ uint32_t occ_index(int32_t sx, int32_t sy, int32_t sz, uint32_t mask) {
   uint32_t k = (uint32_t)sx * 0x9E3779B1u ^ (uint32_t)sy * 0x85EBCA77u ^ (uint32_t)sz * 0xC2B2AE3Du;
   k ^= k >> 15;
   k *= 0x2C1B3C6Du;
   k ^= k >> 12;
-  return k & (kSuperBits - 1u);
+  return k & mask;
+}
+
+__device__ __forceinline__ uint32_t occ_bit(int32_t x, int32_t y, int32_t z) {
+  return (uint32_t)(x & 3) | ((uint32_t)(y & 3) << 2) | ((uint32_t)(z & 3) << 4);
+}
+
+// Home slot of block (x, y, z) in the block index.
+__device__ __forceinline__ uint32_t idx_home(int32_t x, int32_t y, int32_t z, uint32_t mask) {
+  uint32_t k = (uint32_t)x * 0x85EBCA77u ^ (uint32_t)y * 0xC2B2AE3Du ^ (uint32_t)z * 0x27D4EB2Fu;
+  k ^= k >> 16;
+  k *= 0x7FEB352Du;
+  k ^= k >> 15;
+  return k & mask;
+}
+
+// Pointer of block (x, y, z) from the block index, or -1.
+__device__ __forceinline__ int32_t index_lookup(const TableView& t, int32_t x, int32_t y, int32_t z) {
+  if (!key_in_range(x, y, z)) return -1;
+  const uint64_t key = pack_key(x, y, z);
+  const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
+  uint32_t s = idx_home(x, y, z, t.idx_mask);
+  for (uint32_t n = 0; n <= t.idx_mask; ++n) {
+    const uint4 e = t.idx[s];
+    if (e.x == klo && e.y == khi) return (int32_t)e.z;
+    if (e.x == kIdxEmpty && e.y == kIdxEmpty) return -1;
+    s = (s + 1u) & t.idx_mask;
+  }
+  return -1;
 }
 
 // ---- ordered float encoding for atomicMin/atomicMax ----------------------------------

@@ -298,8 +298,20 @@ __global__ __launch_bounds__(256) void k_alloc_insert(TableView t, DevState* st,
     reinterpret_cast<int32_t*>(r + kBucket)[slot] = (int32_t)p;
     t.cnt[h] = (uint8_t)(slot + 1);
     t.blk[p] = make_int4(x, y, z, (int32_t)(h * kBucket + slot));
-    const uint32_t sbit = sb_bit(x >> 2, y >> 2, z >> 2);  // super-block of 4x4x4 blocks
-    atomicOr(&t.sb[sbit >> 5], 1u << (sbit & 31u));
+    atomicOr(reinterpret_cast<unsigned long long*>(&t.occ[occ_index(x >> 2, y >> 2, z >> 2, t.occ_mask)]),
+             1ull << occ_bit(x, y, z));
+    // block index: claim the first empty entry from the home slot (keys are unique,
+    // so no entry can already hold this key)
+    const uint64_t key = pack_key(x, y, z);
+    uint32_t sl = idx_home(x, y, z, t.idx_mask);
+    for (uint32_t n = 0; n <= t.idx_mask; ++n) {
+      unsigned long long* kw = reinterpret_cast<unsigned long long*>(&t.idx[sl]);
+      if (atomicCAS(kw, 0xFFFFFFFFFFFFFFFFull, (unsigned long long)key) == 0xFFFFFFFFFFFFFFFFull) {
+        t.idx[sl].z = (uint32_t)p;
+        break;
+      }
+      sl = (sl + 1u) & t.idx_mask;
+    }
   });
 }
 
@@ -485,53 +497,105 @@ __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, Fr
 // Raycast (RayCastRender.cu:127-517)
 // ---------------------------------------------------------------------------------
 
-// Two most recent block lookups of this ray.  DIAG marks every bucket with live
-// entries that a lookup probes and every block it finds (byte-model counters).
+// Block pointers of the 2x2x2 blocks base + {0,1}^3 around the ray's current
+// sample, looked up on demand (bit i of `known`: block base + (i&1, i>>1&1, i>>2&1)
+// is in p[i]).  When the base moves by one block along an axis the entries that
+// stay in the neighbourhood are kept, so a ray looks up each block it uses about
+// once.  DIAG marks every bucket with live entries that the reference's lookup of a
+// used block probes and every block found (byte-model counters).
 template <bool DIAG> struct BlockCache {
-  int32_t x0, y0, z0, p0;
-  int32_t x1, y1, z1, p1;
+  int32_t bx, by, bz;
+  int32_t p[8];
+  uint32_t known;
   uint8_t* mark_bkt;
   uint8_t* mark_blk;
   uint32_t* stats;  // march statistics (evaluations, super-block skips, block skips, replayed steps)
   __device__ void init(uint8_t* mb, uint8_t* mk, uint32_t* st = nullptr) {
-    x0 = y0 = z0 = INT32_MIN;
-    x1 = y1 = z1 = INT32_MIN;
-    p0 = p1 = -1;
+    bx = by = bz = INT32_MIN;
+    known = 0u;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) p[i] = -1;
     mark_bkt = mb;
     mark_blk = mk;
     stats = st;
   }
   __device__ __forceinline__ int32_t get(const TableView& t, int32_t x, int32_t y, int32_t z) {
-    if (x == x0 && y == y0 && z == z0) return p0;
-    if (x == x1 && y == y1 && z == z1) return p1;
-    const int32_t p = table_lookup(t, x, y, z);
+    const int32_t q = index_lookup(t, x, y, z);
     if (DIAG) {
       if (key_in_range(x, y, z)) {
         const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
         if (t.cnt[h] > 0) mark_bkt[h] = 1;
       }
-      if (p >= 0) mark_blk[p] = 1;
+      if (q >= 0) mark_blk[q] = 1;
     }
-    x1 = x0; y1 = y0; z1 = z0; p1 = p0;
-    x0 = x; y0 = y; z0 = z; p0 = p;
-    return p;
+    return q;
+  }
+  // Moves the base by d in {-1, 0, 1} along the axis of stride `bit` (1, 2, 4).
+  __device__ __forceinline__ void shift(int32_t d, int bit, uint32_t lo_mask) {
+    if (d == 1) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i)
+        if (!(i & bit)) p[i] = p[i | bit];
+      known = (known >> bit) & lo_mask;
+    } else if (d == -1) {
+#pragma unroll
+      for (int i = 7; i >= 0; --i)
+        if (i & bit) p[i] = p[i & ~bit];
+      known = (known << bit) & ~lo_mask & 0xFFu;
+    }
+  }
+  // Pointers of the eight corner blocks of a sample whose corners lie in blocks
+  // {x0, x0 + sx} x {y0, y0 + sy} x {z0, z0 + sz}; corner i takes x from bit 0, y from
+  // bit 1, z from bit 2.
+  __device__ __forceinline__ void corners(const TableView& t, int32_t x0, int32_t y0, int32_t z0, bool sx, bool sy,
+                                          bool sz, int32_t out[8]) {
+    const int32_t dx = (int32_t)((uint32_t)x0 - (uint32_t)bx), dy = (int32_t)((uint32_t)y0 - (uint32_t)by),
+                  dz = (int32_t)((uint32_t)z0 - (uint32_t)bz);
+    if ((dx | dy | dz) != 0) {
+      if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
+        shift(dx, 1, 0x55u);
+        shift(dy, 2, 0x33u);
+        shift(dz, 4, 0x0Fu);
+      } else {
+        known = 0u;
+      }
+      bx = x0;
+      by = y0;
+      bz = z0;
+    }
+    const uint32_t m = (sx ? 1u : 0u) | (sy ? 2u : 0u) | (sz ? 4u : 0u);
+    uint32_t need = 0u;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (((uint32_t)i & ~m) == 0u) need |= 1u << i;
+    uint32_t missing = need & ~known;
+    while (missing) {
+      const int i = __builtin_ctz(missing);
+      missing &= missing - 1u;
+      const int32_t q = get(t, x0 + (i & 1), y0 + ((i >> 1) & 1), z0 + ((i >> 2) & 1));
+#pragma unroll
+      for (int k = 0; k < 8; ++k) p[k] = (k == i) ? q : p[k];
+      known |= 1u << i;
+    }
+    // corner i uses block i & m
+    int32_t a[8], b[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) a[i] = sx ? p[i] : p[i & ~1];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) b[i] = sy ? a[i] : a[i & ~2];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[i] = sz ? b[i] : b[i & ~4];
   }
 };
 
 // Sample status of a trilinear evaluation.
-enum : int { kSampleEmpty = 0, kSampleInvalid = 1, kSampleValid = 2 };
-
-// Block range covered by the eight corners of the last sample (per axis lo/hi).
-struct CornerBlocks {
-  int32_t x0, x1, y0, y1, z0, z1;
-};
+enum : int { kSampleInvalid = 1, kSampleValid = 2 };
 
 // trilinearInterpolationSimpleFastFast (RayCastRender.cu:127-215).  The eight
 // corners' voxel indices come from two quotients per axis (posDual/v and
 // (posDual+v)/v): the reference computes the same quotient once per corner.
-// Returns kSampleValid with *dist (and *rgb), kSampleInvalid when a corner block
-// exists but every corner voxel has weight 0, kSampleEmpty when no corner block
-// exists (then *cb holds the corner blocks' range).
+// Returns kSampleValid with *dist (and *rgb), kSampleInvalid when no corner voxel
+// has a weight (the reference's sum of weights <= 1e-6).
 // Corner voxels and weights of a sample.  REF: the reference's scheme (weights
 // frac(p/v - 1/2), corners rounded from p - v/2 and p + v/2: RayCastRender.cu:139-176),
 // which samples the stored field half a voxel off.  Otherwise standard trilinear
@@ -572,44 +636,45 @@ __device__ __forceinline__ void corner_setup(float px, float py, float pz, float
 template <bool F16, bool COLOR, bool REF, class Cache>
 __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const void* __restrict__ vox_,
                                          const uint32_t* __restrict__ col, float px, float py, float pz, float v,
-                                         float* dist, uint32_t* rgb, CornerBlocks* cb) {
+                                         float* dist, uint32_t* rgb) {
   using S = Store<F16>;
   const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
   int32_t vx[2], vy[2], vz[2];
   float wx, wy, wz;
   corner_setup<REF>(px, py, pz, v, vx, vy, vz, &wx, &wy, &wz);
-  const int32_t bx[2] = {voxel_to_block_axis(vx[0]), voxel_to_block_axis(vx[1])};
-  const int32_t by[2] = {voxel_to_block_axis(vy[0]), voxel_to_block_axis(vy[1])};
-  const int32_t bz[2] = {voxel_to_block_axis(vz[0]), voxel_to_block_axis(vz[1])};
+  // block (floor division by 8) and offset in the block per axis and corner
+  const int32_t bx0 = vx[0] >> 3, bx1 = vx[1] >> 3, by0 = vy[0] >> 3, by1 = vy[1] >> 3;
+  const int32_t bz0 = vz[0] >> 3, bz1 = vz[1] >> 3;
+  const uint32_t ox[2] = {(uint32_t)vx[0] & 7u, (uint32_t)vx[1] & 7u};
+  const uint32_t oy[2] = {((uint32_t)vy[0] & 7u) << 3, ((uint32_t)vy[1] & 7u) << 3};
+  const uint32_t oz[2] = {((uint32_t)vz[0] & 7u) << 6, ((uint32_t)vz[1] & 7u) << 6};
+  int32_t p[8];
+  bc.corners(t, bx0, by0, bz0, bx1 != bx0, by1 != by0, bz1 != bz0, p);
+  // Corners without an allocated block or with weight 0 contribute a zero weight:
+  // adding +-0 leaves every sum unchanged (a sum started at +0 never becomes -0),
+  // so this equals skipping them.
+  const float ux = 1.0f - wx, uy = 1.0f - wy, uz = 1.0f - wz;
   float d = 0.0f, cr = 0.0f, cg = 0.0f, cbl = 0.0f, wsum = 0.0f;
-  bool any_block = false;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const int32_t ix = vx[i & 1], iy = vy[(i >> 1) & 1], iz = vz[(i >> 2) & 1];
-    const int32_t p = bc.get(t, bx[i & 1], by[(i >> 1) & 1], bz[(i >> 2) & 1]);
-    if (p < 0) continue;
-    any_block = true;
-    const size_t vi = (size_t)p * kVox +
-                      (size_t)(voxel_local_axis(ix) + voxel_local_axis(iy) * kBlock + voxel_local_axis(iz) * kBlock * kBlock);
-    const typename S::word w = vox[vi];
-    if (S::weight(w) == 0u) continue;
-    const float wi = ((i & 1) ? wx : (1.0f - wx)) * ((i & 2) ? wy : (1.0f - wy)) * ((i & 4) ? wz : (1.0f - wz));
-    d += wi * S::sdf(w);
-    if (COLOR) {
-      const uint32_t c = col[vi];
-      cr += wi * (float)(c & 0xFFu);
-      cg += wi * (float)((c >> 8) & 0xFFu);
-      cbl += wi * (float)((c >> 16) & 0xFFu);
+    const uint32_t vi = (uint32_t)p[i] * (uint32_t)kVox + ox[i & 1] + oy[(i >> 1) & 1] + oz[(i >> 2) & 1];
+    typename S::word w{};
+    uint32_t c = 0u;
+    if (p[i] >= 0) {
+      w = vox[vi];
+      if (COLOR) c = col[vi];
     }
-    wsum += wi;
+    const float wi = ((i & 1) ? wx : ux) * ((i & 2) ? wy : uy) * ((i & 4) ? wz : uz);
+    const float wk = S::weight(w) != 0u ? wi : 0.0f;
+    d += wk * S::sdf(w);
+    if (COLOR) {
+      cr += wk * (float)(c & 0xFFu);
+      cg += wk * (float)((c >> 8) & 0xFFu);
+      cbl += wk * (float)((c >> 16) & 0xFFu);
+    }
+    wsum += wk;
   }
-  if (wsum <= 1e-6f) {
-    if (any_block) return kSampleInvalid;
-    cb->x0 = bx[0]; cb->x1 = bx[1];
-    cb->y0 = by[0]; cb->y1 = by[1];
-    cb->z0 = bz[0]; cb->z1 = bz[1];
-    return kSampleEmpty;
-  }
+  if (wsum <= 1e-6f) return kSampleInvalid;
   const float inv = 1.0f / wsum;
   *dist = d * inv;
   if (COLOR) {
@@ -624,15 +689,16 @@ __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const vo
 // ---- exact empty-space skipping ---------------------------------------------------
 //
 // A sample is invalid whenever none of its eight corner voxels lies in an allocated
-// block.  The corner voxel indices of a sample at p are r(q) and r(q+1) per axis
-// with q = (p - v/2)/v and r(q) in (q - 1.5, q + 0.5], so they lie in
-// (p/v - 2, p/v + 1].  While p/v stays inside [8*b0 + 2, 8*b1 + 6] on every axis,
-// all corners fall in blocks b0..b1; if those blocks are all unallocated, every
-// such sample is invalid and the march only has to replay its step additions
-// (rayCurrent += v/2) to reach the same sample positions as the reference.
-// The same holds for a 4x4x4-block super-block with no allocated block, read
-// from a hashed occupancy bitmap (a hash collision only makes it look occupied).
-// Boxes are shrunk by a margin covering the float rounding of p and q.
+// block.  Per axis, the corner voxel indices of a sample at p lie in (Q - 2, Q + 1]
+// with Q = p/v (reference sampling: r(q) and r(q + 1) with q = Q - 1/2 and r(q) in
+// (q - 3/2, q + 1/2]; standard trilinear: floor(Q) and floor(Q) + 1).  While Q stays
+// inside [8*b0 + 1, 8*b1 + 7) on every axis, all corners fall in blocks b0..b1; if
+// none of those blocks is allocated, every such sample is invalid and the march only
+// has to replay its step additions (rayCurrent += v/2) to land on the same sample
+// positions as the reference.  The same holds for 32-voxel super-blocks s0..s1 and
+// [32*s0 + 1, 32*s1 + 31).  Both tests read the block occupancy map (a hash collision
+// only makes a block look allocated); boxes are shrunk by a margin covering the float
+// rounding of p and Q.
 
 struct RaySkip {
   float camx, camy, camz;
@@ -664,9 +730,46 @@ __device__ __forceinline__ void box_interval(const RaySkip& k, float lox, float 
   *t1 = tf - fabsf(tf) * 1e-5f - 1e-7f;
 }
 
-__device__ __forceinline__ bool sb_occupied(const uint32_t* __restrict__ sb, int32_t sx, int32_t sy, int32_t sz) {
-  const uint32_t b = sb_bit(sx, sy, sz);
-  return (sb[b >> 5] >> (b & 31u)) & 1u;
+// Occupancy of the blocks around a sample: the range [bx0, bx1] x [by0, by1] x
+// [bz0, bz1] (one or two blocks per axis) and the super-blocks holding it.  The lane
+// keeps the last occupancy word it read (consecutive samples mostly share it).
+struct OccCache {
+  uint32_t idx;
+  uint64_t word;
+};
+
+__device__ __forceinline__ uint64_t occ_word(const TableView& t, OccCache& oc, int32_t sx, int32_t sy, int32_t sz) {
+  const uint32_t wi = occ_index(sx, sy, sz, t.occ_mask);
+  if (wi != oc.idx) {
+    oc.idx = wi;
+    oc.word = t.occ[wi];
+  }
+  return oc.word;
+}
+
+// 0: no block of the super-blocks is allocated; 1: some block of the super-blocks
+// but none of the range is; 2: a block of the range is allocated.
+__device__ __forceinline__ int occ_test(const TableView& t, OccCache& oc, int32_t bx0, int32_t bx1, int32_t by0,
+                                        int32_t by1, int32_t bz0, int32_t bz1) {
+  uint64_t any_word = 0ull, any_bit = 0ull;
+  if ((bx0 >> 2) == (bx1 >> 2) && (by0 >> 2) == (by1 >> 2) && (bz0 >> 2) == (bz1 >> 2)) {
+    const uint64_t w = occ_word(t, oc, bx0 >> 2, by0 >> 2, bz0 >> 2);
+    // bits x + 4y + 16z of the range's blocks (local coordinates in the super-block)
+    const uint32_t mx = (1u << (bx0 & 3)) | (1u << (bx1 & 3));
+    const uint32_t mxy = (mx << ((by0 & 3) << 2)) | (mx << ((by1 & 3) << 2));
+    const uint64_t m = ((uint64_t)mxy << ((bz0 & 3) << 4)) | ((uint64_t)mxy << ((bz1 & 3) << 4));
+    any_word = w;
+    any_bit = w & m;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int32_t x = (i & 1) ? bx1 : bx0, y = (i & 2) ? by1 : by0, z = (i & 4) ? bz1 : bz0;
+      const uint64_t w = occ_word(t, oc, x >> 2, y >> 2, z >> 2);
+      any_word |= w;
+      any_bit |= (w >> occ_bit(x, y, z)) & 1ull;
+    }
+  }
+  return any_bit ? 2 : (any_word ? 1 : 0);
 }
 
 // One ray of renderKernel (RayCastRender.cu:407-517) with the march of
@@ -741,56 +844,97 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
     const float mag = fmaxf(fmaxf(fabsf(camx), fabsf(camy)), fabsf(camz)) + fabsf(ray_end);
     sk.eps = 0.02f + 8e-6f * mag * sk.inv_v;
   }
-  // Each lane first takes every skip its super-block test allows, then the wave
-  // evaluates one sample per lane together (lanes stay in phase for the expensive part).
-  bool try_super = true, stalled = false;
+  // One step per lane and iteration: a skip over samples the occupancy map proves
+  // invalid, or the evaluation of one sample.  Lanes never wait for each other's
+  // skips, so the wave runs max(steps) iterations rather than the sum of the lanes'
+  // skip runs.  A sign change that passes the reference's thresholds parks the lane
+  // with its bracket; once no lane of the wave is still marching, the parked lanes
+  // refine together (three evaluations each), so the refinement runs once per wave
+  // instead of once per lane.  A failed refinement resumes the march exactly where
+  // the reference continues (RayCastRender.cu:372-391).
+  bool stalled = false;
   uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0;
+  OccCache oc;
+  oc.idx = 0xFFFFFFFFu;
+  oc.word = 0ull;
+  const float mq = 1.125f + sk.eps;  // range margin around Q (voxels)
+  float t_busy = -INFINITY;           // samples before this have an allocated block in range
+  bool marching = ray < ray_end, parked = false;
   while (true) {
-    while (try_super && ray < ray_end) {
-      // the corner support (q-2, q+1] of the sample at ray spans 1-8 super-blocks of
-      // 32 voxels; if none holds an allocated block, every sample whose support stays
-      // inside them is invalid
-      const float qx = (camx + ray * wdx) * sk.inv_v, qy = (camy + ray * wdy) * sk.inv_v,
-                  qz = (camz + ray * wdz) * sk.inv_v;
-      const float m = 2.0f + 0.25f;
-      const int32_t sx0 = (int32_t)floorf((qx - m) * (1.0f / 32.0f)), sx1 = (int32_t)floorf((qx + m) * (1.0f / 32.0f));
-      const int32_t sy0 = (int32_t)floorf((qy - m) * (1.0f / 32.0f)), sy1 = (int32_t)floorf((qy + m) * (1.0f / 32.0f));
-      const int32_t sz0 = (int32_t)floorf((qz - m) * (1.0f / 32.0f)), sz1 = (int32_t)floorf((qz + m) * (1.0f / 32.0f));
-      bool empty = (sx1 - sx0) <= 1 && (sy1 - sy0) <= 1 && (sz1 - sz0) <= 1;
-      for (int32_t a = sx0; empty && a <= sx1; ++a)
-        for (int32_t b = sy0; empty && b <= sy1; ++b)
-          for (int32_t c = sz0; empty && c <= sz1; ++c) empty = !sb_occupied(t.sb, a, b, c);
-      if (!empty) break;
-      float t0, t1;
-      box_interval(sk, 32.0f * sx0 + 2.0f, 32.0f * sx1 + 30.0f, 32.0f * sy0 + 2.0f, 32.0f * sy1 + 30.0f,
-                   32.0f * sz0 + 2.0f, 32.0f * sz1 + 30.0f, &t0, &t1);
-      if (!(t0 <= ray && ray < t1)) break;  // also false for a NaN interval
-      last_w = 0u;
-      n_super++;
-      const float from = ray;
-      ray = replay_steps(ray, fminf(t1, ray_end), inc, &stalled);
-      if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
-      if (stalled) break;
+    if (!__any(marching || parked)) break;
+    if (marching) {
+      bool step = true;
+      if (!(ray < t_busy)) {
+        const float qx = (camx + ray * wdx) * sk.inv_v, qy = (camy + ray * wdy) * sk.inv_v,
+                    qz = (camz + ray * wdz) * sk.inv_v;
+        const int32_t bx0 = cvt_i32(floorf((qx - mq) * 0.125f)), bx1 = cvt_i32(floorf((qx + mq) * 0.125f));
+        const int32_t by0 = cvt_i32(floorf((qy - mq) * 0.125f)), by1 = cvt_i32(floorf((qy + mq) * 0.125f));
+        const int32_t bz0 = cvt_i32(floorf((qz - mq) * 0.125f)), bz1 = cvt_i32(floorf((qz + mq) * 0.125f));
+        const int occ = occ_test(t, oc, bx0, bx1, by0, by1, bz0, bz1);
+        // occ 0: the range's super-blocks are empty (skip their box); occ 1: the
+        // range's blocks are (skip its box); occ 2: evaluate every sample until the
+        // ray leaves the range's box
+        const float g = occ == 0 ? 32.0f : 8.0f;
+        const int32_t sh = occ == 0 ? 2 : 0;
+        const float lo = 1.0f, hi = occ == 0 ? 31.0f : 7.0f;
+        float t0, t1;
+        box_interval(sk, g * (float)(bx0 >> sh) + lo, g * (float)(bx1 >> sh) + hi, g * (float)(by0 >> sh) + lo,
+                     g * (float)(by1 >> sh) + hi, g * (float)(bz0 >> sh) + lo, g * (float)(bz1 >> sh) + hi, &t0, &t1);
+        const bool inside = t0 <= ray && ray < t1;  // false for a NaN interval
+        if (occ == 2) {
+          t_busy = inside ? t1 : -INFINITY;
+        } else if (inside) {
+          if (occ == 0) n_super++;
+          else n_block++;
+          last_w = 0u;
+          const float from = ray;
+          ray = replay_steps(ray, fminf(t1, ray_end), inc, &stalled);
+          if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
+          if (stalled || !(ray < ray_end)) marching = false;
+          step = false;
+        }
+      }
+      if (step) {
+        const float ppx = camx + ray * wdx, ppy = camy + ray * wdy, ppz = camz + ray * wdz;
+        float dist;
+        const int stat = trilinear<F16, false, REF>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr);
+        n_eval++;
+        if (stat == kSampleValid) {
+          // findIntersectionBisection's bracket and the acceptance thresholds
+          // (RayCastRender.cu:372-385); the refinement itself comes later
+          if (last_w > 0u && last_sdf < 0.0f && dist > 0.0f && fabsf(last_sdf - dist) < th && fabsf(dist) < th) {
+            h.a = last_alpha;
+            h.ad = last_sdf;
+            h.b = ray;
+            h.bd = dist;
+            parked = true;
+            marching = false;
+          }
+          last_sdf = dist;
+          last_alpha = ray;
+          last_w = 1u;
+        } else {
+          last_w = 0u;
+        }
+        if (marching) {
+          const float next = ray + inc;
+          if (next == ray) marching = false;  // step absorbed by rounding: the reference would never end
+          ray = next;
+          if (!(ray < ray_end)) marching = false;
+        }
+      }
     }
-    if (stalled || !(ray < ray_end)) break;
-    const float ppx = camx + ray * wdx, ppy = camy + ray * wdy, ppz = camz + ray * wdz;
-    float dist;
-    CornerBlocks cb;
-    const int stat = trilinear<F16, false, REF>(t, bc, vox, col, ppx, ppy, ppz, v, &dist, nullptr, &cb);
-    n_eval++;
-    if (stat == kSampleValid) {
-      try_super = false;
-      if (last_w > 0u && last_sdf < 0.0f && dist > 0.0f) {
+    if (!__any(marching)) {
+      if (parked) {
         // findIntersectionBisection (RayCastRender.cu:229-271)
-        float a = last_alpha, ad = last_sdf, bb = ray, bd = dist, c = 0.0f;
+        float a = h.a, ad = h.ad, bb = h.b, bd = h.bd, c = 0.0f;
         uint32_t rgb = 0;
         bool ok = true;
         for (int it = 0; it < 3; ++it) {
           c = a + (ad / (ad - bd)) * (bb - a);
           float cd;
-          CornerBlocks cb2;
           if (trilinear<F16, true, REF>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd,
-                                        &rgb, &cb2) != kSampleValid) {
+                                        &rgb) != kSampleValid) {
             ok = false;
             break;
           }
@@ -802,39 +946,17 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
             bd = cd;
           }
         }
-        if (ok && fabsf(last_sdf - dist) < th && fabsf(dist) < th) {
+        parked = false;
+        if (ok) {
           h.found = true;
           h.alpha = c;
-          h.a = last_alpha;
-          h.ad = last_sdf;
-          h.b = ray;
-          h.bd = dist;
           h.rgb = rgb;
-          break;
+        } else {
+          // the reference keeps marching after the rejected crossing
+          const float next = h.b + inc;
+          ray = next;
+          marching = next != h.b && next < ray_end;
         }
-      }
-      last_sdf = dist;
-      last_alpha = ray;
-      last_w = 1u;
-    } else {
-      last_w = 0u;
-    }
-    const float next = ray + inc;
-    if (next == ray) break;  // step absorbed by rounding: the reference would never end
-    ray = next;
-    if (stat == kSampleEmpty) {
-      // every corner block of this sample is unallocated: skip the samples whose
-      // corners stay inside those blocks
-      try_super = true;
-      float t0, t1;
-      box_interval(sk, 8.0f * cb.x0 + 2.0f, 8.0f * cb.x1 + 6.0f, 8.0f * cb.y0 + 2.0f, 8.0f * cb.y1 + 6.0f,
-                   8.0f * cb.z0 + 2.0f, 8.0f * cb.z1 + 6.0f, &t0, &t1);
-      if (t0 <= ray && ray < t1) {  // false for a NaN interval
-        n_block++;
-        const float from = ray;
-        ray = replay_steps(ray, fminf(t1, ray_end), inc, &stalled);
-        if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
-        if (stalled) break;
       }
     }
   }
@@ -848,7 +970,7 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
 }
 
 template <bool F16, bool DIAG, bool REF>
-__global__ __launch_bounds__(64) void k_render(TableView t, const DevState* __restrict__ st, RenderArgs ra,
+__global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                float* __restrict__ out_depth, float4* __restrict__ out_pos,
                                                float4* __restrict__ out_col, float4* __restrict__ out_nrm,
@@ -1033,8 +1155,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
         if (it == 2) break;
         float fcd;
         uint32_t rgb;
-        CornerBlocks cb;
-        trilinear<F16, false, REF>(t, bc, vox, col, px(fc), py(fc), pz(fc), v, &fcd, &rgb, &cb);
+        trilinear<F16, false, REF>(t, bc, vox, col, px(fc), py(fc), pz(fc), v, &fcd, &rgb);
         trilinear_grad<F16, REF>(t, bc, vox, px(fc), py(fc), pz(fc), v, fcd, &tg[2 + it]);
         const Dual10 cd = eval_dual(tg[2 + it], fcd, c, h, ra.M, 2 + it);
         if (fad * fcd > 0.0f) {
@@ -1254,7 +1375,8 @@ __global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) 
     t.cnt[b] = 0;
     t.claim[b] = kNoClaim;
   }
-  for (size_t w = tid; w < kSuperBits / 32u; w += stride) t.sb[w] = 0u;
+  for (size_t w = tid; w <= t.occ_mask; w += stride) t.occ[w] = 0ull;
+  for (size_t w = tid; w <= t.idx_mask; w += stride) t.idx[w] = make_uint4(kIdxEmpty, kIdxEmpty, 0u, 0u);
   if (tid == 0) {
     st->n_alloc = 0;
     st->n_alloc_prev = 0;

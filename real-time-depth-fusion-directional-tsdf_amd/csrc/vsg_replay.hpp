@@ -18,35 +18,47 @@
 
 namespace vsg {
 
+// ceil(num / den) and floor(num / den) for num, den < 2^24, den > 0: the float
+// quotient is within one of the true one, then corrected exactly in integers.
+__host__ __device__ inline uint32_t ceil_div24(uint32_t num, uint32_t den) {
+  uint32_t k = (uint32_t)((float)num / (float)den);
+  if (k * den < num) ++k;
+  if (k > 0u && (k - 1u) * den >= num) --k;
+  return k;
+}
+
+__host__ __device__ inline uint32_t floor_div24(uint32_t num, uint32_t den) {
+  uint32_t k = (uint32_t)((float)num / (float)den);
+  if (k * den > num) --k;
+  if ((k + 1u) * den <= num) ++k;
+  return k;
+}
+
 // First element >= target of r, fl(r+inc), fl(fl(r+inc)+inc), ... (r itself when
 // r >= target).  *stalled is set when an addition leaves the value unchanged (the
 // reference's loop would never end there).
 __host__ __device__ inline float replay_steps(float r, float target, float inc, bool* stalled) {
   *stalled = false;
   for (int guard = 0; guard < 4096 && r < target; ++guard) {
-    if (r > 0.0f && inc > 0.0f && r >= 1.17549435e-38f) {
+    if (r >= 1.17549435e-38f && inc > 0.0f) {  // r a positive normal number
       int e;
-      (void)frexpf(r, &e);                  // r = m * 2^e, m in [0.5, 1)
-      const float ulp = ldexpf(1.0f, e - 24);
-      const float qi = inc / ulp;           // exact: scaling by a power of two
+      (void)frexpf(r, &e);                    // r = m * 2^e, m in [0.5, 1): ulp = 2^(e-24)
+      const int sc = 24 - e;                  // scaling by 1/ulp is exact (a power of two)
+      const float qi = ldexpf(inc, sc);       // inc / ulp (inf if it overflows)
       const float I = floorf(qi);
       const float phi = qi - I;
       if (phi != 0.5f && I < 16777216.0f) {
         const uint32_t S = (uint32_t)I + (phi > 0.5f ? 1u : 0u);
-        const uint32_t R = (uint32_t)(r / ulp);  // exact integer in [2^23, 2^24)
+        const uint32_t R = (uint32_t)ldexpf(r, sc);  // exact integer in [2^23, 2^24)
         if (S > 0u && R + S < 16777216u) {
-          const float G = target / ulp;          // >= R; may exceed the binade
-          const uint32_t last = (16777215u - R) / S;  // steps that stay in the binade
-          uint32_t k;
+          const float G = ldexpf(target, sc);        // > R; may exceed the binade
           if (G <= 16777215.0f) {
-            const uint32_t Gi = (uint32_t)ceilf(G);
-            k = (Gi - R + S - 1u) / S;
-            if (k <= last) return (float)(R + k * S) * ulp;
+            const uint32_t k = ceil_div24((uint32_t)ceilf(G) - R, S);
+            if (R + k * S < 16777216u) return ldexpf((float)(R + k * S), -sc);
           }
-          k = last;
-          r = (float)(R + k * S) * ulp;
+          // the target lies beyond the binade: its last element, then one real addition
+          r = ldexpf((float)(R + floor_div24(16777215u - R, S) * S), -sc);
           if (r >= target) return r;
-          // the next addition leaves the binade: take it for real
         }
       }
     }
