@@ -7,6 +7,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -131,7 +132,11 @@ struct vsg_render {
   float4* pos = nullptr;
   float4* col = nullptr;
   float4* nrm = nullptr;
-  uint32_t* rc = nullptr;  // {hits, misses} per workgroup of the last render
+  uint32_t* rc = nullptr;  // {hits, misses} per workgroup of the last render (march, then tail pass)
+  size_t rc_words = 0;
+  void* tail = nullptr;          // tail queue: TailRay per pixel
+  uint32_t* tail_ctl = nullptr;  // {count, next, workgroups done}
+  uint32_t budget = 0;           // march steps per lane before the tail pass takes the ray over
   uint32_t calls = 0;
   hipEvent_t ev = nullptr;
   int32_t sampling = VSG_SAMPLE_REFERENCE;
@@ -543,8 +548,18 @@ vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out) {
   if (st == VSG_OK) st = dmalloc(&r->col, n * sizeof(float4), "colour output");
   if (st == VSG_OK) st = dmalloc(&r->nrm, n * sizeof(float4), "normal output");
   const size_t nwg = (size_t)((width + 7) / 8) * ((height + 7) / 8);
-  if (st == VSG_OK) st = dmalloc(&r->rc, nwg * 2 * sizeof(uint32_t), "render counters");
-  if (st == VSG_OK && hipMemset(r->rc, 0, nwg * 2 * sizeof(uint32_t)) != hipSuccess) st = fail(VSG_ERR_HIP, "hipMemset");
+  r->rc_words = 2 * (nwg + tail_workgroups(width, height));
+  if (st == VSG_OK) st = dmalloc(&r->rc, r->rc_words * sizeof(uint32_t), "render counters");
+  if (st == VSG_OK && hipMemset(r->rc, 0, r->rc_words * sizeof(uint32_t)) != hipSuccess)
+    st = fail(VSG_ERR_HIP, "hipMemset");
+  if (st == VSG_OK) st = dmalloc(&r->tail, n * sizeof(TailRay), "tail queue");
+  if (st == VSG_OK) st = dmalloc(&r->tail_ctl, 4 * sizeof(uint32_t), "tail queue controls");
+  if (st == VSG_OK && hipMemset(r->tail_ctl, 0, 4 * sizeof(uint32_t)) != hipSuccess)
+    st = fail(VSG_ERR_HIP, "hipMemset");
+  // steps a lane marches before its ray moves to the cooperative tail pass (tuning
+  // knob: VSG_RAYCAST_STEP_BUDGET, 0 disables the tail pass)
+  r->budget = kDefaultStepBudget;
+  if (const char* env = std::getenv("VSG_RAYCAST_STEP_BUDGET")) r->budget = (uint32_t)std::strtoul(env, nullptr, 10);
   if (st == VSG_OK && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipEventCreate");
   if (st != VSG_OK) {
@@ -563,6 +578,8 @@ vsg_status vsg_render_destroy(vsg_render* r) {
   (void)hipFree(r->col);
   (void)hipFree(r->nrm);
   (void)hipFree(r->rc);
+  (void)hipFree(r->tail);
+  (void)hipFree(r->tail_ctl);
   (void)hipFree(r->jac);
   (void)hipFree(r->partials);
   (void)hipFree(r->gn_out);
@@ -613,6 +630,9 @@ vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3],
   ra.mark_blk = s->diag ? s->mark_blk : nullptr;
   ra.march_stats = s->diag ? s->march_stats : nullptr;
   ra.mode = r->sampling;
+  ra.budget = r->budget;
+  ra.tail = r->tail;
+  ra.tail_ctl = r->tail_ctl;
   hipStream_t strm = r->has_stream ? r->stream : s->stream;
   if (strm != s->stream) {  // order after the scene's pending work
     VSG_HIP(hipEventRecord(r->ev, s->stream));
@@ -663,10 +683,9 @@ vsg_status vsg_render_counters(vsg_render* r, uint32_t* hits, uint32_t* misses) 
   *hits = *misses = 0;
   if (r->calls == 0) return VSG_OK;
   if (r->has_last) VSG_HIP(hipStreamSynchronize(r->last));
-  const size_t nwg = (size_t)((r->W + 7) / 8) * ((r->H + 7) / 8);
-  std::vector<uint32_t> h(2 * nwg);
+  std::vector<uint32_t> h(r->rc_words);
   VSG_HIP(hipMemcpy(h.data(), r->rc, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < nwg; ++i) {
+  for (size_t i = 0; i < r->rc_words / 2; ++i) {
     *hits += h[2 * i];
     *misses += h[2 * i + 1];
   }

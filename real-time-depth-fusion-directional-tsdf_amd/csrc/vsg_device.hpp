@@ -166,14 +166,21 @@ __device__ __forceinline__ int32_t table_lookup(const TableView& t, int32_t x, i
   return reinterpret_cast<const int32_t*>(r + kBucket)[hit];
 }
 
+// 64-bit finaliser (MurmurHash3 fmix64): every output bit depends on every input bit,
+// so spatially clustered keys spread over the table.
+__device__ __forceinline__ uint64_t mix64(uint64_t k) {
+  k ^= k >> 33;
+  k *= 0xFF51AFD7ED558CCDull;
+  k ^= k >> 33;
+  k *= 0xC4CEB9FE1A85EC53ull;
+  k ^= k >> 33;
+  return k;
+}
+
 // Occupancy map word of super-block (sx, sy, sz) and bit of block (x, y, z) in the
 // word of its super-block (x >> 2, y >> 2, z >> 2).
 __device__ __forceinline__ uint32_t occ_index(int32_t sx, int32_t sy, int32_t sz, uint32_t mask) {
-  uint32_t k = (uint32_t)sx * 0x9E3779B1u ^ (uint32_t)sy * 0x85EBCA77u ^ (uint32_t)sz * 0xC2B2AE3Du;
-  k ^= k >> 15;
-  k *= 0x2C1B3C6Du;
-  k ^= k >> 12;
-  return k & mask;
+  return (uint32_t)(mix64(pack_key(sx, sy, sz)) >> 32) & mask;
 }
 
 __device__ __forceinline__ uint32_t occ_bit(int32_t x, int32_t y, int32_t z) {
@@ -182,11 +189,7 @@ __device__ __forceinline__ uint32_t occ_bit(int32_t x, int32_t y, int32_t z) {
 
 // Home slot of block (x, y, z) in the block index.
 __device__ __forceinline__ uint32_t idx_home(int32_t x, int32_t y, int32_t z, uint32_t mask) {
-  uint32_t k = (uint32_t)x * 0x85EBCA77u ^ (uint32_t)y * 0xC2B2AE3Du ^ (uint32_t)z * 0x27D4EB2Fu;
-  k ^= k >> 16;
-  k *= 0x7FEB352Du;
-  k ^= k >> 15;
-  return k & mask;
+  return (uint32_t)(mix64(pack_key(x, y, z)) >> 32) & mask;
 }
 
 // Pointer of block (x, y, z) from the block index, or -1.

@@ -19,7 +19,24 @@ struct RenderArgs {
   uint8_t* mark_blk;  // diagnostics (null when off): blocks a trilinear corner touched
   uint32_t* march_stats;  // diagnostics (null when off): evaluations, super/block skips, replayed steps
   int32_t mode;           // vsg_sampling: 0 the reference's trilinear sampling, 1 standard trilinear
+  uint32_t budget;        // steps per lane before a ray moves to the cooperative tail pass (0: none)
+  void* tail;             // tail queue (TailRay[W*H]) when budget > 0
+  uint32_t* tail_ctl;     // tail queue {count, next, workgroups done}, zero between renders
 };
+
+// A ray whose march continues in the tail pass: its pixel and march state.
+struct TailRay {
+  uint32_t pix;
+  float ray, last_sdf, last_alpha;
+  uint32_t last_w;
+  uint32_t pad[3];
+};
+
+// Default march steps per lane before a ray moves to the tail pass.
+constexpr uint32_t kDefaultStepBudget = 96;
+
+// Workgroups of the tail pass for a W x H render.
+uint32_t tail_workgroups(int32_t W, int32_t H);
 
 void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col);
 void launch_reset_table(hipStream_t s, const TableView& t, DevState* st);

@@ -779,22 +779,34 @@ __device__ __forceinline__ int occ_test(const TableView& t, OccCache& oc, int32_
 struct RayHit {
   bool marched;  // depth range valid (RayCastRender.cu:461-462)
   bool found;
+  bool deferred;  // step budget spent: the march continues from the state below
   float alpha, d2r;
   float cdx, cdy, cdz, wdx, wdy, wdz;
   float a, ad, b, bd;
   uint32_t rgb;
+  float ray, last_sdf, last_alpha;  // state of a deferred march
+  uint32_t last_w;
 };
 
-template <bool F16, bool DIAG, bool REF, class Cache>
-__device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
-                                            const uint32_t* __restrict__ col, uint32_t x, uint32_t y, Cache& bc) {
+// Per-ray constants of the march.
+struct RaySetup {
+  float camx, camy, camz;
+  float ray0, ray_end, inc, th, v;
+  RaySkip sk;
+};
+
+// Ray of pixel (x, y) (RayCastRender.cu:407-470).  Returns false when the depth
+// range is invalid (no march; the pixel keeps the miss values).
+__device__ __forceinline__ bool ray_setup(const RenderArgs& ra, uint32_t x, uint32_t y, RayHit& h, RaySetup& rs) {
   const float MINF = 1e10f;
-  RayHit h;
   h.marched = false;
   h.found = false;
+  h.deferred = false;
   h.alpha = 0.0f;
   h.a = h.ad = h.b = h.bd = 0.0f;
   h.rgb = 0;
+  h.ray = h.last_sdf = h.last_alpha = 0.0f;
+  h.last_w = 0;
   float cdx = ((float)x - ra.cx) / ra.fx, cdy = ((float)y - ra.cy) / ra.fy, cdz = 1.0f;
   const float cl = sqrtf(cdx * cdx + cdy * cdy + cdz * cdz);
   if (cl > 1e-6f) {
@@ -820,30 +832,71 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
   h.wdz = wdz;
   h.d2r = 1.0f / cdz;
   const float minD = ra.minD, maxD = ra.maxD;
-  if (minD == 0.0f || minD == MINF || maxD == 0.0f || maxD == MINF) return h;
+  if (minD == 0.0f || minD == MINF || maxD == 0.0f || maxD == MINF) return false;
   h.marched = true;
-  const float v = ra.voxel;
-  const float camx = ra.cam[0], camy = ra.cam[1], camz = ra.cam[2];
-  const float d2r = h.d2r;
-  float ray = d2r * fmaxf(minD, minD);
-  const float ray_end = d2r * fminf(maxD, maxD);
-  const float th = v * 3.0f;
-  const float inc = v * 0.5f;
-  float last_sdf = 0.0f, last_alpha = 0.0f;
-  uint32_t last_w = 0;
-  RaySkip sk;
-  sk.camx = camx;
-  sk.camy = camy;
-  sk.camz = camz;
+  rs.v = ra.voxel;
+  rs.camx = ra.cam[0];
+  rs.camy = ra.cam[1];
+  rs.camz = ra.cam[2];
+  rs.ray0 = h.d2r * fmaxf(minD, minD);
+  rs.ray_end = h.d2r * fminf(maxD, maxD);
+  rs.th = rs.v * 3.0f;
+  rs.inc = rs.v * 0.5f;
+  RaySkip& sk = rs.sk;
+  sk.camx = rs.camx;
+  sk.camy = rs.camy;
+  sk.camz = rs.camz;
   sk.ix = wdx != 0.0f ? 1.0f / wdx : INFINITY;
   sk.iy = wdy != 0.0f ? 1.0f / wdy : INFINITY;
   sk.iz = wdz != 0.0f ? 1.0f / wdz : INFINITY;
-  sk.v = v;
-  sk.inv_v = 1.0f / v;
-  {
-    const float mag = fmaxf(fmaxf(fabsf(camx), fabsf(camy)), fabsf(camz)) + fabsf(ray_end);
-    sk.eps = 0.02f + 8e-6f * mag * sk.inv_v;
-  }
+  sk.v = rs.v;
+  sk.inv_v = 1.0f / rs.v;
+  const float mag = fmaxf(fmaxf(fabsf(rs.camx), fabsf(rs.camy)), fabsf(rs.camz)) + fabsf(rs.ray_end);
+  sk.eps = 0.02f + 8e-6f * mag * sk.inv_v;
+  return true;
+}
+
+// Skip test at ray parameter `ray`: 0 when the super-blocks around the sample hold
+// no allocated block, 1 when the blocks around it hold none, 2 otherwise; [t0, t1)
+// is the parameter interval of the box in which that holds (the super-blocks' box
+// for 0, the blocks' box for 1 and 2).
+__device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const RaySkip& sk, float wdx, float wdy,
+                                         float wdz, float ray, float* t0, float* t1) {
+  const float mq = 1.125f + sk.eps;  // range margin around Q (voxels)
+  const float qx = (sk.camx + ray * wdx) * sk.inv_v, qy = (sk.camy + ray * wdy) * sk.inv_v,
+              qz = (sk.camz + ray * wdz) * sk.inv_v;
+  const int32_t bx0 = cvt_i32(floorf((qx - mq) * 0.125f)), bx1 = cvt_i32(floorf((qx + mq) * 0.125f));
+  const int32_t by0 = cvt_i32(floorf((qy - mq) * 0.125f)), by1 = cvt_i32(floorf((qy + mq) * 0.125f));
+  const int32_t bz0 = cvt_i32(floorf((qz - mq) * 0.125f)), bz1 = cvt_i32(floorf((qz + mq) * 0.125f));
+  const int occ = occ_test(t, oc, bx0, bx1, by0, by1, bz0, bz1);
+  const float g = occ == 0 ? 32.0f : 8.0f;
+  const int32_t sh = occ == 0 ? 2 : 0;
+  const float lo = 1.0f, hi = occ == 0 ? 31.0f : 7.0f;
+  box_interval(sk, g * (float)(bx0 >> sh) + lo, g * (float)(bx1 >> sh) + hi, g * (float)(by0 >> sh) + lo,
+               g * (float)(by1 >> sh) + hi, g * (float)(bz0 >> sh) + lo, g * (float)(bz1 >> sh) + hi, t0, t1);
+  return occ;
+}
+
+// The march of one ray per lane.  budget > 0: a lane that has taken that many steps
+// (skips or evaluations) stops with h.deferred and its state, to be continued by
+// the cooperative tail pass.
+template <bool F16, bool DIAG, bool REF, class Cache>
+__device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
+                                            const uint32_t* __restrict__ col, uint32_t x, uint32_t y, Cache& bc,
+                                            uint32_t budget) {
+  RayHit h;
+  RaySetup rs;
+  if (!ray_setup(ra, x, y, h, rs)) return h;
+  const float v = rs.v;
+  const float camx = rs.camx, camy = rs.camy, camz = rs.camz;
+  const float wdx = h.wdx, wdy = h.wdy, wdz = h.wdz;
+  float ray = rs.ray0;
+  const float ray_end = rs.ray_end;
+  const float th = rs.th;
+  const float inc = rs.inc;
+  float last_sdf = 0.0f, last_alpha = 0.0f;
+  uint32_t last_w = 0;
+  const RaySkip& sk = rs.sk;
   // One step per lane and iteration: a skip over samples the occupancy map proves
   // invalid, or the evaluation of one sample.  Lanes never wait for each other's
   // skips, so the wave runs max(steps) iterations rather than the sum of the lanes'
@@ -853,37 +906,29 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
   // instead of once per lane.  A failed refinement resumes the march exactly where
   // the reference continues (RayCastRender.cu:372-391).
   bool stalled = false;
-  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0;
+  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_steps = 0;
   OccCache oc;
   oc.idx = 0xFFFFFFFFu;
   oc.word = 0ull;
-  const float mq = 1.125f + sk.eps;  // range margin around Q (voxels)
-  float t_busy = -INFINITY;           // samples before this have an allocated block in range
   bool marching = ray < ray_end, parked = false;
   while (true) {
     if (!__any(marching || parked)) break;
+    if (marching && budget != 0u && n_steps >= budget) {
+      h.deferred = true;
+      h.ray = ray;
+      h.last_sdf = last_sdf;
+      h.last_alpha = last_alpha;
+      h.last_w = last_w;
+      marching = false;
+    }
     if (marching) {
+      n_steps++;
       bool step = true;
-      if (!(ray < t_busy)) {
-        const float qx = (camx + ray * wdx) * sk.inv_v, qy = (camy + ray * wdy) * sk.inv_v,
-                    qz = (camz + ray * wdz) * sk.inv_v;
-        const int32_t bx0 = cvt_i32(floorf((qx - mq) * 0.125f)), bx1 = cvt_i32(floorf((qx + mq) * 0.125f));
-        const int32_t by0 = cvt_i32(floorf((qy - mq) * 0.125f)), by1 = cvt_i32(floorf((qy + mq) * 0.125f));
-        const int32_t bz0 = cvt_i32(floorf((qz - mq) * 0.125f)), bz1 = cvt_i32(floorf((qz + mq) * 0.125f));
-        const int occ = occ_test(t, oc, bx0, bx1, by0, by1, bz0, bz1);
-        // occ 0: the range's super-blocks are empty (skip their box); occ 1: the
-        // range's blocks are (skip its box); occ 2: evaluate every sample until the
-        // ray leaves the range's box
-        const float g = occ == 0 ? 32.0f : 8.0f;
-        const int32_t sh = occ == 0 ? 2 : 0;
-        const float lo = 1.0f, hi = occ == 0 ? 31.0f : 7.0f;
+      {
         float t0, t1;
-        box_interval(sk, g * (float)(bx0 >> sh) + lo, g * (float)(bx1 >> sh) + hi, g * (float)(by0 >> sh) + lo,
-                     g * (float)(by1 >> sh) + hi, g * (float)(bz0 >> sh) + lo, g * (float)(bz1 >> sh) + hi, &t0, &t1);
+        const int occ = skip_test(t, oc, sk, wdx, wdy, wdz, ray, &t0, &t1);
         const bool inside = t0 <= ray && ray < t1;  // false for a NaN interval
-        if (occ == 2) {
-          t_busy = inside ? t1 : -INFINITY;
-        } else if (inside) {
+        if (occ != 2 && inside) {
           if (occ == 0) n_super++;
           else n_block++;
           last_w = 0u;
@@ -986,12 +1031,22 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
   const uint32_t x = blockIdx.x * 8u + (threadIdx.x & 7u);
   const uint32_t y = blockIdx.y * 8u + (threadIdx.x >> 3);
   uint32_t hit = 0, miss = 0;
+  bool deferred = false;
+  TailRay tr;
   if (x < (uint32_t)ra.W && y < (uint32_t)ra.H) {
     const float MINF = 1e10f;
     const int pix = (int)y * ra.W + (int)x;
     BlockCache<DIAG> bc;
     bc.init(ra.mark_bkt, ra.mark_blk, ra.march_stats);
-    const RayHit h = march_ray<F16, DIAG, REF>(t, ra, vox, col, x, y, bc);
+    const RayHit h = march_ray<F16, DIAG, REF>(t, ra, vox, col, x, y, bc, ra.budget);
+    deferred = h.deferred;
+    tr.pix = (uint32_t)pix;
+    tr.ray = h.ray;
+    tr.last_sdf = h.last_sdf;
+    tr.last_alpha = h.last_alpha;
+    tr.last_w = h.last_w;
+    tr.pad[0] = tr.pad[1] = tr.pad[2] = 0u;
+    if (!deferred) {
     float o_depth = MINF;
     float4 o_pos = make_float4(MINF, MINF, MINF, MINF), o_col = o_pos;
     if (h.found) {
@@ -1007,6 +1062,18 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
     out_pos[pix] = o_pos;
     out_col[pix] = o_col;
     out_nrm[pix] = make_float4(MINF, MINF, MINF, MINF);
+    }
+  }
+  // deferred rays: one queue reservation per wave
+  const uint64_t dm = __ballot(deferred);
+  if (dm != 0ull) {
+    uint32_t base = 0;
+    if (threadIdx.x == 0) base = atomicAdd(&ra.tail_ctl[0], (uint32_t)__popcll(dm));
+    base = __shfl(base, 0, 64);
+    if (deferred) {
+      const uint32_t rank = (uint32_t)__popcll(dm & ((1ull << threadIdx.x) - 1ull));
+      if (base + rank < (uint32_t)ra.W * (uint32_t)ra.H) reinterpret_cast<TailRay*>(ra.tail)[base + rank] = tr;
+    }
   }
   // one wave per workgroup: one write of its hit / miss counts (summed when read)
   hit = wave_sum(hit);
@@ -1014,6 +1081,216 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
   if (threadIdx.x == 0) {
     rc[2 * wg] = hit;
     rc[2 * wg + 1] = miss;
+  }
+}
+
+// Tail pass: the rays whose march ran out of steps, one ray per group of kTailGroup
+// lanes.  A group takes the skip test at the window start like a single lane would;
+// otherwise its lanes evaluate kTailGroup consecutive samples at once (the positions
+// the reference's additions reach, computed by the same additions) and the group
+// resolves them in order: the first sign change that passes the thresholds is
+// refined, and the march continues after the window's last sample.  Groups take
+// rays from the queue until it is empty; the last workgroup clears the queue
+// controls for the next render.
+constexpr int kTailGroup = 16;
+
+template <bool F16, bool DIAG, bool REF>
+__global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, const void* __restrict__ vox,
+                                                    const uint32_t* __restrict__ col, float* __restrict__ out_depth,
+                                                    float4* __restrict__ out_pos, float4* __restrict__ out_col,
+                                                    float4* __restrict__ out_nrm, uint32_t* __restrict__ rc) {
+  constexpr int G = kTailGroup;
+  const int lane = threadIdx.x;
+  const int gl = lane & (G - 1);
+  const int gbase = lane & ~(G - 1);
+  const uint32_t n = min(ra.tail_ctl[0], (uint32_t)ra.W * (uint32_t)ra.H);
+  const TailRay* __restrict__ queue = reinterpret_cast<const TailRay*>(ra.tail);
+  const float MINF = 1e10f;
+  uint32_t hit = 0, miss = 0;
+  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0;
+  BlockCache<DIAG> bc;
+  bc.init(ra.mark_bkt, ra.mark_blk, ra.march_stats);
+  OccCache oc;
+  oc.idx = 0xFFFFFFFFu;
+  oc.word = 0ull;
+  RayHit h;
+  RaySetup rs;
+  uint32_t pix = 0;
+  float ray = 0.0f, last_sdf = 0.0f, last_alpha = 0.0f;
+  uint32_t last_w = 0;
+  bool have = false, exhausted = false;
+  while (true) {
+    if (!have && !exhausted) {
+      uint32_t e = 0;
+      if (gl == 0) e = atomicAdd(&ra.tail_ctl[1], 1u);
+      e = __shfl(e, 0, G);
+      if (e < n) {
+        const TailRay q = queue[e];
+        pix = q.pix;
+        (void)ray_setup(ra, pix % (uint32_t)ra.W, pix / (uint32_t)ra.W, h, rs);
+        ray = q.ray;
+        last_sdf = q.last_sdf;
+        last_alpha = q.last_alpha;
+        last_w = q.last_w;
+        have = true;
+      } else {
+        exhausted = true;
+      }
+    }
+    if (!__any(have)) break;
+    if (!have) continue;
+    bool done = false;
+    bool window = true;
+    {
+      float t0, t1;
+      const int occ = skip_test(t, oc, rs.sk, h.wdx, h.wdy, h.wdz, ray, &t0, &t1);
+      const bool inside = t0 <= ray && ray < t1;
+      if (occ != 2 && inside) {
+        if (gl == 0) {
+          if (occ == 0) n_super++;
+          else n_block++;
+        }
+        last_w = 0u;
+        bool stl;
+        const float from = ray;
+        ray = replay_steps(ray, fminf(t1, rs.ray_end), rs.inc, &stl);
+        if (DIAG && gl == 0) n_replay += (uint32_t)((ray - from) / rs.inc + 0.5f);
+        if (stl || !(ray < rs.ray_end)) done = true;
+        window = false;
+      }
+    }
+    if (window) {
+      // sample gl of the window: gl more additions of the step
+      float r = ray;
+      bool in = true;
+      for (int k = 0; k < gl; ++k) {
+        const float nx = r + rs.inc;
+        if (nx == r) {
+          in = false;
+          break;
+        }
+        r = nx;
+      }
+      in = in && r < rs.ray_end;
+      float dist = 0.0f;
+      bool valid = false;
+      if (in) {
+        valid = trilinear<F16, false, REF>(t, bc, vox, col, rs.camx + r * h.wdx, rs.camy + r * h.wdy,
+                                           rs.camz + r * h.wdz, rs.v, &dist, nullptr) == kSampleValid;
+        n_eval++;
+      }
+      // the sample before: lane gl - 1's, or the carried state for the first lane
+      float p_sdf = __shfl_up(dist, 1, G), p_r = __shfl_up(r, 1, G);
+      int p_v = __shfl_up(valid ? 1 : 0, 1, G);
+      if (gl == 0) {
+        p_sdf = last_sdf;
+        p_r = last_alpha;
+        p_v = last_w ? 1 : 0;
+      }
+      const bool cand = valid && p_v && p_sdf < 0.0f && dist > 0.0f && fabsf(p_sdf - dist) < rs.th &&
+                        fabsf(dist) < rs.th;
+      const uint32_t gm = (uint32_t)((__ballot(cand) >> gbase) & ((1ull << G) - 1ull));
+      const uint32_t gi = (uint32_t)((__ballot(in) >> gbase) & ((1ull << G) - 1ull));
+      if (gm != 0u) {
+        const int f = __builtin_ctz(gm);
+        const float a0 = __shfl(p_r, f, G), ad0 = __shfl(p_sdf, f, G), b0 = __shfl(r, f, G), bd0 = __shfl(dist, f, G);
+        // findIntersectionBisection (RayCastRender.cu:229-271), the same in every lane of the group
+        float a = a0, ad = ad0, bb = b0, bd = bd0, c = 0.0f;
+        uint32_t rgb = 0;
+        bool ok = true;
+        for (int it = 0; it < 3; ++it) {
+          c = a + (ad / (ad - bd)) * (bb - a);
+          float cd;
+          if (trilinear<F16, true, REF>(t, bc, vox, col, rs.camx + c * h.wdx, rs.camy + c * h.wdy,
+                                        rs.camz + c * h.wdz, rs.v, &cd, &rgb) != kSampleValid) {
+            ok = false;
+            break;
+          }
+          if (ad * cd > 0.0f) {
+            a = c;
+            ad = cd;
+          } else {
+            bb = c;
+            bd = cd;
+          }
+        }
+        if (ok) {
+          h.found = true;
+          h.alpha = c;
+          h.rgb = rgb;
+          h.a = a0;
+          h.ad = ad0;
+          h.b = b0;
+          h.bd = bd0;
+          done = true;
+        } else {
+          last_sdf = bd0;
+          last_alpha = b0;
+          last_w = 1u;
+          const float nx = b0 + rs.inc;
+          if (nx == b0 || !(nx < rs.ray_end)) done = true;
+          ray = nx;
+        }
+      } else {
+        // no accepted crossing: continue after the window's last sample
+        const int L = 31 - __builtin_clz(gi | 1u);
+        const float rL = __shfl(r, L, G), dL = __shfl(dist, L, G);
+        const int vL = __shfl(valid ? 1 : 0, L, G);
+        if (vL) {
+          last_sdf = dL;
+          last_alpha = rL;
+          last_w = 1u;
+        } else {
+          last_w = 0u;
+        }
+        const float nx = rL + rs.inc;
+        if (L < G - 1 || nx == rL || !(nx < rs.ray_end)) done = true;
+        ray = nx;
+      }
+    }
+    if (done) {
+      if (gl == 0) {
+        float o_depth = MINF;
+        float4 o_pos = make_float4(MINF, MINF, MINF, MINF), o_col = o_pos;
+        if (h.found) {
+          const float alpha = h.alpha;
+          o_depth = alpha / h.d2r;
+          o_pos = make_float4(ra.cam[0] + alpha * h.wdx, ra.cam[1] + alpha * h.wdy, ra.cam[2] + alpha * h.wdz, 1.0f);
+          o_col = make_float4((float)(h.rgb & 0xFFu) / 255.0f, (float)((h.rgb >> 8) & 0xFFu) / 255.0f,
+                              (float)((h.rgb >> 16) & 0xFFu) / 255.0f, 1.0f);
+          hit++;
+        } else {
+          miss++;
+        }
+        out_depth[pix] = o_depth;
+        out_pos[pix] = o_pos;
+        out_col[pix] = o_col;
+        out_nrm[pix] = make_float4(MINF, MINF, MINF, MINF);
+      }
+      have = false;
+    }
+  }
+  if (DIAG && ra.march_stats) {
+    const uint32_t e = wave_sum(n_eval), su = wave_sum(n_super), bl = wave_sum(n_block), re = wave_sum(n_replay);
+    if (lane == 0) {
+      atomicAdd(&ra.march_stats[0], e);
+      atomicAdd(&ra.march_stats[1], su);
+      atomicAdd(&ra.march_stats[2], bl);
+      atomicAdd(&ra.march_stats[3], re);
+    }
+  }
+  hit = wave_sum(hit);
+  miss = wave_sum(miss);
+  if (lane == 0) {
+    rc[2 * blockIdx.x] = hit;
+    rc[2 * blockIdx.x + 1] = miss;
+    // the last workgroup to finish clears the queue controls for the next render
+    __threadfence();
+    if (atomicAdd(&ra.tail_ctl[2], 1u) == gridDim.x - 1u) {
+      ra.tail_ctl[0] = 0u;
+      ra.tail_ctl[1] = 0u;
+      ra.tail_ctl[2] = 0u;
+    }
   }
 }
 
@@ -1125,7 +1402,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
   if (st->n_alloc != 0u) {
     BlockCache<false> bc;
     bc.init(nullptr, nullptr);
-    const RayHit h = march_ray<F16, false, REF>(t, ra, vox, col, x, y, bc);
+    const RayHit h = march_ray<F16, false, REF>(t, ra, vox, col, x, y, bc, 0u);
     if (h.found) {
       const float v = ra.voxel;
       const float camx = ra.cam[0], camy = ra.cam[1], camz = ra.cam[2];
@@ -1466,14 +1743,26 @@ void launch_gn(hipStream_t s, const float* depth, const float* obs, const float*
   hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, partials, (int)nwg, lambda, p, out);
 }
 
+uint32_t tail_workgroups(int32_t W, int32_t H) {
+  const uint64_t n = ((uint64_t)W * (uint64_t)H + 255u) / 256u;
+  return (uint32_t)(n < 1u ? 1u : (n > 2048u ? 2048u : n));
+}
+
 void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                    const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,
                    uint32_t* rc) {
   const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
   const bool diag = ra.mark_blk != nullptr;
   const bool ref = ra.mode == 0;
-#define VSG_RENDER(F, D, R) \
-  hipLaunchKernelGGL((k_render<F, D, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc)
+  uint32_t* rc_tail = rc + 2 * (size_t)grid.x * grid.y;
+  const dim3 tgrid(tail_workgroups(ra.W, ra.H));
+#define VSG_RENDER(F, D, R)                                                                                    \
+  do {                                                                                                         \
+    hipLaunchKernelGGL((k_render<F, D, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc); \
+    if (ra.budget != 0u)                                                                                       \
+      hipLaunchKernelGGL((k_render_tail<F, D, R>), tgrid, dim3(64), 0, s, t, ra, vox, col, depth, pos, colr, nrm, \
+                         rc_tail);                                                                             \
+  } while (0)
   if (fp16) {
     if (diag) {
       if (ref) VSG_RENDER(true, true, true); else VSG_RENDER(true, true, false);
