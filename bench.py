@@ -292,10 +292,14 @@ def main() -> int:
     avg_ms = ktimes[dom][0] / launches
     bytes_per_launch = res["kbytes"].get(dom, 0) / nf
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    # HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (scaled by a
+    # copy kernel of known size) on this workload, recorded in profiles/
     traffic = None
-    tr_path = os.path.join(REPO, "profiles", f"{dom}_room640_traffic.json")
-    if os.path.exists(tr_path):
-        traffic = json.load(open(tr_path)).get("bytes_per_launch")
+    tr_path = os.path.join(REPO, "profiles", f"{dom}_room640_traffic.md")
+    if nf == 300 and args.store == "fp32" and os.path.exists(tr_path):
+        for line in open(tr_path):
+            if line.startswith("bytes_per_launch:"):
+                traffic = int(line.split(":", 1)[1])
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": dom,
                 "bytes_per_launch": int(bytes_per_launch), "avg_launch_ms": round(avg_ms, 4)}

@@ -165,6 +165,12 @@ vsg_status vsg_render_set_stream(vsg_render* render, void* hip_stream);
 typedef enum vsg_sampling { VSG_SAMPLE_REFERENCE = 0, VSG_SAMPLE_TRILINEAR = 1 } vsg_sampling;
 vsg_status vsg_render_set_sampling(vsg_render* render, int32_t mode);
 
+/* Steps (skips or sample evaluations) a ray marches with one lane before the
+ * cooperative tail pass continues it with several lanes; 0 marches every ray with
+ * one lane to the end.  Results do not depend on it (default 96).  No counterpart
+ * in the reference: the tail pass is this library's load balancing. */
+vsg_status vsg_render_set_step_budget(vsg_render* render, uint32_t steps);
+
 /* RayCastRender::render (VoxelSG/src/RayCastRender.cpp:123-183,
  * VoxelSG/src/RayCastRender.cu:407-620): one ray per pixel, marched in half-voxel
  * steps over [min_depth, max_depth] (z-depth), zero crossing refined by three

@@ -7,7 +7,6 @@
 
 #include <cmath>
 #include <cstdio>
-#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -556,10 +555,7 @@ vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out) {
   if (st == VSG_OK) st = dmalloc(&r->tail_ctl, 4 * sizeof(uint32_t), "tail queue controls");
   if (st == VSG_OK && hipMemset(r->tail_ctl, 0, 4 * sizeof(uint32_t)) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipMemset");
-  // steps a lane marches before its ray moves to the cooperative tail pass (tuning
-  // knob: VSG_RAYCAST_STEP_BUDGET, 0 disables the tail pass)
   r->budget = kDefaultStepBudget;
-  if (const char* env = std::getenv("VSG_RAYCAST_STEP_BUDGET")) r->budget = (uint32_t)std::strtoul(env, nullptr, 10);
   if (st == VSG_OK && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipEventCreate");
   if (st != VSG_OK) {
@@ -593,6 +589,13 @@ vsg_status vsg_render_set_sampling(vsg_render* r, int32_t mode) {
   if (mode != VSG_SAMPLE_REFERENCE && mode != VSG_SAMPLE_TRILINEAR)
     return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_sampling: unknown mode");
   r->sampling = mode;
+  return VSG_OK;
+}
+
+vsg_status vsg_render_set_step_budget(vsg_render* r, uint32_t steps) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_step_budget: null renderer");
+  // the queue controls stay consistent: every render that queues rays also drains them
+  r->budget = steps;
   return VSG_OK;
 }
 

@@ -87,6 +87,7 @@ _SIGNATURES = {
     "vsg_render_destroy": (C.c_int, [C.c_void_p]),
     "vsg_render_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "vsg_render_set_sampling": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vsg_render_set_step_budget": (C.c_int, [C.c_void_p, C.c_uint32]),
     "vsg_render_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                    C.POINTER(vsg_intrinsics), C.c_float, C.c_float, C.c_float]),
     "vsg_render_buffers": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4),
@@ -450,6 +451,11 @@ class RayCastRender:
     def set_sampling(self, mode: str) -> None:
         """"reference" (the reference's half-voxel-shifted sampling, default) or "trilinear"."""
         _check(lib().vsg_render_set_sampling(self._h, {"reference": 0, "trilinear": 1}[mode]))
+
+    def set_step_budget(self, steps: int) -> None:
+        """Steps a ray marches with one lane before the cooperative tail pass takes
+        it over (0: no tail pass).  Results do not depend on it."""
+        _check(lib().vsg_render_set_step_budget(self._h, int(steps)))
 
     def adjoint(self, voxelScene: VoxelScene, jac=None, grad_depth=None, grad_sdf=None) -> None:
         """Adjoint of the last render: jac (device float[6*W*H]) receives d depth / d xi for
