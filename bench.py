@@ -174,6 +174,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-frames", type=int, default=30)
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--step-budget", type=int, default=None,
+                    help="raycast steps per lane before the cooperative tail pass (library default if unset)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="first run a plain copy kernel (torch.neg, float32) over this many MiB, 4 times: "
                          "its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc scale the counters of the "
@@ -228,6 +230,8 @@ def main() -> int:
         scene.set_stream(stream.cuda_stream)
         rr = vsg.RayCastRender()
         assert rr.initialize(W, H)
+        if args.step_budget is not None:
+            rr.set_step_budget(args.step_budget)
         return scene, rr
 
     def run_pass(scene, rr, per_frame=None):

@@ -32,6 +32,9 @@ struct TailRay {
   uint32_t pad[3];
 };
 
+// Raycast workgroups go to the XCDs in super-tiles of kSuperTile x kSuperTile 8x8 tiles.
+constexpr uint32_t kSuperTile = 4;
+
 // Default march steps per lane before a ray moves to the tail pass.
 constexpr uint32_t kDefaultStepBudget = 96;
 

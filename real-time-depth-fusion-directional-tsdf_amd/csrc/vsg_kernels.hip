@@ -734,15 +734,24 @@ __device__ __forceinline__ void box_interval(const RaySkip& k, float lox, float 
 // [bz0, bz1] (one or two blocks per axis) and the super-blocks holding it.  The lane
 // keeps the last occupancy word it read (consecutive samples mostly share it).
 struct OccCache {
-  uint32_t idx;
+  int32_t sx, sy, sz;  // super-block of the cached word
   uint64_t word;
+  int32_t r[6];        // block range of the last test that found an allocated block
+  bool busy;
 };
 
+__device__ __forceinline__ void occ_cache_init(OccCache& oc) {
+  oc.sx = oc.sy = oc.sz = INT32_MIN;
+  oc.word = 0ull;
+  oc.busy = false;
+}
+
 __device__ __forceinline__ uint64_t occ_word(const TableView& t, OccCache& oc, int32_t sx, int32_t sy, int32_t sz) {
-  const uint32_t wi = occ_index(sx, sy, sz, t.occ_mask);
-  if (wi != oc.idx) {
-    oc.idx = wi;
-    oc.word = t.occ[wi];
+  if (sx != oc.sx || sy != oc.sy || sz != oc.sz) {
+    oc.sx = sx;
+    oc.sy = sy;
+    oc.sz = sz;
+    oc.word = t.occ[occ_index(sx, sy, sz, t.occ_mask)];
   }
   return oc.word;
 }
@@ -868,7 +877,20 @@ __device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const
   const int32_t bx0 = cvt_i32(floorf((qx - mq) * 0.125f)), bx1 = cvt_i32(floorf((qx + mq) * 0.125f));
   const int32_t by0 = cvt_i32(floorf((qy - mq) * 0.125f)), by1 = cvt_i32(floorf((qy + mq) * 0.125f));
   const int32_t bz0 = cvt_i32(floorf((qz - mq) * 0.125f)), bz1 = cvt_i32(floorf((qz + mq) * 0.125f));
+  *t0 = *t1 = 0.0f;
+  // consecutive samples mostly share their block range: an allocated block in it stays
+  if (oc.busy && bx0 == oc.r[0] && bx1 == oc.r[1] && by0 == oc.r[2] && by1 == oc.r[3] && bz0 == oc.r[4] &&
+      bz1 == oc.r[5])
+    return 2;
   const int occ = occ_test(t, oc, bx0, bx1, by0, by1, bz0, bz1);
+  oc.busy = occ == 2;
+  oc.r[0] = bx0;
+  oc.r[1] = bx1;
+  oc.r[2] = by0;
+  oc.r[3] = by1;
+  oc.r[4] = bz0;
+  oc.r[5] = bz1;
+  if (occ == 2) return 2;
   const float g = occ == 0 ? 32.0f : 8.0f;
   const int32_t sh = occ == 0 ? 2 : 0;
   const float lo = 1.0f, hi = occ == 0 ? 31.0f : 7.0f;
@@ -908,8 +930,7 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
   bool stalled = false;
   uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_steps = 0;
   OccCache oc;
-  oc.idx = 0xFFFFFFFFu;
-  oc.word = 0ull;
+  occ_cache_init(oc);
   bool marching = ray < ray_end, parked = false;
   while (true) {
     if (!__any(marching || parked)) break;
@@ -1020,7 +1041,19 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
                                                float* __restrict__ out_depth, float4* __restrict__ out_pos,
                                                float4* __restrict__ out_col, float4* __restrict__ out_nrm,
                                                uint32_t* __restrict__ rc) {
-  const size_t wg = (size_t)blockIdx.y * gridDim.x + blockIdx.x;
+  // Workgroups are dealt to the 8 XCDs in turn (workgroup b runs on XCD b % 8).  The
+  // image is cut into super-tiles of kSuperTile x kSuperTile 8x8 tiles, dealt to the
+  // XCDs in turn: the rays of a super-tile, which read the same blocks, share one L2,
+  // and every XCD gets super-tiles from all over the image (balanced load).
+  const uint32_t tiles_x = ((uint32_t)ra.W + 7u) / 8u, tiles_y = ((uint32_t)ra.H + 7u) / 8u;
+  const uint32_t stx_n = (tiles_x + kSuperTile - 1u) / kSuperTile;
+  const uint32_t j = blockIdx.x >> 3, within = j % (kSuperTile * kSuperTile);
+  const uint32_t st_idx = (j / (kSuperTile * kSuperTile)) * 8u + (blockIdx.x & 7u);
+  const uint32_t tx = (st_idx % stx_n) * kSuperTile + within % kSuperTile;
+  const uint32_t ty = (st_idx / stx_n) * kSuperTile + within / kSuperTile;
+  if (tx >= tiles_x || ty >= tiles_y) return;
+  const uint32_t tile = ty * tiles_x + tx;
+  const size_t wg = tile;
   if (st->n_alloc == 0u) {  // RayCastRender.cu:553-556: nothing to render, outputs untouched
     if (threadIdx.x == 0) {
       rc[2 * wg] = 0u;
@@ -1028,8 +1061,8 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
     }
     return;
   }
-  const uint32_t x = blockIdx.x * 8u + (threadIdx.x & 7u);
-  const uint32_t y = blockIdx.y * 8u + (threadIdx.x >> 3);
+  const uint32_t x = tx * 8u + (threadIdx.x & 7u);
+  const uint32_t y = ty * 8u + (threadIdx.x >> 3);
   uint32_t hit = 0, miss = 0;
   bool deferred = false;
   TailRay tr;
@@ -1111,8 +1144,7 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
   BlockCache<DIAG> bc;
   bc.init(ra.mark_bkt, ra.mark_blk, ra.march_stats);
   OccCache oc;
-  oc.idx = 0xFFFFFFFFu;
-  oc.word = 0ull;
+  occ_cache_init(oc);
   RayHit h;
   RaySetup rs;
   uint32_t pix = 0;
@@ -1744,17 +1776,20 @@ void launch_gn(hipStream_t s, const float* depth, const float* obs, const float*
 }
 
 uint32_t tail_workgroups(int32_t W, int32_t H) {
-  const uint64_t n = ((uint64_t)W * (uint64_t)H + 255u) / 256u;
-  return (uint32_t)(n < 1u ? 1u : (n > 2048u ? 2048u : n));
+  const uint64_t n = ((uint64_t)W * (uint64_t)H + 63u) / 64u;
+  return (uint32_t)(n < 1u ? 1u : (n > 4096u ? 4096u : n));
 }
 
 void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                    const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,
                    uint32_t* rc) {
-  const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
+  const uint32_t tiles_x = (uint32_t)((ra.W + 7) / 8), tiles_y = (uint32_t)((ra.H + 7) / 8);
+  const uint32_t ntiles = tiles_x * tiles_y;
+  const uint32_t nst = ((tiles_x + kSuperTile - 1u) / kSuperTile) * ((tiles_y + kSuperTile - 1u) / kSuperTile);
+  const dim3 grid(8u * ((nst + 7u) / 8u) * kSuperTile * kSuperTile);
   const bool diag = ra.mark_blk != nullptr;
   const bool ref = ra.mode == 0;
-  uint32_t* rc_tail = rc + 2 * (size_t)grid.x * grid.y;
+  uint32_t* rc_tail = rc + 2 * (size_t)ntiles;
   const dim3 tgrid(tail_workgroups(ra.W, ra.H));
 #define VSG_RENDER(F, D, R)                                                                                    \
   do {                                                                                                         \
