@@ -673,6 +673,9 @@ __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const vo
       cbl += wk * (float)((c >> 16) & 0xFFu);
     }
     wsum += wk;
+    // with colours, gather the corners in two halves: the colour words would
+    // otherwise double the registers held by loads in flight
+    if (COLOR && i == 3) __builtin_amdgcn_sched_barrier(0);
   }
   if (wsum <= 1e-6f) return kSampleInvalid;
   const float inv = 1.0f / wsum;
@@ -702,9 +705,9 @@ __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const vo
 
 struct RaySkip {
   float camx, camy, camz;
-  float ix, iy, iz;  // 1 / direction (inf for a zero component)
+  float wdx, wdy, wdz;  // direction
   float v, inv_v;
-  float eps;         // safety margin in voxels
+  float eps;            // safety margin in voxels
 };
 
 // Parameter interval [t0, t1) of the ray inside the box [lo, hi] (voxel units,
@@ -713,15 +716,16 @@ __device__ __forceinline__ void box_interval(const RaySkip& k, float lox, float 
                                              float hiz, float* t0, float* t1) {
   float tn = -INFINITY, tf = INFINITY;
   const float lo[3] = {lox, loy, loz}, hi[3] = {hix, hiy, hiz};
-  const float c[3] = {k.camx, k.camy, k.camz}, id[3] = {k.ix, k.iy, k.iz};
+  const float c[3] = {k.camx, k.camy, k.camz}, d[3] = {k.wdx, k.wdy, k.wdz};
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
     const float l = (lo[a] + k.eps) * k.v - c[a], h = (hi[a] - k.eps) * k.v - c[a];
-    if (isinf(id[a])) {  // parallel to this slab: inside for every t, or never
+    if (d[a] == 0.0f) {  // parallel to this slab: inside for every t, or never
       if (!(l <= 0.0f && 0.0f <= h)) { tf = -INFINITY; }
       continue;
     }
-    const float ta = l * id[a], tb = h * id[a];
+    const float id = 1.0f / d[a];
+    const float ta = l * id, tb = h * id;
     tn = fmaxf(tn, fminf(ta, tb));
     tf = fminf(tf, fmaxf(ta, tb));
   }
@@ -734,24 +738,23 @@ __device__ __forceinline__ void box_interval(const RaySkip& k, float lox, float 
 // [bz0, bz1] (one or two blocks per axis) and the super-blocks holding it.  The lane
 // keeps the last occupancy word it read (consecutive samples mostly share it).
 struct OccCache {
-  int32_t sx, sy, sz;  // super-block of the cached word
+  uint32_t idx;        // map index of the cached word
   uint64_t word;
-  int32_t r[6];        // block range of the last test that found an allocated block
-  bool busy;
+  int32_t rx, ry, rz;  // block range of the last test that found an allocated block:
+  uint32_t rs;         // lower corner, bits 0-2 its extent minus one per axis, bit 3 set when valid
 };
 
 __device__ __forceinline__ void occ_cache_init(OccCache& oc) {
-  oc.sx = oc.sy = oc.sz = INT32_MIN;
+  oc.idx = 0xFFFFFFFFu;
   oc.word = 0ull;
-  oc.busy = false;
+  oc.rs = 0u;
 }
 
 __device__ __forceinline__ uint64_t occ_word(const TableView& t, OccCache& oc, int32_t sx, int32_t sy, int32_t sz) {
-  if (sx != oc.sx || sy != oc.sy || sz != oc.sz) {
-    oc.sx = sx;
-    oc.sy = sy;
-    oc.sz = sz;
-    oc.word = t.occ[occ_index(sx, sy, sz, t.occ_mask)];
+  const uint32_t wi = occ_index(sx, sy, sz, t.occ_mask);
+  if (wi != oc.idx) {
+    oc.idx = wi;
+    oc.word = t.occ[wi];
   }
   return oc.word;
 }
@@ -855,9 +858,9 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs& ra, uint32_t x, uint
   sk.camx = rs.camx;
   sk.camy = rs.camy;
   sk.camz = rs.camz;
-  sk.ix = wdx != 0.0f ? 1.0f / wdx : INFINITY;
-  sk.iy = wdy != 0.0f ? 1.0f / wdy : INFINITY;
-  sk.iz = wdz != 0.0f ? 1.0f / wdz : INFINITY;
+  sk.wdx = wdx;
+  sk.wdy = wdy;
+  sk.wdz = wdz;
   sk.v = rs.v;
   sk.inv_v = 1.0f / rs.v;
   const float mag = fmaxf(fmaxf(fabsf(rs.camx), fabsf(rs.camy)), fabsf(rs.camz)) + fabsf(rs.ray_end);
@@ -879,17 +882,13 @@ __device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const
   const int32_t bz0 = cvt_i32(floorf((qz - mq) * 0.125f)), bz1 = cvt_i32(floorf((qz + mq) * 0.125f));
   *t0 = *t1 = 0.0f;
   // consecutive samples mostly share their block range: an allocated block in it stays
-  if (oc.busy && bx0 == oc.r[0] && bx1 == oc.r[1] && by0 == oc.r[2] && by1 == oc.r[3] && bz0 == oc.r[4] &&
-      bz1 == oc.r[5])
-    return 2;
+  const uint32_t ext = 8u | (uint32_t)(bx1 - bx0) | ((uint32_t)(by1 - by0) << 1) | ((uint32_t)(bz1 - bz0) << 2);
+  if (ext == oc.rs && bx0 == oc.rx && by0 == oc.ry && bz0 == oc.rz) return 2;
   const int occ = occ_test(t, oc, bx0, bx1, by0, by1, bz0, bz1);
-  oc.busy = occ == 2;
-  oc.r[0] = bx0;
-  oc.r[1] = bx1;
-  oc.r[2] = by0;
-  oc.r[3] = by1;
-  oc.r[4] = bz0;
-  oc.r[5] = bz1;
+  oc.rx = bx0;
+  oc.ry = by0;
+  oc.rz = bz0;
+  oc.rs = occ == 2 ? ext : 0u;
   if (occ == 2) return 2;
   const float g = occ == 0 ? 32.0f : 8.0f;
   const int32_t sh = occ == 0 ? 2 : 0;
@@ -969,10 +968,9 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
           // findIntersectionBisection's bracket and the acceptance thresholds
           // (RayCastRender.cu:372-385); the refinement itself comes later
           if (last_w > 0u && last_sdf < 0.0f && dist > 0.0f && fabsf(last_sdf - dist) < th && fabsf(dist) < th) {
+            // the bracket's lower end; its upper end becomes last_alpha / last_sdf below
             h.a = last_alpha;
             h.ad = last_sdf;
-            h.b = ray;
-            h.bd = dist;
             parked = true;
             marching = false;
           }
@@ -993,14 +991,21 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
     if (!__any(marching)) {
       if (parked) {
         // findIntersectionBisection (RayCastRender.cu:229-271)
+        h.b = last_alpha;
+        h.bd = last_sdf;
         float a = h.a, ad = h.ad, bb = h.b, bd = h.bd, c = 0.0f;
         uint32_t rgb = 0;
         bool ok = true;
+#pragma unroll
         for (int it = 0; it < 3; ++it) {
           c = a + (ad / (ad - bd)) * (bb - a);
           float cd;
-          if (trilinear<F16, true, REF>(t, bc, vox, col, camx + c * wdx, camy + c * wdy, camz + c * wdz, v, &cd,
-                                        &rgb) != kSampleValid) {
+          // the colour of the last point is the one returned
+          const int stat = it < 2 ? trilinear<F16, false, REF>(t, bc, vox, col, camx + c * wdx, camy + c * wdy,
+                                                               camz + c * wdz, v, &cd, &rgb)
+                                  : trilinear<F16, true, REF>(t, bc, vox, col, camx + c * wdx, camy + c * wdy,
+                                                              camz + c * wdz, v, &cd, &rgb);
+          if (stat != kSampleValid) {
             ok = false;
             break;
           }
@@ -1230,11 +1235,14 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
         float a = a0, ad = ad0, bb = b0, bd = bd0, c = 0.0f;
         uint32_t rgb = 0;
         bool ok = true;
+#pragma unroll
         for (int it = 0; it < 3; ++it) {
           c = a + (ad / (ad - bd)) * (bb - a);
           float cd;
-          if (trilinear<F16, true, REF>(t, bc, vox, col, rs.camx + c * h.wdx, rs.camy + c * h.wdy,
-                                        rs.camz + c * h.wdz, rs.v, &cd, &rgb) != kSampleValid) {
+          const float qx = rs.camx + c * h.wdx, qy = rs.camy + c * h.wdy, qz = rs.camz + c * h.wdz;
+          const int stat = it < 2 ? trilinear<F16, false, REF>(t, bc, vox, col, qx, qy, qz, rs.v, &cd, &rgb)
+                                  : trilinear<F16, true, REF>(t, bc, vox, col, qx, qy, qz, rs.v, &cd, &rgb);
+          if (stat != kSampleValid) {
             ok = false;
             break;
           }
