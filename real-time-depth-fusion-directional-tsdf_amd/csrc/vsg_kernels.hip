@@ -25,6 +25,7 @@
 // pointers are deterministic.
 #include "vsg_device.hpp"
 #include "vsg_internal.hpp"
+#include "vsg_div.hpp"
 #include "vsg_replay.hpp"
 
 namespace vsg {
@@ -604,14 +605,16 @@ enum : int { kSampleInvalid = 1, kSampleValid = 2 };
 template <bool REF>
 __device__ __forceinline__ void corner_setup(float px, float py, float pz, float v, int32_t vx[2], int32_t vy[2],
                                              int32_t vz[2], float* wx, float* wy, float* wz) {
+  // quotients by the voxel size, correctly rounded (as the reference's divisions)
+  const float rv = 1.0f / v;
   if (REF) {
     const float off = v;
     const float pdx = px - off / 2.0f, pdy = py - off / 2.0f, pdz = pz - off / 2.0f;
-    const float qx0 = pdx / v, qy0 = pdy / v, qz0 = pdz / v;
+    const float qx0 = div_fixed(pdx, v, rv), qy0 = div_fixed(pdy, v, rv), qz0 = div_fixed(pdz, v, rv);
     *wx = qx0 - floorf(qx0);
     *wy = qy0 - floorf(qy0);
     *wz = qz0 - floorf(qz0);
-    const float qx1 = (pdx + off) / v, qy1 = (pdy + off) / v, qz1 = (pdz + off) / v;
+    const float qx1 = div_fixed(pdx + off, v, rv), qy1 = div_fixed(pdy + off, v, rv), qz1 = div_fixed(pdz + off, v, rv);
     vx[0] = ray_voxel_of_q(qx0);
     vx[1] = ray_voxel_of_q(qx1);
     vy[0] = ray_voxel_of_q(qy0);
@@ -619,7 +622,7 @@ __device__ __forceinline__ void corner_setup(float px, float py, float pz, float
     vz[0] = ray_voxel_of_q(qz0);
     vz[1] = ray_voxel_of_q(qz1);
   } else {
-    const float qx = px / v, qy = py / v, qz = pz / v;
+    const float qx = div_fixed(px, v, rv), qy = div_fixed(py, v, rv), qz = div_fixed(pz, v, rv);
     const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
     *wx = qx - fx;
     *wy = qy - fy;
