@@ -53,8 +53,18 @@ typedef struct vsg_params {
   int32_t sdf_block_num; /* voxel blocks N (Params::SDFBlockNum)                   */
   float voxel_size;      /* metres (Params::voxelSize)                             */
   int32_t store;         /* vsg_store                                              */
-  int32_t reserved[4];   /* must be zero                                           */
+  uint32_t flags;        /* VSG_SCENE_* bits                                       */
+  int32_t reserved[3];   /* must be zero                                           */
 } vsg_params;
+
+/* Directional TSDF extension: two direction slots per voxel {id, weight, dtsdf}
+ * updated beside the reference's TSDF.  The compiled reference computes the
+ * rotated pixel normal and drops it (VoxelScene.cu:1074-1094); its per-direction
+ * fields and knobs exist only as comments (CUDAHashRef.h:34-39,
+ * globalParamasConfig.cpp:101-111: 2 slots, cosine 0.8, weight max 100).  The
+ * update rule is this library's definition of that design (see
+ * vsg_export_directional); the reference's own fields are unchanged by it. */
+#define VSG_SCENE_DIRECTIONAL 1u
 
 typedef struct vsg_intrinsics {
   float fx, fy, cx, cy;
@@ -147,6 +157,18 @@ vsg_status vsg_export_blocks(vsg_scene* scene, int32_t* coords, int32_t* slots, 
  * isZeroCrossing, CUDAHashRef.h:23-45).  Any output may be NULL.  Synchronising. */
 vsg_status vsg_export_voxels(vsg_scene* scene, uint32_t first, uint32_t nblocks, float* sdf, uint16_t* weight,
                              uint8_t* rgbf);
+
+/* Direction slots of the voxels of blocks [first, first+nblocks) to HOST memory,
+ * two per voxel (index 2*voxel + slot), voxels in vsg_export_voxels' order: id 0
+ * free, 1..6 = +x, -x, +y, -y, +z, -z (world axes); weight <= 100; dtsdf the
+ * running mean of the truncated SDF of the updates given to the slot.  An update
+ * goes to the voxel's slot of the direction bin of the pixel's world normal (the
+ * signed axis of its largest component) when that component is at least 0.8, else
+ * to the first free slot, else nowhere.  VSG_ERR_INVALID_ARG when the scene was
+ * created without VSG_SCENE_DIRECTIONAL.  Any output may be NULL.  Synchronising.
+ * No counterpart in the compiled reference (extension of CUDAHashRef.h:34-39). */
+vsg_status vsg_export_directional(vsg_scene* scene, uint32_t first, uint32_t nblocks, uint8_t* ids,
+                                  uint8_t* weights, float* dtsdf);
 
 /* ---- renderer: RayCastRender (VoxelSG/include/RayCastRender.h:20-123) ----------- */
 

@@ -67,12 +67,15 @@ inline bool report(vsg_status st, const char* where) {
 // VoxelScene (VoxelSG/include/VoxelScene.h:52-209)
 class VoxelScene {
  public:
-  explicit VoxelScene(const Params& params, vsg_store store = VSG_STORE_FP32) : m_params(params) {
+  // directional: also keep the two direction slots per voxel (vsg_export_directional)
+  explicit VoxelScene(const Params& params, vsg_store store = VSG_STORE_FP32, bool directional = false)
+      : m_params(params) {
     vsg_params p{};
     p.hash_slot_num = params.hashSlotNum;
     p.sdf_block_num = params.SDFBlockNum;
     p.voxel_size = params.voxelSize;
     p.store = store;
+    p.flags = directional ? VSG_SCENE_DIRECTIONAL : 0u;
     m_isAllocated = vsg_detail::report(vsg_scene_create(&p, &m_scene), "VoxelScene");
   }
   ~VoxelScene() {

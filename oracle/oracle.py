@@ -35,8 +35,10 @@ def lib() -> C.CDLL:
         L.vsgo_create.argtypes = [C.c_uint32, C.c_uint32, C.c_double, C.c_int, C.c_int]
         L.vsgo_destroy.argtypes = [vp]
         L.vsgo_reset.argtypes = [vp]
-        L.vsgo_integrate.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_double, C.c_double,
+        L.vsgo_integrate.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int, vp]
+        L.vsgo_set_directional.argtypes = [vp, C.c_int]
+        L.vsgo_export_directional.argtypes = [vp, C.c_uint64, vp, vp, vp]
         L.vsgo_alloc.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double, vp]
         L.vsgo_render.restype = C.c_int
         L.vsgo_render.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -87,10 +89,12 @@ class OracleScene:
     """The reference's hash/voxel state on the CPU (float or double arithmetic)."""
 
     def __init__(self, num_buckets: int, num_blocks: int, voxel: float, store_fp16: bool = False,
-                 fp64: bool = False):
+                 fp64: bool = False, directional: bool = False):
         self.voxel = voxel
         self.fp64 = fp64
         self._h = C.c_void_p(lib().vsgo_create(num_buckets, num_blocks, voxel, int(store_fp16), int(fp64)))
+        if directional:
+            lib().vsgo_set_directional(self._h, 1)
 
     def __del__(self):
         try:
@@ -103,15 +107,27 @@ class OracleScene:
     def reset(self) -> None:
         lib().vsgo_reset(self._h)
 
-    def integrate(self, points, colors, trunc: float, pose, intr, visible_only: bool = False) -> dict:
+    def integrate(self, points, colors, trunc: float, pose, intr, visible_only: bool = False,
+                  normals=None) -> dict:
+        """One integrateFromScanData; `normals` (camera frame, like points) feed the
+        directional extension when it is on."""
         pts = _f32(points)
         H, W = pts.shape[0], pts.shape[1]
         rgb = np.ascontiguousarray(colors, dtype=np.uint8)
+        nrm = None if normals is None else _f32(normals)
         ctr = np.zeros(8, np.uint32)
         fx, fy, cx, cy = intr
-        lib().vsgo_integrate(self._h, pts.ctypes.data, rgb.ctypes.data, W, H, float(trunc), _pose(pose).ctypes.data,
-                             fx, fy, cx, cy, int(visible_only), ctr.ctypes.data)
+        lib().vsgo_integrate(self._h, pts.ctypes.data, rgb.ctypes.data, None if nrm is None else nrm.ctypes.data, W,
+                             H, float(trunc), _pose(pose).ctypes.data, fx, fy, cx, cy, int(visible_only),
+                             ctr.ctypes.data)
         return dict(zip(COUNTER_NAMES, (int(v) for v in ctr)))
+
+    def export_directional(self, nblocks: int) -> dict:
+        """Direction slots of the first nblocks blocks: id, weight, dtsdf, each [nblocks*512, 2]."""
+        n = int(nblocks) * 512 * 2
+        ids, ws, dts = np.zeros(n, np.uint8), np.zeros(n, np.uint8), np.zeros(n, np.float64)
+        lib().vsgo_export_directional(self._h, int(nblocks), ids.ctypes.data, ws.ctypes.data, dts.ctypes.data)
+        return {"id": ids.reshape(-1, 2), "weight": ws.reshape(-1, 2), "dtsdf": dts.reshape(-1, 2)}
 
     def alloc(self, points, trunc: float, pose) -> None:
         pts = _f32(points)

@@ -26,11 +26,11 @@ template <class R> void to_pose(const double* T, R* M) {
 }
 
 template <class R>
-void do_integrate(Scene<R>& s, const float* pts, const uint8_t* rgb, int W, int H, double trunc, const double* T,
-                  double fx, double fy, double cx, double cy, int visible_only, uint32_t* ctr) {
+void do_integrate(Scene<R>& s, const float* pts, const uint8_t* rgb, const float* nrm, int W, int H, double trunc,
+                  const double* T, double fx, double fy, double cx, double cy, int visible_only, uint32_t* ctr) {
   R M[16];
   to_pose(T, M);
-  integrate_frame(s, pts, rgb, W, H, (R)trunc, M, (R)fx, (R)fy, (R)cx, (R)cy, visible_only);
+  integrate_frame(s, pts, rgb, W, H, (R)trunc, M, (R)fx, (R)fy, (R)cx, (R)cy, visible_only, nrm);
   if (ctr) {
     ctr[0] = s.c_valid_pixels;
     ctr[1] = s.c_new_blocks;
@@ -107,12 +107,40 @@ void vsgo_reset(void* p) {
   if (h->fp64) h->sd.reset(); else h->sf.reset();
 }
 
-// counters: valid_pixels, new_blocks, heap_fail, num_allocated, visible, updated, heap_counter, frames
-void vsgo_integrate(void* p, const float* pts, const uint8_t* rgb, int W, int H, double trunc, const double* T16,
-                    double fx, double fy, double cx, double cy, int visible_only, uint32_t* counters) {
+// counters: valid_pixels, new_blocks, heap_fail, num_allocated, visible, updated, heap_counter, frames.
+// nrm (camera-frame normals, 3 floats per pixel) feeds the directional extension; may be null.
+void vsgo_integrate(void* p, const float* pts, const uint8_t* rgb, const float* nrm, int W, int H, double trunc,
+                    const double* T16, double fx, double fy, double cx, double cy, int visible_only,
+                    uint32_t* counters) {
   Handle* h = (Handle*)p;
-  if (h->fp64) do_integrate(h->sd, pts, rgb, W, H, trunc, T16, fx, fy, cx, cy, visible_only, counters);
-  else do_integrate(h->sf, pts, rgb, W, H, trunc, T16, fx, fy, cx, cy, visible_only, counters);
+  if (h->fp64) do_integrate(h->sd, pts, rgb, nrm, W, H, trunc, T16, fx, fy, cx, cy, visible_only, counters);
+  else do_integrate(h->sf, pts, rgb, nrm, W, H, trunc, T16, fx, fy, cx, cy, visible_only, counters);
+}
+
+// Directional extension on (before the first frame) or off.
+void vsgo_set_directional(void* p, int on) {
+  Handle* h = (Handle*)p;
+  h->sd.directional = on;
+  h->sf.directional = on;
+}
+
+// Direction slots of the first nblocks blocks (pointer order): 2 per voxel each.
+void vsgo_export_directional(void* p, uint64_t nblocks, uint8_t* ids, uint8_t* ws, double* dts) {
+  Handle* h = (Handle*)p;
+  const size_t n = (size_t)nblocks * kVox * 2;
+  for (size_t i = 0; i < n; ++i) {
+    if (h->fp64) {
+      const bool have = i < h->sd.dir_id.size();
+      ids[i] = have ? h->sd.dir_id[i] : 0;
+      ws[i] = have ? h->sd.dir_w[i] : 0;
+      dts[i] = have ? h->sd.dtsdf[i] : 0.0;
+    } else {
+      const bool have = i < h->sf.dir_id.size();
+      ids[i] = have ? h->sf.dir_id[i] : 0;
+      ws[i] = have ? h->sf.dir_w[i] : 0;
+      dts[i] = have ? (double)h->sf.dtsdf[i] : 0.0;
+    }
+  }
 }
 
 // Allocation step only (for allocation-level tests).

@@ -154,6 +154,7 @@ template <class R> struct Scene {
   uint32_t num_blocks = 0;   // Params::SDFBlockNum
   R voxel = R(0);            // Params::voxelSize
   int store_fp16 = 0;        // 0: FP32 SDF, 1: FP16 SDF
+  int directional = 0;       // directional TSDF extension (two direction slots per voxel)
 
   std::vector<I3> slot_pos;      // T = B*10 (HashSlot::pos)
   std::vector<int32_t> slot_ptr; // T (HashSlot::ptr, -1 = free)
@@ -168,6 +169,10 @@ template <class R> struct Scene {
   std::vector<uint8_t> weight;
   std::vector<uint8_t> color;  // 3 per voxel
   std::vector<uint8_t> flags;  // bit0 isUpdated, bit1 isZeroCrossing
+  // directional extension, two slots per voxel (index 2*voxel + slot)
+  std::vector<uint8_t> dir_id;  // 0 free, 1..6 = +x, -x, +y, -y, +z, -z
+  std::vector<uint8_t> dir_w;
+  std::vector<R> dtsdf;
   uint32_t frames = 0;
 
   // per-frame counters
@@ -197,6 +202,9 @@ template <class R> struct Scene {
     weight.clear();
     color.clear();
     flags.clear();
+    dir_id.clear();
+    dir_w.clear();
+    dtsdf.clear();
     frames = 0;
   }
 
@@ -207,6 +215,11 @@ template <class R> struct Scene {
     weight.resize(need, 0);
     color.resize(need * 3, 0);
     flags.resize(need, 0);
+    if (directional) {
+      dir_id.resize(2 * need, 0);
+      dir_w.resize(2 * need, 0);
+      dtsdf.resize(2 * need, R(0));
+    }
   }
 
   R load_sdf(size_t i) const { return store_fp16 ? R(f16_to_f32(sdf_h[i])) : sdf[i]; }
@@ -364,9 +377,75 @@ inline bool block_maybe_visible(const R* M, I3 c, R v, int W, int H, R fx, R fy,
 // ---- A8: integrate one frame into the listed blocks (VoxelScene.cu:918-1146).
 // The normal map is read by the reference but its value never reaches the store
 // (VoxelScene.cu:1075-1094), so it is not an input here.
+// ---- A9: directional TSDF extension ------------------------------------------------
+//
+// The compiled reference computes the world normal of every integrated voxel's pixel
+// and drops it (VoxelScene.cu:1074-1094); the per-direction fields and knobs exist
+// only as comments (CUDAHashRef.h:34-39: dir_id, dir_w, dtsdf for 2 slots;
+// globalParamasConfig.cpp:101-111,416: 2 slots, cosine threshold 0.8, weight max 100).
+// This extension is the build's own definition of that design, pinned by its own
+// tests (no reference behaviour pins it):
+//   * direction bin of a unit world normal n: the signed axis of its largest |n_a|
+//     (ties to the lower axis), id = 1 + 2a + (n_a < 0): +x, -x, +y, -y, +z, -z;
+//   * no update when that |n_a| (the cosine to the bin) is below 0.8 or n is zero;
+//   * the slot whose id matches, else the first free slot (weight 0), else none;
+//   * dtsdf <- (dtsdf * w + sdf) / (w + 1), w <- min(w + 1, 100), with the truncated
+//     SDF of the main update.
+template <class R> inline void dir_update(Scene<R>& s, size_t vi, V3<R> n, R sdfv) {
+  const R ax = std::fabs(n.x), ay = std::fabs(n.y), az = std::fabs(n.z);
+  int a = 0;
+  R c = ax;
+  if (ay > c) {
+    a = 1;
+    c = ay;
+  }
+  if (az > c) {
+    a = 2;
+    c = az;
+  }
+  if (!(c >= R(0.8f))) return;
+  const R comp = a == 0 ? n.x : (a == 1 ? n.y : n.z);
+  const uint8_t id = (uint8_t)(1 + 2 * a + (comp < R(0) ? 1 : 0));
+  int slot = -1;
+  for (int j = 1; j >= 0; --j)
+    if (s.dir_id[2 * vi + j] == id) slot = j;
+  if (slot < 0) {
+    for (int j = 1; j >= 0; --j)
+      if (s.dir_w[2 * vi + j] == 0) slot = j;
+  }
+  if (slot < 0) return;
+  const size_t k = 2 * vi + (size_t)slot;
+  const R ow = (R)s.dir_w[k];
+  const R nw = ow + R(1);
+  s.dtsdf[k] = (s.dtsdf[k] * ow + sdfv * R(1)) / nw;
+  s.dir_w[k] = (uint8_t)std::fmin(R(100), nw);
+  s.dir_id[k] = id;
+}
+
+// World normal of a pixel as the reference computes it (VoxelScene.cu:1074-1094).
+template <class R> inline V3<R> pixel_world_normal(const float* nrm, int pix, const R* M) {
+  V3<R> nc{(R)nrm[3 * pix], (R)nrm[3 * pix + 1], (R)nrm[3 * pix + 2]};
+  R l = std::sqrt(nc.x * nc.x + nc.y * nc.y + nc.z * nc.z);
+  if (l > R(0)) {
+    nc.x /= l;
+    nc.y /= l;
+    nc.z /= l;
+  }
+  V3<R> n{M[0] * nc.x + M[1] * nc.y + M[2] * nc.z, M[4] * nc.x + M[5] * nc.y + M[6] * nc.z,
+          M[8] * nc.x + M[9] * nc.y + M[10] * nc.z};
+  l = std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+  if (l > R(1e-6f)) {
+    n.x /= l;
+    n.y /= l;
+    n.z /= l;
+  }
+  return n;
+}
+
 template <class R>
 inline void integrate_block(Scene<R>& s, I3 bc, int32_t ptr, const float* pts, const uint8_t* rgb, int W, int H,
-                            R trunc, const R* M, R fx, R fy, R cx, R cy, uint32_t* updated) {
+                            R trunc, const R* M, R fx, R fy, R cx, R cy, uint32_t* updated,
+                            const float* nrm = nullptr) {
   const R v = s.voxel;
   V3<R> cam{M[3], M[7], M[11]};
   uint32_t upd = 0;
@@ -410,6 +489,7 @@ inline void integrate_block(Scene<R>& s, I3 bc, int32_t ptr, const float* pts, c
       uint8_t fl = s.flags[vi] | 1u;
       if ((old_sdf > R(0) && stored <= R(0)) || (old_sdf <= R(0) && stored > R(0))) fl |= 2u;
       s.flags[vi] = fl;
+      if (s.directional && nrm) dir_update(s, vi, pixel_world_normal(nrm, pix, M), sdfv);
       upd++;
     }
   }
@@ -422,7 +502,7 @@ inline void integrate_block(Scene<R>& s, I3 bc, int32_t ptr, const float* pts, c
 // result must be identical (that is what the test of the culling checks).
 template <class R>
 inline void integrate_frame(Scene<R>& s, const float* pts, const uint8_t* rgb, int W, int H, R trunc, const R* M,
-                            R fx, R fy, R cx, R cy, int visible_only) {
+                            R fx, R fy, R cx, R cy, int visible_only, const float* nrm = nullptr) {
   alloc_frame(s, pts, W, H, trunc, M);
   // compaction: every live entry, in table order
   std::vector<std::pair<I3, int32_t>> list;
@@ -449,7 +529,7 @@ inline void integrate_frame(Scene<R>& s, const float* pts, const uint8_t* rgb, i
   for (long i = 0; i < (long)list.size(); ++i) {
     if (!keep[i]) continue;
     visible++;
-    integrate_block(s, list[i].first, list[i].second, pts, rgb, W, H, trunc, M, fx, fy, cx, cy, &updated);
+    integrate_block(s, list[i].first, list[i].second, pts, rgb, W, H, trunc, M, fx, fy, cx, cy, &updated, nrm);
   }
   s.c_updated = updated;
   s.c_visible = visible;

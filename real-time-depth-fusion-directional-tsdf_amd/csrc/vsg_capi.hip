@@ -119,6 +119,8 @@ struct vsg_scene {
   uint8_t* mark_rbkt = nullptr;
   uint8_t* mark_blk = nullptr;
   uint32_t* march_stats = nullptr;  // 4 counters
+  uint32_t* dmeta = nullptr;        // directional extension: slot ids and weights per voxel
+  float2* dval = nullptr;           // directional extension: slot dtsdf per voxel
 };
 
 struct vsg_render {
@@ -157,6 +159,8 @@ void free_scene(vsg_scene* s) {
   (void)hipFree(s->t.blk);
   (void)hipFree(s->t.occ);
   (void)hipFree(s->t.idx);
+  (void)hipFree(s->dmeta);
+  (void)hipFree(s->dval);
   (void)hipFree(s->vox);
   (void)hipFree(s->col);
   (void)hipFree(s->vis);
@@ -187,7 +191,7 @@ template <class T> vsg_status dmalloc(T** p, size_t bytes, const char* what) {
 vsg_status queue_reset(vsg_scene* s) {
   {
     TimedLaunch tl(s->stream, VSG_K_RESET);
-    launch_reset_voxels(s->stream, s->st, s->vox, s->word_bytes, s->col);
+    launch_reset_voxels(s->stream, s->st, s->vox, s->word_bytes, s->col, s->dmeta, s->dval);
   }
   vsg_status r = check_launch("reset voxels");
   if (r != VSG_OK) return r;
@@ -238,7 +242,8 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
     return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: voxel_size must be a positive finite number");
   if (p.store != VSG_STORE_FP32 && p.store != VSG_STORE_FP16)
     return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: store must be VSG_STORE_FP32 or VSG_STORE_FP16");
-  for (int i = 0; i < 4; ++i)
+  if ((p.flags & ~VSG_SCENE_DIRECTIONAL) != 0u) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: unknown flags");
+  for (int i = 0; i < 3; ++i)
     if (p.reserved[i] != 0) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_create: reserved fields must be zero");
 
   vsg_scene* s = new vsg_scene();
@@ -273,6 +278,10 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   if (r == VSG_OK) r = dmalloc(&s->vox, N * kVox * s->word_bytes, "voxel store");
   if (r == VSG_OK) r = dmalloc(&s->col, N * kVox * sizeof(uint32_t), "colour store");
   if (r == VSG_OK) r = dmalloc(&s->vis, N * sizeof(uint32_t), "visible list");
+  if (r == VSG_OK && (p.flags & VSG_SCENE_DIRECTIONAL)) {
+    r = dmalloc(&s->dmeta, N * kVox * sizeof(uint32_t), "direction slots");
+    if (r == VSG_OK) r = dmalloc(&s->dval, N * kVox * sizeof(float2), "direction slot values");
+  }
   if (r == VSG_OK) r = dmalloc(&s->st, sizeof(DevState), "scene state");
   if (r == VSG_OK) r = dmalloc(&s->upd_part, kIntegrateGrid * sizeof(uint32_t), "integrate counters");
   if (r != VSG_OK) {
@@ -282,6 +291,8 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   // new scene: voxel memory zero (CUDAHashRef.cu:86-93), then reset
   e = hipMemsetAsync(s->vox, 0, N * kVox * s->word_bytes, s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->col, 0, N * kVox * sizeof(uint32_t), s->stream);
+  if (e == hipSuccess && s->dmeta) e = hipMemsetAsync(s->dmeta, 0, N * kVox * sizeof(uint32_t), s->stream);
+  if (e == hipSuccess && s->dval) e = hipMemsetAsync(s->dval, 0, N * kVox * sizeof(float2), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->t.blk, 0, N * sizeof(int4), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->upd_part, 0, kIntegrateGrid * sizeof(uint32_t), s->stream);
@@ -406,8 +417,8 @@ vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, con
   if ((r = check_launch("compact")) != VSG_OK) return r;
   {
     TimedLaunch tl(s->stream, VSG_K_INTEGRATE);
-    launch_integrate(s->stream, fp16, s->t, s->st, fa, pts, rgb, s->vis, s->vox, s->col, kIntegrateGrid,
-                     s->upd_part);
+    launch_integrate(s->stream, fp16, s->t, s->st, fa, pts, rgb, (const float*)f->normals, s->vis, s->vox, s->col,
+                     s->dmeta, s->dval, kIntegrateGrid, s->upd_part);
   }
   if ((r = check_launch("integrate")) != VSG_OK) return r;
   s->frames++;
@@ -477,6 +488,36 @@ vsg_status vsg_export_blocks(vsg_scene* s, int32_t* coords, int32_t* slots, uint
       coords[3 * i + 2] = h[i].z;
     }
     if (slots) slots[i] = h[i].w;
+  }
+  return VSG_OK;
+}
+
+vsg_status vsg_export_directional(vsg_scene* s, uint32_t first, uint32_t nblocks, uint8_t* ids, uint8_t* weights,
+                                  float* dtsdf) {
+  if (!s) return fail(VSG_ERR_INVALID_ARG, "vsg_export_directional: null scene");
+  if (!s->dmeta) return fail(VSG_ERR_INVALID_ARG, "vsg_export_directional: scene created without VSG_SCENE_DIRECTIONAL");
+  if ((uint64_t)first + nblocks > s->t.num_blocks)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_export_directional: block range beyond the store");
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  const size_t nv = (size_t)nblocks * kVox, off = (size_t)first * kVox;
+  if (nv == 0) return VSG_OK;
+  std::vector<uint32_t> meta(nv);
+  std::vector<float2> val(nv);
+  VSG_HIP(hipMemcpy(meta.data(), s->dmeta + off, nv * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  VSG_HIP(hipMemcpy(val.data(), s->dval + off, nv * sizeof(float2), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < nv; ++i) {
+    if (ids) {
+      ids[2 * i] = (uint8_t)(meta[i] & 0xFFu);
+      ids[2 * i + 1] = (uint8_t)((meta[i] >> 16) & 0xFFu);
+    }
+    if (weights) {
+      weights[2 * i] = (uint8_t)((meta[i] >> 8) & 0xFFu);
+      weights[2 * i + 1] = (uint8_t)(meta[i] >> 24);
+    }
+    if (dtsdf) {
+      dtsdf[2 * i] = val[i].x;
+      dtsdf[2 * i + 1] = val[i].y;
+    }
   }
   return VSG_OK;
 }

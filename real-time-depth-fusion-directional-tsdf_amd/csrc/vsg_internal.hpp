@@ -41,7 +41,8 @@ constexpr uint32_t kDefaultStepBudget = 96;
 // Workgroups of the tail pass for a W x H render.
 uint32_t tail_workgroups(int32_t W, int32_t H);
 
-void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col);
+void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col,
+                         uint32_t* dmeta, float2* dval);
 void launch_reset_table(hipStream_t s, const TableView& t, DevState* st);
 constexpr uint32_t kIntegrateGrid = 2048;  // workgroups of k_integrate (grid-stride over the list)
 void launch_alloc_claim(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
@@ -55,8 +56,8 @@ void launch_alloc_insert(hipStream_t s, const TableView& t, DevState* st, const 
 void launch_compact(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, uint32_t* vis,
                     uint32_t grid);
 void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
-                      const float* pts, const uint8_t* rgb, const uint32_t* vis, void* vox, uint32_t* col,
-                      uint32_t grid, uint32_t* upd_part);
+                      const float* pts, const uint8_t* rgb, const float* nrm, const uint32_t* vis, void* vox,
+                      uint32_t* col, uint32_t* dmeta, float2* dval, uint32_t grid, uint32_t* upd_part);
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                            const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
                            float* grad_sdf);

@@ -414,11 +414,73 @@ template <> struct Store<true> {
   }
 };
 
-template <bool F16>
+// Directional extension (DIR): the voxel's two direction slots {id, weight} are the
+// bytes of dmeta[v] (id0, w0, id1, w1) and their dtsdf values dval[v].  The bin of
+// the pixel's unit world normal is the signed axis of its largest component; no
+// update when that component is below 0.8 (a zero normal included); the slot is
+// the one holding the bin, else the first free one (weight 0), else none; dtsdf
+// becomes (dtsdf * w + sdf) / (w + 1) and w min(w + 1, 100).  The world normal is
+// the reference's (VoxelScene.cu:1074-1094): the camera-frame normal normalised when
+// its length is positive, rotated, normalised when its length exceeds 1e-6.
+__device__ __forceinline__ void dir_update(uint32_t* __restrict__ dmeta, float2* __restrict__ dval, size_t vi,
+                                           const float* __restrict__ nrm, int pix, const float* M, float sdf) {
+  float ncx = nrm[3 * pix], ncy = nrm[3 * pix + 1], ncz = nrm[3 * pix + 2];
+  float l = sqrtf(ncx * ncx + ncy * ncy + ncz * ncz);
+  if (l > 0.0f) {
+    ncx /= l;
+    ncy /= l;
+    ncz /= l;
+  }
+  float nx = M[0] * ncx + M[1] * ncy + M[2] * ncz;
+  float ny = M[4] * ncx + M[5] * ncy + M[6] * ncz;
+  float nz = M[8] * ncx + M[9] * ncy + M[10] * ncz;
+  l = sqrtf(nx * nx + ny * ny + nz * nz);
+  if (l > 1e-6f) {
+    nx /= l;
+    ny /= l;
+    nz /= l;
+  }
+  const float ax = fabsf(nx), ay = fabsf(ny), az = fabsf(nz);
+  int a = 0;
+  float c = ax;
+  if (ay > c) {
+    a = 1;
+    c = ay;
+  }
+  if (az > c) {
+    a = 2;
+    c = az;
+  }
+  if (!(c >= 0.8f)) return;
+  const float comp = a == 0 ? nx : (a == 1 ? ny : nz);
+  const uint32_t id = 1u + 2u * (uint32_t)a + (comp < 0.0f ? 1u : 0u);
+  uint32_t meta = dmeta[vi];
+  const uint32_t id0 = meta & 0xFFu, w0 = (meta >> 8) & 0xFFu, id1 = (meta >> 16) & 0xFFu, w1 = meta >> 24;
+  int slot = id0 == id ? 0 : (id1 == id ? 1 : (w0 == 0u ? 0 : (w1 == 0u ? 1 : -1)));
+  if (slot < 0) return;
+  float2 d = dval[vi];
+  const float ow = (float)(slot == 0 ? w0 : w1);
+  const float nw = ow + 1.0f;
+  const float nd = ((slot == 0 ? d.x : d.y) * ow + sdf * 1.0f) / nw;
+  const uint32_t wq = (uint32_t)fminf(100.0f, nw);
+  if (slot == 0) {
+    d.x = nd;
+    meta = (meta & 0xFFFF0000u) | id | (wq << 8);
+  } else {
+    d.y = nd;
+    meta = (meta & 0x0000FFFFu) | (id << 16) | (wq << 24);
+  }
+  dval[vi] = d;
+  dmeta[vi] = meta;
+}
+
+template <bool F16, bool DIR>
 __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, FrameArgs fa,
                                                    const float* __restrict__ pts, const uint8_t* __restrict__ rgb,
-                                                   const uint32_t* __restrict__ vis, void* __restrict__ vox_,
-                                                   uint32_t* __restrict__ col, uint32_t* __restrict__ upd_part) {
+                                                   const float* __restrict__ nrm, const uint32_t* __restrict__ vis,
+                                                   void* __restrict__ vox_, uint32_t* __restrict__ col,
+                                                   uint32_t* __restrict__ dmeta, float2* __restrict__ dval,
+                                                   uint32_t* __restrict__ upd_part) {
   using S = Store<F16>;
   typename S::word* vox = reinterpret_cast<typename S::word*>(vox_);
   FrameCounters& fc = st->set[fa.cset];
@@ -479,6 +541,7 @@ __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, Fr
       if ((old_sdf > 0.0f && stored <= 0.0f) || (old_sdf <= 0.0f && stored > 0.0f)) fl |= 2u;
       vox[vi] = nwd;
       col[vi] = nc | (fl << 24);
+      if (DIR) dir_update(dmeta, dval, vi, nrm, pix, M, sdf);
       upd++;
     }
   }
@@ -1675,7 +1738,7 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* __restrict__ part
 // again after a reset would start from stale values; zeroing them keeps a reset
 // scene equal to a new one.  Runs before k_reset_table, which clears n_alloc.
 __global__ __launch_bounds__(256) void k_reset_voxels(const DevState* st, void* vox, uint32_t word_bytes,
-                                                      uint32_t* col) {
+                                                      uint32_t* col, uint32_t* dmeta, float2* dval) {
   const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * 256;
   const size_t nvox = (size_t)st->n_alloc * kVox;
@@ -1683,6 +1746,10 @@ __global__ __launch_bounds__(256) void k_reset_voxels(const DevState* st, void* 
     if (word_bytes == 8) reinterpret_cast<uint2*>(vox)[i] = make_uint2(0u, 0u);
     else reinterpret_cast<uint32_t*>(vox)[i] = 0u;
     col[i] = 0u;
+    if (dmeta) {
+      dmeta[i] = 0u;
+      dval[i] = make_float2(0.0f, 0.0f);
+    }
   }
 }
 
@@ -1715,8 +1782,9 @@ __global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) 
 // Launchers
 // ---------------------------------------------------------------------------------
 
-void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col) {
-  hipLaunchKernelGGL(k_reset_voxels, dim3(2048), dim3(256), 0, s, st, vox, word_bytes, col);
+void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col,
+                         uint32_t* dmeta, float2* dval) {
+  hipLaunchKernelGGL(k_reset_voxels, dim3(2048), dim3(256), 0, s, st, vox, word_bytes, col, dmeta, dval);
 }
 
 void launch_reset_table(hipStream_t s, const TableView& t, DevState* st) {
@@ -1752,12 +1820,18 @@ void launch_compact(hipStream_t s, const TableView& t, DevState* st, const Frame
 }
 
 void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
-                      const float* pts, const uint8_t* rgb, const uint32_t* vis, void* vox, uint32_t* col,
-                      uint32_t grid, uint32_t* upd_part) {
-  if (fp16)
-    hipLaunchKernelGGL(k_integrate<true>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col, upd_part);
-  else
-    hipLaunchKernelGGL(k_integrate<false>, dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, vis, vox, col, upd_part);
+                      const float* pts, const uint8_t* rgb, const float* nrm, const uint32_t* vis, void* vox,
+                      uint32_t* col, uint32_t* dmeta, float2* dval, uint32_t grid, uint32_t* upd_part) {
+#define VSG_INTEGRATE(F, D)                                                                                       \
+  hipLaunchKernelGGL((k_integrate<F, D>), dim3(grid), dim3(512), 0, s, t, st, fa, pts, rgb, nrm, vis, vox, col, dmeta, \
+                     dval, upd_part)
+  const bool dir = dmeta != nullptr && nrm != nullptr;
+  if (fp16) {
+    if (dir) VSG_INTEGRATE(true, true); else VSG_INTEGRATE(true, false);
+  } else {
+    if (dir) VSG_INTEGRATE(false, true); else VSG_INTEGRATE(false, false);
+  }
+#undef VSG_INTEGRATE
 }
 
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,

@@ -42,7 +42,10 @@ class VSGError(RuntimeError):
 
 class vsg_params(C.Structure):
     _fields_ = [("hash_slot_num", C.c_int32), ("sdf_block_num", C.c_int32), ("voxel_size", C.c_float),
-                ("store", C.c_int32), ("reserved", C.c_int32 * 4)]
+                ("store", C.c_int32), ("flags", C.c_uint32), ("reserved", C.c_int32 * 3)]
+
+
+SCENE_DIRECTIONAL = 1
 
 
 class vsg_intrinsics(C.Structure):
@@ -82,6 +85,7 @@ _SIGNATURES = {
     "vsg_num_integrated_frames": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "vsg_scene_params": (C.c_int, [C.c_void_p, C.POINTER(vsg_params)]),
     "vsg_export_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "vsg_export_directional": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_export_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_render_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
     "vsg_render_destroy": (C.c_int, [C.c_void_p]),
@@ -182,7 +186,9 @@ class VoxelScene:
     store: "fp32" (f32 SDF + u16 weight) or "fp16" (f16 SDF + u16 weight).
     """
 
-    def __init__(self, params: Params, store: str = "fp32"):
+    def __init__(self, params: Params, store: str = "fp32", directional: bool = False):
+        """directional: also keep the two direction slots per voxel of the
+        directional TSDF extension (export_directional)."""
         self.m_params = params
         self.m_store = store
         p = vsg_params()
@@ -190,6 +196,7 @@ class VoxelScene:
         p.sdf_block_num = params.SDFBlockNum
         p.voxel_size = params.voxelSize
         p.store = {"fp32": STORE_FP32, "fp16": STORE_FP16}[store]
+        p.flags = SCENE_DIRECTIONAL if directional else 0
         h = C.c_void_p()
         _check(lib().vsg_scene_create(C.byref(p), C.byref(h)))
         self._h = h
@@ -342,6 +349,20 @@ class VoxelScene:
             _check(lib().vsg_export_voxels(self._h, first, nblocks, sdf.ctypes.data, w.ctypes.data,
                                            rgbf.ctypes.data))
         return sdf, w, rgbf
+
+    def export_directional(self, first: int = 0, nblocks: int | None = None) -> dict:
+        """Direction slots of blocks [first, first+n): id, weight (uint8) and dtsdf
+        (float32), each [n*512, 2] (id 0 free, 1..6 = +x, -x, +y, -y, +z, -z)."""
+        if nblocks is None:
+            nblocks = self.m_params.SDFBlockNum - self.getHeapFreeCount() - first
+        nv = nblocks * 512
+        ids = np.zeros((nv, 2), np.uint8)
+        ws = np.zeros((nv, 2), np.uint8)
+        dts = np.zeros((nv, 2), np.float32)
+        if nblocks:
+            _check(lib().vsg_export_directional(self._h, first, nblocks, ids.ctypes.data, ws.ctypes.data,
+                                                dts.ctypes.data))
+        return {"id": ids, "weight": ws, "dtsdf": dts}
 
 
 class RayCastRender:

@@ -16,11 +16,11 @@ class GpuRun:
     """A VoxelScene + RayCastRender on cuda:0 fed from host frames."""
 
     def __init__(self, torch, wl: synth.Workload, store: str | None = None, buckets: int | None = None,
-                 blocks: int | None = None):
+                 blocks: int | None = None, directional: bool = False):
         self.torch = torch
         self.wl = wl
         self.params = vsg.Params(buckets or wl.buckets, blocks or wl.blocks, wl.voxel, wl.trunc)
-        self.scene = vsg.VoxelScene(self.params, store or wl.store)
+        self.scene = vsg.VoxelScene(self.params, store or wl.store, directional=directional)
         self.scene.set_stream(torch.cuda.current_stream().cuda_stream)
         self.render = None
 
@@ -57,13 +57,13 @@ class GpuRun:
 
 class OracleRun:
     def __init__(self, wl: synth.Workload, store: str | None = None, buckets: int | None = None,
-                 blocks: int | None = None, fp64: bool = False):
+                 blocks: int | None = None, fp64: bool = False, directional: bool = False):
         self.wl = wl
         self.o = O.OracleScene(buckets or wl.buckets, blocks or wl.blocks, wl.voxel,
-                               store_fp16=(store or wl.store) == "fp16", fp64=fp64)
+                               store_fp16=(store or wl.store) == "fp16", fp64=fp64, directional=directional)
 
     def integrate(self, f: dict, intr) -> dict:
-        return self.o.integrate(f["points"], f["colors"], self.wl.trunc, f["pose"], intr)
+        return self.o.integrate(f["points"], f["colors"], self.wl.trunc, f["pose"], intr, normals=f["normals"])
 
     def raycast(self, f: dict, intr, pixels=None) -> dict:
         H, W = f["depth"].shape

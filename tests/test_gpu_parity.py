@@ -160,3 +160,31 @@ def test_room_late_frame_tail_pass(torch_cuda):
         counts.add((gr["hits"], gr["misses"]))
         print(f"budget {budget}: {int((gr['depth'][pix] < 1e9).sum())} of {pix.size} pixels hit, bit-identical")
     assert len(counts) == 1, counts
+
+
+@pytest.mark.parametrize("store", ["fp32", "fp16"])
+def test_directional_extension_sphere_arc(torch_cuda, store):
+    """Workload 0 with the directional extension on: after 30 frames the direction
+    slots (ids, weights, dtsdf) equal the oracle's bit for bit, and the reference's
+    fields are the same as without the extension."""
+    wl = synth.WORKLOADS[0]
+    sc = synth.SynthScene(0)
+    g, o = GpuRun(torch_cuda, wl, store=store, directional=True), OracleRun(wl, store=store, directional=True)
+    plain = GpuRun(torch_cuda, wl, store=store)
+    for i in range(sc.nframes):
+        f = sc.render(i)
+        g.integrate(f, sc.intrinsics)
+        plain.integrate(f, sc.intrinsics)
+        o.integrate(f, sc.intrinsics)
+    info = compare_state(g, o)
+    n = g.scene.counters()["num_allocated"]
+    gd, od = g.scene.export_directional(0, n), o.o.export_directional(n)
+    assert np.array_equal(gd["id"], od["id"])
+    assert np.array_equal(gd["weight"], od["weight"])
+    assert np.array_equal(gd["dtsdf"].view(np.uint32), od["dtsdf"].astype(np.float32).view(np.uint32))
+    used = gd["weight"] > 0
+    print(f"directional {store}: {info}; slots used {int(used[:, 0].sum())} / {int(used[:, 1].sum())}, "
+          f"ids {np.bincount(gd['id'].reshape(-1), minlength=7).tolist()}")
+    assert used[:, 0].sum() > 10000
+    for a, b in zip(g.voxels(n), plain.voxels(n)):
+        assert np.array_equal(a, b)
