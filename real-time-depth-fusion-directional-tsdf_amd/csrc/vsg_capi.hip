@@ -777,7 +777,8 @@ vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_d
   if (st != VSG_OK) return st;
   {
     TimedLaunch tl(r->last, VSG_K_GN);
-    launch_gn(r->last, r->depth, observed_depth, r->jac, npix, max_residual, lambda, pose, r->partials, r->gn_out);
+    launch_gn(r->last, r->depth, observed_depth, r->jac, r->W, r->H, max_residual, lambda, pose, r->partials,
+              r->gn_out);
   }
   st = check_launch("gauss-newton");
   if (st != VSG_OK) return st;

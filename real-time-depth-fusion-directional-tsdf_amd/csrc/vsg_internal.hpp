@@ -63,7 +63,7 @@ void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const D
                            float* grad_sdf);
 uint32_t gn_partial_count(int npix);
 // out: new pose (16 doubles), delta (6), cost, count, solved flag (25 doubles)
-void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int npix, float max_res,
+void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int W, int H, float max_res,
                double lambda, const double pose[16], double* partials, double* out);
 void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                    const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,

@@ -1587,18 +1587,30 @@ __device__ __forceinline__ double shfl_xor_d(double v, int o) {
 }
 
 // Per-workgroup partial sums of J J^T, J r, r^2 and the count over pixels with a
-// rendered and an observed depth and |r| < max_res (r = rendered - observed).
+// rendered and an observed depth and |r| < max_res (r = rendered - observed), away
+// from depth edges: a pixel whose four neighbours are not all rendered within
+// max(2 cm, 2 % of its depth) of it is left out, since a small pose change can move
+// its ray onto another surface, which the Jacobian does not describe.
 __global__ __launch_bounds__(256) void k_gn_accum(const float* __restrict__ depth, const float* __restrict__ obs,
-                                                  const float* __restrict__ jac, int npix, float max_res,
+                                                  const float* __restrict__ jac, int W, int H, float max_res,
                                                   double* __restrict__ partials) {
+  const int npix = W * H;
   const int pix = blockIdx.x * 256 + threadIdx.x;
   double acc[kGnTerms];
 #pragma unroll
   for (int k = 0; k < kGnTerms; ++k) acc[k] = 0.0;
   if (pix < npix) {
+    const int x = pix % W, y = pix / W;
     const float d = depth[pix], o = obs[pix];
     const float r = d - o;
-    if (d < 1e9f && o > 0.0f && fabsf(r) < max_res) {
+    bool use = d < 1e9f && o > 0.0f && fabsf(r) < max_res && x > 0 && y > 0 && x + 1 < W && y + 1 < H;
+    if (use) {
+      const float tol = fmaxf(0.02f, 0.02f * d);
+      const float nb[4] = {depth[pix - 1], depth[pix + 1], depth[pix - W], depth[pix + W]};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) use = use && nb[k] < 1e9f && fabsf(nb[k] - d) <= tol;
+    }
+    if (use) {
       double Jd[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) Jd[k] = (double)jac[6 * (size_t)pix + k];
@@ -1851,10 +1863,10 @@ void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const D
 
 uint32_t gn_partial_count(int npix) { return (uint32_t)((npix + 255) / 256); }
 
-void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int npix, float max_res,
+void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int W, int H, float max_res,
                double lambda, const double pose[16], double* partials, double* out) {
-  const uint32_t nwg = gn_partial_count(npix);
-  hipLaunchKernelGGL(k_gn_accum, dim3(nwg), dim3(256), 0, s, depth, obs, jac, npix, max_res, partials);
+  const uint32_t nwg = gn_partial_count(W * H);
+  hipLaunchKernelGGL(k_gn_accum, dim3(nwg), dim3(256), 0, s, depth, obs, jac, W, H, max_res, partials);
   Pose64 p;
   for (int i = 0; i < 16; ++i) p.T[i] = pose[i];
   hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, partials, (int)nwg, lambda, p, out);
