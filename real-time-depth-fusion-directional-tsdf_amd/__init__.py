@@ -4,6 +4,8 @@ Modules:
     vsg    ctypes host mirror of the reference's VoxelScene / RayCastRender API
            over the C ABI in include/vsg.h (lib/libvsg.so, HIP kernels for gfx950)
     synth  seeded synthetic depth frames of the fusion workloads (lib/libvsg_synth.so)
+    autodiff  torch.autograd wrapper of the raycast adjoint (rendered depth as a
+              function of a pose perturbation and of the stored SDF)
 
 The directory name is not a Python identifier; load it with :func:`load` from a
 file path, e.g. ``importlib`` or the helper in ``__graft_entry__``.

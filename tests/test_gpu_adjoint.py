@@ -156,30 +156,20 @@ def test_sdf_gradient_central_differences(torch_cuda):
 
 
 def test_autograd_function(torch_cuda):
+    """vsg_amd.autodiff.render_depth: the pose and SDF gradients of a weighted depth
+    loss equal the adjoint's Jacobian and vector-Jacobian product."""
     torch = torch_cuda
+    from vsg_amd import autodiff
     wl, sc, g, o, f = _fused(torch)
     T0 = f["pose"].astype(np.float64)
     mn, mx = synth.depth_range(f["depth"])
-    fx, fy, cx, cy = sc.intrinsics
     n = sc.W * sc.H
-
-    class RaycastDepth(torch.autograd.Function):
-        """depth(xi) = render at T0 * exp(xi^); backward through the raycast adjoint."""
-
-        @staticmethod
-        def forward(ctx, xi):
-            T = T0 @ O.se3_exp(xi.detach().cpu().numpy())
-            d = _render_depth(g, sc, T, mn, mx)
-            return torch.from_numpy(d).cuda()
-
-        @staticmethod
-        def backward(ctx, gout):
-            jac = torch.zeros(6 * n, dtype=torch.float32, device="cuda")
-            g.render.adjoint(g.scene, jac=jac)
-            return (jac.view(n, 6).double() * gout.double().view(n, 1)).sum(0).to(gout.device).cpu()
-
+    g.render = vsg.RayCastRender()
+    assert g.render.initialize(sc.W, sc.H)
     xi = torch.zeros(6, dtype=torch.float64, requires_grad=True)
-    depth = RaycastDepth.apply(xi)
+    sdf = torch.zeros(wl.blocks * 512, dtype=torch.float32, device="cuda", requires_grad=True)
+    depth = autodiff.render_depth(g.render, g.scene, xi, T0, sc.intrinsics, mn, mx, sdf=sdf)
+    assert np.array_equal(depth.cpu().numpy(), _render_depth(g, sc, T0, mn, mx))
     valid = depth < 1e9
     w = torch.linspace(-1.0, 1.0, n, device="cuda") * valid
     loss = (w * torch.where(valid, depth, torch.zeros_like(depth))).sum()
@@ -188,6 +178,14 @@ def test_autograd_function(torch_cuda):
     expect = (J * w.cpu().numpy()[:, None]).sum(0)
     assert np.allclose(xi.grad.numpy(), expect, rtol=1e-5, atol=1e-6)
     assert np.abs(expect).max() > 0
+    gsdf = torch.zeros(wl.blocks * 512, dtype=torch.float32, device="cuda")
+    g.render.adjoint(g.scene, grad_depth=w.float().contiguous(), grad_sdf=gsdf)
+    torch.cuda.synchronize()
+    ref = gsdf.cpu().numpy()
+    got = sdf.grad.cpu().numpy()
+    assert np.abs(ref).max() > 0
+    # the same atomic sums in another order: equal to float rounding
+    assert np.allclose(got, ref, rtol=1e-4, atol=1e-6)
 
 
 @pytest.mark.parametrize("sampling", ["trilinear", "reference"])

@@ -76,3 +76,21 @@ def test_sdf_gradient_matches_central_differences():
     assert checked > 60 and agree / checked > 0.95, (agree, checked)
     # the batched gradient is the sum of the single-pixel ones
     assert np.abs(gs).sum() > 0
+
+
+def test_package_se3_exp_matches_matrix_exponential():
+    """The host-side exp used by the autograd wrapper equals scipy's expm of the twist
+    matrix (including the small-angle branch) and the oracle's exp."""
+    import scipy.linalg
+    from vsg_amd import autodiff
+    rng = np.random.default_rng(12)
+    for scale in (1e-10, 1e-6, 0.3, 2.5):
+        for _ in range(20):
+            xi = rng.normal(size=6) * scale
+            w, u = xi[:3], xi[3:]
+            M = np.zeros((4, 4))
+            M[:3, :3] = [[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]]
+            M[:3, 3] = u
+            E = autodiff.se3_exp(xi)
+            assert np.allclose(E, scipy.linalg.expm(M), atol=1e-12, rtol=1e-12), (scale, xi)
+            assert np.allclose(E, O.se3_exp(xi), atol=1e-12)
