@@ -169,7 +169,7 @@ def test_autograd_function(torch_cuda):
     xi = torch.zeros(6, dtype=torch.float64, requires_grad=True)
     sdf = torch.zeros(wl.blocks * 512, dtype=torch.float32, device="cuda", requires_grad=True)
     depth = autodiff.render_depth(g.render, g.scene, xi, T0, sc.intrinsics, mn, mx, sdf=sdf)
-    assert np.array_equal(depth.cpu().numpy(), _render_depth(g, sc, T0, mn, mx))
+    assert np.array_equal(depth.detach().cpu().numpy(), _render_depth(g, sc, T0, mn, mx))
     valid = depth < 1e9
     w = torch.linspace(-1.0, 1.0, n, device="cuda") * valid
     loss = (w * torch.where(valid, depth, torch.zeros_like(depth))).sum()
