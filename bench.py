@@ -329,7 +329,8 @@ def main() -> int:
                       "ray_blocks_mean": int(np.mean([s["ray_blocks"] for s in st])),
                       "hit_rate": round(float(np.mean([s["hits"] / npix for s in st])), 4)},
         "march_per_ray": {k[6:]: round(float(np.mean([s[k] for s in st])) / npix, 2)
-                          for k in ("march_evals", "march_super_skips", "march_block_skips", "march_replayed")},
+                          for k in ("march_evals", "march_super_skips", "march_block_skips", "march_replayed",
+                                    "march_cell_skips", "march_deferred", "march_tail_windows", "march_tail_skips")},
     }
     if args.compare_fp16 and args.store == "fp32":
         r16 = measure("fp16")

@@ -189,7 +189,7 @@ vsg_status vsg_render_set_sampling(vsg_render* render, int32_t mode);
 
 /* Steps (skips or sample evaluations) a ray marches with one lane before the
  * cooperative tail pass continues it with several lanes; 0 marches every ray with
- * one lane to the end.  Results do not depend on it (default 96).  No counterpart
+ * one lane to the end.  Results do not depend on it (default 48).  No counterpart
  * in the reference: the tail pass is this library's load balancing. */
 vsg_status vsg_render_set_step_budget(vsg_render* render, uint32_t steps);
 
@@ -251,6 +251,16 @@ vsg_status vsg_pose_gn_step(vsg_render* render, vsg_scene* scene, const float* o
 vsg_status vsg_import_voxels(vsg_scene* scene, uint32_t first, uint32_t nblocks, const float* sdf,
                              const uint16_t* weight, const uint8_t* rgbf);
 
+/* The raycaster's positive-cell summary of blocks [first, first+nblocks) to HOST
+ * memory: bit c of masks[b] is set when cell c of the block (4x4x4 voxels, c =
+ * (x >> 2) | (y >> 2) << 1 | (z >> 2) << 2 for voxel (x, y, z) of the block) holds a
+ * voxel with weight > 0 and SDF > 0; counts[8*b + c] is the number of such voxels.
+ * The march skips samples whose corner voxels lie in cells without such a voxel up
+ * to the last one before a cell that has one.  Either output may be NULL.
+ * Synchronising.  No counterpart in the reference. */
+vsg_status vsg_export_cell_masks(vsg_scene* scene, uint32_t first, uint32_t nblocks, uint8_t* masks,
+                                 uint8_t* counts);
+
 /* ---- diagnostics (byte-model counters; no counterpart in the reference) ------------ */
 
 typedef struct vsg_diag {
@@ -258,8 +268,10 @@ typedef struct vsg_diag {
   uint32_t ray_buckets;   /* distinct buckets with live entries the raycast probed    */
   uint32_t any_buckets;   /* union of the two                                         */
   uint32_t ray_blocks;    /* distinct live blocks a raycast trilinear corner touched  */
-  uint32_t march[4];      /* ray march: samples evaluated, super-block skips, block   */
-                          /* skips, samples skipped by replaying the step additions   */
+  uint32_t march[8];      /* ray march: samples evaluated, super-block skips, block   */
+                          /* skips, samples skipped by replaying the step additions,  */
+                          /* positive-cell skips, rays deferred to the tail pass,     */
+                          /* tail-pass sample windows, tail-pass skips                */
 } vsg_diag;
 
 /* When on, integrate and render run instrumented kernel variants that mark the

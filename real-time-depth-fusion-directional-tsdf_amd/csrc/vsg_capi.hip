@@ -118,7 +118,7 @@ struct vsg_scene {
   uint8_t* mark_abkt = nullptr;  // diagnostics marks (B, B, N bytes)
   uint8_t* mark_rbkt = nullptr;
   uint8_t* mark_blk = nullptr;
-  uint32_t* march_stats = nullptr;  // 4 counters
+  uint32_t* march_stats = nullptr;  // kMarchStats counters
   uint32_t* dmeta = nullptr;        // directional extension: slot ids and weights per voxel
   float2* dval = nullptr;           // directional extension: slot dtsdf per voxel
 };
@@ -159,6 +159,8 @@ void free_scene(vsg_scene* s) {
   (void)hipFree(s->t.blk);
   (void)hipFree(s->t.occ);
   (void)hipFree(s->t.idx);
+  (void)hipFree(s->t.pcnt);
+  (void)hipFree(s->t.idxpos);
   (void)hipFree(s->dmeta);
   (void)hipFree(s->dval);
   (void)hipFree(s->vox);
@@ -191,7 +193,7 @@ template <class T> vsg_status dmalloc(T** p, size_t bytes, const char* what) {
 vsg_status queue_reset(vsg_scene* s) {
   {
     TimedLaunch tl(s->stream, VSG_K_RESET);
-    launch_reset_voxels(s->stream, s->st, s->vox, s->word_bytes, s->col, s->dmeta, s->dval);
+    launch_reset_voxels(s->stream, s->st, s->vox, s->word_bytes, s->col, s->dmeta, s->dval, s->t.pcnt);
   }
   vsg_status r = check_launch("reset voxels");
   if (r != VSG_OK) return r;
@@ -275,6 +277,8 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   while (idx_entries < 4 * N) idx_entries <<= 1;
   s->t.idx_mask = (uint32_t)(idx_entries - 1);
   if (r == VSG_OK) r = dmalloc(&s->t.idx, idx_entries * sizeof(uint4), "block index");
+  if (r == VSG_OK) r = dmalloc(&s->t.pcnt, N * 8 * sizeof(uint32_t), "positive-voxel counts per cell");
+  if (r == VSG_OK) r = dmalloc(&s->t.idxpos, N * sizeof(uint32_t), "block index slots");
   if (r == VSG_OK) r = dmalloc(&s->vox, N * kVox * s->word_bytes, "voxel store");
   if (r == VSG_OK) r = dmalloc(&s->col, N * kVox * sizeof(uint32_t), "colour store");
   if (r == VSG_OK) r = dmalloc(&s->vis, N * sizeof(uint32_t), "visible list");
@@ -295,6 +299,8 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   if (e == hipSuccess && s->dval) e = hipMemsetAsync(s->dval, 0, N * kVox * sizeof(float2), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->st, 0, sizeof(DevState), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->t.blk, 0, N * sizeof(int4), s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->t.pcnt, 0, N * 8 * sizeof(uint32_t), s->stream);
+  if (e == hipSuccess) e = hipMemsetAsync(s->t.idxpos, 0, N * sizeof(uint32_t), s->stream);
   if (e == hipSuccess) e = hipMemsetAsync(s->upd_part, 0, kIntegrateGrid * sizeof(uint32_t), s->stream);
   if (e != hipSuccess) {
     free_scene(s);
@@ -841,6 +847,37 @@ vsg_status vsg_import_voxels(vsg_scene* s, uint32_t first, uint32_t nblocks, con
   }
   VSG_HIP(hipMemcpy((uint8_t*)s->vox + off * s->word_bytes, w.data(), w.size(), hipMemcpyHostToDevice));
   if (rgbf) VSG_HIP(hipMemcpy(s->col + off, rgbf, nv * sizeof(uint32_t), hipMemcpyHostToDevice));
+  // the raycaster's positive-cell masks follow the new voxels
+  launch_cell_masks(s->stream, s->p.store == VSG_STORE_FP16, s->t, s->st, first, nblocks, s->vox);
+  vsg_status st = check_launch("cell masks");
+  if (st != VSG_OK) return st;
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  return VSG_OK;
+}
+
+vsg_status vsg_export_cell_masks(vsg_scene* s, uint32_t first, uint32_t nblocks, uint8_t* masks, uint8_t* counts) {
+  if (!s) return fail(VSG_ERR_INVALID_ARG, "vsg_export_cell_masks: null scene");
+  if ((uint64_t)first + nblocks > s->t.num_blocks)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_export_cell_masks: block range beyond the store");
+  if (nblocks == 0) return VSG_OK;
+  if (masks) {
+    uint8_t* d = nullptr;
+    vsg_status st = dmalloc(&d, nblocks, "mask export buffer");
+    if (st != VSG_OK) return st;
+    launch_export_cell_masks(s->stream, s->t, first, nblocks, d);
+    st = check_launch("export cell masks");
+    hipError_t e = hipStreamSynchronize(s->stream);
+    if (st == VSG_OK && e == hipSuccess) e = hipMemcpy(masks, d, nblocks, hipMemcpyDeviceToHost);
+    (void)hipFree(d);
+    if (st != VSG_OK) return st;
+    if (e != hipSuccess) return hip_fail(e, "vsg_export_cell_masks");
+  }
+  if (counts) {
+    VSG_HIP(hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> c((size_t)nblocks * 8);
+    VSG_HIP(hipMemcpy(c.data(), s->t.pcnt + (size_t)first * 8, c.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < c.size(); ++i) counts[i] = (uint8_t)c[i];
+  }
   return VSG_OK;
 }
 
@@ -853,11 +890,11 @@ vsg_status vsg_diag_enable(vsg_scene* s, int32_t on) {
     vsg_status r = dmalloc(&s->mark_abkt, s->t.num_buckets, "diagnostic marks");
     if (r == VSG_OK) r = dmalloc(&s->mark_rbkt, s->t.num_buckets, "diagnostic marks");
     if (r == VSG_OK) r = dmalloc(&s->mark_blk, s->t.num_blocks, "diagnostic marks");
-    if (r == VSG_OK) r = dmalloc(&s->march_stats, 4 * sizeof(uint32_t), "diagnostic counters");
+    if (r == VSG_OK) r = dmalloc(&s->march_stats, kMarchStats * sizeof(uint32_t), "diagnostic counters");
     if (r != VSG_OK) return r;
   }
   if (on) {
-    VSG_HIP(hipMemset(s->march_stats, 0, 4 * sizeof(uint32_t)));
+    VSG_HIP(hipMemset(s->march_stats, 0, kMarchStats * sizeof(uint32_t)));
     VSG_HIP(hipMemset(s->mark_abkt, 0, s->t.num_buckets));
     VSG_HIP(hipMemset(s->mark_rbkt, 0, s->t.num_buckets));
     VSG_HIP(hipMemset(s->mark_blk, 0, s->t.num_blocks));
@@ -881,9 +918,9 @@ vsg_status vsg_diag_read(vsg_scene* s, vsg_diag* out, int32_t clear) {
     out->any_buckets += (a[i] | r[i]) ? 1u : 0u;
   }
   for (size_t i = 0; i < b.size(); ++i) out->ray_blocks += b[i] ? 1u : 0u;
-  VSG_HIP(hipMemcpy(out->march, s->march_stats, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  VSG_HIP(hipMemcpy(out->march, s->march_stats, kMarchStats * sizeof(uint32_t), hipMemcpyDeviceToHost));
   if (clear) {
-    VSG_HIP(hipMemset(s->march_stats, 0, 4 * sizeof(uint32_t)));
+    VSG_HIP(hipMemset(s->march_stats, 0, kMarchStats * sizeof(uint32_t)));
     VSG_HIP(hipMemset(s->mark_abkt, 0, s->t.num_buckets));
     VSG_HIP(hipMemset(s->mark_rbkt, 0, s->t.num_buckets));
     VSG_HIP(hipMemset(s->mark_blk, 0, s->t.num_blocks));

@@ -44,6 +44,8 @@ struct TableView {
   int4* blk;
   uint64_t* occ;
   uint4* idx;
+  uint32_t* pcnt;    // per block and 4^3-voxel cell: voxels with weight > 0 and SDF > 0
+  uint32_t* idxpos;  // per block: its entry slot in the block index
   uint32_t num_buckets;
   uint32_t num_blocks;
   uint32_t occ_mask;
@@ -192,19 +194,50 @@ __device__ __forceinline__ uint32_t idx_home(int32_t x, int32_t y, int32_t z, ui
   return (uint32_t)(mix64(pack_key(x, y, z)) >> 32) & mask;
 }
 
-// Pointer of block (x, y, z) from the block index, or -1.
-__device__ __forceinline__ int32_t index_lookup(const TableView& t, int32_t x, int32_t y, int32_t z) {
+// Block index entry: {key lo, key hi, pointer, positive-cell mask}.  Bit c of the
+// mask is set when cell c of the block (4^3 voxels, c = (x >> 2) | (y >> 2) << 1 |
+// (z >> 2) << 2 in voxel coordinates within the block) holds a voxel with weight > 0
+// and SDF > 0; the integration keeps it exact (per-cell counts in TableView::pcnt).
+
+// Entry slot of block (x, y, z) in the block index, or -1.
+__device__ __forceinline__ int32_t index_slot(const TableView& t, int32_t x, int32_t y, int32_t z) {
   if (!key_in_range(x, y, z)) return -1;
   const uint64_t key = pack_key(x, y, z);
   const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
   uint32_t s = idx_home(x, y, z, t.idx_mask);
   for (uint32_t n = 0; n <= t.idx_mask; ++n) {
     const uint4 e = t.idx[s];
-    if (e.x == klo && e.y == khi) return (int32_t)e.z;
+    if (e.x == klo && e.y == khi) return (int32_t)s;
     if (e.x == kIdxEmpty && e.y == kIdxEmpty) return -1;
     s = (s + 1u) & t.idx_mask;
   }
   return -1;
+}
+
+// Pointer of block (x, y, z) from the block index, or -1; *mask its positive-cell
+// mask (0 when absent).
+__device__ __forceinline__ int32_t index_lookup(const TableView& t, int32_t x, int32_t y, int32_t z,
+                                                uint32_t* mask) {
+  *mask = 0u;
+  if (!key_in_range(x, y, z)) return -1;
+  const uint64_t key = pack_key(x, y, z);
+  const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
+  uint32_t s = idx_home(x, y, z, t.idx_mask);
+  for (uint32_t n = 0; n <= t.idx_mask; ++n) {
+    const uint4 e = t.idx[s];
+    if (e.x == klo && e.y == khi) {
+      *mask = e.w;
+      return (int32_t)e.z;
+    }
+    if (e.x == kIdxEmpty && e.y == kIdxEmpty) return -1;
+    s = (s + 1u) & t.idx_mask;
+  }
+  return -1;
+}
+
+__device__ __forceinline__ int32_t index_lookup(const TableView& t, int32_t x, int32_t y, int32_t z) {
+  uint32_t m;
+  return index_lookup(t, x, y, z, &m);
 }
 
 // ---- ordered float encoding for atomicMin/atomicMax ----------------------------------

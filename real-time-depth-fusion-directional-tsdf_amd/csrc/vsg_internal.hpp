@@ -17,7 +17,8 @@ struct RenderArgs {
   int32_t W, H;
   uint8_t* mark_bkt;  // diagnostics (null when off): buckets with live entries probed
   uint8_t* mark_blk;  // diagnostics (null when off): blocks a trilinear corner touched
-  uint32_t* march_stats;  // diagnostics (null when off): evaluations, super/block skips, replayed steps
+  uint32_t* march_stats;  // diagnostics (null when off): evaluations, super/block skips, replayed steps, cell
+                          // skips, deferred rays, tail windows, tail skips
   int32_t mode;           // vsg_sampling: 0 the reference's trilinear sampling, 1 standard trilinear
   uint32_t budget;        // steps per lane before a ray moves to the cooperative tail pass (0: none)
   void* tail;             // tail queue (TailRay[W*H]) when budget > 0
@@ -32,17 +33,25 @@ struct TailRay {
   uint32_t pad[3];
 };
 
+// Counters of the march statistics (diagnostics).
+constexpr uint32_t kMarchStats = 8;
+
 // Raycast workgroups go to the XCDs in super-tiles of kSuperTile x kSuperTile 8x8 tiles.
 constexpr uint32_t kSuperTile = 4;
 
 // Default march steps per lane before a ray moves to the tail pass.
-constexpr uint32_t kDefaultStepBudget = 96;
+constexpr uint32_t kDefaultStepBudget = 48;
 
 // Workgroups of the tail pass for a W x H render.
 uint32_t tail_workgroups(int32_t W, int32_t H);
 
 void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col,
-                         uint32_t* dmeta, float2* dval);
+                         uint32_t* dmeta, float2* dval, uint32_t* pcnt);
+// Recomputes the positive-cell counts and masks of live blocks [first, first + n).
+void launch_cell_masks(hipStream_t s, bool fp16, const TableView& t, const DevState* st, uint32_t first, uint32_t n,
+                       const void* vox);
+// Positive-cell masks of blocks [first, first + n) into out (device, n bytes).
+void launch_export_cell_masks(hipStream_t s, const TableView& t, uint32_t first, uint32_t n, uint8_t* out);
 void launch_reset_table(hipStream_t s, const TableView& t, DevState* st);
 constexpr uint32_t kIntegrateGrid = 2048;  // workgroups of k_integrate (grid-stride over the list)
 void launch_alloc_claim(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,

@@ -309,6 +309,7 @@ __global__ __launch_bounds__(256) void k_alloc_insert(TableView t, DevState* st,
       unsigned long long* kw = reinterpret_cast<unsigned long long*>(&t.idx[sl]);
       if (atomicCAS(kw, 0xFFFFFFFFFFFFFFFFull, (unsigned long long)key) == 0xFFFFFFFFFFFFFFFFull) {
         t.idx[sl].z = (uint32_t)p;
+        t.idxpos[p] = sl;
         break;
       }
       sl = (sl + 1u) & t.idx_mask;
@@ -474,6 +475,28 @@ __device__ __forceinline__ void dir_update(uint32_t* __restrict__ dmeta, float2*
   dmeta[vi] = meta;
 }
 
+// Adds this wave's positive-voxel changes (delta in {-1, 0, 1} per voxel: +1 when
+// the voxel became positive, weight > 0 and SDF > 0, -1 when it stopped being) to
+// the block's per-cell counts.  A wave holds one z slice of the block: lanes
+// ly * 8 + lx, cells (lx >> 2) | (ly >> 2) << 1 | (lz >> 2) << 2.
+__device__ __forceinline__ void wave_cell_delta(int delta, int lz, uint32_t* counts) {
+  const uint64_t plus = __ballot(delta > 0), minus = __ballot(delta < 0);
+  if ((plus | minus) == 0ull) return;
+  const int lane = threadIdx.x & 63;
+  if (lane < 4) {
+    const uint64_t m = (0x0F0F0F0Full << (4 * (lane & 1))) << (32 * (lane >> 1));
+    const int d = __popcll(plus & m) - __popcll(minus & m);
+    if (d != 0) atomicAdd(&counts[(lane & 1) | ((lane >> 1) << 1) | ((lz >> 2) << 2)], (uint32_t)d);
+  }
+}
+
+// Positive-cell mask of a block from its per-cell counts.
+__device__ __forceinline__ uint32_t cell_mask_of(const uint32_t* counts) {
+  const uint4 a = reinterpret_cast<const uint4*>(counts)[0], b = reinterpret_cast<const uint4*>(counts)[1];
+  return (a.x ? 1u : 0u) | (a.y ? 2u : 0u) | (a.z ? 4u : 0u) | (a.w ? 8u : 0u) | (b.x ? 16u : 0u) |
+         (b.y ? 32u : 0u) | (b.z ? 64u : 0u) | (b.w ? 128u : 0u);
+}
+
 template <bool F16, bool DIR>
 __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, FrameArgs fa,
                                                    const float* __restrict__ pts, const uint8_t* __restrict__ rgb,
@@ -494,56 +517,63 @@ __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, Fr
   for (uint32_t i = blockIdx.x; i < nvis; i += gridDim.x) {
     const uint32_t ptr = vis[i];
     const int4 b = t.blk[ptr];
-    const float vwx = (float)(b.x * kBlock + lx) * v;
-    const float vwy = (float)(b.y * kBlock + ly) * v;
-    const float vwz = (float)(b.z * kBlock + lz) * v;
-    const float rx = vwx - M[3], ry = vwy - M[7], rz = vwz - M[11];
-    const float vcx = M[0] * rx + M[4] * ry + M[8] * rz;
-    const float vcy = M[1] * rx + M[5] * ry + M[9] * rz;
-    const float vcz = M[2] * rx + M[6] * ry + M[10] * rz;
-    const float sxf = vcx * fa.fx / vcz + fa.cx;
-    const float syf = vcy * fa.fy / vcz + fa.cy;
-    const int32_t px = cvt_i32(sxf + 0.5f), py = cvt_i32(syf + 0.5f);
-    if (!(px >= 0 && px < fa.W && py >= 0 && py < fa.H)) continue;
-    const int pix = py * fa.W + px;
-    const float Px = pts[3 * pix], Py = pts[3 * pix + 1], Pz = pts[3 * pix + 2];
-    if (!(Px != 0.0f || Py != 0.0f || Pz != 0.0f)) continue;
-    const float3 D = xform_point(M, Px, Py, Pz);
-    const float sdx = vwx - D.x, sdy = vwy - D.y, sdz = vwz - D.z;
-    float sdf = sqrtf(sdx * sdx + sdy * sdy + sdz * sdz);
-    const float cdx = D.x - camx, cdy = D.y - camy, cdz = D.z - camz;
-    const float cam_dist = sqrtf(cdx * cdx + cdy * cdy + cdz * cdz);
-    const float vox_dist =
-        sqrtf((vwx - camx) * (vwx - camx) + (vwy - camy) * (vwy - camy) + (vwz - camz) * (vwz - camz));
-    if (vox_dist < cam_dist) sdf = -sdf;
-    if (sdf < -tr || sdf > tr) continue;
-    sdf = fmaxf(-tr, fminf(tr, sdf));
-    const size_t vi = (size_t)ptr * kVox + (size_t)idx;
-    const typename S::word w = vox[vi];
-    const uint32_t c = col[vi];
-    const float old_sdf = S::sdf(w);
-    const float old_w = (float)S::weight(w);
-    const float new_w = old_w + 1.0f;
-    if (new_w > 0.0f) {
-      const float nsdf = (old_sdf * old_w + sdf * 1.0f) / new_w;
-      const uint32_t nw = (uint32_t)fminf(255.0f, new_w);
-      const typename S::word nwd = S::make(nsdf, nw);
-      const float stored = S::sdf(nwd);
-      const uint8_t* cp = rgb + 3 * pix;
-      uint32_t nc = 0;
+    int delta = 0;  // +1: the voxel became positive (weight > 0, SDF > 0); -1: it stopped being
+    do {
+      const float vwx = (float)(b.x * kBlock + lx) * v;
+      const float vwy = (float)(b.y * kBlock + ly) * v;
+      const float vwz = (float)(b.z * kBlock + lz) * v;
+      const float rx = vwx - M[3], ry = vwy - M[7], rz = vwz - M[11];
+      const float vcx = M[0] * rx + M[4] * ry + M[8] * rz;
+      const float vcy = M[1] * rx + M[5] * ry + M[9] * rz;
+      const float vcz = M[2] * rx + M[6] * ry + M[10] * rz;
+      const float sxf = vcx * fa.fx / vcz + fa.cx;
+      const float syf = vcy * fa.fy / vcz + fa.cy;
+      const int32_t px = cvt_i32(sxf + 0.5f), py = cvt_i32(syf + 0.5f);
+      if (!(px >= 0 && px < fa.W && py >= 0 && py < fa.H)) break;
+      const int pix = py * fa.W + px;
+      const float Px = pts[3 * pix], Py = pts[3 * pix + 1], Pz = pts[3 * pix + 2];
+      if (!(Px != 0.0f || Py != 0.0f || Pz != 0.0f)) break;
+      const float3 D = xform_point(M, Px, Py, Pz);
+      const float sdx = vwx - D.x, sdy = vwy - D.y, sdz = vwz - D.z;
+      float sdf = sqrtf(sdx * sdx + sdy * sdy + sdz * sdz);
+      const float cdx = D.x - camx, cdy = D.y - camy, cdz = D.z - camz;
+      const float cam_dist = sqrtf(cdx * cdx + cdy * cdy + cdz * cdz);
+      const float vox_dist =
+          sqrtf((vwx - camx) * (vwx - camx) + (vwy - camy) * (vwy - camy) + (vwz - camz) * (vwz - camz));
+      if (vox_dist < cam_dist) sdf = -sdf;
+      if (sdf < -tr || sdf > tr) break;
+      sdf = fmaxf(-tr, fminf(tr, sdf));
+      const size_t vi = (size_t)ptr * kVox + (size_t)idx;
+      const typename S::word w = vox[vi];
+      const uint32_t c = col[vi];
+      const float old_sdf = S::sdf(w);
+      const float old_w = (float)S::weight(w);
+      const float new_w = old_w + 1.0f;
+      if (new_w > 0.0f) {
+        const float nsdf = (old_sdf * old_w + sdf * 1.0f) / new_w;
+        const uint32_t nw = (uint32_t)fminf(255.0f, new_w);
+        const typename S::word nwd = S::make(nsdf, nw);
+        const float stored = S::sdf(nwd);
+        const uint8_t* cp = rgb + 3 * pix;
+        uint32_t nc = 0;
 #pragma unroll
-      for (int ch = 0; ch < 3; ++ch) {
-        const float oc = (float)((c >> (8 * ch)) & 0xFFu);
-        const float ncf = (oc * old_w + (float)cp[ch] * 1.0f) / new_w;
-        nc |= ((uint32_t)ncf & 0xFFu) << (8 * ch);
+        for (int ch = 0; ch < 3; ++ch) {
+          const float oc = (float)((c >> (8 * ch)) & 0xFFu);
+          const float ncf = (oc * old_w + (float)cp[ch] * 1.0f) / new_w;
+          nc |= ((uint32_t)ncf & 0xFFu) << (8 * ch);
+        }
+        uint32_t fl = (c >> 24) | 1u;
+        if ((old_sdf > 0.0f && stored <= 0.0f) || (old_sdf <= 0.0f && stored > 0.0f)) fl |= 2u;
+        vox[vi] = nwd;
+        col[vi] = nc | (fl << 24);
+        if (DIR) dir_update(dmeta, dval, vi, nrm, pix, M, sdf);
+        upd++;
+        const bool was_pos = S::weight(w) != 0u && old_sdf > 0.0f;
+        const bool is_pos = stored > 0.0f;
+        delta = (int)is_pos - (int)was_pos;
       }
-      uint32_t fl = (c >> 24) | 1u;
-      if ((old_sdf > 0.0f && stored <= 0.0f) || (old_sdf <= 0.0f && stored > 0.0f)) fl |= 2u;
-      vox[vi] = nwd;
-      col[vi] = nc | (fl << 24);
-      if (DIR) dir_update(dmeta, dval, vi, nrm, pix, M, sdf);
-      upd++;
-    }
+    } while (false);
+    wave_cell_delta(delta, lz, t.pcnt + (size_t)ptr * 8u);
   }
   // one write per workgroup (summed when the counters are read)
   __shared__ uint32_t ws[8];
@@ -557,34 +587,86 @@ __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, Fr
   }
 }
 
+// Positive-cell masks of the frame's visible blocks (the only blocks whose counts
+// can have changed) into their block index entries, after k_integrate.
+__global__ __launch_bounds__(256) void k_update_masks(TableView t, const DevState* __restrict__ st, int32_t cset,
+                                                      const uint32_t* __restrict__ vis) {
+  const uint32_t nvis = st->set[cset].visible;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < nvis; i += gridDim.x * 256u) {
+    const uint32_t ptr = vis[i];
+    t.idx[t.idxpos[ptr]].w = cell_mask_of(t.pcnt + (size_t)ptr * 8u);
+  }
+}
+
+// Recomputes the per-cell positive-voxel counts and the positive-cell masks of live
+// blocks [first, first + n) from their voxels (after the voxels were overwritten
+// from the host).  One workgroup per block.
+template <bool F16>
+__global__ __launch_bounds__(512) void k_cell_masks(TableView t, const DevState* __restrict__ st, uint32_t first,
+                                                    uint32_t n, const void* __restrict__ vox_) {
+  using S = Store<F16>;
+  const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
+  const int idx = threadIdx.x;
+  const int lz = idx / (kBlock * kBlock);
+  __shared__ __align__(16) uint32_t cnt[8];
+  const uint32_t live = st->n_alloc;  // blocks beyond it have no index entry
+  n = first >= live ? 0u : (n < live - first ? n : live - first);
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const uint32_t ptr = first + i;
+    if (idx < 8) cnt[idx] = 0u;
+    __syncthreads();
+    const typename S::word w = vox[(size_t)ptr * kVox + (size_t)idx];
+    wave_cell_delta(S::weight(w) != 0u && S::sdf(w) > 0.0f ? 1 : 0, lz, cnt);
+    __syncthreads();
+    if (idx < 8) t.pcnt[(size_t)ptr * 8u + (uint32_t)idx] = cnt[idx];
+    if (idx == 0) t.idx[t.idxpos[ptr]].w = cell_mask_of(cnt);
+    __syncthreads();
+  }
+}
+
+// Positive-cell masks of blocks [first, first + n) from their block index entries
+// (0 for a block without an entry).
+__global__ __launch_bounds__(256) void k_export_cell_masks(TableView t, uint32_t first, uint32_t n,
+                                                           uint8_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const int4 b = t.blk[first + i];
+  const int32_t sl = index_slot(t, b.x, b.y, b.z);  // by key: also checks idxpos
+  out[i] = sl >= 0 && t.idx[sl].z == first + i && t.idxpos[first + i] == (uint32_t)sl ? (uint8_t)t.idx[sl].w : 0u;
+}
+
 // ---------------------------------------------------------------------------------
 // Raycast (RayCastRender.cu:127-517)
 // ---------------------------------------------------------------------------------
 
-// Block pointers of the 2x2x2 blocks base + {0,1}^3 around the ray's current
-// sample, looked up on demand (bit i of `known`: block base + (i&1, i>>1&1, i>>2&1)
-// is in p[i]).  When the base moves by one block along an axis the entries that
-// stay in the neighbourhood are kept, so a ray looks up each block it uses about
-// once.  DIAG marks every bucket with live entries that the reference's lookup of a
-// used block probes and every block found (byte-model counters).
+// Block pointers and positive-cell masks of the 2x2x2 blocks base + {0,1}^3 around
+// the ray's current sample, looked up on demand (bit i of `known`: block base + (i&1,
+// i>>1&1, i>>2&1) is in p[i], its mask in byte i of pm).  A request that fits the
+// current window keeps the base; otherwise the base moves, and when it moves by one
+// block along an axis the entries that stay in the window are kept, so a ray looks
+// up each block it uses about once.  DIAG marks every bucket with live entries that
+// the reference's lookup of a used block probes and every block found (byte-model
+// counters).
 template <bool DIAG> struct BlockCache {
   int32_t bx, by, bz;
   int32_t p[8];
+  uint64_t pm;
   uint32_t known;
   uint8_t* mark_bkt;
   uint8_t* mark_blk;
-  uint32_t* stats;  // march statistics (evaluations, super-block skips, block skips, replayed steps)
+  uint32_t* stats;  // march statistics (evaluations, super-block skips, block skips, replayed steps, cell skips)
   __device__ void init(uint8_t* mb, uint8_t* mk, uint32_t* st = nullptr) {
     bx = by = bz = INT32_MIN;
     known = 0u;
+    pm = 0ull;
 #pragma unroll
     for (int i = 0; i < 8; ++i) p[i] = -1;
     mark_bkt = mb;
     mark_blk = mk;
     stats = st;
   }
-  __device__ __forceinline__ int32_t get(const TableView& t, int32_t x, int32_t y, int32_t z) {
-    const int32_t q = index_lookup(t, x, y, z);
+  __device__ __forceinline__ int32_t get(const TableView& t, int32_t x, int32_t y, int32_t z, uint32_t* mask) {
+    const int32_t q = index_lookup(t, x, y, z, mask);
     if (DIAG) {
       if (key_in_range(x, y, z)) {
         const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
@@ -594,32 +676,35 @@ template <bool DIAG> struct BlockCache {
     }
     return q;
   }
-  // Moves the base by d in {-1, 0, 1} along the axis of stride `bit` (1, 2, 4).
-  __device__ __forceinline__ void shift(int32_t d, int bit, uint32_t lo_mask) {
+  // Moves the base by d in {-1, 0, 1} along the axis of stride `bit` (1, 2, 4);
+  // lo_mask: the entries without that bit, bytes: the same entries' bytes of pm.
+  __device__ __forceinline__ void shift(int32_t d, int bit, uint32_t lo_mask, uint64_t bytes) {
     if (d == 1) {
 #pragma unroll
       for (int i = 0; i < 8; ++i)
         if (!(i & bit)) p[i] = p[i | bit];
       known = (known >> bit) & lo_mask;
+      pm = (pm >> (8 * bit)) & bytes;
     } else if (d == -1) {
 #pragma unroll
       for (int i = 7; i >= 0; --i)
         if (i & bit) p[i] = p[i & ~bit];
       known = (known << bit) & ~lo_mask & 0xFFu;
+      pm = (pm << (8 * bit)) & ~bytes;
     }
   }
-  // Pointers of the eight corner blocks of a sample whose corners lie in blocks
-  // {x0, x0 + sx} x {y0, y0 + sy} x {z0, z0 + sz}; corner i takes x from bit 0, y from
-  // bit 1, z from bit 2.
-  __device__ __forceinline__ void corners(const TableView& t, int32_t x0, int32_t y0, int32_t z0, bool sx, bool sy,
-                                          bool sz, int32_t out[8]) {
+  // Makes blocks {x0, x0 + sx} x {y0, y0 + sy} x {z0, z0 + sz} known; *hx (*hy, *hz)
+  // is set when x0 is base + 1 (the window's upper block).
+  __device__ __forceinline__ void ensure(const TableView& t, int32_t x0, int32_t y0, int32_t z0, bool sx, bool sy,
+                                         bool sz, bool* hx, bool* hy, bool* hz) {
     const int32_t dx = (int32_t)((uint32_t)x0 - (uint32_t)bx), dy = (int32_t)((uint32_t)y0 - (uint32_t)by),
                   dz = (int32_t)((uint32_t)z0 - (uint32_t)bz);
-    if ((dx | dy | dz) != 0) {
+    const bool fx = dx == 0 || (dx == 1 && !sx), fy = dy == 0 || (dy == 1 && !sy), fz = dz == 0 || (dz == 1 && !sz);
+    if (!(fx && fy && fz)) {
       if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
-        shift(dx, 1, 0x55u);
-        shift(dy, 2, 0x33u);
-        shift(dz, 4, 0x0Fu);
+        shift(dx, 1, 0x55u, 0x00FF00FF00FF00FFull);
+        shift(dy, 2, 0x33u, 0x0000FFFF0000FFFFull);
+        shift(dz, 4, 0x0Fu, 0x00000000FFFFFFFFull);
       } else {
         known = 0u;
       }
@@ -627,28 +712,66 @@ template <bool DIAG> struct BlockCache {
       by = y0;
       bz = z0;
     }
+    *hx = x0 != bx;
+    *hy = y0 != by;
+    *hz = z0 != bz;
+    const uint32_t ox = *hx ? 1u : 0u, oy = *hy ? 2u : 0u, oz = *hz ? 4u : 0u;
     const uint32_t m = (sx ? 1u : 0u) | (sy ? 2u : 0u) | (sz ? 4u : 0u);
+    const uint32_t o = ox | oy | oz;
     uint32_t need = 0u;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
       if (((uint32_t)i & ~m) == 0u) need |= 1u << i;
+    // the requested blocks are entries o + (subset of m); o and m share no bit
+    need <<= o;
     uint32_t missing = need & ~known;
     while (missing) {
       const int i = __builtin_ctz(missing);
       missing &= missing - 1u;
-      const int32_t q = get(t, x0 + (i & 1), y0 + ((i >> 1) & 1), z0 + ((i >> 2) & 1));
+      uint32_t mk;
+      const int32_t q = get(t, bx + (i & 1), by + ((i >> 1) & 1), bz + ((i >> 2) & 1), &mk);
 #pragma unroll
       for (int k = 0; k < 8; ++k) p[k] = (k == i) ? q : p[k];
+      pm = (pm & ~(0xFFull << (8 * i))) | ((uint64_t)(mk & 0xFFu) << (8 * i));
       known |= 1u << i;
     }
-    // corner i uses block i & m
-    int32_t a[8], b[8];
+  }
+  // Pointers of the eight corner blocks of a sample whose corners lie in blocks
+  // {x0, x0 + sx} x {y0, y0 + sy} x {z0, z0 + sz}; corner i takes x from bit 0, y from
+  // bit 1, z from bit 2.
+  __device__ __forceinline__ void corners(const TableView& t, int32_t x0, int32_t y0, int32_t z0, bool sx, bool sy,
+                                          bool sz, int32_t out[8]) {
+    bool hx, hy, hz;
+    ensure(t, x0, y0, z0, sx, sy, sz, &hx, &hy, &hz);
+    // corner i uses entry (i & m) | o
+    // bit-blends rather than selects of array elements (keeps p[] in registers)
+    const uint32_t kx = sx ? ~0u : 0u, khx = hx ? ~0u : 0u, ky = sy ? ~0u : 0u, khy = hy ? ~0u : 0u;
+    const uint32_t kz = sz ? ~0u : 0u, khz = hz ? ~0u : 0u;
+    uint32_t a[8], b[8];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) a[i] = sx ? p[i] : p[i & ~1];
+    for (int i = 0; i < 8; ++i)
+      a[i] = ((uint32_t)p[i] & kx) | ((uint32_t)p[i | 1] & ~kx & khx) | ((uint32_t)p[i & ~1] & ~kx & ~khx);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) b[i] = sy ? a[i] : a[i & ~2];
+    for (int i = 0; i < 8; ++i) b[i] = (a[i] & ky) | (a[i | 2] & ~ky & khy) | (a[i & ~2] & ~ky & ~khy);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) out[i] = sz ? b[i] : b[i & ~4];
+    for (int i = 0; i < 8; ++i) out[i] = (int32_t)((b[i] & kz) | (b[i | 4] & ~kz & khz) | (b[i & ~4] & ~kz & ~khz));
+  }
+  // True when no cell of cells [cx0, cx1] x [cy0, cy1] x [cz0, cz1] (one or two per
+  // axis, 4^3 voxels each) holds a voxel with weight > 0 and SDF > 0.
+  __device__ __forceinline__ bool cells_free(const TableView& t, int32_t cx0, int32_t cx1, int32_t cy0, int32_t cy1,
+                                             int32_t cz0, int32_t cz1) {
+    bool hx, hy, hz;
+    ensure(t, cx0 >> 1, cy0 >> 1, cz0 >> 1, (cx1 >> 1) != (cx0 >> 1), (cy1 >> 1) != (cy0 >> 1),
+           (cz1 >> 1) != (cz0 >> 1), &hx, &hy, &hz);
+    // cell index along x relative to the window: 2 * block offset + cell in block;
+    // bit of the cell in pm: 8 * block + cell, block = ox + 2 oy + 4 oz, cell = cx + 2 cy + 4 cz
+    const uint32_t ix0 = (uint32_t)(cx0 - 2 * bx), ix1 = (uint32_t)(cx1 - 2 * bx);
+    const uint32_t iy0 = (uint32_t)(cy0 - 2 * by), iy1 = (uint32_t)(cy1 - 2 * by);
+    const uint32_t iz0 = (uint32_t)(cz0 - 2 * bz), iz1 = (uint32_t)(cz1 - 2 * bz);
+    const uint64_t mx = (1ull << (((ix0 >> 1) << 3) | (ix0 & 1u))) | (1ull << (((ix1 >> 1) << 3) | (ix1 & 1u)));
+    const uint64_t mxy = (mx << (((iy0 >> 1) << 4) | ((iy0 & 1u) << 1))) | (mx << (((iy1 >> 1) << 4) | ((iy1 & 1u) << 1)));
+    const uint64_t m = (mxy << (((iz0 >> 1) << 5) | ((iz0 & 1u) << 2))) | (mxy << (((iz1 >> 1) << 5) | ((iz1 & 1u) << 2)));
+    return (pm & m) == 0ull;
   }
 };
 
@@ -964,6 +1087,38 @@ __device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const
   return occ;
 }
 
+// Positive-cell test inside allocated blocks.  The sample's corner voxels lie in the
+// 4^3-voxel cells c0..c1 per axis (Q in [4*c0 + 1, 4*c1 + 3) keeps them there, as for
+// blocks).  A sample can be the upper end of an accepted crossing only if its
+// distance is positive, which needs a corner voxel with weight > 0 and SDF > 0 (the
+// distance is a positively weighted mean of the weighted corners' SDFs).  Returns
+// true with [t0, t1), the parameter interval of the cells' box, when no cell holds
+// such a voxel: no sample in the box can complete a crossing, and only the box's
+// last sample matters for what follows (it is the lower end of the next bracket).
+// *last: key of the last cell range found to hold a positive voxel (consecutive
+// samples mostly share it; a key collision only costs a skipped test).
+template <class Cache>
+__device__ __forceinline__ bool cell_skip(const TableView& t, Cache& bc, const RaySkip& sk, float wdx, float wdy,
+                                          float wdz, float ray, uint32_t* last, float* t0, float* t1) {
+  const float mq = 1.125f + sk.eps;
+  const float qx = (sk.camx + ray * wdx) * sk.inv_v, qy = (sk.camy + ray * wdy) * sk.inv_v,
+              qz = (sk.camz + ray * wdz) * sk.inv_v;
+  const int32_t cx0 = cvt_i32(floorf((qx - mq) * 0.25f)), cx1 = cvt_i32(floorf((qx + mq) * 0.25f));
+  const int32_t cy0 = cvt_i32(floorf((qy - mq) * 0.25f)), cy1 = cvt_i32(floorf((qy + mq) * 0.25f));
+  const int32_t cz0 = cvt_i32(floorf((qz - mq) * 0.25f)), cz1 = cvt_i32(floorf((qz + mq) * 0.25f));
+  const uint32_t key = ((uint32_t)cx0 & 0x1FFu) | (((uint32_t)cy0 & 0x1FFu) << 9) | (((uint32_t)cz0 & 0x1FFu) << 18) |
+                       ((uint32_t)(cx1 - cx0) << 27) | ((uint32_t)(cy1 - cy0) << 28) | ((uint32_t)(cz1 - cz0) << 29) |
+                       0x80000000u;
+  if (key == *last) return false;
+  if (!bc.cells_free(t, cx0, cx1, cy0, cy1, cz0, cz1)) {
+    *last = key;
+    return false;
+  }
+  box_interval(sk, 4.0f * (float)cx0 + 1.0f, 4.0f * (float)cx1 + 3.0f, 4.0f * (float)cy0 + 1.0f,
+               4.0f * (float)cy1 + 3.0f, 4.0f * (float)cz0 + 1.0f, 4.0f * (float)cz1 + 3.0f, t0, t1);
+  return true;
+}
+
 // The march of one ray per lane.  budget > 0: a lane that has taken that many steps
 // (skips or evaluations) stops with h.deferred and its state, to be continued by
 // the cooperative tail pass.
@@ -993,7 +1148,8 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
   // instead of once per lane.  A failed refinement resumes the march exactly where
   // the reference continues (RayCastRender.cu:372-391).
   bool stalled = false;
-  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_steps = 0;
+  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_steps = 0, n_cell = 0;
+  uint32_t cell_key = 0u;
   OccCache oc;
   occ_cache_init(oc);
   bool marching = ray < ray_end, parked = false;
@@ -1023,6 +1179,26 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
           if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
           if (stalled || !(ray < ray_end)) marching = false;
           step = false;
+        } else if (occ == 2 && cell_skip(t, bc, sk, wdx, wdy, wdz, ray, &cell_key, &t0, &t1) && t0 <= ray &&
+                   ray < t1) {
+          n_cell++;
+          if (!(t1 < ray_end)) {
+            // every remaining sample lies in the box: no crossing can be accepted
+            marching = false;
+            step = false;
+          } else {
+            // on to the box's last sample, evaluated below
+            float prev;
+            const float from = ray;
+            (void)replay_steps(ray, t1, inc, &stalled, &prev);
+            if (stalled) {
+              marching = false;
+              step = false;
+            } else {
+              ray = prev;
+              if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
+            }
+          }
         }
       }
       if (step) {
@@ -1102,6 +1278,7 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
     atomicAdd(&bc.stats[1], n_super);
     atomicAdd(&bc.stats[2], n_block);
     atomicAdd(&bc.stats[3], n_replay);
+    atomicAdd(&bc.stats[4], n_cell);
   }
   return h;
 }
@@ -1172,6 +1349,7 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
   const uint64_t dm = __ballot(deferred);
   if (dm != 0ull) {
     uint32_t base = 0;
+    if (DIAG && threadIdx.x == 0 && ra.march_stats) atomicAdd(&ra.march_stats[5], (uint32_t)__popcll(dm));
     if (threadIdx.x == 0) base = atomicAdd(&ra.tail_ctl[0], (uint32_t)__popcll(dm));
     base = __shfl(base, 0, 64);
     if (deferred) {
@@ -1211,7 +1389,7 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
   const TailRay* __restrict__ queue = reinterpret_cast<const TailRay*>(ra.tail);
   const float MINF = 1e10f;
   uint32_t hit = 0, miss = 0;
-  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0;
+  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_window = 0, n_tskip = 0;
   BlockCache<DIAG> bc;
   bc.init(ra.mark_bkt, ra.mark_blk, ra.march_stats);
   OccCache oc;
@@ -1252,6 +1430,7 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
         if (gl == 0) {
           if (occ == 0) n_super++;
           else n_block++;
+          n_tskip++;
         }
         last_w = 0u;
         bool stl;
@@ -1263,6 +1442,7 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
       }
     }
     if (window) {
+      if (gl == 0) n_window++;
       // sample gl of the window: gl more additions of the step
       float r = ray;
       bool in = true;
@@ -1378,11 +1558,14 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
   }
   if (DIAG && ra.march_stats) {
     const uint32_t e = wave_sum(n_eval), su = wave_sum(n_super), bl = wave_sum(n_block), re = wave_sum(n_replay);
+    const uint32_t wi = wave_sum(n_window), ts = wave_sum(n_tskip);
     if (lane == 0) {
       atomicAdd(&ra.march_stats[0], e);
       atomicAdd(&ra.march_stats[1], su);
       atomicAdd(&ra.march_stats[2], bl);
       atomicAdd(&ra.march_stats[3], re);
+      atomicAdd(&ra.march_stats[6], wi);
+      atomicAdd(&ra.march_stats[7], ts);
     }
   }
   hit = wave_sum(hit);
@@ -1437,7 +1620,8 @@ __device__ __forceinline__ void trilinear_grad(const TableView& t, Cache& bc, co
     out->vox[i] = -1;
     out->dfds[i] = 0.0f;
     const int32_t ix = vx[i & 1], iy = vy[(i >> 1) & 1], iz = vz[(i >> 2) & 1];
-    const int32_t p = bc.get(t, voxel_to_block_axis(ix), voxel_to_block_axis(iy), voxel_to_block_axis(iz));
+    uint32_t mk;
+    const int32_t p = bc.get(t, voxel_to_block_axis(ix), voxel_to_block_axis(iy), voxel_to_block_axis(iz), &mk);
     if (p < 0) continue;
     const int64_t vi = (int64_t)p * kVox +
                        (voxel_local_axis(ix) + voxel_local_axis(iy) * kBlock + voxel_local_axis(iz) * kBlock * kBlock);
@@ -1750,10 +1934,11 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* __restrict__ part
 // again after a reset would start from stale values; zeroing them keeps a reset
 // scene equal to a new one.  Runs before k_reset_table, which clears n_alloc.
 __global__ __launch_bounds__(256) void k_reset_voxels(const DevState* st, void* vox, uint32_t word_bytes,
-                                                      uint32_t* col, uint32_t* dmeta, float2* dval) {
+                                                      uint32_t* col, uint32_t* dmeta, float2* dval, uint32_t* pcnt) {
   const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * 256;
   const size_t nvox = (size_t)st->n_alloc * kVox;
+  for (size_t i = tid; i < (size_t)st->n_alloc * 8u; i += stride) pcnt[i] = 0;
   for (size_t i = tid; i < nvox; i += stride) {
     if (word_bytes == 8) reinterpret_cast<uint2*>(vox)[i] = make_uint2(0u, 0u);
     else reinterpret_cast<uint32_t*>(vox)[i] = 0u;
@@ -1795,8 +1980,21 @@ __global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) 
 // ---------------------------------------------------------------------------------
 
 void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col,
-                         uint32_t* dmeta, float2* dval) {
-  hipLaunchKernelGGL(k_reset_voxels, dim3(2048), dim3(256), 0, s, st, vox, word_bytes, col, dmeta, dval);
+                         uint32_t* dmeta, float2* dval, uint32_t* pcnt) {
+  hipLaunchKernelGGL(k_reset_voxels, dim3(2048), dim3(256), 0, s, st, vox, word_bytes, col, dmeta, dval, pcnt);
+}
+
+void launch_cell_masks(hipStream_t s, bool fp16, const TableView& t, const DevState* st, uint32_t first, uint32_t n,
+                       const void* vox) {
+  if (n == 0u) return;
+  const uint32_t grid = n < 2048u ? n : 2048u;
+  if (fp16) hipLaunchKernelGGL(k_cell_masks<true>, dim3(grid), dim3(512), 0, s, t, st, first, n, vox);
+  else hipLaunchKernelGGL(k_cell_masks<false>, dim3(grid), dim3(512), 0, s, t, st, first, n, vox);
+}
+
+void launch_export_cell_masks(hipStream_t s, const TableView& t, uint32_t first, uint32_t n, uint8_t* out) {
+  if (n == 0u) return;
+  hipLaunchKernelGGL(k_export_cell_masks, dim3((n + 255u) / 256u), dim3(256), 0, s, t, first, n, out);
 }
 
 void launch_reset_table(hipStream_t s, const TableView& t, DevState* st) {
@@ -1844,6 +2042,7 @@ void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st
     if (dir) VSG_INTEGRATE(false, true); else VSG_INTEGRATE(false, false);
   }
 #undef VSG_INTEGRATE
+  hipLaunchKernelGGL(k_update_masks, dim3(256), dim3(256), 0, s, t, st, fa.cset, vis);
 }
 
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,

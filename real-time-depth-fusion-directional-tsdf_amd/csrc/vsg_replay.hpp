@@ -35,10 +35,12 @@ __host__ __device__ inline uint32_t floor_div24(uint32_t num, uint32_t den) {
 }
 
 // First element >= target of r, fl(r+inc), fl(fl(r+inc)+inc), ... (r itself when
-// r >= target).  *stalled is set when an addition leaves the value unchanged (the
-// reference's loop would never end there).
-__host__ __device__ inline float replay_steps(float r, float target, float inc, bool* stalled) {
+// r >= target); *prev receives the element before it (r when none precedes it).
+// *stalled is set when an addition leaves the value unchanged (the reference's loop
+// would never end there).
+__host__ __device__ inline float replay_steps(float r, float target, float inc, bool* stalled, float* prev) {
   *stalled = false;
+  *prev = r;
   for (int guard = 0; guard < 4096 && r < target; ++guard) {
     if (r >= 1.17549435e-38f && inc > 0.0f) {  // r a positive normal number
       int e;
@@ -53,11 +55,18 @@ __host__ __device__ inline float replay_steps(float r, float target, float inc, 
         if (S > 0u && R + S < 16777216u) {
           const float G = ldexpf(target, sc);        // > R; may exceed the binade
           if (G <= 16777215.0f) {
-            const uint32_t k = ceil_div24((uint32_t)ceilf(G) - R, S);
-            if (R + k * S < 16777216u) return ldexpf((float)(R + k * S), -sc);
+            const uint32_t k = ceil_div24((uint32_t)ceilf(G) - R, S);  // >= 1
+            if (R + k * S < 16777216u) {
+              *prev = ldexpf((float)(R + (k - 1u) * S), -sc);
+              return ldexpf((float)(R + k * S), -sc);
+            }
           }
           // the target lies beyond the binade: its last element, then one real addition
-          r = ldexpf((float)(R + floor_div24(16777215u - R, S) * S), -sc);
+          const uint32_t m = floor_div24(16777215u - R, S);
+          if (m > 0u) {
+            *prev = ldexpf((float)(R + (m - 1u) * S), -sc);
+            r = ldexpf((float)(R + m * S), -sc);
+          }
           if (r >= target) return r;
         }
       }
@@ -67,9 +76,15 @@ __host__ __device__ inline float replay_steps(float r, float target, float inc, 
       *stalled = true;
       return r;
     }
+    *prev = r;
     r = nx;
   }
   return r;
+}
+
+__host__ __device__ inline float replay_steps(float r, float target, float inc, bool* stalled) {
+  float prev;
+  return replay_steps(r, target, inc, stalled, &prev);
 }
 
 }  // namespace vsg

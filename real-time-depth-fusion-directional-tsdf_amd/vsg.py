@@ -66,7 +66,7 @@ class vsg_counters(C.Structure):
 
 class vsg_diag(C.Structure):
     _fields_ = [("alloc_buckets", C.c_uint32), ("ray_buckets", C.c_uint32), ("any_buckets", C.c_uint32),
-                ("ray_blocks", C.c_uint32), ("march", C.c_uint32 * 4)]
+                ("ray_blocks", C.c_uint32), ("march", C.c_uint32 * 8)]
 
 
 # Every symbol include/vsg.h declares, with its ctypes signature.
@@ -101,6 +101,7 @@ _SIGNATURES = {
     "vsg_pose_gn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
                                    C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "vsg_import_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vsg_export_cell_masks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "vsg_diag_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsg_diag_read": (C.c_int, [C.c_void_p, C.POINTER(vsg_diag), C.c_int32]),
     "vsg_timing_enable": (C.c_int, [C.c_int32]),
@@ -324,8 +325,21 @@ class VoxelScene:
         d = vsg_diag()
         _check(lib().vsg_diag_read(self._h, C.byref(d), 1 if clear else 0))
         out = {name: getattr(d, name) for name, _ in vsg_diag._fields_ if name != "march"}
-        out.update(zip(("march_evals", "march_super_skips", "march_block_skips", "march_replayed"), d.march))
+        out.update(zip(("march_evals", "march_super_skips", "march_block_skips", "march_replayed",
+                        "march_cell_skips", "march_deferred", "march_tail_windows", "march_tail_skips"), d.march))
         return out
+
+    def export_cell_masks(self, first: int = 0, nblocks: int | None = None):
+        """(masks uint8 [n], counts uint8 [n, 8]) of blocks [first, first+n): bit c of a
+        block's mask is set when its 4^3-voxel cell c holds a voxel with weight > 0 and
+        SDF > 0; counts are those voxels per cell."""
+        if nblocks is None:
+            nblocks = self.m_params.SDFBlockNum - self.getHeapFreeCount() - first
+        masks = np.zeros(nblocks, np.uint8)
+        counts = np.zeros((nblocks, 8), np.uint8)
+        if nblocks:
+            _check(lib().vsg_export_cell_masks(self._h, first, nblocks, masks.ctypes.data, counts.ctypes.data))
+        return masks, counts
 
     def export_blocks(self):
         """(coords int32 [n,3], hash entry indices int32 [n]) in pointer order."""

@@ -1,7 +1,7 @@
 """The march's closed-form replay of step additions (csrc/vsg_replay.hpp) lands on the
-same float as the literal loop `while (r < target) r += inc` for random and edge-case
-inputs, including binade crossings, ties (inc a half-integer number of ulps) and
-stalls.  Host build of the same header the kernels use."""
+same float as the literal loop `while (r < target) r += inc`, and reports the same
+element before it, for random and edge-case inputs, including binade crossings, ties
+(inc a half-integer number of ulps) and stalls.  Host build of the same header the kernels use."""
 import os
 import subprocess
 
@@ -17,11 +17,13 @@ DRIVER = r"""
 #include <random>
 #include "vsg_replay.hpp"
 
-static float loop_replay(float r, float target, float inc, bool* stalled) {
+static float loop_replay(float r, float target, float inc, bool* stalled, float* prev) {
   *stalled = false;
+  *prev = r;
   while (r < target) {
     float nx = r + inc;
     if (nx == r) { *stalled = true; return r; }
+    *prev = r;
     r = nx;
   }
   return r;
@@ -29,13 +31,17 @@ static float loop_replay(float r, float target, float inc, bool* stalled) {
 
 static int check(float r, float t, float inc) {
   bool s1, s2;
-  float a = vsg::replay_steps(r, t, inc, &s1);
-  float b = loop_replay(r, t, inc, &s2);
-  uint32_t ua, ub;
+  float pa, pb;
+  float a = vsg::replay_steps(r, t, inc, &s1, &pa);
+  float b = loop_replay(r, t, inc, &s2, &pb);
+  uint32_t ua, ub, upa, upb;
   std::memcpy(&ua, &a, 4);
   std::memcpy(&ub, &b, 4);
-  if (ua != ub || s1 != s2) {
-    std::printf("MISMATCH r=%.9g t=%.9g inc=%.9g got %.9g (%d) want %.9g (%d)\n", r, t, inc, a, s1, b, s2);
+  std::memcpy(&upa, &pa, 4);
+  std::memcpy(&upb, &pb, 4);
+  if (ua != ub || s1 != s2 || upa != upb) {
+    std::printf("MISMATCH r=%.9g t=%.9g inc=%.9g got %.9g/%.9g (%d) want %.9g/%.9g (%d)\n", r, t, inc, a, pa, s1,
+                b, pb, s2);
     return 1;
   }
   return 0;
