@@ -694,7 +694,8 @@ template <bool DIAG> struct BlockCache {
     }
   }
   // Makes blocks {x0, x0 + sx} x {y0, y0 + sy} x {z0, z0 + sz} known; *hx (*hy, *hz)
-  // is set when x0 is base + 1 (the window's upper block).
+  // is set when x0 is base + 1 (the window's upper block).  A request that fits the
+  // current window leaves the base; otherwise the base moves to (x0, y0, z0).
   __device__ __forceinline__ void ensure(const TableView& t, int32_t x0, int32_t y0, int32_t z0, bool sx, bool sy,
                                          bool sz, bool* hx, bool* hy, bool* hz) {
     const int32_t dx = (int32_t)((uint32_t)x0 - (uint32_t)bx), dy = (int32_t)((uint32_t)y0 - (uint32_t)by),
@@ -743,8 +744,8 @@ template <bool DIAG> struct BlockCache {
                                           bool sz, int32_t out[8]) {
     bool hx, hy, hz;
     ensure(t, x0, y0, z0, sx, sy, sz, &hx, &hy, &hz);
-    // corner i uses entry (i & m) | o
-    // bit-blends rather than selects of array elements (keeps p[] in registers)
+    // corner i uses entry (i & m) | o, o the window offset (hx, hy, hz); bit-blends
+    // rather than selects of array elements keep p[] in registers
     const uint32_t kx = sx ? ~0u : 0u, khx = hx ? ~0u : 0u, ky = sy ? ~0u : 0u, khy = hy ? ~0u : 0u;
     const uint32_t kz = sz ? ~0u : 0u, khz = hz ? ~0u : 0u;
     uint32_t a[8], b[8];
@@ -913,7 +914,8 @@ __device__ __forceinline__ void box_interval(const RaySkip& k, float lox, float 
       if (!(l <= 0.0f && 0.0f <= h)) { tf = -INFINITY; }
       continue;
     }
-    const float id = 1.0f / d[a];
+    // hardware reciprocal (1 ulp): its error is far inside the relative margin below
+    const float id = __builtin_amdgcn_rcpf(d[a]);
     const float ta = l * id, tb = h * id;
     tn = fmaxf(tn, fminf(ta, tb));
     tf = fminf(tf, fmaxf(ta, tb));
@@ -952,23 +954,30 @@ __device__ __forceinline__ uint64_t occ_word(const TableView& t, OccCache& oc, i
 // but none of the range is; 2: a block of the range is allocated.
 __device__ __forceinline__ int occ_test(const TableView& t, OccCache& oc, int32_t bx0, int32_t bx1, int32_t by0,
                                         int32_t by1, int32_t bz0, int32_t bz1) {
+  // one word per super-block the range touches (one or two per axis); in each, the
+  // bits x + 4y + 16z of the range's blocks (local coordinates in the super-block)
   uint64_t any_word = 0ull, any_bit = 0ull;
-  if ((bx0 >> 2) == (bx1 >> 2) && (by0 >> 2) == (by1 >> 2) && (bz0 >> 2) == (bz1 >> 2)) {
-    const uint64_t w = occ_word(t, oc, bx0 >> 2, by0 >> 2, bz0 >> 2);
-    // bits x + 4y + 16z of the range's blocks (local coordinates in the super-block)
-    const uint32_t mx = (1u << (bx0 & 3)) | (1u << (bx1 & 3));
-    const uint32_t mxy = (mx << ((by0 & 3) << 2)) | (mx << ((by1 & 3) << 2));
-    const uint64_t m = ((uint64_t)mxy << ((bz0 & 3) << 4)) | ((uint64_t)mxy << ((bz1 & 3) << 4));
-    any_word = w;
-    any_bit = w & m;
-  } else {
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const int32_t x = (i & 1) ? bx1 : bx0, y = (i & 2) ? by1 : by0, z = (i & 4) ? bz1 : bz0;
-      const uint64_t w = occ_word(t, oc, x >> 2, y >> 2, z >> 2);
-      any_word |= w;
-      any_bit |= (w >> occ_bit(x, y, z)) & 1ull;
-    }
+  const int32_t sx0 = bx0 >> 2, sy0 = by0 >> 2, sz0 = bz0 >> 2;
+  const uint32_t ex = (bx1 >> 2) != sx0 ? 1u : 0u, ey = (by1 >> 2) != sy0 ? 1u : 0u, ez = (bz1 >> 2) != sz0 ? 1u : 0u;
+  const uint32_t nsb = 1u << (ex + ey + ez);
+  for (uint32_t k = 0; k < nsb; ++k) {
+    // k enumerates the split axes' choices (x first)
+    uint32_t b = k;
+    const uint32_t cx = ex ? (b & 1u) : 0u;
+    b >>= ex;
+    const uint32_t cy = ey ? (b & 1u) : 0u;
+    b >>= ey;
+    const uint32_t cz = ez ? (b & 1u) : 0u;
+    // blocks of the range in this super-block: both, or the one on its side of the split
+    const int32_t xl = ex ? (cx ? bx1 : bx0) : bx0, xh = ex ? xl : bx1;
+    const int32_t yl = ey ? (cy ? by1 : by0) : by0, yh = ey ? yl : by1;
+    const int32_t zl = ez ? (cz ? bz1 : bz0) : bz0, zh = ez ? zl : bz1;
+    const uint32_t mx = (1u << (xl & 3)) | (1u << (xh & 3));
+    const uint32_t mxy = (mx << ((yl & 3) << 2)) | (mx << ((yh & 3) << 2));
+    const uint64_t m = ((uint64_t)mxy << ((zl & 3) << 4)) | ((uint64_t)mxy << ((zh & 3) << 4));
+    const uint64_t w = occ_word(t, oc, sx0 + (int32_t)cx, sy0 + (int32_t)cy, sz0 + (int32_t)cz);
+    any_word |= w;
+    any_bit |= w & m;
   }
   return any_bit ? 2 : (any_word ? 1 : 0);
 }
@@ -1371,9 +1380,11 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // otherwise its lanes evaluate kTailGroup consecutive samples at once (the positions
 // the reference's additions reach, computed by the same additions) and the group
 // resolves them in order: the first sign change that passes the thresholds is
-// refined, and the march continues after the window's last sample.  Groups take
-// rays from the queue until it is empty; the last workgroup clears the queue
-// controls for the next render.
+// refined, and the march continues after the window's last sample.  Group g starts
+// with queue entry g; afterwards the groups of a wave that need a ray take them
+// with one atomic per wave (a single queue counter would otherwise serialise the
+// fetches), until the queue is empty.  The last workgroup clears the queue controls
+// for the next render.
 constexpr int kTailGroup = 16;
 
 template <bool F16, bool DIAG, bool REF>
@@ -1387,6 +1398,9 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
   const int gbase = lane & ~(G - 1);
   const uint32_t n = min(ra.tail_ctl[0], (uint32_t)ra.W * (uint32_t)ra.H);
   const TailRay* __restrict__ queue = reinterpret_cast<const TailRay*>(ra.tail);
+  constexpr uint32_t kGroups = 64u / (uint32_t)G;  // groups per wave
+  const uint32_t n_static = gridDim.x * kGroups;     // entries handed out without atomics
+  uint32_t next_static = blockIdx.x * kGroups + (uint32_t)(gbase / G);
   const float MINF = 1e10f;
   uint32_t hit = 0, miss = 0;
   uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_window = 0, n_tskip = 0;
@@ -1401,10 +1415,25 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
   uint32_t last_w = 0;
   bool have = false, exhausted = false;
   while (true) {
-    if (!have && !exhausted) {
-      uint32_t e = 0;
-      if (gl == 0) e = atomicAdd(&ra.tail_ctl[1], 1u);
-      e = __shfl(e, 0, G);
+    // queue entries for the groups that need one: the first statically, then one
+    // reservation per wave
+    const bool want = !have && !exhausted;
+    uint32_t e = 0xFFFFFFFFu;
+    if (next_static != 0xFFFFFFFFu) {
+      if (want) {
+        e = next_static;
+        next_static = 0xFFFFFFFFu;
+      }
+    } else {
+      const uint64_t need = __ballot(want && gl == 0);
+      if (need != 0ull) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&ra.tail_ctl[1], (uint32_t)__popcll(need));
+        base = __shfl(base, 0, 64);
+        if (want) e = n_static + base + (uint32_t)__popcll(need & ((1ull << gbase) - 1ull));
+      }
+    }
+    if (want) {
       if (e < n) {
         const TailRay q = queue[e];
         pix = q.pix;
