@@ -168,21 +168,24 @@ __device__ __forceinline__ int32_t table_lookup(const TableView& t, int32_t x, i
   return reinterpret_cast<const int32_t*>(r + kBucket)[hit];
 }
 
-// 64-bit finaliser (MurmurHash3 fmix64): every output bit depends on every input bit,
-// so spatially clustered keys spread over the table.
-__device__ __forceinline__ uint64_t mix64(uint64_t k) {
-  k ^= k >> 33;
-  k *= 0xFF51AFD7ED558CCDull;
-  k ^= k >> 33;
-  k *= 0xC4CEB9FE1A85EC53ull;
-  k ^= k >> 33;
-  return k;
+// 32-bit hash of a block or super-block coordinate: odd multipliers per axis, then
+// the MurmurHash3 32-bit finaliser (every output bit depends on every input bit), so
+// spatially clustered keys spread over the table.  32-bit arithmetic keeps it cheap
+// in the ray march, which hashes on every block lookup.
+__device__ __forceinline__ uint32_t hash3(int32_t x, int32_t y, int32_t z) {
+  uint32_t h = (uint32_t)x * 0x9E3779B1u + (uint32_t)y * 0x85EBCA77u + (uint32_t)z * 0xC2B2AE3Du;
+  h ^= h >> 16;
+  h *= 0x85EBCA6Bu;
+  h ^= h >> 13;
+  h *= 0xC2B2AE35u;
+  h ^= h >> 16;
+  return h;
 }
 
 // Occupancy map word of super-block (sx, sy, sz) and bit of block (x, y, z) in the
 // word of its super-block (x >> 2, y >> 2, z >> 2).
 __device__ __forceinline__ uint32_t occ_index(int32_t sx, int32_t sy, int32_t sz, uint32_t mask) {
-  return (uint32_t)(mix64(pack_key(sx, sy, sz)) >> 32) & mask;
+  return hash3(sx, sy, sz) & mask;
 }
 
 __device__ __forceinline__ uint32_t occ_bit(int32_t x, int32_t y, int32_t z) {
@@ -191,7 +194,7 @@ __device__ __forceinline__ uint32_t occ_bit(int32_t x, int32_t y, int32_t z) {
 
 // Home slot of block (x, y, z) in the block index.
 __device__ __forceinline__ uint32_t idx_home(int32_t x, int32_t y, int32_t z, uint32_t mask) {
-  return (uint32_t)(mix64(pack_key(x, y, z)) >> 32) & mask;
+  return hash3(x, y, z) & mask;
 }
 
 // Block index entry: {key lo, key hi, pointer, positive-cell mask}.  Bit c of the
