@@ -639,28 +639,29 @@ __global__ __launch_bounds__(256) void k_export_cell_masks(TableView t, uint32_t
 // Raycast (RayCastRender.cu:127-517)
 // ---------------------------------------------------------------------------------
 
-// Block pointers and positive-cell masks of the 2x2x2 blocks base + {0,1}^3 around
-// the ray's current sample, looked up on demand (bit i of `known`: block base + (i&1,
-// i>>1&1, i>>2&1) is in p[i], its mask in byte i of pm).  A request that fits the
-// current window keeps the base; otherwise the base moves, and when it moves by one
-// block along an axis the entries that stay in the window are kept, so a ray looks
-// up each block it uses about once.  DIAG marks every bucket with live entries that
-// the reference's lookup of a used block probes and every block found (byte-model
-// counters).
+// Block pointers and positive-cell masks of the 2x2x2 blocks around the ray's
+// current sample, looked up on demand.  The window holds blocks [bx, bx + 1] x
+// [by, by + 1] x [bz, bz + 1]; block (x, y, z) sits in slot (x & 1) | (y & 1) << 1 |
+// (z & 1) << 2, so when the window moves by one block along an axis the blocks that
+// stay keep their slots and only the slots of the blocks that leave are forgotten.
+// The slots' pointers live in LDS (slot s of lane l at slots[64 s + l]: lanes read
+// different slots without a select network); bit s of `known`: slot s holds its
+// window block; byte s of pm: its positive-cell mask.  DIAG marks every bucket with
+// live entries that the reference's lookup of a used block probes and every block
+// found (byte-model counters).
 template <bool DIAG> struct BlockCache {
   int32_t bx, by, bz;
-  int32_t p[8];
+  int32_t* slots;  // this lane's slot 0 in LDS (slot s at slots[64 s])
   uint64_t pm;
   uint32_t known;
   uint8_t* mark_bkt;
   uint8_t* mark_blk;
   uint32_t* stats;  // march statistics (evaluations, super-block skips, block skips, replayed steps, cell skips)
-  __device__ void init(uint8_t* mb, uint8_t* mk, uint32_t* st = nullptr) {
+  __device__ void init(int32_t* lane_slots, uint8_t* mb, uint8_t* mk, uint32_t* st = nullptr) {
     bx = by = bz = INT32_MIN;
     known = 0u;
     pm = 0ull;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) p[i] = -1;
+    slots = lane_slots;
     mark_bkt = mb;
     mark_blk = mk;
     stats = st;
@@ -676,63 +677,39 @@ template <bool DIAG> struct BlockCache {
     }
     return q;
   }
-  // Moves the base by d in {-1, 0, 1} along the axis of stride `bit` (1, 2, 4);
-  // lo_mask: the entries without that bit, bytes: the same entries' bytes of pm.
-  __device__ __forceinline__ void shift(int32_t d, int bit, uint32_t lo_mask, uint64_t bytes) {
-    if (d == 1) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i)
-        if (!(i & bit)) p[i] = p[i | bit];
-      known = (known >> bit) & lo_mask;
-      pm = (pm >> (8 * bit)) & bytes;
-    } else if (d == -1) {
-#pragma unroll
-      for (int i = 7; i >= 0; --i)
-        if (i & bit) p[i] = p[i & ~bit];
-      known = (known << bit) & ~lo_mask & 0xFFu;
-      pm = (pm << (8 * bit)) & ~bytes;
+  // The window along one axis after a request for blocks [c0, c0 + s]: unchanged
+  // when they fit, else moved (by one block: the block that leaves frees its slots;
+  // farther: nothing is kept).  lo: the slots whose bit of this axis is clear.
+  __device__ __forceinline__ int32_t move_axis(int32_t base, int32_t c0, bool s, uint32_t lo) {
+    const int32_t d = (int32_t)((uint32_t)c0 - (uint32_t)base);
+    if (d == 0 || (d == 1 && !s)) return base;
+    if (d == 1 || d == -1) {
+      const int32_t gone = d == 1 ? base : base + 1;
+      known &= (gone & 1) ? lo : (~lo & 0xFFu);
+      return d == 1 ? base + 1 : base - 1;
     }
+    known = 0u;
+    return c0;
   }
-  // Makes blocks {x0, x0 + sx} x {y0, y0 + sy} x {z0, z0 + sz} known; *hx (*hy, *hz)
-  // is set when x0 is base + 1 (the window's upper block).  A request that fits the
-  // current window leaves the base; otherwise the base moves to (x0, y0, z0).
+  // Makes blocks {x0, x0 + sx} x {y0, y0 + sy} x {z0, z0 + sz} known.
   __device__ __forceinline__ void ensure(const TableView& t, int32_t x0, int32_t y0, int32_t z0, bool sx, bool sy,
-                                         bool sz, bool* hx, bool* hy, bool* hz) {
-    const int32_t dx = (int32_t)((uint32_t)x0 - (uint32_t)bx), dy = (int32_t)((uint32_t)y0 - (uint32_t)by),
-                  dz = (int32_t)((uint32_t)z0 - (uint32_t)bz);
-    const bool fx = dx == 0 || (dx == 1 && !sx), fy = dy == 0 || (dy == 1 && !sy), fz = dz == 0 || (dz == 1 && !sz);
-    if (!(fx && fy && fz)) {
-      if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1) {
-        shift(dx, 1, 0x55u, 0x00FF00FF00FF00FFull);
-        shift(dy, 2, 0x33u, 0x0000FFFF0000FFFFull);
-        shift(dz, 4, 0x0Fu, 0x00000000FFFFFFFFull);
-      } else {
-        known = 0u;
-      }
-      bx = x0;
-      by = y0;
-      bz = z0;
-    }
-    *hx = x0 != bx;
-    *hy = y0 != by;
-    *hz = z0 != bz;
-    const uint32_t ox = *hx ? 1u : 0u, oy = *hy ? 2u : 0u, oz = *hz ? 4u : 0u;
-    const uint32_t m = (sx ? 1u : 0u) | (sy ? 2u : 0u) | (sz ? 4u : 0u);
-    const uint32_t o = ox | oy | oz;
-    uint32_t need = 0u;
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-      if (((uint32_t)i & ~m) == 0u) need |= 1u << i;
-    // the requested blocks are entries o + (subset of m); o and m share no bit
-    need <<= o;
-    uint32_t missing = need & ~known;
+                                         bool sz) {
+    bx = move_axis(bx, x0, sx, 0x55u);
+    by = move_axis(by, y0, sy, 0x33u);
+    bz = move_axis(bz, z0, sz, 0x0Fu);
+    const uint32_t ex = sx ? 0xFFu : ((x0 & 1) ? 0xAAu : 0x55u);
+    const uint32_t ey = sy ? 0xFFu : ((y0 & 1) ? 0xCCu : 0x33u);
+    const uint32_t ez = sz ? 0xFFu : ((z0 & 1) ? 0xF0u : 0x0Fu);
+    uint32_t missing = ex & ey & ez & ~known;
     while (missing) {
       const int i = __builtin_ctz(missing);
       missing &= missing - 1u;
+      // the window block with slot i's parities
+      const int32_t x = bx + (int32_t)(((uint32_t)i ^ (uint32_t)bx) & 1u);
+      const int32_t y = by + (int32_t)((((uint32_t)i >> 1) ^ (uint32_t)by) & 1u);
+      const int32_t z = bz + (int32_t)((((uint32_t)i >> 2) ^ (uint32_t)bz) & 1u);
       uint32_t mk;
-      const int32_t q = get(t, bx + (i & 1), by + ((i >> 1) & 1), bz + ((i >> 2) & 1), &mk);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) p[k] = (k == i) ? q : p[k];
+      slots[64 * i] = get(t, x, y, z, &mk);
       pm = (pm & ~(0xFFull << (8 * i))) | ((uint64_t)(mk & 0xFFu) << (8 * i));
       known |= 1u << i;
     }
@@ -742,36 +719,29 @@ template <bool DIAG> struct BlockCache {
   // bit 1, z from bit 2.
   __device__ __forceinline__ void corners(const TableView& t, int32_t x0, int32_t y0, int32_t z0, bool sx, bool sy,
                                           bool sz, int32_t out[8]) {
-    bool hx, hy, hz;
-    ensure(t, x0, y0, z0, sx, sy, sz, &hx, &hy, &hz);
-    // corner i uses entry (i & m) | o, o the window offset (hx, hy, hz); bit-blends
-    // rather than selects of array elements keep p[] in registers
-    const uint32_t kx = sx ? ~0u : 0u, khx = hx ? ~0u : 0u, ky = sy ? ~0u : 0u, khy = hy ? ~0u : 0u;
-    const uint32_t kz = sz ? ~0u : 0u, khz = hz ? ~0u : 0u;
-    uint32_t a[8], b[8];
+    ensure(t, x0, y0, z0, sx, sy, sz);
+    const uint32_t px = (uint32_t)x0 & 1u, py = ((uint32_t)y0 & 1u) << 1, pz = ((uint32_t)z0 & 1u) << 2;
+    const uint32_t qx = sx ? 1u : 0u, qy = sy ? 2u : 0u, qz = sz ? 4u : 0u;
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
-      a[i] = ((uint32_t)p[i] & kx) | ((uint32_t)p[i | 1] & ~kx & khx) | ((uint32_t)p[i & ~1] & ~kx & ~khx);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) b[i] = (a[i] & ky) | (a[i | 2] & ~ky & khy) | (a[i & ~2] & ~ky & ~khy);
-#pragma unroll
-    for (int i = 0; i < 8; ++i) out[i] = (int32_t)((b[i] & kz) | (b[i | 4] & ~kz & khz) | (b[i & ~4] & ~kz & ~khz));
+    for (int i = 0; i < 8; ++i) out[i] = slots[64 * ((px ^ (i & qx)) | (py ^ (i & qy)) | (pz ^ (i & qz)))];
   }
   // True when no cell of cells [cx0, cx1] x [cy0, cy1] x [cz0, cz1] (one or two per
   // axis, 4^3 voxels each) holds a voxel with weight > 0 and SDF > 0.
   __device__ __forceinline__ bool cells_free(const TableView& t, int32_t cx0, int32_t cx1, int32_t cy0, int32_t cy1,
                                              int32_t cz0, int32_t cz1) {
-    bool hx, hy, hz;
     ensure(t, cx0 >> 1, cy0 >> 1, cz0 >> 1, (cx1 >> 1) != (cx0 >> 1), (cy1 >> 1) != (cy0 >> 1),
-           (cz1 >> 1) != (cz0 >> 1), &hx, &hy, &hz);
-    // cell index along x relative to the window: 2 * block offset + cell in block;
-    // bit of the cell in pm: 8 * block + cell, block = ox + 2 oy + 4 oz, cell = cx + 2 cy + 4 cz
-    const uint32_t ix0 = (uint32_t)(cx0 - 2 * bx), ix1 = (uint32_t)(cx1 - 2 * bx);
-    const uint32_t iy0 = (uint32_t)(cy0 - 2 * by), iy1 = (uint32_t)(cy1 - 2 * by);
-    const uint32_t iz0 = (uint32_t)(cz0 - 2 * bz), iz1 = (uint32_t)(cz1 - 2 * bz);
-    const uint64_t mx = (1ull << (((ix0 >> 1) << 3) | (ix0 & 1u))) | (1ull << (((ix1 >> 1) << 3) | (ix1 & 1u)));
-    const uint64_t mxy = (mx << (((iy0 >> 1) << 4) | ((iy0 & 1u) << 1))) | (mx << (((iy1 >> 1) << 4) | ((iy1 & 1u) << 1)));
-    const uint64_t m = (mxy << (((iz0 >> 1) << 5) | ((iz0 & 1u) << 2))) | (mxy << (((iz1 >> 1) << 5) | ((iz1 & 1u) << 2)));
+           (cz1 >> 1) != (cz0 >> 1));
+    // bit of cell c in pm: 8 * slot + cell in block; per axis the slot bit is bit 1
+    // of c (the block's parity) and the cell in block bit 0 of c
+    const uint32_t x0 = (((uint32_t)cx0 & 2u) << 2) | ((uint32_t)cx0 & 1u);
+    const uint32_t x1 = (((uint32_t)cx1 & 2u) << 2) | ((uint32_t)cx1 & 1u);
+    const uint32_t y0 = (((uint32_t)cy0 & 2u) << 3) | (((uint32_t)cy0 & 1u) << 1);
+    const uint32_t y1 = (((uint32_t)cy1 & 2u) << 3) | (((uint32_t)cy1 & 1u) << 1);
+    const uint32_t z0 = (((uint32_t)cz0 & 2u) << 4) | (((uint32_t)cz0 & 1u) << 2);
+    const uint32_t z1 = (((uint32_t)cz1 & 2u) << 4) | (((uint32_t)cz1 & 1u) << 2);
+    const uint64_t mx = (1ull << x0) | (1ull << x1);
+    const uint64_t mxy = (mx << y0) | (mx << y1);
+    const uint64_t m = (mxy << z0) | (mxy << z1);
     return (pm & m) == 0ull;
   }
 };
@@ -1326,8 +1296,9 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
   if (x < (uint32_t)ra.W && y < (uint32_t)ra.H) {
     const float MINF = 1e10f;
     const int pix = (int)y * ra.W + (int)x;
+    __shared__ int32_t slot_lds[8 * 64];
     BlockCache<DIAG> bc;
-    bc.init(ra.mark_bkt, ra.mark_blk, ra.march_stats);
+    bc.init(slot_lds + threadIdx.x, ra.mark_bkt, ra.mark_blk, ra.march_stats);
     const RayHit h = march_ray<F16, DIAG, REF>(t, ra, vox, col, x, y, bc, ra.budget);
     deferred = h.deferred;
     tr.pix = (uint32_t)pix;
@@ -1404,8 +1375,9 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
   const float MINF = 1e10f;
   uint32_t hit = 0, miss = 0;
   uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_window = 0, n_tskip = 0;
+  __shared__ int32_t slot_lds[8 * 64];
   BlockCache<DIAG> bc;
-  bc.init(ra.mark_bkt, ra.mark_blk, ra.march_stats);
+  bc.init(slot_lds + threadIdx.x, ra.mark_bkt, ra.mark_blk, ra.march_stats);
   OccCache oc;
   occ_cache_init(oc);
   RayHit h;
@@ -1719,8 +1691,9 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
   const int pix = (int)y * ra.W + (int)x;
   float J[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
   if (st->n_alloc != 0u) {
+    __shared__ int32_t slot_lds[8 * 64];
     BlockCache<false> bc;
-    bc.init(nullptr, nullptr);
+    bc.init(slot_lds + threadIdx.x, nullptr, nullptr);
     const RayHit h = march_ray<F16, false, REF>(t, ra, vox, col, x, y, bc, 0u);
     if (h.found) {
       const float v = ra.voxel;
