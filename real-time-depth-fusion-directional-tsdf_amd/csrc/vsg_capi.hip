@@ -158,6 +158,7 @@ void free_scene(vsg_scene* s) {
   (void)hipFree(s->t.claim);
   (void)hipFree(s->t.blk);
   (void)hipFree(s->t.occ);
+  (void)hipFree(s->t.hyp);
   (void)hipFree(s->t.idx);
   (void)hipFree(s->t.pcnt);
   (void)hipFree(s->t.idxpos);
@@ -272,6 +273,11 @@ vsg_status vsg_scene_create(const vsg_params* params, vsg_scene** out) {
   while (occ_words < N) occ_words <<= 1;
   s->t.occ_mask = (uint32_t)(occ_words - 1);
   if (r == VSG_OK) r = dmalloc(&s->t.occ, occ_words * sizeof(uint64_t), "block occupancy map");
+  // hyper-block map: one word per 4x4x4 super-blocks, at least 2^12 words
+  size_t hyp_words = 1u << 12;
+  while (hyp_words < occ_words / 16) hyp_words <<= 1;
+  s->t.hyp_mask = (uint32_t)(hyp_words - 1);
+  if (r == VSG_OK) r = dmalloc(&s->t.hyp, hyp_words * sizeof(uint64_t), "hyper-block occupancy map");
   // block index: a power of two of at least 4 entries per block slot
   size_t idx_entries = 1u << 16;
   while (idx_entries < 4 * N) idx_entries <<= 1;

@@ -31,7 +31,9 @@ constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
 // occ is the block occupancy map: one 64-bit word per super-block of 4x4x4 blocks,
 // hashed into occ_mask + 1 words; bit occ_bit(x, y, z) of the word of block
 // (x, y, z)'s super-block is set once the block is allocated.  A hash collision
-// only sets extra bits, so a clear bit proves the block absent.
+// only sets extra bits, so a clear bit proves the block absent.  hyp is the same one
+// level up: one word per hyper-block of 4x4x4 super-blocks, bit hyp_bit of a
+// super-block set once one of its blocks is allocated.
 // idx is the block index the raycaster reads: an open-addressing table of
 // idx_mask + 1 16-byte entries {packed key lo, hi, block pointer, 0} (linear
 // probing, load factor <= 1/4, empty key all ones), filled when a block is
@@ -43,12 +45,14 @@ struct TableView {
   uint32_t* claim;
   int4* blk;
   uint64_t* occ;
+  uint64_t* hyp;
   uint4* idx;
   uint32_t* pcnt;    // per block and 4^3-voxel cell: voxels with weight > 0 and SDF > 0
   uint32_t* idxpos;  // per block: its entry slot in the block index
   uint32_t num_buckets;
   uint32_t num_blocks;
   uint32_t occ_mask;
+  uint32_t hyp_mask;
   uint32_t idx_mask;
 };
 
@@ -191,6 +195,14 @@ __device__ __forceinline__ uint32_t occ_index(int32_t sx, int32_t sy, int32_t sz
 __device__ __forceinline__ uint32_t occ_bit(int32_t x, int32_t y, int32_t z) {
   return (uint32_t)(x & 3) | ((uint32_t)(y & 3) << 2) | ((uint32_t)(z & 3) << 4);
 }
+
+// Hyper-block map word of hyper-block (hx, hy, hz) and bit of super-block (sx, sy, sz)
+// in the word of its hyper-block (sx >> 2, sy >> 2, sz >> 2).
+__device__ __forceinline__ uint32_t hyp_index(int32_t hx, int32_t hy, int32_t hz, uint32_t mask) {
+  return hash3(hx, hy, hz) & mask;
+}
+
+__device__ __forceinline__ uint32_t hyp_bit(int32_t sx, int32_t sy, int32_t sz) { return occ_bit(sx, sy, sz); }
 
 // Home slot of block (x, y, z) in the block index.
 __device__ __forceinline__ uint32_t idx_home(int32_t x, int32_t y, int32_t z, uint32_t mask) {

@@ -301,6 +301,8 @@ __global__ __launch_bounds__(256) void k_alloc_insert(TableView t, DevState* st,
     t.blk[p] = make_int4(x, y, z, (int32_t)(h * kBucket + slot));
     atomicOr(reinterpret_cast<unsigned long long*>(&t.occ[occ_index(x >> 2, y >> 2, z >> 2, t.occ_mask)]),
              1ull << occ_bit(x, y, z));
+    atomicOr(reinterpret_cast<unsigned long long*>(&t.hyp[hyp_index(x >> 4, y >> 4, z >> 4, t.hyp_mask)]),
+             1ull << hyp_bit(x >> 2, y >> 2, z >> 2));
     // block index: claim the first empty entry from the home slot (keys are unique,
     // so no entry can already hold this key)
     const uint64_t key = pack_key(x, y, z);
@@ -901,6 +903,8 @@ __device__ __forceinline__ void box_interval(const RaySkip& k, float lox, float 
 struct OccCache {
   uint32_t idx;        // map index of the cached word
   uint64_t word;
+  uint32_t hidx;       // hyper-block map index of the cached hyper word
+  uint64_t hword;
   int32_t rx, ry, rz;  // block range of the last test that found an allocated block:
   uint32_t rs;         // lower corner, bits 0-2 its extent minus one per axis, bit 3 set when valid
 };
@@ -908,6 +912,8 @@ struct OccCache {
 __device__ __forceinline__ void occ_cache_init(OccCache& oc) {
   oc.idx = 0xFFFFFFFFu;
   oc.word = 0ull;
+  oc.hidx = 0xFFFFFFFFu;
+  oc.hword = 0ull;
   oc.rs = 0u;
 }
 
@@ -1038,8 +1044,9 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs& ra, uint32_t x, uint
 
 // Skip test at ray parameter `ray`: 0 when the super-blocks around the sample hold
 // no allocated block, 1 when the blocks around it hold none, 2 otherwise; [t0, t1)
-// is the parameter interval of the box in which that holds (the super-blocks' box
-// for 0, the blocks' box for 1 and 2).
+// is the parameter interval of the box in which that holds (for 0, the hyper-block's
+// box when the blocks lie in one hyper-block with no allocated block, else the
+// super-blocks'; the blocks' box for 1 and 2).
 __device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const RaySkip& sk, float wdx, float wdy,
                                          float wdz, float ray, float* t0, float* t1) {
   const float mq = 1.125f + sk.eps;  // range margin around Q (voxels)
@@ -1058,9 +1065,18 @@ __device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const
   oc.rz = bz0;
   oc.rs = occ == 2 ? ext : 0u;
   if (occ == 2) return 2;
-  const float g = occ == 0 ? 32.0f : 8.0f;
-  const int32_t sh = occ == 0 ? 2 : 0;
-  const float lo = 1.0f, hi = occ == 0 ? 31.0f : 7.0f;
+  bool hyper = false;
+  if (occ == 0 && (bx0 >> 4) == (bx1 >> 4) && (by0 >> 4) == (by1 >> 4) && (bz0 >> 4) == (bz1 >> 4)) {
+    const uint32_t hi_ = hyp_index(bx0 >> 4, by0 >> 4, bz0 >> 4, t.hyp_mask);
+    if (hi_ != oc.hidx) {
+      oc.hidx = hi_;
+      oc.hword = t.hyp[hi_];
+    }
+    hyper = oc.hword == 0ull;
+  }
+  const float g = hyper ? 128.0f : (occ == 0 ? 32.0f : 8.0f);
+  const int32_t sh = hyper ? 4 : (occ == 0 ? 2 : 0);
+  const float lo = 1.0f, hi = hyper ? 127.0f : (occ == 0 ? 31.0f : 7.0f);
   box_interval(sk, g * (float)(bx0 >> sh) + lo, g * (float)(bx1 >> sh) + hi, g * (float)(by0 >> sh) + lo,
                g * (float)(by1 >> sh) + hi, g * (float)(bz0 >> sh) + lo, g * (float)(bz1 >> sh) + hi, t0, t1);
   return occ;
@@ -1962,6 +1978,7 @@ __global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) 
     t.claim[b] = kNoClaim;
   }
   for (size_t w = tid; w <= t.occ_mask; w += stride) t.occ[w] = 0ull;
+  for (size_t w = tid; w <= t.hyp_mask; w += stride) t.hyp[w] = 0ull;
   for (size_t w = tid; w <= t.idx_mask; w += stride) t.idx[w] = make_uint4(kIdxEmpty, kIdxEmpty, 0u, 0u);
   if (tid == 0) {
     st->n_alloc = 0;
