@@ -98,10 +98,11 @@ struct FrameArgs {
 
 // float->int like the hardware conversion: saturating, NaN -> 0.
 __device__ __forceinline__ int32_t cvt_i32(float x) {
-  if (x != x) return 0;
-  if (x >= 2147483648.0f) return INT32_MAX;
-  if (x <= -2147483648.0f) return INT32_MIN;
-  return (int32_t)x;
+  // v_cvt_i32_f32 truncates toward zero, saturates out-of-range values and maps NaN
+  // to 0: the conversion wanted, in one instruction
+  int32_t r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
 }
 
 __device__ __forceinline__ float sgnf(float q) { return q > 0.0f ? 1.0f : (q < 0.0f ? -1.0f : 0.0f); }
@@ -113,7 +114,10 @@ __device__ __forceinline__ int32_t alloc_voxel_axis(float p, float v) {
 }
 
 // Raycaster world->voxel (RayCastRender.cu:17-25): floor(q + 0.5*sign(q)).
-__device__ __forceinline__ int32_t ray_voxel_of_q(float q) { return cvt_i32(floorf(q + 0.5f * sgnf(q))); }
+__device__ __forceinline__ int32_t ray_voxel_of_q(float q) {
+  // q + 0.5 sign(q), with sign(+-0) = 0 (a NaN stays NaN and converts to 0)
+  return cvt_i32(floorf(q == 0.0f ? q : q + copysignf(0.5f, q)));
+}
 
 // virtualVoxelPosToSDFBlock (VoxelScene.cu:91-95, RayCastRender.cu:28-34).
 __device__ __forceinline__ int32_t voxel_to_block_axis(int32_t vi) {
