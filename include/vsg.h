@@ -283,6 +283,12 @@ vsg_status vsg_diag_enable(vsg_scene* scene, int32_t on);
  * clears them afterwards. */
 vsg_status vsg_diag_read(vsg_scene* scene, vsg_diag* out, int32_t clear);
 
+/* The kernels' float -> int32 conversion applied on the device to n HOST floats
+ * (truncation toward zero, out-of-range values saturated, NaN -> 0: the rule every
+ * index computation of the path follows, the oracle's too), for verification.
+ * Synchronising.  No counterpart in the reference. */
+vsg_status vsg_diag_convert(const float* in, int32_t* out, uint32_t n);
+
 /* ---- kernel timing ---------------------------------------------------------- */
 
 /* Kernel ids for vsg_timing_read. */

@@ -909,6 +909,27 @@ vsg_status vsg_diag_enable(vsg_scene* s, int32_t on) {
   return VSG_OK;
 }
 
+vsg_status vsg_diag_convert(const float* in, int32_t* out, uint32_t n) {
+  if (!in || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_diag_convert: null argument");
+  if (n == 0) return VSG_OK;
+  float* din = nullptr;
+  int32_t* dout = nullptr;
+  vsg_status st = dmalloc(&din, (size_t)n * sizeof(float), "conversion input");
+  if (st == VSG_OK) st = dmalloc(&dout, (size_t)n * sizeof(int32_t), "conversion output");
+  hipError_t e = hipSuccess;
+  if (st == VSG_OK) e = hipMemcpy(din, in, (size_t)n * sizeof(float), hipMemcpyHostToDevice);
+  if (st == VSG_OK && e == hipSuccess) {
+    launch_convert(nullptr, din, dout, n);
+    st = check_launch("convert");
+    if (st == VSG_OK) e = hipMemcpy(out, dout, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost);
+  }
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  if (st != VSG_OK) return st;
+  if (e != hipSuccess) return hip_fail(e, "vsg_diag_convert");
+  return VSG_OK;
+}
+
 vsg_status vsg_diag_read(vsg_scene* s, vsg_diag* out, int32_t clear) {
   if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_diag_read: null argument");
   std::memset(out, 0, sizeof(*out));

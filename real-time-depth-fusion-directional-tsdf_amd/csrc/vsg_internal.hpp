@@ -67,6 +67,8 @@ void launch_compact(hipStream_t s, const TableView& t, DevState* st, const Frame
 void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
                       const float* pts, const uint8_t* rgb, const float* nrm, const uint32_t* vis, void* vox,
                       uint32_t* col, uint32_t* dmeta, float2* dval, uint32_t grid, uint32_t* upd_part);
+// out[i] = cvt_i32(in[i]) for i < n (device buffers).
+void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n);
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                            const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
                            float* grad_sdf);

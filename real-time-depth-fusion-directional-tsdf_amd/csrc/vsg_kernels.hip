@@ -814,21 +814,21 @@ __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const vo
   bc.corners(t, bx0, by0, bz0, bx1 != bx0, by1 != by0, bz1 != bz0, p);
   // Corners without an allocated block or with weight 0 contribute a zero weight:
   // adding +-0 leaves every sum unchanged (a sum started at +0 never becomes -0),
-  // so this equals skipping them.
+  // so this equals skipping them.  Every corner is loaded (a corner without a block
+  // reads voxel 0 of the store and its values are replaced by zeros), so the gather
+  // has no branches.
   const float ux = 1.0f - wx, uy = 1.0f - wy, uz = 1.0f - wz;
   float d = 0.0f, cr = 0.0f, cg = 0.0f, cbl = 0.0f, wsum = 0.0f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const uint32_t vi = (uint32_t)p[i] * (uint32_t)kVox + ox[i & 1] + oy[(i >> 1) & 1] + oz[(i >> 2) & 1];
-    typename S::word w{};
-    uint32_t c = 0u;
-    if (p[i] >= 0) {
-      w = vox[vi];
-      if (COLOR) c = col[vi];
-    }
+    const bool ok = p[i] >= 0;
+    const uint32_t vi =
+        ok ? (uint32_t)p[i] * (uint32_t)kVox + ox[i & 1] + oy[(i >> 1) & 1] + oz[(i >> 2) & 1] : 0u;
+    const typename S::word w = vox[vi];
+    const uint32_t c = COLOR ? (ok ? col[vi] : 0u) : 0u;
     const float wi = ((i & 1) ? wx : ux) * ((i & 2) ? wy : uy) * ((i & 4) ? wz : uz);
-    const float wk = S::weight(w) != 0u ? wi : 0.0f;
-    d += wk * S::sdf(w);
+    const float wk = ok && S::weight(w) != 0u ? wi : 0.0f;
+    d += wk * (ok ? S::sdf(w) : 0.0f);
     if (COLOR) {
       cr += wk * (float)(c & 0xFFu);
       cg += wk * (float)((c >> 8) & 0xFFu);
@@ -1994,6 +1994,11 @@ __global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) 
   }
 }
 
+__global__ __launch_bounds__(256) void k_convert(const float* __restrict__ in, int32_t* __restrict__ out, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) out[i] = cvt_i32(in[i]);
+}
+
 // ---------------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------------
@@ -2093,6 +2098,10 @@ void launch_gn(hipStream_t s, const float* depth, const float* obs, const float*
 uint32_t tail_workgroups(int32_t W, int32_t H) {
   const uint64_t n = ((uint64_t)W * (uint64_t)H + 63u) / 64u;
   return (uint32_t)(n < 1u ? 1u : (n > 4096u ? 4096u : n));
+}
+
+void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n) {
+  if (n) hipLaunchKernelGGL(k_convert, dim3((n + 255u) / 256u), dim3(256), 0, s, in, out, n);
 }
 
 void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,

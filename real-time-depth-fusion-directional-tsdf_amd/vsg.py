@@ -103,6 +103,7 @@ _SIGNATURES = {
     "vsg_import_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_export_cell_masks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "vsg_diag_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vsg_diag_convert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "vsg_diag_read": (C.c_int, [C.c_void_p, C.POINTER(vsg_diag), C.c_int32]),
     "vsg_timing_enable": (C.c_int, [C.c_int32]),
     "vsg_timing_reset": (C.c_int, []),
@@ -539,6 +540,14 @@ class RayCastRender:
 
     def isAllocated(self) -> bool:
         return self.m_isAllocated
+
+
+def diag_convert(values) -> np.ndarray:
+    """The kernels' float -> int32 conversion (truncation, saturation, NaN -> 0) on the device."""
+    x = np.ascontiguousarray(values, np.float32).reshape(-1)
+    out = np.zeros(x.size, np.int32)
+    _check(lib().vsg_diag_convert(x.ctypes.data, out.ctypes.data, x.size))
+    return out
 
 
 def timing_enable(on: bool = True) -> None:
