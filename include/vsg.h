@@ -217,6 +217,38 @@ vsg_status vsg_render_download(vsg_render* render, float* depth, float* position
 /* Hit / miss counts of the last render (RayCastRender.cu:599-616).  Synchronising. */
 vsg_status vsg_render_counters(vsg_render* render, uint32_t* hits, uint32_t* misses);
 
+/* ---- surface-point extraction ------------------------------------------------------- */
+
+/* Capacity of the extraction output (m_maxExtractedPoints, RayCastRender.cpp:62:
+ * default 10,000,000).  Takes effect at the next extraction (the buffers are
+ * reallocated then). */
+vsg_status vsg_render_set_max_points(vsg_render* render, uint32_t max_points);
+
+/* RayCastRender::extractSurfacePoints (VoxelSG/src/RayCastRender.cpp:307-364,
+ * kernelExtractSurfacePoints VoxelSG/src/RayCastRender.cu:684-868): every voxel with
+ * weight >= max(1, min_weight) and SDF in [min_sdf, max_sdf] emits a point (the mean of
+ * its zero crossings towards the 6 neighbours, or its centre when it has none and
+ * |sdf| < 0.001), a normal (central differences of getVoxel, computeNormalFromSDF
+ * :629-676) and its colour / 255.  Points are in hash-entry order (bucket, slot), then
+ * voxel order (x fastest): the reference appends them in atomicAdd order, which no
+ * run repeats.  *total (may be NULL) = points found; *stored (may be NULL) =
+ * min(total, max points) written to the device buffers of vsg_extracted_points.  The
+ * reference returns its atomic counter, which can exceed its buffer.  Synchronising
+ * (the count is read back, as in the reference). */
+vsg_status vsg_extract_surface_points(vsg_render* render, vsg_scene* scene, float min_sdf, float max_sdf,
+                                      uint32_t min_weight, uint32_t* stored, uint64_t* total);
+
+/* Device buffers of the last extraction (m_d_extractedPositions / Colors / Normals,
+ * RayCastRender.h:33-36): float4[max points] each (position w = 1, colour a = 1,
+ * normal w = 0); *count = points stored.  Any output may be NULL. */
+vsg_status vsg_extracted_points(const vsg_render* render, float** positions, float** colors, float** normals,
+                                uint32_t* count);
+
+/* The first min(capacity, stored) points of the last extraction to HOST float4
+ * arrays (any may be NULL); *count = points copied.  Synchronising. */
+vsg_status vsg_extracted_points_download(vsg_render* render, float* positions, float* colors, float* normals,
+                                         uint32_t capacity, uint32_t* count);
+
 /* ---- differentiable raycast and pose refinement (no counterpart in the reference;
  *      the reference's renderer is forward only) ---------------------------------------- */
 
@@ -303,7 +335,8 @@ enum {
   VSG_K_RESET = 7,
   VSG_K_ADJOINT = 8,
   VSG_K_GN = 9,
-  VSG_K_COUNT = 10
+  VSG_K_EXTRACT = 10,
+  VSG_K_COUNT = 11
 };
 
 /* When enabled, every kernel launch is bracketed by HIP events on the stream it

@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -173,6 +174,7 @@ class RayCastRender {
   RayCastRender() = default;
   ~RayCastRender() {
     if (m_render) vsg_render_destroy(m_render);
+    if (m_extractOnly) vsg_render_destroy(m_extractOnly);
   }
   RayCastRender(const RayCastRender&) = delete;
   RayCastRender& operator=(const RayCastRender&) = delete;
@@ -184,6 +186,7 @@ class RayCastRender {
     }
     if (!vsg_detail::report(vsg_render_create(width, height, &m_render), "RayCastRender: Failed to allocate GPU memory!"))
       return false;
+    vsg_render_set_max_points(m_render, (uint32_t)m_maxExtractedPoints);
     m_width = width;
     m_height = height;
     m_isAllocated = true;
@@ -267,6 +270,85 @@ class RayCastRender {
     return true;
   }
 
+  // RayCastRender::extractSurfacePoints (RayCastRender.cpp:307-364): points in
+  // hash-entry order, then voxel order.  Returns the number found (the reference's
+  // counter); the first getMaxExtractedPoints() are kept.  Works without initialize(),
+  // as in the reference.
+  int extractSurfacePoints(const VoxelScene* voxelScene, float minSDF = -0.01f, float maxSDF = 0.01f,
+                           unsigned int minWeight = 1) {
+    if (!voxelScene || !voxelScene->isAllocated()) {
+      std::fprintf(stderr, "RayCastRender::extractSurfacePoints: Invalid voxelScene!\n");
+      return 0;
+    }
+    vsg_render* r = extractor();
+    if (!r) return 0;
+    uint32_t stored = 0;
+    uint64_t total = 0;
+    if (!vsg_detail::report(vsg_extract_surface_points(r, voxelScene->handle(), minSDF, maxSDF, minWeight, &stored,
+                                                       &total),
+                            "RayCastRender::extractSurfacePoints"))
+      return 0;
+    m_numExtractedPoints = (int)stored;
+    return total > 0x7FFFFFFFull ? 0x7FFFFFFF : (int)total;
+  }
+
+  // RayCastRender::saveExtractedSurfacePointsPLY (RayCastRender.cpp:370-446): ASCII
+  // PLY of the extracted points with finite positions (x y z nx ny nz red green blue).
+  bool saveExtractedSurfacePointsPLY(const std::string& filePath) {
+    vsg_render* r = m_isAllocated ? m_render : m_extractOnly;
+    if (!r || m_numExtractedPoints == 0) {
+      std::fprintf(stderr, "RayCastRender::saveExtractedSurfacePointsPLY: No points to save!\n");
+      return false;
+    }
+    const size_t n = (size_t)m_numExtractedPoints;
+    std::vector<float4> p(n), c(n), nr(n);
+    uint32_t got = 0;
+    if (!vsg_detail::report(vsg_extracted_points_download(r, (float*)p.data(), (float*)c.data(), (float*)nr.data(),
+                                                          (uint32_t)n, &got),
+                            "RayCastRender::saveExtractedSurfacePointsPLY"))
+      return false;
+    auto finite3 = [](const float4& v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); };
+    size_t valid = 0;
+    for (size_t i = 0; i < got; ++i) valid += finite3(p[i]) ? 1 : 0;
+    FILE* fp = std::fopen(filePath.c_str(), "w");
+    if (!fp) {
+      std::fprintf(stderr, "RayCastRender::saveExtractedSurfacePointsPLY: Failed to open file: %s\n",
+                   filePath.c_str());
+      return false;
+    }
+    std::fprintf(fp, "ply\nformat ascii 1.0\ncomment Generated by RayCastRender::extractSurfacePoints\n");
+    std::fprintf(fp, "element vertex %zu\n", valid);
+    std::fprintf(fp, "property float x\nproperty float y\nproperty float z\n");
+    std::fprintf(fp, "property float nx\nproperty float ny\nproperty float nz\n");
+    std::fprintf(fp, "property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n");
+    auto chan = [](float v) -> int {
+      if (!std::isfinite(v)) return 255;
+      v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+      return (int)(v * 255.0f + 0.5f);
+    };
+    for (size_t i = 0; i < got; ++i) {
+      if (!finite3(p[i])) continue;
+      std::fprintf(fp, "%g %g %g %g %g %g %d %d %d\n", p[i].x, p[i].y, p[i].z, nr[i].x, nr[i].y, nr[i].z,
+                   chan(c[i].x), chan(c[i].y), chan(c[i].z));
+    }
+    std::fclose(fp);
+    return true;
+  }
+
+  // Capacity of the extraction (m_maxExtractedPoints, RayCastRender.cpp:62; 10M).
+  void setMaxExtractedPoints(int maxPoints) {
+    m_maxExtractedPoints = maxPoints < 0 ? 0 : maxPoints;
+    if (m_render) vsg_render_set_max_points(m_render, (uint32_t)m_maxExtractedPoints);
+    if (m_extractOnly) vsg_render_set_max_points(m_extractOnly, (uint32_t)m_maxExtractedPoints);
+  }
+  int getMaxExtractedPoints() const { return m_maxExtractedPoints; }
+  int getNumExtractedPoints() const { return m_numExtractedPoints; }
+
+  // Device buffers of the extracted points (float4 each).
+  float4* getExtractedPositions() const { return (float4*)ext(0); }
+  float4* getExtractedColors() const { return (float4*)ext(1); }
+  float4* getExtractedNormals() const { return (float4*)ext(2); }
+
   float4* getOutputDepth() const { return (float4*)buf(0); }  // a float[W*H], as in the reference
   float4* getOutputColor() const { return (float4*)buf(2); }
   float4* getOutputNormal() const { return (float4*)buf(3); }
@@ -284,7 +366,26 @@ class RayCastRender {
     if (m_render) vsg_render_buffers(m_render, &b[0], &b[1], &b[2], &b[3]);
     return b[i];
   }
+  // the renderer extractions run on: the initialised one, else a 1x1 one of its own
+  vsg_render* extractor() {
+    if (m_isAllocated) return m_render;
+    if (!m_extractOnly) {
+      if (!vsg_detail::report(vsg_render_create(1, 1, &m_extractOnly), "RayCastRender::extractSurfacePoints"))
+        return nullptr;
+      vsg_render_set_max_points(m_extractOnly, (uint32_t)m_maxExtractedPoints);
+    }
+    return m_extractOnly;
+  }
+  float* ext(int i) const {
+    float* b[3] = {nullptr, nullptr, nullptr};
+    vsg_render* r = m_isAllocated ? m_render : m_extractOnly;
+    if (r) vsg_extracted_points(r, &b[0], &b[1], &b[2], nullptr);
+    return b[i];
+  }
   vsg_render* m_render = nullptr;
+  vsg_render* m_extractOnly = nullptr;
   int m_width = 0, m_height = 0;
   bool m_isAllocated = false;
+  int m_maxExtractedPoints = 10000000;
+  int m_numExtractedPoints = 0;
 };

@@ -50,6 +50,8 @@ def lib() -> C.CDLL:
         L.vsgo_export_blocks.restype = C.c_uint64
         L.vsgo_export_blocks.argtypes = [vp, vp, vp, vp, C.c_uint64]
         L.vsgo_export_voxels.argtypes = [vp, C.c_uint64, vp, vp, vp, vp]
+        L.vsgo_extract_surface_points.restype = C.c_uint64
+        L.vsgo_extract_surface_points.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, C.c_uint64, vp, vp, vp]
         L.vsgo_hash.restype = C.c_uint32
         L.vsgo_hash.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
         L.vsgo_alloc_world_to_block.argtypes = [vp, C.c_float, vp, vp]
@@ -201,6 +203,22 @@ class OracleScene:
         if n:
             lib().vsgo_export_blocks(self._h, coords.ctypes.data, ptrs.ctypes.data, slots.ctypes.data, n)
         return coords, ptrs, slots
+
+    def extract_surface_points(self, min_sdf: float = -0.01, max_sdf: float = 0.01, min_weight: int = 1,
+                               max_points: int = 10_000_000) -> dict:
+        """Surface points (RayCastRender.cu:684-868) in hash-entry then voxel order: the
+        first max_points of them (positions and normals float64 [n, 3], colours uint8
+        [n, 3]) and the number found (total)."""
+        total = int(lib().vsgo_extract_surface_points(self._h, float(min_sdf), float(max_sdf), int(min_weight), 0,
+                                                      None, None, None))
+        n = min(total, int(max_points))
+        pos = np.zeros((n, 3), np.float64)
+        nrm = np.zeros((n, 3), np.float64)
+        col = np.zeros((n, 3), np.uint8)
+        if n:
+            lib().vsgo_extract_surface_points(self._h, float(min_sdf), float(max_sdf), int(min_weight), n,
+                                              pos.ctypes.data, nrm.ctypes.data, col.ctypes.data)
+        return {"pos": pos, "nrm": nrm, "col": col, "total": total}
 
     def export_voxels(self, nblocks: int | None = None):
         """(sdf float64 [n*512], weight uint8, colour uint8 [n*512,3], flags uint8) for pointers [0, n)."""

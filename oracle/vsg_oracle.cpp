@@ -190,6 +190,27 @@ void vsgo_export_voxels(void* p, uint64_t nblocks, double* sdf, uint8_t* weight,
   else do_export_voxels(h->sf, nblocks, sdf, weight, color3, flags);
 }
 
+// Surface points of the current store (positions, normals: 3 doubles each; colours:
+// 3 bytes), the first `cap` in hash-entry order; returns the number found.
+uint64_t vsgo_extract_surface_points(void* p, double min_sdf, double max_sdf, uint32_t min_weight, uint64_t cap,
+                                     double* pos3, double* nrm3, uint8_t* col3) {
+  Handle* h = (Handle*)p;
+  auto run = [&](auto& sc, auto tag) -> uint64_t {
+    using R = decltype(tag);
+    std::vector<SurfacePoint<R>> pts;
+    const uint64_t total = extract_surface_points(sc, (R)min_sdf, (R)max_sdf, min_weight, cap, pts);
+    for (size_t i = 0; i < pts.size(); ++i) {
+      for (int k = 0; k < 3; ++k) {
+        if (pos3) pos3[3 * i + k] = (double)pts[i].p[k];
+        if (nrm3) nrm3[3 * i + k] = (double)pts[i].n[k];
+        if (col3) col3[3 * i + k] = pts[i].c[k];
+      }
+    }
+    return total;
+  };
+  return h->fp64 ? run(h->sd, 0.0) : run(h->sf, 0.0f);
+}
+
 // ---- unit-level entry points (float arithmetic, as the reference kernels)
 uint32_t vsgo_hash(int32_t x, int32_t y, int32_t z, uint32_t num_buckets) {
   return hash_bucket(I3{x, y, z}, num_buckets);

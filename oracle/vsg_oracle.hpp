@@ -15,6 +15,7 @@
 //   compaction .................. VoxelSG/src/VoxelScene.cu:1275-1300
 //   voxel integration ........... VoxelSG/src/VoxelScene.cu:918-1146
 //   raycast ..................... VoxelSG/src/RayCastRender.cu:13-517
+//   surface-point extraction .... VoxelSG/src/RayCastRender.cu:629-868, RayCastRender.cpp:307-364
 //   depth-range rule ............ VoxelSG/src/voxelSG.cpp:594-608
 //
 // Conventions this oracle fixes where the reference leaves them open:
@@ -28,6 +29,10 @@
 //     rounded sum (the build compiles this file with -ffp-contract=off).
 //   * float->int conversions saturate and map NaN to 0 (what the GPU's
 //     conversion instruction does; C++'s own cast is undefined there).
+//   * Surface points come out in hash-entry order (bucket, then slot), then voxel
+//     order within the block: the order a sequential run of the reference's
+//     one-thread-per-entry kernel produces (its GPU order is that of atomic
+//     arrival).  At most max_points are kept: the first ones in that order.
 //
 // Templated on the arithmetic type R: R=float is the parity oracle (bit-exact
 // with the GPU path), R=double is the precision oracle.  The voxel store can
@@ -917,6 +922,109 @@ inline bool adjoint_pixel(const Scene<R>& s, int x, int y, V3<R> cam, const R* M
       dds[8 * e + i] = c.d[6 + e] * tg[e].dfds[i] * cdd[2];
     }
   return true;
+}
+
+// ---- surface-point extraction (RayCastRender.cu:629-868 kernelExtractSurfacePoints and
+// computeNormalFromSDF, RayCastRender.cpp:307-364 extractSurfacePoints)
+template <class R> struct SurfacePoint {
+  R p[3];
+  R n[3];
+  uint8_t c[3];
+};
+
+// computeNormalFromSDF (RayCastRender.cu:629-676): central differences of getVoxel at
+// +-v/2 per axis (an axis whose two voxels are not both weighted contributes 0),
+// normalised when the length exceeds 1e-6.
+template <class R> inline V3<R> sdf_normal(const Scene<R>& s, V3<R> p, R v) {
+  const R eps = v * R(0.5f);
+  R g[3] = {R(0), R(0), R(0)};
+  for (int a = 0; a < 3; ++a) {
+    V3<R> pp = p, pm = p;
+    if (a == 0) { pp.x = p.x + eps; pm.x = p.x - eps; }
+    if (a == 1) { pp.y = p.y + eps; pm.y = p.y - eps; }
+    if (a == 2) { pp.z = p.z + eps; pm.z = p.z - eps; }
+    const VoxelRead<R> vp = get_voxel(s, pp, v), vm = get_voxel(s, pm, v);
+    if (vp.w > 0 && vm.w > 0) g[a] = (vp.sdf - vm.sdf) / (R(2.0f) * eps);
+  }
+  const R len = std::sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  if (len > R(1e-6f)) {
+    g[0] = g[0] / len;
+    g[1] = g[1] / len;
+    g[2] = g[2] / len;
+  }
+  return V3<R>{g[0], g[1], g[2]};
+}
+
+// kernelExtractSurfacePoints (RayCastRender.cu:684-868), entries in hash order.  A voxel
+// with weight >= max(1, min_weight) and SDF in [min_sdf, max_sdf] emits a point: the
+// mean of the zero crossings towards its 6 neighbours (a neighbour inside the block by
+// index, one outside through getVoxel at the voxel centre +- v; weightless neighbours
+// and equal signs contribute nothing; crossing at t = sdf / (sdf - nsdf) along the
+// axis), or its centre when there is none and |sdf| < 0.001.  Returns the number of
+// points found; keeps the first max_points.
+template <class R>
+inline uint64_t extract_surface_points(const Scene<R>& s, R min_sdf, R max_sdf, uint32_t min_weight,
+                                       uint64_t max_points, std::vector<SurfacePoint<R>>& out) {
+  const R v = s.voxel;
+  static const int off[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, 1}, {0, 0, -1}};
+  uint64_t total = 0;
+  out.clear();
+  for (size_t e = 0; e < s.slot_ptr.size(); ++e) {
+    const int32_t ptr = s.slot_ptr[e];
+    if (ptr == -1) continue;
+    const I3 b = s.slot_pos[e];
+    const V3<R> bw{R(b.x * kBlock) * v, R(b.y * kBlock) * v, R(b.z * kBlock) * v};
+    for (int z = 0; z < kBlock; ++z)
+      for (int y = 0; y < kBlock; ++y)
+        for (int x = 0; x < kBlock; ++x) {
+          const size_t vi = (size_t)ptr * kVox + (size_t)(x + y * kBlock + z * kBlock * kBlock);
+          const uint32_t w = s.weight[vi];
+          if (w == 0 || w < min_weight) continue;
+          const R sd = s.load_sdf(vi);
+          if (sd < min_sdf || sd > max_sdf) continue;
+          const V3<R> c{bw.x + (R(x) + R(0.5f)) * v, bw.y + (R(y) + R(0.5f)) * v, bw.z + (R(z) + R(0.5f)) * v};
+          V3<R> acc{R(0), R(0), R(0)};
+          int n = 0;
+          for (int d = 0; d < 6; ++d) {
+            const int nx = x + off[d][0], ny = y + off[d][1], nz = z + off[d][2];
+            const R dx = R(off[d][0]), dy = R(off[d][1]), dz = R(off[d][2]);
+            R nsd;
+            if (nx < 0 || nx >= kBlock || ny < 0 || ny >= kBlock || nz < 0 || nz >= kBlock) {
+              const VoxelRead<R> nv = get_voxel(s, V3<R>{c.x + dx * v, c.y + dy * v, c.z + dz * v}, v);
+              if (nv.w == 0) continue;
+              nsd = nv.sdf;
+            } else {
+              const size_t ni = (size_t)ptr * kVox + (size_t)(nx + ny * kBlock + nz * kBlock * kBlock);
+              if (s.weight[ni] == 0) continue;
+              nsd = s.load_sdf(ni);
+            }
+            if ((sd < R(0) && nsd > R(0)) || (sd > R(0) && nsd < R(0))) {
+              const R t = sd / (sd - nsd);
+              acc.x = acc.x + (c.x + dx * v * t);
+              acc.y = acc.y + (c.y + dy * v * t);
+              acc.z = acc.z + (c.z + dz * v * t);
+              n++;
+            }
+          }
+          V3<R> iso;
+          if (n == 0) {
+            if (!(std::fabs(sd) < R(0.001f))) continue;
+            iso = c;
+          } else {
+            iso = V3<R>{acc.x / R(n), acc.y / R(n), acc.z / R(n)};
+          }
+          const V3<R> nrm = sdf_normal(s, iso, v);
+          if (total < max_points) {
+            SurfacePoint<R> sp;
+            sp.p[0] = iso.x; sp.p[1] = iso.y; sp.p[2] = iso.z;
+            sp.n[0] = nrm.x; sp.n[1] = nrm.y; sp.n[2] = nrm.z;
+            sp.c[0] = s.color[3 * vi]; sp.c[1] = s.color[3 * vi + 1]; sp.c[2] = s.color[3 * vi + 2];
+            out.push_back(sp);
+          }
+          total++;
+        }
+  }
+  return total;
 }
 
 }  // namespace vsgo

@@ -4,8 +4,13 @@
 // compute the depth range from the host depth map (voxelSG.cpp:594-608), render,
 // download.  Frames come from the synthetic generator (libvsg_synth.so).
 //
-// usage: vsg_shim_example <workload 0|1> <frames>
-// prints: frames, allocated blocks, hits, misses, and a checksum of the last depth map.
+// After the last frame, RayCastRender::extractSurfacePoints on a renderer that was
+// never initialised (as the reference allows) and, when a path is given,
+// saveExtractedSurfacePointsPLY.
+//
+// usage: vsg_shim_example <workload 0|1> <frames> [points.ply]
+// prints: frames, allocated blocks, hits, misses, a checksum of the last depth map,
+// and the number of surface points found.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -78,8 +83,11 @@ int main(int argc, char** argv) {
     for (size_t k = 0; k < n; ++k)
       if (out[k] < 1e9f) checksum += (double)out[k] * (double)((k % 97) + 1);
   }
-  std::printf("frames %d blocks %u hits %u misses %u checksum %.9e\n", scene.getNumIntegratedFrames(),
-              (unsigned)(params.SDFBlockNum - scene.getHeapFreeCount()), hits, misses, checksum);
+  RayCastRender extractor;
+  const int points = extractor.extractSurfacePoints(&scene);
+  if (argc > 3 && !extractor.saveExtractedSurfacePointsPLY(argv[3])) return 6;
+  std::printf("frames %d blocks %u hits %u misses %u checksum %.9e points %d\n", scene.getNumIntegratedFrames(),
+              (unsigned)(params.SDFBlockNum - scene.getHeapFreeCount()), hits, misses, checksum, points);
   (void)hipFree(d_pts);
   (void)hipFree(d_nrm);
   (void)hipFree(d_rgb);

@@ -146,6 +146,17 @@ struct vsg_render {
   float* jac = nullptr;       // GN scratch: Jacobian per pixel
   double* partials = nullptr; // GN scratch: per-workgroup sums
   double* gn_out = nullptr;   // GN result (25 doubles)
+  // surface-point extraction: outputs of capacity ext_cap, scratch sized for a scene of
+  // ext_buckets buckets and ext_blocks blocks
+  uint32_t max_points = 10000000u;
+  uint32_t ext_cap = 0;
+  float4* ext_pos = nullptr;
+  float4* ext_col = nullptr;
+  float4* ext_nrm = nullptr;
+  uint32_t ext_count = 0;
+  ExtractScratch ext{};
+  uint32_t ext_buckets = 0, ext_blocks = 0;
+  uint32_t* ext_total = nullptr;  // pinned host: pstart and bcount of the last block slot
 };
 
 namespace {
@@ -189,6 +200,25 @@ template <class T> vsg_status dmalloc(T** p, size_t bytes, const char* what) {
                                  " bytes) failed: " + hipGetErrorString(e));
   }
   return VSG_OK;
+}
+
+void free_extract_outputs(vsg_render* r) {
+  (void)hipFree(r->ext_pos);
+  (void)hipFree(r->ext_col);
+  (void)hipFree(r->ext_nrm);
+  r->ext_pos = r->ext_col = r->ext_nrm = nullptr;
+  r->ext_cap = 0;
+  r->ext_count = 0;
+}
+
+void free_extract_scratch(vsg_render* r) {
+  (void)hipFree(r->ext.bucket_start);
+  (void)hipFree(r->ext.order);
+  (void)hipFree(r->ext.bcount);
+  (void)hipFree(r->ext.pstart);
+  (void)hipFree(r->ext.sums);
+  r->ext = ExtractScratch{};
+  r->ext_buckets = r->ext_blocks = 0;
 }
 
 vsg_status queue_reset(vsg_scene* s) {
@@ -632,6 +662,9 @@ vsg_status vsg_render_destroy(vsg_render* r) {
   (void)hipFree(r->jac);
   (void)hipFree(r->partials);
   (void)hipFree(r->gn_out);
+  free_extract_outputs(r);
+  free_extract_scratch(r);
+  if (r->ext_total) (void)hipHostFree(r->ext_total);
   if (r->ev) (void)hipEventDestroy(r->ev);
   delete r;
   return VSG_OK;
@@ -745,6 +778,101 @@ vsg_status vsg_render_counters(vsg_render* r, uint32_t* hits, uint32_t* misses) 
     *hits += h[2 * i];
     *misses += h[2 * i + 1];
   }
+  return VSG_OK;
+}
+
+// ---- surface-point extraction ----------------------------------------------------------
+
+vsg_status vsg_render_set_max_points(vsg_render* r, uint32_t max_points) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_max_points: null renderer");
+  r->max_points = max_points;
+  return VSG_OK;
+}
+
+vsg_status vsg_extract_surface_points(vsg_render* r, vsg_scene* s, float min_sdf, float max_sdf, uint32_t min_weight,
+                                      uint32_t* stored, uint64_t* total) {
+  if (!r || !s) return fail(VSG_ERR_INVALID_ARG, "vsg_extract_surface_points: null argument");
+  if (std::isnan(min_sdf) || std::isnan(max_sdf))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_extract_surface_points: SDF range must not be NaN");
+  const uint32_t B = s->t.num_buckets, N = s->t.num_blocks;
+  // point counts and starts are 32-bit: at most 2^32 - 1 voxels
+  if ((uint64_t)N * kVox > 0xFFFFFFFFull)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_extract_surface_points: scenes of more than 2^23 blocks are not supported");
+  hipStream_t strm = r->has_stream ? r->stream : s->stream;
+  if (r->has_last) VSG_HIP(hipStreamSynchronize(r->last));  // outputs and scratch are reused
+  if (r->ext_cap != r->max_points || !r->ext_pos) {
+    free_extract_outputs(r);
+    const size_t cap = r->max_points ? r->max_points : 1u;
+    vsg_status st = dmalloc(&r->ext_pos, cap * sizeof(float4), "extracted positions");
+    if (st == VSG_OK) st = dmalloc(&r->ext_col, cap * sizeof(float4), "extracted colours");
+    if (st == VSG_OK) st = dmalloc(&r->ext_nrm, cap * sizeof(float4), "extracted normals");
+    if (st != VSG_OK) {
+      free_extract_outputs(r);
+      return st;
+    }
+    r->ext_cap = r->max_points;
+  }
+  if (r->ext_buckets != B || r->ext_blocks != N) {
+    free_extract_scratch(r);
+    const uint32_t sums = extract_sums_words(B > N ? B : N);
+    vsg_status st = dmalloc(&r->ext.bucket_start, (size_t)B * sizeof(uint32_t), "extraction bucket starts");
+    if (st == VSG_OK) st = dmalloc(&r->ext.order, (size_t)N * sizeof(uint32_t), "extraction block order");
+    if (st == VSG_OK) st = dmalloc(&r->ext.bcount, (size_t)N * sizeof(uint32_t), "extraction block counts");
+    if (st == VSG_OK) st = dmalloc(&r->ext.pstart, (size_t)N * sizeof(uint32_t), "extraction block starts");
+    if (st == VSG_OK) st = dmalloc(&r->ext.sums, (size_t)sums * sizeof(uint32_t), "extraction scan sums");
+    if (st != VSG_OK) {
+      free_extract_scratch(r);
+      return st;
+    }
+    r->ext_buckets = B;
+    r->ext_blocks = N;
+  }
+  if (!r->ext_total && hipHostMalloc((void**)&r->ext_total, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    r->ext_total = nullptr;
+    return fail(VSG_ERR_OOM, "vsg_extract_surface_points: pinned host allocation failed");
+  }
+  if (strm != s->stream) {  // order after the scene's pending work
+    VSG_HIP(hipEventRecord(r->ev, s->stream));
+    VSG_HIP(hipStreamWaitEvent(strm, r->ev, 0));
+  }
+  {
+    TimedLaunch tl(strm, VSG_K_EXTRACT);
+    launch_extract(strm, s->p.store == VSG_STORE_FP16, s->t, s->st, s->vox, s->col, s->p.voxel_size, min_sdf, max_sdf,
+                   min_weight, r->ext_cap, r->ext, r->ext_pos, r->ext_col, r->ext_nrm);
+  }
+  vsg_status st = check_launch("extract surface points");
+  if (st != VSG_OK) return st;
+  VSG_HIP(hipMemcpyAsync(&r->ext_total[0], r->ext.pstart + (N - 1u), sizeof(uint32_t), hipMemcpyDeviceToHost, strm));
+  VSG_HIP(hipMemcpyAsync(&r->ext_total[1], r->ext.bcount + (N - 1u), sizeof(uint32_t), hipMemcpyDeviceToHost, strm));
+  VSG_HIP(hipStreamSynchronize(strm));
+  const uint64_t found = (uint64_t)r->ext_total[0] + (uint64_t)r->ext_total[1];
+  r->ext_count = (uint32_t)(found < r->ext_cap ? found : r->ext_cap);
+  r->last = strm;
+  r->has_last = true;
+  if (stored) *stored = r->ext_count;
+  if (total) *total = found;
+  return VSG_OK;
+}
+
+vsg_status vsg_extracted_points(const vsg_render* r, float** positions, float** colors, float** normals,
+                                uint32_t* count) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_extracted_points: null renderer");
+  if (positions) *positions = (float*)r->ext_pos;
+  if (colors) *colors = (float*)r->ext_col;
+  if (normals) *normals = (float*)r->ext_nrm;
+  if (count) *count = r->ext_count;
+  return VSG_OK;
+}
+
+vsg_status vsg_extracted_points_download(vsg_render* r, float* positions, float* colors, float* normals,
+                                         uint32_t capacity, uint32_t* count) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_extracted_points_download: null renderer");
+  if (r->has_last) VSG_HIP(hipStreamSynchronize(r->last));
+  const size_t n = r->ext_count < capacity ? r->ext_count : capacity;
+  if (n && positions) VSG_HIP(hipMemcpy(positions, r->ext_pos, n * sizeof(float4), hipMemcpyDeviceToHost));
+  if (n && colors) VSG_HIP(hipMemcpy(colors, r->ext_col, n * sizeof(float4), hipMemcpyDeviceToHost));
+  if (n && normals) VSG_HIP(hipMemcpy(normals, r->ext_nrm, n * sizeof(float4), hipMemcpyDeviceToHost));
+  if (count) *count = (uint32_t)n;
   return VSG_OK;
 }
 

@@ -67,6 +67,26 @@ void launch_compact(hipStream_t s, const TableView& t, DevState* st, const Frame
 void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,
                       const float* pts, const uint8_t* rgb, const float* nrm, const uint32_t* vis, void* vox,
                       uint32_t* col, uint32_t* dmeta, float2* dval, uint32_t grid, uint32_t* upd_part);
+// Scratch of the surface-point extraction (device buffers): bucket_start[num_buckets],
+// order, bcount, pstart[num_blocks], sums[extract_sums_words(max(num_buckets,
+// num_blocks))].
+struct ExtractScratch {
+  uint32_t* bucket_start;
+  uint32_t* order;
+  uint32_t* bcount;
+  uint32_t* pstart;
+  uint32_t* sums;
+};
+constexpr uint32_t kScanChunkSize = 4096;
+inline uint32_t extract_sums_words(uint32_t n) { return (n + kScanChunkSize - 1u) / kScanChunkSize + 1u; }
+
+// Surface points of the scene (RayCastRender.cu:629-907) in hash-entry order, then
+// voxel order; the first max_points written to pos / colr / nrm.  The number found is
+// pstart[num_blocks - 1] + bcount[num_blocks - 1] once the work is done.
+void launch_extract(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const void* vox,
+                    const uint32_t* col, float voxel, float min_sdf, float max_sdf, uint32_t min_weight,
+                    uint32_t max_points, const ExtractScratch& x, float4* pos, float4* colr, float4* nrm);
+
 // out[i] = cvt_i32(in[i]) for i < n (device buffers).
 void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n);
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,

@@ -1994,6 +1994,217 @@ __global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) 
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Surface-point extraction (RayCastRender.cu:629-907, RayCastRender.cpp:307-364)
+// ---------------------------------------------------------------------------------
+
+// Device-wide exclusive scan of n values (u8 or u32) into out, in chunks of
+// kScanChunk per 1024-thread workgroup: chunk sums, a scan of the sums by one
+// workgroup (1024 sums per pass), then the chunks again.
+constexpr uint32_t kScanChunk = kScanChunkSize;
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_sums(const T* __restrict__ in, uint32_t n, uint32_t* __restrict__ sums) {
+  const uint32_t base = blockIdx.x * kScanChunk;
+  uint32_t v = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanChunk / 1024u; ++k) {
+    const uint32_t i = base + k * 1024u + threadIdx.x;
+    v += i < n ? (uint32_t)in[i] : 0u;
+  }
+  uint32_t total;
+  (void)block_excl_scan<1024>(v, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* __restrict__ sums, uint32_t nchunks) {
+  uint32_t carry = 0;
+  for (uint32_t base = 0; base < nchunks; base += 1024u) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nchunks ? sums[i] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan<1024>(v, &total);
+    if (i < nchunks) sums[i] = ex + carry;
+    carry += total;
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_apply(const T* __restrict__ in, uint32_t n,
+                                                     const uint32_t* __restrict__ sums, uint32_t* __restrict__ out) {
+  // thread t holds elements base + 4t .. base + 4t + 3 (consecutive, in order)
+  const uint32_t base = blockIdx.x * kScanChunk + threadIdx.x * (kScanChunk / 1024u);
+  uint32_t v[kScanChunk / 1024u], run = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kScanChunk / 1024u; ++k) {
+    const uint32_t i = base + k;
+    v[k] = i < n ? (uint32_t)in[i] : 0u;
+    run += v[k];
+  }
+  uint32_t total;
+  uint32_t ex = block_excl_scan<1024>(run, &total) + sums[blockIdx.x];
+#pragma unroll
+  for (uint32_t k = 0; k < kScanChunk / 1024u; ++k) {
+    const uint32_t i = base + k;
+    if (i < n) out[i] = ex;
+    ex += v[k];
+  }
+}
+
+// Rank of every live block in hash-entry order (bucket, then slot): bucket b's entries
+// start at bucket_start[b] (the exclusive scan of the bucket counts).
+__global__ __launch_bounds__(256) void k_ext_order(TableView t, const DevState* __restrict__ st,
+                                                   const uint32_t* __restrict__ bucket_start,
+                                                   uint32_t* __restrict__ order) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= st->n_alloc) return;
+  const uint32_t e = (uint32_t)t.blk[p].w;
+  order[bucket_start[e / kBucket] + e % kBucket] = p;
+}
+
+struct ExtractArgs {
+  float voxel;
+  float min_sdf, max_sdf;
+  uint32_t min_weight;
+  uint32_t max_points;
+};
+
+// getVoxel (RayCastRender.cu:55-121) at world position p: weight and SDF, zero when the
+// block is not allocated.
+template <bool F16>
+__device__ __forceinline__ uint32_t ext_get_voxel(const TableView& t, const void* __restrict__ vox_, float px, float py,
+                                                  float pz, float v, float* sdf) {
+  using S = Store<F16>;
+  const int32_t vx = ray_voxel_of_q(px / v), vy = ray_voxel_of_q(py / v), vz = ray_voxel_of_q(pz / v);
+  const int32_t q = index_lookup(t, voxel_to_block_axis(vx), voxel_to_block_axis(vy), voxel_to_block_axis(vz));
+  if (q < 0) {
+    *sdf = 0.0f;
+    return 0u;
+  }
+  const size_t vi = (size_t)q * kVox + (size_t)(voxel_local_axis(vx) + voxel_local_axis(vy) * kBlock +
+                                                 voxel_local_axis(vz) * kBlock * kBlock);
+  const typename S::word w = reinterpret_cast<const typename S::word*>(vox_)[vi];
+  *sdf = S::sdf(w);
+  return S::weight(w);
+}
+
+// The surface point of voxel l of block (bx, by, bz) at pointer ptr, if it has one
+// (kernelExtractSurfacePoints, RayCastRender.cu:720-846, and computeNormalFromSDF,
+// :629-676).
+template <bool F16>
+__device__ __forceinline__ bool ext_point(const TableView& t, const void* __restrict__ vox_, const ExtractArgs& a,
+                                          uint32_t ptr, int4 b, int l, float3* iso, float3* nrm) {
+  using S = Store<F16>;
+  const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
+  const float v = a.voxel;
+  const int x = l % kBlock, y = (l / kBlock) % kBlock, z = l / (kBlock * kBlock);
+  const typename S::word w = vox[(size_t)ptr * kVox + (size_t)l];
+  const uint32_t wt = S::weight(w);
+  if (wt == 0u || wt < a.min_weight) return false;
+  const float sd = S::sdf(w);
+  if (sd < a.min_sdf || sd > a.max_sdf) return false;
+  const float bwx = (float)(b.x * kBlock) * v, bwy = (float)(b.y * kBlock) * v, bwz = (float)(b.z * kBlock) * v;
+  const float cx = bwx + ((float)x + 0.5f) * v, cy = bwy + ((float)y + 0.5f) * v, cz = bwz + ((float)z + 0.5f) * v;
+  float ax = 0.0f, ay = 0.0f, az = 0.0f;
+  int n = 0;
+#pragma unroll
+  for (int d = 0; d < 6; ++d) {
+    const int ox = d == 0 ? 1 : (d == 1 ? -1 : 0), oy = d == 2 ? 1 : (d == 3 ? -1 : 0), oz = d == 4 ? 1 : (d == 5 ? -1 : 0);
+    const float dx = (float)ox, dy = (float)oy, dz = (float)oz;
+    const int nx = x + ox, ny = y + oy, nz = z + oz;
+    float nsd;
+    if (nx < 0 || nx >= kBlock || ny < 0 || ny >= kBlock || nz < 0 || nz >= kBlock) {
+      if (ext_get_voxel<F16>(t, vox_, cx + dx * v, cy + dy * v, cz + dz * v, v, &nsd) == 0u) continue;
+    } else {
+      const typename S::word nw = vox[(size_t)ptr * kVox + (size_t)(nx + ny * kBlock + nz * kBlock * kBlock)];
+      if (S::weight(nw) == 0u) continue;
+      nsd = S::sdf(nw);
+    }
+    if ((sd < 0.0f && nsd > 0.0f) || (sd > 0.0f && nsd < 0.0f)) {
+      const float tt = sd / (sd - nsd);
+      ax = ax + (cx + dx * v * tt);
+      ay = ay + (cy + dy * v * tt);
+      az = az + (cz + dz * v * tt);
+      n++;
+    }
+  }
+  float px, py, pz;
+  if (n == 0) {
+    if (!(fabsf(sd) < 0.001f)) return false;
+    px = cx;
+    py = cy;
+    pz = cz;
+  } else {
+    px = ax / (float)n;
+    py = ay / (float)n;
+    pz = az / (float)n;
+  }
+  *iso = make_float3(px, py, pz);
+  // central differences of getVoxel at +-v/2 per axis
+  const float eps = v * 0.5f;
+  float g[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+  for (int ax_ = 0; ax_ < 3; ++ax_) {
+    float sp, sm;
+    const uint32_t wp = ext_get_voxel<F16>(t, vox_, ax_ == 0 ? px + eps : px, ax_ == 1 ? py + eps : py,
+                                           ax_ == 2 ? pz + eps : pz, v, &sp);
+    const uint32_t wm = ext_get_voxel<F16>(t, vox_, ax_ == 0 ? px - eps : px, ax_ == 1 ? py - eps : py,
+                                           ax_ == 2 ? pz - eps : pz, v, &sm);
+    if (wp > 0u && wm > 0u) g[ax_] = (sp - sm) / (2.0f * eps);
+  }
+  const float len = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+  if (len > 1e-6f) {
+    g[0] = g[0] / len;
+    g[1] = g[1] / len;
+    g[2] = g[2] / len;
+  }
+  *nrm = make_float3(g[0], g[1], g[2]);
+  return true;
+}
+
+// Points per block, blocks in hash order: one 512-thread workgroup per block.
+template <bool F16>
+__global__ __launch_bounds__(512) void k_ext_count(TableView t, const DevState* __restrict__ st,
+                                                   const uint32_t* __restrict__ order, const void* __restrict__ vox,
+                                                   ExtractArgs a, uint32_t* __restrict__ bcount) {
+  const uint32_t nb = st->n_alloc;
+  for (uint32_t k = blockIdx.x; k < nb; k += gridDim.x) {
+    const uint32_t ptr = order[k];
+    float3 iso, nrm;
+    const bool emit = ext_point<F16>(t, vox, a, ptr, t.blk[ptr], (int)threadIdx.x, &iso, &nrm);
+    uint32_t total;
+    (void)block_excl_scan<512>(emit ? 1u : 0u, &total);
+    if (threadIdx.x == 0) bcount[k] = total;
+  }
+  // no points beyond the live blocks (the scan runs over all block slots)
+  for (uint32_t k = nb + blockIdx.x * 512u + threadIdx.x; k < t.num_blocks; k += gridDim.x * 512u) bcount[k] = 0u;
+}
+
+// The points: block k's start pstart[k], then voxel order; the first max_points kept.
+template <bool F16>
+__global__ __launch_bounds__(512) void k_ext_write(TableView t, const DevState* __restrict__ st,
+                                                   const uint32_t* __restrict__ order, const void* __restrict__ vox,
+                                                   const uint32_t* __restrict__ col, ExtractArgs a,
+                                                   const uint32_t* __restrict__ pstart, float4* __restrict__ out_pos,
+                                                   float4* __restrict__ out_col, float4* __restrict__ out_nrm) {
+  const uint32_t nb = st->n_alloc;
+  for (uint32_t k = blockIdx.x; k < nb; k += gridDim.x) {
+    const uint32_t ptr = order[k];
+    float3 iso, nrm;
+    const bool emit = ext_point<F16>(t, vox, a, ptr, t.blk[ptr], (int)threadIdx.x, &iso, &nrm);
+    uint32_t total;
+    const uint32_t local = block_excl_scan<512>(emit ? 1u : 0u, &total);
+    const uint64_t idx = (uint64_t)pstart[k] + local;
+    if (emit && idx < a.max_points) {
+      const uint32_t c = col[(size_t)ptr * kVox + threadIdx.x];
+      out_pos[idx] = make_float4(iso.x, iso.y, iso.z, 1.0f);
+      out_col[idx] = make_float4((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f,
+                                 (float)((c >> 16) & 0xFFu) / 255.0f, 1.0f);
+      out_nrm[idx] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_convert(const float* __restrict__ in, int32_t* __restrict__ out, uint32_t n) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i < n) out[i] = cvt_i32(in[i]);
@@ -2098,6 +2309,34 @@ void launch_gn(hipStream_t s, const float* depth, const float* obs, const float*
 uint32_t tail_workgroups(int32_t W, int32_t H) {
   const uint64_t n = ((uint64_t)W * (uint64_t)H + 63u) / 64u;
   return (uint32_t)(n < 1u ? 1u : (n > 4096u ? 4096u : n));
+}
+
+template <class T> static void launch_scan(hipStream_t s, const T* in, uint32_t n, uint32_t* sums, uint32_t* out) {
+  if (n == 0u) return;
+  const uint32_t nchunks = (n + kScanChunk - 1u) / kScanChunk;
+  hipLaunchKernelGGL(k_scan_sums<T>, dim3(nchunks), dim3(1024), 0, s, in, n, sums);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, s, sums, nchunks);
+  hipLaunchKernelGGL(k_scan_apply<T>, dim3(nchunks), dim3(1024), 0, s, in, n, sums, out);
+}
+
+void launch_extract(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const void* vox,
+                    const uint32_t* col, float voxel, float min_sdf, float max_sdf, uint32_t min_weight,
+                    uint32_t max_points, const ExtractScratch& x, float4* pos, float4* colr, float4* nrm) {
+  // blocks in hash-entry order
+  launch_scan<uint8_t>(s, t.cnt, t.num_buckets, x.sums, x.bucket_start);
+  hipLaunchKernelGGL(k_ext_order, dim3((t.num_blocks + 255u) / 256u), dim3(256), 0, s, t, st, x.bucket_start,
+                     x.order);
+  const ExtractArgs a{voxel, min_sdf, max_sdf, min_weight, max_points};
+  const dim3 grid(t.num_blocks < 4096u ? t.num_blocks : 4096u);
+  if (fp16) hipLaunchKernelGGL(k_ext_count<true>, grid, dim3(512), 0, s, t, st, x.order, vox, a, x.bcount);
+  else hipLaunchKernelGGL(k_ext_count<false>, grid, dim3(512), 0, s, t, st, x.order, vox, a, x.bcount);
+  // block starts over all num_blocks counts (those beyond the live blocks are cleared)
+  launch_scan<uint32_t>(s, x.bcount, t.num_blocks, x.sums, x.pstart);
+  if (fp16)
+    hipLaunchKernelGGL(k_ext_write<true>, grid, dim3(512), 0, s, t, st, x.order, vox, col, a, x.pstart, pos, colr, nrm);
+  else
+    hipLaunchKernelGGL(k_ext_write<false>, grid, dim3(512), 0, s, t, st, x.order, vox, col, a, x.pstart, pos, colr,
+                       nrm);
 }
 
 void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n) {

@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "NOT_READY"}
 STORE_FP32 = 0
 STORE_FP16 = 1
 KERNELS = ["alloc_claim", "alloc_resolve", "alloc_scan", "alloc_insert", "compact", "integrate", "raycast",
-           "reset", "adjoint", "gauss_newton"]
+           "reset", "adjoint", "gauss_newton", "extract"]
 
 SDF_BLOCK_SIZE = 8      # VoxelSG/include/globalPramasConfig.h:8
 HASH_BUCKET_SIZE = 10   # VoxelSG/include/globalPramasConfig.h:9
@@ -97,6 +97,12 @@ _SIGNATURES = {
     "vsg_render_buffers": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4),
     "vsg_render_download": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "vsg_render_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "vsg_render_set_max_points": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "vsg_extract_surface_points": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_uint32,
+                                             C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)]),
+    "vsg_extracted_points": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 3 + [C.POINTER(C.c_uint32)]),
+    "vsg_extracted_points_download": (C.c_int, [C.c_void_p] + [C.c_void_p] * 3 + [C.c_uint32,
+                                                                                   C.POINTER(C.c_uint32)]),
     "vsg_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_pose_gn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
                                    C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -387,7 +393,10 @@ class RayCastRender:
         self.m_isAllocated = False
         self.m_width = 0
         self.m_height = 0
+        self.m_numExtractedPoints = 0
+        self.m_maxExtractedPoints = 10000000
         self._h = C.c_void_p()
+        self._ext_h = C.c_void_p()
 
     def initialize(self, width: int, height: int) -> bool:
         """RayCastRender::initialize (VoxelSG/src/RayCastRender.cpp:88-118)."""
@@ -401,6 +410,7 @@ class RayCastRender:
                   + lib().vsg_last_error().decode(errors="replace"), file=sys.stderr)
             return False
         self._h = h
+        _check(lib().vsg_render_set_max_points(h, self.m_maxExtractedPoints))
         self.m_width, self.m_height = int(width), int(height)
         self.m_isAllocated = True
         return True
@@ -410,6 +420,9 @@ class RayCastRender:
             _check(lib().vsg_render_destroy(self._h))
             self._h = C.c_void_p()
             self.m_isAllocated = False
+        if self._ext_h.value:
+            _check(lib().vsg_render_destroy(self._ext_h))
+            self._ext_h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -482,6 +495,91 @@ class RayCastRender:
             fh.write("property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n")
             for p, c in zip(pts, cols):
                 fh.write(f"{p[0]} {p[1]} {p[2]} {c[0]} {c[1]} {c[2]}\n")
+        return True
+
+    def set_max_points(self, max_points: int) -> None:
+        """Capacity of the surface-point extraction (m_maxExtractedPoints, RayCastRender.cpp:62)."""
+        self.m_maxExtractedPoints = int(max_points)
+        for h in (self._h, self._ext_h):
+            if h:
+                _check(lib().vsg_render_set_max_points(h, self.m_maxExtractedPoints))
+
+    def _extractor(self) -> C.c_void_p:
+        """The renderer extractions run on: the initialised one, else a 1x1 one of its own
+        (the reference extracts without initialize())."""
+        if self.m_isAllocated:
+            return self._h
+        if not self._ext_h:
+            _check(lib().vsg_render_create(1, 1, C.byref(self._ext_h)))
+            _check(lib().vsg_render_set_max_points(self._ext_h, self.m_maxExtractedPoints))
+        return self._ext_h
+
+    def extractSurfacePoints(self, voxelScene: VoxelScene, minSDF: float = -0.01, maxSDF: float = 0.01,
+                             minWeight: int = 1) -> int:
+        """RayCastRender::extractSurfacePoints (VoxelSG/src/RayCastRender.cpp:307-364): one
+        point per voxel of weight >= max(1, minWeight) and SDF in [minSDF, maxSDF] that has a
+        zero crossing towards a neighbour (or |sdf| < 0.001), with its normal and colour, in
+        hash-entry order then voxel order.  Returns the number of points found, as the
+        reference's counter does; the first getMaxPoints() of them are kept
+        (m_numExtractedPoints)."""
+        if voxelScene is None:
+            print("RayCastRender::extractSurfacePoints: Invalid voxelScene!", file=sys.stderr)
+            return 0
+        stored, total = C.c_uint32(), C.c_uint64()
+        _check(lib().vsg_extract_surface_points(self._extractor(), voxelScene.handle, float(minSDF), float(maxSDF),
+                                                int(minWeight), C.byref(stored), C.byref(total)))
+        self.m_numExtractedPoints = stored.value
+        return total.value
+
+    def extractedBuffers(self) -> dict:
+        """Device pointers of the extracted points (float4 positions, colours, normals) and
+        the number stored."""
+        ptrs = [C.c_void_p() for _ in range(3)]
+        n = C.c_uint32()
+        h = self._h if self.m_isAllocated else self._ext_h
+        if not h:
+            return {"pos": 0, "col": 0, "nrm": 0, "count": 0}
+        _check(lib().vsg_extracted_points(h, *[C.byref(x) for x in ptrs], C.byref(n)))
+        return {"pos": ptrs[0].value or 0, "col": ptrs[1].value or 0, "nrm": ptrs[2].value or 0, "count": n.value}
+
+    def downloadExtractedPoints(self) -> dict:
+        """The stored points of the last extraction: pos, col, nrm as float32 [n, 4]."""
+        n = self.extractedBuffers()["count"]
+        out = {k: np.zeros((n, 4), np.float32) for k in ("pos", "col", "nrm")}
+        got = C.c_uint32()
+        _check(lib().vsg_extracted_points_download(self._h if self.m_isAllocated else self._ext_h, out["pos"].ctypes.data, out["col"].ctypes.data,
+                                                   out["nrm"].ctypes.data, n, C.byref(got)))
+        assert got.value == n
+        return out
+
+    def saveExtractedSurfacePointsPLY(self, filePath: str) -> bool:
+        """RayCastRender::saveExtractedSurfacePointsPLY (VoxelSG/src/RayCastRender.cpp:369-446):
+        ASCII PLY of the extracted points with finite positions: x y z nx ny nz red green blue,
+        colour channels round(clamp(c, 0, 1) * 255) (255 when not finite)."""
+        n = self.extractedBuffers()["count"]
+        if n == 0:
+            print("RayCastRender::saveExtractedSurfacePointsPLY: No points to save!", file=sys.stderr)
+            return False
+        r = self.downloadExtractedPoints()
+        ok = np.isfinite(r["pos"][:, :3]).all(axis=1)
+        d = os.path.dirname(os.path.abspath(filePath))
+        os.makedirs(d, exist_ok=True)
+
+        def chan(v):
+            if not np.isfinite(v):
+                return 255
+            c = np.float32(min(max(np.float32(v), np.float32(0.0)), np.float32(1.0)))
+            return int(np.float32(c * np.float32(255.0)) + np.float32(0.5))
+
+        with open(filePath, "w") as fh:
+            fh.write("ply\nformat ascii 1.0\ncomment Generated by RayCastRender::extractSurfacePoints\n")
+            fh.write(f"element vertex {int(ok.sum())}\n")
+            fh.write("property float x\nproperty float y\nproperty float z\n")
+            fh.write("property float nx\nproperty float ny\nproperty float nz\n")
+            fh.write("property uchar red\nproperty uchar green\nproperty uchar blue\nend_header\n")
+            for p, nn, c in zip(r["pos"][ok], r["nrm"][ok], r["col"][ok]):
+                fh.write(f"{p[0]:g} {p[1]:g} {p[2]:g} {nn[0]:g} {nn[1]:g} {nn[2]:g} "
+                         f"{chan(c[0])} {chan(c[1])} {chan(c[2])}\n")
         return True
 
     def set_sampling(self, mode: str) -> None:
