@@ -174,6 +174,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--precision-frames", type=int, default=30)
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-extract", action="store_true",
+                    help="skip timing the surface-point extraction of the fused scene")
     ap.add_argument("--step-budget", type=int, default=None,
                     help="raycast steps per lane before the cooperative tail pass (library default if unset)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -273,6 +275,25 @@ def main() -> int:
         torch.cuda.synchronize()
         t_total = time.perf_counter() - t_start
         ktimes = vsg.timing_read()
+        ext = None
+        if not args.no_extract:
+            # surface-point extraction of the fused scene (reference defaults: SDF in
+            # [-0.01, 0.01], weight >= 1), outside the timed region
+            n_alloc = scene.counters()["num_allocated"]
+            rr.extractSurfacePoints(scene)  # first call allocates the outputs
+            torch.cuda.synchronize()
+            vsg.timing_reset()
+            reps = 10
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                npts = rr.extractSurfacePoints(scene)
+            wall = (time.perf_counter() - t0) / reps
+            ems, en = vsg.timing_read()["extract"]
+            ebytes = n_alloc * 512 * b_sw + min(npts, rr.m_maxExtractedPoints) * (4 + 48)
+            egbs = ebytes / (ems / en * 1e-3) / 1e9
+            ext = {"points": npts, "blocks": n_alloc, "avg_ms": round(ems / en, 4), "wall_ms": round(wall * 1e3, 4),
+                   "algorithmic_bytes": int(ebytes), "gbs": round(egbs, 1), "frac": round(egbs / HBM_PEAK_GBS, 5)}
+            log(f"[{store}] extraction: {npts} points from {n_alloc} blocks in {ems / en:.3f} ms")
         vsg.timing_enable(False)
         scene.close()
         rr.close()
@@ -283,7 +304,7 @@ def main() -> int:
                 kb[k] += v
             fb += frame_bytes(st, npix, b_sw)
         log(f"[{store}] {t_total:.3f} s for {args.steps * nf} frames")
-        return {"t_total": t_total, "ktimes": ktimes, "kbytes": kb, "fbytes": fb, "stats": stats}
+        return {"t_total": t_total, "ktimes": ktimes, "kbytes": kb, "fbytes": fb, "stats": stats, "extract": ext}
 
     res = measure(args.store)
     t_total, ktimes = res["t_total"], res["ktimes"]
@@ -310,7 +331,7 @@ def main() -> int:
     fused_gbs = res["fbytes"] * args.steps / t_total / 1e9
     per_kernel = {}
     for k, (ms, n) in ktimes.items():
-        if n == 0:
+        if n == 0 or k == "extract":
             continue
         e = {"avg_ms": round(ms / n, 4), "launches": n, "share": round(ms / sum(kern_ms.values()), 4)}
         if k in res["kbytes"]:
@@ -332,11 +353,15 @@ def main() -> int:
                           for k in ("march_evals", "march_super_skips", "march_block_skips", "march_replayed",
                                     "march_cell_skips", "march_deferred", "march_tail_windows", "march_tail_skips")},
     }
+    if res["extract"] is not None:
+        extra["surface_points"] = res["extract"]
     if args.compare_fp16 and args.store == "fp32":
         r16 = measure("fp16")
         extra["fp16_store"] = {"value": round(frames_done / r16["t_total"], 3),
                                "speedup_vs_fp32": round(t_total / r16["t_total"], 4),
                                "ms_per_frame": round(r16["t_total"] / frames_done * 1e3, 4)}
+        if r16["extract"] is not None:
+            extra["fp16_store"]["surface_points"] = r16["extract"]
     if not args.no_precision:
         log(f"precision: FP32 and FP16 stores vs the FP64 oracle on {args.precision_frames} frames")
         extra["precision_vs_fp64"] = precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm,

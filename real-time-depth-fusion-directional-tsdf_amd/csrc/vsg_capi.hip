@@ -213,7 +213,9 @@ void free_extract_outputs(vsg_render* r) {
 
 void free_extract_scratch(vsg_render* r) {
   (void)hipFree(r->ext.bucket_start);
-  (void)hipFree(r->ext.order);
+  (void)hipFree(r->ext.oblk);
+  (void)hipFree(r->ext.nbp);
+  (void)hipFree(r->ext.ebits);
   (void)hipFree(r->ext.bcount);
   (void)hipFree(r->ext.pstart);
   (void)hipFree(r->ext.sums);
@@ -816,7 +818,10 @@ vsg_status vsg_extract_surface_points(vsg_render* r, vsg_scene* s, float min_sdf
     free_extract_scratch(r);
     const uint32_t sums = extract_sums_words(B > N ? B : N);
     vsg_status st = dmalloc(&r->ext.bucket_start, (size_t)B * sizeof(uint32_t), "extraction bucket starts");
-    if (st == VSG_OK) st = dmalloc(&r->ext.order, (size_t)N * sizeof(uint32_t), "extraction block order");
+    if (st == VSG_OK) st = dmalloc(&r->ext.oblk, (size_t)N * sizeof(int4), "extraction block order");
+    if (st == VSG_OK)
+      st = dmalloc(&r->ext.nbp, (size_t)N * kExtNbWords * sizeof(int32_t), "extraction neighbour pointers");
+    if (st == VSG_OK) st = dmalloc(&r->ext.ebits, (size_t)N * 8 * sizeof(uint64_t), "extraction emit bits");
     if (st == VSG_OK) st = dmalloc(&r->ext.bcount, (size_t)N * sizeof(uint32_t), "extraction block counts");
     if (st == VSG_OK) st = dmalloc(&r->ext.pstart, (size_t)N * sizeof(uint32_t), "extraction block starts");
     if (st == VSG_OK) st = dmalloc(&r->ext.sums, (size_t)sums * sizeof(uint32_t), "extraction scan sums");

@@ -68,15 +68,18 @@ void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st
                       const float* pts, const uint8_t* rgb, const float* nrm, const uint32_t* vis, void* vox,
                       uint32_t* col, uint32_t* dmeta, float2* dval, uint32_t grid, uint32_t* upd_part);
 // Scratch of the surface-point extraction (device buffers): bucket_start[num_buckets],
-// order, bcount, pstart[num_blocks], sums[extract_sums_words(max(num_buckets,
-// num_blocks))].
+// oblk, bcount, pstart[num_blocks], nbp[kExtNbWords * num_blocks], ebits[8 * num_blocks],
+// sums[extract_sums_words(max(num_buckets, num_blocks))].
 struct ExtractScratch {
   uint32_t* bucket_start;
-  uint32_t* order;
+  int4* oblk;
+  int32_t* nbp;
+  uint64_t* ebits;
   uint32_t* bcount;
   uint32_t* pstart;
   uint32_t* sums;
 };
+constexpr uint32_t kExtNbWords = 32;
 constexpr uint32_t kScanChunkSize = 4096;
 inline uint32_t extract_sums_words(uint32_t n) { return (n + kScanChunkSize - 1u) / kScanChunkSize + 1u; }
 

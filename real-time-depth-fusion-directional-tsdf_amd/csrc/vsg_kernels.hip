@@ -2051,15 +2051,30 @@ __global__ __launch_bounds__(1024) void k_scan_apply(const T* __restrict__ in, u
   }
 }
 
-// Rank of every live block in hash-entry order (bucket, then slot): bucket b's entries
-// start at bucket_start[b] (the exclusive scan of the bucket counts).
+// Every live block at its rank in hash-entry order (bucket, then slot): bucket b's
+// entries start at bucket_start[b] (the exclusive scan of the bucket counts).  oblk[k]
+// = (block coordinates, pointer) of rank k.
 __global__ __launch_bounds__(256) void k_ext_order(TableView t, const DevState* __restrict__ st,
-                                                   const uint32_t* __restrict__ bucket_start,
-                                                   uint32_t* __restrict__ order) {
+                                                   const uint32_t* __restrict__ bucket_start, int4* __restrict__ oblk) {
   const uint32_t p = blockIdx.x * 256u + threadIdx.x;
   if (p >= st->n_alloc) return;
-  const uint32_t e = (uint32_t)t.blk[p].w;
-  order[bucket_start[e / kBucket] + e % kBucket] = p;
+  const int4 b = t.blk[p];
+  const uint32_t e = (uint32_t)b.w;
+  oblk[bucket_start[e / kBucket] + e % kBucket] = make_int4(b.x, b.y, b.z, (int32_t)p);
+}
+
+// Pointers of the 3x3x3 blocks around every live block (-1 where none is allocated),
+// kExtNb words per block: nbp[kExtNb * k + i] for neighbour i = (dx + 1) + 3 (dy + 1)
+// + 9 (dz + 1) of rank k.
+constexpr uint32_t kExtNb = kExtNbWords;
+__global__ __launch_bounds__(256) void k_ext_neighbours(TableView t, const DevState* __restrict__ st,
+                                                        const int4* __restrict__ oblk, int32_t* __restrict__ nbp) {
+  const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t k = g / kExtNb, i = g % kExtNb;
+  if (k >= st->n_alloc || i >= 27u) return;
+  const int4 b = oblk[k];
+  const int dx = (int)(i % 3u) - 1, dy = (int)((i / 3u) % 3u) - 1, dz = (int)(i / 9u) - 1;
+  nbp[(size_t)k * kExtNb + i] = (dx | dy | dz) == 0 ? b.w : index_lookup(t, b.x + dx, b.y + dy, b.z + dz);
 }
 
 struct ExtractArgs {
@@ -2069,87 +2084,175 @@ struct ExtractArgs {
   uint32_t max_points;
 };
 
+// One block of the extraction in LDS (one wave per block): its 512 voxel words, the
+// pointers of the 3x3x3 blocks around it (-1 where none is allocated), and per axis the
+// voxel getVoxel reads at (corner + (i + 0.5) v + o v) for local coordinate i and
+// offset o in {-1, 0, +1}: (block offset + 1) * 8 + local coordinate, kExtFar when it
+// lies beyond the neighbouring blocks.  getVoxel's rounding of one coordinate depends
+// on that coordinate alone, so a neighbour's voxel follows from three table entries.
+constexpr uint8_t kExtFar = 255;
+template <bool F16> struct ExtBlock {
+  typename Store<F16>::word sv[kVox];
+  int32_t nbp[27];
+  uint8_t ax[3][kBlock][3];
+  uint16_t list[kVox];  // compacted voxel indices
+  uint64_t em[8];       // emitting voxels of the block, one bit per voxel
+};
+
 // getVoxel (RayCastRender.cu:55-121) at world position p: weight and SDF, zero when the
-// block is not allocated.
+// block is not allocated.  Blocks next to the staged one come from its pointer table,
+// the staged block from LDS, others from the block index.
 template <bool F16>
-__device__ __forceinline__ uint32_t ext_get_voxel(const TableView& t, const void* __restrict__ vox_, float px, float py,
-                                                  float pz, float v, float* sdf) {
+__device__ __forceinline__ uint32_t ext_get_voxel(const TableView& t, const void* __restrict__ vox_,
+                                                  const ExtBlock<F16>& eb, int4 b, float px, float py, float pz,
+                                                  float v, float* sdf) {
   using S = Store<F16>;
   const int32_t vx = ray_voxel_of_q(px / v), vy = ray_voxel_of_q(py / v), vz = ray_voxel_of_q(pz / v);
-  const int32_t q = index_lookup(t, voxel_to_block_axis(vx), voxel_to_block_axis(vy), voxel_to_block_axis(vz));
-  if (q < 0) {
-    *sdf = 0.0f;
-    return 0u;
+  const int32_t bx = voxel_to_block_axis(vx), by = voxel_to_block_axis(vy), bz = voxel_to_block_axis(vz);
+  const int l = voxel_local_axis(vx) + voxel_local_axis(vy) * kBlock + voxel_local_axis(vz) * kBlock * kBlock;
+  const int32_t dx = bx - b.x, dy = by - b.y, dz = bz - b.z;
+  typename S::word w;
+  if ((dx | dy | dz) == 0) {
+    w = eb.sv[l];
+  } else {
+    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+    const int32_t q = near ? eb.nbp[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] : index_lookup(t, bx, by, bz);
+    if (q < 0) {
+      *sdf = 0.0f;
+      return 0u;
+    }
+    w = reinterpret_cast<const typename S::word*>(vox_)[(size_t)q * kVox + (size_t)l];
   }
-  const size_t vi = (size_t)q * kVox + (size_t)(voxel_local_axis(vx) + voxel_local_axis(vy) * kBlock +
-                                                 voxel_local_axis(vz) * kBlock * kBlock);
-  const typename S::word w = reinterpret_cast<const typename S::word*>(vox_)[vi];
   *sdf = S::sdf(w);
   return S::weight(w);
 }
 
-// The surface point of voxel l of block (bx, by, bz) at pointer ptr, if it has one
-// (kernelExtractSurfacePoints, RayCastRender.cu:720-846, and computeNormalFromSDF,
-// :629-676).
+// Voxel centre coordinate along one axis (block coordinate bc, local i).
+__device__ __forceinline__ float ext_centre(int32_t bc, int i, float v) {
+  return (float)(bc * kBlock) * v + ((float)i + 0.5f) * v;
+}
+
+// Fills eb.ax for block b (lanes of one wave; the caller synchronises).
+template <bool F16> __device__ __forceinline__ void ext_axis_table(ExtBlock<F16>& eb, int4 b, float v) {
+  for (uint32_t j = threadIdx.x; j < 72u; j += 64u) {
+    const int a = (int)(j / 24u), i = (int)((j / 3u) % 8u), o = (int)(j % 3u) - 1;
+    const int32_t bc = a == 0 ? b.x : (a == 1 ? b.y : b.z);
+    const float p = ext_centre(bc, i, v) + (float)o * v;
+    const int32_t vi = ray_voxel_of_q(p / v);
+    const int32_t db = voxel_to_block_axis(vi) - bc;
+    eb.ax[a][i][o + 1] = (db >= -1 && db <= 1) ? (uint8_t)((db + 1) * kBlock + voxel_local_axis(vi)) : kExtFar;
+  }
+}
+
+// The neighbour of voxel (x, y, z) at offset (ox, oy, oz) (one axis +-1) as the
+// reference reads it (RayCastRender.cu:766-800): by index inside the block, through
+// getVoxel at the voxel centre + offset * v outside it.  Returns its weight.
 template <bool F16>
-__device__ __forceinline__ bool ext_point(const TableView& t, const void* __restrict__ vox_, const ExtractArgs& a,
-                                          uint32_t ptr, int4 b, int l, float3* iso, float3* nrm) {
+__device__ __forceinline__ uint32_t ext_neighbour(const TableView& t, const void* __restrict__ vox_,
+                                                  const ExtBlock<F16>& eb, int4 b, float v, int x, int y, int z,
+                                                  int ox, int oy, int oz, float* nsd) {
   using S = Store<F16>;
-  const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
-  const float v = a.voxel;
-  const int x = l % kBlock, y = (l / kBlock) % kBlock, z = l / (kBlock * kBlock);
-  const typename S::word w = vox[(size_t)ptr * kVox + (size_t)l];
+  const int nx = x + ox, ny = y + oy, nz = z + oz;
+  typename S::word w;
+  if (nx >= 0 && nx < kBlock && ny >= 0 && ny < kBlock && nz >= 0 && nz < kBlock) {
+    w = eb.sv[nx + ny * kBlock + nz * kBlock * kBlock];
+  } else {
+    const uint32_t ex = eb.ax[0][x][ox + 1], ey = eb.ax[1][y][oy + 1], ez = eb.ax[2][z][oz + 1];
+    if (ex == kExtFar || ey == kExtFar || ez == kExtFar)
+      return ext_get_voxel<F16>(t, vox_, eb, b, ext_centre(b.x, x, v) + (float)ox * v,
+                                ext_centre(b.y, y, v) + (float)oy * v, ext_centre(b.z, z, v) + (float)oz * v, v, nsd);
+    const int dx = (int)(ex / kBlock) - 1, dy = (int)(ey / kBlock) - 1, dz = (int)(ez / kBlock) - 1;
+    const int l = (int)(ex % kBlock) + (int)(ey % kBlock) * kBlock + (int)(ez % kBlock) * kBlock * kBlock;
+    if ((dx | dy | dz) == 0) {
+      w = eb.sv[l];
+    } else {
+      const int32_t q = eb.nbp[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
+      if (q < 0) {
+        *nsd = 0.0f;
+        return 0u;
+      }
+      w = reinterpret_cast<const typename S::word*>(vox_)[(size_t)q * kVox + (size_t)l];
+    }
+  }
+  *nsd = S::sdf(w);
+  return S::weight(w);
+}
+
+__device__ __forceinline__ void ext_dir(int d, int* ox, int* oy, int* oz) {
+  *ox = d == 0 ? 1 : (d == 1 ? -1 : 0);
+  *oy = d == 2 ? 1 : (d == 3 ? -1 : 0);
+  *oz = d == 4 ? 1 : (d == 5 ? -1 : 0);
+}
+
+// Voxel passes the weight and SDF thresholds (RayCastRender.cu:738-752).
+template <bool F16> __device__ __forceinline__ bool ext_in_band(typename Store<F16>::word w, const ExtractArgs& a) {
+  using S = Store<F16>;
   const uint32_t wt = S::weight(w);
-  if (wt == 0u || wt < a.min_weight) return false;
   const float sd = S::sdf(w);
-  if (sd < a.min_sdf || sd > a.max_sdf) return false;
-  const float bwx = (float)(b.x * kBlock) * v, bwy = (float)(b.y * kBlock) * v, bwz = (float)(b.z * kBlock) * v;
-  const float cx = bwx + ((float)x + 0.5f) * v, cy = bwy + ((float)y + 0.5f) * v, cz = bwz + ((float)z + 0.5f) * v;
+  return wt != 0u && wt >= a.min_weight && !(sd < a.min_sdf) && !(sd > a.max_sdf);
+}
+
+// Whether in-band voxel l of the staged block emits a point: a weighted neighbour of
+// the opposite sign, or |sdf| < 0.001 (RayCastRender.cu:760-834).
+template <bool F16>
+__device__ __forceinline__ bool ext_emits(const TableView& t, const void* __restrict__ vox_, const ExtBlock<F16>& eb,
+                                          int4 b, float v, int l) {
+  using S = Store<F16>;
+  const int x = l % kBlock, y = (l / kBlock) % kBlock, z = l / (kBlock * kBlock);
+  const float sd = S::sdf(eb.sv[l]);
+  for (int d = 0; d < 6; ++d) {
+    int ox, oy, oz;
+    ext_dir(d, &ox, &oy, &oz);
+    float nsd;
+    if (ext_neighbour<F16>(t, vox_, eb, b, v, x, y, z, ox, oy, oz, &nsd) == 0u) continue;
+    if ((sd < 0.0f && nsd > 0.0f) || (sd > 0.0f && nsd < 0.0f)) return true;
+  }
+  return fabsf(sd) < 0.001f;
+}
+
+// The point of emitting voxel l of the staged block: the mean of its zero crossings,
+// else its centre (RayCastRender.cu:760-840).
+template <bool F16>
+__device__ __forceinline__ float3 ext_iso(const TableView& t, const void* __restrict__ vox_, const ExtBlock<F16>& eb,
+                                          int4 b, float v, int l) {
+  using S = Store<F16>;
+  const int x = l % kBlock, y = (l / kBlock) % kBlock, z = l / (kBlock * kBlock);
+  const float sd = S::sdf(eb.sv[l]);
+  const float cx = ext_centre(b.x, x, v), cy = ext_centre(b.y, y, v), cz = ext_centre(b.z, z, v);
   float ax = 0.0f, ay = 0.0f, az = 0.0f;
   int n = 0;
-#pragma unroll
   for (int d = 0; d < 6; ++d) {
-    const int ox = d == 0 ? 1 : (d == 1 ? -1 : 0), oy = d == 2 ? 1 : (d == 3 ? -1 : 0), oz = d == 4 ? 1 : (d == 5 ? -1 : 0);
-    const float dx = (float)ox, dy = (float)oy, dz = (float)oz;
-    const int nx = x + ox, ny = y + oy, nz = z + oz;
+    int ox, oy, oz;
+    ext_dir(d, &ox, &oy, &oz);
     float nsd;
-    if (nx < 0 || nx >= kBlock || ny < 0 || ny >= kBlock || nz < 0 || nz >= kBlock) {
-      if (ext_get_voxel<F16>(t, vox_, cx + dx * v, cy + dy * v, cz + dz * v, v, &nsd) == 0u) continue;
-    } else {
-      const typename S::word nw = vox[(size_t)ptr * kVox + (size_t)(nx + ny * kBlock + nz * kBlock * kBlock)];
-      if (S::weight(nw) == 0u) continue;
-      nsd = S::sdf(nw);
-    }
+    if (ext_neighbour<F16>(t, vox_, eb, b, v, x, y, z, ox, oy, oz, &nsd) == 0u) continue;
     if ((sd < 0.0f && nsd > 0.0f) || (sd > 0.0f && nsd < 0.0f)) {
       const float tt = sd / (sd - nsd);
-      ax = ax + (cx + dx * v * tt);
-      ay = ay + (cy + dy * v * tt);
-      az = az + (cz + dz * v * tt);
+      ax = ax + (cx + (float)ox * v * tt);
+      ay = ay + (cy + (float)oy * v * tt);
+      az = az + (cz + (float)oz * v * tt);
       n++;
     }
   }
-  float px, py, pz;
-  if (n == 0) {
-    if (!(fabsf(sd) < 0.001f)) return false;
-    px = cx;
-    py = cy;
-    pz = cz;
-  } else {
-    px = ax / (float)n;
-    py = ay / (float)n;
-    pz = az / (float)n;
-  }
-  *iso = make_float3(px, py, pz);
-  // central differences of getVoxel at +-v/2 per axis
+  if (n == 0) return make_float3(cx, cy, cz);
+  return make_float3(ax / (float)n, ay / (float)n, az / (float)n);
+}
+
+// computeNormalFromSDF (RayCastRender.cu:629-676): central differences of getVoxel at
+// +-v/2 per axis (0 for an axis whose two voxels are not both weighted), normalised
+// when the length exceeds 1e-6.
+template <bool F16>
+__device__ __forceinline__ float3 ext_normal(const TableView& t, const void* __restrict__ vox_,
+                                             const ExtBlock<F16>& eb, int4 b, float3 p, float v) {
   const float eps = v * 0.5f;
   float g[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int ax_ = 0; ax_ < 3; ++ax_) {
     float sp, sm;
-    const uint32_t wp = ext_get_voxel<F16>(t, vox_, ax_ == 0 ? px + eps : px, ax_ == 1 ? py + eps : py,
-                                           ax_ == 2 ? pz + eps : pz, v, &sp);
-    const uint32_t wm = ext_get_voxel<F16>(t, vox_, ax_ == 0 ? px - eps : px, ax_ == 1 ? py - eps : py,
-                                           ax_ == 2 ? pz - eps : pz, v, &sm);
+    const uint32_t wp = ext_get_voxel<F16>(t, vox_, eb, b, ax_ == 0 ? p.x + eps : p.x, ax_ == 1 ? p.y + eps : p.y,
+                                           ax_ == 2 ? p.z + eps : p.z, v, &sp);
+    const uint32_t wm = ext_get_voxel<F16>(t, vox_, eb, b, ax_ == 0 ? p.x - eps : p.x, ax_ == 1 ? p.y - eps : p.y,
+                                           ax_ == 2 ? p.z - eps : p.z, v, &sm);
     if (wp > 0u && wm > 0u) g[ax_] = (sp - sm) / (2.0f * eps);
   }
   const float len = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
@@ -2158,50 +2261,174 @@ __device__ __forceinline__ bool ext_point(const TableView& t, const void* __rest
     g[1] = g[1] / len;
     g[2] = g[2] / len;
   }
-  *nrm = make_float3(g[0], g[1], g[2]);
-  return true;
+  return make_float3(g[0], g[1], g[2]);
 }
 
-// Points per block, blocks in hash order: one 512-thread workgroup per block.
+// The extraction's walk over the live blocks in hash order, one wave per block: lane
+// L holds voxels L + 64 z (z = 0..7).  Workgroup g takes ranks g, g + G, ...; the next
+// block's voxel words and neighbour pointers are loaded while the current block is
+// processed, and the list entry of the block after that one.
+template <bool F16> struct ExtWalk {
+  using W = typename Store<F16>::word;
+  const W* vw;
+  const int4* oblk;
+  const int32_t* nbp;
+  uint32_t nb;
+  int4 b;       // current block (coordinates, pointer)
+  int4 bn;      // next block's list entry
+  W w[8];       // this lane's voxel words of the current block
+  int32_t nq;   // neighbour pointer (lanes < 27)
+
+  __device__ void fetch(uint32_t k, int4 bk) {
+    const size_t base = (size_t)bk.w * kVox + threadIdx.x;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) w[z] = vw[base + 64u * (uint32_t)z];
+    nq = threadIdx.x < 27u ? nbp[(size_t)k * kExtNb + threadIdx.x] : -1;
+  }
+  __device__ void stage(ExtBlock<F16>& eb, float v) const {
+#pragma unroll
+    for (int z = 0; z < 8; ++z) eb.sv[threadIdx.x + 64u * (uint32_t)z] = w[z];
+    if (threadIdx.x < 27u) eb.nbp[threadIdx.x] = nq;
+    ext_axis_table<F16>(eb, b, v);
+  }
+};
+
+// Points per block (bcount) and the emitting voxels (ebits, 8 words per block).
 template <bool F16>
-__global__ __launch_bounds__(512) void k_ext_count(TableView t, const DevState* __restrict__ st,
-                                                   const uint32_t* __restrict__ order, const void* __restrict__ vox,
-                                                   ExtractArgs a, uint32_t* __restrict__ bcount) {
-  const uint32_t nb = st->n_alloc;
-  for (uint32_t k = blockIdx.x; k < nb; k += gridDim.x) {
-    const uint32_t ptr = order[k];
-    float3 iso, nrm;
-    const bool emit = ext_point<F16>(t, vox, a, ptr, t.blk[ptr], (int)threadIdx.x, &iso, &nrm);
-    uint32_t total;
-    (void)block_excl_scan<512>(emit ? 1u : 0u, &total);
-    if (threadIdx.x == 0) bcount[k] = total;
+__global__ __launch_bounds__(64) void k_ext_count(TableView t, const DevState* __restrict__ st,
+                                                  const int4* __restrict__ oblk, const int32_t* __restrict__ nbp,
+                                                  const void* __restrict__ vox, ExtractArgs a,
+                                                  uint32_t* __restrict__ bcount, uint64_t* __restrict__ ebits) {
+  using S = Store<F16>;
+  __shared__ ExtBlock<F16> eb;
+  ExtWalk<F16> wk;
+  wk.vw = reinterpret_cast<const typename S::word*>(vox);
+  wk.oblk = oblk;
+  wk.nbp = nbp;
+  wk.nb = st->n_alloc;
+  const uint32_t G = gridDim.x, lane = threadIdx.x;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t k = blockIdx.x;
+  if (k < wk.nb) {
+    wk.b = oblk[k];
+    wk.fetch(k, wk.b);
+    wk.bn = k + G < wk.nb ? oblk[k + G] : make_int4(0, 0, 0, 0);
+  }
+  for (; k < wk.nb; k += G) {
+    // in-band voxels of the current block, in voxel order, before its words are staged
+    uint32_t nband = 0;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const bool in = ext_in_band<F16>(wk.w[z], a);
+      const uint64_t m = __ballot(in);
+      if (in) eb.list[nband + (uint32_t)__popcll(m & below)] = (uint16_t)(lane + 64u * (uint32_t)z);
+      nband += (uint32_t)__popcll(m);
+    }
+    wk.stage(eb, a.voxel);
+    if (lane < 8u) eb.em[lane] = 0ull;
+    __syncthreads();
+    const int4 b = wk.b;
+    if (k + G < wk.nb) wk.fetch(k + G, wk.bn);
+    wk.b = wk.bn;
+    wk.bn = k + 2u * G < wk.nb ? oblk[k + 2u * G] : make_int4(0, 0, 0, 0);
+    uint32_t c = 0;
+    for (uint32_t j0 = 0; j0 < nband; j0 += 64u) {
+      const uint32_t j = j0 + lane;
+      if (j < nband) {
+        const int l = eb.list[j];
+        if (ext_emits<F16>(t, vox, eb, b, a.voxel, l)) {
+          c++;
+          atomicOr(reinterpret_cast<unsigned long long*>(&eb.em[l >> 6]), 1ull << (l & 63));
+        }
+      }
+    }
+    c = wave_sum(c);
+    __syncthreads();
+    if (lane == 0) bcount[k] = c;
+    if (lane < 8u) ebits[(size_t)k * 8u + lane] = eb.em[lane];
+    __syncthreads();  // eb is read above until every lane is done
   }
   // no points beyond the live blocks (the scan runs over all block slots)
-  for (uint32_t k = nb + blockIdx.x * 512u + threadIdx.x; k < t.num_blocks; k += gridDim.x * 512u) bcount[k] = 0u;
+  for (uint32_t j = wk.nb + blockIdx.x * 64u + lane; j < t.num_blocks; j += G * 64u) bcount[j] = 0u;
 }
 
 // The points: block k's start pstart[k], then voxel order; the first max_points kept.
 template <bool F16>
-__global__ __launch_bounds__(512) void k_ext_write(TableView t, const DevState* __restrict__ st,
-                                                   const uint32_t* __restrict__ order, const void* __restrict__ vox,
-                                                   const uint32_t* __restrict__ col, ExtractArgs a,
-                                                   const uint32_t* __restrict__ pstart, float4* __restrict__ out_pos,
-                                                   float4* __restrict__ out_col, float4* __restrict__ out_nrm) {
-  const uint32_t nb = st->n_alloc;
-  for (uint32_t k = blockIdx.x; k < nb; k += gridDim.x) {
-    const uint32_t ptr = order[k];
-    float3 iso, nrm;
-    const bool emit = ext_point<F16>(t, vox, a, ptr, t.blk[ptr], (int)threadIdx.x, &iso, &nrm);
-    uint32_t total;
-    const uint32_t local = block_excl_scan<512>(emit ? 1u : 0u, &total);
-    const uint64_t idx = (uint64_t)pstart[k] + local;
-    if (emit && idx < a.max_points) {
-      const uint32_t c = col[(size_t)ptr * kVox + threadIdx.x];
-      out_pos[idx] = make_float4(iso.x, iso.y, iso.z, 1.0f);
-      out_col[idx] = make_float4((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f,
-                                 (float)((c >> 16) & 0xFFu) / 255.0f, 1.0f);
-      out_nrm[idx] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+__global__ __launch_bounds__(64) void k_ext_write(TableView t, const DevState* __restrict__ st,
+                                                  const int4* __restrict__ oblk, const int32_t* __restrict__ nbp,
+                                                  const void* __restrict__ vox, const uint32_t* __restrict__ col,
+                                                  ExtractArgs a, const uint32_t* __restrict__ bcount,
+                                                  const uint64_t* __restrict__ ebits,
+                                                  const uint32_t* __restrict__ pstart, float4* __restrict__ out_pos,
+                                                  float4* __restrict__ out_col, float4* __restrict__ out_nrm) {
+  using S = Store<F16>;
+  __shared__ ExtBlock<F16> eb;
+  ExtWalk<F16> wk;
+  wk.vw = reinterpret_cast<const typename S::word*>(vox);
+  wk.oblk = oblk;
+  wk.nbp = nbp;
+  wk.nb = st->n_alloc;
+  const uint32_t G = gridDim.x, lane = threadIdx.x;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t k = blockIdx.x, start = 0, cnt = 0, sn = 0, cn = 0;
+  uint64_t em = 0, emn = 0;  // lane z < 8: emit word z of the current / next block
+  if (k < wk.nb) {
+    wk.b = oblk[k];
+    start = pstart[k];
+    cnt = bcount[k];
+    if (lane < 8u) em = ebits[(size_t)k * 8u + lane];
+    if (cnt != 0u) wk.fetch(k, wk.b);
+    if (k + G < wk.nb) {
+      wk.bn = oblk[k + G];
+      sn = pstart[k + G];
+      cn = bcount[k + G];
+      if (lane < 8u) emn = ebits[(size_t)(k + G) * 8u + lane];
     }
+  }
+  for (; k < wk.nb; k += G) {
+    if (start >= a.max_points) break;  // starts do not decrease with k
+    const int4 b = wk.b;
+    const uint32_t s0 = start, c0 = cnt;
+    if (c0 != 0u) {
+      // emitting voxels in voxel order
+      uint32_t n = 0;
+#pragma unroll
+      for (int z = 0; z < 8; ++z) {
+        const uint64_t m = __shfl(em, z, 64);
+        if ((m >> lane) & 1ull) eb.list[n + (uint32_t)__popcll(m & below)] = (uint16_t)(lane + 64u * (uint32_t)z);
+        n += (uint32_t)__popcll(m);
+      }
+      wk.stage(eb, a.voxel);
+      __syncthreads();
+    }
+    if (k + G < wk.nb && cn != 0u) wk.fetch(k + G, wk.bn);
+    wk.b = wk.bn;
+    start = sn;
+    cnt = cn;
+    em = emn;
+    const uint32_t k2 = k + 2u * G;
+    if (k2 < wk.nb) {
+      wk.bn = oblk[k2];
+      sn = pstart[k2];
+      cn = bcount[k2];
+      if (lane < 8u) emn = ebits[(size_t)k2 * 8u + lane];
+    }
+    if (c0 == 0u) continue;
+    for (uint32_t j0 = 0; j0 < c0; j0 += 64u) {
+      const uint32_t j = j0 + lane;
+      const uint64_t idx = (uint64_t)s0 + j;
+      if (j < c0 && idx < a.max_points) {
+        const int l = eb.list[j];
+        const uint32_t c = col[(size_t)b.w * kVox + (size_t)l];
+        const float3 iso = ext_iso<F16>(t, vox, eb, b, a.voxel, l);
+        const float3 nrm = ext_normal<F16>(t, vox, eb, b, iso, a.voxel);
+        out_pos[idx] = make_float4(iso.x, iso.y, iso.z, 1.0f);
+        out_col[idx] = make_float4((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f,
+                                   (float)((c >> 16) & 0xFFu) / 255.0f, 1.0f);
+        out_nrm[idx] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
+      }
+    }
+    __syncthreads();  // eb is read above until every lane is done
   }
 }
 
@@ -2319,24 +2546,44 @@ template <class T> static void launch_scan(hipStream_t s, const T* in, uint32_t 
   hipLaunchKernelGGL(k_scan_apply<T>, dim3(nchunks), dim3(1024), 0, s, in, n, sums, out);
 }
 
+// Workgroups of `threads` lanes of `kernel` the GPU holds at once (at least 1).
+template <class K> static uint32_t resident_workgroups(K kernel, int threads) {
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || cus * per < 1) {
+    (void)hipGetLastError();
+    return 1024u;
+  }
+  return (uint32_t)(cus * per);
+}
+
 void launch_extract(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const void* vox,
                     const uint32_t* col, float voxel, float min_sdf, float max_sdf, uint32_t min_weight,
                     uint32_t max_points, const ExtractScratch& x, float4* pos, float4* colr, float4* nrm) {
-  // blocks in hash-entry order
+  // blocks in hash-entry order, their neighbour pointers
   launch_scan<uint8_t>(s, t.cnt, t.num_buckets, x.sums, x.bucket_start);
-  hipLaunchKernelGGL(k_ext_order, dim3((t.num_blocks + 255u) / 256u), dim3(256), 0, s, t, st, x.bucket_start,
-                     x.order);
+  hipLaunchKernelGGL(k_ext_order, dim3((t.num_blocks + 255u) / 256u), dim3(256), 0, s, t, st, x.bucket_start, x.oblk);
+  const uint64_t nthreads = (uint64_t)t.num_blocks * kExtNb;
+  hipLaunchKernelGGL(k_ext_neighbours, dim3((uint32_t)((nthreads + 255u) / 256u)), dim3(256), 0, s, t, st, x.oblk,
+                     x.nbp);
   const ExtractArgs a{voxel, min_sdf, max_sdf, min_weight, max_points};
-  const dim3 grid(t.num_blocks < 4096u ? t.num_blocks : 4096u);
-  if (fp16) hipLaunchKernelGGL(k_ext_count<true>, grid, dim3(512), 0, s, t, st, x.order, vox, a, x.bcount);
-  else hipLaunchKernelGGL(k_ext_count<false>, grid, dim3(512), 0, s, t, st, x.order, vox, a, x.bcount);
+  // one wave per block, as many waves as the GPU holds at once (the walk is grid-stride)
+  auto grid_of = [&](uint32_t resident) { return dim3(t.num_blocks < resident ? t.num_blocks : resident); };
+  if (fp16)
+    hipLaunchKernelGGL(k_ext_count<true>, grid_of(resident_workgroups(k_ext_count<true>, 64)), dim3(64), 0, s, t, st,
+                       x.oblk, x.nbp, vox, a, x.bcount, x.ebits);
+  else
+    hipLaunchKernelGGL(k_ext_count<false>, grid_of(resident_workgroups(k_ext_count<false>, 64)), dim3(64), 0, s, t,
+                       st, x.oblk, x.nbp, vox, a, x.bcount, x.ebits);
   // block starts over all num_blocks counts (those beyond the live blocks are cleared)
   launch_scan<uint32_t>(s, x.bcount, t.num_blocks, x.sums, x.pstart);
   if (fp16)
-    hipLaunchKernelGGL(k_ext_write<true>, grid, dim3(512), 0, s, t, st, x.order, vox, col, a, x.pstart, pos, colr, nrm);
+    hipLaunchKernelGGL(k_ext_write<true>, grid_of(resident_workgroups(k_ext_write<true>, 64)), dim3(64), 0, s, t, st,
+                       x.oblk, x.nbp, vox, col, a, x.bcount, x.ebits, x.pstart, pos, colr, nrm);
   else
-    hipLaunchKernelGGL(k_ext_write<false>, grid, dim3(512), 0, s, t, st, x.order, vox, col, a, x.pstart, pos, colr,
-                       nrm);
+    hipLaunchKernelGGL(k_ext_write<false>, grid_of(resident_workgroups(k_ext_write<false>, 64)), dim3(64), 0, s, t,
+                       st, x.oblk, x.nbp, vox, col, a, x.bcount, x.ebits, x.pstart, pos, colr, nrm);
 }
 
 void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n) {
