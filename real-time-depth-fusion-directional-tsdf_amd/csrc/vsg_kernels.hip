@@ -2153,26 +2153,25 @@ __device__ __forceinline__ uint32_t ext_neighbour(const TableView& t, const void
                                                   int ox, int oy, int oz, float* nsd) {
   using S = Store<F16>;
   const int nx = x + ox, ny = y + oy, nz = z + oz;
-  typename S::word w;
-  if (nx >= 0 && nx < kBlock && ny >= 0 && ny < kBlock && nz >= 0 && nz < kBlock) {
-    w = eb.sv[nx + ny * kBlock + nz * kBlock * kBlock];
-  } else {
-    const uint32_t ex = eb.ax[0][x][ox + 1], ey = eb.ax[1][y][oy + 1], ez = eb.ax[2][z][oz + 1];
-    if (ex == kExtFar || ey == kExtFar || ez == kExtFar)
-      return ext_get_voxel<F16>(t, vox_, eb, b, ext_centre(b.x, x, v) + (float)ox * v,
-                                ext_centre(b.y, y, v) + (float)oy * v, ext_centre(b.z, z, v) + (float)oz * v, v, nsd);
-    const int dx = (int)(ex / kBlock) - 1, dy = (int)(ey / kBlock) - 1, dz = (int)(ez / kBlock) - 1;
-    const int l = (int)(ex % kBlock) + (int)(ey % kBlock) * kBlock + (int)(ez % kBlock) * kBlock * kBlock;
-    if ((dx | dy | dz) == 0) {
-      w = eb.sv[l];
-    } else {
-      const int32_t q = eb.nbp[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)];
-      if (q < 0) {
-        *nsd = 0.0f;
-        return 0u;
-      }
-      w = reinterpret_cast<const typename S::word*>(vox_)[(size_t)q * kVox + (size_t)l];
+  const bool inb = nx >= 0 && nx < kBlock && ny >= 0 && ny < kBlock && nz >= 0 && nz < kBlock;
+  // per axis (block offset + 1) * 8 + local: the index itself inside the block, the
+  // getVoxel table outside it
+  const uint32_t ex = inb ? (uint32_t)(kBlock + nx) : eb.ax[0][x][ox + 1];
+  const uint32_t ey = inb ? (uint32_t)(kBlock + ny) : eb.ax[1][y][oy + 1];
+  const uint32_t ez = inb ? (uint32_t)(kBlock + nz) : eb.ax[2][z][oz + 1];
+  if (ex == kExtFar || ey == kExtFar || ez == kExtFar)
+    return ext_get_voxel<F16>(t, vox_, eb, b, ext_centre(b.x, x, v) + (float)ox * v,
+                              ext_centre(b.y, y, v) + (float)oy * v, ext_centre(b.z, z, v) + (float)oz * v, v, nsd);
+  const int nbi = (int)(ex / kBlock) + 3 * (int)(ey / kBlock) + 9 * (int)(ez / kBlock);  // 13: this block
+  const int l = (int)(ex % kBlock) + (int)(ey % kBlock) * kBlock + (int)(ez % kBlock) * kBlock * kBlock;
+  typename S::word w = eb.sv[l];
+  if (nbi != 13) {
+    const int32_t q = eb.nbp[nbi];
+    if (q < 0) {
+      *nsd = 0.0f;
+      return 0u;
     }
+    w = reinterpret_cast<const typename S::word*>(vox_)[(size_t)q * kVox + (size_t)l];
   }
   *nsd = S::sdf(w);
   return S::weight(w);
@@ -2238,21 +2237,49 @@ __device__ __forceinline__ float3 ext_iso(const TableView& t, const void* __rest
   return make_float3(ax / (float)n, ay / (float)n, az / (float)n);
 }
 
+// The voxel getVoxel reads at voxel indices (vx, vy, vz) (block index, local index),
+// from the staged block, its neighbours' pointers or the block index.
+template <bool F16>
+__device__ __forceinline__ uint32_t ext_read_voxel(const TableView& t, const void* __restrict__ vox_,
+                                                   const ExtBlock<F16>& eb, int4 b, int32_t vx, int32_t vy,
+                                                   int32_t vz, float* sdf) {
+  using S = Store<F16>;
+  const int32_t bx = voxel_to_block_axis(vx), by = voxel_to_block_axis(vy), bz = voxel_to_block_axis(vz);
+  const int l = voxel_local_axis(vx) + voxel_local_axis(vy) * kBlock + voxel_local_axis(vz) * kBlock * kBlock;
+  const int32_t dx = bx - b.x, dy = by - b.y, dz = bz - b.z;
+  typename S::word w = eb.sv[l];
+  if ((dx | dy | dz) != 0) {
+    const bool near = dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1 && dz >= -1 && dz <= 1;
+    const int32_t q = near ? eb.nbp[(dx + 1) + 3 * (dy + 1) + 9 * (dz + 1)] : index_lookup(t, bx, by, bz);
+    if (q < 0) {
+      *sdf = 0.0f;
+      return 0u;
+    }
+    w = reinterpret_cast<const typename S::word*>(vox_)[(size_t)q * kVox + (size_t)l];
+  }
+  *sdf = S::sdf(w);
+  return S::weight(w);
+}
+
 // computeNormalFromSDF (RayCastRender.cu:629-676): central differences of getVoxel at
 // +-v/2 per axis (0 for an axis whose two voxels are not both weighted), normalised
-// when the length exceeds 1e-6.
+// when the length exceeds 1e-6.  getVoxel's voxel index of a coordinate depends on
+// that coordinate alone: three per axis (p - eps, p, p + eps) serve all six reads.
 template <bool F16>
 __device__ __forceinline__ float3 ext_normal(const TableView& t, const void* __restrict__ vox_,
                                              const ExtBlock<F16>& eb, int4 b, float3 p, float v) {
   const float eps = v * 0.5f;
+  const int32_t x0 = ray_voxel_of_q(p.x / v), xm = ray_voxel_of_q((p.x - eps) / v), xp = ray_voxel_of_q((p.x + eps) / v);
+  const int32_t y0 = ray_voxel_of_q(p.y / v), ym = ray_voxel_of_q((p.y - eps) / v), yp = ray_voxel_of_q((p.y + eps) / v);
+  const int32_t z0 = ray_voxel_of_q(p.z / v), zm = ray_voxel_of_q((p.z - eps) / v), zp = ray_voxel_of_q((p.z + eps) / v);
   float g[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int ax_ = 0; ax_ < 3; ++ax_) {
     float sp, sm;
-    const uint32_t wp = ext_get_voxel<F16>(t, vox_, eb, b, ax_ == 0 ? p.x + eps : p.x, ax_ == 1 ? p.y + eps : p.y,
-                                           ax_ == 2 ? p.z + eps : p.z, v, &sp);
-    const uint32_t wm = ext_get_voxel<F16>(t, vox_, eb, b, ax_ == 0 ? p.x - eps : p.x, ax_ == 1 ? p.y - eps : p.y,
-                                           ax_ == 2 ? p.z - eps : p.z, v, &sm);
+    const uint32_t wp = ext_read_voxel<F16>(t, vox_, eb, b, ax_ == 0 ? xp : x0, ax_ == 1 ? yp : y0, ax_ == 2 ? zp : z0,
+                                            &sp);
+    const uint32_t wm = ext_read_voxel<F16>(t, vox_, eb, b, ax_ == 0 ? xm : x0, ax_ == 1 ? ym : y0, ax_ == 2 ? zm : z0,
+                                            &sm);
     if (wp > 0u && wm > 0u) g[ax_] = (sp - sm) / (2.0f * eps);
   }
   const float len = sqrtf(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
@@ -2370,6 +2397,8 @@ __global__ __launch_bounds__(64) void k_ext_write(TableView t, const DevState* _
   wk.nb = st->n_alloc;
   const uint32_t G = gridDim.x, lane = threadIdx.x;
   const uint64_t below = (1ull << lane) - 1ull;
+  __shared__ float c255[256];  // c / 255 for the 8-bit colour channels
+  for (uint32_t i = lane; i < 256u; i += 64u) c255[i] = (float)i / 255.0f;
   uint32_t k = blockIdx.x, start = 0, cnt = 0, sn = 0, cn = 0;
   uint64_t em = 0, emn = 0;  // lane z < 8: emit word z of the current / next block
   if (k < wk.nb) {
@@ -2423,8 +2452,7 @@ __global__ __launch_bounds__(64) void k_ext_write(TableView t, const DevState* _
         const float3 iso = ext_iso<F16>(t, vox, eb, b, a.voxel, l);
         const float3 nrm = ext_normal<F16>(t, vox, eb, b, iso, a.voxel);
         out_pos[idx] = make_float4(iso.x, iso.y, iso.z, 1.0f);
-        out_col[idx] = make_float4((float)(c & 0xFFu) / 255.0f, (float)((c >> 8) & 0xFFu) / 255.0f,
-                                   (float)((c >> 16) & 0xFFu) / 255.0f, 1.0f);
+        out_col[idx] = make_float4(c255[c & 0xFFu], c255[(c >> 8) & 0xFFu], c255[(c >> 16) & 0xFFu], 1.0f);
         out_nrm[idx] = make_float4(nrm.x, nrm.y, nrm.z, 0.0f);
       }
     }
