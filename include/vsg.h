@@ -249,6 +249,47 @@ vsg_status vsg_extracted_points(const vsg_render* render, float** positions, flo
 vsg_status vsg_extracted_points_download(vsg_render* render, float* positions, float* colors, float* normals,
                                          uint32_t capacity, uint32_t* count);
 
+/* ---- depth maps from point clouds: CustomDepthMapGenerator
+ *      (VoxelSG/include/CustomDepthMapGenerator.h, VoxelSG/src/CustomDepthMapGenerator.cpp) -- */
+
+typedef struct vsg_depthgen vsg_depthgen;
+
+/* CustomDepthMapGenerator::ProjectionMethod (CustomDepthMapGenerator.h:30-34); CUSTOM_CAMERA
+ * projects as PERSPECTIVE, as in the reference (CustomDepthMapGenerator.cpp:564-573). */
+typedef enum vsg_projection {
+  VSG_PROJ_PERSPECTIVE = 0,
+  VSG_PROJ_ORTHOGRAPHIC = 1,
+  VSG_PROJ_CUSTOM_CAMERA = 2
+} vsg_projection;
+
+/* A generator for width x height maps (the reference's CameraParams width/height).
+ * Its work runs on a stream of its own unless vsg_depthgen_set_stream gives one (give
+ * the scene's, vsg_scene_get_stream, to integrate the maps without a host round trip). */
+vsg_status vsg_depthgen_create(int32_t width, int32_t height, vsg_depthgen** out);
+vsg_status vsg_depthgen_destroy(vsg_depthgen* gen);
+vsg_status vsg_depthgen_set_stream(vsg_depthgen* gen, void* hip_stream);
+/* setProjectionMethod (CustomDepthMapGenerator.cpp:76-79). */
+vsg_status vsg_depthgen_set_projection(vsg_depthgen* gen, int32_t method);
+
+/* generateFromPointCloud (VoxelSG/src/CustomDepthMapGenerator.cpp:81-222) on DEVICE
+ * arrays: camera-frame points float3[n]; normals float3[n_normals] and colours
+ * uchar3[n_colors] for the first points (NULL with a count of 0).  Each point with
+ * z > 0 projecting inside the image lands on pixel (floor u, floor v); the last such
+ * point in array order wins the pixel (the reference's loop order).  Outputs (device,
+ * W*H): pointmap float3 (the point), depth float (its z; may be NULL), normalmap float3
+ * (the normal of the last point with one that lands there), colormap uchar3 (likewise);
+ * zeros where no point lands.  With n_normals == 0 the normals are the reference's
+ * finite differences of the depth map (:164-208).  The outputs are vsg_frame's
+ * inputs.  Asynchronous. */
+vsg_status vsg_depthgen_from_points(vsg_depthgen* gen, const vsg_intrinsics* intrinsics, const float* points,
+                                    uint32_t n, const float* normals, uint32_t n_normals, const uint8_t* colors,
+                                    uint32_t n_colors, float* pointmap, float* depth, float* normalmap,
+                                    uint8_t* colormap);
+
+/* Pixels with depth > 0 of the last call (the reference prints it,
+ * CustomDepthMapGenerator.cpp:211-219).  Synchronising. */
+vsg_status vsg_depthgen_valid_pixels(vsg_depthgen* gen, uint32_t* count);
+
 /* ---- differentiable raycast and pose refinement (no counterpart in the reference;
  *      the reference's renderer is forward only) ---------------------------------------- */
 
@@ -336,7 +377,8 @@ enum {
   VSG_K_ADJOINT = 8,
   VSG_K_GN = 9,
   VSG_K_EXTRACT = 10,
-  VSG_K_COUNT = 11
+  VSG_K_DEPTHGEN = 11,
+  VSG_K_COUNT = 12
 };
 
 /* When enabled, every kernel launch is bracketed by HIP events on the stream it

@@ -52,6 +52,9 @@ def lib() -> C.CDLL:
         L.vsgo_export_voxels.argtypes = [vp, C.c_uint64, vp, vp, vp, vp]
         L.vsgo_extract_surface_points.restype = C.c_uint64
         L.vsgo_extract_surface_points.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, C.c_uint64, vp, vp, vp]
+        L.vsgo_depthmap_from_points.restype = C.c_uint32
+        L.vsgo_depthmap_from_points.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
+                                                vp, C.c_uint64, vp, C.c_uint64, vp, C.c_uint64, vp, vp, vp, vp]
         L.vsgo_hash.restype = C.c_uint32
         L.vsgo_hash.argtypes = [C.c_int32, C.c_int32, C.c_int32, C.c_uint32]
         L.vsgo_alloc_world_to_block.argtypes = [vp, C.c_float, vp, vp]
@@ -304,3 +307,23 @@ def se3_exp(xi) -> np.ndarray:
     T[:3, :3] = R
     T[:3, 3] = V @ u
     return T
+
+
+def depthmap_from_points(points, normals, colors, W: int, H: int, intrinsics, orthographic: bool = False) -> dict:
+    """generateFromPointCloud (CustomDepthMapGenerator.cpp:81-222) on host arrays: points
+    float32 [n, 3] (camera frame), normals float32 [m, 3] (m <= n; m == 0: finite
+    differences), colours uint8 [k, 3] (k <= n).  Returns pointmap / normalmap float32
+    [H*W, 3], depth float32 [H*W], colormap uint8 [H*W, 3] and the valid pixel count."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    nrm = np.ascontiguousarray(normals if normals is not None else np.zeros((0, 3)), np.float32).reshape(-1, 3)
+    col = np.ascontiguousarray(colors if colors is not None else np.zeros((0, 3)), np.uint8).reshape(-1, 3)
+    fx, fy, cx, cy = (float(v) for v in intrinsics)
+    npix = W * H
+    out = {"pointmap": np.zeros((npix, 3), np.float32), "depth": np.zeros(npix, np.float32),
+           "normalmap": np.zeros((npix, 3), np.float32), "colormap": np.zeros((npix, 3), np.uint8)}
+    valid = lib().vsgo_depthmap_from_points(int(W), int(H), fx, fy, cx, cy, 1 if orthographic else 0,
+                                            pts.ctypes.data, len(pts), nrm.ctypes.data, len(nrm), col.ctypes.data,
+                                            len(col), out["pointmap"].ctypes.data, out["depth"].ctypes.data,
+                                            out["normalmap"].ctypes.data, out["colormap"].ctypes.data)
+    out["valid"] = int(valid)
+    return out

@@ -326,4 +326,11 @@ void vsgo_depth_range(const float* depth, long n, long stride, float* out2) {
   out2[1] = mx;
 }
 
+uint32_t vsgo_depthmap_from_points(int W, int H, float fx, float fy, float cx, float cy, int ortho, const float* pts,
+                                   uint64_t n, const float* nrm, uint64_t n_nrm, const uint8_t* col, uint64_t n_col,
+                                   float* pmap, float* depth, float* nmap, uint8_t* cmap) {
+  return depthmap_from_points(W, H, fx, fy, cx, cy, ortho != 0, pts, (size_t)n, nrm, (size_t)n_nrm, col, (size_t)n_col,
+                              pmap, depth, nmap, cmap);
+}
+
 }  // extern "C"

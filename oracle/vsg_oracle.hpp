@@ -16,6 +16,7 @@
 //   voxel integration ........... VoxelSG/src/VoxelScene.cu:918-1146
 //   raycast ..................... VoxelSG/src/RayCastRender.cu:13-517
 //   surface-point extraction .... VoxelSG/src/RayCastRender.cu:629-868, RayCastRender.cpp:307-364
+//   depth map from points ....... VoxelSG/src/CustomDepthMapGenerator.cpp:81-222,550-587
 //   depth-range rule ............ VoxelSG/src/voxelSG.cpp:594-608
 //
 // Conventions this oracle fixes where the reference leaves them open:
@@ -1025,6 +1026,84 @@ inline uint64_t extract_surface_points(const Scene<R>& s, R min_sdf, R max_sdf, 
         }
   }
   return total;
+}
+
+// ---- depth map from a camera-frame point cloud: generateFromPointCloud
+// (CustomDepthMapGenerator.cpp:81-222) with projectPoint (:550-578) and isPointInBounds
+// (:580-587), the reference's own sequential loop (its CUDA path is a TODO that falls
+// back to this loop, :603-611).  Points in array order: z <= 0 skipped; perspective
+// u = x fx / z + cx, v = y fy / z + cy (orthographic u = x + cx, v = y + cy); in bounds
+// when 0 <= u < W and 0 <= v < H; pixel (floor u, floor v); every later point
+// overwrites the pixel's point and depth, and its normal / colour when it has one
+// (i < n_normals / n_colors).  With no normals at all, finite differences of the depth
+// map (:164-208).  Outputs W*H: pmap float3, depth, nmap float3, cmap uchar3 (zeros
+// where no point lands).  Returns the pixels with depth > 0.
+inline uint32_t depthmap_from_points(int W, int H, float fx, float fy, float cx, float cy, bool ortho,
+                                     const float* pts, size_t n, const float* nrm, size_t n_nrm, const uint8_t* col,
+                                     size_t n_col, float* pmap, float* depth, float* nmap, uint8_t* cmap) {
+  const size_t npix = (size_t)W * H;
+  for (size_t i = 0; i < 3 * npix; ++i) {
+    pmap[i] = 0.0f;
+    nmap[i] = 0.0f;
+    cmap[i] = 0;
+  }
+  for (size_t i = 0; i < npix; ++i) depth[i] = 0.0f;
+  for (size_t i = 0; i < n; ++i) {
+    const float x = pts[3 * i], y = pts[3 * i + 1], z = pts[3 * i + 2];
+    if (z <= 0.0f) continue;
+    float u, v;
+    if (ortho) {
+      u = x + cx;
+      v = y + cy;
+    } else {
+      u = (x * fx / z) + cx;
+      v = (y * fy / z) + cy;
+    }
+    if (!(u >= 0.0f && u < (float)W && v >= 0.0f && v < (float)H)) continue;
+    const int px = cvt_i32(std::floor(u)), py = cvt_i32(std::floor(v));
+    if (px < 0 || py < 0 || px >= W || py >= H) continue;
+    const size_t p = (size_t)py * W + px;
+    pmap[3 * p] = x;
+    pmap[3 * p + 1] = y;
+    pmap[3 * p + 2] = z;
+    depth[p] = z;
+    if (i < n_nrm) {
+      nmap[3 * p] = nrm[3 * i];
+      nmap[3 * p + 1] = nrm[3 * i + 1];
+      nmap[3 * p + 2] = nrm[3 * i + 2];
+    }
+    if (i < n_col) {
+      cmap[3 * p] = col[3 * i];
+      cmap[3 * p + 1] = col[3 * i + 1];
+      cmap[3 * p + 2] = col[3 * i + 2];
+    }
+  }
+  if (n_nrm == 0) {
+    for (int y = 1; y < H - 1; ++y)
+      for (int x = 1; x < W - 1; ++x) {
+        const size_t p = (size_t)y * W + x;
+        const float d = depth[p];
+        if (!(d > 0.0f)) continue;
+        const float c0 = ((float)x - cx) * d / fx, c1 = ((float)y - cy) * d / fy, c2 = d;
+        const float dr = depth[p + 1];
+        const float r0 = ((float)(x + 1) - cx) * dr / fx, r1 = ((float)y - cy) * dr / fy, r2 = dr;
+        const float dd = depth[p + W];
+        const float d0 = ((float)x - cx) * dd / fx, d1 = ((float)(y + 1) - cy) * dd / fy, d2 = dd;
+        const float a0 = r0 - c0, a1 = r1 - c1, a2 = r2 - c2;  // right - centre
+        const float b0 = d0 - c0, b1 = d1 - c1, b2 = d2 - c2;  // down - centre
+        const float n0 = a1 * b2 - a2 * b1, n1 = a2 * b0 - a0 * b2, n2 = a0 * b1 - a1 * b0;
+        const float len = std::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+        if (len > 0.0f) {
+          const float s = 1.0f / len;
+          nmap[3 * p] = n0 * s;
+          nmap[3 * p + 1] = n1 * s;
+          nmap[3 * p + 2] = n2 * s;
+        }
+      }
+  }
+  uint32_t valid = 0;
+  for (size_t p = 0; p < npix; ++p) valid += depth[p] > 0.0f ? 1u : 0u;
+  return valid;
 }
 
 }  // namespace vsgo

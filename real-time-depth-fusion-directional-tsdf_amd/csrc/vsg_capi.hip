@@ -123,6 +123,15 @@ struct vsg_scene {
   float2* dval = nullptr;           // directional extension: slot dtsdf per voxel
 };
 
+struct vsg_depthgen {
+  int32_t W = 0, H = 0;
+  int32_t method = VSG_PROJ_PERSPECTIVE;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  uint32_t* win = nullptr;    // last point per pixel (3 * W * H words: points, normals, colours)
+  uint32_t* valid = nullptr;  // pixels with depth > 0 of the last call
+};
+
 struct vsg_render {
   int32_t W = 0, H = 0;
   hipStream_t stream = nullptr;  // used when has_stream, else the scene's stream
@@ -1137,6 +1146,102 @@ vsg_status vsg_timing_read(double ms[VSG_K_COUNT], uint32_t launches[VSG_K_COUNT
     if (ms) ms[i] = g_timing.ms[i];
     if (launches) launches[i] = g_timing.n[i];
   }
+  return VSG_OK;
+}
+
+// ---- depth maps from point clouds ------------------------------------------------------
+
+vsg_status vsg_depthgen_create(int32_t width, int32_t height, vsg_depthgen** out) {
+  if (!out) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_create: null argument");
+  *out = nullptr;
+  if (width <= 0 || height <= 0 || (int64_t)width * height > (1 << 26))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_create: width and height must be positive, at most 2^26 pixels");
+  vsg_depthgen* g = new vsg_depthgen();
+  g->W = width;
+  g->H = height;
+  hipError_t e = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete g;
+    return hip_fail(e, "hipStreamCreateWithFlags");
+  }
+  g->own_stream = true;
+  const size_t n = (size_t)width * height;
+  vsg_status st = dmalloc(&g->win, 3 * n * sizeof(uint32_t), "depth-map winners");
+  if (st == VSG_OK) st = dmalloc(&g->valid, sizeof(uint32_t), "depth-map counter");
+  if (st == VSG_OK && hipMemset(g->valid, 0, sizeof(uint32_t)) != hipSuccess) st = fail(VSG_ERR_HIP, "hipMemset");
+  if (st != VSG_OK) {
+    vsg_depthgen_destroy(g);
+    return st;
+  }
+  *out = g;
+  return VSG_OK;
+}
+
+vsg_status vsg_depthgen_destroy(vsg_depthgen* g) {
+  if (!g) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_destroy: null generator");
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(g->win);
+  (void)hipFree(g->valid);
+  if (g->own_stream && g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return VSG_OK;
+}
+
+vsg_status vsg_depthgen_set_stream(vsg_depthgen* g, void* hip_stream) {
+  if (!g) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_set_stream: null generator");
+  VSG_HIP(hipStreamSynchronize(g->stream));
+  if (g->own_stream) VSG_HIP(hipStreamDestroy(g->stream));
+  g->stream = (hipStream_t)hip_stream;
+  g->own_stream = false;
+  return VSG_OK;
+}
+
+vsg_status vsg_depthgen_set_projection(vsg_depthgen* g, int32_t method) {
+  if (!g) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_set_projection: null generator");
+  if (method != VSG_PROJ_PERSPECTIVE && method != VSG_PROJ_ORTHOGRAPHIC && method != VSG_PROJ_CUSTOM_CAMERA)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_set_projection: unknown method");
+  g->method = method;
+  return VSG_OK;
+}
+
+vsg_status vsg_depthgen_from_points(vsg_depthgen* g, const vsg_intrinsics* k, const float* points, uint32_t n,
+                                    const float* normals, uint32_t n_normals, const uint8_t* colors,
+                                    uint32_t n_colors, float* pointmap, float* depth, float* normalmap,
+                                    uint8_t* colormap) {
+  if (!g || !k || !pointmap || !normalmap || !colormap)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_from_points: null argument");
+  if (n != 0u && !points) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_from_points: null points");
+  if (n == 0xFFFFFFFFu) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_from_points: at most 2^32 - 2 points");
+  if ((n_normals != 0u && !normals) || (n_colors != 0u && !colors))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_from_points: null normals or colours with a nonzero count");
+  const float kk[4] = {k->fx, k->fy, k->cx, k->cy};
+  if (!all_finite(kk, 4)) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_from_points: intrinsics must be finite");
+  DepthGenArgs a;
+  a.fx = k->fx;
+  a.fy = k->fy;
+  a.cx = k->cx;
+  a.cy = k->cy;
+  a.W = g->W;
+  a.H = g->H;
+  a.orthographic = g->method == VSG_PROJ_ORTHOGRAPHIC ? 1 : 0;
+  a.n = n;
+  a.n_normals = n_normals < n ? n_normals : n;
+  a.n_colors = n_colors < n ? n_colors : n;
+  const size_t npix = (size_t)g->W * g->H;
+  uint32_t* win_n = a.n_normals < n && a.n_normals != 0u ? g->win + npix : nullptr;
+  uint32_t* win_c = a.n_colors < n && a.n_colors != 0u ? g->win + 2 * npix : nullptr;
+  {
+    TimedLaunch tl(g->stream, VSG_K_DEPTHGEN);
+    launch_depthgen(g->stream, a, points, normals, colors, g->win, win_n, win_c, pointmap, depth, normalmap, colormap,
+                    g->valid);
+  }
+  return check_launch("depth map from points");
+}
+
+vsg_status vsg_depthgen_valid_pixels(vsg_depthgen* g, uint32_t* count) {
+  if (!g || !count) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_valid_pixels: null argument");
+  VSG_HIP(hipStreamSynchronize(g->stream));
+  VSG_HIP(hipMemcpy(count, g->valid, sizeof(uint32_t), hipMemcpyDeviceToHost));
   return VSG_OK;
 }
 

@@ -90,6 +90,21 @@ void launch_extract(hipStream_t s, bool fp16, const TableView& t, const DevState
                     const uint32_t* col, float voxel, float min_sdf, float max_sdf, uint32_t min_weight,
                     uint32_t max_points, const ExtractScratch& x, float4* pos, float4* colr, float4* nrm);
 
+// Depth map from a camera-frame point cloud (CustomDepthMapGenerator.cpp:81-222).
+struct DepthGenArgs {
+  float fx, fy, cx, cy;
+  int32_t W, H;
+  int32_t orthographic;  // ProjectionMethod::ORTHOGRAPHIC, else perspective
+  uint32_t n;            // points
+  uint32_t n_normals;    // normals given for points [0, n_normals)
+  uint32_t n_colors;     // colours given for points [0, n_colors)
+};
+// win: W*H words; win_n / win_c: W*H words when n_normals / n_colors < n, else null.
+// valid receives the number of pixels with depth > 0.  out_depth may be null.
+void launch_depthgen(hipStream_t s, const DepthGenArgs& g, const float* pts, const float* nrm, const uint8_t* rgb,
+                     uint32_t* win, uint32_t* win_n, uint32_t* win_c, float* out_pts, float* out_depth,
+                     float* out_nrm, uint8_t* out_rgb, uint32_t* valid);
+
 // out[i] = cvt_i32(in[i]) for i < n (device buffers).
 void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n);
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,

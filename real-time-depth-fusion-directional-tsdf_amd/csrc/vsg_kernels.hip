@@ -2460,6 +2460,129 @@ __global__ __launch_bounds__(64) void k_ext_write(TableView t, const DevState* _
   }
 }
 
+// ---------------------------------------------------------------------------------
+// Depth maps from point clouds (CustomDepthMapGenerator.cpp:81-222)
+// ---------------------------------------------------------------------------------
+
+// Pixel of camera-frame point (x, y, z), or -1 (CustomDepthMapGenerator.cpp:110-140 with
+// projectPoint :550-578 and isPointInBounds :580-587): points with z <= 0 are skipped;
+// perspective u = x fx / z + cx (orthographic u = x + cx); in bounds when 0 <= u < W and
+// 0 <= v < H; pixel (floor u, floor v).
+__device__ __forceinline__ int32_t dg_pixel(const DepthGenArgs& g, float x, float y, float z) {
+  if (z <= 0.0f) return -1;
+  float u, v;
+  if (g.orthographic) {
+    u = x + g.cx;
+    v = y + g.cy;
+  } else {
+    u = (x * g.fx / z) + g.cx;
+    v = (y * g.fy / z) + g.cy;
+  }
+  if (!(u >= 0.0f && u < (float)g.W && v >= 0.0f && v < (float)g.H)) return -1;
+  const int32_t px = cvt_i32(floorf(u)), py = cvt_i32(floorf(v));
+  if (px < 0 || py < 0 || px >= g.W || py >= g.H) return -1;
+  return py * g.W + px;
+}
+
+__global__ __launch_bounds__(256) void k_dg_clear(uint32_t* __restrict__ win, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) win[i] = 0u;
+}
+
+// Last writer in point order: the largest point index (+1) per pixel; per pixel also
+// the largest index below n_normals / n_colors when those arrays are shorter.
+__global__ __launch_bounds__(256) void k_dg_claim(DepthGenArgs g, const float* __restrict__ pts,
+                                                  uint32_t* __restrict__ win, uint32_t* __restrict__ win_n,
+                                                  uint32_t* __restrict__ win_c) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < g.n; i += gridDim.x * 256u) {
+    const int32_t p = dg_pixel(g, pts[3 * (size_t)i], pts[3 * (size_t)i + 1], pts[3 * (size_t)i + 2]);
+    if (p < 0) continue;
+    atomicMax(&win[p], i + 1u);
+    if (win_n && i < g.n_normals) atomicMax(&win_n[p], i + 1u);
+    if (win_c && i < g.n_colors) atomicMax(&win_c[p], i + 1u);
+  }
+}
+
+// Outputs per pixel: the winning point, its z, the normal and colour of the last point
+// that has one; zeros where no point lands (CustomDepthMapGenerator.cpp:103-161).
+__global__ __launch_bounds__(256) void k_dg_write(DepthGenArgs g, const float* __restrict__ pts,
+                                                  const float* __restrict__ nrm, const uint8_t* __restrict__ rgb,
+                                                  const uint32_t* __restrict__ win, const uint32_t* __restrict__ win_n,
+                                                  const uint32_t* __restrict__ win_c, float* __restrict__ out_pts,
+                                                  float* __restrict__ out_depth, float* __restrict__ out_nrm,
+                                                  uint8_t* __restrict__ out_rgb, uint32_t* __restrict__ valid) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t npix = (uint32_t)g.W * (uint32_t)g.H;
+  bool ok = false;
+  if (p < npix) {
+    const uint32_t w = win[p];
+    float x = 0.0f, y = 0.0f, z = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
+    uint8_t r = 0, gg = 0, b = 0;
+    if (w != 0u) {
+      const size_t i = w - 1u;
+      x = pts[3 * i];
+      y = pts[3 * i + 1];
+      z = pts[3 * i + 2];
+      ok = z > 0.0f;
+      const uint32_t wn = win_n ? win_n[p] : (i < g.n_normals ? w : 0u);
+      if (wn != 0u && nrm) {
+        nx = nrm[3 * (size_t)(wn - 1u)];
+        ny = nrm[3 * (size_t)(wn - 1u) + 1];
+        nz = nrm[3 * (size_t)(wn - 1u) + 2];
+      }
+      const uint32_t wc = win_c ? win_c[p] : (i < g.n_colors ? w : 0u);
+      if (wc != 0u && rgb) {
+        r = rgb[3 * (size_t)(wc - 1u)];
+        gg = rgb[3 * (size_t)(wc - 1u) + 1];
+        b = rgb[3 * (size_t)(wc - 1u) + 2];
+      }
+    }
+    out_pts[3 * (size_t)p] = x;
+    out_pts[3 * (size_t)p + 1] = y;
+    out_pts[3 * (size_t)p + 2] = z;
+    if (out_depth) out_depth[p] = z;
+    out_nrm[3 * (size_t)p] = nx;
+    out_nrm[3 * (size_t)p + 1] = ny;
+    out_nrm[3 * (size_t)p + 2] = nz;
+    out_rgb[3 * (size_t)p] = r;
+    out_rgb[3 * (size_t)p + 1] = gg;
+    out_rgb[3 * (size_t)p + 2] = b;
+  }
+  // pixels with depth > 0 (CustomDepthMapGenerator.cpp:211-216)
+  const uint64_t m = __ballot(ok);
+  if ((threadIdx.x & 63u) == 0u && m != 0ull) atomicAdd(valid, (uint32_t)__popcll(m));
+}
+
+// Finite-difference normals when the cloud has none (CustomDepthMapGenerator.cpp:164-208):
+// interior pixels with depth > 0 get normalize((right - centre) x (down - centre)) of
+// the back-projected depths (the neighbours' depths as they are, 0 included), scaled by
+// 1 / length when the length is positive; other pixels keep 0.
+__global__ __launch_bounds__(256) void k_dg_fd_normals(DepthGenArgs g, const float* __restrict__ pmap,
+                                                       float* __restrict__ out_nrm) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  const int32_t W = g.W, H = g.H;
+  if (p >= (uint32_t)W * (uint32_t)H) return;
+  const int32_t x = (int32_t)(p % (uint32_t)W), y = (int32_t)(p / (uint32_t)W);
+  if (x < 1 || y < 1 || x >= W - 1 || y >= H - 1) return;
+  const float dc = pmap[3 * (size_t)p + 2];
+  if (!(dc > 0.0f)) return;
+  const float dr = pmap[3 * (size_t)(p + 1u) + 2], dd = pmap[3 * (size_t)(p + (uint32_t)W) + 2];
+  const float fx = g.fx, fy = g.fy, cx = g.cx, cy = g.cy;
+  const float ccx = ((float)x - cx) * dc / fx, ccy = ((float)y - cy) * dc / fy, ccz = dc;
+  const float rx = ((float)(x + 1) - cx) * dr / fx, ry = ((float)y - cy) * dr / fy, rz = dr;
+  const float ddx = ((float)x - cx) * dd / fx, ddy = ((float)(y + 1) - cy) * dd / fy, ddz = dd;
+  const float ax = rx - ccx, ay = ry - ccy, az = rz - ccz;
+  const float bx = ddx - ccx, by = ddy - ccy, bz = ddz - ccz;
+  const float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+  const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+  if (len > 0.0f) {
+    const float inv = 1.0f / len;
+    out_nrm[3 * (size_t)p] = nx * inv;
+    out_nrm[3 * (size_t)p + 1] = ny * inv;
+    out_nrm[3 * (size_t)p + 2] = nz * inv;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_convert(const float* __restrict__ in, int32_t* __restrict__ out, uint32_t n) {
   const uint32_t i = blockIdx.x * 256u + threadIdx.x;
   if (i < n) out[i] = cvt_i32(in[i]);
@@ -2612,6 +2735,24 @@ void launch_extract(hipStream_t s, bool fp16, const TableView& t, const DevState
   else
     hipLaunchKernelGGL(k_ext_write<false>, grid_of(resident_workgroups(k_ext_write<false>, 64)), dim3(64), 0, s, t,
                        st, x.oblk, x.nbp, vox, col, a, x.bcount, x.ebits, x.pstart, pos, colr, nrm);
+}
+
+void launch_depthgen(hipStream_t s, const DepthGenArgs& g, const float* pts, const float* nrm, const uint8_t* rgb,
+                     uint32_t* win, uint32_t* win_n, uint32_t* win_c, float* out_pts, float* out_depth,
+                     float* out_nrm, uint8_t* out_rgb, uint32_t* valid) {
+  const uint32_t npix = (uint32_t)g.W * (uint32_t)g.H;
+  const dim3 pgrid((npix + 255u) / 256u);
+  hipLaunchKernelGGL(k_dg_clear, pgrid, dim3(256), 0, s, win, npix);
+  if (win_n) hipLaunchKernelGGL(k_dg_clear, pgrid, dim3(256), 0, s, win_n, npix);
+  if (win_c) hipLaunchKernelGGL(k_dg_clear, pgrid, dim3(256), 0, s, win_c, npix);
+  hipLaunchKernelGGL(k_dg_clear, dim3(1), dim3(256), 0, s, valid, 1u);
+  if (g.n != 0u) {
+    const uint32_t blocks = (g.n + 255u) / 256u;
+    hipLaunchKernelGGL(k_dg_claim, dim3(blocks < 65536u ? blocks : 65536u), dim3(256), 0, s, g, pts, win, win_n, win_c);
+  }
+  hipLaunchKernelGGL(k_dg_write, pgrid, dim3(256), 0, s, g, pts, nrm, rgb, win, win_n, win_c, out_pts, out_depth,
+                     out_nrm, out_rgb, valid);
+  if (g.n_normals == 0u) hipLaunchKernelGGL(k_dg_fd_normals, pgrid, dim3(256), 0, s, g, out_pts, out_nrm);
 }
 
 void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n) {

@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "NOT_READY"}
 STORE_FP32 = 0
 STORE_FP16 = 1
 KERNELS = ["alloc_claim", "alloc_resolve", "alloc_scan", "alloc_insert", "compact", "integrate", "raycast",
-           "reset", "adjoint", "gauss_newton", "extract"]
+           "reset", "adjoint", "gauss_newton", "extract", "depthgen"]
 
 SDF_BLOCK_SIZE = 8      # VoxelSG/include/globalPramasConfig.h:8
 HASH_BUCKET_SIZE = 10   # VoxelSG/include/globalPramasConfig.h:9
@@ -103,6 +103,14 @@ _SIGNATURES = {
     "vsg_extracted_points": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 3 + [C.POINTER(C.c_uint32)]),
     "vsg_extracted_points_download": (C.c_int, [C.c_void_p] + [C.c_void_p] * 3 + [C.c_uint32,
                                                                                    C.POINTER(C.c_uint32)]),
+    "vsg_depthgen_create": (C.c_int, [C.c_int32, C.c_int32, C.POINTER(C.c_void_p)]),
+    "vsg_depthgen_destroy": (C.c_int, [C.c_void_p]),
+    "vsg_depthgen_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "vsg_depthgen_set_projection": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vsg_depthgen_from_points": (C.c_int, [C.c_void_p, C.POINTER(vsg_intrinsics), C.c_void_p, C.c_uint32, C.c_void_p,
+                                           C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p]),
+    "vsg_depthgen_valid_pixels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "vsg_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_pose_gn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
                                    C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -638,6 +646,153 @@ class RayCastRender:
 
     def isAllocated(self) -> bool:
         return self.m_isAllocated
+
+
+class CameraParams:
+    """CustomDepthMapGenerator::CameraParams (VoxelSG/include/CustomDepthMapGenerator.h:18-27),
+    with the reference's defaults."""
+
+    def __init__(self, fx=400.0, fy=400.0, cx=200.0, cy=240.0, nearPlane=0.1, farPlane=10.0, width=400, height=480):
+        self.fx, self.fy, self.cx, self.cy = float(fx), float(fy), float(cx), float(cy)
+        self.nearPlane, self.farPlane = float(nearPlane), float(farPlane)
+        self.width, self.height = int(width), int(height)
+
+
+class GeneratedDepthMap:
+    """CustomDepthMapGenerator::GeneratedDepthMap (CustomDepthMapGenerator.h:37-45): depthmap
+    [H*W], pointmap / normalmap [H*W, 3] float32, colormap [H*W, 3] uint8 (numpy arrays for
+    host results, torch CUDA tensors for device results)."""
+
+    def __init__(self, width=0, height=0, depthmap=None, pointmap=None, normalmap=None, colormap=None, valid=0):
+        self.width, self.height = width, height
+        self.depthmap, self.pointmap, self.normalmap, self.colormap = depthmap, pointmap, normalmap, colormap
+        self.valid = valid
+
+
+class CustomDepthMapGenerator:
+    """CustomDepthMapGenerator (VoxelSG/include/CustomDepthMapGenerator.h:15-144) over the
+    GPU generator of the C ABI (vsg_depthgen_*): the reference's generateFromPointCloud runs
+    on the CPU (its CUDA path is a TODO, CustomDepthMapGenerator.cpp:593-611)."""
+
+    PERSPECTIVE, ORTHOGRAPHIC, CUSTOM_CAMERA = 0, 1, 2
+
+    def __init__(self):
+        self.cameraParams_ = CameraParams()
+        self.projectionMethod_ = self.PERSPECTIVE
+        self._h = C.c_void_p()
+        self._wh = (0, 0)
+        self._stream = None
+
+    def close(self) -> None:
+        if self._h.value:
+            _check(lib().vsg_depthgen_destroy(self._h))
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def setCameraParams(self, *args) -> None:
+        """setCameraParams(CameraParams) or setCameraParams(fx, fy, cx, cy, nearPlane, farPlane,
+        width, height) (CustomDepthMapGenerator.cpp:31-51)."""
+        if len(args) == 1 and isinstance(args[0], CameraParams):
+            self.cameraParams_ = args[0]
+        else:
+            self.cameraParams_ = CameraParams(*args)
+
+    def getCameraParams(self) -> CameraParams:
+        return self.cameraParams_
+
+    def setProjectionMethod(self, method) -> None:
+        """PERSPECTIVE (default), ORTHOGRAPHIC or CUSTOM_CAMERA (projects as PERSPECTIVE)."""
+        if isinstance(method, str):
+            method = {"perspective": 0, "orthographic": 1, "custom_camera": 2}[method.lower()]
+        self.projectionMethod_ = int(method)
+
+    def getProjectionMethod(self) -> int:
+        return self.projectionMethod_
+
+    def set_stream(self, hip_stream: int) -> None:
+        """Stream of the generator's work (e.g. a scene's, VoxelScene.stream())."""
+        self._stream = int(hip_stream)
+        if self._h.value:
+            _check(lib().vsg_depthgen_set_stream(self._h, self._stream))
+
+    def _handle(self) -> C.c_void_p:
+        wh = (self.cameraParams_.width, self.cameraParams_.height)
+        if self._h.value and self._wh != wh:
+            self.close()
+        if not self._h.value:
+            _check(lib().vsg_depthgen_create(wh[0], wh[1], C.byref(self._h)))
+            self._wh = wh
+            if self._stream is not None:
+                _check(lib().vsg_depthgen_set_stream(self._h, self._stream))
+        _check(lib().vsg_depthgen_set_projection(self._h, self.projectionMethod_))
+        return self._h
+
+    def generate_device(self, points, normals=None, colors=None, out: dict | None = None) -> GeneratedDepthMap:
+        """generateFromPointCloud on torch CUDA tensors: points float32 [n, 3], normals float32
+        [m, 3] (m <= n), colours uint8 [k, 3] (k <= n).  Returns device maps (the inputs of
+        VoxelScene.integrateFromScanData); `out` may hold preallocated "pointmap",
+        "depthmap", "normalmap", "colormap" tensors.  Asynchronous on the generator's
+        stream; valid is filled by valid_pixels()."""
+        import torch
+        cp = self.cameraParams_
+        npix = cp.width * cp.height
+        dev = points.device
+        if out is None:
+            out = {"pointmap": torch.empty((npix, 3), dtype=torch.float32, device=dev),
+                   "depthmap": torch.empty(npix, dtype=torch.float32, device=dev),
+                   "normalmap": torch.empty((npix, 3), dtype=torch.float32, device=dev),
+                   "colormap": torch.empty((npix, 3), dtype=torch.uint8, device=dev)}
+        n = int(points.shape[0]) if points is not None else 0
+        nn = int(normals.shape[0]) if normals is not None else 0
+        nc = int(colors.shape[0]) if colors is not None else 0
+        k = vsg_intrinsics(cp.fx, cp.fy, cp.cx, cp.cy)
+        _check(lib().vsg_depthgen_from_points(self._handle(), C.byref(k), _dev_ptr(points) if n else None, n,
+                                              _dev_ptr(normals) if nn else None, nn, _dev_ptr(colors) if nc else None,
+                                              nc, _dev_ptr(out["pointmap"]), _dev_ptr(out.get("depthmap")),
+                                              _dev_ptr(out["normalmap"]), _dev_ptr(out["colormap"])))
+        return GeneratedDepthMap(cp.width, cp.height, out.get("depthmap"), out["pointmap"], out["normalmap"],
+                                 out["colormap"])
+
+    def valid_pixels(self) -> int:
+        """Pixels with depth > 0 of the last generation (synchronising)."""
+        c = C.c_uint32()
+        _check(lib().vsg_depthgen_valid_pixels(self._handle(), C.byref(c)))
+        return c.value
+
+    def generateFromPointCloud(self, points, normals=None, colors=None) -> GeneratedDepthMap:
+        """generateFromPointCloud (VoxelSG/src/CustomDepthMapGenerator.cpp:81-222) on host
+        arrays (numpy): camera-frame points [n, 3], normals [m, 3] (empty: finite-difference
+        normals), colours uint8 [k, 3].  Returns host maps."""
+        import torch
+        dev = torch.device("cuda")
+        pts = torch.as_tensor(np.ascontiguousarray(points, np.float32).reshape(-1, 3)).to(dev)
+        nrm = None if normals is None or len(normals) == 0 else \
+            torch.as_tensor(np.ascontiguousarray(normals, np.float32).reshape(-1, 3)).to(dev)
+        col = None if colors is None or len(colors) == 0 else \
+            torch.as_tensor(np.ascontiguousarray(colors, np.uint8).reshape(-1, 3)).to(dev)
+        torch.cuda.synchronize()
+        g = self.generate_device(pts, nrm, col)
+        valid = self.valid_pixels()
+        return GeneratedDepthMap(g.width, g.height, g.depthmap.cpu().numpy(), g.pointmap.cpu().numpy(),
+                                 g.normalmap.cpu().numpy(), g.colormap.cpu().numpy(), valid)
+
+    def generateFromCombinedPointClouds(self, src0, src45) -> GeneratedDepthMap:
+        """generateFromCombinedPointClouds (CustomDepthMapGenerator.cpp:613-643): src0's points,
+        normals and colours followed by src45's.  Each source is (points, normals, colors)."""
+        parts = [p for p in (src0, src45) if p is not None and p[0] is not None and len(p[0])]
+        if not parts:
+            print("CustomDepthMapGenerator: Both point clouds are empty!", file=sys.stderr)
+            return GeneratedDepthMap()
+        # each array is appended as it is, as the reference appends its vectors
+        shapes = ((np.float32, 3), (np.float32, 3), (np.uint8, 3))
+        cat = [np.concatenate([np.asarray(p[j] if p[j] is not None else np.zeros((0, w)), dt).reshape(-1, w)
+                               for p in parts]) for j, (dt, w) in enumerate(shapes)]
+        return self.generateFromPointCloud(*cat)
 
 
 def diag_convert(values) -> np.ndarray:
