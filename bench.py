@@ -159,6 +159,48 @@ def precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm, nframes: int, np
     return out
 
 
+def depthgen_bench(torch, vsg, sc, frames, reps: int = 20) -> dict:
+    """Depth maps from a point cloud (CustomDepthMapGenerator::generateFromPointCloud): the
+    valid points of frames 10 and 13 of the sequence (the second moved into frame 10's
+    camera), shuffled, 737k points -> 640x480 maps.  Algorithmic bytes: 27 B read per
+    point (position, normal, colour), 31 B written per pixel (point, depth, normal,
+    colour)."""
+    rng = np.random.default_rng(0)
+    parts = []
+    T10 = frames[10]["pose"].astype(np.float64)
+    for j in (10, 13):
+        f = frames[j]
+        ok = f["depth"].reshape(-1) > 0
+        p = f["points"].reshape(-1, 3)[ok].astype(np.float64)
+        M = np.linalg.inv(T10) @ f["pose"].astype(np.float64)
+        parts.append(((p @ M[:3, :3].T + M[:3, 3]).astype(np.float32), f["normals"].reshape(-1, 3)[ok],
+                      f["colors"].reshape(-1, 3)[ok]))
+    pts, nrm, col = (np.concatenate([q[k] for q in parts]) for k in range(3))
+    perm = rng.permutation(len(pts))
+    dev = torch.device("cuda:0")
+    P, N, Cc = (torch.from_numpy(np.ascontiguousarray(a[perm])).to(dev) for a in (pts, nrm, col))
+    g = vsg.CustomDepthMapGenerator()
+    fx, fy, cx, cy = sc.intrinsics
+    g.setCameraParams(fx, fy, cx, cy, 0.1, 10.0, sc.W, sc.H)
+    d = g.generate_device(P, N, Cc)
+    torch.cuda.synchronize()
+    out = {"pointmap": d.pointmap, "depthmap": d.depthmap, "normalmap": d.normalmap, "colormap": d.colormap}
+    vsg.timing_enable(True)
+    vsg.timing_reset()
+    for _ in range(reps):
+        g.generate_device(P, N, Cc, out=out)
+    valid = g.valid_pixels()
+    ms, n = vsg.timing_read()["depthgen"]
+    vsg.timing_enable(False)
+    g.close()
+    npix = sc.W * sc.H
+    b = len(pts) * 27 + npix * 31
+    gbs = b / (ms / n * 1e-3) / 1e9
+    log(f"depth maps from points: {len(pts)} points -> {valid} pixels in {ms / n:.4f} ms")
+    return {"points": int(len(pts)), "valid_pixels": int(valid), "avg_ms": round(ms / n, 4),
+            "algorithmic_bytes": int(b), "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 5)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -176,6 +218,8 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-extract", action="store_true",
                     help="skip timing the surface-point extraction of the fused scene")
+    ap.add_argument("--no-depthgen", action="store_true",
+                    help="skip timing the depth maps from a point cloud")
     ap.add_argument("--step-budget", type=int, default=None,
                     help="raycast steps per lane before the cooperative tail pass (library default if unset)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -355,6 +399,8 @@ def main() -> int:
     }
     if res["extract"] is not None:
         extra["surface_points"] = res["extract"]
+    if not args.no_depthgen:
+        extra["depth_from_points"] = depthgen_bench(torch, vsg, sc, frames)
     if args.compare_fp16 and args.store == "fp32":
         r16 = measure("fp16")
         extra["fp16_store"] = {"value": round(frames_done / r16["t_total"], 3),

@@ -389,3 +389,158 @@ class RayCastRender {
   int m_maxExtractedPoints = 10000000;
   int m_numExtractedPoints = 0;
 };
+
+// CustomDepthMapGenerator (VoxelSG/include/CustomDepthMapGenerator.h:15-144): depth maps
+// from camera-frame point clouds on the GPU (the reference's own generator runs on the
+// CPU; its CUDA path is a TODO, CustomDepthMapGenerator.cpp:593-611).  Points, normals
+// and colours are float3 / float3 / uchar3 arrays (cv::Point3f and cv::Vec3b have the
+// same layout; the cv overloads take them when VSG_HAVE_OPENCV is defined).
+class CustomDepthMapGenerator {
+ public:
+  struct CameraParams {  // CustomDepthMapGenerator.h:18-27, same defaults
+    float fx = 400.0f, fy = 400.0f, cx = 200.0f, cy = 240.0f;
+    float nearPlane = 0.1f, farPlane = 10.0f;
+    int width = 400, height = 480;
+  };
+  enum class ProjectionMethod { PERSPECTIVE, ORTHOGRAPHIC, CUSTOM_CAMERA };
+  struct GeneratedDepthMap {  // CustomDepthMapGenerator.h:37-45 (host arrays, W*H each)
+    std::vector<float> depthmap;
+    std::vector<float3> pointmap;
+    std::vector<float3> normalmap;
+    std::vector<uchar3> colormap;
+    int width = 0, height = 0;
+  };
+
+  CustomDepthMapGenerator() = default;
+  ~CustomDepthMapGenerator() { release(); }
+  CustomDepthMapGenerator(const CustomDepthMapGenerator&) = delete;
+  CustomDepthMapGenerator& operator=(const CustomDepthMapGenerator&) = delete;
+
+  void setCameraParams(const CameraParams& params) { m_cam = params; }
+  void setCameraParams(float fx, float fy, float cx, float cy, float nearPlane, float farPlane, int width, int height) {
+    m_cam.fx = fx;
+    m_cam.fy = fy;
+    m_cam.cx = cx;
+    m_cam.cy = cy;
+    m_cam.nearPlane = nearPlane;
+    m_cam.farPlane = farPlane;
+    m_cam.width = width;
+    m_cam.height = height;
+  }
+  const CameraParams& getCameraParams() const { return m_cam; }
+  void setProjectionMethod(ProjectionMethod method) { m_method = method; }
+  ProjectionMethod getProjectionMethod() const { return m_method; }
+  // Queue the generator's work on this stream (e.g. a VoxelScene's, to integrate the
+  // device maps without a host round trip).
+  void setStream(hipStream_t stream) {
+    m_stream = stream;
+    m_haveStream = true;
+    if (m_gen) vsg_depthgen_set_stream(m_gen, (void*)stream);
+  }
+
+  // generateFromPointCloud on DEVICE arrays into DEVICE maps (W*H each; depth may be
+  // null).  Asynchronous.  Returns false on error.
+  bool generateDevice(const float3* d_points, size_t n, const float3* d_normals, size_t nNormals,
+                      const uchar3* d_colors, size_t nColors, float3* d_pointmap, float* d_depth, float3* d_normalmap,
+                      uchar3* d_colormap) {
+    vsg_depthgen* g = gen();
+    if (!g) return false;
+    vsg_intrinsics k{m_cam.fx, m_cam.fy, m_cam.cx, m_cam.cy};
+    return vsg_detail::report(
+        vsg_depthgen_from_points(g, &k, (const float*)d_points, (uint32_t)n, (const float*)d_normals,
+                                 (uint32_t)nNormals, (const uint8_t*)d_colors, (uint32_t)nColors, (float*)d_pointmap,
+                                 d_depth, (float*)d_normalmap, (uint8_t*)d_colormap),
+        "CustomDepthMapGenerator::generateFromPointCloud");
+  }
+
+  // CustomDepthMapGenerator::generateFromPointCloud (CustomDepthMapGenerator.cpp:81-222)
+  // on host arrays.
+  GeneratedDepthMap generateFromPointCloud(const std::vector<float3>& points,
+                                           const std::vector<float3>& normals = std::vector<float3>(),
+                                           const std::vector<uchar3>& colors = std::vector<uchar3>()) {
+    return generateHost(points.data(), points.size(), normals.data(), normals.size(), colors.data(), colors.size());
+  }
+
+#ifdef VSG_HAVE_OPENCV
+  GeneratedDepthMap generateFromPointCloud(const std::vector<cv::Point3f>& points,
+                                           const std::vector<cv::Point3f>& normals = std::vector<cv::Point3f>(),
+                                           const std::vector<cv::Vec3b>& colors = std::vector<cv::Vec3b>()) {
+    static_assert(sizeof(cv::Point3f) == sizeof(float3) && sizeof(cv::Vec3b) == sizeof(uchar3), "layout");
+    return generateHost((const float3*)points.data(), points.size(), (const float3*)normals.data(), normals.size(),
+                        (const uchar3*)colors.data(), colors.size());
+  }
+#endif
+
+  // Valid pixels of the last generation (depth > 0).  Synchronising.
+  int validPixels() {
+    uint32_t c = 0;
+    if (m_gen) vsg_depthgen_valid_pixels(m_gen, &c);
+    return (int)c;
+  }
+
+ private:
+  GeneratedDepthMap generateHost(const float3* pts, size_t n, const float3* nrm, size_t nn, const uchar3* col,
+                                 size_t nc) {
+    GeneratedDepthMap r;
+    const size_t npix = (size_t)m_cam.width * m_cam.height;
+    vsg_depthgen* g = gen();
+    if (!g || n >= 0xFFFFFFFFull) return r;
+    void *dp = nullptr, *dn = nullptr, *dc = nullptr, *out = nullptr;
+    const size_t bytes_out = npix * (3 * sizeof(float) + sizeof(float) + 3 * sizeof(float) + 3);
+    bool ok = hipMalloc(&out, bytes_out) == hipSuccess;
+    if (ok && n) ok = hipMalloc(&dp, n * sizeof(float3)) == hipSuccess;
+    if (ok && nn) ok = hipMalloc(&dn, nn * sizeof(float3)) == hipSuccess;
+    if (ok && nc) ok = hipMalloc(&dc, nc * sizeof(uchar3)) == hipSuccess;
+    if (ok && n) ok = hipMemcpy(dp, pts, n * sizeof(float3), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && nn) ok = hipMemcpy(dn, nrm, nn * sizeof(float3), hipMemcpyHostToDevice) == hipSuccess;
+    if (ok && nc) ok = hipMemcpy(dc, col, nc * sizeof(uchar3), hipMemcpyHostToDevice) == hipSuccess;
+    float3* opts = (float3*)out;
+    float* odep = (float*)(opts + npix);
+    float3* onrm = (float3*)(odep + npix);
+    uchar3* ocol = (uchar3*)(onrm + npix);
+    if (ok) ok = generateDevice((const float3*)dp, n, (const float3*)dn, nn, (const uchar3*)dc, nc, opts, odep, onrm, ocol);
+    uint32_t valid = 0;
+    if (ok) ok = vsg_depthgen_valid_pixels(m_gen, &valid) == VSG_OK;  // synchronises the generator's stream
+    if (ok) {
+      r.width = m_cam.width;
+      r.height = m_cam.height;
+      r.pointmap.resize(npix);
+      r.depthmap.resize(npix);
+      r.normalmap.resize(npix);
+      r.colormap.resize(npix);
+      ok = hipMemcpy(r.pointmap.data(), opts, npix * sizeof(float3), hipMemcpyDeviceToHost) == hipSuccess &&
+           hipMemcpy(r.depthmap.data(), odep, npix * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess &&
+           hipMemcpy(r.normalmap.data(), onrm, npix * sizeof(float3), hipMemcpyDeviceToHost) == hipSuccess &&
+           hipMemcpy(r.colormap.data(), ocol, npix * sizeof(uchar3), hipMemcpyDeviceToHost) == hipSuccess;
+      if (!ok) r = GeneratedDepthMap();
+    }
+    if (!ok) std::fprintf(stderr, "CustomDepthMapGenerator: GPU generation failed\n");
+    (void)hipFree(dp);
+    (void)hipFree(dn);
+    (void)hipFree(dc);
+    (void)hipFree(out);
+    return r;
+  }
+  vsg_depthgen* gen() {
+    if (m_gen && (m_genW != m_cam.width || m_genH != m_cam.height)) release();
+    if (!m_gen) {
+      if (!vsg_detail::report(vsg_depthgen_create(m_cam.width, m_cam.height, &m_gen), "CustomDepthMapGenerator"))
+        return nullptr;
+      m_genW = m_cam.width;
+      m_genH = m_cam.height;
+      if (m_haveStream) vsg_depthgen_set_stream(m_gen, (void*)m_stream);
+    }
+    vsg_depthgen_set_projection(m_gen, (int32_t)m_method);
+    return m_gen;
+  }
+  void release() {
+    if (m_gen) vsg_depthgen_destroy(m_gen);
+    m_gen = nullptr;
+  }
+  CameraParams m_cam;
+  ProjectionMethod m_method = ProjectionMethod::PERSPECTIVE;
+  vsg_depthgen* m_gen = nullptr;
+  int m_genW = 0, m_genH = 0;
+  hipStream_t m_stream = nullptr;
+  bool m_haveStream = false;
+};

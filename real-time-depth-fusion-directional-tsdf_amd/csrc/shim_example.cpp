@@ -8,7 +8,11 @@
 // never initialised (as the reference allows) and, when a path is given,
 // saveExtractedSurfacePointsPLY.
 //
-// usage: vsg_shim_example <workload 0|1> <frames> [points.ply]
+// With a fourth argument "gen", every frame goes through CustomDepthMapGenerator first: the
+// valid pixels' points (with normals and colours, in pixel order) form a point cloud whose
+// depth maps, generated on the device on the scene's stream, are integrated.
+//
+// usage: vsg_shim_example <workload 0|1> <frames> [points.ply|-] [gen]
 // prints: frames, allocated blocks, hits, misses, a checksum of the last depth map,
 // and the number of surface points found.
 #include <hip/hip_runtime.h>
@@ -17,6 +21,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 #include "vsg_voxelscene.hpp"
@@ -55,6 +60,21 @@ int main(int argc, char** argv) {
       hipMalloc(&d_rgb, 3 * n) != hipSuccess || hipMalloc(&d_T, 64) != hipSuccess)
     return 2;
 
+  const bool viaGenerator = argc > 4 && std::string(argv[4]) == "gen";
+  CustomDepthMapGenerator generator;
+  generator.setCameraParams(fx, fy, cx, cy, 0.1f, 10.0f, W, H);
+  void* sceneStream = nullptr;
+  vsg_scene_get_stream(scene.handle(), &sceneStream);
+  generator.setStream((hipStream_t)sceneStream);
+  float3 *c_pts = nullptr, *c_nrm = nullptr, *m_pts = nullptr, *m_nrm = nullptr;
+  uchar3 *c_rgb = nullptr, *m_rgb = nullptr;
+  if (viaGenerator && (hipMalloc(&c_pts, 12 * n) != hipSuccess || hipMalloc(&c_nrm, 12 * n) != hipSuccess ||
+                       hipMalloc(&c_rgb, 3 * n) != hipSuccess || hipMalloc(&m_pts, 12 * n) != hipSuccess ||
+                       hipMalloc(&m_nrm, 12 * n) != hipSuccess || hipMalloc(&m_rgb, 3 * n) != hipSuccess))
+    return 2;
+  std::vector<float> cp, cn;
+  std::vector<uint8_t> cc;
+
   unsigned int hits = 0, misses = 0;
   double checksum = 0.0;
   for (int i = 0; i < nframes; ++i) {
@@ -67,7 +87,26 @@ int main(int argc, char** argv) {
         hipMemcpy(d_rgb, rgb.data(), 3 * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_nrm, nrm.data(), 12 * n, hipMemcpyHostToDevice) != hipSuccess)
       return 3;
-    scene.integrateFromScanData(d_pts, d_rgb, d_nrm, W, H, truncation, T, fx, fy, cx, cy);
+    if (viaGenerator) {
+      cp.clear();
+      cn.clear();
+      cc.clear();
+      for (size_t k = 0; k < n; ++k) {
+        if (!(depth[k] > 0.0f)) continue;
+        cp.insert(cp.end(), &pts[3 * k], &pts[3 * k] + 3);
+        cn.insert(cn.end(), &nrm[3 * k], &nrm[3 * k] + 3);
+        cc.insert(cc.end(), &rgb[3 * k], &rgb[3 * k] + 3);
+      }
+      const size_t m = cp.size() / 3;
+      if (hipMemcpy(c_pts, cp.data(), 12 * m, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(c_nrm, cn.data(), 12 * m, hipMemcpyHostToDevice) != hipSuccess ||
+          hipMemcpy(c_rgb, cc.data(), 3 * m, hipMemcpyHostToDevice) != hipSuccess)
+        return 3;
+      if (!generator.generateDevice(c_pts, m, c_nrm, m, c_rgb, m, m_pts, nullptr, m_nrm, m_rgb)) return 7;
+      scene.integrateFromScanData(m_pts, m_rgb, m_nrm, W, H, truncation, T, fx, fy, cx, cy);
+    } else {
+      scene.integrateFromScanData(d_pts, d_rgb, d_nrm, W, H, truncation, T, fx, fy, cx, cy);
+    }
 
     RayCastRender renderer;  // created per frame, as the driver does (voxelSG.cpp:578-579)
     if (!renderer.initialize(W, H)) return 4;
@@ -85,13 +124,19 @@ int main(int argc, char** argv) {
   }
   RayCastRender extractor;
   const int points = extractor.extractSurfacePoints(&scene);
-  if (argc > 3 && !extractor.saveExtractedSurfacePointsPLY(argv[3])) return 6;
+  if (argc > 3 && std::string(argv[3]) != "-" && !extractor.saveExtractedSurfacePointsPLY(argv[3])) return 6;
   std::printf("frames %d blocks %u hits %u misses %u checksum %.9e points %d\n", scene.getNumIntegratedFrames(),
               (unsigned)(params.SDFBlockNum - scene.getHeapFreeCount()), hits, misses, checksum, points);
   (void)hipFree(d_pts);
   (void)hipFree(d_nrm);
   (void)hipFree(d_rgb);
   (void)hipFree(d_T);
+  (void)hipFree(c_pts);
+  (void)hipFree(c_nrm);
+  (void)hipFree(c_rgb);
+  (void)hipFree(m_pts);
+  (void)hipFree(m_nrm);
+  (void)hipFree(m_rgb);
   vsgs_destroy(sc);
   return 0;
 }

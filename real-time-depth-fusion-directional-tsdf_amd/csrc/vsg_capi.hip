@@ -129,7 +129,7 @@ struct vsg_depthgen {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   uint32_t* win = nullptr;    // last point per pixel (3 * W * H words: points, normals, colours)
-  uint32_t* valid = nullptr;  // pixels with depth > 0 of the last call
+  uint32_t* valid = nullptr;  // pixels with depth > 0 of the last call (counted on request)
 };
 
 struct vsg_render {
@@ -1169,6 +1169,7 @@ vsg_status vsg_depthgen_create(int32_t width, int32_t height, vsg_depthgen** out
   vsg_status st = dmalloc(&g->win, 3 * n * sizeof(uint32_t), "depth-map winners");
   if (st == VSG_OK) st = dmalloc(&g->valid, sizeof(uint32_t), "depth-map counter");
   if (st == VSG_OK && hipMemset(g->valid, 0, sizeof(uint32_t)) != hipSuccess) st = fail(VSG_ERR_HIP, "hipMemset");
+  if (st == VSG_OK && hipMemset(g->win, 0, 3 * n * sizeof(uint32_t)) != hipSuccess) st = fail(VSG_ERR_HIP, "hipMemset");
   if (st != VSG_OK) {
     vsg_depthgen_destroy(g);
     return st;
@@ -1232,14 +1233,16 @@ vsg_status vsg_depthgen_from_points(vsg_depthgen* g, const vsg_intrinsics* k, co
   uint32_t* win_c = a.n_colors < n && a.n_colors != 0u ? g->win + 2 * npix : nullptr;
   {
     TimedLaunch tl(g->stream, VSG_K_DEPTHGEN);
-    launch_depthgen(g->stream, a, points, normals, colors, g->win, win_n, win_c, pointmap, depth, normalmap, colormap,
-                    g->valid);
+    launch_depthgen(g->stream, a, points, normals, colors, g->win, win_n, win_c, pointmap, depth, normalmap, colormap);
   }
   return check_launch("depth map from points");
 }
 
 vsg_status vsg_depthgen_valid_pixels(vsg_depthgen* g, uint32_t* count) {
   if (!g || !count) return fail(VSG_ERR_INVALID_ARG, "vsg_depthgen_valid_pixels: null argument");
+  launch_depthgen_count(g->stream, g->win, (uint32_t)((size_t)g->W * g->H), g->valid);
+  vsg_status st = check_launch("depth map pixel count");
+  if (st != VSG_OK) return st;
   VSG_HIP(hipStreamSynchronize(g->stream));
   VSG_HIP(hipMemcpy(count, g->valid, sizeof(uint32_t), hipMemcpyDeviceToHost));
   return VSG_OK;

@@ -100,10 +100,12 @@ struct DepthGenArgs {
   uint32_t n_colors;     // colours given for points [0, n_colors)
 };
 // win: W*H words; win_n / win_c: W*H words when n_normals / n_colors < n, else null.
-// valid receives the number of pixels with depth > 0.  out_depth may be null.
+// out_depth may be null.
 void launch_depthgen(hipStream_t s, const DepthGenArgs& g, const float* pts, const float* nrm, const uint8_t* rgb,
                      uint32_t* win, uint32_t* win_n, uint32_t* win_c, float* out_pts, float* out_depth,
-                     float* out_nrm, uint8_t* out_rgb, uint32_t* valid);
+                     float* out_nrm, uint8_t* out_rgb);
+// *valid = pixels of the last launch_depthgen with a point (its win array).
+void launch_depthgen_count(hipStream_t s, const uint32_t* win, uint32_t npix, uint32_t* valid);
 
 // out[i] = cvt_i32(in[i]) for i < n (device buffers).
 void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n);

@@ -2503,54 +2503,81 @@ __global__ __launch_bounds__(256) void k_dg_claim(DepthGenArgs g, const float* _
   }
 }
 
-// Outputs per pixel: the winning point, its z, the normal and colour of the last point
-// that has one; zeros where no point lands (CustomDepthMapGenerator.cpp:103-161).
-__global__ __launch_bounds__(256) void k_dg_write(DepthGenArgs g, const float* __restrict__ pts,
-                                                  const float* __restrict__ nrm, const uint8_t* __restrict__ rgb,
-                                                  const uint32_t* __restrict__ win, const uint32_t* __restrict__ win_n,
-                                                  const uint32_t* __restrict__ win_c, float* __restrict__ out_pts,
-                                                  float* __restrict__ out_depth, float* __restrict__ out_nrm,
-                                                  uint8_t* __restrict__ out_rgb, uint32_t* __restrict__ valid) {
+// Every point that won its pixel writes it: the point, its z, and its normal / colour
+// when it also won the pixel's normal / colour (CustomDepthMapGenerator.cpp:143-161).
+// The points are read in order; the writes scatter.
+__global__ __launch_bounds__(256) void k_dg_scatter(DepthGenArgs g, const float* __restrict__ pts,
+                                                    const float* __restrict__ nrm, const uint8_t* __restrict__ rgb,
+                                                    const uint32_t* __restrict__ win,
+                                                    const uint32_t* __restrict__ win_n,
+                                                    const uint32_t* __restrict__ win_c, float* __restrict__ out_pts,
+                                                    float* __restrict__ out_depth, float* __restrict__ out_nrm,
+                                                    uint8_t* __restrict__ out_rgb) {
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < g.n; i += gridDim.x * 256u) {
+    const float x = pts[3 * (size_t)i], y = pts[3 * (size_t)i + 1], z = pts[3 * (size_t)i + 2];
+    const int32_t p = dg_pixel(g, x, y, z);
+    if (p < 0) continue;
+    const size_t q = (size_t)p;
+    if (win[p] == i + 1u) {
+      out_pts[3 * q] = x;
+      out_pts[3 * q + 1] = y;
+      out_pts[3 * q + 2] = z;
+      if (out_depth) out_depth[p] = z;
+    }
+    if (i < g.n_normals && (win_n ? win_n[p] : win[p]) == i + 1u) {
+      out_nrm[3 * q] = nrm[3 * (size_t)i];
+      out_nrm[3 * q + 1] = nrm[3 * (size_t)i + 1];
+      out_nrm[3 * q + 2] = nrm[3 * (size_t)i + 2];
+    }
+    if (i < g.n_colors && (win_c ? win_c[p] : win[p]) == i + 1u) {
+      out_rgb[3 * q] = rgb[3 * (size_t)i];
+      out_rgb[3 * q + 1] = rgb[3 * (size_t)i + 1];
+      out_rgb[3 * q + 2] = rgb[3 * (size_t)i + 2];
+    }
+  }
+}
+
+// Zeros where no point (no normal, no colour) landed (CustomDepthMapGenerator.cpp:103-107).
+__global__ __launch_bounds__(256) void k_dg_fill(DepthGenArgs g, const uint32_t* __restrict__ win,
+                                                 const uint32_t* __restrict__ win_n,
+                                                 const uint32_t* __restrict__ win_c, float* __restrict__ out_pts,
+                                                 float* __restrict__ out_depth, float* __restrict__ out_nrm,
+                                                 uint8_t* __restrict__ out_rgb) {
   const uint32_t p = blockIdx.x * 256u + threadIdx.x;
   const uint32_t npix = (uint32_t)g.W * (uint32_t)g.H;
-  bool ok = false;
   if (p < npix) {
+    const size_t q = p;
     const uint32_t w = win[p];
-    float x = 0.0f, y = 0.0f, z = 0.0f, nx = 0.0f, ny = 0.0f, nz = 0.0f;
-    uint8_t r = 0, gg = 0, b = 0;
-    if (w != 0u) {
-      const size_t i = w - 1u;
-      x = pts[3 * i];
-      y = pts[3 * i + 1];
-      z = pts[3 * i + 2];
-      ok = z > 0.0f;
-      const uint32_t wn = win_n ? win_n[p] : (i < g.n_normals ? w : 0u);
-      if (wn != 0u && nrm) {
-        nx = nrm[3 * (size_t)(wn - 1u)];
-        ny = nrm[3 * (size_t)(wn - 1u) + 1];
-        nz = nrm[3 * (size_t)(wn - 1u) + 2];
-      }
-      const uint32_t wc = win_c ? win_c[p] : (i < g.n_colors ? w : 0u);
-      if (wc != 0u && rgb) {
-        r = rgb[3 * (size_t)(wc - 1u)];
-        gg = rgb[3 * (size_t)(wc - 1u) + 1];
-        b = rgb[3 * (size_t)(wc - 1u) + 2];
-      }
+    if (w == 0u) {
+      out_pts[3 * q] = 0.0f;
+      out_pts[3 * q + 1] = 0.0f;
+      out_pts[3 * q + 2] = 0.0f;
+      if (out_depth) out_depth[p] = 0.0f;
     }
-    out_pts[3 * (size_t)p] = x;
-    out_pts[3 * (size_t)p + 1] = y;
-    out_pts[3 * (size_t)p + 2] = z;
-    if (out_depth) out_depth[p] = z;
-    out_nrm[3 * (size_t)p] = nx;
-    out_nrm[3 * (size_t)p + 1] = ny;
-    out_nrm[3 * (size_t)p + 2] = nz;
-    out_rgb[3 * (size_t)p] = r;
-    out_rgb[3 * (size_t)p + 1] = gg;
-    out_rgb[3 * (size_t)p + 2] = b;
+    const uint32_t wn = g.n_normals == 0u ? 0u : (win_n ? win_n[p] : w);
+    if (wn == 0u) {
+      out_nrm[3 * q] = 0.0f;
+      out_nrm[3 * q + 1] = 0.0f;
+      out_nrm[3 * q + 2] = 0.0f;
+    }
+    const uint32_t wc = g.n_colors == 0u ? 0u : (win_c ? win_c[p] : w);
+    if (wc == 0u) {
+      out_rgb[3 * q] = 0;
+      out_rgb[3 * q + 1] = 0;
+      out_rgb[3 * q + 2] = 0;
+    }
   }
-  // pixels with depth > 0 (CustomDepthMapGenerator.cpp:211-216)
-  const uint64_t m = __ballot(ok);
-  if ((threadIdx.x & 63u) == 0u && m != 0ull) atomicAdd(valid, (uint32_t)__popcll(m));
+}
+
+// Pixels with depth > 0 (CustomDepthMapGenerator.cpp:211-216): those with a winner
+// (every winner has z > 0).  *valid starts at 0; one addition per workgroup.
+__global__ __launch_bounds__(1024) void k_dg_count(const uint32_t* __restrict__ win, uint32_t npix,
+                                                   uint32_t* __restrict__ valid) {
+  uint32_t c = 0;
+  for (uint32_t p = blockIdx.x * 1024u + threadIdx.x; p < npix; p += gridDim.x * 1024u) c += win[p] != 0u ? 1u : 0u;
+  uint32_t total;
+  (void)block_excl_scan<1024>(c, &total);
+  if (threadIdx.x == 0 && total != 0u) atomicAdd(valid, total);
 }
 
 // Finite-difference normals when the cloud has none (CustomDepthMapGenerator.cpp:164-208):
@@ -2737,21 +2764,30 @@ void launch_extract(hipStream_t s, bool fp16, const TableView& t, const DevState
                        st, x.oblk, x.nbp, vox, col, a, x.bcount, x.ebits, x.pstart, pos, colr, nrm);
 }
 
+void launch_depthgen_count(hipStream_t s, const uint32_t* win, uint32_t npix, uint32_t* valid) {
+  hipLaunchKernelGGL(k_dg_clear, dim3(1), dim3(256), 0, s, valid, 1u);
+  const uint32_t wgs = (npix + 1023u) / 1024u;
+  hipLaunchKernelGGL(k_dg_count, dim3(wgs < 64u ? wgs : 64u), dim3(1024), 0, s, win, npix, valid);
+}
+
 void launch_depthgen(hipStream_t s, const DepthGenArgs& g, const float* pts, const float* nrm, const uint8_t* rgb,
                      uint32_t* win, uint32_t* win_n, uint32_t* win_c, float* out_pts, float* out_depth,
-                     float* out_nrm, uint8_t* out_rgb, uint32_t* valid) {
+                     float* out_nrm, uint8_t* out_rgb) {
   const uint32_t npix = (uint32_t)g.W * (uint32_t)g.H;
   const dim3 pgrid((npix + 255u) / 256u);
   hipLaunchKernelGGL(k_dg_clear, pgrid, dim3(256), 0, s, win, npix);
   if (win_n) hipLaunchKernelGGL(k_dg_clear, pgrid, dim3(256), 0, s, win_n, npix);
   if (win_c) hipLaunchKernelGGL(k_dg_clear, pgrid, dim3(256), 0, s, win_c, npix);
-  hipLaunchKernelGGL(k_dg_clear, dim3(1), dim3(256), 0, s, valid, 1u);
   if (g.n != 0u) {
     const uint32_t blocks = (g.n + 255u) / 256u;
     hipLaunchKernelGGL(k_dg_claim, dim3(blocks < 65536u ? blocks : 65536u), dim3(256), 0, s, g, pts, win, win_n, win_c);
   }
-  hipLaunchKernelGGL(k_dg_write, pgrid, dim3(256), 0, s, g, pts, nrm, rgb, win, win_n, win_c, out_pts, out_depth,
-                     out_nrm, out_rgb, valid);
+  if (g.n != 0u) {
+    const uint32_t blocks = (g.n + 255u) / 256u;
+    hipLaunchKernelGGL(k_dg_scatter, dim3(blocks < 65536u ? blocks : 65536u), dim3(256), 0, s, g, pts, nrm, rgb, win,
+                       win_n, win_c, out_pts, out_depth, out_nrm, out_rgb);
+  }
+  hipLaunchKernelGGL(k_dg_fill, pgrid, dim3(256), 0, s, g, win, win_n, win_c, out_pts, out_depth, out_nrm, out_rgb);
   if (g.n_normals == 0u) hipLaunchKernelGGL(k_dg_fd_normals, pgrid, dim3(256), 0, s, g, out_pts, out_nrm);
 }
 

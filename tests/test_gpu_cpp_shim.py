@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from fusion_util import GpuRun
-from vsg_amd import synth
+from vsg_amd import synth, vsg
 
 pytestmark = pytest.mark.gpu
 
@@ -33,6 +33,42 @@ def test_cpp_shim_matches_python_path(torch_cuda, workload, frames):
     ok = d < 1e9
     checksum = float(np.sum(d[ok].astype(np.float64) * ((k[ok] % 97) + 1)))
     assert int(got["frames"]) == frames
+    assert int(got["blocks"]) == g.scene.counters()["num_allocated"]
+    assert (int(got["hits"]), int(got["misses"])) == (r["hits"], r["misses"])
+    assert abs(float(got["checksum"]) - checksum) <= 1e-9 * abs(checksum)
+
+
+def test_cpp_generator_path_matches_python(torch_cuda):
+    """The C++ CustomDepthMapGenerator (device maps on the scene's stream, then integrate)
+    gives the scene and renders the Python mirror gives for the same clouds: each frame's
+    valid pixels, in pixel order, with their normals and colours."""
+    frames = 3
+    out = subprocess.run([EXE, "1", str(frames), "-", "gen"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    tok = out.stdout.split()
+    got = {tok[i]: tok[i + 1] for i in range(0, len(tok) - 1, 2)}
+    torch = torch_cuda
+    wl = synth.WORKLOADS[1]
+    sc = synth.SynthScene(1)
+    g = GpuRun(torch, wl)
+    gen = vsg.CustomDepthMapGenerator()
+    fx, fy, cx, cy = sc.intrinsics
+    gen.setCameraParams(fx, fy, cx, cy, 0.1, 10.0, sc.W, sc.H)
+    gen.set_stream(g.scene.stream())
+    for i in range(frames):
+        f = sc.render(i)
+        ok = f["depth"].reshape(-1) > 0
+        cloud = [torch.from_numpy(np.ascontiguousarray(f[k].reshape(-1, 3)[ok])).cuda()
+                 for k in ("points", "normals", "colors")]
+        d = gen.generate_device(*cloud)
+        g.scene.integrateFromScanData(d.pointmap, d.colormap, d.normalmap, sc.W, sc.H, wl.trunc, f["pose"], fx, fy,
+                                      cx, cy)
+        torch.cuda.synchronize()
+    r = g.raycast(f, sc.intrinsics)
+    dd = r["depth"]
+    k = np.arange(dd.size)
+    hit = dd < 1e9
+    checksum = float(np.sum(dd[hit].astype(np.float64) * ((k[hit] % 97) + 1)))
     assert int(got["blocks"]) == g.scene.counters()["num_allocated"]
     assert (int(got["hits"]), int(got["misses"])) == (r["hits"], r["misses"])
     assert abs(float(got["checksum"]) - checksum) <= 1e-9 * abs(checksum)
