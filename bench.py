@@ -201,6 +201,73 @@ def depthgen_bench(torch, vsg, sc, frames, reps: int = 20) -> dict:
             "algorithmic_bytes": int(b), "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 5)}
 
 
+def se3_exp(xi) -> np.ndarray:
+    """exp of the twist xi = (omega, upsilon) as a 4x4 (Rodrigues; V = I + B [w] + C [w]^2)."""
+    w, u = np.asarray(xi[:3], np.float64), np.asarray(xi[3:], np.float64)
+    th = float(np.linalg.norm(w))
+    K = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+    if th < 1e-12:
+        R, V = np.eye(3) + K, np.eye(3) + 0.5 * K
+    else:
+        A, B, C = np.sin(th) / th, (1 - np.cos(th)) / th ** 2, (th - np.sin(th)) / th ** 3
+        R = np.eye(3) + A * K + B * K @ K
+        V = np.eye(3) + B * K + C * K @ K
+    T = np.eye(4)
+    T[:3, :3] = R
+    T[:3, 3] = V @ u
+    return T
+
+
+def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
+    """Differentiable pose refinement (the differentiation configuration): the even frames
+    0..126 fused at their poses, the 64 odd frames 1..127 perturbed by seeded uniform
+    rotations <= 2 deg and translations <= 2 cm, 10 Gauss-Newton iterations each (FP64
+    6x6 solve on the device; residual window 20 -> 8 -> 3 cm; smoothed-gradient
+    Jacobian, standard trilinear sampling)."""
+    dev = torch.device("cuda:0")
+    scene = vsg.VoxelScene(vsg.Params(wl.buckets, wl.blocks, wl.voxel, wl.trunc), "fp32")
+    fx, fy, cx, cy = sc.intrinsics
+    for i in range(0, 128, 2):
+        f = frames[i]
+        p, c, n = (torch.from_numpy(f[k]).to(dev) for k in ("points", "colors", "normals"))
+        scene.integrateFromScanData(p, c, n, sc.W, sc.H, wl.trunc, f["pose"], fx, fy, cx, cy)
+        torch.cuda.synchronize()
+    rr = vsg.RayCastRender()
+    assert rr.initialize(sc.W, sc.H)
+    rr.set_sampling("trilinear")
+    rng = np.random.default_rng(7)
+    errs, t_gn = [], 0.0
+    for i in range(1, 128, 2):
+        axis = rng.normal(size=3)
+        axis /= np.linalg.norm(axis)
+        tdir = rng.normal(size=3)
+        tdir /= np.linalg.norm(tdir)
+        xi = np.concatenate([axis * np.radians(rng.uniform(0, 2.0)), tdir * rng.uniform(0, 0.02)])
+        f = frames[i]
+        Tgt = f["pose"].astype(np.float64)
+        T = Tgt @ se3_exp(xi)
+        obs = torch.from_numpy(f["depth"].reshape(-1)).to(dev)
+        mn, mx = synth.depth_range(f["depth"])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for it in range(10):
+            res = 0.2 if it < 3 else (0.08 if it < 6 else 0.03)
+            T, _ = rr.gn_step(scene, obs, fx, fy, cx, cy, mn, mx, T, max_residual=res, lam=1e-3)
+        t_gn += time.perf_counter() - t0
+        E = np.linalg.inv(Tgt) @ T
+        errs.append((float(np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1)))),
+                     float(np.linalg.norm(E[:3, 3]) * 1e3)))
+    scene.close()
+    rr.close()
+    E = np.array(errs)
+    ok = (E[:, 0] <= 0.1) & (E[:, 1] <= 1.0)
+    log(f"pose refinement: {int(ok.sum())}/64 within 0.1 deg / 1 mm")
+    return {"frames": 64, "iterations": 10, "within_0.1deg_1mm": int(ok.sum()),
+            "median_deg": round(float(np.median(E[:, 0])), 4), "median_mm": round(float(np.median(E[:, 1])), 3),
+            "max_deg": round(float(E[:, 0].max()), 3), "max_mm": round(float(E[:, 1].max()), 2),
+            "ms_per_frame": round(t_gn / 64 * 1e3, 2), "ms_per_iteration": round(t_gn / 640 * 1e3, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +287,8 @@ def main() -> int:
                     help="skip timing the surface-point extraction of the fused scene")
     ap.add_argument("--no-depthgen", action="store_true",
                     help="skip timing the depth maps from a point cloud")
+    ap.add_argument("--no-pose", action="store_true",
+                    help="skip the pose-refinement run (64 held-out frames, 10 iterations each)")
     ap.add_argument("--step-budget", type=int, default=None,
                     help="raycast steps per lane before the cooperative tail pass (library default if unset)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -401,6 +470,8 @@ def main() -> int:
         extra["surface_points"] = res["extract"]
     if not args.no_depthgen:
         extra["depth_from_points"] = depthgen_bench(torch, vsg, sc, frames)
+    if not args.no_pose and nf >= 128:
+        extra["pose_refinement"] = pose_refinement_bench(torch, vsg, synth, sc, frames, wl)
     if args.compare_fp16 and args.store == "fp32":
         r16 = measure("fp16")
         extra["fp16_store"] = {"value": round(frames_done / r16["t_total"], 3),

@@ -305,8 +305,19 @@ vsg_status vsg_depthgen_valid_pixels(vsg_depthgen* gen, uint32_t* count);
 vsg_status vsg_render_adjoint(vsg_render* render, vsg_scene* scene, float* jac, const float* grad_depth,
                               float* grad_sdf);
 
+/* Jacobian of vsg_pose_gn_step.  VSG_GN_SMOOTHED (default): the rendered crossing
+ * differentiated through the implicit function f(p(t)) = 0 with the field's gradient
+ * taken by central differences over +-1 voxel at the crossing (pixels where a sample is
+ * unweighted or the ray meets the surface at |cos| < 0.2 are left out).  VSG_GN_EXACT:
+ * the derivative vsg_render_adjoint returns, of the forward's own computation; its
+ * local trilinear gradient carries the noise of the fused field, which biases the
+ * normal equations towards short steps. */
+typedef enum vsg_gn_jacobian { VSG_GN_SMOOTHED = 0, VSG_GN_EXACT = 1 } vsg_gn_jacobian;
+vsg_status vsg_render_set_gn_jacobian(vsg_render* render, int32_t mode);
+
 /* One Gauss-Newton step of frame-to-model pose refinement: renders at pose (camera
- * position = its translation), takes the adjoint's Jacobian, accumulates in FP64
+ * position = its translation), takes the Jacobian of vsg_render_set_gn_jacobian's mode,
+ * accumulates in FP64
  * H = sum J J^T and g = sum J r over pixels with a rendered and an observed depth
  * (device float[W*H], 0 = invalid) and |r| < max_residual, r = rendered - observed;
  * solves (H + lambda diag H) delta = -g by FP64 Cholesky and composes

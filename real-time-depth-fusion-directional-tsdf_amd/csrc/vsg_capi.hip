@@ -155,6 +155,7 @@ struct vsg_render {
   float* jac = nullptr;       // GN scratch: Jacobian per pixel
   double* partials = nullptr; // GN scratch: per-workgroup sums
   double* gn_out = nullptr;   // GN result (25 doubles)
+  int32_t gn_jacobian = VSG_GN_SMOOTHED;  // vsg_gn_jacobian of vsg_pose_gn_step
   // surface-point extraction: outputs of capacity ext_cap, scratch sized for a scene of
   // ext_buckets buckets and ext_blocks blocks
   uint32_t max_points = 10000000u;
@@ -902,9 +903,17 @@ vsg_status vsg_render_adjoint(vsg_render* r, vsg_scene* s, float* jac, const flo
   {
     TimedLaunch tl(r->last, VSG_K_ADJOINT);
     launch_render_adjoint(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, jac, grad_depth,
-                          grad_sdf);
+                          grad_sdf, false);
   }
   return check_launch("render adjoint");
+}
+
+vsg_status vsg_render_set_gn_jacobian(vsg_render* r, int32_t mode) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_gn_jacobian: null renderer");
+  if (mode != VSG_GN_SMOOTHED && mode != VSG_GN_EXACT)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_gn_jacobian: unknown mode");
+  r->gn_jacobian = mode;
+  return VSG_OK;
 }
 
 vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_depth, const vsg_intrinsics* k,
@@ -927,7 +936,14 @@ vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_d
   cam[2] = pf[11];
   vsg_status st = vsg_render_frame(r, s, cam, pf, k, min_depth, max_depth, 0.0f);
   if (st != VSG_OK) return st;
-  st = vsg_render_adjoint(r, s, r->jac, nullptr, nullptr);
+  {
+    RenderArgs ra = r->last_args;
+    ra.mark_bkt = ra.mark_blk = nullptr;
+    TimedLaunch tl(r->last, VSG_K_ADJOINT);
+    launch_render_adjoint(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->jac, nullptr,
+                          nullptr, r->gn_jacobian == VSG_GN_SMOOTHED);
+  }
+  st = check_launch("render adjoint");
   if (st != VSG_OK) return st;
   {
     TimedLaunch tl(r->last, VSG_K_GN);

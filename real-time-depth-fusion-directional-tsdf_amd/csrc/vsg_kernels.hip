@@ -1700,7 +1700,7 @@ template <bool F16, bool REF>
 __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                        const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                        float* __restrict__ jac, const float* __restrict__ grad_depth,
-                                                       float* __restrict__ grad_sdf) {
+                                                       float* __restrict__ grad_sdf, int smooth) {
   const uint32_t x = blockIdx.x * 8u + (threadIdx.x & 7u);
   const uint32_t y = blockIdx.y * 8u + (threadIdx.x >> 3);
   if (x >= (uint32_t)ra.W || y >= (uint32_t)ra.H) return;
@@ -1757,6 +1757,48 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k) J[k] = c.d[k] * h.cdz;
+      if (smooth) {
+        // The same crossing differentiated through the implicit function f(p(t)) = 0
+        // with the field's gradient taken by central differences over +-1 voxel at the
+        // crossing: dt = -(g . dp) / (g . w).  For the camera-frame perturbation
+        // exp(xi^), the surface point moves by -omega x q - upsilon in the camera frame
+        // (q = t cd), so dt/domega = t (g_c x cd) / (g_c . cd), dt/dupsilon = -g_c /
+        // (g_c . cd), g_c the gradient in the camera frame; 0 where a sample is
+        // unweighted or the ray grazes the surface (|cos| < 0.2).
+        const float tc = c.v;
+        const float qx = px(tc), qy = py(tc), qz = pz(tc);
+        float gw[3];
+        bool ok = true;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          float fp_, fm_;
+          uint32_t rgb;
+          const float ex = a == 0 ? v : 0.0f, ey = a == 1 ? v : 0.0f, ez = a == 2 ? v : 0.0f;
+          ok = ok && trilinear<F16, false, REF>(t, bc, vox, col, qx + ex, qy + ey, qz + ez, v, &fp_, &rgb) ==
+                         kSampleValid;
+          ok = ok && trilinear<F16, false, REF>(t, bc, vox, col, qx - ex, qy - ey, qz - ez, v, &fm_, &rgb) ==
+                         kSampleValid;
+          gw[a] = ok ? (fp_ - fm_) : 0.0f;
+        }
+        const float* M = ra.M;
+        const float gcx = M[0] * gw[0] + M[4] * gw[1] + M[8] * gw[2];
+        const float gcy = M[1] * gw[0] + M[5] * gw[1] + M[9] * gw[2];
+        const float gcz = M[2] * gw[0] + M[6] * gw[1] + M[10] * gw[2];
+        const float gl = sqrtf(gcx * gcx + gcy * gcy + gcz * gcz);
+        const float den = gcx * h.cdx + gcy * h.cdy + gcz * h.cdz;
+        if (ok && gl > 0.0f && fabsf(den) >= 0.2f * gl) {
+          const float s_ = h.cdz / den;
+          J[0] = s_ * tc * (gcy * h.cdz - gcz * h.cdy);
+          J[1] = s_ * tc * (gcz * h.cdx - gcx * h.cdz);
+          J[2] = s_ * tc * (gcx * h.cdy - gcy * h.cdx);
+          J[3] = -s_ * gcx;
+          J[4] = -s_ * gcy;
+          J[5] = -s_ * gcz;
+        } else {
+#pragma unroll
+          for (int k = 0; k < 6; ++k) J[k] = 0.0f;
+        }
+      }
       if (grad_sdf && grad_depth) {
         const float g = grad_depth[pix] * h.cdz;
         if (g != 0.0f) {
@@ -2687,11 +2729,13 @@ void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st
 
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                            const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
-                           float* grad_sdf) {
+                           float* grad_sdf, bool smooth) {
   const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
   const bool ref = ra.mode == 0;
-#define VSG_ADJ(F, R) \
-  hipLaunchKernelGGL((k_render_adjoint<F, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf)
+  const int sm = smooth ? 1 : 0;
+#define VSG_ADJ(F, R)                                                                                                 \
+  hipLaunchKernelGGL((k_render_adjoint<F, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf, \
+                     sm)
   if (fp16) {
     if (ref) VSG_ADJ(true, true); else VSG_ADJ(true, false);
   } else {

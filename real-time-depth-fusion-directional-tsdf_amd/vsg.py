@@ -112,6 +112,7 @@ _SIGNATURES = {
                                            C.c_void_p]),
     "vsg_depthgen_valid_pixels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "vsg_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vsg_render_set_gn_jacobian": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsg_pose_gn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
                                    C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "vsg_import_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -604,6 +605,11 @@ class RayCastRender:
         pose * exp(xi^); grad_sdf (device float[N*512]) accumulates grad_depth^T d depth / d sdf."""
         _check(lib().vsg_render_adjoint(self._h, voxelScene.handle, _dev_ptr(jac), _dev_ptr(grad_depth),
                                         _dev_ptr(grad_sdf)))
+
+    def set_gn_jacobian(self, mode: str) -> None:
+        """"smoothed" (default: implicit-function derivative with the field gradient by central
+        differences over +-1 voxel) or "exact" (the adjoint of the forward's computation)."""
+        _check(lib().vsg_render_set_gn_jacobian(self._h, {"smoothed": 0, "exact": 1}[mode]))
 
     def gn_step(self, voxelScene: VoxelScene, observed_depth, fx, fy, cx, cy, minDepth, maxDepth, pose,
                 max_residual: float = 0.05, lam: float = 1e-4):
