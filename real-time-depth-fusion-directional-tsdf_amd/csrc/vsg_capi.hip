@@ -903,7 +903,7 @@ vsg_status vsg_render_adjoint(vsg_render* r, vsg_scene* s, float* jac, const flo
   {
     TimedLaunch tl(r->last, VSG_K_ADJOINT);
     launch_render_adjoint(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, jac, grad_depth,
-                          grad_sdf, false);
+                          grad_sdf);
   }
   return check_launch("render adjoint");
 }
@@ -940,8 +940,11 @@ vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_d
     RenderArgs ra = r->last_args;
     ra.mark_bkt = ra.mark_blk = nullptr;
     TimedLaunch tl(r->last, VSG_K_ADJOINT);
-    launch_render_adjoint(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->jac, nullptr,
-                          nullptr, r->gn_jacobian == VSG_GN_SMOOTHED);
+    if (r->gn_jacobian == VSG_GN_SMOOTHED)
+      launch_gn_jacobian(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->depth, r->jac);
+    else
+      launch_render_adjoint(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->jac, nullptr,
+                            nullptr);
   }
   st = check_launch("render adjoint");
   if (st != VSG_OK) return st;

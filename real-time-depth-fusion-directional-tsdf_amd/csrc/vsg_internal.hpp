@@ -109,12 +109,14 @@ void launch_depthgen_count(hipStream_t s, const uint32_t* win, uint32_t npix, ui
 
 // out[i] = cvt_i32(in[i]) for i < n (device buffers).
 void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n);
-// smooth: the pose Jacobian through the implicit function at the crossing with the
-// field gradient by central differences over +-1 voxel (the Gauss-Newton's default),
-// instead of the derivative of the forward's own computation.
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                            const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
-                           float* grad_sdf, bool smooth);
+                           float* grad_sdf);
+// The pose Jacobian of each pixel of the render `ra` describes (depth: its output)
+// through the implicit function at the crossing, with the field gradient by central
+// differences over +-1 voxel (the Gauss-Newton's default).
+void launch_gn_jacobian(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
+                        const void* vox, const uint32_t* col, const float* depth, float* jac);
 uint32_t gn_partial_count(int npix);
 // out: new pose (16 doubles), delta (6), cost, count, solved flag (25 doubles)
 void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int W, int H, float max_res,

@@ -1696,11 +1696,80 @@ __device__ __forceinline__ Dual10 eval_dual(const TriGradF& tg, float fval, cons
   return e;
 }
 
+// Pose Jacobian of the depth of a ray that crossed the surface at parameter tc, through
+// the implicit function f(p(t)) = 0 with the field's gradient taken by central
+// differences over +-1 voxel at the crossing: dt = -(g . dp) / (g . w).  For the
+// camera-frame perturbation exp(xi^) the surface point moves by -omega x q - upsilon in
+// the camera frame (q = t cd), so dt/domega = t (g_c x cd) / (g_c . cd) and dt/dupsilon =
+// -g_c / (g_c . cd), g_c the gradient in the camera frame; depth = t cd_z.  0 where a
+// sample is unweighted or the ray meets the surface at |cos| < 0.2.
+template <bool F16, bool REF, class Cache>
+__device__ __forceinline__ void smoothed_pose_jacobian(const TableView& t, Cache& bc, const void* __restrict__ vox,
+                                                       const uint32_t* __restrict__ col, const RenderArgs& ra,
+                                                       const RayHit& h, float tc, float J[6]) {
+  const float v = ra.voxel;
+  const float qx = ra.cam[0] + tc * h.wdx, qy = ra.cam[1] + tc * h.wdy, qz = ra.cam[2] + tc * h.wdz;
+  float gw[3];
+  bool ok = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    float fp_ = 0.0f, fm_ = 0.0f;
+    uint32_t rgb;
+    const float ex = a == 0 ? v : 0.0f, ey = a == 1 ? v : 0.0f, ez = a == 2 ? v : 0.0f;
+    ok = ok && trilinear<F16, false, REF>(t, bc, vox, col, qx + ex, qy + ey, qz + ez, v, &fp_, &rgb) == kSampleValid;
+    ok = ok && trilinear<F16, false, REF>(t, bc, vox, col, qx - ex, qy - ey, qz - ez, v, &fm_, &rgb) == kSampleValid;
+    gw[a] = ok ? (fp_ - fm_) : 0.0f;
+  }
+  const float* M = ra.M;
+  const float gcx = M[0] * gw[0] + M[4] * gw[1] + M[8] * gw[2];
+  const float gcy = M[1] * gw[0] + M[5] * gw[1] + M[9] * gw[2];
+  const float gcz = M[2] * gw[0] + M[6] * gw[1] + M[10] * gw[2];
+  const float gl = sqrtf(gcx * gcx + gcy * gcy + gcz * gcz);
+  const float den = gcx * h.cdx + gcy * h.cdy + gcz * h.cdz;
+  if (ok && gl > 0.0f && fabsf(den) >= 0.2f * gl) {
+    const float s_ = h.cdz / den;
+    J[0] = s_ * tc * (gcy * h.cdz - gcz * h.cdy);
+    J[1] = s_ * tc * (gcz * h.cdx - gcx * h.cdz);
+    J[2] = s_ * tc * (gcx * h.cdy - gcy * h.cdx);
+    J[3] = -s_ * gcx;
+    J[4] = -s_ * gcy;
+    J[5] = -s_ * gcz;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) J[k] = 0.0f;
+  }
+}
+
+// The smoothed pose Jacobian of every pixel of the last render, from its rendered depth
+// (no second march): t = depth * d2r.
+template <bool F16, bool REF>
+__global__ __launch_bounds__(64) void k_gn_jacobian(TableView t, const DevState* __restrict__ st, RenderArgs ra,
+                                                    const void* __restrict__ vox, const uint32_t* __restrict__ col,
+                                                    const float* __restrict__ depth, float* __restrict__ jac) {
+  const uint32_t x = blockIdx.x * 8u + (threadIdx.x & 7u);
+  const uint32_t y = blockIdx.y * 8u + (threadIdx.x >> 3);
+  if (x >= (uint32_t)ra.W || y >= (uint32_t)ra.H) return;
+  const int pix = (int)y * ra.W + (int)x;
+  float J[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  const float d = depth[pix];
+  if (st->n_alloc != 0u && d > 0.0f && d < 1e9f) {
+    __shared__ int32_t slot_lds[8 * 64];
+    BlockCache<false> bc;
+    bc.init(slot_lds + threadIdx.x, nullptr, nullptr);
+    RayHit h;
+    RaySetup rs;
+    (void)ray_setup(ra, x, y, h, rs);
+    smoothed_pose_jacobian<F16, REF>(t, bc, vox, col, ra, h, d * h.d2r, J);
+  }
+#pragma unroll
+  for (int k = 0; k < 6; ++k) jac[6 * (size_t)pix + k] = J[k];
+}
+
 template <bool F16, bool REF>
 __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                        const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                        float* __restrict__ jac, const float* __restrict__ grad_depth,
-                                                       float* __restrict__ grad_sdf, int smooth) {
+                                                       float* __restrict__ grad_sdf) {
   const uint32_t x = blockIdx.x * 8u + (threadIdx.x & 7u);
   const uint32_t y = blockIdx.y * 8u + (threadIdx.x >> 3);
   if (x >= (uint32_t)ra.W || y >= (uint32_t)ra.H) return;
@@ -1757,48 +1826,6 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
       }
 #pragma unroll
       for (int k = 0; k < 6; ++k) J[k] = c.d[k] * h.cdz;
-      if (smooth) {
-        // The same crossing differentiated through the implicit function f(p(t)) = 0
-        // with the field's gradient taken by central differences over +-1 voxel at the
-        // crossing: dt = -(g . dp) / (g . w).  For the camera-frame perturbation
-        // exp(xi^), the surface point moves by -omega x q - upsilon in the camera frame
-        // (q = t cd), so dt/domega = t (g_c x cd) / (g_c . cd), dt/dupsilon = -g_c /
-        // (g_c . cd), g_c the gradient in the camera frame; 0 where a sample is
-        // unweighted or the ray grazes the surface (|cos| < 0.2).
-        const float tc = c.v;
-        const float qx = px(tc), qy = py(tc), qz = pz(tc);
-        float gw[3];
-        bool ok = true;
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          float fp_, fm_;
-          uint32_t rgb;
-          const float ex = a == 0 ? v : 0.0f, ey = a == 1 ? v : 0.0f, ez = a == 2 ? v : 0.0f;
-          ok = ok && trilinear<F16, false, REF>(t, bc, vox, col, qx + ex, qy + ey, qz + ez, v, &fp_, &rgb) ==
-                         kSampleValid;
-          ok = ok && trilinear<F16, false, REF>(t, bc, vox, col, qx - ex, qy - ey, qz - ez, v, &fm_, &rgb) ==
-                         kSampleValid;
-          gw[a] = ok ? (fp_ - fm_) : 0.0f;
-        }
-        const float* M = ra.M;
-        const float gcx = M[0] * gw[0] + M[4] * gw[1] + M[8] * gw[2];
-        const float gcy = M[1] * gw[0] + M[5] * gw[1] + M[9] * gw[2];
-        const float gcz = M[2] * gw[0] + M[6] * gw[1] + M[10] * gw[2];
-        const float gl = sqrtf(gcx * gcx + gcy * gcy + gcz * gcz);
-        const float den = gcx * h.cdx + gcy * h.cdy + gcz * h.cdz;
-        if (ok && gl > 0.0f && fabsf(den) >= 0.2f * gl) {
-          const float s_ = h.cdz / den;
-          J[0] = s_ * tc * (gcy * h.cdz - gcz * h.cdy);
-          J[1] = s_ * tc * (gcz * h.cdx - gcx * h.cdz);
-          J[2] = s_ * tc * (gcx * h.cdy - gcy * h.cdx);
-          J[3] = -s_ * gcx;
-          J[4] = -s_ * gcy;
-          J[5] = -s_ * gcz;
-        } else {
-#pragma unroll
-          for (int k = 0; k < 6; ++k) J[k] = 0.0f;
-        }
-      }
       if (grad_sdf && grad_depth) {
         const float g = grad_depth[pix] * h.cdz;
         if (g != 0.0f) {
@@ -2727,15 +2754,26 @@ void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st
   hipLaunchKernelGGL(k_update_masks, dim3(256), dim3(256), 0, s, t, st, fa.cset, vis);
 }
 
-void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
-                           const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
-                           float* grad_sdf, bool smooth) {
+void launch_gn_jacobian(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
+                        const void* vox, const uint32_t* col, const float* depth, float* jac) {
   const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
   const bool ref = ra.mode == 0;
-  const int sm = smooth ? 1 : 0;
-#define VSG_ADJ(F, R)                                                                                                 \
-  hipLaunchKernelGGL((k_render_adjoint<F, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf, \
-                     sm)
+#define VSG_GNJ(F, R) hipLaunchKernelGGL((k_gn_jacobian<F, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, jac)
+  if (fp16) {
+    if (ref) VSG_GNJ(true, true); else VSG_GNJ(true, false);
+  } else {
+    if (ref) VSG_GNJ(false, true); else VSG_GNJ(false, false);
+  }
+#undef VSG_GNJ
+}
+
+void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
+                           const void* vox, const uint32_t* col, float* jac, const float* grad_depth,
+                           float* grad_sdf) {
+  const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
+  const bool ref = ra.mode == 0;
+#define VSG_ADJ(F, R) \
+  hipLaunchKernelGGL((k_render_adjoint<F, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf)
   if (fp16) {
     if (ref) VSG_ADJ(true, true); else VSG_ADJ(true, false);
   } else {
