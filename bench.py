@@ -223,7 +223,8 @@ def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
     0..126 fused at their poses, the 64 odd frames 1..127 perturbed by seeded uniform
     rotations <= 2 deg and translations <= 2 cm, 10 Gauss-Newton iterations each (FP64
     6x6 solve on the device; residual window 20 -> 8 -> 3 cm; smoothed-gradient
-    Jacobian, standard trilinear sampling)."""
+    Jacobian, residuals weighted by the sensor's noise model, standard trilinear
+    sampling)."""
     dev = torch.device("cuda:0")
     scene = vsg.VoxelScene(vsg.Params(wl.buckets, wl.blocks, wl.voxel, wl.trunc), "fp32")
     fx, fy, cx, cy = sc.intrinsics
@@ -235,6 +236,7 @@ def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
     rr = vsg.RayCastRender()
     assert rr.initialize(sc.W, sc.H)
     rr.set_sampling("trilinear")
+    rr.set_gn_noise(0.0012, 0.0019, 0.4)  # the sequence's depth noise model
     rng = np.random.default_rng(7)
     errs, t_gn = [], 0.0
     for i in range(1, 128, 2):

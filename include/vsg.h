@@ -315,6 +315,12 @@ vsg_status vsg_render_adjoint(vsg_render* render, vsg_scene* scene, float* jac, 
 typedef enum vsg_gn_jacobian { VSG_GN_SMOOTHED = 0, VSG_GN_EXACT = 1 } vsg_gn_jacobian;
 vsg_status vsg_render_set_gn_jacobian(vsg_render* render, int32_t mode);
 
+/* Residual weights of vsg_pose_gn_step from the depth sensor's noise model
+ * sigma(z) = a + b (z - z0)^2 of the observed depth z: each pixel counts
+ * (a / sigma(z))^2.  a = 0 (default): all pixels count 1.  No counterpart in the
+ * reference. */
+vsg_status vsg_render_set_gn_noise(vsg_render* render, float a, float b, float z0);
+
 /* One Gauss-Newton step of frame-to-model pose refinement: renders at pose (camera
  * position = its translation), takes the Jacobian of vsg_render_set_gn_jacobian's mode,
  * accumulates in FP64

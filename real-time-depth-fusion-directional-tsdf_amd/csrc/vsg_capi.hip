@@ -156,6 +156,7 @@ struct vsg_render {
   double* partials = nullptr; // GN scratch: per-workgroup sums
   double* gn_out = nullptr;   // GN result (25 doubles)
   int32_t gn_jacobian = VSG_GN_SMOOTHED;  // vsg_gn_jacobian of vsg_pose_gn_step
+  float gn_noise[3] = {0.0f, 0.0f, 0.0f};  // depth noise model {a, b, z0} of the residual weights (a = 0: none)
   // surface-point extraction: outputs of capacity ext_cap, scratch sized for a scene of
   // ext_buckets buckets and ext_blocks blocks
   uint32_t max_points = 10000000u;
@@ -908,6 +909,16 @@ vsg_status vsg_render_adjoint(vsg_render* r, vsg_scene* s, float* jac, const flo
   return check_launch("render adjoint");
 }
 
+vsg_status vsg_render_set_gn_noise(vsg_render* r, float a, float b, float z0) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_gn_noise: null renderer");
+  if (!std::isfinite(a) || !std::isfinite(b) || !std::isfinite(z0) || a < 0.0f || b < 0.0f)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_gn_noise: a and b must be finite and >= 0");
+  r->gn_noise[0] = a;
+  r->gn_noise[1] = b;
+  r->gn_noise[2] = z0;
+  return VSG_OK;
+}
+
 vsg_status vsg_render_set_gn_jacobian(vsg_render* r, int32_t mode) {
   if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_gn_jacobian: null renderer");
   if (mode != VSG_GN_SMOOTHED && mode != VSG_GN_EXACT)
@@ -950,8 +961,8 @@ vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_d
   if (st != VSG_OK) return st;
   {
     TimedLaunch tl(r->last, VSG_K_GN);
-    launch_gn(r->last, r->depth, observed_depth, r->jac, r->W, r->H, max_residual, lambda, pose, r->partials,
-              r->gn_out);
+    launch_gn(r->last, r->depth, observed_depth, r->jac, r->W, r->H, max_residual, r->gn_noise, lambda, pose,
+              r->partials, r->gn_out);
   }
   st = check_launch("gauss-newton");
   if (st != VSG_OK) return st;

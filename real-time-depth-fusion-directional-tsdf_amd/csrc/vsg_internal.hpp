@@ -119,8 +119,9 @@ void launch_gn_jacobian(hipStream_t s, bool fp16, const TableView& t, const DevS
                         const void* vox, const uint32_t* col, const float* depth, float* jac);
 uint32_t gn_partial_count(int npix);
 // out: new pose (16 doubles), delta (6), cost, count, solved flag (25 doubles)
+// noise = {a, b, z0}: residual weights (a / (a + b (obs - z0)^2))^2 when a > 0.
 void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int W, int H, float max_res,
-               double lambda, const double pose[16], double* partials, double* out);
+               const float noise[3], double lambda, const double pose[16], double* partials, double* out);
 void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                    const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,
                    uint32_t* rc);

@@ -1862,9 +1862,12 @@ __device__ __forceinline__ double shfl_xor_d(double v, int o) {
 // from depth edges: a pixel whose four neighbours are not all rendered within
 // max(2 cm, 2 % of its depth) of it is left out, since a small pose change can move
 // its ray onto another surface, which the Jacobian does not describe.
+// Residuals r = rendered - observed over pixels with both depths, |r| < max_res and no
+// depth edge around them; weight (a / sigma(o))^2 with the sensor's noise model
+// sigma(o) = a + b (o - z0)^2 when noise.x > 0 (noise = {a, b, z0}), else 1.
 __global__ __launch_bounds__(256) void k_gn_accum(const float* __restrict__ depth, const float* __restrict__ obs,
                                                   const float* __restrict__ jac, int W, int H, float max_res,
-                                                  double* __restrict__ partials) {
+                                                  float3 noise, double* __restrict__ partials) {
   const int npix = W * H;
   const int pix = blockIdx.x * 256 + threadIdx.x;
   double acc[kGnTerms];
@@ -1882,6 +1885,12 @@ __global__ __launch_bounds__(256) void k_gn_accum(const float* __restrict__ dept
       for (int k = 0; k < 4; ++k) use = use && nb[k] < 1e9f && fabsf(nb[k] - d) <= tol;
     }
     if (use) {
+      double w = 1.0;
+      if (noise.x > 0.0f) {
+        const double dz = (double)o - (double)noise.z;
+        const double sg = (double)noise.x + (double)noise.y * dz * dz;
+        w = ((double)noise.x / sg) * ((double)noise.x / sg);
+      }
       double Jd[6];
 #pragma unroll
       for (int k = 0; k < 6; ++k) Jd[k] = (double)jac[6 * (size_t)pix + k];
@@ -1889,10 +1898,10 @@ __global__ __launch_bounds__(256) void k_gn_accum(const float* __restrict__ dept
 #pragma unroll
       for (int i = 0; i < 6; ++i)
 #pragma unroll
-        for (int j = i; j < 6; ++j) acc[n++] = Jd[i] * Jd[j];
+        for (int j = i; j < 6; ++j) acc[n++] = w * Jd[i] * Jd[j];
 #pragma unroll
-      for (int i = 0; i < 6; ++i) acc[21 + i] = Jd[i] * (double)r;
-      acc[27] = (double)r * (double)r;
+      for (int i = 0; i < 6; ++i) acc[21 + i] = w * Jd[i] * (double)r;
+      acc[27] = w * (double)r * (double)r;
       acc[28] = 1.0;
     }
   }
@@ -2785,9 +2794,10 @@ void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const D
 uint32_t gn_partial_count(int npix) { return (uint32_t)((npix + 255) / 256); }
 
 void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int W, int H, float max_res,
-               double lambda, const double pose[16], double* partials, double* out) {
+               const float noise[3], double lambda, const double pose[16], double* partials, double* out) {
   const uint32_t nwg = gn_partial_count(W * H);
-  hipLaunchKernelGGL(k_gn_accum, dim3(nwg), dim3(256), 0, s, depth, obs, jac, W, H, max_res, partials);
+  const float3 nz = make_float3(noise[0], noise[1], noise[2]);
+  hipLaunchKernelGGL(k_gn_accum, dim3(nwg), dim3(256), 0, s, depth, obs, jac, W, H, max_res, nz, partials);
   Pose64 p;
   for (int i = 0; i < 16; ++i) p.T[i] = pose[i];
   hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, partials, (int)nwg, lambda, p, out);

@@ -113,6 +113,7 @@ _SIGNATURES = {
     "vsg_depthgen_valid_pixels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "vsg_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_render_set_gn_jacobian": (C.c_int, [C.c_void_p, C.c_int32]),
+    "vsg_render_set_gn_noise": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
     "vsg_pose_gn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
                                    C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "vsg_import_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -610,6 +611,11 @@ class RayCastRender:
         """"smoothed" (default: implicit-function derivative with the field gradient by central
         differences over +-1 voxel) or "exact" (the adjoint of the forward's computation)."""
         _check(lib().vsg_render_set_gn_jacobian(self._h, {"smoothed": 0, "exact": 1}[mode]))
+
+    def set_gn_noise(self, a: float, b: float = 0.0, z0: float = 0.0) -> None:
+        """Residual weights (a / sigma(z))^2 with sigma(z) = a + b (z - z0)^2 of the observed
+        depth (a = 0: uniform)."""
+        _check(lib().vsg_render_set_gn_noise(self._h, float(a), float(b), float(z0)))
 
     def gn_step(self, voxelScene: VoxelScene, observed_depth, fx, fy, cx, cy, minDepth, maxDepth, pose,
                 max_residual: float = 0.05, lam: float = 1e-4):
