@@ -256,9 +256,7 @@ def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
             res = 0.2 if it < 3 else (0.08 if it < 6 else 0.03)
             T, _ = rr.gn_step(scene, obs, fx, fy, cx, cy, mn, mx, T, max_residual=res, lam=1e-3)
         t_gn += time.perf_counter() - t0
-        E = np.linalg.inv(Tgt) @ T
-        errs.append((float(np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1)))),
-                     float(np.linalg.norm(E[:3, 3]) * 1e3)))
+        errs.append(_pose_err(Tgt, T))
     scene.close()
     rr.close()
     E = np.array(errs)
@@ -268,6 +266,83 @@ def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
             "median_deg": round(float(np.median(E[:, 0])), 4), "median_mm": round(float(np.median(E[:, 1])), 3),
             "max_deg": round(float(E[:, 0].max()), 3), "max_mm": round(float(E[:, 1].max()), 2),
             "ms_per_frame": round(t_gn / 64 * 1e3, 2), "ms_per_iteration": round(t_gn / 640 * 1e3, 3)}
+
+
+def large_run_bench(torch, vsg, synth, nframes: int) -> dict:
+    """The large combined configuration: 1024x768 room (8x8x3 m, 60 primitives), voxel 2 mm,
+    truncation 1 cm, 2^22 buckets and blocks, FP16 store.  Every frame after the first is
+    tracked from a prior perturbed by a seeded uniform <= 1 deg / 1 cm (10 Gauss-Newton
+    iterations, smoothed Jacobian, noise-weighted residuals, standard trilinear sampling),
+    then integrated at the refined pose and raycast there (the reference's sampling).
+    Per-frame times of the three stages and the pose errors."""
+    wl = synth.WORKLOADS[4]
+    sc = synth.SynthScene(4, nframes=nframes)
+    fx, fy, cx, cy = sc.intrinsics
+    dev = torch.device("cuda:0")
+    scene = vsg.VoxelScene(vsg.Params(wl.buckets, wl.blocks, wl.voxel, wl.trunc), "fp16")
+    stream = torch.cuda.current_stream().cuda_stream
+    scene.set_stream(stream)
+    tracker = vsg.RayCastRender()
+    assert tracker.initialize(sc.W, sc.H)
+    tracker.set_sampling("trilinear")
+    tracker.set_gn_noise(0.0012, 0.0019, 0.4)
+    rr = vsg.RayCastRender()
+    assert rr.initialize(sc.W, sc.H)
+    rng = np.random.default_rng(5)
+    t_int = t_trk = t_ray = 0.0
+    prior, refined = [], []
+    for i in range(nframes):
+        f = sc.render(i)
+        Tgt = f["pose"].astype(np.float64)
+        p, c, n = (torch.from_numpy(f[k]).to(dev) for k in ("points", "colors", "normals"))
+        obs = torch.from_numpy(f["depth"].reshape(-1)).to(dev)
+        mn, mx = synth.depth_range(f["depth"])
+        torch.cuda.synchronize()
+        T = Tgt
+        t0 = time.perf_counter()
+        if i > 0:
+            axis = rng.normal(size=3)
+            axis /= np.linalg.norm(axis)
+            tdir = rng.normal(size=3)
+            tdir /= np.linalg.norm(tdir)
+            T = Tgt @ se3_exp(np.concatenate([axis * np.radians(rng.uniform(0, 1.0)), tdir * rng.uniform(0, 0.01)]))
+            prior.append(_pose_err(Tgt, T))
+            for it in range(10):
+                res = 0.2 if it < 3 else (0.08 if it < 6 else 0.03)
+                T, _ = tracker.gn_step(scene, obs, fx, fy, cx, cy, mn, mx, T, max_residual=res, lam=1e-3)
+            refined.append(_pose_err(Tgt, T))
+        t1 = time.perf_counter()
+        Tf = T.astype(np.float32)
+        scene.integrateFromScanData(p, c, n, sc.W, sc.H, wl.trunc, Tf, fx, fy, cx, cy)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        rr.render(scene, Tf[:3, 3], Tf, fx, fy, cx, cy, mn, mx)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        t_trk += t1 - t0
+        t_int += t2 - t1
+        t_ray += t3 - t2
+    blocks = scene.counters()["num_allocated"]
+    scene.close()
+    tracker.close()
+    rr.close()
+    P, R = np.array(prior), np.array(refined)
+    log(f"large run: {nframes} frames, refined pose median {np.median(R[:, 0]):.3f} deg {np.median(R[:, 1]):.2f} mm")
+    return {"workload": "room 1024x768, voxel 2 mm, trunc 0.01 m, 2^22 buckets, FP16 store", "frames": nframes,
+            "blocks": int(blocks),
+            "ms_per_frame": {"track_10_iterations": round(t_trk / max(nframes - 1, 1) * 1e3, 3),
+                             "integrate": round(t_int / nframes * 1e3, 3), "raycast": round(t_ray / nframes * 1e3, 3)},
+            "prior_error_median": {"deg": round(float(np.median(P[:, 0])), 3), "mm": round(float(np.median(P[:, 1])), 2)},
+            "refined_error_median": {"deg": round(float(np.median(R[:, 0])), 3),
+                                     "mm": round(float(np.median(R[:, 1])), 2)},
+            "final_pose_error": {"deg": round(float(R[-1, 0]), 3), "mm": round(float(R[-1, 1]), 2)},
+            "note": "wall-clock per stage with a synchronisation after each (host-issued GN iterations)"}
+
+
+def _pose_err(Tgt, T):
+    E = np.linalg.inv(Tgt) @ T
+    return (float(np.degrees(np.arccos(np.clip((np.trace(E[:3, :3]) - 1) / 2, -1, 1)))),
+            float(np.linalg.norm(E[:3, 3]) * 1e3))
 
 
 def main() -> int:
@@ -291,6 +366,8 @@ def main() -> int:
                     help="skip timing the depth maps from a point cloud")
     ap.add_argument("--no-pose", action="store_true",
                     help="skip the pose-refinement run (64 held-out frames, 10 iterations each)")
+    ap.add_argument("--large-frames", type=int, default=100,
+                    help="frames of the 1024x768 tracked run (0: skip; the configuration has 1000)")
     ap.add_argument("--step-budget", type=int, default=None,
                     help="raycast steps per lane before the cooperative tail pass (library default if unset)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -474,6 +551,9 @@ def main() -> int:
         extra["depth_from_points"] = depthgen_bench(torch, vsg, sc, frames)
     if not args.no_pose and nf >= 128:
         extra["pose_refinement"] = pose_refinement_bench(torch, vsg, synth, sc, frames, wl)
+    if args.large_frames > 0:
+        log(f"large run: {args.large_frames} frames 1024x768 with tracking")
+        extra["large_run"] = large_run_bench(torch, vsg, synth, args.large_frames)
     if args.compare_fp16 and args.store == "fp32":
         r16 = measure("fp16")
         extra["fp16_store"] = {"value": round(frames_done / r16["t_total"], 3),
