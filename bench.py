@@ -377,7 +377,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU: "
-              f"each GPU would own a share of the hash buckets, see DESIGN.md)", file=sys.stderr)
+              f"each GPU would own a share of the hash buckets)", file=sys.stderr)
         return 2
 
     import torch
