@@ -1044,9 +1044,9 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs& ra, uint32_t x, uint
 
 // Skip test at ray parameter `ray`: 0 when the super-blocks around the sample hold
 // no allocated block, 1 when the blocks around it hold none, 2 otherwise; [t0, t1)
-// is the parameter interval of the box in which that holds (for 0, the hyper-block's
-// box when the blocks lie in one hyper-block with no allocated block, else the
-// super-blocks'; the blocks' box for 1 and 2).
+// is the parameter interval of the largest box known to hold no allocated block (for
+// 0: the hyper-block, an aligned 2x2x2 group of super-blocks or the super-blocks; for
+// 1: an aligned 2x2x2 group of blocks or the blocks; the blocks' box for 2).
 __device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const RaySkip& sk, float wdx, float wdy,
                                          float wdz, float ray, float* t0, float* t1) {
   const float mq = 1.125f + sk.eps;  // range margin around Q (voxels)
@@ -1065,18 +1065,33 @@ __device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const
   oc.rz = bz0;
   oc.rs = occ == 2 ? ext : 0u;
   if (occ == 2) return 2;
-  bool hyper = false;
-  if (occ == 0 && (bx0 >> 4) == (bx1 >> 4) && (by0 >> 4) == (by1 >> 4) && (bz0 >> 4) == (bz1 >> 4)) {
+  // box levels in blocks: 1 (the range's blocks), 2 (an aligned 2x2x2 group of blocks
+  // with no allocated block, read from the super-block's word), 4 (super-blocks), 8 (an
+  // aligned 2x2x2 group of empty super-blocks, read from the hyper-block's word), 16
+  // (an empty hyper-block).  Bits x + 4y + 16z of a word: a group at local origin
+  // (gx, gy, gz) in {0, 2}^3 is 0x330033 << (gx + 4 gy + 16 gz).
+  int32_t sh = occ == 0 ? 2 : 0;
+  if (occ == 1) {
+    if ((bx0 >> 1) == (bx1 >> 1) && (by0 >> 1) == (by1 >> 1) && (bz0 >> 1) == (bz1 >> 1)) {
+      const uint64_t w = occ_word(t, oc, bx0 >> 2, by0 >> 2, bz0 >> 2);
+      const uint32_t gsh = (uint32_t)(bx0 & 2) + 4u * (uint32_t)(by0 & 2) + 16u * (uint32_t)(bz0 & 2);
+      if ((w & (0x0000000000330033ull << gsh)) == 0ull) sh = 1;
+    }
+  } else if ((bx0 >> 4) == (bx1 >> 4) && (by0 >> 4) == (by1 >> 4) && (bz0 >> 4) == (bz1 >> 4)) {
     const uint32_t hi_ = hyp_index(bx0 >> 4, by0 >> 4, bz0 >> 4, t.hyp_mask);
     if (hi_ != oc.hidx) {
       oc.hidx = hi_;
       oc.hword = t.hyp[hi_];
     }
-    hyper = oc.hword == 0ull;
+    if (oc.hword == 0ull) {
+      sh = 4;
+    } else if ((bx0 >> 3) == (bx1 >> 3) && (by0 >> 3) == (by1 >> 3) && (bz0 >> 3) == (bz1 >> 3)) {
+      const uint32_t gsh = (uint32_t)((bx0 >> 2) & 2) + 4u * (uint32_t)((by0 >> 2) & 2) + 16u * (uint32_t)((bz0 >> 2) & 2);
+      if ((oc.hword & (0x0000000000330033ull << gsh)) == 0ull) sh = 3;
+    }
   }
-  const float g = hyper ? 128.0f : (occ == 0 ? 32.0f : 8.0f);
-  const int32_t sh = hyper ? 4 : (occ == 0 ? 2 : 0);
-  const float lo = 1.0f, hi = hyper ? 127.0f : (occ == 0 ? 31.0f : 7.0f);
+  const float g = (float)(8 << sh);
+  const float lo = 1.0f, hi = g - 1.0f;
   box_interval(sk, g * (float)(bx0 >> sh) + lo, g * (float)(bx1 >> sh) + hi, g * (float)(by0 >> sh) + lo,
                g * (float)(by1 >> sh) + hi, g * (float)(bz0 >> sh) + lo, g * (float)(bz1 >> sh) + hi, t0, t1);
   return occ;
