@@ -217,6 +217,62 @@ vsg_status vsg_render_download(vsg_render* render, float* depth, float* position
 /* Hit / miss counts of the last render (RayCastRender.cu:599-616).  Synchronising. */
 vsg_status vsg_render_counters(vsg_render* render, uint32_t* hits, uint32_t* misses);
 
+/* ---- meshing: marching cubes (VoxelScene::saveToMesh, VoxelScene.cpp:838-845, a stub
+ *      in the reference; its unbuilt MarchingCubes.cu:352-467 sketches the kernel) ---- */
+
+/* The triangle table (int8 [256][32]): case k (bit c set when corner c = (c & 1,
+ * c >> 1 & 1, c >> 2 & 1) has sdf < iso) lists its triangles as edge triples, then -1.
+ * Edge e joins corners {0,1},{2,3},{4,5},{6,7} (along x), {0,2},{1,3},{4,6},{5,7}
+ * (along y), {0,4},{1,5},{2,6},{3,7} (along z).  Generated from the cube's geometry
+ * (no copy of a published table): faces whose four edges change sign cut off each
+ * inside corner; triangle normals (v1 - v0) x (v2 - v0) point to the outside corners.
+ * Host-only (needs no GPU). */
+vsg_status vsg_mc_table(int8_t* out);
+
+typedef struct vsg_mesh vsg_mesh; /* MarchingCubesOutput (VoxelSG/include/MarchingCubes.h:19-29) */
+
+/* allocateMarchingCubesOutput (VoxelSG/src/MarchingCubes.cu:474-489): room for
+ * max_triangles triangles (3 float3 vertices and 3 float3 normals each, device memory),
+ * the triangle table on the device and the scan scratch (sized at the first call on a
+ * scene).  max_triangles must be positive, at most 2^28. */
+vsg_status vsg_mesh_create(uint32_t max_triangles, vsg_mesh** out);
+
+/* freeMarchingCubesOutput (MarchingCubes.cu:491-501). */
+vsg_status vsg_mesh_destroy(vsg_mesh* mesh);
+
+/* runMarchingCubesCUDA (MarchingCubes.cu:511-571, kernel :352-467) over every live
+ * block of the scene, in hash-entry order (bucket, slot), then cube order (x fastest),
+ * then table order (the reference walks the compactified list of the last frame and
+ * appends in atomicAdd order).  Cube (x, y, z) spans voxels x..x+1, y..y+1, z..z+1 at
+ * voxel centres (8 block + i + 0.5) voxel_size; cross_block = 0 keeps the reference's
+ * cubes inside one block (x, y, z <= 6: gaps along block borders); cross_block = 1 adds
+ * the cubes at 7, whose far corners come from the +x / +y / +z neighbours (skipped
+ * where one is not allocated), for a closed surface.  Cubes with a corner of weight
+ * below min_weight are skipped (0: none, as in the reference).  Case bit c when
+ * sdf < iso (table of vsg_mc_table); vertex on the edge from corner a to b (a < b):
+ * the midpoint when |sdf_b - sdf_a| < 1e-6, else pa + mu (pb - pa), mu = (iso - sdf_a)
+ * / (sdf_b - sdf_a) clamped to [0, 1] (vertexInterp :313-327), so a vertex shared by
+ * neighbouring cubes is the same float3 in both; the normal of all three vertices is
+ * the unit (v1 - v0) x (v2 - v0), zero when its length is at most 1e-6 (computeNormal
+ * :329-348).  voxel_size <= 0: the scene's (MarchingCubesParams::voxelSize, whose
+ * default 1.0 the reference uses as given).  *total (may be NULL) = triangles found;
+ * *stored (may be NULL) = min(total, max_triangles) written.  Enqueued on the scene's
+ * stream; synchronising (the count is read back, as copyMarchingCubesTriangleCount
+ * :503-509 does).  VSG_ERR_INVALID_ARG for a NaN iso or voxel size, or when the scene's
+ * live blocks could hold 2^32 or more triangles (more than about 800,000 blocks). */
+vsg_status vsg_marching_cubes(vsg_mesh* mesh, vsg_scene* scene, float iso, float voxel_size, int32_t cross_block,
+                              uint32_t min_weight, uint32_t* stored, uint64_t* total);
+
+/* Device buffers of the last run (MarchingCubesOutput::d_vertices / d_normals):
+ * float3 [3 * max_triangles] each, triangle i at 3 i .. 3 i + 2; *count = triangles
+ * stored.  Any output may be NULL. */
+vsg_status vsg_mesh_buffers(const vsg_mesh* mesh, float** vertices, float** normals, uint32_t* count);
+
+/* The first min(capacity, stored) triangles of the last run to HOST arrays of 9 floats
+ * per triangle (vertices / normals, either may be NULL); *count = triangles copied.
+ * Synchronising. */
+vsg_status vsg_mesh_download(vsg_mesh* mesh, float* vertices, float* normals, uint32_t capacity, uint32_t* count);
+
 /* ---- surface-point extraction ------------------------------------------------------- */
 
 /* Capacity of the extraction output (m_maxExtractedPoints, RayCastRender.cpp:62:
@@ -395,7 +451,8 @@ enum {
   VSG_K_GN = 9,
   VSG_K_EXTRACT = 10,
   VSG_K_DEPTHGEN = 11,
-  VSG_K_COUNT = 12
+  VSG_K_MESH = 12,
+  VSG_K_COUNT = 13
 };
 
 /* When enabled, every kernel launch is bracketed by HIP events on the stream it

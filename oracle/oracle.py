@@ -52,6 +52,8 @@ def lib() -> C.CDLL:
         L.vsgo_export_voxels.argtypes = [vp, C.c_uint64, vp, vp, vp, vp]
         L.vsgo_extract_surface_points.restype = C.c_uint64
         L.vsgo_extract_surface_points.argtypes = [vp, C.c_double, C.c_double, C.c_uint32, C.c_uint64, vp, vp, vp]
+        L.vsgo_marching_cubes.restype = C.c_uint64
+        L.vsgo_marching_cubes.argtypes = [vp, C.c_double, C.c_double, C.c_int32, C.c_uint32, vp, C.c_uint64, vp, vp]
         L.vsgo_depthmap_from_points.restype = C.c_uint32
         L.vsgo_depthmap_from_points.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float, C.c_int,
                                                 vp, C.c_uint64, vp, C.c_uint64, vp, C.c_uint64, vp, vp, vp, vp]
@@ -222,6 +224,23 @@ class OracleScene:
             lib().vsgo_extract_surface_points(self._h, float(min_sdf), float(max_sdf), int(min_weight), n,
                                               pos.ctypes.data, nrm.ctypes.data, col.ctypes.data)
         return {"pos": pos, "nrm": nrm, "col": col, "total": total}
+
+    def marching_cubes(self, table, iso: float = 0.0, voxel: float | None = None, cross_block: bool = False,
+                       min_weight: int = 0, max_triangles: int = 50_000_000) -> dict:
+        """Marching-cubes triangles (MarchingCubes.cu:352-467) with the given table (int8
+        [256, 32]) in hash-entry, cube and table order: vertices and per-vertex normals
+        float64 [n, 3, 3] of the first max_triangles, and the number found (total)."""
+        tab = np.ascontiguousarray(table, np.int8)
+        assert tab.shape == (256, 32)
+        v = self.voxel if voxel is None else voxel
+        args = (float(iso), float(v), int(bool(cross_block)), int(min_weight), tab.ctypes.data)
+        total = int(lib().vsgo_marching_cubes(self._h, *args, 0, None, None))
+        n = min(total, int(max_triangles))
+        verts = np.zeros((n, 3, 3), np.float64)
+        nrms = np.zeros((n, 3, 3), np.float64)
+        if n:
+            lib().vsgo_marching_cubes(self._h, *args, n, verts.ctypes.data, nrms.ctypes.data)
+        return {"verts": verts, "nrms": nrms, "total": total}
 
     def export_voxels(self, nblocks: int | None = None):
         """(sdf float64 [n*512], weight uint8, colour uint8 [n*512,3], flags uint8) for pointers [0, n)."""

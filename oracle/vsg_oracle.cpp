@@ -211,6 +211,25 @@ uint64_t vsgo_extract_surface_points(void* p, double min_sdf, double max_sdf, ui
   return h->fp64 ? run(h->sd, 0.0) : run(h->sf, 0.0f);
 }
 
+// Marching cubes over the scene with the given table (int8 [256][32]); vertices and
+// normals: 9 values per triangle for the first cap triangles.  Returns the number found.
+uint64_t vsgo_marching_cubes(void* p, double iso, double voxel, int32_t cross, uint32_t min_weight,
+                             const int8_t* table, uint64_t cap, double* verts9, double* nrms9) {
+  Handle* h = (Handle*)p;
+  const int8_t(*tab)[32] = reinterpret_cast<const int8_t(*)[32]>(table);
+  auto run = [&](auto& sc, auto tag) -> uint64_t {
+    using R = decltype(tag);
+    std::vector<R> vv, nn;
+    const uint64_t total = marching_cubes(sc, (R)iso, (R)voxel, cross, min_weight, tab, cap, vv, nn);
+    for (size_t i = 0; i < vv.size(); ++i) {
+      if (verts9) verts9[i] = (double)vv[i];
+      if (nrms9) nrms9[i] = (double)nn[i];
+    }
+    return total;
+  };
+  return h->fp64 ? run(h->sd, 0.0) : run(h->sf, 0.0f);
+}
+
 // ---- unit-level entry points (float arithmetic, as the reference kernels)
 uint32_t vsgo_hash(int32_t x, int32_t y, int32_t z, uint32_t num_buckets) {
   return hash_bucket(I3{x, y, z}, num_buckets);

@@ -1028,6 +1028,104 @@ inline uint64_t extract_surface_points(const Scene<R>& s, R min_sdf, R max_sdf, 
   return total;
 }
 
+// ---- marching cubes (MarchingCubes.cu:313-467: vertexInterp, computeNormal and
+// marchingCubesKernel).  Blocks in hash-entry order; in each block the cubes in voxel
+// order (x fastest); cube (x, y, z) spans voxels x..x+1, y..y+1, z..z+1.  As in the
+// reference, cubes stay inside the block (x, y, z <= 6); with `cross`, cubes at 7 take
+// their far corners from the +x / +y / +z neighbours and are skipped when one of those
+// is not allocated.  A cube is skipped when a corner's weight is below min_weight (0:
+// no test, as in the reference).  Corner c (bit 0: +x, bit 1: +y, bit 2: +z) sits at the
+// voxel centre (8 block + i + 0.5) v, block and local index i taken in the block that
+// holds the voxel.  Case bit c is set when sdf < iso; `table` (256 rows of 32) lists each
+// case's triangles as edge triples up to -1.  The vertex on the edge from corner a to
+// corner b (a < b, the edge numbering of the table) is vertexInterp(iso, pa, pb, sa, sb):
+// the midpoint 0.5 (pa + pb) when |sb - sa| < 1e-6, else pa + mu (pb - pa) with
+// mu = (iso - sa) / (sb - sa) clamped to [0, 1].  A triangle's normal (all three
+// vertices) is (v1 - v0) x (v2 - v0) scaled by 1 / length, zero when the length is at
+// most 1e-6.  Returns the number of triangles found and keeps the first max_tris:
+// 9 vertex and 9 normal coordinates each.
+template <class R>
+inline uint64_t marching_cubes(const Scene<R>& s, R iso, R v, int cross, uint32_t min_weight,
+                               const int8_t (*table)[32], uint64_t max_tris, std::vector<R>& verts,
+                               std::vector<R>& nrms) {
+  static const int edge[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3},
+                                  {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+  auto find = [&](I3 b) -> int32_t {
+    const size_t hp = (size_t)hash_bucket(b, s.num_buckets) * kBucket;
+    for (int j = 0; j < kBucket; ++j)
+      if (s.slot_ptr[hp + j] != -1 && s.slot_pos[hp + j] == b) return s.slot_ptr[hp + j];
+    return -1;
+  };
+  auto centre = [&](int32_t bc, int i) { return R(bc * kBlock) * v + (R(i) + R(0.5f)) * v; };
+  verts.clear();
+  nrms.clear();
+  uint64_t total = 0;
+  const int last = cross ? kBlock : kBlock - 1;
+  for (size_t e = 0; e < s.slot_ptr.size(); ++e) {
+    if (s.slot_ptr[e] == -1) continue;
+    const I3 b = s.slot_pos[e];
+    int32_t nb[8];  // blocks at offsets (c & 1, c >> 1 & 1, c >> 2 & 1)
+    for (int c = 0; c < 8; ++c)
+      nb[c] = c == 0 ? s.slot_ptr[e] : (cross ? find(I3{b.x + (c & 1), b.y + ((c >> 1) & 1), b.z + (c >> 2)}) : -1);
+    for (int z = 0; z < last; ++z)
+      for (int y = 0; y < last; ++y)
+        for (int x = 0; x < last; ++x) {
+          R sd[8], P[8][3];
+          bool ok = true;
+          int k = 0;
+          for (int c = 0; c < 8 && ok; ++c) {
+            const int g[3] = {x + (c & 1), y + ((c >> 1) & 1), z + (c >> 2)};
+            const int o = (g[0] >> 3) | ((g[1] >> 3) << 1) | ((g[2] >> 3) << 2);
+            if (nb[o] < 0) {
+              ok = false;
+              break;
+            }
+            const size_t vi = (size_t)nb[o] * kVox + (size_t)((g[0] & 7) + (g[1] & 7) * kBlock + (g[2] & 7) * kBlock * kBlock);
+            if (s.weight[vi] < min_weight) ok = false;
+            sd[c] = s.load_sdf(vi);
+            const int32_t bc[3] = {b.x, b.y, b.z};
+            for (int d = 0; d < 3; ++d) P[c][d] = centre(bc[d] + (g[d] >> 3), g[d] & 7);
+            if (sd[c] < iso) k |= 1 << c;
+          }
+          if (!ok) continue;
+          R V[12][3];
+          for (int ed = 0; ed < 12; ++ed) {
+            const int a = edge[ed][0], bb = edge[ed][1];
+            const R den = sd[bb] - sd[a];
+            if (std::fabs(den) < R(1e-6f)) {
+              for (int d = 0; d < 3; ++d) V[ed][d] = R(0.5f) * (P[a][d] + P[bb][d]);
+            } else {
+              R mu = (iso - sd[a]) / den;
+              mu = std::min(std::max(mu, R(0)), R(1));
+              for (int d = 0; d < 3; ++d) V[ed][d] = P[a][d] + mu * (P[bb][d] - P[a][d]);
+            }
+          }
+          for (int t = 0; t < 10 && table[k][3 * t] != -1; ++t, ++total) {
+            if (total >= max_tris) continue;
+            const R* v0 = V[table[k][3 * t]];
+            const R* v1 = V[table[k][3 * t + 1]];
+            const R* v2 = V[table[k][3 * t + 2]];
+            const R a[3] = {v1[0] - v0[0], v1[1] - v0[1], v1[2] - v0[2]};
+            const R c[3] = {v2[0] - v0[0], v2[1] - v0[1], v2[2] - v0[2]};
+            R n[3] = {a[1] * c[2] - a[2] * c[1], a[2] * c[0] - a[0] * c[2], a[0] * c[1] - a[1] * c[0]};
+            const R len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+            if (len > R(1e-6f)) {
+              const R inv = R(1) / len;
+              for (int d = 0; d < 3; ++d) n[d] *= inv;
+            } else {
+              n[0] = n[1] = n[2] = R(0);
+            }
+            for (const R* q : {v0, v1, v2})
+              for (int d = 0; d < 3; ++d) {
+                verts.push_back(q[d]);
+                nrms.push_back(n[d]);
+              }
+          }
+        }
+  }
+  return total;
+}
+
 // ---- depth map from a camera-frame point cloud: generateFromPointCloud
 // (CustomDepthMapGenerator.cpp:81-222) with projectPoint (:550-578) and isPointInBounds
 // (:580-587), the reference's own sequential loop (its CUDA path is a TODO that falls

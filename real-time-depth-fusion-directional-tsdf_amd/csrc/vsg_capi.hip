@@ -15,6 +15,7 @@
 #include "vsg.h"
 #include "vsg_device.hpp"
 #include "vsg_internal.hpp"
+#include "vsg_mc_table.hpp"
 
 using namespace vsg;
 
@@ -1275,6 +1276,149 @@ vsg_status vsg_depthgen_valid_pixels(vsg_depthgen* g, uint32_t* count) {
   if (st != VSG_OK) return st;
   VSG_HIP(hipStreamSynchronize(g->stream));
   VSG_HIP(hipMemcpy(count, g->valid, sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return VSG_OK;
+}
+
+// ---- meshing ---------------------------------------------------------------------------
+
+vsg_status vsg_mc_table(int8_t* out) {
+  if (!out) return fail(VSG_ERR_INVALID_ARG, "vsg_mc_table: null argument");
+  int8_t tri[256][kMcRow];
+  if (build_mc_table(tri) < 0) return fail(VSG_ERR_NOT_READY, "vsg_mc_table: table generation failed");
+  std::memcpy(out, tri, sizeof(tri));
+  return VSG_OK;
+}
+
+struct vsg_mesh {
+  uint32_t cap = 0;         // maxTriangles
+  float3* verts = nullptr;  // 3 * cap
+  float3* nrms = nullptr;   // 3 * cap
+  int8_t* table = nullptr;  // [256][kMcRow]
+  int most = 0;             // triangles of the largest case
+  ExtractScratch x{};       // scan scratch for a scene of x_buckets buckets, x_blocks blocks
+  uint32_t x_buckets = 0, x_blocks = 0;
+  uint32_t* found = nullptr;  // pinned host: pstart and bcount of the last block slot
+  uint32_t count = 0;         // triangles stored by the last run
+  hipStream_t last = nullptr;
+  bool has_last = false;
+};
+
+static void free_mesh_scratch(vsg_mesh* m) {
+  (void)hipFree(m->x.bucket_start);
+  (void)hipFree(m->x.oblk);
+  (void)hipFree(m->x.nbp);
+  (void)hipFree(m->x.bcount);
+  (void)hipFree(m->x.pstart);
+  (void)hipFree(m->x.sums);
+  m->x = ExtractScratch{};
+  m->x_buckets = m->x_blocks = 0;
+}
+
+vsg_status vsg_mesh_create(uint32_t max_triangles, vsg_mesh** out) {
+  if (!out) return fail(VSG_ERR_INVALID_ARG, "vsg_mesh_create: null argument");
+  *out = nullptr;
+  if (max_triangles == 0u || max_triangles > (1u << 28))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_mesh_create: max_triangles must be in [1, 2^28]");
+  int8_t tri[256][kMcRow];
+  const int most = build_mc_table(tri);
+  if (most < 0) return fail(VSG_ERR_NOT_READY, "vsg_mesh_create: table generation failed");
+  vsg_mesh* m = new vsg_mesh();
+  m->cap = max_triangles;
+  m->most = most;
+  vsg_status st = dmalloc(&m->verts, (size_t)max_triangles * 3 * sizeof(float3), "mesh vertices");
+  if (st == VSG_OK) st = dmalloc(&m->nrms, (size_t)max_triangles * 3 * sizeof(float3), "mesh normals");
+  if (st == VSG_OK) st = dmalloc(&m->table, sizeof(tri), "triangle table");
+  if (st == VSG_OK && hipMemcpy(m->table, tri, sizeof(tri), hipMemcpyHostToDevice) != hipSuccess)
+    st = fail(VSG_ERR_HIP, "hipMemcpy");
+  if (st == VSG_OK && hipHostMalloc((void**)&m->found, 2 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess) {
+    m->found = nullptr;
+    st = fail(VSG_ERR_OOM, "vsg_mesh_create: pinned host allocation failed");
+  }
+  if (st != VSG_OK) {
+    vsg_mesh_destroy(m);
+    return st;
+  }
+  *out = m;
+  return VSG_OK;
+}
+
+vsg_status vsg_mesh_destroy(vsg_mesh* m) {
+  if (!m) return fail(VSG_ERR_INVALID_ARG, "vsg_mesh_destroy: null mesh");
+  if (m->has_last) (void)hipStreamSynchronize(m->last);
+  (void)hipFree(m->verts);
+  (void)hipFree(m->nrms);
+  (void)hipFree(m->table);
+  free_mesh_scratch(m);
+  if (m->found) (void)hipHostFree(m->found);
+  delete m;
+  return VSG_OK;
+}
+
+vsg_status vsg_marching_cubes(vsg_mesh* m, vsg_scene* s, float iso, float voxel_size, int32_t cross_block,
+                              uint32_t min_weight, uint32_t* stored, uint64_t* total) {
+  if (!m || !s) return fail(VSG_ERR_INVALID_ARG, "vsg_marching_cubes: null argument");
+  if (std::isnan(iso) || std::isnan(voxel_size))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_marching_cubes: iso value and voxel size must not be NaN");
+  const uint32_t B = s->t.num_buckets, N = s->t.num_blocks;
+  hipStream_t strm = s->stream;
+  if (m->has_last) VSG_HIP(hipStreamSynchronize(m->last));  // outputs and scratch are reused
+  VSG_HIP(hipStreamSynchronize(strm));
+  uint32_t live = 0;
+  VSG_HIP(hipMemcpy(&live, &s->st->n_alloc, sizeof(live), hipMemcpyDeviceToHost));
+  // triangle counts and starts are 32-bit
+  if ((uint64_t)live * kVox * (uint64_t)m->most > 0xFFFFFFFFull)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_marching_cubes: the scene's blocks could hold 2^32 or more triangles");
+  if (m->x_buckets != B || m->x_blocks != N) {
+    free_mesh_scratch(m);
+    const uint32_t sums = extract_sums_words(B > N ? B : N);
+    vsg_status st = dmalloc(&m->x.bucket_start, (size_t)B * sizeof(uint32_t), "mesh bucket starts");
+    if (st == VSG_OK) st = dmalloc(&m->x.oblk, (size_t)N * sizeof(int4), "mesh block order");
+    if (st == VSG_OK) st = dmalloc(&m->x.nbp, (size_t)N * kExtNbWords * sizeof(int32_t), "mesh neighbour pointers");
+    if (st == VSG_OK) st = dmalloc(&m->x.bcount, (size_t)N * sizeof(uint32_t), "mesh block counts");
+    if (st == VSG_OK) st = dmalloc(&m->x.pstart, (size_t)N * sizeof(uint32_t), "mesh block starts");
+    if (st == VSG_OK) st = dmalloc(&m->x.sums, (size_t)sums * sizeof(uint32_t), "mesh scan sums");
+    if (st != VSG_OK) {
+      free_mesh_scratch(m);
+      return st;
+    }
+    m->x_buckets = B;
+    m->x_blocks = N;
+  }
+  const McArgs a{voxel_size > 0.0f ? voxel_size : s->p.voxel_size, iso, min_weight, cross_block ? 1u : 0u, m->cap};
+  {
+    TimedLaunch tl(strm, VSG_K_MESH);
+    launch_marching_cubes(strm, s->p.store == VSG_STORE_FP16, s->t, s->st, s->vox, m->table, a, m->x, m->verts,
+                          m->nrms);
+  }
+  vsg_status st = check_launch("marching cubes");
+  if (st != VSG_OK) return st;
+  VSG_HIP(hipMemcpyAsync(&m->found[0], m->x.pstart + (N - 1u), sizeof(uint32_t), hipMemcpyDeviceToHost, strm));
+  VSG_HIP(hipMemcpyAsync(&m->found[1], m->x.bcount + (N - 1u), sizeof(uint32_t), hipMemcpyDeviceToHost, strm));
+  VSG_HIP(hipStreamSynchronize(strm));
+  const uint64_t found = (uint64_t)m->found[0] + (uint64_t)m->found[1];
+  m->count = (uint32_t)(found < m->cap ? found : m->cap);
+  m->last = strm;
+  m->has_last = true;
+  if (stored) *stored = m->count;
+  if (total) *total = found;
+  return VSG_OK;
+}
+
+vsg_status vsg_mesh_buffers(const vsg_mesh* m, float** vertices, float** normals, uint32_t* count) {
+  if (!m) return fail(VSG_ERR_INVALID_ARG, "vsg_mesh_buffers: null mesh");
+  if (vertices) *vertices = (float*)m->verts;
+  if (normals) *normals = (float*)m->nrms;
+  if (count) *count = m->count;
+  return VSG_OK;
+}
+
+vsg_status vsg_mesh_download(vsg_mesh* m, float* vertices, float* normals, uint32_t capacity, uint32_t* count) {
+  if (!m) return fail(VSG_ERR_INVALID_ARG, "vsg_mesh_download: null mesh");
+  if (m->has_last) VSG_HIP(hipStreamSynchronize(m->last));
+  const size_t n = m->count < capacity ? m->count : capacity;
+  if (n && vertices) VSG_HIP(hipMemcpy(vertices, m->verts, n * 3 * sizeof(float3), hipMemcpyDeviceToHost));
+  if (n && normals) VSG_HIP(hipMemcpy(normals, m->nrms, n * 3 * sizeof(float3), hipMemcpyDeviceToHost));
+  if (count) *count = (uint32_t)n;
   return VSG_OK;
 }
 

@@ -90,6 +90,21 @@ void launch_extract(hipStream_t s, bool fp16, const TableView& t, const DevState
                     const uint32_t* col, float voxel, float min_sdf, float max_sdf, uint32_t min_weight,
                     uint32_t max_points, const ExtractScratch& x, float4* pos, float4* colr, float4* nrm);
 
+// Marching cubes (MarchingCubes.cu:352-467) over the scene's live blocks in hash-entry
+// order, then cube order, then table order: 3 vertices and 3 normals per triangle, the
+// first max_tris kept; the number found is pstart[num_blocks - 1] + bcount[num_blocks
+// - 1] once the work is done.  table: int8 [256][kMcRow] on the device.  The scratch's
+// ebits is unused; nbp only with cross-block cubes.
+struct McArgs {
+  float voxel, iso;
+  uint32_t min_weight;
+  uint32_t cross;  // 1: cubes at local 7 take corners from the +x / +y / +z neighbours
+  uint32_t max_tris;
+};
+void launch_marching_cubes(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const void* vox,
+                           const int8_t* table, const McArgs& a, const ExtractScratch& x, float3* verts,
+                           float3* nrms);
+
 // Depth map from a camera-frame point cloud (CustomDepthMapGenerator.cpp:81-222).
 struct DepthGenArgs {
   float fx, fy, cx, cy;

@@ -26,6 +26,7 @@
 #include "vsg_device.hpp"
 #include "vsg_internal.hpp"
 #include "vsg_div.hpp"
+#include "vsg_mc_table.hpp"
 #include "vsg_replay.hpp"
 
 namespace vsg {
@@ -2554,6 +2555,198 @@ __global__ __launch_bounds__(64) void k_ext_write(TableView t, const DevState* _
 }
 
 // ---------------------------------------------------------------------------------
+// Marching cubes (MarchingCubes.cu:313-467, runMarchingCubesCUDA :511-571)
+// ---------------------------------------------------------------------------------
+
+// One block's cube corners in LDS (one wave per block): SDF and usability of the voxels
+// at local (x, y, z) in 0..8 (x fastest, 9^3), the ninth layer from the +x / +y / +z
+// neighbours; a corner is usable when its block is allocated and its weight is at
+// least min_weight; and the triangle count of every case.
+constexpr int kMcG = kBlock + 1;
+constexpr int kMcGrid = kMcG * kMcG * kMcG;
+struct McBlock {
+  float sd[kMcGrid];
+  uint8_t ok[kMcGrid];
+  uint8_t ntri[256];
+};
+
+// Stages block k's corners: its own voxels coalesced (lane L, voxel L + 64 z), the
+// ninth layer (cross-block cubes only) through the neighbour pointers.
+template <bool F16>
+__device__ __forceinline__ void mc_stage(McBlock& g, const typename Store<F16>::word* __restrict__ vw,
+                                         const int32_t* __restrict__ nbp, uint32_t k, int4 b, const McArgs& a) {
+  using S = Store<F16>;
+  const uint32_t lane = threadIdx.x;
+  const size_t base = (size_t)b.w * kVox + lane;
+  typename S::word w[8];
+#pragma unroll
+  for (int z = 0; z < 8; ++z) w[z] = vw[base + 64u * (uint32_t)z];
+#pragma unroll
+  for (int z = 0; z < 8; ++z) {
+    const uint32_t l = lane + 64u * (uint32_t)z;
+    const uint32_t i = (l & 7u) + kMcG * ((l >> 3) & 7u) + kMcG * kMcG * (l >> 6);
+    g.sd[i] = S::sdf(w[z]);
+    g.ok[i] = S::weight(w[z]) >= a.min_weight ? 1 : 0;
+  }
+  if (a.cross) {
+    for (uint32_t i = lane; i < (uint32_t)kMcGrid; i += 64u) {
+      const uint32_t x = i % kMcG, y = (i / kMcG) % kMcG, z = i / (kMcG * kMcG);
+      if (x < 8u && y < 8u && z < 8u) continue;
+      const int32_t q = nbp[(size_t)k * kExtNb + 13u + (x >> 3) + 3u * (y >> 3) + 9u * (z >> 3)];
+      if (q < 0) {
+        g.sd[i] = 0.0f;
+        g.ok[i] = 0;
+        continue;
+      }
+      const typename S::word v = vw[(size_t)q * kVox + (x & 7u) + kBlock * (y & 7u) + kBlock * kBlock * (z & 7u)];
+      g.sd[i] = S::sdf(v);
+      g.ok[i] = S::weight(v) >= a.min_weight ? 1 : 0;
+    }
+  }
+}
+
+// Case of cube l = x + 8 y + 64 z of the staged block (bit c: corner c = (c & 1,
+// c >> 1 & 1, c >> 2 & 1) has sdf < iso), or -1 when the cube is not marched: beyond
+// local 6 without cross-block cubes, or a corner not usable.
+__device__ __forceinline__ int mc_case(const McBlock& g, uint32_t l, const McArgs& a) {
+  const uint32_t x = l & 7u, y = (l >> 3) & 7u, z = l >> 6;
+  if (!a.cross && (x == 7u || y == 7u || z == 7u)) return -1;
+  const uint32_t i0 = x + kMcG * y + kMcG * kMcG * z;
+  int k = 0;
+  bool ok = true;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const uint32_t i = i0 + (uint32_t)(c & 1) + kMcG * (uint32_t)((c >> 1) & 1) + kMcG * kMcG * (uint32_t)(c >> 2);
+    ok = ok && g.ok[i] != 0;
+    k |= (g.sd[i] < a.iso ? 1 : 0) << c;
+  }
+  return ok ? k : -1;
+}
+
+// Triangles per block (bcount; zero for the block slots beyond the live blocks).
+template <bool F16>
+__global__ __launch_bounds__(64) void k_mc_count(const DevState* __restrict__ st, uint32_t num_blocks,
+                                                 const int4* __restrict__ oblk, const int32_t* __restrict__ nbp,
+                                                 const void* __restrict__ vox, const int8_t* __restrict__ table,
+                                                 McArgs a, uint32_t* __restrict__ bcount) {
+  __shared__ McBlock g;
+  const uint32_t lane = threadIdx.x, G = gridDim.x, nb = st->n_alloc;
+  for (uint32_t j = lane; j < 256u; j += 64u) {
+    int n = 0;
+    while (n < 10 && table[j * kMcRow + 3u * (uint32_t)n] >= 0) ++n;
+    g.ntri[j] = (uint8_t)n;
+  }
+  const auto* vw = reinterpret_cast<const typename Store<F16>::word*>(vox);
+  for (uint32_t k = blockIdx.x; k < nb; k += G) {
+    __syncthreads();  // g is read below until every lane is done
+    mc_stage<F16>(g, vw, nbp, k, oblk[k], a);
+    __syncthreads();
+    uint32_t c = 0;
+#pragma unroll
+    for (int z = 0; z < 8; ++z) {
+      const int cs = mc_case(g, lane + 64u * (uint32_t)z, a);
+      c += cs >= 0 ? g.ntri[cs] : 0u;
+    }
+    c = wave_sum(c);
+    if (lane == 0) bcount[k] = c;
+  }
+  for (uint32_t j = nb + blockIdx.x * 64u + lane; j < num_blocks; j += G * 64u) bcount[j] = 0u;
+}
+
+// Corner position along one axis: voxel centre (8 block + i + 0.5) v of local grid
+// coordinate gi (0..8) of block coordinate bc, taken in the block that holds the voxel,
+// so that neighbouring blocks compute the same position.
+__device__ __forceinline__ float mc_coord(int32_t bc, uint32_t gi, float v) {
+  return ext_centre(gi == 8u ? bc + 1 : bc, (int)(gi & 7u), v);
+}
+
+// vertexInterp (MarchingCubes.cu:313-327) on edge e of cube (x, y, z): from its lower
+// corner a to its upper corner b (the edge's axis e >> 2; corners a = the two other
+// coordinates' bits e & 3 with a zero inserted at the axis, b = a + that axis' bit).
+__device__ __forceinline__ float3 mc_vertex(const McBlock& g, int4 b, uint32_t x, uint32_t y, uint32_t z, int e,
+                                            const McArgs& a) {
+  const int axis = e >> 2, j = e & 3;
+  const int ca = ((j >> axis) << (axis + 1)) | (j & ((1 << axis) - 1));
+  const int cb = ca | (1 << axis);
+  const uint32_t ax = x + (uint32_t)(ca & 1), ay = y + (uint32_t)((ca >> 1) & 1), az = z + (uint32_t)(ca >> 2);
+  const uint32_t bx = x + (uint32_t)(cb & 1), by = y + (uint32_t)((cb >> 1) & 1), bz = z + (uint32_t)(cb >> 2);
+  const float sa = g.sd[ax + kMcG * ay + kMcG * kMcG * az], sb = g.sd[bx + kMcG * by + kMcG * kMcG * bz];
+  const float p1x = mc_coord(b.x, ax, a.voxel), p1y = mc_coord(b.y, ay, a.voxel), p1z = mc_coord(b.z, az, a.voxel);
+  const float p2x = mc_coord(b.x, bx, a.voxel), p2y = mc_coord(b.y, by, a.voxel), p2z = mc_coord(b.z, bz, a.voxel);
+  const float den = sb - sa;
+  if (fabsf(den) < 1e-6f) return make_float3(0.5f * (p1x + p2x), 0.5f * (p1y + p2y), 0.5f * (p1z + p2z));
+  float mu = (a.iso - sa) / den;
+  mu = fminf(fmaxf(mu, 0.0f), 1.0f);
+  return make_float3(p1x + mu * (p2x - p1x), p1y + mu * (p2y - p1y), p1z + mu * (p2z - p1z));
+}
+
+// The triangles: block k's start pstart[k], then cube order, then table order; three
+// vertices and three copies of the face normal (computeNormal, MarchingCubes.cu:329-348)
+// per triangle; the first max_tris kept.
+template <bool F16>
+__global__ __launch_bounds__(64) void k_mc_write(const DevState* __restrict__ st, const int4* __restrict__ oblk,
+                                                 const int32_t* __restrict__ nbp, const void* __restrict__ vox,
+                                                 const int8_t* __restrict__ table, McArgs a,
+                                                 const uint32_t* __restrict__ bcount,
+                                                 const uint32_t* __restrict__ pstart, float3* __restrict__ out_v,
+                                                 float3* __restrict__ out_n) {
+  __shared__ McBlock g;
+  const uint32_t lane = threadIdx.x, G = gridDim.x, nb = st->n_alloc;
+  for (uint32_t j = lane; j < 256u; j += 64u) {
+    int n = 0;
+    while (n < 10 && table[j * kMcRow + 3u * (uint32_t)n] >= 0) ++n;
+    g.ntri[j] = (uint8_t)n;
+  }
+  const auto* vw = reinterpret_cast<const typename Store<F16>::word*>(vox);
+  for (uint32_t k = blockIdx.x; k < nb; k += G) {
+    const uint32_t start = pstart[k];
+    if (start >= a.max_tris) break;  // starts do not decrease with k
+    if (bcount[k] == 0u) continue;
+    const int4 b = oblk[k];
+    __syncthreads();  // g is read below until every lane is done
+    mc_stage<F16>(g, vw, nbp, k, b, a);
+    __syncthreads();
+    uint32_t run = start;
+    for (int z = 0; z < 8; ++z) {
+      const uint32_t l = lane + 64u * (uint32_t)z;
+      const int cs = mc_case(g, l, a);
+      const uint32_t c = cs >= 0 ? g.ntri[cs] : 0u;
+      const uint32_t inc = wave_incl_scan(c);
+      const uint32_t first = run + inc - c;
+      run += __shfl(inc, 63, 64);
+      const uint32_t x = l & 7u, y = (l >> 3) & 7u, zz = l >> 6;
+      for (uint32_t t = 0; t < c; ++t) {
+        const uint32_t o = first + t;
+        if (o >= a.max_tris) break;
+        const int8_t* row = table + (size_t)cs * kMcRow + 3u * t;
+        const float3 v0 = mc_vertex(g, b, x, y, zz, row[0], a);
+        const float3 v1 = mc_vertex(g, b, x, y, zz, row[1], a);
+        const float3 v2 = mc_vertex(g, b, x, y, zz, row[2], a);
+        const float ax = v1.x - v0.x, ay = v1.y - v0.y, az = v1.z - v0.z;
+        const float bx = v2.x - v0.x, by = v2.y - v0.y, bz = v2.z - v0.z;
+        float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+        const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+        if (len > 1e-6f) {
+          const float inv = 1.0f / len;
+          nx *= inv;
+          ny *= inv;
+          nz *= inv;
+        } else {
+          nx = ny = nz = 0.0f;
+        }
+        const float3 n = make_float3(nx, ny, nz);
+        out_v[3u * o] = v0;
+        out_v[3u * o + 1u] = v1;
+        out_v[3u * o + 2u] = v2;
+        out_n[3u * o] = n;
+        out_n[3u * o + 1u] = n;
+        out_n[3u * o + 2u] = n;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------
 // Depth maps from point clouds (CustomDepthMapGenerator.cpp:81-222)
 // ---------------------------------------------------------------------------------
 
@@ -2869,6 +3062,32 @@ void launch_extract(hipStream_t s, bool fp16, const TableView& t, const DevState
   else
     hipLaunchKernelGGL(k_ext_write<false>, grid_of(resident_workgroups(k_ext_write<false>, 64)), dim3(64), 0, s, t,
                        st, x.oblk, x.nbp, vox, col, a, x.bcount, x.ebits, x.pstart, pos, colr, nrm);
+}
+
+void launch_marching_cubes(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const void* vox,
+                           const int8_t* table, const McArgs& a, const ExtractScratch& x, float3* verts,
+                           float3* nrms) {
+  launch_scan<uint8_t>(s, t.cnt, t.num_buckets, x.sums, x.bucket_start);
+  hipLaunchKernelGGL(k_ext_order, dim3((t.num_blocks + 255u) / 256u), dim3(256), 0, s, t, st, x.bucket_start, x.oblk);
+  if (a.cross) {
+    const uint64_t nthreads = (uint64_t)t.num_blocks * kExtNb;
+    hipLaunchKernelGGL(k_ext_neighbours, dim3((uint32_t)((nthreads + 255u) / 256u)), dim3(256), 0, s, t, st, x.oblk,
+                       x.nbp);
+  }
+  auto grid_of = [&](uint32_t resident) { return dim3(t.num_blocks < resident ? t.num_blocks : resident); };
+  if (fp16)
+    hipLaunchKernelGGL(k_mc_count<true>, grid_of(resident_workgroups(k_mc_count<true>, 64)), dim3(64), 0, s, st,
+                       t.num_blocks, x.oblk, x.nbp, vox, table, a, x.bcount);
+  else
+    hipLaunchKernelGGL(k_mc_count<false>, grid_of(resident_workgroups(k_mc_count<false>, 64)), dim3(64), 0, s, st,
+                       t.num_blocks, x.oblk, x.nbp, vox, table, a, x.bcount);
+  launch_scan<uint32_t>(s, x.bcount, t.num_blocks, x.sums, x.pstart);
+  if (fp16)
+    hipLaunchKernelGGL(k_mc_write<true>, grid_of(resident_workgroups(k_mc_write<true>, 64)), dim3(64), 0, s, st,
+                       x.oblk, x.nbp, vox, table, a, x.bcount, x.pstart, verts, nrms);
+  else
+    hipLaunchKernelGGL(k_mc_write<false>, grid_of(resident_workgroups(k_mc_write<false>, 64)), dim3(64), 0, s, st,
+                       x.oblk, x.nbp, vox, table, a, x.bcount, x.pstart, verts, nrms);
 }
 
 void launch_depthgen_count(hipStream_t s, const uint32_t* win, uint32_t npix, uint32_t* valid) {

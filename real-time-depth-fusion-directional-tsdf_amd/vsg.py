@@ -28,7 +28,7 @@ STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "NOT_READY"}
 STORE_FP32 = 0
 STORE_FP16 = 1
 KERNELS = ["alloc_claim", "alloc_resolve", "alloc_scan", "alloc_insert", "compact", "integrate", "raycast",
-           "reset", "adjoint", "gauss_newton", "extract", "depthgen"]
+           "reset", "adjoint", "gauss_newton", "extract", "depthgen", "mesh"]
 
 SDF_BLOCK_SIZE = 8      # VoxelSG/include/globalPramasConfig.h:8
 HASH_BUCKET_SIZE = 10   # VoxelSG/include/globalPramasConfig.h:9
@@ -111,6 +111,13 @@ _SIGNATURES = {
                                            C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
                                            C.c_void_p]),
     "vsg_depthgen_valid_pixels": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "vsg_mc_table": (C.c_int, [C.c_void_p]),
+    "vsg_mesh_create": (C.c_int, [C.c_uint32, C.c_void_p]),
+    "vsg_mesh_destroy": (C.c_int, [C.c_void_p]),
+    "vsg_marching_cubes": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_float, C.c_int32, C.c_uint32, C.c_void_p,
+                                     C.c_void_p]),
+    "vsg_mesh_buffers": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "vsg_mesh_download": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "vsg_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_render_set_gn_jacobian": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsg_render_set_gn_noise": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
@@ -307,10 +314,30 @@ class VoxelScene:
             print(f"Free Blocks: {free} / {p.SDFBlockNum} ({free * 100.0 / p.SDFBlockNum}%)")
         print("========================\n")
 
-    def saveToMesh(self, filename: str) -> bool:
-        """Stub in the reference (VoxelSG/src/VoxelScene.cpp:838-845): returns False."""
-        print("VoxelScene::saveToMesh: Not implemented yet", file=sys.stderr)
-        return False
+    def saveToMesh(self, filename: str, iso: float = 0.0, min_weight: int = 1) -> bool:
+        """VoxelScene::saveToMesh (a stub in the reference, VoxelSG/src/VoxelScene.cpp:838-845):
+        marching cubes over the whole scene, cubes across block borders included and cubes
+        with a corner of weight below min_weight left out, written as a binary PLY with the
+        vertices shared by neighbouring triangles merged (equal float3) and one face per
+        triangle.  False (and a message) when the scene has no triangles."""
+        out = MarchingCubesOutput()
+        allocateMarchingCubesOutput(out, 1 << 20)
+        try:
+            mc = MarchingCubesParams()
+            mc.isoValue = iso
+            mc.voxelSize = 0.0  # the scene's
+            total = runMarchingCubesCUDA(self, self.m_params, mc, out, cross_block=True, min_weight=min_weight)
+            if total > out.maxTriangles:
+                allocateMarchingCubesOutput(out, total)
+                total = runMarchingCubesCUDA(self, self.m_params, mc, out, cross_block=True, min_weight=min_weight)
+            if total == 0:
+                print("VoxelScene::saveToMesh: no triangles", file=sys.stderr)
+                return False
+            verts = downloadMarchingCubes(out)["verts"]
+        finally:
+            freeMarchingCubesOutput(out)
+        write_mesh_ply(filename, verts)
+        return True
 
     # -- extensions -----------------------------------------------------------
     def set_stream(self, hip_stream: int) -> None:
@@ -660,6 +687,135 @@ class RayCastRender:
         return self.m_isAllocated
 
 
+# ---- marching cubes (VoxelSG/include/MarchingCubes.h, VoxelSG/src/MarchingCubes.cu) ------------
+
+class MarchingCubesParams:
+    """MarchingCubesParams (MarchingCubes.h:7-16), same defaults; voxelSize <= 0 selects the
+    scene's voxel size (runMarchingCubesCUDA uses voxelSize when it is positive, and the
+    default is 1.0)."""
+
+    def __init__(self):
+        self.isoValue = 0.0
+        self.voxelSize = 1.0
+        self.maxTriangles = 0
+
+
+class MarchingCubesOutput:
+    """MarchingCubesOutput (MarchingCubes.h:18-29): device buffers of float3 [3 * maxTriangles]
+    (d_vertices, d_normals as device addresses) and the triangle count of the last run."""
+
+    def __init__(self):
+        self.d_vertices = 0
+        self.d_normals = 0
+        self.maxTriangles = 0
+        self.triangleCount = 0
+        self._h = C.c_void_p()
+
+
+def allocateMarchingCubesOutput(output: MarchingCubesOutput, maxTriangles: int) -> None:
+    """allocateMarchingCubesOutput (MarchingCubes.cu:474-489)."""
+    if maxTriangles <= 0:
+        print("allocateMarchingCubesOutput: maxTriangles must be positive.", file=sys.stderr)
+        return
+    freeMarchingCubesOutput(output)
+    h = C.c_void_p()
+    _check(lib().vsg_mesh_create(int(maxTriangles), C.byref(h)))
+    output._h = h
+    v, n = C.c_void_p(), C.c_void_p()
+    _check(lib().vsg_mesh_buffers(h, C.byref(v), C.byref(n), None))
+    output.d_vertices, output.d_normals = v.value or 0, n.value or 0
+    output.maxTriangles = int(maxTriangles)
+    output.triangleCount = 0
+
+
+def freeMarchingCubesOutput(output: MarchingCubesOutput) -> None:
+    """freeMarchingCubesOutput (MarchingCubes.cu:491-501)."""
+    if output._h and output._h.value:
+        _check(lib().vsg_mesh_destroy(output._h))
+    output._h = C.c_void_p()
+    output.d_vertices = output.d_normals = 0
+    output.maxTriangles = 0
+    output.triangleCount = 0
+
+
+def runMarchingCubesCUDA(scene: "VoxelScene", params: "Params", paramsMC: MarchingCubesParams,
+                         output: MarchingCubesOutput, cross_block: bool = False, min_weight: int = 0) -> int:
+    """runMarchingCubesCUDA (MarchingCubes.cu:511-571) over every live block of the scene in
+    hash-entry, cube and table order; voxel size paramsMC.voxelSize when positive, else the
+    scene's (params.voxelSize).  cross_block adds the cubes across block borders (the
+    reference's stay inside one block); min_weight > 0 leaves out cubes with a lighter
+    corner.  Returns the triangles found (the reference's counter); min(found,
+    maxTriangles) are stored."""
+    if not (output._h and output._h.value):
+        print("runMarchingCubesCUDA: Output buffers not allocated.", file=sys.stderr)
+        return 0
+    voxel = paramsMC.voxelSize if paramsMC.voxelSize > 0 else params.voxelSize
+    stored, total = C.c_uint32(), C.c_uint64()
+    _check(lib().vsg_marching_cubes(output._h, scene._h, float(paramsMC.isoValue), float(voxel), int(bool(cross_block)),
+                                    int(min_weight), C.byref(stored), C.byref(total)))
+    output.triangleCount = int(total.value)
+    return int(total.value)
+
+
+def copyMarchingCubesTriangleCount(output: MarchingCubesOutput) -> int:
+    """copyMarchingCubesTriangleCount (MarchingCubes.cu:503-509): triangles found by the last run."""
+    return output.triangleCount
+
+
+def downloadMarchingCubes(output: MarchingCubesOutput) -> dict:
+    """The stored triangles: vertices and normals float32 [n, 3, 3] (triangle, vertex, xyz)."""
+    n = min(output.triangleCount, output.maxTriangles)
+    verts = np.zeros((n, 3, 3), np.float32)
+    nrms = np.zeros((n, 3, 3), np.float32)
+    got = C.c_uint32()
+    _check(lib().vsg_mesh_download(output._h, verts.ctypes.data, nrms.ctypes.data, n, C.byref(got)))
+    assert got.value == n
+    return {"verts": verts, "nrms": nrms}
+
+
+def weld_vertices(verts: np.ndarray):
+    """(unique vertices float32 [m, 3] in first-use order, faces int32 [n, 3]) of a triangle
+    soup [n, 3, 3]: vertices with equal bits become one."""
+    flat = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    keys = flat.view(np.uint32).view(np.dtype((np.void, 12))).reshape(-1)
+    _, first, inv = np.unique(keys, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(len(order))
+    return flat[first[order]], rank[inv.reshape(-1)].reshape(-1, 3).astype(np.int32)
+
+
+def write_mesh_ply(path: str, verts: np.ndarray) -> None:
+    """Binary little-endian PLY of a triangle soup [n, 3, 3]: welded vertices (float x y z)
+    and faces (uchar count 3, int indices)."""
+    v, f = weld_vertices(verts)
+    d = os.path.dirname(os.path.abspath(path))
+    os.makedirs(d, exist_ok=True)
+    face = np.zeros(len(f), dtype=[("n", "u1"), ("i", "<i4", (3,))])
+    face["n"] = 3
+    face["i"] = f
+    with open(path, "wb") as fh:
+        fh.write((f"ply\nformat binary_little_endian 1.0\ncomment Generated by VoxelScene::saveToMesh\n"
+                  f"element vertex {len(v)}\nproperty float x\nproperty float y\nproperty float z\n"
+                  f"element face {len(f)}\nproperty list uchar int vertex_indices\nend_header\n").encode())
+        fh.write(np.ascontiguousarray(v, "<f4").tobytes())
+        fh.write(face.tobytes())
+
+
+def read_mesh_ply(path: str):
+    """(vertices float32 [m, 3], faces int32 [n, 3]) of a PLY written by write_mesh_ply."""
+    with open(path, "rb") as fh:
+        data = fh.read()
+    end = data.index(b"end_header\n") + len(b"end_header\n")
+    head = data[:end].decode().split("\n")
+    nv = int(next(line for line in head if line.startswith("element vertex")).split()[-1])
+    nf = int(next(line for line in head if line.startswith("element face")).split()[-1])
+    v = np.frombuffer(data, "<f4", 3 * nv, end).reshape(nv, 3)
+    face = np.frombuffer(data, [("n", "u1"), ("i", "<i4", (3,))], nf, end + 12 * nv)
+    assert np.all(face["n"] == 3)
+    return v.copy(), face["i"].copy()
+
+
 class CameraParams:
     """CustomDepthMapGenerator::CameraParams (VoxelSG/include/CustomDepthMapGenerator.h:18-27),
     with the reference's defaults."""
@@ -805,6 +961,13 @@ class CustomDepthMapGenerator:
         cat = [np.concatenate([np.asarray(p[j] if p[j] is not None else np.zeros((0, w)), dt).reshape(-1, w)
                                for p in parts]) for j, (dt, w) in enumerate(shapes)]
         return self.generateFromPointCloud(*cat)
+
+
+def mc_table() -> np.ndarray:
+    """The marching-cubes triangle table, int8 [256, 32] (edge triples, then -1)."""
+    out = np.zeros((256, 32), np.int8)
+    _check(lib().vsg_mc_table(out.ctypes.data))
+    return out
 
 
 def diag_convert(values) -> np.ndarray:

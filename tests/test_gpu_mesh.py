@@ -1,0 +1,149 @@
+"""Marching cubes (runMarchingCubesCUDA / marchingCubesKernel, MarchingCubes.cu:352-571)
+on the GPU vs the oracle, through the C ABI, and VoxelScene::saveToMesh.
+
+Bar: the same number of triangles; vertices and normals bit-identical (float32) in the
+same order (hash entry, cube, table).  The reference appends triangles in atomicAdd
+order, so its order is not reproducible; the set of triangles is what it defines.  At
+the 640x480 room's full size (no oracle run): every triangle side appears at most once
+per direction (consistent orientation), the reference's in-block cubes give a subset of
+the cross-block triangles, and the stored prefix does not depend on the capacity.
+"""
+import numpy as np
+import pytest
+
+from fusion_util import GpuRun, OracleRun
+from vsg_amd import synth, vsg
+
+pytestmark = pytest.mark.gpu
+
+
+def _integrated(torch_cuda, wl_id, frames, store=None, oracle=True):
+    wl = synth.WORKLOADS[wl_id]
+    sc = synth.SynthScene(wl_id)
+    g = GpuRun(torch_cuda, wl, store=store)
+    o = OracleRun(wl, store=store) if oracle else None
+    for i in frames:
+        f = sc.render(i)
+        g.integrate(f, sc.intrinsics)
+        if o is not None:
+            o.integrate(f, sc.intrinsics)
+    return g, o, sc
+
+
+def _run(g, out, iso=0.0, cross=False, min_weight=0, voxel=0.0):
+    mc = vsg.MarchingCubesParams()
+    mc.isoValue = iso
+    mc.voxelSize = voxel
+    total = vsg.runMarchingCubesCUDA(g.scene, g.params, mc, out, cross_block=cross, min_weight=min_weight)
+    assert vsg.copyMarchingCubesTriangleCount(out) == total
+    return total, vsg.downloadMarchingCubes(out)
+
+
+def _same(a, b, what):
+    a = np.ascontiguousarray(a, np.float32)
+    b = np.ascontiguousarray(b.astype(np.float32))
+    assert a.shape == b.shape, (what, a.shape, b.shape)
+    same = (a.view(np.uint32) == b.view(np.uint32)).reshape(len(a), -1).all(axis=1)
+    assert same.all(), f"{what} differ at {int((~same).sum())} triangles, first {int(np.argmin(same))}"
+
+
+@pytest.mark.parametrize("store", ["fp32", "fp16"])
+def test_sphere_arc_mesh_matches_oracle(torch_cuda, store):
+    """Workload 0 after 10 frames: the reference's cubes (inside blocks, no weight test)
+    and cross-block cubes with weight >= 1, at iso 0 and iso 0.002."""
+    g, o, _ = _integrated(torch_cuda, 0, range(10), store)
+    out = vsg.MarchingCubesOutput()
+    vsg.allocateMarchingCubesOutput(out, 1 << 22)
+    T = vsg.mc_table()
+    for iso, cross, mw in ((0.0, False, 0), (0.0, True, 1), (0.002, True, 0)):
+        total, got = _run(g, out, iso, cross, mw)
+        ref = o.o.marching_cubes(T, iso=iso, cross_block=cross, min_weight=mw)
+        assert total == ref["total"] > 1000, (iso, cross, mw, total, ref["total"])
+        _same(got["verts"], ref["verts"], "vertices")
+        _same(got["nrms"], ref["nrms"], "normals")
+        print(f"sphere {store} iso {iso} cross {cross} w>={mw}: {total} triangles, bit-identical")
+    vsg.freeMarchingCubesOutput(out)
+
+
+def test_room_mesh_matches_oracle(torch_cuda):
+    """Workload 1 (640x480 room, 2^20 buckets) after 4 frames, cross-block cubes with
+    weight >= 1, the scene's voxel size and an explicit one."""
+    g, o, _ = _integrated(torch_cuda, 1, range(4))
+    out = vsg.MarchingCubesOutput()
+    vsg.allocateMarchingCubesOutput(out, 1 << 22)
+    T = vsg.mc_table()
+    for voxel in (0.0, 0.01):
+        total, got = _run(g, out, cross=True, min_weight=1, voxel=voxel)
+        ref = o.o.marching_cubes(T, cross_block=True, min_weight=1, voxel=(voxel or None))
+        assert total == ref["total"] > 100000
+        _same(got["verts"], ref["verts"], "vertices")
+        _same(got["nrms"], ref["nrms"], "normals")
+        print(f"room voxel {voxel}: {total} triangles, bit-identical")
+    vsg.freeMarchingCubesOutput(out)
+
+
+def test_triangle_cap_and_empty_scene(torch_cuda):
+    g, o, _ = _integrated(torch_cuda, 0, range(3))
+    T = vsg.mc_table()
+    full = o.o.marching_cubes(T, cross_block=True)
+    for cap in (1, 777, full["total"], full["total"] + 5):
+        out = vsg.MarchingCubesOutput()
+        vsg.allocateMarchingCubesOutput(out, cap)
+        total, got = _run(g, out, cross=True)
+        n = min(cap, total)
+        assert total == full["total"] and len(got["verts"]) == n
+        _same(got["verts"], full["verts"][:n], "vertices")
+        _same(got["nrms"], full["nrms"][:n], "normals")
+        vsg.freeMarchingCubesOutput(out)
+    out = vsg.MarchingCubesOutput()
+    vsg.allocateMarchingCubesOutput(out, 16)
+    assert _run(g, out, min_weight=10_000)[0] == 0
+    empty = GpuRun(torch_cuda, synth.WORKLOADS[0])
+    total, got = _run(empty, out)
+    assert total == 0 and got["verts"].shape == (0, 3, 3)
+    vsg.freeMarchingCubesOutput(out)
+
+
+def _sides(verts):
+    """Directed triangle sides as rows of 6 uint32 (the two vertices' bits)."""
+    v = np.ascontiguousarray(verts, np.float32).view(np.uint32).reshape(-1, 3, 3)
+    return np.concatenate([np.concatenate([v[:, i], v[:, (i + 1) % 3]], 1) for i in range(3)])
+
+
+def _rows(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.dtype((np.void, a.dtype.itemsize * a.shape[1]))).reshape(-1)
+
+
+def test_room_full_size_mesh_properties(torch_cuda, tmp_path):
+    """The 640x480 room after 30 frames (no oracle): consistent orientation, in-block
+    triangles a subset of the cross-block ones, saveToMesh's PLY holds the same faces."""
+    g, _, _ = _integrated(torch_cuda, 1, range(30), oracle=False)
+    out = vsg.MarchingCubesOutput()
+    vsg.allocateMarchingCubesOutput(out, 1 << 23)
+    total, got = _run(g, out, cross=True, min_weight=1)
+    assert 100000 < total <= out.maxTriangles
+    # a corner with sdf == iso puts the vertices of all its edges on it (mu = 0 or 1),
+    # where sheets of several cubes meet: leave out triangles with a vertex within 1e-3
+    # voxels of a voxel centre
+    q = got["verts"].astype(np.float64) / g.wl.voxel - 0.5
+    at_corner = (np.abs(q - np.round(q)) < 1e-3).all(axis=-1).any(axis=-1)
+    vt = got["verts"][~at_corner]
+    s = _rows(_sides(vt))
+    u, c = np.unique(s, return_counts=True)
+    assert c.max() == 1, "a side traversed twice in one direction"
+    rev = _rows(_sides(vt)[:, [3, 4, 5, 0, 1, 2]])
+    paired = np.isin(rev, u).mean()
+    print(f"room 30 frames: {total} triangles ({int(at_corner.sum())} at voxel centres), "
+          f"{paired:.4f} of the other sides paired")
+    assert at_corner.mean() < 0.01 and paired > 0.98
+    t_in, got_in = _run(g, out, cross=False, min_weight=1)
+    assert 0 < t_in < total
+    assert np.isin(_rows(got_in["verts"].reshape(-1, 9)), _rows(got["verts"].reshape(-1, 9))).all()
+    vsg.freeMarchingCubesOutput(out)
+    path = str(tmp_path / "room.ply")
+    assert g.scene.saveToMesh(path)
+    v, f = vsg.read_mesh_ply(path)
+    assert len(f) == total
+    w, fw = vsg.weld_vertices(got["verts"])
+    assert np.array_equal(v, w) and np.array_equal(f, fw)
