@@ -362,6 +362,8 @@ def main() -> int:
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-extract", action="store_true",
                     help="skip timing the surface-point extraction of the fused scene")
+    ap.add_argument("--no-mesh", action="store_true",
+                    help="skip timing the marching cubes of the fused scene")
     ap.add_argument("--no-depthgen", action="store_true",
                     help="skip timing the depth maps from a point cloud")
     ap.add_argument("--no-pose", action="store_true",
@@ -486,6 +488,35 @@ def main() -> int:
             ext = {"points": npts, "blocks": n_alloc, "avg_ms": round(ems / en, 4), "wall_ms": round(wall * 1e3, 4),
                    "algorithmic_bytes": int(ebytes), "gbs": round(egbs, 1), "frac": round(egbs / HBM_PEAK_GBS, 5)}
             log(f"[{store}] extraction: {npts} points from {n_alloc} blocks in {ems / en:.3f} ms")
+        mesh = None
+        if not args.no_mesh:
+            # marching cubes of the fused scene, outside the timed region: the reference's
+            # cubes (inside blocks, no weight test) and saveToMesh's (cross-block cubes,
+            # weight >= 1); algorithmic bytes: the voxel store once and 72 bytes per
+            # triangle (3 float3 vertices, 3 float3 normals)
+            n_alloc = scene.counters()["num_allocated"]
+            out = vsg.MarchingCubesOutput()
+            vsg.allocateMarchingCubesOutput(out, 1 << 23)
+            mcp = vsg.MarchingCubesParams()
+            mcp.voxelSize = 0.0
+            mesh = {}
+            for name, cross, mw in (("in_block", False, 0), ("cross_block_w1", True, 1)):
+                vsg.runMarchingCubesCUDA(scene, scene.m_params, mcp, out, cross_block=cross, min_weight=mw)
+                torch.cuda.synchronize()
+                vsg.timing_reset()
+                reps = 10
+                t0 = time.perf_counter()
+                for _ in range(reps):
+                    ntri = vsg.runMarchingCubesCUDA(scene, scene.m_params, mcp, out, cross_block=cross, min_weight=mw)
+                wall = (time.perf_counter() - t0) / reps
+                mms, mn = vsg.timing_read()["mesh"]
+                mbytes = n_alloc * 512 * b_sw + min(ntri, out.maxTriangles) * 72
+                mgbs = mbytes / (mms / mn * 1e-3) / 1e9
+                mesh[name] = {"triangles": ntri, "blocks": n_alloc, "avg_ms": round(mms / mn, 4),
+                              "wall_ms": round(wall * 1e3, 4), "algorithmic_bytes": int(mbytes),
+                              "gbs": round(mgbs, 1), "frac": round(mgbs / HBM_PEAK_GBS, 5)}
+                log(f"[{store}] marching cubes {name}: {ntri} triangles in {mms / mn:.3f} ms")
+            vsg.freeMarchingCubesOutput(out)
         vsg.timing_enable(False)
         scene.close()
         rr.close()
@@ -496,7 +527,8 @@ def main() -> int:
                 kb[k] += v
             fb += frame_bytes(st, npix, b_sw)
         log(f"[{store}] {t_total:.3f} s for {args.steps * nf} frames")
-        return {"t_total": t_total, "ktimes": ktimes, "kbytes": kb, "fbytes": fb, "stats": stats, "extract": ext}
+        return {"t_total": t_total, "ktimes": ktimes, "kbytes": kb, "fbytes": fb, "stats": stats, "extract": ext,
+                "mesh": mesh}
 
     res = measure(args.store)
     t_total, ktimes = res["t_total"], res["ktimes"]
@@ -523,7 +555,7 @@ def main() -> int:
     fused_gbs = res["fbytes"] * args.steps / t_total / 1e9
     per_kernel = {}
     for k, (ms, n) in ktimes.items():
-        if n == 0 or k == "extract":
+        if n == 0 or k in ("extract", "mesh"):
             continue
         e = {"avg_ms": round(ms / n, 4), "launches": n, "share": round(ms / sum(kern_ms.values()), 4)}
         if k in res["kbytes"]:
@@ -547,6 +579,8 @@ def main() -> int:
     }
     if res["extract"] is not None:
         extra["surface_points"] = res["extract"]
+    if res["mesh"] is not None:
+        extra["marching_cubes"] = res["mesh"]
     if not args.no_depthgen:
         extra["depth_from_points"] = depthgen_bench(torch, vsg, sc, frames)
     if not args.no_pose and nf >= 128:
@@ -561,6 +595,8 @@ def main() -> int:
                                "ms_per_frame": round(r16["t_total"] / frames_done * 1e3, 4)}
         if r16["extract"] is not None:
             extra["fp16_store"]["surface_points"] = r16["extract"]
+        if r16["mesh"] is not None:
+            extra["fp16_store"]["marching_cubes"] = r16["mesh"]
     if not args.no_precision:
         log(f"precision: FP32 and FP16 stores vs the FP64 oracle on {args.precision_frames} frames")
         extra["precision_vs_fp64"] = precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm,

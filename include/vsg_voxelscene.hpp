@@ -20,7 +20,9 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "vsg.h"
@@ -62,6 +64,51 @@ inline bool report(vsg_status st, const char* where) {
   if (st == VSG_OK) return true;
   std::fprintf(stderr, "%s: %s\n", where, vsg_last_error());
   return false;
+}
+
+// Binary little-endian PLY of a triangle soup (9 floats per triangle): vertices with
+// equal bits merged in first-use order, faces as (uchar 3, int indices).
+inline bool write_mesh_ply(const std::string& path, const float* tri, uint32_t n) {
+  struct Key {
+    uint32_t b[3];
+    bool operator==(const Key& o) const { return b[0] == o.b[0] && b[1] == o.b[1] && b[2] == o.b[2]; }
+  };
+  struct Hash {
+    size_t operator()(const Key& k) const {
+      return (size_t)k.b[0] * 0x9E3779B97F4A7C15ull ^ (size_t)k.b[1] * 0xC2B2AE3D27D4EB4Full ^ (size_t)k.b[2];
+    }
+  };
+  std::unordered_map<Key, int32_t, Hash> index;
+  index.reserve((size_t)n * 2);
+  std::vector<float> verts;
+  std::vector<int32_t> faces((size_t)n * 3);
+  for (size_t i = 0; i < (size_t)n * 3; ++i) {
+    Key k;
+    std::memcpy(k.b, tri + 3 * i, 12);
+    auto it = index.find(k);
+    if (it == index.end()) {
+      it = index.emplace(k, (int32_t)(verts.size() / 3)).first;
+      verts.insert(verts.end(), tri + 3 * i, tri + 3 * i + 3);
+    }
+    faces[i] = it->second;
+  }
+  std::FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) {
+    std::fprintf(stderr, "saveToMesh: cannot open %s\n", path.c_str());
+    return false;
+  }
+  std::fprintf(f,
+               "ply\nformat binary_little_endian 1.0\ncomment Generated by VoxelScene::saveToMesh\n"
+               "element vertex %zu\nproperty float x\nproperty float y\nproperty float z\n"
+               "element face %u\nproperty list uchar int vertex_indices\nend_header\n",
+               verts.size() / 3, n);
+  std::fwrite(verts.data(), sizeof(float), verts.size(), f);
+  for (uint32_t t = 0; t < n; ++t) {
+    const unsigned char three = 3;
+    std::fwrite(&three, 1, 1, f);
+    std::fwrite(&faces[3 * (size_t)t], sizeof(int32_t), 3, f);
+  }
+  return std::fclose(f) == 0;
 }
 }  // namespace vsg_detail
 
@@ -154,10 +201,39 @@ class VoxelScene {
     std::printf("========================\n\n");
   }
 
-  // Stub in the reference (VoxelSG/src/VoxelScene.cpp:838-845).
-  bool saveToMesh(const std::string&) {
-    std::fprintf(stderr, "VoxelScene::saveToMesh: Not implemented yet\n");
-    return false;
+  // VoxelScene::saveToMesh (a stub in the reference, VoxelSG/src/VoxelScene.cpp:838-845):
+  // marching cubes over the whole scene (cubes across block borders included, cubes with
+  // a corner of weight below minWeight left out), written as a binary little-endian PLY:
+  // vertices shared by neighbouring triangles merged (equal float3, first-use order),
+  // one face per triangle.  False when the scene has no triangles or on an error.
+  bool saveToMesh(const std::string& filename, float isoValue = 0.0f, unsigned minWeight = 1) {
+    if (!m_isAllocated) return false;
+    uint32_t cap = 1u << 20, stored = 0;
+    uint64_t total = 0;
+    vsg_mesh* mesh = nullptr;
+    for (int pass = 0; pass < 2; ++pass) {
+      if (!vsg_detail::report(vsg_mesh_create(cap, &mesh), "VoxelScene::saveToMesh")) return false;
+      if (!vsg_detail::report(vsg_marching_cubes(mesh, m_scene, isoValue, 0.0f, 1, minWeight, &stored, &total),
+                              "VoxelScene::saveToMesh")) {
+        vsg_mesh_destroy(mesh);
+        return false;
+      }
+      if (total <= cap || total > (1u << 28)) break;
+      vsg_mesh_destroy(mesh);
+      mesh = nullptr;
+      cap = (uint32_t)total;
+    }
+    if (stored == 0) {
+      vsg_mesh_destroy(mesh);
+      std::fprintf(stderr, "VoxelScene::saveToMesh: no triangles\n");
+      return false;
+    }
+    std::vector<float> v((size_t)stored * 9);
+    uint32_t got = 0;
+    const bool ok = vsg_detail::report(vsg_mesh_download(mesh, v.data(), nullptr, stored, &got),
+                                       "VoxelScene::saveToMesh");
+    vsg_mesh_destroy(mesh);
+    return ok && vsg_detail::write_mesh_ply(filename, v.data(), got);
   }
 
   vsg_scene* handle() const { return m_scene; }
@@ -544,3 +620,63 @@ class CustomDepthMapGenerator {
   hipStream_t m_stream = nullptr;
   bool m_haveStream = false;
 };
+
+// MarchingCubes (VoxelSG/include/MarchingCubes.h:7-59).  The output's device buffers
+// are the library's; d_triangleCount is not kept on the device (the count of the last
+// run is triangleCount), and the run also takes the cross-block and weight options.
+struct MarchingCubesParams {  // MarchingCubes.h:7-16, same defaults
+  float isoValue = 0.0f;
+  float voxelSize = 1.0f;  // used when positive, else Params::voxelSize
+  int maxTriangles = 0;
+};
+
+struct MarchingCubesOutput {  // MarchingCubes.h:18-29
+  float3* d_vertices = nullptr;
+  float3* d_normals = nullptr;
+  int* d_triangleCount = nullptr;
+  int maxTriangles = 0;
+  long long triangleCount = 0;  // triangles found by the last run
+  vsg_mesh* mesh = nullptr;
+};
+
+// MarchingCubes.cu:491-501
+inline void freeMarchingCubesOutput(MarchingCubesOutput& output) {
+  if (output.mesh) vsg_mesh_destroy(output.mesh);
+  output = MarchingCubesOutput{};
+}
+
+// MarchingCubes.cu:474-489
+inline void allocateMarchingCubesOutput(MarchingCubesOutput& output, int maxTriangles) {
+  if (maxTriangles <= 0) {
+    std::printf("allocateMarchingCubesOutput: maxTriangles must be positive.\n");
+    return;
+  }
+  freeMarchingCubesOutput(output);
+  if (!vsg_detail::report(vsg_mesh_create((uint32_t)maxTriangles, &output.mesh), "allocateMarchingCubesOutput")) return;
+  float *v = nullptr, *n = nullptr;
+  vsg_mesh_buffers(output.mesh, &v, &n, nullptr);
+  output.d_vertices = reinterpret_cast<float3*>(v);
+  output.d_normals = reinterpret_cast<float3*>(n);
+  output.maxTriangles = maxTriangles;
+}
+
+// MarchingCubes.cu:511-571 over every live block of the scene, in hash-entry, cube and
+// table order; crossBlock adds the cubes across block borders (the reference's stay
+// inside one block), minWeight > 0 leaves out cubes with a lighter corner.
+inline void runMarchingCubesCUDA(VoxelScene& scene, const Params& params, const MarchingCubesParams& paramsMC,
+                                 MarchingCubesOutput& output, bool crossBlock = false, unsigned minWeight = 0) {
+  if (!output.mesh) {
+    std::printf("runMarchingCubesCUDA: Output buffers not allocated.\n");
+    return;
+  }
+  const float voxel = paramsMC.voxelSize > 0.0f ? paramsMC.voxelSize : params.voxelSize;
+  uint64_t total = 0;
+  if (vsg_detail::report(vsg_marching_cubes(output.mesh, scene.handle(), paramsMC.isoValue, voxel, crossBlock ? 1 : 0,
+                                            minWeight, nullptr, &total),
+                         "runMarchingCubesCUDA"))
+    output.triangleCount = (long long)total;
+}
+
+// MarchingCubes.cu:503-509: the triangles found (may exceed maxTriangles, as the
+// reference's counter does).
+inline long long copyMarchingCubesTriangleCount(const MarchingCubesOutput& output) { return output.triangleCount; }

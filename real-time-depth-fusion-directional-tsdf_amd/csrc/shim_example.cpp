@@ -12,9 +12,12 @@
 // valid pixels' points (with normals and colours, in pixel order) form a point cloud whose
 // depth maps, generated on the device on the scene's stream, are integrated.
 //
-// usage: vsg_shim_example <workload 0|1> <frames> [points.ply|-] [gen]
+// Then runMarchingCubesCUDA with the reference's cubes (inside blocks, no weight test)
+// and, when a mesh path is given, VoxelScene::saveToMesh.
+//
+// usage: vsg_shim_example <workload 0|1> <frames> [points.ply|-] [gen|-] [mesh.ply]
 // prints: frames, allocated blocks, hits, misses, a checksum of the last depth map,
-// and the number of surface points found.
+// the number of surface points found and the number of triangles.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -125,8 +128,17 @@ int main(int argc, char** argv) {
   RayCastRender extractor;
   const int points = extractor.extractSurfacePoints(&scene);
   if (argc > 3 && std::string(argv[3]) != "-" && !extractor.saveExtractedSurfacePointsPLY(argv[3])) return 6;
-  std::printf("frames %d blocks %u hits %u misses %u checksum %.9e points %d\n", scene.getNumIntegratedFrames(),
-              (unsigned)(params.SDFBlockNum - scene.getHeapFreeCount()), hits, misses, checksum, points);
+  MarchingCubesOutput mc;
+  allocateMarchingCubesOutput(mc, 1 << 22);
+  MarchingCubesParams mcp;
+  mcp.voxelSize = 0.0f;  // the scene's
+  runMarchingCubesCUDA(scene, params, mcp, mc);
+  const long long triangles = copyMarchingCubesTriangleCount(mc);
+  freeMarchingCubesOutput(mc);
+  if (argc > 5 && !scene.saveToMesh(argv[5])) return 7;
+  std::printf("frames %d blocks %u hits %u misses %u checksum %.9e points %d triangles %lld\n",
+              scene.getNumIntegratedFrames(), (unsigned)(params.SDFBlockNum - scene.getHeapFreeCount()), hits, misses,
+              checksum, points, triangles);
   (void)hipFree(d_pts);
   (void)hipFree(d_nrm);
   (void)hipFree(d_rgb);

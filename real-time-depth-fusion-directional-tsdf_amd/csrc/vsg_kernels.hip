@@ -2589,9 +2589,18 @@ __device__ __forceinline__ void mc_stage(McBlock& g, const typename Store<F16>::
     g.ok[i] = S::weight(w[z]) >= a.min_weight ? 1 : 0;
   }
   if (a.cross) {
-    for (uint32_t i = lane; i < (uint32_t)kMcGrid; i += 64u) {
-      const uint32_t x = i % kMcG, y = (i / kMcG) % kMcG, z = i / (kMcG * kMcG);
-      if (x < 8u && y < 8u && z < 8u) continue;
+    // the 217 points of the ninth layer: x = 8 (81), then y = 8 with x < 8 (72), then
+    // z = 8 with x, y < 8 (64)
+    for (uint32_t j = lane; j < (uint32_t)(kMcGrid - kVox); j += 64u) {
+      uint32_t x, y, z;
+      if (j < 81u) {
+        x = 8u, y = j % 9u, z = j / 9u;
+      } else if (j < 153u) {
+        x = (j - 81u) % 8u, y = 8u, z = (j - 81u) / 8u;
+      } else {
+        x = (j - 153u) % 8u, y = (j - 153u) / 8u, z = 8u;
+      }
+      const uint32_t i = x + kMcG * y + kMcG * kMcG * z;
       const int32_t q = nbp[(size_t)k * kExtNb + 13u + (x >> 3) + 3u * (y >> 3) + 9u * (z >> 3)];
       if (q < 0) {
         g.sd[i] = 0.0f;
@@ -2682,7 +2691,10 @@ __device__ __forceinline__ float3 mc_vertex(const McBlock& g, int4 b, uint32_t x
 
 // The triangles: block k's start pstart[k], then cube order, then table order; three
 // vertices and three copies of the face normal (computeNormal, MarchingCubes.cu:329-348)
-// per triangle; the first max_tris kept.
+// per triangle; the first max_tris kept.  The block's triangles are spread evenly over
+// the lanes (lane L takes triangles L, L + 64, ... of the block, finding its cube by a
+// binary search of the cubes' inclusive prefix sums), so neighbouring lanes write
+// neighbouring triangles.
 template <bool F16>
 __global__ __launch_bounds__(64) void k_mc_write(const DevState* __restrict__ st, const int4* __restrict__ oblk,
                                                  const int32_t* __restrict__ nbp, const void* __restrict__ vox,
@@ -2691,6 +2703,8 @@ __global__ __launch_bounds__(64) void k_mc_write(const DevState* __restrict__ st
                                                  const uint32_t* __restrict__ pstart, float3* __restrict__ out_v,
                                                  float3* __restrict__ out_n) {
   __shared__ McBlock g;
+  __shared__ uint32_t incl[kVox];  // triangles of cubes 0..l of the block
+  __shared__ uint8_t cases[kVox];
   const uint32_t lane = threadIdx.x, G = gridDim.x, nb = st->n_alloc;
   for (uint32_t j = lane; j < 256u; j += 64u) {
     int n = 0;
@@ -2703,45 +2717,58 @@ __global__ __launch_bounds__(64) void k_mc_write(const DevState* __restrict__ st
     if (start >= a.max_tris) break;  // starts do not decrease with k
     if (bcount[k] == 0u) continue;
     const int4 b = oblk[k];
-    __syncthreads();  // g is read below until every lane is done
+    __syncthreads();  // g, incl and cases are read below until every lane is done
     mc_stage<F16>(g, vw, nbp, k, b, a);
     __syncthreads();
-    uint32_t run = start;
+    uint32_t run = 0;
+#pragma unroll
     for (int z = 0; z < 8; ++z) {
       const uint32_t l = lane + 64u * (uint32_t)z;
       const int cs = mc_case(g, l, a);
       const uint32_t c = cs >= 0 ? g.ntri[cs] : 0u;
       const uint32_t inc = wave_incl_scan(c);
-      const uint32_t first = run + inc - c;
+      incl[l] = run + inc;
+      cases[l] = (uint8_t)(cs >= 0 ? cs : 0);
       run += __shfl(inc, 63, 64);
-      const uint32_t x = l & 7u, y = (l >> 3) & 7u, zz = l >> 6;
-      for (uint32_t t = 0; t < c; ++t) {
-        const uint32_t o = first + t;
-        if (o >= a.max_tris) break;
-        const int8_t* row = table + (size_t)cs * kMcRow + 3u * t;
-        const float3 v0 = mc_vertex(g, b, x, y, zz, row[0], a);
-        const float3 v1 = mc_vertex(g, b, x, y, zz, row[1], a);
-        const float3 v2 = mc_vertex(g, b, x, y, zz, row[2], a);
-        const float ax = v1.x - v0.x, ay = v1.y - v0.y, az = v1.z - v0.z;
-        const float bx = v2.x - v0.x, by = v2.y - v0.y, bz = v2.z - v0.z;
-        float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
-        const float len = sqrtf(nx * nx + ny * ny + nz * nz);
-        if (len > 1e-6f) {
-          const float inv = 1.0f / len;
-          nx *= inv;
-          ny *= inv;
-          nz *= inv;
-        } else {
-          nx = ny = nz = 0.0f;
-        }
-        const float3 n = make_float3(nx, ny, nz);
-        out_v[3u * o] = v0;
-        out_v[3u * o + 1u] = v1;
-        out_v[3u * o + 2u] = v2;
-        out_n[3u * o] = n;
-        out_n[3u * o + 1u] = n;
-        out_n[3u * o + 2u] = n;
+    }
+    __syncthreads();
+    const uint32_t total = run;
+    for (uint32_t j = lane; j < total; j += 64u) {
+      const uint32_t o = start + j;
+      if (o >= a.max_tris) break;
+      // cube of triangle j: the first l with incl[l] > j
+      uint32_t lo = 0, hi = kVox - 1u;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (incl[mid] > j) hi = mid;
+        else lo = mid + 1u;
       }
+      const uint32_t l = lo, cs = cases[l];
+      const uint32_t t = j - (incl[l] - g.ntri[cs]);
+      const uint32_t x = l & 7u, y = (l >> 3) & 7u, zz = l >> 6;
+      const int8_t* row = table + (size_t)cs * kMcRow + 3u * t;
+      const float3 v0 = mc_vertex(g, b, x, y, zz, row[0], a);
+      const float3 v1 = mc_vertex(g, b, x, y, zz, row[1], a);
+      const float3 v2 = mc_vertex(g, b, x, y, zz, row[2], a);
+      const float ax = v1.x - v0.x, ay = v1.y - v0.y, az = v1.z - v0.z;
+      const float bx = v2.x - v0.x, by = v2.y - v0.y, bz = v2.z - v0.z;
+      float nx = ay * bz - az * by, ny = az * bx - ax * bz, nz = ax * by - ay * bx;
+      const float len = sqrtf(nx * nx + ny * ny + nz * nz);
+      if (len > 1e-6f) {
+        const float inv = 1.0f / len;
+        nx *= inv;
+        ny *= inv;
+        nz *= inv;
+      } else {
+        nx = ny = nz = 0.0f;
+      }
+      const float3 n = make_float3(nx, ny, nz);
+      out_v[3u * o] = v0;
+      out_v[3u * o + 1u] = v1;
+      out_v[3u * o + 2u] = v2;
+      out_n[3u * o] = n;
+      out_n[3u * o + 1u] = n;
+      out_n[3u * o + 2u] = n;
     }
   }
 }

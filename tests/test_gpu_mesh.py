@@ -8,6 +8,9 @@ the 640x480 room's full size (no oracle run): every triangle side appears at mos
 per direction (consistent orientation), the reference's in-block cubes give a subset of
 the cross-block triangles, and the stored prefix does not depend on the capacity.
 """
+import os
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -147,3 +150,24 @@ def test_room_full_size_mesh_properties(torch_cuda, tmp_path):
     assert len(f) == total
     w, fw = vsg.weld_vertices(got["verts"])
     assert np.array_equal(v, w) and np.array_equal(f, fw)
+
+
+def test_cpp_shim_marching_cubes_and_mesh_file(torch_cuda, tmp_path):
+    """The C++ class API's runMarchingCubesCUDA count and saveToMesh file equal the
+    Python mirror's (same bytes)."""
+    exe = os.path.join(os.path.dirname(synth.__file__), "lib", "vsg_shim_example")
+    mesh_cpp = str(tmp_path / "cpp.ply")
+    out = subprocess.run([exe, "0", "4", "-", "-", mesh_cpp], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    tok = out.stdout.split()
+    got = {tok[i]: tok[i + 1] for i in range(0, len(tok) - 1, 2)}
+    g, _, _ = _integrated(torch_cuda, 0, range(4), oracle=False)
+    o = vsg.MarchingCubesOutput()
+    vsg.allocateMarchingCubesOutput(o, 1 << 22)
+    total, _ = _run(g, o)
+    vsg.freeMarchingCubesOutput(o)
+    assert int(got["triangles"]) == total > 0
+    mesh_py = str(tmp_path / "py.ply")
+    assert g.scene.saveToMesh(mesh_py)
+    with open(mesh_cpp, "rb") as a, open(mesh_py, "rb") as b:
+        assert a.read() == b.read()
