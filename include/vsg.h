@@ -407,6 +407,39 @@ vsg_status vsg_import_voxels(vsg_scene* scene, uint32_t first, uint32_t nblocks,
 vsg_status vsg_export_cell_masks(vsg_scene* scene, uint32_t first, uint32_t nblocks, uint8_t* masks,
                                  uint8_t* counts);
 
+/* ---- streaming: VoxelStreamingManager (VoxelSG/include/VoxelStreamingManager.h:120-301) -- */
+
+/* Bytes of one block image (the unit vsg_stream_out writes and vsg_stream_in reads):
+ * the block's 512 voxel words as the store holds them (FP32 store: {f32 sdf, u32
+ * weight}; FP16 store: {f16 sdf | u16 weight << 16}), then 512 colour words {r, g, b,
+ * flags}, then with VSG_SCENE_DIRECTIONAL 512 slot words {id0, w0, id1, w1} and 512
+ * float pairs (dtsdf of the two slots); voxels in vsg_export_voxels' order.  Replaces
+ * the reference's SDFBlock (VoxelStreamingManager.h:33-42), 512 x 12-byte VoxelData. */
+vsg_status vsg_block_image_bytes(const vsg_scene* scene, uint32_t* out);
+
+/* Stream-out (VoxelStreamingManager::streamOutFindBlocksOnGPU + streamOutCopyBlocksCUDA,
+ * VoxelSG/src/VoxelStreamingManager.cpp:303-500, VoxelStreamingManager.cu:36-91,
+ * 258-441): every allocated block whose world position (coordinate * 8 * voxel size,
+ * SDFBlockToWorld) lies at a distance >= radius from center is removed from the scene,
+ * at most `capacity` of them (the first in the scene's block order): its coordinate
+ * goes to coords[3*i..3*i+2] and its image to images + i * vsg_block_image_bytes
+ * (HOST memory), its hash entry is freed and its block returns to the free store
+ * with zeroed voxels.  *count = blocks removed; when it equals capacity more may
+ * remain.  Synchronising. */
+vsg_status vsg_stream_out(vsg_scene* scene, const float center[3], float radius, uint32_t capacity,
+                          int32_t* coords, void* images, uint32_t* count);
+
+/* Stream-in (VoxelStreamingManager::streamInInsertToHashTable + streamInHashAllocCUDA /
+ * streamInHashInitCUDA, VoxelSG/src/VoxelStreamingManager.cpp:566-833,
+ * VoxelStreamingManager.cu:443-719): inserts n blocks (HOST coords and images in
+ * vsg_stream_out's layout) in input order, as a sequential loop would: status[i] = 0
+ * inserted, 1 already allocated (the image is not applied), 2 home bucket full (the
+ * reference then probes other buckets, where its own lookups never look), 3 no free
+ * block, 4 coordinate beyond the key range.  *inserted = blocks inserted.  status and
+ * inserted may be NULL.  Synchronising. */
+vsg_status vsg_stream_in(vsg_scene* scene, uint32_t n, const int32_t* coords, const void* images, uint8_t* status,
+                         uint32_t* inserted);
+
 /* ---- diagnostics (byte-model counters; no counterpart in the reference) ------------ */
 
 typedef struct vsg_diag {

@@ -76,6 +76,9 @@ def lib() -> C.CDLL:
         L.vsgo_get_sdf.restype = C.c_double
         L.vsgo_get_sdf.argtypes = [vp, C.c_int64]
         L.vsgo_set_sdf.argtypes = [vp, C.c_int64, C.c_double]
+        L.vsgo_stream_out.restype = C.c_uint64
+        L.vsgo_stream_out.argtypes = [vp, vp, C.c_double, C.c_uint64, vp, vp, vp, vp, vp, vp, vp]
+        L.vsgo_stream_in.argtypes = [vp, C.c_uint64, vp, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -198,6 +201,49 @@ class OracleScene:
 
     def heap_counter(self) -> int:
         return int(lib().vsgo_heap_counter(self._h))
+
+    def stream_out(self, center, radius: float, capacity: int = 1 << 30, directional: bool = False) -> dict:
+        """Stream-out (VoxelStreamingManager.cu:36-91,258-358): the live blocks (table
+        order) at distance >= radius from center, at most capacity, removed from the
+        scene.  Returns coords int32 [m, 3], sdf float64 [m, 512], weight uint16 [m, 512],
+        rgbf uint8 [m, 512, 4] and, with directional, dir_id / dir_w uint8 and dtsdf
+        float64 [m, 512, 2]."""
+        c = np.ascontiguousarray(center, np.float64).reshape(3)
+        cap = int(min(capacity, max(self.num_allocated(), 1)))
+        coords = np.zeros((cap, 3), np.int32)
+        sdf = np.zeros((cap, 512), np.float64)
+        weight = np.zeros((cap, 512), np.uint16)
+        rgbf = np.zeros((cap, 512, 4), np.uint8)
+        dir_id = np.zeros((cap, 512, 2), np.uint8)
+        dir_w = np.zeros((cap, 512, 2), np.uint8)
+        dtsdf = np.zeros((cap, 512, 2), np.float64)
+        dptr = (dir_id.ctypes.data, dir_w.ctypes.data, dtsdf.ctypes.data) if directional else (None, None, None)
+        m = int(lib().vsgo_stream_out(self._h, c.ctypes.data, float(radius), cap, coords.ctypes.data, sdf.ctypes.data,
+                                      weight.ctypes.data, rgbf.ctypes.data, *dptr))
+        out = {"coords": coords[:m], "sdf": sdf[:m], "weight": weight[:m], "rgbf": rgbf[:m]}
+        if directional:
+            out.update(dir_id=dir_id[:m], dir_w=dir_w[:m], dtsdf=dtsdf[:m])
+        return out
+
+    def stream_in(self, blocks: dict) -> np.ndarray:
+        """Stream-in of the blocks (stream_out's layout) in order; status uint8 per block:
+        0 inserted, 1 already allocated, 2 bucket full, 3 heap empty, 4 out of range."""
+        coords = np.ascontiguousarray(blocks["coords"], np.int32)
+        n = len(coords)
+        sdf = np.ascontiguousarray(blocks["sdf"], np.float64)
+        weight = np.ascontiguousarray(blocks["weight"], np.uint16)
+        rgbf = np.ascontiguousarray(blocks["rgbf"], np.uint8)
+        status = np.zeros(n, np.uint8)
+        if "dir_id" in blocks:
+            keep = [np.ascontiguousarray(blocks[k], t) for k, t in (("dir_id", np.uint8), ("dir_w", np.uint8),
+                                                                     ("dtsdf", np.float64))]
+            dptr = tuple(a.ctypes.data for a in keep)
+        else:
+            dptr = (None, None, None)
+        if n:
+            lib().vsgo_stream_in(self._h, n, coords.ctypes.data, sdf.ctypes.data, weight.ctypes.data,
+                                 rgbf.ctypes.data, *dptr, status.ctypes.data)
+        return status
 
     def export_blocks(self):
         """(coords int32 [n,3], ptrs int32 [n], hash entry indices int32 [n]) in pointer order."""

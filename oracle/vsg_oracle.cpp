@@ -352,4 +352,68 @@ uint32_t vsgo_depthmap_from_points(int W, int H, float fx, float fy, float cx, f
                               pmap, depth, nmap, cmap);
 }
 
+// Stream-out of the blocks at distance >= radius from c3 (at most cap, table order):
+// returns how many; coords 3 per block, sdf / weight 512 per block, rgbf 4 per voxel,
+// direction slots 2 per voxel (any output but coords may be null).
+uint64_t vsgo_stream_out(void* p, const double* c3, double radius, uint64_t cap, int32_t* coords, double* sdf,
+                         uint16_t* weight, uint8_t* rgbf, uint8_t* dir_id, uint8_t* dir_w, double* dtsdf) {
+  Handle* h = (Handle*)p;
+  auto run = [&](auto& sc) -> uint64_t {
+    using R = typename std::decay_t<decltype(sc)>::Real;
+    auto out = sc.stream_out(V3<R>{(R)c3[0], (R)c3[1], (R)c3[2]}, (R)radius, (size_t)cap);
+    for (size_t k = 0; k < out.size(); ++k) {
+      const auto& b = out[k];
+      coords[3 * k] = b.c.x;
+      coords[3 * k + 1] = b.c.y;
+      coords[3 * k + 2] = b.c.z;
+      for (int v = 0; v < kVox; ++v) {
+        const size_t i = k * kVox + v;
+        if (sdf) sdf[i] = (double)b.sdf[v];
+        if (weight) weight[i] = b.weight[v];
+        if (rgbf)
+          for (int ch = 0; ch < 4; ++ch) rgbf[4 * i + ch] = b.rgbf[4 * v + ch];
+        if (!b.dir_id.empty())
+          for (int s = 0; s < 2; ++s) {
+            if (dir_id) dir_id[2 * i + s] = b.dir_id[2 * v + s];
+            if (dir_w) dir_w[2 * i + s] = b.dir_w[2 * v + s];
+            if (dtsdf) dtsdf[2 * i + s] = (double)b.dtsdf[2 * v + s];
+          }
+      }
+    }
+    return out.size();
+  };
+  return h->fp64 ? run(h->sd) : run(h->sf);
+}
+
+// Stream-in of n blocks in order (layout of vsgo_stream_out; direction slots may be
+// null); status per block (0 inserted, 1 already allocated, 2 bucket full, 3 heap
+// empty, 4 out of the key range).
+void vsgo_stream_in(void* p, uint64_t n, const int32_t* coords, const double* sdf, const uint16_t* weight,
+                    const uint8_t* rgbf, const uint8_t* dir_id, const uint8_t* dir_w, const double* dtsdf,
+                    uint8_t* status) {
+  Handle* h = (Handle*)p;
+  auto run = [&](auto& sc) {
+    using S = std::decay_t<decltype(sc)>;
+    using R = typename S::Real;
+    for (uint64_t k = 0; k < n; ++k) {
+      typename S::Streamed b;
+      b.c = I3{coords[3 * k], coords[3 * k + 1], coords[3 * k + 2]};
+      for (int v = 0; v < kVox; ++v) {
+        const size_t i = k * kVox + v;
+        b.sdf.push_back((R)sdf[i]);
+        b.weight.push_back(weight[i]);
+        for (int ch = 0; ch < 4; ++ch) b.rgbf.push_back(rgbf[4 * i + ch]);
+        if (dir_id && dir_w && dtsdf)
+          for (int s = 0; s < 2; ++s) {
+            b.dir_id.push_back(dir_id[2 * i + s]);
+            b.dir_w.push_back(dir_w[2 * i + s]);
+            b.dtsdf.push_back((R)dtsdf[2 * i + s]);
+          }
+      }
+      status[k] = (uint8_t)sc.stream_in(b);
+    }
+  };
+  if (h->fp64) run(h->sd); else run(h->sf);
+}
+
 }  // extern "C"

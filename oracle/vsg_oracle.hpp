@@ -156,6 +156,7 @@ template <class R> inline V3<R> xform_point(const R* M, V3<R> p) {
 
 // ---- Voxel hash state (VoxelSG/include/CUDAHashRef.h:11-79, CUDAHashRef.cu:40-152)
 template <class R> struct Scene {
+  using Real = R;
   uint32_t num_buckets = 0;  // Params::hashSlotNum
   uint32_t num_blocks = 0;   // Params::SDFBlockNum
   R voxel = R(0);            // Params::voxelSize
@@ -271,6 +272,108 @@ template <class R> struct Scene {
   }
 
   size_t num_allocated() const { return (size_t)(num_blocks - 1u - heap_counter); }
+
+  // ---- streaming (VoxelStreamingManager.cu:36-91,258-358,443-719) --------------
+  // One streamed block: coordinate and voxels (weight widened to u16 as the GPU
+  // store keeps it; colour rgb + flags; direction slots when the extension is on).
+  struct Streamed {
+    I3 c;
+    std::vector<R> sdf;
+    std::vector<uint16_t> weight;
+    std::vector<uint8_t> rgbf;  // 4 per voxel
+    std::vector<uint8_t> dir_id, dir_w;
+    std::vector<R> dtsdf;
+  };
+
+  // Stream-out: live entries in table order (the compacted list, VoxelScene.cu:
+  // 1275-1300) whose world position (SDFBlockToWorld: coordinate * 8 * voxel,
+  // VoxelScene.cu:101-107) lies at distance >= radius from c
+  // (VoxelStreamingManager.cu:55-66), at most cap: voxels copied out then zeroed,
+  // slot freed (ptr -1, pos 0), block index pushed on the heap (:279-356).
+  std::vector<Streamed> stream_out(V3<R> c, R radius, size_t cap) {
+    std::vector<Streamed> out;
+    for (size_t e = 0; e < slot_ptr.size() && out.size() < cap; ++e) {
+      const int32_t ptr = slot_ptr[e];
+      if (ptr == -1) continue;
+      const I3 b = slot_pos[e];
+      const R dx = R(b.x * kBlock) * voxel - c.x;
+      const R dy = R(b.y * kBlock) * voxel - c.y;
+      const R dz = R(b.z * kBlock) * voxel - c.z;
+      if (!(std::sqrt(dx * dx + dy * dy + dz * dz) >= radius)) continue;
+      ensure_block(ptr);
+      Streamed sb;
+      sb.c = b;
+      for (int v = 0; v < kVox; ++v) {
+        const size_t i = (size_t)ptr * kVox + v;
+        sb.sdf.push_back(load_sdf(i));
+        sb.weight.push_back(weight[i]);
+        for (int ch = 0; ch < 3; ++ch) sb.rgbf.push_back(color[3 * i + ch]);
+        sb.rgbf.push_back(flags[i]);
+        store_sdf(i, R(0));
+        weight[i] = 0;
+        color[3 * i] = color[3 * i + 1] = color[3 * i + 2] = 0;
+        flags[i] = 0;
+        if (directional) {
+          for (int k = 0; k < 2; ++k) {
+            sb.dir_id.push_back(dir_id[2 * i + k]);
+            sb.dir_w.push_back(dir_w[2 * i + k]);
+            sb.dtsdf.push_back(dtsdf[2 * i + k]);
+            dir_id[2 * i + k] = dir_w[2 * i + k] = 0;
+            dtsdf[2 * i + k] = R(0);
+          }
+        }
+      }
+      slot_ptr[e] = -1;
+      slot_pos[e] = I3{0, 0, 0};
+      heap_counter += 1u;  // atomicAdd returns the old top; the block goes above it
+      heap[heap_counter] = (uint32_t)ptr;
+      out.push_back(std::move(sb));
+    }
+    return out;
+  }
+
+  // Stream-in of one block, inputs taken in order: 4 coordinate beyond the GPU
+  // store's key range (+-2^20 blocks), 1 already allocated (nothing written), 2 home
+  // bucket full (the reference probes on into other buckets,
+  // VoxelStreamingManager.cu:531-596, where its lookups never look), 3 heap empty,
+  // else 0: first free slot of the home bucket, block popped from the heap as
+  // allocBlock pops it (VoxelScene.cu:183-214), voxels written.
+  int stream_in(const Streamed& sb) {
+    const I3 b = sb.c;
+    const int32_t lim = 1 << 20;
+    if (b.x < -lim || b.x >= lim || b.y < -lim || b.y >= lim || b.z < -lim || b.z >= lim) return 4;
+    const uint32_t h = hash_bucket(b, num_buckets);
+    const size_t hp = (size_t)h * kBucket;
+    long first_empty = -1;
+    for (int j = 0; j < kBucket; ++j) {
+      const size_t i = hp + j;
+      if (slot_ptr[i] != -1 && slot_pos[i] == b) return 1;
+      if (first_empty == -1 && slot_ptr[i] == -1) first_empty = (long)i;
+    }
+    if (first_empty == -1) return 2;
+    const uint32_t heap_index = heap_counter;
+    if (!(heap_index <= num_blocks && heap_index != 0xFFFFFFFFu)) return 3;
+    heap_counter -= 1u;
+    const uint32_t ptr = heap[heap_index];
+    slot_pos[first_empty] = b;
+    slot_ptr[first_empty] = (int32_t)ptr;
+    ensure_block((int32_t)ptr);
+    for (int v = 0; v < kVox; ++v) {
+      const size_t i = (size_t)ptr * kVox + v;
+      store_sdf(i, sb.sdf[v]);
+      weight[i] = (uint8_t)sb.weight[v];
+      for (int ch = 0; ch < 3; ++ch) color[3 * i + ch] = sb.rgbf[4 * v + ch];
+      flags[i] = sb.rgbf[4 * v + 3];
+      if (directional && !sb.dir_id.empty()) {
+        for (int k = 0; k < 2; ++k) {
+          dir_id[2 * i + k] = sb.dir_id[2 * v + k];
+          dir_w[2 * i + k] = sb.dir_w[2 * v + k];
+          dtsdf[2 * i + k] = sb.dtsdf[2 * v + k];
+        }
+      }
+    }
+    return 0;
+  }
 };
 
 // ---- A5: block DDA from block(rayStart) to block(rayEnd) (VoxelScene.cu:295-481).

@@ -1423,3 +1423,136 @@ vsg_status vsg_mesh_download(vsg_mesh* m, float* vertices, float* normals, uint3
 }
 
 }  // extern "C"
+
+// ---- streaming (VoxelStreamingManager.cu / .cpp) ----------------------------------
+
+namespace {
+
+BlockPlanes planes_of(vsg_scene* s) {
+  return BlockPlanes{(uint8_t*)s->vox, s->word_bytes, s->col, s->dmeta, s->dval};
+}
+
+// Device scratch of one streaming call, freed when the call returns.
+struct CallScratch {
+  std::vector<void*> bufs;
+  ~CallScratch() {
+    for (void* b : bufs) (void)hipFree(b);
+  }
+  template <class T> vsg_status get(T** p, size_t bytes, const char* what) {
+    vsg_status st = dmalloc(p, bytes ? bytes : 16, what);
+    if (st == VSG_OK) bufs.push_back((void*)*p);
+    return st;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+vsg_status vsg_block_image_bytes(const vsg_scene* s, uint32_t* out) {
+  if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_block_image_bytes: null argument");
+  *out = (uint32_t)kVox * (s->word_bytes + 4u + (s->dmeta ? 12u : 0u));
+  return VSG_OK;
+}
+
+vsg_status vsg_stream_out(vsg_scene* s, const float center[3], float radius, uint32_t capacity, int32_t* coords,
+                          void* images, uint32_t* count) {
+  if (!s || !center || !count) return fail(VSG_ERR_INVALID_ARG, "vsg_stream_out: null argument");
+  if (capacity > 0 && (!coords || !images))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_stream_out: coords and images are required when capacity > 0");
+  *count = 0;
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  uint32_t n = 0;
+  VSG_HIP(hipMemcpy(&n, &s->st->n_alloc, sizeof(n), hipMemcpyDeviceToHost));
+  if (n == 0 || capacity == 0) return VSG_OK;
+  const BlockPlanes bp = planes_of(s);
+  const size_t ib = block_image_bytes(bp);
+  CallScratch cs;
+  uint8_t* sel = nullptr;
+  uint32_t *rank = nullptr, *sums = nullptr;
+  vsg_status st;
+  if ((st = cs.get(&sel, n, "stream-out flags")) != VSG_OK) return st;
+  if ((st = cs.get(&rank, (size_t)n * 4, "stream-out ranks")) != VSG_OK) return st;
+  if ((st = cs.get(&sums, (size_t)stream_scan_words(n) * 4, "stream-out scan")) != VSG_OK) return st;
+  const uint32_t selected = stream_out_select(s->stream, s->t, n, center, radius, s->p.voxel_size, sel, rank, sums);
+  if ((st = check_launch("stream-out selection")) != VSG_OK) return st;
+  if (selected == 0xFFFFFFFFu) return fail(VSG_ERR_HIP, "vsg_stream_out: selection failed");
+  const uint32_t m = selected < capacity ? selected : capacity;
+  if (m == 0) return VSG_OK;
+  uint32_t *out = nullptr, *remap = nullptr;
+  int4* dcoords = nullptr;
+  uint8_t* dimg = nullptr;
+  if ((st = cs.get(&out, (size_t)m * 4, "stream-out list")) != VSG_OK) return st;
+  if ((st = cs.get(&remap, (size_t)n * 4, "stream-out pointer map")) != VSG_OK) return st;
+  if ((st = cs.get(&dcoords, (size_t)m * sizeof(int4), "stream-out coordinates")) != VSG_OK) return st;
+  if ((st = cs.get(&dimg, (size_t)m * ib, "stream-out images")) != VSG_OK) return st;
+  stream_out_remove(s->stream, s->t, s->st, bp, n, m, sel, rank, out, remap, dcoords, dimg);
+  if ((st = check_launch("stream-out")) != VSG_OK) return st;
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  std::vector<int4> hc(m);
+  VSG_HIP(hipMemcpy(hc.data(), dcoords, (size_t)m * sizeof(int4), hipMemcpyDeviceToHost));
+  VSG_HIP(hipMemcpy(images, dimg, (size_t)m * ib, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < m; ++i) {
+    coords[3 * i] = hc[i].x;
+    coords[3 * i + 1] = hc[i].y;
+    coords[3 * i + 2] = hc[i].z;
+  }
+  *count = m;
+  return VSG_OK;
+}
+
+vsg_status vsg_stream_in(vsg_scene* s, uint32_t n, const int32_t* coords, const void* images, uint8_t* status,
+                         uint32_t* inserted) {
+  if (!s) return fail(VSG_ERR_INVALID_ARG, "vsg_stream_in: null scene");
+  if (n > 0 && (!coords || !images)) return fail(VSG_ERR_INVALID_ARG, "vsg_stream_in: null coords or images");
+  if (inserted) *inserted = 0;
+  if (n == 0) return VSG_OK;
+  VSG_HIP(hipStreamSynchronize(s->stream));
+  const BlockPlanes bp = planes_of(s);
+  const size_t ib = block_image_bytes(bp);
+  const bool fp16 = s->p.store == VSG_STORE_FP16;
+  CallScratch cs;
+  int4* dcoords = nullptr;
+  uint8_t *dimg = nullptr, *dstatus = nullptr, *ok = nullptr;
+  uint32_t *entry = nullptr, *rank = nullptr, *sums = nullptr;
+  vsg_status st;
+  if ((st = cs.get(&dcoords, (size_t)n * sizeof(int4), "stream-in coordinates")) != VSG_OK) return st;
+  if ((st = cs.get(&dimg, (size_t)n * ib, "stream-in images")) != VSG_OK) return st;
+  if ((st = cs.get(&dstatus, n, "stream-in status")) != VSG_OK) return st;
+  if ((st = cs.get(&ok, n, "stream-in flags")) != VSG_OK) return st;
+  if ((st = cs.get(&entry, (size_t)n * 4, "stream-in entries")) != VSG_OK) return st;
+  if ((st = cs.get(&rank, (size_t)n * 4, "stream-in ranks")) != VSG_OK) return st;
+  if ((st = cs.get(&sums, (size_t)stream_scan_words(n) * 4, "stream-in scan")) != VSG_OK) return st;
+  std::vector<int4> hc(n);
+  for (uint32_t i = 0; i < n; ++i) hc[i] = make_int4(coords[3 * i], coords[3 * i + 1], coords[3 * i + 2], 0);
+  VSG_HIP(hipMemcpy(dcoords, hc.data(), (size_t)n * sizeof(int4), hipMemcpyHostToDevice));
+  VSG_HIP(hipMemcpy(dimg, images, (size_t)n * ib, hipMemcpyHostToDevice));
+  std::vector<uint8_t> hs(n, 0xFF), part(n);
+  uint32_t total = 0;
+  // Batches no larger than the free blocks, so every insertion of a batch gets a
+  // block; once none is free the rest are classified without inserting (3 = no
+  // block), as a sequential loop over the inputs would.
+  uint32_t first = 0;
+  while (first < n) {
+    uint32_t n0 = 0;
+    VSG_HIP(hipMemcpy(&n0, &s->st->n_alloc, sizeof(n0), hipMemcpyDeviceToHost));
+    const uint32_t free_blocks = s->t.num_blocks - n0;
+    const bool nospace = free_blocks == 0;
+    const uint32_t len = nospace ? n - first : (n - first < free_blocks ? n - first : free_blocks);
+    const uint32_t got = stream_in_insert(s->stream, s->t, len, dcoords + first, dstatus + first, entry + first,
+                                          ok + first, rank + first, sums, nospace);
+    if ((st = check_launch("stream-in insertion")) != VSG_OK) return st;
+    if (got == 0xFFFFFFFFu) return fail(VSG_ERR_HIP, "vsg_stream_in: insertion failed");
+    stream_in_place(s->stream, fp16, s->t, s->st, bp, len, n0, got, dcoords + first, ok + first, rank + first,
+                    entry + first, dimg + (size_t)first * ib);
+    if ((st = check_launch("stream-in placement")) != VSG_OK) return st;
+    VSG_HIP(hipStreamSynchronize(s->stream));
+    total += got;
+    first += len;
+  }
+  if (status) VSG_HIP(hipMemcpy(status, dstatus, n, hipMemcpyDeviceToHost));
+  if (inserted) *inserted = total;
+  return VSG_OK;
+}
+
+}  // extern "C"

@@ -122,6 +122,41 @@ void launch_depthgen(hipStream_t s, const DepthGenArgs& g, const float* pts, con
 // *valid = pixels of the last launch_depthgen with a point (its win array).
 void launch_depthgen_count(hipStream_t s, const uint32_t* win, uint32_t npix, uint32_t* valid);
 
+// Per-block arrays of the store, in the order a block image holds them: voxel words
+// (kVox * word_bytes), colours (kVox * 4 B), and with the directional extension the
+// slot metadata (kVox * 4 B) and slot values (kVox * 8 B).
+struct BlockPlanes {
+  uint8_t* vox;
+  uint32_t word_bytes;
+  uint32_t* col;
+  uint32_t* dmeta;  // null without the directional extension
+  float2* dval;
+};
+uint32_t block_image_bytes(const BlockPlanes& bp);
+// Words of the scan partials for n elements.
+uint32_t stream_scan_words(uint32_t n);
+// Marks the live blocks [0, n) at distance >= radius from c (sel: n bytes), ranks
+// them in pointer order (rank: n words) and returns how many there are (synchronises
+// the stream; ~0 on a HIP error).
+uint32_t stream_out_select(hipStream_t s, const TableView& t, uint32_t n, const float c[3], float radius, float voxel,
+                           uint8_t* sel, uint32_t* rank, uint32_t* sums);
+// Removes the first m selected blocks: their coordinates into coords[0, m) and
+// images into images (m images); out: m words, remap: n words of scratch.
+void stream_out_remove(hipStream_t s, const TableView& t, DevState* st, const BlockPlanes& bp, uint32_t n, uint32_t m,
+                       const uint8_t* sel, const uint32_t* rank, uint32_t* out, uint32_t* remap, int4* coords,
+                       uint8_t* images);
+// Inserts the n input blocks into their buckets (status per input: 0 inserted, 1
+// already live, 2 bucket full, 3 no free block (nospace), 4 coordinate out of
+// range) and returns how many were inserted (synchronises; ~0 on a HIP error).
+// entry, rank: n words; ok: n bytes.
+uint32_t stream_in_insert(hipStream_t s, const TableView& t, uint32_t n, const int4* coords, uint8_t* status,
+                          uint32_t* entry, uint8_t* ok, uint32_t* rank, uint32_t* sums, bool nospace);
+// Gives the inserted blocks pointers n0 + rank, index entries and their planes from
+// the images, and recomputes their positive-cell counts and masks.
+void stream_in_place(hipStream_t s, bool fp16, const TableView& t, DevState* st, const BlockPlanes& bp, uint32_t n,
+                     uint32_t n0, uint32_t inserted, const int4* coords, const uint8_t* ok, const uint32_t* rank,
+                     const uint32_t* entry, const uint8_t* images);
+
 // out[i] = cvt_i32(in[i]) for i < n (device buffers).
 void launch_convert(hipStream_t s, const float* in, int32_t* out, uint32_t n);
 void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,

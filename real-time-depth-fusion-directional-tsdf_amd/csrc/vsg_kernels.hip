@@ -3182,4 +3182,333 @@ void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState*
 #undef VSG_RENDER
 }
 
+// ---------------------------------------------------------------------------------
+// Streaming (VoxelStreamingManager.cu:36-91,258-358,443-719)
+// ---------------------------------------------------------------------------------
+//
+// Live block pointers stay dense ([0, n_alloc)) through streaming, so every other
+// kernel keeps its invariants.  Stream-out removes a set of blocks: their images are
+// gathered, the kept blocks at pointers >= n_alloc - m move into the holes the
+// removed ones leave below that bound (sources and destinations are disjoint), the
+// bucket records drop the removed entries (keeping slot order) and take the moved
+// pointers, the freed tail is zeroed, and the block index and occupancy maps are
+// rebuilt from the remaining blocks.  Stream-in inserts blocks in rounds of one
+// insertion per bucket, the smallest input index first, which is what a sequential
+// loop over the inputs produces; pointers follow input order.
+
+// Copies `bytes` (a multiple of 16) from src to dst with the workgroup's 256 lanes.
+__device__ __forceinline__ void wg_copy(void* __restrict__ dst, const void* __restrict__ src, uint32_t bytes) {
+  uint4* d = reinterpret_cast<uint4*>(dst);
+  const uint4* s = reinterpret_cast<const uint4*>(src);
+  for (uint32_t i = threadIdx.x; i < bytes / 16u; i += 256u) d[i] = s[i];
+}
+
+// Copies the planes of block p into image `img` (or back when to_image is false).
+__device__ __forceinline__ void image_copy(const BlockPlanes& bp, uint32_t p, uint8_t* img, bool to_image) {
+  const uint32_t vb = kVox * bp.word_bytes;
+  uint8_t* planes[4] = {bp.vox + (size_t)p * vb, reinterpret_cast<uint8_t*>(bp.col + (size_t)p * kVox),
+                        reinterpret_cast<uint8_t*>(bp.dmeta + (size_t)p * kVox),
+                        reinterpret_cast<uint8_t*>(bp.dval + (size_t)p * kVox)};
+  const uint32_t bytes[4] = {vb, kVox * 4u, kVox * 4u, kVox * 8u};
+  const int nplanes = bp.dmeta ? 4 : 2;
+  uint32_t off = 0;
+  for (int k = 0; k < nplanes; ++k) {
+    if (to_image) wg_copy(img + off, planes[k], bytes[k]);
+    else wg_copy(planes[k], img + off, bytes[k]);
+    off += bytes[k];
+  }
+}
+
+// sel[p] = 1 when block p lies at distance >= radius from the centre, its world
+// position being coordinate * 8 * voxel (SDFBlockToWorld, VoxelScene.cu:101-107;
+// test as in streamOutFindBlocksKernel_OPTIMIZED, VoxelStreamingManager.cu:55-66).
+__global__ __launch_bounds__(256) void k_so_select(const int4* __restrict__ blk, uint32_t n, float cx, float cy,
+                                                   float cz, float radius, float voxel, uint8_t* __restrict__ sel) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= n) return;
+  const int4 b = blk[p];
+  const float dx = (float)(b.x * kBlock) * voxel - cx;
+  const float dy = (float)(b.y * kBlock) * voxel - cy;
+  const float dz = (float)(b.z * kBlock) * voxel - cz;
+  sel[p] = sqrtf(dx * dx + dy * dy + dz * dz) >= radius ? 1 : 0;
+}
+
+// The first m selected blocks (pointer order) are removed: out[r] = pointer of the
+// r-th; remap[p] = p for kept blocks below k = n - m, ~0 for removed ones (movers'
+// entries are written by k_so_move).
+__global__ __launch_bounds__(256) void k_so_list(uint32_t n, uint32_t k, uint32_t m, const uint8_t* __restrict__ sel,
+                                                 const uint32_t* __restrict__ rank, uint32_t* __restrict__ out,
+                                                 uint32_t* __restrict__ remap) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= n) return;
+  const bool removed = sel[p] && rank[p] < m;
+  if (removed) out[rank[p]] = p;
+  if (removed) remap[p] = 0xFFFFFFFFu;
+  else if (p < k) remap[p] = p;
+}
+
+// One workgroup per removed block: its coordinates and image.
+__global__ __launch_bounds__(256) void k_so_gather(BlockPlanes bp, const int4* __restrict__ blk,
+                                                   const uint32_t* __restrict__ out, int4* __restrict__ coords,
+                                                   uint8_t* __restrict__ images, uint32_t image_bytes) {
+  const uint32_t r = blockIdx.x, p = out[r];
+  if (threadIdx.x == 0) coords[r] = blk[p];
+  image_copy(bp, p, images + (size_t)r * image_bytes, true);
+}
+
+// One workgroup per pointer q in [k, n): a kept block there moves to the hole of
+// its rank among the movers (the holes are the removed pointers below k, which are
+// out[0 .. holes) in increasing order).
+__global__ __launch_bounds__(256) void k_so_move(BlockPlanes bp, uint32_t* __restrict__ pcnt, uint32_t n, uint32_t k,
+                                                 uint32_t m, const uint8_t* __restrict__ sel,
+                                                 const uint32_t* __restrict__ rank, const uint32_t* __restrict__ out,
+                                                 uint32_t* __restrict__ remap) {
+  const uint32_t q = k + blockIdx.x;
+  if (q >= n) return;
+  const uint32_t rq = rank[q] < m ? rank[q] : m;  // removed blocks below q
+  if (sel[q] && rank[q] < m) return;             // removed itself
+  const uint32_t rk = rank[k] < m ? rank[k] : m;  // removed blocks below k (the holes)
+  const uint32_t dst = out[(q - k) - (rq - rk)];
+  if (threadIdx.x == 0) remap[q] = dst;
+  if (threadIdx.x < 8) pcnt[(size_t)dst * 8u + threadIdx.x] = pcnt[(size_t)q * 8u + threadIdx.x];
+  const uint32_t vb = kVox * bp.word_bytes;
+  wg_copy(bp.vox + (size_t)dst * vb, bp.vox + (size_t)q * vb, vb);
+  wg_copy(bp.col + (size_t)dst * kVox, bp.col + (size_t)q * kVox, kVox * 4u);
+  if (bp.dmeta) {
+    wg_copy(bp.dmeta + (size_t)dst * kVox, bp.dmeta + (size_t)q * kVox, kVox * 4u);
+    wg_copy(bp.dval + (size_t)dst * kVox, bp.dval + (size_t)q * kVox, kVox * 8u);
+  }
+}
+
+// Zeroes the planes and cell counts of blocks [k, n) (free again).
+__global__ __launch_bounds__(256) void k_so_clear(BlockPlanes bp, uint32_t* __restrict__ pcnt, uint32_t k, uint32_t n) {
+  const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+  const size_t v0 = (size_t)k * kVox, v1 = (size_t)n * kVox;
+  for (size_t i = (size_t)k * 8u + tid; i < (size_t)n * 8u; i += stride) pcnt[i] = 0u;
+  for (size_t i = v0 + tid; i < v1; i += stride) {
+    if (bp.word_bytes == 8) reinterpret_cast<uint2*>(bp.vox)[i] = make_uint2(0u, 0u);
+    else reinterpret_cast<uint32_t*>(bp.vox)[i] = 0u;
+    bp.col[i] = 0u;
+    if (bp.dmeta) {
+      bp.dmeta[i] = 0u;
+      bp.dval[i] = make_float2(0.0f, 0.0f);
+    }
+  }
+}
+
+// Per bucket: drops removed entries (the kept ones keep their order), renames moved
+// pointers and rewrites blk[] of every kept entry with its new slot.
+__global__ __launch_bounds__(256) void k_so_buckets(TableView t, const uint32_t* __restrict__ remap) {
+  for (uint32_t h = blockIdx.x * 256u + threadIdx.x; h < t.num_buckets; h += gridDim.x * 256u) {
+    const int c = t.cnt[h];
+    if (c == 0) continue;
+    uint64_t* r = t.rec + (size_t)h * kRecWords;
+    int32_t* ptr = reinterpret_cast<int32_t*>(r + kBucket);
+    int j2 = 0;
+    for (int j = 0; j < c; ++j) {
+      const uint64_t key = r[j];
+      const int32_t p = ptr[j];
+      const uint32_t np = remap[p];
+      if (np == 0xFFFFFFFFu) continue;
+      const int4 b = t.blk[p];
+      r[j2] = key;
+      ptr[j2] = (int32_t)np;
+      t.blk[np] = make_int4(b.x, b.y, b.z, (int32_t)(h * kBucket + j2));
+      ++j2;
+    }
+    t.cnt[h] = (uint8_t)j2;
+  }
+}
+
+// Empties the block index and the occupancy maps.
+__global__ __launch_bounds__(256) void k_clear_maps(TableView t) {
+  const size_t tid = (size_t)blockIdx.x * 256 + threadIdx.x, stride = (size_t)gridDim.x * 256;
+  for (size_t w = tid; w <= t.occ_mask; w += stride) t.occ[w] = 0ull;
+  for (size_t w = tid; w <= t.hyp_mask; w += stride) t.hyp[w] = 0ull;
+  for (size_t w = tid; w <= t.idx_mask; w += stride) t.idx[w] = make_uint4(kIdxEmpty, kIdxEmpty, 0u, 0u);
+}
+
+// Occupancy bits and a block index entry (pointer p, positive-cell mask `mask`) for
+// block b: the same writes the allocation's insertion makes.
+__device__ __forceinline__ void index_block(const TableView& t, int4 b, uint32_t p, uint32_t mask) {
+  atomicOr(reinterpret_cast<unsigned long long*>(&t.occ[occ_index(b.x >> 2, b.y >> 2, b.z >> 2, t.occ_mask)]),
+           1ull << occ_bit(b.x, b.y, b.z));
+  atomicOr(reinterpret_cast<unsigned long long*>(&t.hyp[hyp_index(b.x >> 4, b.y >> 4, b.z >> 4, t.hyp_mask)]),
+           1ull << hyp_bit(b.x >> 2, b.y >> 2, b.z >> 2));
+  const uint64_t key = pack_key(b.x, b.y, b.z);
+  uint32_t sl = idx_home(b.x, b.y, b.z, t.idx_mask);
+  for (uint32_t n = 0; n <= t.idx_mask; ++n) {
+    unsigned long long* kw = reinterpret_cast<unsigned long long*>(&t.idx[sl]);
+    if (atomicCAS(kw, 0xFFFFFFFFFFFFFFFFull, (unsigned long long)key) == 0xFFFFFFFFFFFFFFFFull) {
+      t.idx[sl].z = p;
+      t.idx[sl].w = mask;
+      t.idxpos[p] = sl;
+      break;
+    }
+    sl = (sl + 1u) & t.idx_mask;
+  }
+}
+
+// Re-indexes live blocks [0, n) (masks from their cell counts).
+__global__ __launch_bounds__(256) void k_reindex(TableView t, uint32_t n) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  if (p >= n) return;
+  index_block(t, t.blk[p], p, cell_mask_of(t.pcnt + (size_t)p * 8u));
+}
+
+// Sets the live block count.
+__global__ void k_set_alloc(DevState* st, uint32_t n) {
+  st->n_alloc = n;
+  st->n_alloc_prev = n;
+}
+
+// Stream-in status codes (per input block).
+constexpr uint8_t kSiPending = 0xFF;
+
+// Inputs out of the key range fail (4); the others start pending.
+__global__ __launch_bounds__(256) void k_si_init(const int4* __restrict__ coords, uint32_t n,
+                                                 uint8_t* __restrict__ status) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const int4 c = coords[i];
+  status[i] = key_in_range(c.x, c.y, c.z) ? kSiPending : 4;
+}
+
+// One claim round: a pending input whose block is live is done (1), one whose home
+// bucket is full fails (2); with no free block left (nospace) the others fail (3),
+// else they bid for their bucket with their index.
+__global__ __launch_bounds__(256) void k_si_claim(TableView t, const int4* __restrict__ coords, uint32_t n,
+                                                  uint8_t* __restrict__ status, int32_t nospace) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n || status[i] != kSiPending) return;
+  const int4 c = coords[i];
+  if (table_lookup(t, c.x, c.y, c.z) >= 0) {
+    status[i] = 1;
+    return;
+  }
+  const uint32_t h = hash_bucket(c.x, c.y, c.z, t.num_buckets);
+  if (t.cnt[h] >= kBucket) {
+    status[i] = 2;
+    return;
+  }
+  if (nospace) {
+    status[i] = 3;
+    return;
+  }
+  atomicMin(&t.claim[h], i);
+}
+
+// The round's winners take the next slot of their bucket (pointer set later);
+// entry[i] = bucket * 10 + slot.
+__global__ __launch_bounds__(256) void k_si_insert(TableView t, const int4* __restrict__ coords, uint32_t n,
+                                                   uint8_t* __restrict__ status, uint32_t* __restrict__ entry) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n || status[i] != kSiPending) return;
+  const int4 c = coords[i];
+  const uint32_t h = hash_bucket(c.x, c.y, c.z, t.num_buckets);
+  if (t.claim[h] != i) return;
+  t.claim[h] = kNoClaim;
+  const int slot = t.cnt[h];
+  uint64_t* r = t.rec + (size_t)h * kRecWords;
+  r[slot] = pack_key(c.x, c.y, c.z);
+  reinterpret_cast<int32_t*>(r + kBucket)[slot] = 0;
+  t.cnt[h] = (uint8_t)(slot + 1);
+  entry[i] = h * kBucket + (uint32_t)slot;
+  status[i] = 0;
+}
+
+// ok[i] = 1 for inserted inputs; inputs still pending after the last round fail (2).
+__global__ __launch_bounds__(256) void k_si_flags(uint32_t n, uint8_t* __restrict__ status, uint8_t* __restrict__ ok) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  if (status[i] == kSiPending) status[i] = 2;
+  ok[i] = status[i] == 0 ? 1 : 0;
+}
+
+// One workgroup per input: an inserted block takes pointer n0 + its rank, its record
+// pointer, blk entry, occupancy bits, index entry and planes (from its image).
+__global__ __launch_bounds__(256) void k_si_place(TableView t, BlockPlanes bp, const int4* __restrict__ coords,
+                                                  const uint8_t* __restrict__ ok, const uint32_t* __restrict__ rank,
+                                                  const uint32_t* __restrict__ entry, uint32_t n0,
+                                                  const uint8_t* __restrict__ images, uint32_t image_bytes) {
+  const uint32_t i = blockIdx.x;
+  if (!ok[i]) return;
+  const uint32_t p = n0 + rank[i];
+  if (threadIdx.x == 0) {
+    const int4 c = coords[i];
+    const uint32_t e = entry[i];
+    reinterpret_cast<int32_t*>(t.rec + (size_t)(e / kBucket) * kRecWords + kBucket)[e % kBucket] = (int32_t)p;
+    t.blk[p] = make_int4(c.x, c.y, c.z, (int32_t)e);
+    index_block(t, c, p, 0u);
+  }
+  image_copy(bp, p, const_cast<uint8_t*>(images) + (size_t)i * image_bytes, false);
+}
+
+uint32_t block_image_bytes(const BlockPlanes& bp) { return kVox * (bp.word_bytes + 4u + (bp.dmeta ? 12u : 0u)); }
+
+uint32_t stream_scan_words(uint32_t n) { return extract_sums_words(n); }
+
+uint32_t stream_out_select(hipStream_t s, const TableView& t, uint32_t n, const float c[3], float radius, float voxel,
+                           uint8_t* sel, uint32_t* rank, uint32_t* sums) {
+  if (n == 0u) return 0u;
+  hipLaunchKernelGGL(k_so_select, dim3((n + 255u) / 256u), dim3(256), 0, s, t.blk, n, c[0], c[1], c[2], radius, voxel,
+                     sel);
+  launch_scan(s, sel, n, sums, rank);
+  uint32_t last_rank = 0;
+  uint8_t last_sel = 0;
+  if (hipMemcpyAsync(&last_rank, rank + (n - 1u), 4, hipMemcpyDeviceToHost, s) != hipSuccess) return 0xFFFFFFFFu;
+  if (hipMemcpyAsync(&last_sel, sel + (n - 1u), 1, hipMemcpyDeviceToHost, s) != hipSuccess) return 0xFFFFFFFFu;
+  if (hipStreamSynchronize(s) != hipSuccess) return 0xFFFFFFFFu;
+  return last_rank + last_sel;
+}
+
+void stream_out_remove(hipStream_t s, const TableView& t, DevState* st, const BlockPlanes& bp, uint32_t n, uint32_t m,
+                       const uint8_t* sel, const uint32_t* rank, uint32_t* out, uint32_t* remap, int4* coords,
+                       uint8_t* images) {
+  if (m == 0u) return;
+  const uint32_t k = n - m;
+  const uint32_t g = (n + 255u) / 256u;
+  hipLaunchKernelGGL(k_so_list, dim3(g), dim3(256), 0, s, n, k, m, sel, rank, out, remap);
+  hipLaunchKernelGGL(k_so_gather, dim3(m), dim3(256), 0, s, bp, t.blk, out, coords, images, block_image_bytes(bp));
+  hipLaunchKernelGGL(k_so_move, dim3(m), dim3(256), 0, s, bp, t.pcnt, n, k, m, sel, rank, out, remap);
+  hipLaunchKernelGGL(k_so_clear, dim3(m < 2048u ? m : 2048u), dim3(256), 0, s, bp, t.pcnt, k, n);
+  const uint32_t gb = t.num_buckets / 256u + 1u;
+  hipLaunchKernelGGL(k_so_buckets, dim3(gb < 4096u ? gb : 4096u), dim3(256), 0, s, t, remap);
+  hipLaunchKernelGGL(k_clear_maps, dim3(2048), dim3(256), 0, s, t);
+  hipLaunchKernelGGL(k_set_alloc, dim3(1), dim3(1), 0, s, st, k);
+  if (k > 0u) hipLaunchKernelGGL(k_reindex, dim3((k + 255u) / 256u), dim3(256), 0, s, t, k);
+}
+
+uint32_t stream_in_insert(hipStream_t s, const TableView& t, uint32_t n, const int4* coords, uint8_t* status,
+                          uint32_t* entry, uint8_t* ok, uint32_t* rank, uint32_t* sums, bool nospace) {
+  if (n == 0u) return 0u;
+  const uint32_t g = (n + 255u) / 256u;
+  hipLaunchKernelGGL(k_si_init, dim3(g), dim3(256), 0, s, coords, n, status);
+  // each round inserts one input per bucket; after kBucket + 1 rounds every input is
+  // inserted, live, or facing a full bucket
+  for (int round = 0; round <= kBucket; ++round) {
+    hipLaunchKernelGGL(k_si_claim, dim3(g), dim3(256), 0, s, t, coords, n, status, (int32_t)nospace);
+    if (nospace) break;
+    hipLaunchKernelGGL(k_si_insert, dim3(g), dim3(256), 0, s, t, coords, n, status, entry);
+  }
+  hipLaunchKernelGGL(k_si_flags, dim3(g), dim3(256), 0, s, n, status, ok);
+  launch_scan(s, ok, n, sums, rank);
+  uint32_t last_rank = 0;
+  uint8_t last_ok = 0;
+  if (hipMemcpyAsync(&last_rank, rank + (n - 1u), 4, hipMemcpyDeviceToHost, s) != hipSuccess) return 0xFFFFFFFFu;
+  if (hipMemcpyAsync(&last_ok, ok + (n - 1u), 1, hipMemcpyDeviceToHost, s) != hipSuccess) return 0xFFFFFFFFu;
+  if (hipStreamSynchronize(s) != hipSuccess) return 0xFFFFFFFFu;
+  return last_rank + last_ok;
+}
+
+void stream_in_place(hipStream_t s, bool fp16, const TableView& t, DevState* st, const BlockPlanes& bp, uint32_t n,
+                     uint32_t n0, uint32_t inserted, const int4* coords, const uint8_t* ok, const uint32_t* rank,
+                     const uint32_t* entry, const uint8_t* images) {
+  if (inserted == 0u) return;
+  hipLaunchKernelGGL(k_si_place, dim3(n), dim3(256), 0, s, t, bp, coords, ok, rank, entry, n0, images,
+                     block_image_bytes(bp));
+  hipLaunchKernelGGL(k_set_alloc, dim3(1), dim3(1), 0, s, st, n0 + inserted);
+  launch_cell_masks(s, fp16, t, st, n0, inserted, bp.vox);
+}
+
 }  // namespace vsg

@@ -125,6 +125,11 @@ _SIGNATURES = {
                                    C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "vsg_import_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_export_cell_masks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "vsg_block_image_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
+    "vsg_stream_out": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_uint32, C.c_void_p, C.c_void_p,
+                                 C.POINTER(C.c_uint32)]),
+    "vsg_stream_in": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.POINTER(C.c_uint32)]),
     "vsg_diag_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsg_diag_convert": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32]),
     "vsg_diag_read": (C.c_int, [C.c_void_p, C.POINTER(vsg_diag), C.c_int32]),
@@ -384,6 +389,42 @@ class VoxelScene:
         if nblocks:
             _check(lib().vsg_export_cell_masks(self._h, first, nblocks, masks.ctypes.data, counts.ctypes.data))
         return masks, counts
+
+    # -- streaming (VoxelStreamingManager.cu:36-91,258-358,443-719) -------------
+    def block_image_bytes(self) -> int:
+        n = C.c_uint32()
+        _check(lib().vsg_block_image_bytes(self._h, C.byref(n)))
+        return n.value
+
+    def stream_out(self, center, radius: float, capacity: int = 100000):
+        """Removes up to `capacity` blocks at distance >= radius from center (block
+        world position = coordinate * 8 * voxel size).  Returns (coords int32 [m, 3],
+        images uint8 [m, block_image_bytes()]); decode_block_images reads the images."""
+        c = np.ascontiguousarray(center, np.float32).reshape(3)
+        nb = self.block_image_bytes()
+        cap = int(capacity)
+        coords = np.zeros((cap, 3), np.int32)
+        images = np.zeros((cap, nb), np.uint8)
+        n = C.c_uint32()
+        _check(lib().vsg_stream_out(self._h, c.ctypes.data, float(radius), cap, coords.ctypes.data,
+                                    images.ctypes.data, C.byref(n)))
+        return coords[:n.value].copy(), images[:n.value].copy()
+
+    def stream_in(self, coords, images) -> np.ndarray:
+        """Inserts blocks (stream_out's layout) in order; returns the status per block:
+        0 inserted, 1 already allocated, 2 home bucket full, 3 no free block, 4 out of
+        the key range."""
+        coords = np.ascontiguousarray(coords, np.int32).reshape(-1, 3)
+        images = np.ascontiguousarray(images, np.uint8)
+        n = len(coords)
+        if images.shape != (n, self.block_image_bytes()):
+            raise VSGError(1, "stream_in: images must be [n, block_image_bytes()] bytes")
+        status = np.zeros(n, np.uint8)
+        got = C.c_uint32()
+        _check(lib().vsg_stream_in(self._h, n, coords.ctypes.data if n else None,
+                                   images.ctypes.data if n else None, status.ctypes.data if n else None,
+                                   C.byref(got)))
+        return status
 
     def export_blocks(self):
         """(coords int32 [n,3], hash entry indices int32 [n]) in pointer order."""
@@ -991,3 +1032,191 @@ def timing_read() -> dict:
     n = (C.c_uint32 * len(KERNELS))()
     _check(lib().vsg_timing_read(ms, n))
     return {k: (ms[i], n[i]) for i, k in enumerate(KERNELS)}
+
+
+# ---- streaming: VoxelStreamingManager (VoxelSG/include/VoxelStreamingManager.h:120-301) ----
+
+def decode_block_images(images: np.ndarray, store: str = "fp32", directional: bool = False) -> dict:
+    """Voxels of block images (vsg_block_image_bytes' layout): sdf float32 [m, 512],
+    weight uint16 [m, 512], rgbf uint8 [m, 512, 4], and with directional the slot ids
+    and weights uint8 [m, 512, 2] and dtsdf float32 [m, 512, 2]."""
+    im = np.ascontiguousarray(images, np.uint8)
+    m = im.shape[0]
+    wb = 8 if store == "fp32" else 4
+    vox = im[:, :512 * wb]
+    if wb == 8:
+        words = vox.view(np.uint32).reshape(m, 512, 2)
+        sdf = words[:, :, 0].view(np.float32).copy()
+        weight = (words[:, :, 1] & 0xFFFF).astype(np.uint16)
+    else:
+        words = vox.view(np.uint32).reshape(m, 512)
+        sdf = (words & 0xFFFF).astype(np.uint16).view(np.float16).astype(np.float32)
+        weight = (words >> 16).astype(np.uint16)
+    off = 512 * wb
+    out = {"sdf": sdf, "weight": weight, "rgbf": im[:, off:off + 2048].reshape(m, 512, 4).copy()}
+    if directional:
+        meta = im[:, off + 2048:off + 4096].reshape(m, 512, 4)
+        out["dir_id"] = meta[:, :, 0::2].copy()
+        out["dir_w"] = meta[:, :, 1::2].copy()
+        out["dtsdf"] = im[:, off + 4096:off + 8192].copy().view(np.float32).reshape(m, 512, 2)
+    return out
+
+
+class VoxelStreamingManager:
+    """VoxelStreamingManager (VoxelSG/include/VoxelStreamingManager.h:120-301,
+    VoxelSG/src/VoxelStreamingManager.cpp): blocks far from a sphere centre move from
+    the scene (HBM) to a host chunk grid and come back when the sphere reaches them.
+
+    The chunk grid has gridDimensions chunks of voxelExtents metres from minGridPos
+    (chunk units).  A block belongs to the chunk of its world position (coordinate *
+    8 * voxel size) rounded half away from zero (worldToChunks, .cpp:905-922); blocks
+    outside the grid are dropped as the reference drops them (updateChunkGrid,
+    .cpp:875-879) and counted in droppedOutsideGrid.  The four stages keep the
+    reference's names; multiThreaded is accepted and ignored (every stage runs
+    synchronously on the caller's thread)."""
+
+    def __init__(self, scene: VoxelScene, params: Params, voxelExtents, gridDimensions, minGridPos,
+                 initialChunkListSize: int = 1000, streamingEnabled: bool = False):
+        self.scene = scene
+        self.params = params
+        self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash = 100000  # VoxelStreamingManager.cpp:113
+        self.initialize(voxelExtents, gridDimensions, minGridPos, initialChunkListSize, streamingEnabled)
+
+    def initialize(self, voxelExtents, gridDimensions, minGridPos, initialChunkListSize: int = 1000,
+                   streamingEnabled: bool = False) -> None:
+        self.m_voxelExtents = np.asarray(voxelExtents, np.float32).reshape(3)
+        self.m_gridDimensions = np.asarray(gridDimensions, np.int64).reshape(3)
+        self.m_minGridPos = np.asarray(minGridPos, np.int64).reshape(3)
+        self.m_maxGridPos = self.m_minGridPos + self.m_gridDimensions
+        self.m_initialChunkListSize = int(initialChunkListSize)
+        self.m_grid: dict[int, list] = {}  # chunk index -> [coords list, images list]
+        self.droppedOutsideGrid = 0
+        self.nStreamedOut = 0
+        self.nStreamedIn = 0
+        self._out = (np.zeros((0, 3), np.int32), np.zeros((0, 0), np.uint8))
+        self._in = (np.zeros((0, 3), np.int32), np.zeros((0, 0), np.uint8))
+
+    # -- chunk geometry (VoxelStreamingManager.cpp:905-990) ----------------------
+    def worldToChunks(self, posWorld) -> np.ndarray:
+        p = np.asarray(posWorld, np.float32).reshape(-1, 3) / self.m_voxelExtents
+        s = np.sign(p).astype(np.float32)
+        return np.trunc(p + s * np.float32(0.5)).astype(np.int64)
+
+    def blockWorldPos(self, coords) -> np.ndarray:
+        return (np.asarray(coords, np.int64).reshape(-1, 3) * 8).astype(np.float32) * np.float32(self.params.voxelSize)
+
+    def isValidChunk(self, chunk) -> np.ndarray:
+        c = np.asarray(chunk, np.int64).reshape(-1, 3)
+        return np.all((c >= self.m_minGridPos) & (c < self.m_maxGridPos), axis=1)
+
+    def linearizeChunkPos(self, chunk) -> np.ndarray:
+        p = np.asarray(chunk, np.int64).reshape(-1, 3) - self.m_minGridPos
+        d = self.m_gridDimensions
+        return p[:, 2] * d[0] * d[1] + p[:, 1] * d[0] + p[:, 0]
+
+    def isChunkInSphere(self, chunk, center, radius: float) -> bool:
+        cw = np.asarray(chunk, np.float32).reshape(3) * self.m_voxelExtents
+        chunk_radius = np.float32(0.5) * np.float32(self.m_voxelExtents.max()) * np.float32(np.sqrt(np.float32(3.0)))
+        d = cw - np.asarray(center, np.float32).reshape(3)
+        dist = np.float32(np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1] + d[2] * d[2])))
+        return bool(dist - chunk_radius <= np.float32(radius))
+
+    # -- stream-out ---------------------------------------------------------------
+    def streamOutFindBlocksOnGPU(self, sphereCenter, radius: float, useParts: bool = False,
+                                 multiThreaded: bool = False) -> int:
+        """Removes the blocks at distance >= radius from sphereCenter from the scene
+        (at most m_maxNumberOfSDFBlocksIntegrateFromGlobalHash per call) and holds
+        them for streamOutCopyToChunkGrid.  Returns how many."""
+        coords, images = self.scene.stream_out(sphereCenter, radius, self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash)
+        self._out = (coords, images)
+        self.nStreamedOut += len(coords)
+        return len(coords)
+
+    def streamOutCopyToChunkGrid(self, multiThreaded: bool = False) -> None:
+        coords, images = self._out
+        self.updateChunkGrid(coords, images)
+        self._out = (np.zeros((0, 3), np.int32), np.zeros((0, 0), np.uint8))
+
+    def updateChunkGrid(self, coords, images) -> None:
+        if len(coords) == 0:
+            return
+        chunks = self.worldToChunks(self.blockWorldPos(coords))
+        valid = self.isValidChunk(chunks)
+        self.droppedOutsideGrid += int((~valid).sum())
+        idx = self.linearizeChunkPos(chunks)
+        for i in np.nonzero(valid)[0]:
+            entry = self.m_grid.setdefault(int(idx[i]), [[], []])
+            entry[0].append(np.asarray(coords[i], np.int32))
+            entry[1].append(np.asarray(images[i], np.uint8))
+
+    # -- stream-in ------------------------------------------------------------------
+    def streamInFindChunk(self, sphereCenter, radius: float, useParts: bool = False) -> int:
+        """Collects the blocks of the streamed-out chunks that the sphere touches (the
+        chunk box around the centre's chunk, isChunkInSphere), removing them from the
+        grid (VoxelStreamingManager.cpp:993-1130).  Returns how many."""
+        cam = self.worldToChunks(sphereCenter)[0]
+        rad = np.ceil(np.float32(radius) / self.m_voxelExtents).astype(np.int64)
+        lo = np.maximum(cam - rad, self.m_minGridPos)
+        hi = np.minimum(cam + rad, self.m_maxGridPos - 1)
+        cs, ims = [], []
+        cap = self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash
+        for x in range(lo[0], hi[0] + 1):
+            for y in range(lo[1], hi[1] + 1):
+                for z in range(lo[2], hi[2] + 1):
+                    ci = int(self.linearizeChunkPos((x, y, z))[0])
+                    entry = self.m_grid.get(ci)
+                    if not entry or not entry[0] or not self.isChunkInSphere((x, y, z), sphereCenter, radius):
+                        continue
+                    if len(entry[0]) + len(cs) > cap:
+                        self._set_in(cs, ims)
+                        return len(cs)
+                    cs.extend(entry[0])
+                    ims.extend(entry[1])
+                    del self.m_grid[ci]
+                    if useParts:
+                        self._set_in(cs, ims)
+                        return len(cs)
+        self._set_in(cs, ims)
+        return len(cs)
+
+    def _set_in(self, cs, ims) -> None:
+        nb = self.scene.block_image_bytes()
+        self._in = (np.array(cs, np.int32).reshape(-1, 3),
+                    np.array(ims, np.uint8).reshape(-1, nb) if ims else np.zeros((0, nb), np.uint8))
+
+    def streamInCopyToGPUBuffer(self, sphereCenter, radius: float, useParts: bool = False,
+                                multiThreaded: bool = False) -> int:
+        return self.streamInFindChunk(sphereCenter, radius, useParts)
+
+    def streamInInsertToHashTable(self, multiThreaded: bool = False) -> np.ndarray:
+        """Inserts the collected blocks; those that could not be inserted (bucket full,
+        no free block) go back to the chunk grid.  Returns the status per block."""
+        coords, images = self._in
+        self._in = (np.zeros((0, 3), np.int32), np.zeros((0, 0), np.uint8))
+        if len(coords) == 0:
+            return np.zeros(0, np.uint8)
+        status = self.scene.stream_in(coords, images)
+        self.nStreamedIn += int((status == 0).sum())
+        back = (status == 2) | (status == 3)
+        if back.any():
+            self.updateChunkGrid(coords[back], images[back])
+        return status
+
+    # -- one-call forms (VoxelStreamingManager.h: streamOut / streamIn) -------------
+    def streamOut(self, scene: VoxelScene, cameraPos, radius: float) -> int:
+        """Moves every block at distance >= radius from cameraPos to the chunk grid."""
+        total = 0
+        while True:
+            n = self.streamOutFindBlocksOnGPU(cameraPos, radius)
+            self.streamOutCopyToChunkGrid()
+            total += n
+            if n < self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash:
+                return total
+
+    def streamIn(self, scene: VoxelScene, cameraPos, radius: float) -> np.ndarray:
+        """Moves the blocks of the chunks the sphere touches back into the scene."""
+        self.streamInCopyToGPUBuffer(cameraPos, radius)
+        return self.streamInInsertToHashTable()
+
+    def numStreamedBlocks(self) -> int:
+        return sum(len(e[0]) for e in self.m_grid.values())
