@@ -364,6 +364,8 @@ def main() -> int:
                     help="skip timing the surface-point extraction of the fused scene")
     ap.add_argument("--no-mesh", action="store_true",
                     help="skip timing the marching cubes of the fused scene")
+    ap.add_argument("--no-stream", action="store_true",
+                    help="skip timing a stream-out / stream-in round trip of the fused scene")
     ap.add_argument("--no-depthgen", action="store_true",
                     help="skip timing the depth maps from a point cloud")
     ap.add_argument("--no-pose", action="store_true",
@@ -517,6 +519,32 @@ def main() -> int:
                               "gbs": round(mgbs, 1), "frac": round(mgbs / HBM_PEAK_GBS, 5)}
                 log(f"[{store}] marching cubes {name}: {ntri} triangles in {mms / mn:.3f} ms")
             vsg.freeMarchingCubesOutput(out)
+        streaming = None
+        if not args.no_stream:
+            # streaming round trip of the fused scene: the blocks beyond the median block
+            # distance from the last camera go to pinned host memory and come back (wall
+            # clock of the synchronising calls, PCIe copies of the block images included)
+            torch.cuda.synchronize()
+            coords, _ = scene.export_blocks()
+            cam = frames[nf - 1]["pose"][:3, 3]
+            dist = np.linalg.norm(coords.astype(np.float64) * 8 * wl.voxel - cam, axis=1)
+            radius = float(np.median(dist))
+            nbi = scene.block_image_bytes()
+            hc = torch.empty((len(coords), 3), dtype=torch.int32, pin_memory=True).numpy()
+            him = torch.empty((len(coords), nbi), dtype=torch.uint8, pin_memory=True).numpy()
+            t0 = time.perf_counter()
+            oc, oim = scene.stream_out(cam.astype(np.float32), radius, capacity=len(coords), coords_out=hc,
+                                       images_out=him)
+            t_out = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            st_in = scene.stream_in(oc, oim)
+            t_in = time.perf_counter() - t0
+            assert np.all(st_in == 0) and scene.counters()["num_allocated"] == len(coords)
+            streaming = {"blocks": int(len(oc)), "of_blocks": int(len(coords)), "image_bytes": int(oim.nbytes),
+                      "out_ms": round(t_out * 1e3, 3), "in_ms": round(t_in * 1e3, 3),
+                      "out_gbs": round(oim.nbytes / t_out / 1e9, 2), "in_gbs": round(oim.nbytes / t_in / 1e9, 2)}
+            log(f"[{store}] streaming: {len(oc)} of {len(coords)} blocks out in {t_out * 1e3:.2f} ms, "
+                f"in in {t_in * 1e3:.2f} ms")
         vsg.timing_enable(False)
         scene.close()
         rr.close()
@@ -528,7 +556,7 @@ def main() -> int:
             fb += frame_bytes(st, npix, b_sw)
         log(f"[{store}] {t_total:.3f} s for {args.steps * nf} frames")
         return {"t_total": t_total, "ktimes": ktimes, "kbytes": kb, "fbytes": fb, "stats": stats, "extract": ext,
-                "mesh": mesh}
+                "mesh": mesh, "stream": streaming}
 
     res = measure(args.store)
     t_total, ktimes = res["t_total"], res["ktimes"]
@@ -581,6 +609,8 @@ def main() -> int:
         extra["surface_points"] = res["extract"]
     if res["mesh"] is not None:
         extra["marching_cubes"] = res["mesh"]
+    if res["stream"] is not None:
+        extra["streaming"] = res["stream"]
     if not args.no_depthgen:
         extra["depth_from_points"] = depthgen_bench(torch, vsg, sc, frames)
     if not args.no_pose and nf >= 128:

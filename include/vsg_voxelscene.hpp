@@ -18,6 +18,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -242,6 +243,194 @@ class VoxelScene {
   Params m_params;
   vsg_scene* m_scene = nullptr;
   bool m_isAllocated = false;
+};
+
+// VoxelStreamingManager (VoxelSG/include/VoxelStreamingManager.h:120-301): blocks far
+// from a sphere centre move from the scene to a host chunk grid (gridDimensions
+// chunks of voxelExtents metres from minGridPos, in chunk units) and back when the
+// sphere reaches them.  A block belongs to the chunk of its world position
+// (coordinate * 8 * voxel size) rounded half away from zero (worldToChunks,
+// VoxelStreamingManager.cpp:905-922); blocks outside the grid are dropped, as the
+// reference drops them (updateChunkGrid, :875-879), and counted.  The four stages
+// keep the reference's names and order; they run synchronously on the caller's
+// thread, so multiThreaded and streamingEnabled are accepted and ignored (the
+// reference's worker thread and Windows events have no counterpart).
+class VoxelStreamingManager {
+ public:
+  VoxelStreamingManager(VoxelScene& scene, const Params& params, const float3& voxelExtents, const int3& gridDimensions,
+                        const int3& minGridPos, unsigned int initialChunkListSize = 1000, bool streamingEnabled = false)
+      : m_scene(&scene), m_params(params) {
+    initialize(voxelExtents, gridDimensions, minGridPos, initialChunkListSize, streamingEnabled);
+  }
+
+  void initialize(const float3& voxelExtents, const int3& gridDimensions, const int3& minGridPos,
+                  unsigned int initialChunkListSize, bool /*streamingEnabled*/) {
+    m_voxelExtents = voxelExtents;
+    m_gridDimensions = gridDimensions;
+    m_minGridPos = minGridPos;
+    m_maxGridPos = make_int3(minGridPos.x + gridDimensions.x, minGridPos.y + gridDimensions.y,
+                             minGridPos.z + gridDimensions.z);
+    m_initialChunkListSize = initialChunkListSize;
+    m_grid.assign((size_t)gridDimensions.x * gridDimensions.y * gridDimensions.z, Chunk{});
+    vsg_block_image_bytes(m_scene->handle(), &m_imageBytes);
+  }
+
+  // Removes the blocks at distance >= radius from sphereCenter from the scene (at
+  // most m_maxNumberOfSDFBlocksIntegrateFromGlobalHash per call) and holds them for
+  // streamOutCopyToChunkGrid.  Returns how many.
+  unsigned int streamOutFindBlocksOnGPU(const float3& sphereCenter, float radius, bool /*useParts*/ = false,
+                                        bool /*multiThreaded*/ = false) {
+    const uint32_t cap = m_maxNumberOfSDFBlocksIntegrateFromGlobalHash;
+    m_outCoords.resize((size_t)cap * 3);
+    m_outImages.resize((size_t)cap * m_imageBytes);
+    const float c[3] = {sphereCenter.x, sphereCenter.y, sphereCenter.z};
+    uint32_t n = 0;
+    vsg_detail::report(vsg_stream_out(m_scene->handle(), c, radius, cap, m_outCoords.data(), m_outImages.data(), &n),
+                       "VoxelStreamingManager::streamOutFindBlocksOnGPU");
+    nStreamdOutBlocks_ = n;
+    return n;
+  }
+
+  void streamOutCopyToChunkGrid(bool /*multiThreaded*/ = false) {
+    updateChunkGrid(m_outCoords.data(), m_outImages.data(), nStreamdOutBlocks_);
+    nStreamdOutBlocks_ = 0;
+  }
+
+  // Collects the blocks of the streamed-out chunks the sphere touches (the chunk box
+  // around the centre's chunk, isChunkInSphere) and empties those chunks
+  // (streamInFindChunk, VoxelStreamingManager.cpp:993-1130).  Returns how many.
+  unsigned int streamInCopyToGPUBuffer(const float3& sphereCenter, float radius, bool useParts = false,
+                                       bool /*multiThreaded*/ = false) {
+    m_inCoords.clear();
+    m_inImages.clear();
+    const int3 cam = worldToChunks(sphereCenter);
+    const int3 rad = make_int3((int)std::ceil(radius / m_voxelExtents.x), (int)std::ceil(radius / m_voxelExtents.y),
+                               (int)std::ceil(radius / m_voxelExtents.z));
+    const int3 lo = make_int3(std::max(cam.x - rad.x, m_minGridPos.x), std::max(cam.y - rad.y, m_minGridPos.y),
+                              std::max(cam.z - rad.z, m_minGridPos.z));
+    const int3 hi = make_int3(std::min(cam.x + rad.x, m_maxGridPos.x - 1), std::min(cam.y + rad.y, m_maxGridPos.y - 1),
+                              std::min(cam.z + rad.z, m_maxGridPos.z - 1));
+    for (int x = lo.x; x <= hi.x; ++x)
+      for (int y = lo.y; y <= hi.y; ++y)
+        for (int z = lo.z; z <= hi.z; ++z) {
+          const int3 ch = make_int3(x, y, z);
+          Chunk& c = m_grid[linearizeChunkPos(ch)];
+          if (c.coords.empty() || !isChunkInSphere(ch, sphereCenter, radius)) continue;
+          const size_t nb = c.coords.size() / 3;
+          if (nb + m_inCoords.size() / 3 > m_maxNumberOfSDFBlocksIntegrateFromGlobalHash) return nStreamdInBlocks();
+          m_inCoords.insert(m_inCoords.end(), c.coords.begin(), c.coords.end());
+          m_inImages.insert(m_inImages.end(), c.images.begin(), c.images.end());
+          c = Chunk{};
+          if (useParts) return nStreamdInBlocks();
+        }
+    return nStreamdInBlocks();
+  }
+
+  // Inserts the collected blocks; blocks that find their bucket full or no free block
+  // go back to the chunk grid.  Returns how many were inserted.
+  unsigned int streamInInsertToHashTable(bool /*multiThreaded*/ = false) {
+    const uint32_t n = nStreamdInBlocks();
+    if (n == 0) return 0;
+    std::vector<uint8_t> status(n);
+    uint32_t inserted = 0;
+    vsg_detail::report(vsg_stream_in(m_scene->handle(), n, m_inCoords.data(), m_inImages.data(), status.data(),
+                                     &inserted),
+                       "VoxelStreamingManager::streamInInsertToHashTable");
+    std::vector<int32_t> bc;
+    std::vector<uint8_t> bi;
+    for (uint32_t i = 0; i < n; ++i)
+      if (status[i] == 2 || status[i] == 3) {
+        bc.insert(bc.end(), &m_inCoords[3 * (size_t)i], &m_inCoords[3 * (size_t)i] + 3);
+        bi.insert(bi.end(), &m_inImages[(size_t)i * m_imageBytes], &m_inImages[(size_t)(i + 1) * m_imageBytes]);
+      }
+    if (!bc.empty()) updateChunkGrid(bc.data(), bi.data(), (uint32_t)(bc.size() / 3));
+    m_inCoords.clear();
+    m_inImages.clear();
+    return inserted;
+  }
+
+  // One-call forms (VoxelStreamingManager.h: streamOut / streamIn).
+  void streamOut(VoxelScene& /*scene*/, const float3& cameraPos, float radius) {
+    for (;;) {
+      const unsigned int n = streamOutFindBlocksOnGPU(cameraPos, radius);
+      streamOutCopyToChunkGrid();
+      if (n < m_maxNumberOfSDFBlocksIntegrateFromGlobalHash) return;
+    }
+  }
+  void streamIn(VoxelScene& /*scene*/, const float3& cameraPos, float radius) {
+    streamInCopyToGPUBuffer(cameraPos, radius);
+    streamInInsertToHashTable();
+  }
+
+  size_t numStreamedBlocks() const {
+    size_t n = 0;
+    for (const Chunk& c : m_grid) n += c.coords.size() / 3;
+    return n;
+  }
+  size_t droppedOutsideGrid() const { return m_dropped; }
+
+  unsigned int m_maxNumberOfSDFBlocksIntegrateFromGlobalHash = 100000;  // VoxelStreamingManager.cpp:113
+
+ private:
+  struct Chunk {
+    std::vector<int32_t> coords;  // 3 per block
+    std::vector<uint8_t> images;  // vsg_block_image_bytes per block
+  };
+
+  unsigned int nStreamdInBlocks() const { return (unsigned int)(m_inCoords.size() / 3); }
+
+  void updateChunkGrid(const int32_t* coords, const uint8_t* images, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) {
+      const float v = m_params.voxelSize;
+      const float3 w = make_float3((float)(coords[3 * i] * SDF_BLOCK_SIZE) * v,
+                                   (float)(coords[3 * i + 1] * SDF_BLOCK_SIZE) * v,
+                                   (float)(coords[3 * i + 2] * SDF_BLOCK_SIZE) * v);
+      const int3 ch = worldToChunks(w);
+      if (!isValidChunk(ch)) {
+        ++m_dropped;
+        continue;
+      }
+      Chunk& c = m_grid[linearizeChunkPos(ch)];
+      if (c.coords.empty()) {
+        c.coords.reserve((size_t)m_initialChunkListSize * 3);
+      }
+      c.coords.insert(c.coords.end(), coords + 3 * (size_t)i, coords + 3 * (size_t)i + 3);
+      c.images.insert(c.images.end(), images + (size_t)i * m_imageBytes, images + (size_t)(i + 1) * m_imageBytes);
+    }
+  }
+
+  int3 worldToChunks(const float3& p) const {
+    const float x = p.x / m_voxelExtents.x, y = p.y / m_voxelExtents.y, z = p.z / m_voxelExtents.z;
+    auto sgn = [](float q) { return q > 0.0f ? 1.0f : (q < 0.0f ? -1.0f : 0.0f); };
+    return make_int3((int)(x + sgn(x) * 0.5f), (int)(y + sgn(y) * 0.5f), (int)(z + sgn(z) * 0.5f));
+  }
+  bool isValidChunk(const int3& c) const {
+    return c.x >= m_minGridPos.x && c.y >= m_minGridPos.y && c.z >= m_minGridPos.z && c.x < m_maxGridPos.x &&
+           c.y < m_maxGridPos.y && c.z < m_maxGridPos.z;
+  }
+  size_t linearizeChunkPos(const int3& c) const {
+    return (size_t)(c.z - m_minGridPos.z) * m_gridDimensions.x * m_gridDimensions.y +
+           (size_t)(c.y - m_minGridPos.y) * m_gridDimensions.x + (size_t)(c.x - m_minGridPos.x);
+  }
+  bool isChunkInSphere(const int3& c, const float3& center, float radius) const {
+    const float ext = std::max(std::max(m_voxelExtents.x, m_voxelExtents.y), m_voxelExtents.z);
+    const float chunkRadius = 0.5f * ext * std::sqrt(3.0f);
+    const float dx = c.x * m_voxelExtents.x - center.x, dy = c.y * m_voxelExtents.y - center.y,
+                dz = c.z * m_voxelExtents.z - center.z;
+    return std::sqrt(dx * dx + dy * dy + dz * dz) - chunkRadius <= radius;
+  }
+
+  VoxelScene* m_scene;
+  Params m_params;
+  float3 m_voxelExtents{};
+  int3 m_gridDimensions{}, m_minGridPos{}, m_maxGridPos{};
+  unsigned int m_initialChunkListSize = 1000;
+  uint32_t m_imageBytes = 0;
+  std::vector<Chunk> m_grid;
+  std::vector<int32_t> m_outCoords, m_inCoords;
+  std::vector<uint8_t> m_outImages, m_inImages;
+  uint32_t nStreamdOutBlocks_ = 0;
+  size_t m_dropped = 0;
 };
 
 // RayCastRender (VoxelSG/include/RayCastRender.h:20-123)

@@ -15,9 +15,15 @@
 // Then runMarchingCubesCUDA with the reference's cubes (inside blocks, no weight test)
 // and, when a mesh path is given, VoxelScene::saveToMesh.
 //
-// usage: vsg_shim_example <workload 0|1> <frames> [points.ply|-] [gen|-] [mesh.ply]
+// With a sixth argument "stream:<radius>", a VoxelStreamingManager (1 m chunks, 64^3
+// chunks from -32) runs before every integration from the second frame on: stream-out of
+// the blocks at distance >= radius from the camera, then stream-in of the chunks within
+// radius (the order of VoxelSG/src/voxelSG.cpp:503-552).
+//
+// usage: vsg_shim_example <workload 0|1> <frames> [points.ply|-] [gen|-] [mesh.ply|-] [stream:<radius>]
 // prints: frames, allocated blocks, hits, misses, a checksum of the last depth map,
-// the number of surface points found and the number of triangles.
+// the number of surface points found, the number of triangles and the number of
+// blocks held in the streaming chunk grid.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -78,6 +84,12 @@ int main(int argc, char** argv) {
   std::vector<float> cp, cn;
   std::vector<uint8_t> cc;
 
+  const std::string streamArg = argc > 6 ? argv[6] : "";
+  const bool streaming = streamArg.rfind("stream:", 0) == 0;
+  const float streamRadius = streaming ? std::strtof(streamArg.c_str() + 7, nullptr) : 0.0f;
+  VoxelStreamingManager streamer(scene, params, make_float3(1.0f, 1.0f, 1.0f), make_int3(64, 64, 64),
+                                 make_int3(-32, -32, -32));
+
   unsigned int hits = 0, misses = 0;
   double checksum = 0.0;
   for (int i = 0; i < nframes; ++i) {
@@ -90,6 +102,11 @@ int main(int argc, char** argv) {
         hipMemcpy(d_rgb, rgb.data(), 3 * n, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(d_nrm, nrm.data(), 12 * n, hipMemcpyHostToDevice) != hipSuccess)
       return 3;
+    if (streaming && i > 0) {
+      const float3 cam = make_float3(T[3], T[7], T[11]);
+      streamer.streamOut(scene, cam, streamRadius);
+      streamer.streamIn(scene, cam, streamRadius);
+    }
     if (viaGenerator) {
       cp.clear();
       cn.clear();
@@ -135,10 +152,10 @@ int main(int argc, char** argv) {
   runMarchingCubesCUDA(scene, params, mcp, mc);
   const long long triangles = copyMarchingCubesTriangleCount(mc);
   freeMarchingCubesOutput(mc);
-  if (argc > 5 && !scene.saveToMesh(argv[5])) return 7;
-  std::printf("frames %d blocks %u hits %u misses %u checksum %.9e points %d triangles %lld\n",
+  if (argc > 5 && std::string(argv[5]) != "-" && !scene.saveToMesh(argv[5])) return 7;
+  std::printf("frames %d blocks %u hits %u misses %u checksum %.9e points %d triangles %lld streamed %zu\n",
               scene.getNumIntegratedFrames(), (unsigned)(params.SDFBlockNum - scene.getHeapFreeCount()), hits, misses,
-              checksum, points, triangles);
+              checksum, points, triangles, streamer.numStreamedBlocks());
   (void)hipFree(d_pts);
   (void)hipFree(d_nrm);
   (void)hipFree(d_rgb);

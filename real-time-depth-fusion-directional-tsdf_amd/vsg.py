@@ -396,19 +396,27 @@ class VoxelScene:
         _check(lib().vsg_block_image_bytes(self._h, C.byref(n)))
         return n.value
 
-    def stream_out(self, center, radius: float, capacity: int = 100000):
+    def stream_out(self, center, radius: float, capacity: int = 100000, coords_out: np.ndarray | None = None,
+                   images_out: np.ndarray | None = None):
         """Removes up to `capacity` blocks at distance >= radius from center (block
         world position = coordinate * 8 * voxel size).  Returns (coords int32 [m, 3],
-        images uint8 [m, block_image_bytes()]); decode_block_images reads the images."""
+        images uint8 [m, block_image_bytes()]), views of coords_out / images_out when
+        given (C-contiguous, at least capacity rows; pinned host memory makes the copy
+        from the device faster); decode_block_images reads the images."""
         c = np.ascontiguousarray(center, np.float32).reshape(3)
         nb = self.block_image_bytes()
         cap = int(capacity)
-        coords = np.zeros((cap, 3), np.int32)
-        images = np.zeros((cap, nb), np.uint8)
+        coords = np.empty((cap, 3), np.int32) if coords_out is None else coords_out
+        images = np.empty((cap, nb), np.uint8) if images_out is None else images_out
+        if (coords.dtype != np.int32 or images.dtype != np.uint8 or not coords.flags.c_contiguous
+                or not images.flags.c_contiguous or coords.shape[0] < cap or images.shape[0] < cap
+                or coords.shape[1:] != (3,) or images.shape[1:] != (nb,)):
+            raise VSGError(1, "stream_out: outputs must be C-contiguous int32 [>= capacity, 3] and uint8 "
+                              "[>= capacity, block_image_bytes()]")
         n = C.c_uint32()
         _check(lib().vsg_stream_out(self._h, c.ctypes.data, float(radius), cap, coords.ctypes.data,
                                     images.ctypes.data, C.byref(n)))
-        return coords[:n.value].copy(), images[:n.value].copy()
+        return coords[:n.value], images[:n.value]
 
     def stream_in(self, coords, images) -> np.ndarray:
         """Inserts blocks (stream_out's layout) in order; returns the status per block:
@@ -1128,7 +1136,7 @@ class VoxelStreamingManager:
         (at most m_maxNumberOfSDFBlocksIntegrateFromGlobalHash per call) and holds
         them for streamOutCopyToChunkGrid.  Returns how many."""
         coords, images = self.scene.stream_out(sphereCenter, radius, self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash)
-        self._out = (coords, images)
+        self._out = (coords.copy(), images.copy())
         self.nStreamedOut += len(coords)
         return len(coords)
 

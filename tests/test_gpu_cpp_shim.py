@@ -72,3 +72,36 @@ def test_cpp_generator_path_matches_python(torch_cuda):
     assert int(got["blocks"]) == g.scene.counters()["num_allocated"]
     assert (int(got["hits"]), int(got["misses"])) == (r["hits"], r["misses"])
     assert abs(float(got["checksum"]) - checksum) <= 1e-9 * abs(checksum)
+
+
+def test_cpp_streaming_manager_matches_python(torch_cuda):
+    """The C++ VoxelStreamingManager (stream-out beyond 1.2 m of the camera, then stream-in
+    within it, before every integration from the second frame on) gives the scene,
+    renders and chunk-grid contents the Python VoxelStreamingManager gives."""
+    frames, radius = 6, 1.2
+    out = subprocess.run([EXE, "1", str(frames), "-", "-", "-", f"stream:{radius}"], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    tok = out.stdout.split()
+    got = {tok[i]: tok[i + 1] for i in range(0, len(tok) - 1, 2)}
+    wl = synth.WORKLOADS[1]
+    sc = synth.SynthScene(1)
+    g = GpuRun(torch_cuda, wl)
+    mgr = vsg.VoxelStreamingManager(g.scene, g.params, (1.0, 1.0, 1.0), (64, 64, 64), (-32, -32, -32))
+    for i in range(frames):
+        f = sc.render(i)
+        if i > 0:
+            cam = f["pose"][:3, 3].astype(np.float32)
+            mgr.streamOut(g.scene, cam, radius)
+            mgr.streamIn(g.scene, cam, radius)
+        g.integrate(f, sc.intrinsics)
+    r = g.raycast(f, sc.intrinsics)
+    d = r["depth"]
+    k = np.arange(d.size)
+    ok = d < 1e9
+    checksum = float(np.sum(d[ok].astype(np.float64) * ((k[ok] % 97) + 1)))
+    assert mgr.numStreamedBlocks() > 0
+    assert int(got["streamed"]) == mgr.numStreamedBlocks()
+    assert int(got["blocks"]) == g.scene.counters()["num_allocated"]
+    assert (int(got["hits"]), int(got["misses"])) == (r["hits"], r["misses"])
+    assert abs(float(got["checksum"]) - checksum) <= 1e-9 * abs(checksum)
