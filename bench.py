@@ -103,7 +103,10 @@ def error_stats(err: np.ndarray) -> dict:
         return {"n": 0}
     return {"n": int(err.size), "max_mm": round(float(err.max()) * 1e3, 5),
             "rms_mm": round(float(np.sqrt(np.mean(err ** 2))) * 1e3, 5),
-            "p999_mm": round(float(np.quantile(err, 0.999)) * 1e3, 5)}
+            "p50_mm": round(float(np.quantile(err, 0.5)) * 1e3, 5),
+            "p99_mm": round(float(np.quantile(err, 0.99)) * 1e3, 5),
+            "p999_mm": round(float(np.quantile(err, 0.999)) * 1e3, 5),
+            "within_0.5mm": round(float(np.mean(err <= 5e-4)), 5)}
 
 
 def precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm, nframes: int, npix_check: int) -> dict:
@@ -128,6 +131,7 @@ def precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm, nframes: int, np
     od = o.render(W, H, T[:3, 3], T, sc.intrinsics, mn, mx, pixels=pix)["depth"][pix]
     okey = {tuple(c): k for k, c in enumerate(oc)}
     out = {"frames": nframes, "pixels": npix_check}
+    gdepth = {}
     for store in ("fp32", "fp16"):
         scene = vsg.VoxelScene(vsg.Params(wl.buckets, wl.blocks, wl.voxel, wl.trunc), store)
         scene.set_stream(torch.cuda.current_stream().cuda_stream)
@@ -138,6 +142,7 @@ def precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm, nframes: int, np
         rr.initialize(W, H)
         rr.render(scene, T[:3, 3], T, fx, fy, cx, cy, mn, mx)
         gd = rr.downloadDepthFloat()[pix]
+        gdepth[store] = gd
         gc, _ = scene.export_blocks()
         gsdf, gw, _ = scene.export_voxels(0, len(gc))
         gi, oi = [], []
@@ -156,6 +161,9 @@ def precision_report(torch, vsg, sc, frames, wl, pts, rgb, nrm, nframes: int, np
                       "blocks_gpu": int(len(gc)), "blocks_fp64": int(len(oc)), "blocks_common": int(len(gi))}
         rr.close()
         scene.close()
+    # what the FP16 store alone changes: the same march over FP16 vs FP32 voxels
+    v = (gdepth["fp16"] < 1e9) & (gdepth["fp32"] < 1e9)
+    out["fp16_vs_fp32_depth"] = error_stats(gdepth["fp16"][v].astype(np.float64) - gdepth["fp32"][v])
     return out
 
 
