@@ -3206,9 +3206,12 @@ __device__ __forceinline__ void wg_copy(void* __restrict__ dst, const void* __re
 // Copies the planes of block p into image `img` (or back when to_image is false).
 __device__ __forceinline__ void image_copy(const BlockPlanes& bp, uint32_t p, uint8_t* img, bool to_image) {
   const uint32_t vb = kVox * bp.word_bytes;
-  uint8_t* planes[4] = {bp.vox + (size_t)p * vb, reinterpret_cast<uint8_t*>(bp.col + (size_t)p * kVox),
-                        reinterpret_cast<uint8_t*>(bp.dmeta + (size_t)p * kVox),
-                        reinterpret_cast<uint8_t*>(bp.dval + (size_t)p * kVox)};
+  uint8_t* planes[4] = {bp.vox + (size_t)p * vb, reinterpret_cast<uint8_t*>(bp.col + (size_t)p * kVox), nullptr,
+                        nullptr};
+  if (bp.dmeta) {
+    planes[2] = reinterpret_cast<uint8_t*>(bp.dmeta + (size_t)p * kVox);
+    planes[3] = reinterpret_cast<uint8_t*>(bp.dval + (size_t)p * kVox);
+  }
   const uint32_t bytes[4] = {vb, kVox * 4u, kVox * 4u, kVox * 8u};
   const int nplanes = bp.dmeta ? 4 : 2;
   uint32_t off = 0;
