@@ -276,7 +276,7 @@ def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
             "ms_per_frame": round(t_gn / 64 * 1e3, 2), "ms_per_iteration": round(t_gn / 640 * 1e3, 3)}
 
 
-def large_run_bench(torch, vsg, synth, nframes: int) -> dict:
+def large_run_bench(torch, vsg, synth, nframes: int, stream_radius: float = 0.0) -> dict:
     """The large combined configuration: 1024x768 room (8x8x3 m, 60 primitives), voxel 2 mm,
     truncation 1 cm, 2^22 buckets and blocks, FP16 store.  Every frame after the first is
     tracked from a prior perturbed by a seeded uniform <= 1 deg / 1 cm (10 Gauss-Newton
@@ -285,7 +285,9 @@ def large_run_bench(torch, vsg, synth, nframes: int) -> dict:
     The tracking runs against a companion model of the same frames (voxel 5 mm,
     truncation 8 cm: wider than the sensor noise, 1-2.6 cm at 2-4 m), integrated at the
     same refined poses; the 2 mm / 1 cm model's noisy shell pulls the alignment away from
-    the true pose.  Per-frame times of the stages and the pose errors."""
+    the true pose.  With stream_radius > 0, every 25 frames the 2 mm scene's blocks beyond
+    that distance from the camera go to a VoxelStreamingManager chunk grid (1 m chunks) and
+    the chunks within it come back.  Per-frame times of the stages and the pose errors."""
     wl = synth.WORKLOADS[4]
     sc = synth.SynthScene(4, nframes=nframes)
     fx, fy, cx, cy = sc.intrinsics
@@ -293,6 +295,10 @@ def large_run_bench(torch, vsg, synth, nframes: int) -> dict:
     scene = vsg.VoxelScene(vsg.Params(wl.buckets, wl.blocks, wl.voxel, wl.trunc), "fp16")
     stream = torch.cuda.current_stream().cuda_stream
     scene.set_stream(stream)
+    mgr = None
+    t_stream, max_blocks, n_stream = 0.0, 0, 0
+    if stream_radius > 0:
+        mgr = vsg.VoxelStreamingManager(scene, scene.m_params, (1.0, 1.0, 1.0), (64, 64, 64), (-32, -32, -32))
     track_voxel, track_trunc = 0.005, 0.08
     tscene = vsg.VoxelScene(vsg.Params(1 << 20, 1 << 20, track_voxel, track_trunc), "fp16")
     tscene.set_stream(stream)
@@ -337,6 +343,13 @@ def large_run_bench(torch, vsg, synth, nframes: int) -> dict:
         rr.render(scene, Tf[:3, 3], Tf, fx, fy, cx, cy, mn, mx)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
+        if mgr is not None and i % 25 == 24:
+            max_blocks = max(max_blocks, scene.counters()["num_allocated"])
+            ts0 = time.perf_counter()
+            mgr.streamOut(scene, Tf[:3, 3], stream_radius)
+            mgr.streamIn(scene, Tf[:3, 3], stream_radius)
+            t_stream += time.perf_counter() - ts0
+            n_stream += 1
         t_trk += t1 - t0
         t_int += t2 - t1
         t_ray += t3 - t2
@@ -351,6 +364,10 @@ def large_run_bench(torch, vsg, synth, nframes: int) -> dict:
     return {"workload": "room 1024x768, voxel 2 mm, trunc 0.01 m, 2^22 buckets, FP16 store", "frames": nframes,
             "blocks": int(blocks),
             "tracking_model": {"voxel": track_voxel, "trunc": track_trunc, "blocks": int(tblocks)},
+            **({"streaming": {"radius": stream_radius, "every_frames": 25, "max_blocks_before": int(max_blocks),
+                              "blocks_in_chunk_grid": int(mgr.numStreamedBlocks()),
+                              "dropped_outside_grid": int(mgr.droppedOutsideGrid),
+                              "ms_per_round_trip": round(t_stream / max(n_stream, 1) * 1e3, 2)}} if mgr else {}),
             "ms_per_frame": {"track_10_iterations": round(t_trk / max(nframes - 1, 1) * 1e3, 3),
                              "integrate": round(t_int / nframes * 1e3, 3), "raycast": round(t_ray / nframes * 1e3, 3),
                              "integrate_tracking_model": round(t_tint / nframes * 1e3, 3)},
@@ -388,6 +405,8 @@ def main() -> int:
                     help="skip timing the surface-point extraction of the fused scene")
     ap.add_argument("--no-mesh", action="store_true",
                     help="skip timing the marching cubes of the fused scene")
+    ap.add_argument("--large-stream-radius", type=float, default=0.0,
+                    help="stream the large run's blocks beyond this distance (m) out every 25 frames (0: off)")
     ap.add_argument("--no-stream", action="store_true",
                     help="skip timing a stream-out / stream-in round trip of the fused scene")
     ap.add_argument("--no-depthgen", action="store_true",
@@ -641,7 +660,7 @@ def main() -> int:
         extra["pose_refinement"] = pose_refinement_bench(torch, vsg, synth, sc, frames, wl)
     if args.large_frames > 0:
         log(f"large run: {args.large_frames} frames 1024x768 with tracking")
-        extra["large_run"] = large_run_bench(torch, vsg, synth, args.large_frames)
+        extra["large_run"] = large_run_bench(torch, vsg, synth, args.large_frames, args.large_stream_radius)
     if args.compare_fp16 and args.store == "fp32":
         r16 = measure("fp16")
         extra["fp16_store"] = {"value": round(frames_done / r16["t_total"], 3),
