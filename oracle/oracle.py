@@ -76,6 +76,7 @@ def lib() -> C.CDLL:
         L.vsgo_get_sdf.restype = C.c_double
         L.vsgo_get_sdf.argtypes = [vp, C.c_int64]
         L.vsgo_set_sdf.argtypes = [vp, C.c_int64, C.c_double]
+        L.vsgo_set_voxels.argtypes = [vp, C.c_uint64, vp, vp, vp, vp]
         L.vsgo_stream_out.restype = C.c_uint64
         L.vsgo_stream_out.argtypes = [vp, vp, C.c_double, C.c_uint64, vp, vp, vp, vp, vp, vp, vp]
         L.vsgo_stream_in.argtypes = [vp, C.c_uint64, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -195,6 +196,17 @@ class OracleScene:
 
     def set_sdf(self, idx: int, value: float) -> None:
         lib().vsgo_set_sdf(self._h, int(idx), float(value))
+
+    def set_voxels(self, idx, sdf, weight, rgb=None) -> None:
+        """Overwrites SDF, weight and (rgb [n, 3] given) colour of voxels (global indices
+        ptr*512 + voxel) of allocated blocks."""
+        i = np.ascontiguousarray(idx, np.int64).ravel()
+        s = np.ascontiguousarray(np.broadcast_to(np.asarray(sdf, np.float64).ravel(), i.shape))
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(weight, np.uint8).ravel(), i.shape))
+        c = None if rgb is None else np.ascontiguousarray(np.asarray(rgb, np.uint8).reshape(i.size, 3))
+        if i.size:
+            lib().vsgo_set_voxels(self._h, i.size, i.ctypes.data, s.ctypes.data, w.ctypes.data,
+                                  None if c is None else c.ctypes.data)
 
     def num_allocated(self) -> int:
         return int(lib().vsgo_num_allocated(self._h))

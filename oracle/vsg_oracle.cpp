@@ -328,6 +328,29 @@ void vsgo_set_sdf(void* p, int64_t idx, double value) {
   else h->sf.store_sdf((size_t)idx, (float)value);
 }
 
+// Overwrites SDF, weight and (rgb3 != null) colour of n voxels (global indices
+// ptr*512 + voxel) of allocated blocks: designed fields for the raycaster's
+// refinement and acceptance checks.
+void vsgo_set_voxels(void* p, uint64_t n, const int64_t* idx, const double* sdf, const uint8_t* weight,
+                     const uint8_t* rgb3) {
+  Handle* h = (Handle*)p;
+  for (uint64_t i = 0; i < n; ++i) {
+    const size_t k = (size_t)idx[i];
+    uint8_t* col;
+    if (h->fp64) {
+      h->sd.store_sdf(k, sdf[i]);
+      h->sd.weight[k] = weight[i];
+      col = &h->sd.color[3 * k];
+    } else {
+      h->sf.store_sdf(k, (float)sdf[i]);
+      h->sf.weight[k] = weight[i];
+      col = &h->sf.color[3 * k];
+    }
+    if (rgb3)
+      for (int c = 0; c < 3; ++c) col[c] = rgb3[3 * i + c];
+  }
+}
+
 // Driver's per-frame depth range (VoxelSG/src/voxelSG.cpp:594-608), float arithmetic.
 void vsgo_depth_range(const float* depth, long n, long stride, float* out2) {
   float mn = 1e10f, mx = -1e10f;
