@@ -143,6 +143,37 @@ vsg_status vsg_num_integrated_frames(const vsg_scene* scene, uint32_t* out);
 /* Params the scene was created with (VoxelScene::getParams, VoxelScene.h:133-153). */
 vsg_status vsg_scene_params(const vsg_scene* scene, vsg_params* out);
 
+/* The scene's device arrays (VoxelScene::getHashData, VoxelScene.h:125-136, which
+ * returns the reference's CUDAHashRef of device pointers).  The layout is this
+ * library's, not the reference's HashSlot / VoxelData arrays:
+ *   buckets      num_buckets records of 128 bytes: 10 uint64 keys (the block
+ *                coordinates as 21-bit two's-complement fields, x << 42 | y << 21 | z),
+ *                then 10 int32 block pointers; slots [0, bucket_used[b]) are live;
+ *   bucket_used  num_buckets bytes;
+ *   blocks       int32[4] per block pointer p < num_allocated: (x, y, z, bucket*10 + slot);
+ *   voxels       num_blocks * 512 words of voxel_bytes, voxel order x fastest:
+ *                FP32 store {float sdf, uint16 weight, uint16 0}, FP16 store
+ *                {binary16 sdf | uint16 weight << 16};
+ *   colors       num_blocks * 512 words {r, g, b, flags} (flags bit0 isUpdated, bit1
+ *                isZeroCrossing);
+ *   block_index  index_entries entries {uint32 key lo, key hi, pointer, positive-cell
+ *                mask}, open addressing, all-ones key = empty;
+ *   num_allocated_device  device uint32: live blocks (pointers 0 .. n-1).
+ * The pointers stay valid until vsg_scene_destroy; writing through them bypasses the
+ * raycaster's occupancy and cell summaries (use vsg_import_voxels to overwrite voxels). */
+typedef struct vsg_hash_data {
+  void* buckets;
+  uint8_t* bucket_used;
+  int32_t* blocks;
+  void* voxels;
+  uint32_t* colors;
+  void* block_index;
+  uint32_t* num_allocated_device;
+  uint32_t num_buckets, num_blocks, voxel_bytes, index_entries;
+  int32_t store; /* vsg_store */
+} vsg_hash_data;
+vsg_status vsg_scene_hash_data(const vsg_scene* scene, vsg_hash_data* out);
+
 /* Allocated blocks in pointer order (block k has pointer k), copied to HOST
  * memory: coords[3*k..3*k+2] = block coordinate, slots[k] = hash entry index
  * (bucket*10 + slot).  The reference downloads its compacted list the same way
@@ -197,12 +228,23 @@ vsg_status vsg_render_set_step_budget(vsg_render* render, uint32_t steps);
  * VoxelSG/src/RayCastRender.cu:407-620): one ray per pixel, marched in half-voxel
  * steps over [min_depth, max_depth] (z-depth), zero crossing refined by three
  * regula-falsi steps.  cam_pos and pose are HOST arrays (the reference passes the
- * pose as a device pointer; the C++ shim copies it).  voxel_size <= 0 uses the
+ * pose as a device pointer: vsg_render_frame_device_pose).  voxel_size <= 0 uses the
  * scene's.  Work is queued on the renderer's stream after the scene's pending
  * work.  When the scene holds no block the outputs are left untouched
  * (RayCastRender.cu:553-556).  Asynchronous. */
 vsg_status vsg_render_frame(vsg_render* render, vsg_scene* scene, const float cam_pos[3], const float pose[16],
                             const vsg_intrinsics* intrinsics, float min_depth, float max_depth, float voxel_size);
+
+/* vsg_render_frame with the pose where the reference's render takes it
+ * (RayCastRender.h:66-70, `float* d_cameraTransform`): pose_device is a DEVICE
+ * pointer to 16 floats, row-major camera->world, that the kernels read when they run
+ * (the reference copies nothing either).  cam_pos stays a host array (a float3 by
+ * value in the reference).  The pose values must still be in place when the render
+ * runs on the stream, and when a later vsg_render_adjoint of this render runs (it
+ * reads them again).  No host synchronisation.  Asynchronous. */
+vsg_status vsg_render_frame_device_pose(vsg_render* render, vsg_scene* scene, const float cam_pos[3],
+                                        const float* pose_device, const vsg_intrinsics* intrinsics, float min_depth,
+                                        float max_depth, float voxel_size);
 
 /* Device output buffers (RayCastRender::getOutputDepth/Color/Normal/Positions,
  * RayCastRender.h:110-113): depth is float[W*H] (z-depth, 1e10 = no hit);
@@ -390,6 +432,18 @@ vsg_status vsg_render_set_gn_noise(vsg_render* render, float a, float b, float z
 vsg_status vsg_pose_gn_step(vsg_render* render, vsg_scene* scene, const float* observed_depth,
                             const vsg_intrinsics* intrinsics, float min_depth, float max_depth, float max_residual,
                             double lambda, double pose[16], double stats[8]);
+
+/* `iterations` Gauss-Newton steps as vsg_pose_gn_step, step i with residual window
+ * max_residuals[i] (HOST float[iterations]), with the pose kept on the device between
+ * steps: every step's render reads the previous solve's pose from device memory, so
+ * the steps queue without a host round trip and the pose is read back once at the end
+ * (same results as as many vsg_pose_gn_step calls).  stats as vsg_pose_gn_step's, of
+ * the last step.  The last render's pose is not kept, so vsg_render_adjoint reports
+ * VSG_ERR_NOT_READY after a refinement of more than one step.  Synchronising.  No
+ * counterpart in the reference. */
+vsg_status vsg_pose_gn_refine(vsg_render* render, vsg_scene* scene, const float* observed_depth,
+                              const vsg_intrinsics* intrinsics, float min_depth, float max_depth, uint32_t iterations,
+                              const float* max_residuals, double lambda, double pose[16], double stats[8]);
 
 /* Overwrites the voxels of blocks [first, first+nblocks) from HOST memory (the
  * layout vsg_export_voxels writes; rgbf may be NULL to keep colours and flags).

@@ -11,8 +11,10 @@
 //     camera->world; the cv::Mat overload exists when VSG_HAVE_OPENCV is defined
 //     (OpenCV is used by the reference only as a pose container);
 //   * Params is filled directly instead of from GlobalParamsConfig;
-//   * getHashData() is not provided: the device hash layout is this library's own
-//     (vsg_export_blocks / vsg_export_voxels copy it out in the reference's terms);
+//   * getHashData() returns vsg_hash_data, this library's device arrays (their layout
+//     is documented in vsg.h), in place of the reference's CUDAHashRef;
+//   * RayCastRender::render takes the device pose pointer and reads it in the kernels,
+//     as the reference does; no host copy, no synchronisation;
 //   * errors print and return, as the reference does (VoxelScene.cpp:420-428).
 #pragma once
 
@@ -185,6 +187,14 @@ class VoxelScene {
     return n;
   }
   bool isAllocated() const { return m_isAllocated; }
+
+  // VoxelScene::getHashData (VoxelSG/include/VoxelScene.h:125-136): the scene's device
+  // arrays, in this library's layout (vsg_hash_data in vsg.h).
+  vsg_hash_data getHashData() const {
+    vsg_hash_data h{};
+    vsg_detail::report(vsg_scene_hash_data(m_scene, &h), "VoxelScene::getHashData");
+    return h;
+  }
   Params& getParams() { return m_params; }
   const Params& getParams() const { return m_params; }
 
@@ -460,14 +470,23 @@ class RayCastRender {
 
   // RayCastRender::render (VoxelSG/include/RayCastRender.h:66-70): the pose is a
   // DEVICE float[16] row-major camera->world, as the reference's driver passes it.
+  // The kernels read the 16 floats when they run, so the buffer must hold the pose
+  // until then (the reference's driver keeps it for the whole run).
   void render(const VoxelScene* voxelScene, float3 cameraPos, float* cameraTransform, float fx, float fy, float cx,
               float cy, float minDepth, float maxDepth) {
-    float T[16];
-    if (hipMemcpy(T, cameraTransform, sizeof(T), hipMemcpyDeviceToHost) != hipSuccess) {
-      std::fprintf(stderr, "RayCastRender: cannot read the camera transform\n");
+    if (!m_isAllocated) {
+      std::fprintf(stderr, "RayCastRender: Not initialized!\n");
       return;
     }
-    renderHostPose(voxelScene, cameraPos, T, fx, fy, cx, cy, minDepth, maxDepth);
+    if (!voxelScene || !voxelScene->isAllocated()) {
+      std::fprintf(stderr, "RayCastRender: Invalid voxel scene!\n");
+      return;
+    }
+    const float cam[3] = {cameraPos.x, cameraPos.y, cameraPos.z};
+    vsg_intrinsics k{fx, fy, cx, cy};
+    vsg_detail::report(vsg_render_frame_device_pose(m_render, voxelScene->handle(), cam, cameraTransform, &k, minDepth,
+                                                    maxDepth, voxelScene->getParams().voxelSize),
+                       "RayCastRender::render");
   }
 
   // Same, with the pose in host memory (no copy).

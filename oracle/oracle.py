@@ -70,6 +70,8 @@ def lib() -> C.CDLL:
         L.vsgo_trilinear.restype = C.c_int
         L.vsgo_trilinear.argtypes = [vp, vp, C.c_float, vp, vp]
         L.vsgo_depth_range.argtypes = [vp, C.c_long, C.c_long, vp]
+        L.vsgo_march_brackets.argtypes = [vp, C.c_int, vp, vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_double, C.c_double, vp, C.c_long, vp]
         L.vsgo_render_adjoint.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.c_double, C.c_double, C.c_double,
                                           C.c_double, C.c_double, C.c_double, C.c_double, vp, C.c_long, vp, vp, vp,
                                           vp, vp]
@@ -184,6 +186,20 @@ class OracleScene:
                                   None if gs is None else gs.ctypes.data)
         self.last_depth = depth
         return J, hit.astype(bool), gs
+
+    def march_brackets(self, W: int, cam, pose, intr, minD: float, maxD: float, pixels,
+                       voxel: float | None = None) -> np.ndarray:
+        """[n, 3] float64: the accepted bracket's ray parameters (a, b) and the refined
+        alpha of each listed pixel, NaN where the ray misses."""
+        pix = np.ascontiguousarray(pixels, dtype=np.int32)
+        out = np.zeros((pix.size, 3), np.float64)
+        camd = np.ascontiguousarray(np.asarray(cam, np.float64).reshape(3))
+        fx, fy, cx, cy = intr
+        if pix.size:
+            lib().vsgo_march_brackets(self._h, W, camd.ctypes.data, _pose(pose).ctypes.data, fx, fy, cx, cy, minD,
+                                      maxD, self.voxel if voxel is None else voxel, pix.ctypes.data, pix.size,
+                                      out.ctypes.data)
+        return out
 
     def depth_at(self, W, H, pose, intr, minD, maxD, pixels) -> np.ndarray:
         """Rendered depth of listed pixels in the oracle's arithmetic (float64 output)."""

@@ -6,6 +6,7 @@
 // oracle rounds them to float (exact for values that came from float).
 #include <algorithm>
 #include <cstdint>
+#include <limits>
 #include <utility>
 #include <vector>
 
@@ -281,6 +282,32 @@ int vsgo_trilinear(void* p, const float* p3, float v, float* dist, uint8_t* col3
     return ok;
   }
   return trilinear(h->sf, V3<float>{p3[0], p3[1], p3[2]}, v, dist, col3) ? 1 : 0;
+}
+
+// Accepted crossing of each listed pixel's march: the bracket's ray parameters
+// (a, b) and the refined alpha, or NaN without a hit.  Used to tell pixels whose
+// depth differs between arithmetics because a different sample pair was accepted
+// from pixels that accepted the same pair.
+void vsgo_march_brackets(void* p, int W, const double* cam3, const double* T16, double fx, double fy, double cx,
+                         double cy, double minD, double maxD, double voxel, const int32_t* pix, long n, double* out3) {
+  Handle* h = (Handle*)p;
+  auto run = [&](auto& s, auto tag) {
+    using R = decltype(tag);
+    R M[16];
+    to_pose(T16, M);
+    V3<R> cam{(R)cam3[0], (R)cam3[1], (R)cam3[2]};
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long k = 0; k < n; ++k) {
+      MarchResult<R> m = march_pixel(s, pix[k] % W, pix[k] / W, cam, M, (R)fx, (R)fy, (R)cx, (R)cy, (R)minD, (R)maxD,
+                                     (R)voxel);
+      const double nan = std::numeric_limits<double>::quiet_NaN();
+      out3[3 * k] = m.hit ? (double)m.a : nan;
+      out3[3 * k + 1] = m.hit ? (double)m.b : nan;
+      out3[3 * k + 2] = m.hit ? (double)m.alpha : nan;
+    }
+  };
+  if (h->fp64) run(h->sd, 0.0);
+  else run(h->sf, 0.0f);
 }
 
 // Raycast adjoint for listed pixels: J (6 per pixel, d depth / d xi for T * exp(xi^)),

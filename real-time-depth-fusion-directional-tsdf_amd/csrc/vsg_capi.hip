@@ -145,6 +145,8 @@ struct vsg_render {
   float4* nrm = nullptr;
   uint32_t* rc = nullptr;  // {hits, misses} per workgroup of the last render (march, then tail pass)
   size_t rc_words = 0;
+  size_t rc_march_words = 0;     // the march's slice of rc (the tail pass's follows it)
+  uint32_t last_budget = 0;      // step budget of the last render (0: the tail pass did not run)
   void* tail = nullptr;          // tail queue: TailRay per pixel
   uint32_t* tail_ctl = nullptr;  // {count, next, workgroups done}
   uint32_t budget = 0;           // march steps per lane before the tail pass takes the ray over
@@ -156,6 +158,8 @@ struct vsg_render {
   float* jac = nullptr;       // GN scratch: Jacobian per pixel
   double* partials = nullptr; // GN scratch: per-workgroup sums
   double* gn_out = nullptr;   // GN result (25 doubles)
+  float* gn_posef = nullptr;  // GN pose for the next render: 16 floats + camera position (3)
+  float* cam_buf = nullptr;   // camera position of a render with a device pose (3 floats)
   int32_t gn_jacobian = VSG_GN_SMOOTHED;  // vsg_gn_jacobian of vsg_pose_gn_step
   float gn_noise[3] = {0.0f, 0.0f, 0.0f};  // depth noise model {a, b, z0} of the residual weights (a = 0: none)
   // surface-point extraction: outputs of capacity ext_cap, scratch sized for a scene of
@@ -521,6 +525,23 @@ vsg_status vsg_num_integrated_frames(const vsg_scene* s, uint32_t* out) {
   return VSG_OK;
 }
 
+vsg_status vsg_scene_hash_data(const vsg_scene* s, vsg_hash_data* out) {
+  if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_hash_data: null argument");
+  out->buckets = s->t.rec;
+  out->bucket_used = s->t.cnt;
+  out->blocks = (int32_t*)s->t.blk;
+  out->voxels = s->vox;
+  out->colors = s->col;
+  out->block_index = s->t.idx;
+  out->num_allocated_device = &s->st->n_alloc;
+  out->num_buckets = s->t.num_buckets;
+  out->num_blocks = s->t.num_blocks;
+  out->voxel_bytes = s->word_bytes;
+  out->index_entries = s->t.idx_mask + 1u;
+  out->store = s->p.store;
+  return VSG_OK;
+}
+
 vsg_status vsg_scene_params(const vsg_scene* s, vsg_params* out) {
   if (!s || !out) return fail(VSG_ERR_INVALID_ARG, "vsg_scene_params: null argument");
   *out = s->p;
@@ -645,11 +666,13 @@ vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out) {
   if (st == VSG_OK) st = dmalloc(&r->nrm, n * sizeof(float4), "normal output");
   const size_t nwg = (size_t)((width + 7) / 8) * ((height + 7) / 8);
   r->rc_words = 2 * (nwg + tail_workgroups(width, height));
+  r->rc_march_words = 2 * nwg;
   if (st == VSG_OK) st = dmalloc(&r->rc, r->rc_words * sizeof(uint32_t), "render counters");
   if (st == VSG_OK && hipMemset(r->rc, 0, r->rc_words * sizeof(uint32_t)) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipMemset");
   if (st == VSG_OK) st = dmalloc(&r->tail, n * sizeof(TailRay), "tail queue");
   if (st == VSG_OK) st = dmalloc(&r->tail_ctl, 4 * sizeof(uint32_t), "tail queue controls");
+  if (st == VSG_OK) st = dmalloc(&r->cam_buf, 4 * sizeof(float), "camera position");
   if (st == VSG_OK && hipMemset(r->tail_ctl, 0, 4 * sizeof(uint32_t)) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipMemset");
   r->budget = kDefaultStepBudget;
@@ -676,6 +699,8 @@ vsg_status vsg_render_destroy(vsg_render* r) {
   (void)hipFree(r->jac);
   (void)hipFree(r->partials);
   (void)hipFree(r->gn_out);
+  (void)hipFree(r->gn_posef);
+  (void)hipFree(r->cam_buf);
   free_extract_outputs(r);
   free_extract_scratch(r);
   if (r->ext_total) (void)hipHostFree(r->ext_total);
@@ -707,19 +732,16 @@ vsg_status vsg_render_set_stream(vsg_render* r, void* hip_stream) {
   return VSG_OK;
 }
 
-vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3], const float pose[16],
-                            const vsg_intrinsics* k, float min_depth, float max_depth, float voxel_size) {
-  if (!r || !s || !cam_pos || !pose || !k) return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: null argument");
-  if (!all_finite(cam_pos, 3) || !all_finite(pose, 16))
-    return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: camera position and pose must be finite");
+namespace {
+
+// Queues one render with the pose fields of `ra` already set (by value or device
+// pointers); fills in the rest from the renderer and the scene.
+vsg_status render_launch(vsg_render* r, vsg_scene* s, RenderArgs& ra, const vsg_intrinsics* k, float min_depth,
+                         float max_depth, float voxel_size) {
   const float kk[4] = {k->fx, k->fy, k->cx, k->cy};
   if (!all_finite(kk, 4)) return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: intrinsics must be finite");
   if (std::isnan(min_depth) || std::isnan(max_depth) || std::isinf(min_depth) || std::isinf(max_depth))
     return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: depth range must be finite");
-  RenderArgs ra;
-  std::memset(&ra, 0, sizeof(ra));
-  for (int i = 0; i < 16; ++i) ra.M[i] = pose[i];
-  for (int i = 0; i < 3; ++i) ra.cam[i] = cam_pos[i];
   ra.fx = k->fx;
   ra.fy = k->fy;
   ra.cx = k->cx;
@@ -741,11 +763,20 @@ vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3],
     VSG_HIP(hipEventRecord(r->ev, s->stream));
     VSG_HIP(hipStreamWaitEvent(strm, r->ev, 0));
   }
-  uint32_t* rc = r->rc;
+  if (ra.Mdev && !ra.camdev) {  // device pose, camera position by value: stage it on the stream
+    launch_set_cam(strm, r->cam_buf, ra.cam);
+    ra.camdev = r->cam_buf;
+  }
+  if (ra.Mdev && s->diag) {  // the diagnostics kernels take the pose by value: read it back
+    VSG_HIP(hipStreamSynchronize(strm));
+    VSG_HIP(hipMemcpy(ra.M, ra.Mdev, 16 * sizeof(float), hipMemcpyDeviceToHost));
+    VSG_HIP(hipMemcpy(ra.cam, ra.camdev, 3 * sizeof(float), hipMemcpyDeviceToHost));
+    ra.Mdev = ra.camdev = nullptr;
+  }
   {
     TimedLaunch tl(strm, VSG_K_RAYCAST);
     launch_render(strm, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->depth, r->pos, r->col,
-                  r->nrm, rc);
+                  r->nrm, r->rc);
   }
   vsg_status st = check_launch("render");
   if (st != VSG_OK) return st;
@@ -756,9 +787,37 @@ vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3],
   r->last = strm;
   r->has_last = true;
   r->calls++;
+  r->last_budget = ra.budget;
   r->last_args = ra;
   r->has_args = true;
   return VSG_OK;
+}
+
+}  // namespace
+
+vsg_status vsg_render_frame(vsg_render* r, vsg_scene* s, const float cam_pos[3], const float pose[16],
+                            const vsg_intrinsics* k, float min_depth, float max_depth, float voxel_size) {
+  if (!r || !s || !cam_pos || !pose || !k) return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: null argument");
+  if (!all_finite(cam_pos, 3) || !all_finite(pose, 16))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame: camera position and pose must be finite");
+  RenderArgs ra;
+  std::memset(&ra, 0, sizeof(ra));
+  for (int i = 0; i < 16; ++i) ra.M[i] = pose[i];
+  for (int i = 0; i < 3; ++i) ra.cam[i] = cam_pos[i];
+  return render_launch(r, s, ra, k, min_depth, max_depth, voxel_size);
+}
+
+vsg_status vsg_render_frame_device_pose(vsg_render* r, vsg_scene* s, const float cam_pos[3], const float* pose_device,
+                                        const vsg_intrinsics* k, float min_depth, float max_depth, float voxel_size) {
+  if (!r || !s || !cam_pos || !pose_device || !k)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame_device_pose: null argument");
+  if (!all_finite(cam_pos, 3))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_render_frame_device_pose: camera position must be finite");
+  RenderArgs ra;
+  std::memset(&ra, 0, sizeof(ra));
+  ra.Mdev = pose_device;
+  for (int i = 0; i < 3; ++i) ra.cam[i] = cam_pos[i];
+  return render_launch(r, s, ra, k, min_depth, max_depth, voxel_size);
 }
 
 vsg_status vsg_render_buffers(const vsg_render* r, float** depth, float** positions, float** colors, float** normals) {
@@ -786,9 +845,11 @@ vsg_status vsg_render_counters(vsg_render* r, uint32_t* hits, uint32_t* misses) 
   *hits = *misses = 0;
   if (r->calls == 0) return VSG_OK;
   if (r->has_last) VSG_HIP(hipStreamSynchronize(r->last));
-  std::vector<uint32_t> h(r->rc_words);
+  // without the tail pass its slice still holds an earlier render's counts
+  const size_t words = r->last_budget == 0 ? r->rc_march_words : r->rc_words;
+  std::vector<uint32_t> h(words);
   VSG_HIP(hipMemcpy(h.data(), r->rc, h.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
-  for (size_t i = 0; i < r->rc_words / 2; ++i) {
+  for (size_t i = 0; i < words / 2; ++i) {
     *hits += h[2 * i];
     *misses += h[2 * i + 1];
   }
@@ -928,29 +989,23 @@ vsg_status vsg_render_set_gn_jacobian(vsg_render* r, int32_t mode) {
   return VSG_OK;
 }
 
-vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_depth, const vsg_intrinsics* k,
-                            float min_depth, float max_depth, float max_residual, double lambda, double pose[16],
-                            double stats[8]) {
-  if (!r || !s || !observed_depth || !k || !pose) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_step: null argument");
-  for (int i = 0; i < 16; ++i)
-    if (!std::isfinite(pose[i])) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_step: pose must be finite");
+namespace {
+
+vsg_status gn_scratch(vsg_render* r) {
+  if (r->jac) return VSG_OK;
   const int npix = r->W * r->H;
-  if (!r->jac) {
-    vsg_status st = dmalloc(&r->jac, (size_t)npix * 6 * sizeof(float), "Jacobian scratch");
-    if (st == VSG_OK) st = dmalloc(&r->partials, (size_t)gn_partial_count(npix) * 29 * sizeof(double), "GN scratch");
-    if (st == VSG_OK) st = dmalloc(&r->gn_out, 32 * sizeof(double), "GN result");
-    if (st != VSG_OK) return st;
-  }
-  float pf[16], cam[3];
-  for (int i = 0; i < 16; ++i) pf[i] = (float)pose[i];
-  cam[0] = pf[3];
-  cam[1] = pf[7];
-  cam[2] = pf[11];
-  vsg_status st = vsg_render_frame(r, s, cam, pf, k, min_depth, max_depth, 0.0f);
-  if (st != VSG_OK) return st;
+  vsg_status st = dmalloc(&r->jac, (size_t)npix * 6 * sizeof(float), "Jacobian scratch");
+  if (st == VSG_OK) st = dmalloc(&r->partials, (size_t)gn_partial_count(npix) * 29 * sizeof(double), "GN scratch");
+  if (st == VSG_OK) st = dmalloc(&r->gn_out, 32 * sizeof(double), "GN result");
+  if (st == VSG_OK) st = dmalloc(&r->gn_posef, 20 * sizeof(float), "GN pose");
+  return st;
+}
+
+// The Jacobian of the last render (the mode vsg_render_set_gn_jacobian chose).
+vsg_status gn_jacobian(vsg_render* r, vsg_scene* s) {
+  RenderArgs ra = r->last_args;
+  ra.mark_bkt = ra.mark_blk = nullptr;
   {
-    RenderArgs ra = r->last_args;
-    ra.mark_bkt = ra.mark_blk = nullptr;
     TimedLaunch tl(r->last, VSG_K_ADJOINT);
     if (r->gn_jacobian == VSG_GN_SMOOTHED)
       launch_gn_jacobian(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->depth, r->jac);
@@ -958,12 +1013,40 @@ vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_d
       launch_render_adjoint(r->last, s->p.store == VSG_STORE_FP16, s->t, s->st, ra, s->vox, s->col, r->jac, nullptr,
                             nullptr);
   }
-  st = check_launch("render adjoint");
+  return check_launch("render adjoint");
+}
+
+void gn_stats(const double out[25], double stats[8]) {
+  stats[0] = out[22];
+  stats[1] = out[23];
+  stats[2] = out[24];
+  stats[3] = std::sqrt(out[16] * out[16] + out[17] * out[17] + out[18] * out[18]);
+  stats[4] = std::sqrt(out[19] * out[19] + out[20] * out[20] + out[21] * out[21]);
+  stats[5] = stats[6] = stats[7] = 0.0;
+}
+
+}  // namespace
+
+vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_depth, const vsg_intrinsics* k,
+                            float min_depth, float max_depth, float max_residual, double lambda, double pose[16],
+                            double stats[8]) {
+  if (!r || !s || !observed_depth || !k || !pose) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_step: null argument");
+  for (int i = 0; i < 16; ++i)
+    if (!std::isfinite(pose[i])) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_step: pose must be finite");
+  vsg_status st = gn_scratch(r);
+  if (st != VSG_OK) return st;
+  float pf[16], cam[3];
+  for (int i = 0; i < 16; ++i) pf[i] = (float)pose[i];
+  cam[0] = pf[3];
+  cam[1] = pf[7];
+  cam[2] = pf[11];
+  st = vsg_render_frame(r, s, cam, pf, k, min_depth, max_depth, 0.0f);
+  if (st == VSG_OK) st = gn_jacobian(r, s);
   if (st != VSG_OK) return st;
   {
     TimedLaunch tl(r->last, VSG_K_GN);
     launch_gn(r->last, r->depth, observed_depth, r->jac, r->W, r->H, max_residual, r->gn_noise, lambda, pose,
-              r->partials, r->gn_out);
+              nullptr, r->partials, r->gn_out, nullptr);
   }
   st = check_launch("gauss-newton");
   if (st != VSG_OK) return st;
@@ -971,14 +1054,58 @@ vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_d
   VSG_HIP(hipStreamSynchronize(r->last));
   VSG_HIP(hipMemcpy(out, r->gn_out, sizeof(out), hipMemcpyDeviceToHost));
   for (int i = 0; i < 16; ++i) pose[i] = out[i];
-  if (stats) {
-    stats[0] = out[22];
-    stats[1] = out[23];
-    stats[2] = out[24];
-    stats[3] = std::sqrt(out[16] * out[16] + out[17] * out[17] + out[18] * out[18]);
-    stats[4] = std::sqrt(out[19] * out[19] + out[20] * out[20] + out[21] * out[21]);
-    stats[5] = stats[6] = stats[7] = 0.0;
+  if (stats) gn_stats(out, stats);
+  return VSG_OK;
+}
+
+vsg_status vsg_pose_gn_refine(vsg_render* r, vsg_scene* s, const float* observed_depth, const vsg_intrinsics* k,
+                              float min_depth, float max_depth, uint32_t iterations, const float* max_residuals,
+                              double lambda, double pose[16], double stats[8]) {
+  if (!r || !s || !observed_depth || !k || !pose || (iterations && !max_residuals))
+    return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_refine: null argument");
+  for (int i = 0; i < 16; ++i)
+    if (!std::isfinite(pose[i])) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_refine: pose must be finite");
+  for (uint32_t i = 0; i < iterations; ++i)
+    if (!std::isfinite(max_residuals[i]))
+      return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_refine: residual windows must be finite");
+  if (iterations == 0) {
+    if (stats)
+      for (int i = 0; i < 8; ++i) stats[i] = 0.0;
+    return VSG_OK;
   }
+  vsg_status st = gn_scratch(r);
+  if (st != VSG_OK) return st;
+  for (uint32_t it = 0; it < iterations; ++it) {
+    RenderArgs ra;
+    std::memset(&ra, 0, sizeof(ra));
+    if (it == 0) {  // the host pose, rounded to float as vsg_pose_gn_step does
+      for (int i = 0; i < 16; ++i) ra.M[i] = (float)pose[i];
+      ra.cam[0] = ra.M[3];
+      ra.cam[1] = ra.M[7];
+      ra.cam[2] = ra.M[11];
+    } else {  // the previous solve's pose, left on the device
+      ra.Mdev = r->gn_posef;
+      ra.camdev = r->gn_posef + 16;
+    }
+    st = render_launch(r, s, ra, k, min_depth, max_depth, 0.0f);
+    if (st == VSG_OK) st = gn_jacobian(r, s);
+    if (st != VSG_OK) return st;
+    {
+      TimedLaunch tl(r->last, VSG_K_GN);
+      launch_gn(r->last, r->depth, observed_depth, r->jac, r->W, r->H, max_residuals[it], r->gn_noise, lambda, pose,
+                it == 0 ? nullptr : r->gn_out, r->partials, r->gn_out, r->gn_posef);
+    }
+    st = check_launch("gauss-newton");
+    if (st != VSG_OK) return st;
+  }
+  double out[25];
+  VSG_HIP(hipStreamSynchronize(r->last));
+  VSG_HIP(hipMemcpy(out, r->gn_out, sizeof(out), hipMemcpyDeviceToHost));
+  for (int i = 0; i < 16; ++i) pose[i] = out[i];
+  if (stats) gn_stats(out, stats);
+  // the last render's pose lives in gn_posef, which the next iteration would overwrite:
+  // an adjoint of this render must not read it through a device pointer
+  if (iterations > 1) r->has_args = false;
   return VSG_OK;
 }
 

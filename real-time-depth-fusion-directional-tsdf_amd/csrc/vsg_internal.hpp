@@ -11,6 +11,8 @@ namespace vsg {
 struct RenderArgs {
   float M[16];
   float cam[3];
+  const float* Mdev;    // when non-null: the pose is the 16 floats here (device memory), not M
+  const float* camdev;  // when non-null: the camera position is the 3 floats here, not cam
   float fx, fy, cx, cy;
   float minD, maxD;
   float voxel;
@@ -170,8 +172,14 @@ void launch_gn_jacobian(hipStream_t s, bool fp16, const TableView& t, const DevS
 uint32_t gn_partial_count(int npix);
 // out: new pose (16 doubles), delta (6), cost, count, solved flag (25 doubles)
 // noise = {a, b, z0}: residual weights (a / (a + b (obs - z0)^2))^2 when a > 0.
+// One Gauss-Newton accumulation and solve.  The pose comes from pose_in (device, may be
+// the previous result `out`) or, when pose_in is null, from pose (host, by value); posef
+// (may be null) receives the new pose in the form a render reads (16 + 3 floats).
 void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int W, int H, float max_res,
-               const float noise[3], double lambda, const double pose[16], double* partials, double* out);
+               const float noise[3], double lambda, const double pose[16], const double* pose_in, double* partials,
+               double* out, float* posef);
+// Writes the 3 floats of c to cam (device) on stream s.
+void launch_set_cam(hipStream_t s, float* cam, const float c[3]);
 void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState* st, const RenderArgs& ra,
                    const void* vox, const uint32_t* col, float* depth, float4* pos, float4* colr, float4* nrm,
                    uint32_t* rc);

@@ -982,8 +982,43 @@ struct RaySetup {
   RaySkip sk;
 };
 
+// Rows 0-2 of the render's camera->world pose: the caller's device copy when the
+// render was given one (RayCastRender.h:66-70 passes a device pointer), else the copy
+// passed by value.  All lanes read the same words.
+__device__ __forceinline__ float uniform_load(const float* p) {
+  // the address is the same in every lane: keep the value in a scalar register
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(*p)));
+}
+
+// DEV: the render reads its pose from ra.Mdev and its camera position from
+// ra.camdev (both device memory); otherwise from the by-value copies.  A compile-time
+// choice, so that the by-value kernels carry no code or registers for the other.
+template <bool DEV> __device__ __forceinline__ void render_pose(const RenderArgs& ra, float M[12]) {
+  if (DEV) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) M[i] = uniform_load(ra.Mdev + i);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) M[i] = ra.M[i];
+  }
+}
+
+// The render's camera position (device copy or by value, as render_pose).
+template <bool DEV> __device__ __forceinline__ void render_cam(const RenderArgs& ra, float c[3]) {
+  if (DEV) {
+    c[0] = uniform_load(ra.camdev);
+    c[1] = uniform_load(ra.camdev + 1);
+    c[2] = uniform_load(ra.camdev + 2);
+  } else {
+    c[0] = ra.cam[0];
+    c[1] = ra.cam[1];
+    c[2] = ra.cam[2];
+  }
+}
+
 // Ray of pixel (x, y) (RayCastRender.cu:407-470).  Returns false when the depth
 // range is invalid (no march; the pixel keeps the miss values).
+template <bool DEV>
 __device__ __forceinline__ bool ray_setup(const RenderArgs& ra, uint32_t x, uint32_t y, RayHit& h, RaySetup& rs) {
   const float MINF = 1e10f;
   h.marched = false;
@@ -1001,7 +1036,8 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs& ra, uint32_t x, uint
     cdy /= cl;
     cdz /= cl;
   }
-  const float* M = ra.M;
+  float M[12];
+  render_pose<DEV>(ra, M);
   float wdx = M[0] * cdx + M[1] * cdy + M[2] * cdz;
   float wdy = M[4] * cdx + M[5] * cdy + M[6] * cdz;
   float wdz = M[8] * cdx + M[9] * cdy + M[10] * cdz;
@@ -1022,9 +1058,11 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs& ra, uint32_t x, uint
   if (minD == 0.0f || minD == MINF || maxD == 0.0f || maxD == MINF) return false;
   h.marched = true;
   rs.v = ra.voxel;
-  rs.camx = ra.cam[0];
-  rs.camy = ra.cam[1];
-  rs.camz = ra.cam[2];
+  float cam[3];
+  render_cam<DEV>(ra, cam);
+  rs.camx = cam[0];
+  rs.camy = cam[1];
+  rs.camz = cam[2];
   rs.ray0 = h.d2r * fmaxf(minD, minD);
   rs.ray_end = h.d2r * fminf(maxD, maxD);
   rs.th = rs.v * 3.0f;
@@ -1133,13 +1171,13 @@ __device__ __forceinline__ bool cell_skip(const TableView& t, Cache& bc, const R
 // The march of one ray per lane.  budget > 0: a lane that has taken that many steps
 // (skips or evaluations) stops with h.deferred and its state, to be continued by
 // the cooperative tail pass.
-template <bool F16, bool DIAG, bool REF, class Cache>
+template <bool F16, bool DIAG, bool REF, bool DEV, class Cache>
 __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
                                             const uint32_t* __restrict__ col, uint32_t x, uint32_t y, Cache& bc,
                                             uint32_t budget) {
   RayHit h;
   RaySetup rs;
-  if (!ray_setup(ra, x, y, h, rs)) return h;
+  if (!ray_setup<DEV>(ra, x, y, h, rs)) return h;
   const float v = rs.v;
   const float camx = rs.camx, camy = rs.camy, camz = rs.camz;
   const float wdx = h.wdx, wdy = h.wdy, wdz = h.wdz;
@@ -1294,7 +1332,7 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
   return h;
 }
 
-template <bool F16, bool DIAG, bool REF>
+template <bool F16, bool DIAG, bool REF, bool DEV>
 __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                float* __restrict__ out_depth, float4* __restrict__ out_pos,
@@ -1331,7 +1369,7 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
     __shared__ int32_t slot_lds[8 * 64];
     BlockCache<DIAG> bc;
     bc.init(slot_lds + threadIdx.x, ra.mark_bkt, ra.mark_blk, ra.march_stats);
-    const RayHit h = march_ray<F16, DIAG, REF>(t, ra, vox, col, x, y, bc, ra.budget);
+    const RayHit h = march_ray<F16, DIAG, REF, DEV>(t, ra, vox, col, x, y, bc, ra.budget);
     deferred = h.deferred;
     tr.pix = (uint32_t)pix;
     tr.ray = h.ray;
@@ -1345,7 +1383,9 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
     if (h.found) {
       const float alpha = h.alpha;
       o_depth = alpha / h.d2r;
-      o_pos = make_float4(ra.cam[0] + alpha * h.wdx, ra.cam[1] + alpha * h.wdy, ra.cam[2] + alpha * h.wdz, 1.0f);
+      float cam[3];
+      render_cam<DEV>(ra, cam);
+      o_pos = make_float4(cam[0] + alpha * h.wdx, cam[1] + alpha * h.wdy, cam[2] + alpha * h.wdz, 1.0f);
       o_col = make_float4((float)(h.rgb & 0xFFu) / 255.0f, (float)((h.rgb >> 8) & 0xFFu) / 255.0f,
                           (float)((h.rgb >> 16) & 0xFFu) / 255.0f, 1.0f);
     }
@@ -1390,7 +1430,7 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // for the next render.
 constexpr int kTailGroup = 16;
 
-template <bool F16, bool DIAG, bool REF>
+template <bool F16, bool DIAG, bool REF, bool DEV>
 __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, const void* __restrict__ vox,
                                                     const uint32_t* __restrict__ col, float* __restrict__ out_depth,
                                                     float4* __restrict__ out_pos, float4* __restrict__ out_col,
@@ -1441,7 +1481,7 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
       if (e < n) {
         const TailRay q = queue[e];
         pix = q.pix;
-        (void)ray_setup(ra, pix % (uint32_t)ra.W, pix / (uint32_t)ra.W, h, rs);
+        (void)ray_setup<DEV>(ra, pix % (uint32_t)ra.W, pix / (uint32_t)ra.W, h, rs);
         ray = q.ray;
         last_sdf = q.last_sdf;
         last_alpha = q.last_alpha;
@@ -1574,7 +1614,9 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
         if (h.found) {
           const float alpha = h.alpha;
           o_depth = alpha / h.d2r;
-          o_pos = make_float4(ra.cam[0] + alpha * h.wdx, ra.cam[1] + alpha * h.wdy, ra.cam[2] + alpha * h.wdz, 1.0f);
+          float cam[3];
+          render_cam<DEV>(ra, cam);
+          o_pos = make_float4(cam[0] + alpha * h.wdx, cam[1] + alpha * h.wdy, cam[2] + alpha * h.wdz, 1.0f);
           o_col = make_float4((float)(h.rgb & 0xFFu) / 255.0f, (float)((h.rgb >> 8) & 0xFFu) / 255.0f,
                               (float)((h.rgb >> 16) & 0xFFu) / 255.0f, 1.0f);
           hit++;
@@ -1719,12 +1761,14 @@ __device__ __forceinline__ Dual10 eval_dual(const TriGradF& tg, float fval, cons
 // the camera frame (q = t cd), so dt/domega = t (g_c x cd) / (g_c . cd) and dt/dupsilon =
 // -g_c / (g_c . cd), g_c the gradient in the camera frame; depth = t cd_z.  0 where a
 // sample is unweighted or the ray meets the surface at |cos| < 0.2.
-template <bool F16, bool REF, class Cache>
+template <bool F16, bool REF, bool DEV, class Cache>
 __device__ __forceinline__ void smoothed_pose_jacobian(const TableView& t, Cache& bc, const void* __restrict__ vox,
                                                        const uint32_t* __restrict__ col, const RenderArgs& ra,
                                                        const RayHit& h, float tc, float J[6]) {
   const float v = ra.voxel;
-  const float qx = ra.cam[0] + tc * h.wdx, qy = ra.cam[1] + tc * h.wdy, qz = ra.cam[2] + tc * h.wdz;
+  float cam[3];
+  render_cam<DEV>(ra, cam);
+  const float qx = cam[0] + tc * h.wdx, qy = cam[1] + tc * h.wdy, qz = cam[2] + tc * h.wdz;
   float gw[3];
   bool ok = true;
 #pragma unroll
@@ -1736,7 +1780,8 @@ __device__ __forceinline__ void smoothed_pose_jacobian(const TableView& t, Cache
     ok = ok && trilinear<F16, false, REF>(t, bc, vox, col, qx - ex, qy - ey, qz - ez, v, &fm_, &rgb) == kSampleValid;
     gw[a] = ok ? (fp_ - fm_) : 0.0f;
   }
-  const float* M = ra.M;
+  float M[12];
+  render_pose<DEV>(ra, M);
   const float gcx = M[0] * gw[0] + M[4] * gw[1] + M[8] * gw[2];
   const float gcy = M[1] * gw[0] + M[5] * gw[1] + M[9] * gw[2];
   const float gcz = M[2] * gw[0] + M[6] * gw[1] + M[10] * gw[2];
@@ -1758,7 +1803,7 @@ __device__ __forceinline__ void smoothed_pose_jacobian(const TableView& t, Cache
 
 // The smoothed pose Jacobian of every pixel of the last render, from its rendered depth
 // (no second march): t = depth * d2r.
-template <bool F16, bool REF>
+template <bool F16, bool REF, bool DEV>
 __global__ __launch_bounds__(64) void k_gn_jacobian(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                     const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                     const float* __restrict__ depth, float* __restrict__ jac) {
@@ -1774,14 +1819,14 @@ __global__ __launch_bounds__(64) void k_gn_jacobian(TableView t, const DevState*
     bc.init(slot_lds + threadIdx.x, nullptr, nullptr);
     RayHit h;
     RaySetup rs;
-    (void)ray_setup(ra, x, y, h, rs);
-    smoothed_pose_jacobian<F16, REF>(t, bc, vox, col, ra, h, d * h.d2r, J);
+    (void)ray_setup<DEV>(ra, x, y, h, rs);
+    smoothed_pose_jacobian<F16, REF, DEV>(t, bc, vox, col, ra, h, d * h.d2r, J);
   }
 #pragma unroll
   for (int k = 0; k < 6; ++k) jac[6 * (size_t)pix + k] = J[k];
 }
 
-template <bool F16, bool REF>
+template <bool F16, bool REF, bool DEV>
 __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                        const void* __restrict__ vox, const uint32_t* __restrict__ col,
                                                        float* __restrict__ jac, const float* __restrict__ grad_depth,
@@ -1795,10 +1840,13 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
     __shared__ int32_t slot_lds[8 * 64];
     BlockCache<false> bc;
     bc.init(slot_lds + threadIdx.x, nullptr, nullptr);
-    const RayHit h = march_ray<F16, false, REF>(t, ra, vox, col, x, y, bc, 0u);
+    const RayHit h = march_ray<F16, false, REF, DEV>(t, ra, vox, col, x, y, bc, 0u);
     if (h.found) {
       const float v = ra.voxel;
-      const float camx = ra.cam[0], camy = ra.cam[1], camz = ra.cam[2];
+      float cam[3], M[12];
+      render_cam<DEV>(ra, cam);
+      render_pose<DEV>(ra, M);
+      const float camx = cam[0], camy = cam[1], camz = cam[2];
       auto px = [&](float r) { return camx + r * h.wdx; };
       auto py = [&](float r) { return camy + r * h.wdy; };
       auto pz = [&](float r) { return camz + r * h.wdz; };
@@ -1810,8 +1858,8 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
       b.v = h.b;
       trilinear_grad<F16, REF>(t, bc, vox, px(h.a), py(h.a), pz(h.a), v, h.ad, &tg[0]);
       trilinear_grad<F16, REF>(t, bc, vox, px(h.b), py(h.b), pz(h.b), v, h.bd, &tg[1]);
-      ad = eval_dual(tg[0], h.ad, a, h, ra.M, 0);
-      bd = eval_dual(tg[1], h.bd, b, h, ra.M, 1);
+      ad = eval_dual(tg[0], h.ad, a, h, M, 0);
+      bd = eval_dual(tg[1], h.bd, b, h, M, 1);
       float fa = h.a, fad = h.ad, fb = h.b, fbd = h.bd;
       for (int it = 0; it < 3; ++it) {
         const float fc = fa + (fad / (fad - fbd)) * (fb - fa);  // the forward's point
@@ -1827,7 +1875,7 @@ __global__ __launch_bounds__(64) void k_render_adjoint(TableView t, const DevSta
         uint32_t rgb;
         trilinear<F16, false, REF>(t, bc, vox, col, px(fc), py(fc), pz(fc), v, &fcd, &rgb);
         trilinear_grad<F16, REF>(t, bc, vox, px(fc), py(fc), pz(fc), v, fcd, &tg[2 + it]);
-        const Dual10 cd = eval_dual(tg[2 + it], fcd, c, h, ra.M, 2 + it);
+        const Dual10 cd = eval_dual(tg[2 + it], fcd, c, h, M, 2 + it);
         if (fad * fcd > 0.0f) {
           fa = fc;
           fad = fcd;
@@ -1942,8 +1990,12 @@ struct Pose64 {
 // Sums the partials in a fixed order, solves (H + lambda diag(H)) delta = -g by
 // Cholesky and composes pose * exp(delta^), all in FP64.  out: new pose (16),
 // delta (6), cost, count, solved flag.
+// pose_in (may be null: the pose passed by value) may be `out` itself (the previous
+// iteration's result): lane 0 reads it before writing.  posef (may be null) receives the
+// new pose as 16 floats and the camera position as 3 more, the form a render reads.
 __global__ __launch_bounds__(64) void k_gn_solve(const double* __restrict__ partials, int nwg, double lambda,
-                                                 Pose64 pose, double* __restrict__ out) {
+                                                 Pose64 pose, const double* pose_in, double* out,
+                                                 float* __restrict__ posef) {
   __shared__ double sum[kGnTerms];
   if (threadIdx.x < kGnTerms) {
     double s = 0.0;
@@ -1952,6 +2004,8 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* __restrict__ part
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
+  double P[16];
+  for (int i = 0; i < 16; ++i) P[i] = pose_in ? pose_in[i] : pose.T[i];
   double H[36], g[6];
   int n = 0;
   for (int i = 0; i < 6; ++i)
@@ -2025,8 +2079,8 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* __restrict__ part
   double Tn[16];
   for (int i = 0; i < 3; ++i) {
     for (int j = 0; j < 3; ++j)
-      Tn[4 * i + j] = pose.T[4 * i] * Rd[j] + pose.T[4 * i + 1] * Rd[3 + j] + pose.T[4 * i + 2] * Rd[6 + j];
-    Tn[4 * i + 3] = pose.T[4 * i] * td[0] + pose.T[4 * i + 1] * td[1] + pose.T[4 * i + 2] * td[2] + pose.T[4 * i + 3];
+      Tn[4 * i + j] = P[4 * i] * Rd[j] + P[4 * i + 1] * Rd[3 + j] + P[4 * i + 2] * Rd[6 + j];
+    Tn[4 * i + 3] = P[4 * i] * td[0] + P[4 * i + 1] * td[1] + P[4 * i + 2] * td[2] + P[4 * i + 3];
   }
   Tn[12] = Tn[13] = Tn[14] = 0.0;
   Tn[15] = 1.0;
@@ -2035,6 +2089,12 @@ __global__ __launch_bounds__(64) void k_gn_solve(const double* __restrict__ part
   out[22] = sum[27];
   out[23] = sum[28];
   out[24] = ok ? 1.0 : 0.0;
+  if (posef) {
+    for (int i = 0; i < 16; ++i) posef[i] = (float)Tn[i];
+    posef[16] = (float)Tn[3];
+    posef[17] = (float)Tn[7];
+    posef[18] = (float)Tn[11];
+  }
 }
 
 // ---------------------------------------------------------------------------------
@@ -3002,7 +3062,14 @@ void launch_gn_jacobian(hipStream_t s, bool fp16, const TableView& t, const DevS
                         const void* vox, const uint32_t* col, const float* depth, float* jac) {
   const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
   const bool ref = ra.mode == 0;
-#define VSG_GNJ(F, R) hipLaunchKernelGGL((k_gn_jacobian<F, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, jac)
+  const bool dev = ra.Mdev != nullptr;
+#define VSG_GNJ(F, R)                                                                                      \
+  do {                                                                                                     \
+    if (dev)                                                                                               \
+      hipLaunchKernelGGL((k_gn_jacobian<F, R, true>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, jac); \
+    else                                                                                                   \
+      hipLaunchKernelGGL((k_gn_jacobian<F, R, false>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, jac); \
+  } while (0)
   if (fp16) {
     if (ref) VSG_GNJ(true, true); else VSG_GNJ(true, false);
   } else {
@@ -3016,8 +3083,16 @@ void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const D
                            float* grad_sdf) {
   const dim3 grid((ra.W + 7) / 8, (ra.H + 7) / 8);
   const bool ref = ra.mode == 0;
-#define VSG_ADJ(F, R) \
-  hipLaunchKernelGGL((k_render_adjoint<F, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, grad_sdf)
+  const bool dev = ra.Mdev != nullptr;
+#define VSG_ADJ(F, R)                                                                                          \
+  do {                                                                                                         \
+    if (dev)                                                                                                   \
+      hipLaunchKernelGGL((k_render_adjoint<F, R, true>), grid, dim3(64), 0, s, t, st, ra, vox, col, jac, grad_depth, \
+                         grad_sdf);                                                                            \
+    else                                                                                                       \
+      hipLaunchKernelGGL((k_render_adjoint<F, R, false>), grid, dim3(64), 0, s, t, st, ra, vox, col, jac,        \
+                         grad_depth, grad_sdf);                                                                \
+  } while (0)
   if (fp16) {
     if (ref) VSG_ADJ(true, true); else VSG_ADJ(true, false);
   } else {
@@ -3029,13 +3104,14 @@ void launch_render_adjoint(hipStream_t s, bool fp16, const TableView& t, const D
 uint32_t gn_partial_count(int npix) { return (uint32_t)((npix + 255) / 256); }
 
 void launch_gn(hipStream_t s, const float* depth, const float* obs, const float* jac, int W, int H, float max_res,
-               const float noise[3], double lambda, const double pose[16], double* partials, double* out) {
+               const float noise[3], double lambda, const double pose[16], const double* pose_in, double* partials,
+               double* out, float* posef) {
   const uint32_t nwg = gn_partial_count(W * H);
   const float3 nz = make_float3(noise[0], noise[1], noise[2]);
   hipLaunchKernelGGL(k_gn_accum, dim3(nwg), dim3(256), 0, s, depth, obs, jac, W, H, max_res, nz, partials);
   Pose64 p;
-  for (int i = 0; i < 16; ++i) p.T[i] = pose[i];
-  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, partials, (int)nwg, lambda, p, out);
+  for (int i = 0; i < 16; ++i) p.T[i] = pose ? pose[i] : 0.0;
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, partials, (int)nwg, lambda, p, pose_in, out, posef);
 }
 
 uint32_t tail_workgroups(int32_t W, int32_t H) {
@@ -3159,12 +3235,22 @@ void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState*
   const bool ref = ra.mode == 0;
   uint32_t* rc_tail = rc + 2 * (size_t)ntiles;
   const dim3 tgrid(tail_workgroups(ra.W, ra.H));
-#define VSG_RENDER(F, D, R)                                                                                    \
-  do {                                                                                                         \
-    hipLaunchKernelGGL((k_render<F, D, R>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, rc); \
-    if (ra.budget != 0u)                                                                                       \
-      hipLaunchKernelGGL((k_render_tail<F, D, R>), tgrid, dim3(64), 0, s, t, ra, vox, col, depth, pos, colr, nrm, \
-                         rc_tail);                                                                             \
+  // device poses (ra.Mdev) are instantiated without diagnostics only: the caller reads a
+  // device pose back into the by-value copy when diagnostics are on
+  const bool dev = ra.Mdev != nullptr && !diag;
+#define VSG_RENDER_DEV(F, D, R, DV)                                                                             \
+  do {                                                                                                          \
+    hipLaunchKernelGGL((k_render<F, D, R, DV>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, \
+                       rc);                                                                                     \
+    if (ra.budget != 0u)                                                                                        \
+      hipLaunchKernelGGL((k_render_tail<F, D, R, DV>), tgrid, dim3(64), 0, s, t, ra, vox, col, depth, pos, colr,  \
+                         nrm, rc_tail);                                                                         \
+  } while (0)
+#define VSG_RENDER(F, D, R)                       \
+  do {                                            \
+    if (D) VSG_RENDER_DEV(F, true, R, false);     \
+    else if (dev) VSG_RENDER_DEV(F, false, R, true); \
+    else VSG_RENDER_DEV(F, false, R, false);      \
   } while (0)
   if (fp16) {
     if (diag) {
@@ -3180,6 +3266,17 @@ void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState*
     }
   }
 #undef VSG_RENDER
+#undef VSG_RENDER_DEV
+}
+
+__global__ void k_set_cam(float* __restrict__ cam, float x, float y, float z) {
+  cam[0] = x;
+  cam[1] = y;
+  cam[2] = z;
+}
+
+void launch_set_cam(hipStream_t s, float* cam, const float c[3]) {
+  hipLaunchKernelGGL(k_set_cam, dim3(1), dim3(1), 0, s, cam, c[0], c[1], c[2]);
 }
 
 // ---------------------------------------------------------------------------------

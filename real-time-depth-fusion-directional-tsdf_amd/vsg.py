@@ -64,6 +64,13 @@ class vsg_counters(C.Structure):
                 ("reserved", C.c_uint32 * 7)]
 
 
+class vsg_hash_data(C.Structure):
+    _fields_ = [("buckets", C.c_void_p), ("bucket_used", C.c_void_p), ("blocks", C.c_void_p), ("voxels", C.c_void_p),
+                ("colors", C.c_void_p), ("block_index", C.c_void_p), ("num_allocated_device", C.c_void_p),
+                ("num_buckets", C.c_uint32), ("num_blocks", C.c_uint32), ("voxel_bytes", C.c_uint32),
+                ("index_entries", C.c_uint32), ("store", C.c_int32)]
+
+
 class vsg_diag(C.Structure):
     _fields_ = [("alloc_buckets", C.c_uint32), ("ray_buckets", C.c_uint32), ("any_buckets", C.c_uint32),
                 ("ray_blocks", C.c_uint32), ("march", C.c_uint32 * 8)]
@@ -84,6 +91,7 @@ _SIGNATURES = {
     "vsg_heap_free_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "vsg_num_integrated_frames": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "vsg_scene_params": (C.c_int, [C.c_void_p, C.POINTER(vsg_params)]),
+    "vsg_scene_hash_data": (C.c_int, [C.c_void_p, C.POINTER(vsg_hash_data)]),
     "vsg_export_blocks": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(C.c_uint32)]),
     "vsg_export_directional": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_export_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -94,6 +102,8 @@ _SIGNATURES = {
     "vsg_render_set_step_budget": (C.c_int, [C.c_void_p, C.c_uint32]),
     "vsg_render_frame": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
                                    C.POINTER(vsg_intrinsics), C.c_float, C.c_float, C.c_float]),
+    "vsg_render_frame_device_pose": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_float), C.c_void_p,
+                                               C.POINTER(vsg_intrinsics), C.c_float, C.c_float, C.c_float]),
     "vsg_render_buffers": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_void_p)] * 4),
     "vsg_render_download": (C.c_int, [C.c_void_p] + [C.c_void_p] * 4),
     "vsg_render_counters": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
@@ -123,6 +133,9 @@ _SIGNATURES = {
     "vsg_render_set_gn_noise": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
     "vsg_pose_gn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
                                    C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "vsg_pose_gn_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
+                                     C.c_float, C.c_uint32, C.POINTER(C.c_float), C.c_double, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]),
     "vsg_import_voxels": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_export_cell_masks": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "vsg_block_image_bytes": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
@@ -352,6 +365,13 @@ class VoxelScene:
         s = C.c_void_p()
         _check(lib().vsg_scene_get_stream(self._h, C.byref(s)))
         return s.value or 0
+
+    def getHashData(self) -> dict:
+        """VoxelScene::getHashData (VoxelScene.h:125-136): the scene's device arrays
+        (addresses and sizes, this library's layout; vsg_hash_data in include/vsg.h)."""
+        h = vsg_hash_data()
+        _check(lib().vsg_scene_hash_data(self._h, C.byref(h)))
+        return {name: getattr(h, name) for name, _ in vsg_hash_data._fields_}
 
     def counters(self) -> dict:
         c = vsg_counters()
@@ -703,6 +723,36 @@ class RayCastRender:
                                       float(minDepth), float(maxDepth), float(max_residual), float(lam), P, st))
         return (np.array(P[:], np.float64).reshape(4, 4),
                 {"cost": st[0], "count": int(st[1]), "solved": bool(st[2]), "rot": st[3], "trans": st[4]})
+
+    def gn_refine(self, voxelScene: VoxelScene, observed_depth, fx, fy, cx, cy, minDepth, maxDepth, pose,
+                  max_residuals, lam: float = 1e-4):
+        """len(max_residuals) Gauss-Newton steps with the pose kept on the device between
+        them (one read-back); returns (new pose float64 4x4, stats dict of the last step)."""
+        P = (C.c_double * 16)(*np.asarray(pose, np.float64).reshape(16).tolist())
+        st = (C.c_double * 8)()
+        res = [float(r) for r in max_residuals]
+        R = (C.c_float * max(1, len(res)))(*res)
+        k = vsg_intrinsics(fx, fy, cx, cy)
+        _check(lib().vsg_pose_gn_refine(self._h, voxelScene.handle, _dev_ptr(observed_depth), C.byref(k),
+                                        float(minDepth), float(maxDepth), len(res), R, float(lam), P, st))
+        return (np.array(P[:], np.float64).reshape(4, 4),
+                {"cost": st[0], "count": int(st[1]), "solved": bool(st[2]), "rot": st[3], "trans": st[4]})
+
+    def render_device_pose(self, voxelScene: VoxelScene, cameraPos, pose_device, fx: float, fy: float, cx: float,
+                           cy: float, minDepth: float, maxDepth: float, voxelSize: float = 0.0) -> None:
+        """RayCastRender::render with the pose as the reference passes it: a device
+        buffer of 16 floats (torch CUDA tensor or address), read by the kernels."""
+        if not self.m_isAllocated:
+            print("RayCastRender: Not initialized!", file=sys.stderr)
+            return
+        if voxelScene is None or not voxelScene.isAllocated():
+            print("RayCastRender: Invalid voxel scene!", file=sys.stderr)
+            return
+        cp = np.asarray(cameraPos, dtype=np.float32).reshape(3)
+        k = vsg_intrinsics(fx, fy, cx, cy)
+        _check(lib().vsg_render_frame_device_pose(self._h, voxelScene.handle, (C.c_float * 3)(*cp.tolist()),
+                                                  _dev_ptr(pose_device), C.byref(k), float(minDepth),
+                                                  float(maxDepth), float(voxelSize)))
 
     def counters(self):
         h, m = C.c_uint32(), C.c_uint32()
