@@ -160,6 +160,15 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with build_lib.py (no CPU fallback exists)")
+        # PyTorch-ROCm ships its own HIP runtime and asks for it as "libamdhip64.so";
+        # loaded after libvsg.so (which asks for "libamdhip64.so.7") it would be a second
+        # runtime in the process, with its own streams.  Loading torch first makes
+        # libvsg.so bind to torch's runtime, so stream handles and device pointers
+        # cross between them.  Without torch the system runtime is used.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in _SIGNATURES.items():
             fn = getattr(L, name)
