@@ -130,6 +130,18 @@ vsg_status vsg_scene_get_stream(const vsg_scene* scene, void** hip_stream);
 vsg_status vsg_integrate(vsg_scene* scene, const vsg_frame* frame, float truncation, const float pose[16],
                          const vsg_intrinsics* intrinsics);
 
+/* vsg_integrate without any allocation: no hipMalloc, hipFree or allocating library
+ * call, whatever the frame size, for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid it).  No counterpart in
+ * the reference.  Same results as vsg_integrate.  Its per-pixel and per-workgroup
+ * scratch is a static device array of the library (4.14 MiB, held by every GPU the
+ * library runs on until the process ends), so frames are limited to 2^20 pixels
+ * (VSG_ERR_INVALID_ARG beyond).  All calls on one GPU share that array: calls that
+ * overlap on one GPU (scenes on different streams) race; serialise them.
+ * Asynchronous. */
+vsg_status vsg_integrate_noalloc(vsg_scene* scene, const vsg_frame* frame, float truncation, const float pose[16],
+                                 const vsg_intrinsics* intrinsics);
+
 /* Counters of the last integrate (synchronises the scene's stream). */
 vsg_status vsg_scene_counters(vsg_scene* scene, vsg_counters* out);
 

@@ -399,36 +399,33 @@ vsg_status vsg_scene_get_stream(const vsg_scene* scene, void** hip_stream) {
   return VSG_OK;
 }
 
-vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, const float pose[16],
-                         const vsg_intrinsics* k) {
-  if (!s || !f || !pose || !k) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: null argument");
-  if (!f->points || !f->colors) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: null point or colour map");
-  if (f->width <= 0 || f->height <= 0) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: empty frame");
+namespace {
+
+vsg_status check_frame(const char* who, vsg_scene* s, const vsg_frame* f, float truncation, const float pose[16],
+                       const vsg_intrinsics* k) {
+  char msg[160];
+  auto bad = [&](const char* what) {
+    std::snprintf(msg, sizeof(msg), "%s: %s", who, what);
+    return fail(VSG_ERR_INVALID_ARG, msg);
+  };
+  if (!s || !f || !pose || !k) return bad("null argument");
+  if (!f->points || !f->colors) return bad("null point or colour map");
+  if (f->width <= 0 || f->height <= 0) return bad("empty frame");
   const size_t npix = (size_t)f->width * (size_t)f->height;
-  if (npix > (1u << 22))  // request keys are (pixel << 10 | DDA step) in 32 bits
-    return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: frames are limited to 2^22 pixels");
-  if (!std::isfinite(truncation)) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: truncation must be finite");
-  if (!all_finite(pose, 16)) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: pose must be finite");
+  if (npix > (1u << 22)) return bad("frames are limited to 2^22 pixels");  // keys are (pixel << 10 | DDA step)
+  if (!std::isfinite(truncation)) return bad("truncation must be finite");
+  if (!all_finite(pose, 16)) return bad("pose must be finite");
   const float kk[4] = {k->fx, k->fy, k->cx, k->cy};
-  if (!all_finite(kk, 4)) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate: intrinsics must be finite");
+  if (!all_finite(kk, 4)) return bad("intrinsics must be finite");
+  return VSG_OK;
+}
 
+// The frame's kernels with the given per-pixel and per-workgroup scratch (every word
+// of [0, npix) and [0, nwg) is written before it is read).
+vsg_status integrate_frame(vsg_scene* s, const vsg_frame* f, float truncation, const float pose[16],
+                           const vsg_intrinsics* k, uint32_t* pixcnt, uint32_t* wgsum, uint32_t* claim_part) {
+  const size_t npix = (size_t)f->width * (size_t)f->height;
   const uint32_t nwg = (uint32_t)((npix + 255) / 256);
-  if (npix > s->pix_cap) {
-    VSG_HIP(hipStreamSynchronize(s->stream));
-    (void)hipFree(s->pixcnt);
-    (void)hipFree(s->wgsum);
-    (void)hipFree(s->claim_part);
-    s->pixcnt = nullptr;
-    s->wgsum = nullptr;
-    s->claim_part = nullptr;
-    s->pix_cap = 0;
-    vsg_status r = dmalloc(&s->pixcnt, npix * sizeof(uint32_t), "pixel scratch");
-    if (r == VSG_OK) r = dmalloc(&s->wgsum, (size_t)nwg * sizeof(uint32_t), "workgroup scratch");
-    if (r == VSG_OK) r = dmalloc(&s->claim_part, (size_t)nwg * 8 * sizeof(uint32_t), "workgroup counters");
-    if (r != VSG_OK) return r;
-    s->pix_cap = npix;
-  }
-
   FrameArgs fa;
   std::memset(&fa, 0, sizeof(fa));
   for (int i = 0; i < 16; ++i) {
@@ -452,22 +449,22 @@ vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, con
   vsg_status r;
   {
     TimedLaunch tl(s->stream, VSG_K_ALLOC_CLAIM);
-    launch_alloc_claim(s->stream, s->t, fa, pts, s->pixcnt, s->diag ? s->mark_abkt : nullptr, s->claim_part);
+    launch_alloc_claim(s->stream, s->t, fa, pts, pixcnt, s->diag ? s->mark_abkt : nullptr, claim_part);
   }
   if ((r = check_launch("alloc claim")) != VSG_OK) return r;
   {
     TimedLaunch tl(s->stream, VSG_K_ALLOC_RESOLVE);
-    launch_alloc_resolve(s->stream, s->t, fa, pts, s->pixcnt, s->wgsum);
+    launch_alloc_resolve(s->stream, s->t, fa, pts, pixcnt, wgsum);
   }
   if ((r = check_launch("alloc resolve")) != VSG_OK) return r;
   {
     TimedLaunch tl(s->stream, VSG_K_ALLOC_SCAN);
-    launch_alloc_scan(s->stream, s->st, s->wgsum, s->claim_part, nwg, fa.cset, s->t.num_blocks);
+    launch_alloc_scan(s->stream, s->st, wgsum, claim_part, nwg, fa.cset, s->t.num_blocks);
   }
   if ((r = check_launch("alloc scan")) != VSG_OK) return r;
   {
     TimedLaunch tl(s->stream, VSG_K_ALLOC_INSERT);
-    launch_alloc_insert(s->stream, s->t, s->st, fa, pts, s->pixcnt, s->wgsum);
+    launch_alloc_insert(s->stream, s->t, s->st, fa, pts, pixcnt, wgsum);
   }
   if ((r = check_launch("alloc insert")) != VSG_OK) return r;
   {
@@ -483,6 +480,81 @@ vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, con
   if ((r = check_launch("integrate")) != VSG_OK) return r;
   s->frames++;
   return VSG_OK;
+}
+
+}  // namespace
+
+vsg_status vsg_integrate(vsg_scene* s, const vsg_frame* f, float truncation, const float pose[16],
+                         const vsg_intrinsics* k) {
+  vsg_status r = check_frame("vsg_integrate", s, f, truncation, pose, k);
+  if (r != VSG_OK) return r;
+  const size_t npix = (size_t)f->width * (size_t)f->height;
+  const uint32_t nwg = (uint32_t)((npix + 255) / 256);
+  if (npix > s->pix_cap) {
+    VSG_HIP(hipStreamSynchronize(s->stream));
+    (void)hipFree(s->pixcnt);
+    (void)hipFree(s->wgsum);
+    (void)hipFree(s->claim_part);
+    s->pixcnt = nullptr;
+    s->wgsum = nullptr;
+    s->claim_part = nullptr;
+    s->pix_cap = 0;
+    r = dmalloc(&s->pixcnt, npix * sizeof(uint32_t), "pixel scratch");
+    if (r == VSG_OK) r = dmalloc(&s->wgsum, (size_t)nwg * sizeof(uint32_t), "workgroup scratch");
+    if (r == VSG_OK) r = dmalloc(&s->claim_part, (size_t)nwg * 8 * sizeof(uint32_t), "workgroup counters");
+    if (r != VSG_OK) return r;
+    s->pix_cap = npix;
+  }
+  return integrate_frame(s, f, truncation, pose, k, s->pixcnt, s->wgsum, s->claim_part);
+}
+
+// ---- integration without allocation ---------------------------------------------------
+//
+// The scratch of vsg_integrate_noalloc: per-pixel counts, then per-workgroup sums, then
+// per-workgroup claim counters, for frames of up to kNoAllocPixels pixels.  One copy per
+// GPU (each device holds its own, for the life of the process); its device address is
+// looked up once per device with that device current.
+constexpr uint32_t kNoAllocPixels = 1u << 20;
+constexpr uint32_t kNoAllocGroups = kNoAllocPixels / 256u;
+constexpr uint32_t kNoAllocWords = kNoAllocPixels + kNoAllocGroups + 8u * kNoAllocGroups;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (looked up on first use).
+vsg_status scratch_on_current_device(uint32_t** out) {
+  int dev = 0;
+  VSG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(VSG_ERR_INVALID_ARG, "vsg_integrate_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(VSG_ERR_HIP, "vsg_integrate_noalloc: hipGetSymbolAddress(scratch) failed");
+    }
+    g_scratch_ptr[dev] = (uint32_t*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return VSG_OK;
+}
+
+}  // namespace
+
+vsg_status vsg_integrate_noalloc(vsg_scene* s, const vsg_frame* f, float truncation, const float pose[16],
+                                 const vsg_intrinsics* k) {
+  vsg_status r = check_frame("vsg_integrate_noalloc", s, f, truncation, pose, k);
+  if (r != VSG_OK) return r;
+  if ((size_t)f->width * (size_t)f->height > kNoAllocPixels)
+    return fail(VSG_ERR_INVALID_ARG, "vsg_integrate_noalloc: frames are limited to 2^20 pixels (its static scratch)");
+  uint32_t* base = nullptr;
+  if ((r = scratch_on_current_device(&base)) != VSG_OK) return r;
+  return integrate_frame(s, f, truncation, pose, k, base, base + kNoAllocPixels, base + kNoAllocPixels + kNoAllocGroups);
 }
 
 vsg_status vsg_scene_counters(vsg_scene* s, vsg_counters* out) {

@@ -87,6 +87,8 @@ _SIGNATURES = {
     "vsg_scene_get_stream": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "vsg_integrate": (C.c_int, [C.c_void_p, C.POINTER(vsg_frame), C.c_float, C.POINTER(C.c_float),
                                 C.POINTER(vsg_intrinsics)]),
+    "vsg_integrate_noalloc": (C.c_int, [C.c_void_p, C.POINTER(vsg_frame), C.c_float, C.POINTER(C.c_float),
+                                        C.POINTER(vsg_intrinsics)]),
     "vsg_scene_counters": (C.c_int, [C.c_void_p, C.POINTER(vsg_counters)]),
     "vsg_heap_free_count": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
     "vsg_num_integrated_frames": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32)]),
