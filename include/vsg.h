@@ -491,7 +491,10 @@ vsg_status vsg_block_image_bytes(const vsg_scene* scene, uint32_t* out);
  * goes to coords[3*i..3*i+2] and its image to images + i * vsg_block_image_bytes
  * (HOST memory), its hash entry is freed and its block returns to the free store
  * with zeroed voxels.  *count = blocks removed; when it equals capacity more may
- * remain.  Synchronising. */
+ * remain.  Synchronising.  The device scratch of vsg_stream_out and vsg_stream_in is
+ * held by the scene between calls (as large as the largest call needed: for
+ * vsg_stream_out about min(capacity, allocated blocks) images plus 13 bytes per
+ * allocated block) and freed with it. */
 vsg_status vsg_stream_out(vsg_scene* scene, const float center[3], float radius, uint32_t capacity,
                           int32_t* coords, void* images, uint32_t* count);
 

@@ -265,6 +265,14 @@ class VoxelScene {
 // keep the reference's names and order; they run synchronously on the caller's
 // thread, so multiThreaded and streamingEnabled are accepted and ignored (the
 // reference's worker thread and Windows events have no counterpart).
+//
+// Two departures from the reference's chunk lists (chunkData, VoxelStreamingManager.h:
+// 78-118): a chunk holds one image per block coordinate, the newest (a block streamed
+// out, re-created by integration and streamed out again replaces its older image, where
+// the reference appends a second copy); and a chunk larger than what one stream-in may
+// still collect is collected in parts over several calls, where the reference returns
+// at that chunk (VoxelStreamingManager.cpp:1080-1085) and never collects it or any
+// chunk after it.
 class VoxelStreamingManager {
  public:
   VoxelStreamingManager(VoxelScene& scene, const Params& params, const float3& voxelExtents, const int3& gridDimensions,
@@ -307,8 +315,11 @@ class VoxelStreamingManager {
   }
 
   // Collects the blocks of the streamed-out chunks the sphere touches (the chunk box
-  // around the centre's chunk, isChunkInSphere) and empties those chunks
-  // (streamInFindChunk, VoxelStreamingManager.cpp:993-1130).  Returns how many.
+  // around the centre's chunk, isChunkInSphere) and removes them from the grid
+  // (streamInFindChunk, VoxelStreamingManager.cpp:993-1130), at most
+  // m_maxNumberOfSDFBlocksIntegrateFromGlobalHash: a chunk that does not fit whole
+  // gives its first blocks and keeps the rest for the next call.  With useParts, one
+  // chunk (or the part that fits) per call.  Returns how many.
   unsigned int streamInCopyToGPUBuffer(const float3& sphereCenter, float radius, bool useParts = false,
                                        bool /*multiThreaded*/ = false) {
     m_inCoords.clear();
@@ -326,12 +337,13 @@ class VoxelStreamingManager {
           const int3 ch = make_int3(x, y, z);
           Chunk& c = m_grid[linearizeChunkPos(ch)];
           if (c.coords.empty() || !isChunkInSphere(ch, sphereCenter, radius)) continue;
-          const size_t nb = c.coords.size() / 3;
-          if (nb + m_inCoords.size() / 3 > m_maxNumberOfSDFBlocksIntegrateFromGlobalHash) return nStreamdInBlocks();
-          m_inCoords.insert(m_inCoords.end(), c.coords.begin(), c.coords.end());
-          m_inImages.insert(m_inImages.end(), c.images.begin(), c.images.end());
-          c = Chunk{};
-          if (useParts) return nStreamdInBlocks();
+          const size_t room = m_maxNumberOfSDFBlocksIntegrateFromGlobalHash - nStreamdInBlocks();
+          const size_t take = std::min(c.coords.size() / 3, room);
+          m_inCoords.insert(m_inCoords.end(), c.coords.begin(), c.coords.begin() + 3 * take);
+          m_inImages.insert(m_inImages.end(), c.images.begin(), c.images.begin() + take * m_imageBytes);
+          c.dropFront(take, m_imageBytes);
+          if (useParts || nStreamdInBlocks() == m_maxNumberOfSDFBlocksIntegrateFromGlobalHash)
+            return nStreamdInBlocks();
         }
     return nStreamdInBlocks();
   }
@@ -367,9 +379,14 @@ class VoxelStreamingManager {
       if (n < m_maxNumberOfSDFBlocksIntegrateFromGlobalHash) return;
     }
   }
+  // As many collect + insert passes as the cap needs: a pass that fills the cap and
+  // inserts something is followed by another.
   void streamIn(VoxelScene& /*scene*/, const float3& cameraPos, float radius) {
-    streamInCopyToGPUBuffer(cameraPos, radius);
-    streamInInsertToHashTable();
+    for (;;) {
+      const unsigned int n = streamInCopyToGPUBuffer(cameraPos, radius);
+      const unsigned int inserted = streamInInsertToHashTable();
+      if (n < m_maxNumberOfSDFBlocksIntegrateFromGlobalHash || inserted == 0) return;
+    }
   }
 
   size_t numStreamedBlocks() const {
@@ -382,9 +399,38 @@ class VoxelStreamingManager {
   unsigned int m_maxNumberOfSDFBlocksIntegrateFromGlobalHash = 100000;  // VoxelStreamingManager.cpp:113
 
  private:
+  // One image per block coordinate, in first-stream-out order; `where` maps a block's
+  // packed coordinate to its position in coords/images.
   struct Chunk {
     std::vector<int32_t> coords;  // 3 per block
     std::vector<uint8_t> images;  // vsg_block_image_bytes per block
+    std::unordered_map<uint64_t, uint32_t> where;
+
+    // 21 bits per axis, as the scene's own keys: injective over the coordinates a
+    // scene can hold ([-2^20, 2^20) per axis).
+    static uint64_t key(const int32_t* c) {
+      return ((uint64_t)(c[0] & 0x1FFFFF) << 42) | ((uint64_t)(c[1] & 0x1FFFFF) << 21) | (uint64_t)(c[2] & 0x1FFFFF);
+    }
+    void put(const int32_t* c, const uint8_t* image, size_t bytes) {
+      const auto it = where.find(key(c));
+      if (it != where.end()) {
+        std::copy(image, image + bytes, images.begin() + (size_t)it->second * bytes);
+        return;
+      }
+      where.emplace(key(c), (uint32_t)(coords.size() / 3));
+      coords.insert(coords.end(), c, c + 3);
+      images.insert(images.end(), image, image + bytes);
+    }
+    void dropFront(size_t k, size_t bytes) {
+      if (k == coords.size() / 3) {
+        *this = Chunk{};
+        return;
+      }
+      coords.erase(coords.begin(), coords.begin() + 3 * k);
+      images.erase(images.begin(), images.begin() + k * bytes);
+      where.clear();
+      for (size_t i = 0; i < coords.size() / 3; ++i) where.emplace(key(&coords[3 * i]), (uint32_t)i);
+    }
   };
 
   unsigned int nStreamdInBlocks() const { return (unsigned int)(m_inCoords.size() / 3); }
@@ -404,8 +450,7 @@ class VoxelStreamingManager {
       if (c.coords.empty()) {
         c.coords.reserve((size_t)m_initialChunkListSize * 3);
       }
-      c.coords.insert(c.coords.end(), coords + 3 * (size_t)i, coords + 3 * (size_t)i + 3);
-      c.images.insert(c.images.end(), images + (size_t)i * m_imageBytes, images + (size_t)(i + 1) * m_imageBytes);
+      c.put(coords + 3 * (size_t)i, images + (size_t)i * m_imageBytes, m_imageBytes);
     }
   }
 

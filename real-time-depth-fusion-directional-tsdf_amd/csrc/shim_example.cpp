@@ -15,12 +15,14 @@
 // Then runMarchingCubesCUDA with the reference's cubes (inside blocks, no weight test)
 // and, when a mesh path is given, VoxelScene::saveToMesh.
 //
-// With a sixth argument "stream:<radius>", a VoxelStreamingManager (1 m chunks, 64^3
-// chunks from -32) runs before every integration from the second frame on: stream-out of
-// the blocks at distance >= radius from the camera, then stream-in of the chunks within
+// With a sixth argument "stream:<radius>[:<cap>]", a VoxelStreamingManager (1 m chunks,
+// 64^3 chunks from -32; at most <cap> blocks per stream-out or stream-in pass, 100000 when
+// not given) runs before every integration from the second frame on: stream-out of the
+// blocks at distance >= radius from the camera, then stream-in of the chunks within
 // radius (the order of VoxelSG/src/voxelSG.cpp:503-552).
 //
-// usage: vsg_shim_example <workload 0|1> <frames> [points.ply|-] [gen|-] [mesh.ply|-] [stream:<radius>]
+// usage: vsg_shim_example <workload 0|1> <frames> [points.ply|-] [gen|-] [mesh.ply|-]
+//                         [stream:<radius>[:<cap>]]
 // prints: frames, allocated blocks, hits, misses, a checksum of the last depth map,
 // the number of surface points found, the number of triangles and the number of
 // blocks held in the streaming chunk grid.
@@ -86,9 +88,18 @@ int main(int argc, char** argv) {
 
   const std::string streamArg = argc > 6 ? argv[6] : "";
   const bool streaming = streamArg.rfind("stream:", 0) == 0;
-  const float streamRadius = streaming ? std::strtof(streamArg.c_str() + 7, nullptr) : 0.0f;
+  char* capArg = nullptr;
+  const float streamRadius = streaming ? std::strtof(streamArg.c_str() + 7, &capArg) : 0.0f;
   VoxelStreamingManager streamer(scene, params, make_float3(1.0f, 1.0f, 1.0f), make_int3(64, 64, 64),
                                  make_int3(-32, -32, -32));
+  if (streaming && capArg && *capArg == ':') {
+    const long cap = std::strtol(capArg + 1, nullptr, 10);
+    if (cap <= 0) {
+      std::fprintf(stderr, "stream cap must be positive\n");
+      return 2;
+    }
+    streamer.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash = (unsigned int)cap;
+  }
 
   unsigned int hits = 0, misses = 0;
   double checksum = 0.0;

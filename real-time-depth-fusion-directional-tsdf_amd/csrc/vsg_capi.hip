@@ -122,6 +122,8 @@ struct vsg_scene {
   uint32_t* march_stats = nullptr;  // kMarchStats counters
   uint32_t* dmeta = nullptr;        // directional extension: slot ids and weights per voxel
   float2* dval = nullptr;           // directional extension: slot dtsdf per voxel
+  uint8_t* stream_arena = nullptr;  // device scratch of vsg_stream_out / vsg_stream_in, grown on demand
+  size_t stream_arena_bytes = 0;
 };
 
 struct vsg_depthgen {
@@ -203,6 +205,7 @@ void free_scene(vsg_scene* s) {
   (void)hipFree(s->mark_rbkt);
   (void)hipFree(s->mark_blk);
   (void)hipFree(s->march_stats);
+  (void)hipFree(s->stream_arena);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
@@ -1631,18 +1634,31 @@ BlockPlanes planes_of(vsg_scene* s) {
   return BlockPlanes{(uint8_t*)s->vox, s->word_bytes, s->col, s->dmeta, s->dval};
 }
 
-// Device scratch of one streaming call, freed when the call returns.
-struct CallScratch {
-  std::vector<void*> bufs;
-  ~CallScratch() {
-    for (void* b : bufs) (void)hipFree(b);
-  }
-  template <class T> vsg_status get(T** p, size_t bytes, const char* what) {
-    vsg_status st = dmalloc(p, bytes ? bytes : 16, what);
-    if (st == VSG_OK) bufs.push_back((void*)*p);
-    return st;
+// The device scratch of a streaming call: consecutive 256-byte aligned pieces of the
+// scene's stream arena.  The arena is kept between calls and replaced by a larger one
+// only when a call needs more than it holds (both entry points synchronise the scene's
+// stream first, so no kernel still uses the old one).
+struct ArenaPlan {
+  size_t bytes = 0;
+  size_t add(size_t n) {
+    const size_t at = bytes;
+    bytes += (n + 255) & ~(size_t)255;
+    return at;
   }
 };
+
+vsg_status stream_arena(vsg_scene* s, size_t bytes, uint8_t** base) {
+  if (bytes > s->stream_arena_bytes) {
+    (void)hipFree(s->stream_arena);
+    s->stream_arena = nullptr;
+    s->stream_arena_bytes = 0;
+    vsg_status st = dmalloc(&s->stream_arena, bytes, "streaming scratch");
+    if (st != VSG_OK) return st;
+    s->stream_arena_bytes = bytes;
+  }
+  *base = s->stream_arena;
+  return VSG_OK;
+}
 
 }  // namespace
 
@@ -1666,25 +1682,27 @@ vsg_status vsg_stream_out(vsg_scene* s, const float center[3], float radius, uin
   if (n == 0 || capacity == 0) return VSG_OK;
   const BlockPlanes bp = planes_of(s);
   const size_t ib = block_image_bytes(bp);
-  CallScratch cs;
-  uint8_t* sel = nullptr;
-  uint32_t *rank = nullptr, *sums = nullptr;
+  // pieces for at most min(n, capacity) blocks leaving the scene
+  const size_t mmax = capacity < n ? capacity : n;
+  ArenaPlan plan;
+  const size_t o_sel = plan.add(n), o_rank = plan.add((size_t)n * 4),
+               o_sums = plan.add((size_t)stream_scan_words(n) * 4), o_out = plan.add(mmax * 4),
+               o_remap = plan.add((size_t)n * 4), o_coords = plan.add(mmax * sizeof(int4)), o_img = plan.add(mmax * ib);
+  uint8_t* base = nullptr;
   vsg_status st;
-  if ((st = cs.get(&sel, n, "stream-out flags")) != VSG_OK) return st;
-  if ((st = cs.get(&rank, (size_t)n * 4, "stream-out ranks")) != VSG_OK) return st;
-  if ((st = cs.get(&sums, (size_t)stream_scan_words(n) * 4, "stream-out scan")) != VSG_OK) return st;
+  if ((st = stream_arena(s, plan.bytes, &base)) != VSG_OK) return st;
+  uint8_t* sel = base + o_sel;
+  uint32_t* rank = (uint32_t*)(base + o_rank);
+  uint32_t* sums = (uint32_t*)(base + o_sums);
   const uint32_t selected = stream_out_select(s->stream, s->t, n, center, radius, s->p.voxel_size, sel, rank, sums);
   if ((st = check_launch("stream-out selection")) != VSG_OK) return st;
   if (selected == 0xFFFFFFFFu) return fail(VSG_ERR_HIP, "vsg_stream_out: selection failed");
   const uint32_t m = selected < capacity ? selected : capacity;
   if (m == 0) return VSG_OK;
-  uint32_t *out = nullptr, *remap = nullptr;
-  int4* dcoords = nullptr;
-  uint8_t* dimg = nullptr;
-  if ((st = cs.get(&out, (size_t)m * 4, "stream-out list")) != VSG_OK) return st;
-  if ((st = cs.get(&remap, (size_t)n * 4, "stream-out pointer map")) != VSG_OK) return st;
-  if ((st = cs.get(&dcoords, (size_t)m * sizeof(int4), "stream-out coordinates")) != VSG_OK) return st;
-  if ((st = cs.get(&dimg, (size_t)m * ib, "stream-out images")) != VSG_OK) return st;
+  uint32_t* out = (uint32_t*)(base + o_out);
+  uint32_t* remap = (uint32_t*)(base + o_remap);
+  int4* dcoords = (int4*)(base + o_coords);
+  uint8_t* dimg = base + o_img;
   stream_out_remove(s->stream, s->t, s->st, bp, n, m, sel, rank, out, remap, dcoords, dimg);
   if ((st = check_launch("stream-out")) != VSG_OK) return st;
   VSG_HIP(hipStreamSynchronize(s->stream));
@@ -1710,23 +1728,24 @@ vsg_status vsg_stream_in(vsg_scene* s, uint32_t n, const int32_t* coords, const 
   const BlockPlanes bp = planes_of(s);
   const size_t ib = block_image_bytes(bp);
   const bool fp16 = s->p.store == VSG_STORE_FP16;
-  CallScratch cs;
-  int4* dcoords = nullptr;
-  uint8_t *dimg = nullptr, *dstatus = nullptr, *ok = nullptr;
-  uint32_t *entry = nullptr, *rank = nullptr, *sums = nullptr;
+  ArenaPlan plan;
+  const size_t o_coords = plan.add((size_t)n * sizeof(int4)), o_img = plan.add((size_t)n * ib),
+               o_status = plan.add(n), o_ok = plan.add(n), o_entry = plan.add((size_t)n * 4),
+               o_rank = plan.add((size_t)n * 4), o_sums = plan.add((size_t)stream_scan_words(n) * 4);
+  uint8_t* base = nullptr;
   vsg_status st;
-  if ((st = cs.get(&dcoords, (size_t)n * sizeof(int4), "stream-in coordinates")) != VSG_OK) return st;
-  if ((st = cs.get(&dimg, (size_t)n * ib, "stream-in images")) != VSG_OK) return st;
-  if ((st = cs.get(&dstatus, n, "stream-in status")) != VSG_OK) return st;
-  if ((st = cs.get(&ok, n, "stream-in flags")) != VSG_OK) return st;
-  if ((st = cs.get(&entry, (size_t)n * 4, "stream-in entries")) != VSG_OK) return st;
-  if ((st = cs.get(&rank, (size_t)n * 4, "stream-in ranks")) != VSG_OK) return st;
-  if ((st = cs.get(&sums, (size_t)stream_scan_words(n) * 4, "stream-in scan")) != VSG_OK) return st;
+  if ((st = stream_arena(s, plan.bytes, &base)) != VSG_OK) return st;
+  int4* dcoords = (int4*)(base + o_coords);
+  uint8_t* dimg = base + o_img;
+  uint8_t* dstatus = base + o_status;
+  uint8_t* ok = base + o_ok;
+  uint32_t* entry = (uint32_t*)(base + o_entry);
+  uint32_t* rank = (uint32_t*)(base + o_rank);
+  uint32_t* sums = (uint32_t*)(base + o_sums);
   std::vector<int4> hc(n);
   for (uint32_t i = 0; i < n; ++i) hc[i] = make_int4(coords[3 * i], coords[3 * i + 1], coords[3 * i + 2], 0);
   VSG_HIP(hipMemcpy(dcoords, hc.data(), (size_t)n * sizeof(int4), hipMemcpyHostToDevice));
   VSG_HIP(hipMemcpy(dimg, images, (size_t)n * ib, hipMemcpyHostToDevice));
-  std::vector<uint8_t> hs(n, 0xFF), part(n);
   uint32_t total = 0;
   // Batches no larger than the free blocks, so every insertion of a batch gets a
   // block; once none is free the rest are classified without inserting (3 = no

@@ -1131,6 +1131,50 @@ def decode_block_images(images: np.ndarray, store: str = "fp32", directional: bo
     return out
 
 
+class _Chunk:
+    """One chunk of the streaming grid: one image per block coordinate, in the order the
+    coordinates first arrived; a coordinate that arrives again keeps its place and takes
+    the newer image.  Coordinates are packed as the scene's keys (21 bits per axis,
+    injective over the coordinates a scene can hold)."""
+
+    def __init__(self, image_bytes: int):
+        self.keys = np.zeros(0, np.int64)
+        self.coords = np.zeros((0, 3), np.int32)
+        self.images = np.zeros((0, image_bytes), np.uint8)
+
+    def __len__(self) -> int:
+        return len(self.keys)
+
+    @staticmethod
+    def pack(coords) -> np.ndarray:
+        c = np.asarray(coords, np.int64).reshape(-1, 3) & 0x1FFFFF
+        return (c[:, 0] << 42) | (c[:, 1] << 21) | c[:, 2]
+
+    def put(self, coords, images) -> None:
+        k = self.pack(coords)
+        if len(np.unique(k)) < len(k):  # repeated within the batch: first place, last image
+            u, first = np.unique(k, return_index=True)
+            last = len(k) - 1 - np.unique(k[::-1], return_index=True)[1]
+            o = np.argsort(first)
+            k, coords, images = k[first[o]], coords[first[o]], images[last[o]]
+        if len(self.keys):
+            srt = np.argsort(self.keys)
+            j = np.minimum(np.searchsorted(self.keys[srt], k), len(srt) - 1)
+            hit = self.keys[srt[j]] == k
+            self.images[srt[j[hit]]] = images[hit]
+            k, coords, images = k[~hit], coords[~hit], images[~hit]
+        if len(k):
+            self.keys = np.concatenate([self.keys, k])
+            self.coords = np.concatenate([self.coords, coords])
+            self.images = np.concatenate([self.images, images])
+
+    def take_front(self, n: int):
+        """Removes and returns the first n blocks (coords, images)."""
+        c, im = self.coords[:n], self.images[:n]
+        self.keys, self.coords, self.images = self.keys[n:].copy(), self.coords[n:].copy(), self.images[n:].copy()
+        return c, im
+
+
 class VoxelStreamingManager:
     """VoxelStreamingManager (VoxelSG/include/VoxelStreamingManager.h:120-301,
     VoxelSG/src/VoxelStreamingManager.cpp): blocks far from a sphere centre move from
@@ -1142,7 +1186,15 @@ class VoxelStreamingManager:
     outside the grid are dropped as the reference drops them (updateChunkGrid,
     .cpp:875-879) and counted in droppedOutsideGrid.  The four stages keep the
     reference's names; multiThreaded is accepted and ignored (every stage runs
-    synchronously on the caller's thread)."""
+    synchronously on the caller's thread).
+
+    Two departures from the reference's chunk lists (chunkData, VoxelStreamingManager.h:
+    78-118): a chunk holds one image per block coordinate, the newest (a block streamed
+    out, re-created by integration and streamed out again replaces its older image,
+    where the reference appends a second copy to the chunk's list); and a chunk larger
+    than what one stream-in may still collect (m_maxNumberOfSDFBlocksIntegrateFromGlobalHash)
+    is collected in parts over several calls, where the reference returns at that chunk
+    (.cpp:1080-1085) and never collects it or any chunk after it."""
 
     def __init__(self, scene: VoxelScene, params: Params, voxelExtents, gridDimensions, minGridPos,
                  initialChunkListSize: int = 1000, streamingEnabled: bool = False):
@@ -1158,7 +1210,8 @@ class VoxelStreamingManager:
         self.m_minGridPos = np.asarray(minGridPos, np.int64).reshape(3)
         self.m_maxGridPos = self.m_minGridPos + self.m_gridDimensions
         self.m_initialChunkListSize = int(initialChunkListSize)
-        self.m_grid: dict[int, list] = {}  # chunk index -> [coords list, images list]
+        # chunk index -> _Chunk (one image per block coordinate, first-stream-out order)
+        self.m_grid: dict[int, _Chunk] = {}
         self.droppedOutsideGrid = 0
         self.nStreamedOut = 0
         self.nStreamedIn = 0
@@ -1212,46 +1265,56 @@ class VoxelStreamingManager:
         chunks = self.worldToChunks(self.blockWorldPos(coords))
         valid = self.isValidChunk(chunks)
         self.droppedOutsideGrid += int((~valid).sum())
-        idx = self.linearizeChunkPos(chunks)
-        for i in np.nonzero(valid)[0]:
-            entry = self.m_grid.setdefault(int(idx[i]), [[], []])
-            entry[0].append(np.asarray(coords[i], np.int32))
-            entry[1].append(np.asarray(images[i], np.uint8))
+        if not valid.any():
+            return
+        coords = np.asarray(coords, np.int32).reshape(-1, 3)[valid]
+        images = np.asarray(images, np.uint8).reshape(len(valid), -1)[valid]
+        idx = self.linearizeChunkPos(chunks[valid])
+        order = np.argsort(idx, kind="stable")
+        cut = np.flatnonzero(np.diff(idx[order])) + 1
+        for rows in np.split(order, cut):
+            ci = int(idx[rows[0]])
+            chunk = self.m_grid.get(ci)
+            if chunk is None:
+                chunk = self.m_grid[ci] = _Chunk(images.shape[1])
+            chunk.put(coords[rows], images[rows])
 
     # -- stream-in ------------------------------------------------------------------
     def streamInFindChunk(self, sphereCenter, radius: float, useParts: bool = False) -> int:
         """Collects the blocks of the streamed-out chunks that the sphere touches (the
         chunk box around the centre's chunk, isChunkInSphere), removing them from the
-        grid (VoxelStreamingManager.cpp:993-1130).  Returns how many."""
+        grid (VoxelStreamingManager.cpp:993-1130), at most
+        m_maxNumberOfSDFBlocksIntegrateFromGlobalHash of them: a chunk that does not fit
+        whole gives what fits and keeps the rest for the next call.  With useParts, one
+        chunk (or the part that fits) per call.  Returns how many."""
         cam = self.worldToChunks(sphereCenter)[0]
         rad = np.ceil(np.float32(radius) / self.m_voxelExtents).astype(np.int64)
         lo = np.maximum(cam - rad, self.m_minGridPos)
         hi = np.minimum(cam + rad, self.m_maxGridPos - 1)
-        cs, ims = [], []
+        cs, ims, got = [], [], 0
         cap = self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash
         for x in range(lo[0], hi[0] + 1):
             for y in range(lo[1], hi[1] + 1):
                 for z in range(lo[2], hi[2] + 1):
                     ci = int(self.linearizeChunkPos((x, y, z))[0])
-                    entry = self.m_grid.get(ci)
-                    if not entry or not entry[0] or not self.isChunkInSphere((x, y, z), sphereCenter, radius):
+                    chunk = self.m_grid.get(ci)
+                    if chunk is None or not self.isChunkInSphere((x, y, z), sphereCenter, radius):
                         continue
-                    if len(entry[0]) + len(cs) > cap:
-                        self._set_in(cs, ims)
-                        return len(cs)
-                    cs.extend(entry[0])
-                    ims.extend(entry[1])
-                    del self.m_grid[ci]
-                    if useParts:
-                        self._set_in(cs, ims)
-                        return len(cs)
-        self._set_in(cs, ims)
-        return len(cs)
+                    c, im = chunk.take_front(cap - got)
+                    cs.append(c)
+                    ims.append(im)
+                    got += len(c)
+                    if len(chunk) == 0:
+                        del self.m_grid[ci]
+                    if useParts or got == cap:
+                        return self._set_in(cs, ims)
+        return self._set_in(cs, ims)
 
-    def _set_in(self, cs, ims) -> None:
+    def _set_in(self, cs, ims) -> int:
         nb = self.scene.block_image_bytes()
-        self._in = (np.array(cs, np.int32).reshape(-1, 3),
-                    np.array(ims, np.uint8).reshape(-1, nb) if ims else np.zeros((0, nb), np.uint8))
+        self._in = (np.concatenate(cs) if cs else np.zeros((0, 3), np.int32),
+                    np.concatenate(ims) if ims else np.zeros((0, nb), np.uint8))
+        return len(self._in[0])
 
     def streamInCopyToGPUBuffer(self, sphereCenter, radius: float, useParts: bool = False,
                                 multiThreaded: bool = False) -> int:
@@ -1283,9 +1346,16 @@ class VoxelStreamingManager:
                 return total
 
     def streamIn(self, scene: VoxelScene, cameraPos, radius: float) -> np.ndarray:
-        """Moves the blocks of the chunks the sphere touches back into the scene."""
-        self.streamInCopyToGPUBuffer(cameraPos, radius)
-        return self.streamInInsertToHashTable()
+        """Moves the blocks of the chunks the sphere touches back into the scene, in as
+        many collect + insert passes as the cap needs (a pass that fills the cap and
+        inserts something is followed by another).  Returns the status per block over
+        all passes."""
+        out = []
+        while True:
+            n = self.streamInCopyToGPUBuffer(cameraPos, radius)
+            out.append(self.streamInInsertToHashTable())
+            if n < self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash or not np.any(out[-1] == 0):
+                return np.concatenate(out)
 
     def numStreamedBlocks(self) -> int:
-        return sum(len(e[0]) for e in self.m_grid.values())
+        return sum(len(c) for c in self.m_grid.values())

@@ -74,13 +74,16 @@ def test_cpp_generator_path_matches_python(torch_cuda):
     assert abs(float(got["checksum"]) - checksum) <= 1e-9 * abs(checksum)
 
 
-def test_cpp_streaming_manager_matches_python(torch_cuda):
+@pytest.mark.parametrize("cap", [100000, 400])
+def test_cpp_streaming_manager_matches_python(torch_cuda, cap):
     """The C++ VoxelStreamingManager (stream-out beyond 1.2 m of the camera, then stream-in
     within it, before every integration from the second frame on) gives the scene,
-    renders and chunk-grid contents the Python VoxelStreamingManager gives."""
+    renders and chunk-grid contents the Python VoxelStreamingManager gives: blocks that
+    integration re-creates beyond the radius replace their older images in both grids,
+    and with a cap of 400 blocks per pass both collect chunks in parts."""
     frames, radius = 6, 1.2
-    out = subprocess.run([EXE, "1", str(frames), "-", "-", "-", f"stream:{radius}"], capture_output=True, text=True,
-                         timeout=120)
+    out = subprocess.run([EXE, "1", str(frames), "-", "-", "-", f"stream:{radius}:{cap}"], capture_output=True,
+                         text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     tok = out.stdout.split()
     got = {tok[i]: tok[i + 1] for i in range(0, len(tok) - 1, 2)}
@@ -88,6 +91,7 @@ def test_cpp_streaming_manager_matches_python(torch_cuda):
     sc = synth.SynthScene(1)
     g = GpuRun(torch_cuda, wl)
     mgr = vsg.VoxelStreamingManager(g.scene, g.params, (1.0, 1.0, 1.0), (64, 64, 64), (-32, -32, -32))
+    mgr.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash = cap
     for i in range(frames):
         f = sc.render(i)
         if i > 0:
@@ -102,6 +106,8 @@ def test_cpp_streaming_manager_matches_python(torch_cuda):
     checksum = float(np.sum(d[ok].astype(np.float64) * ((k[ok] % 97) + 1)))
     assert mgr.numStreamedBlocks() > 0
     assert int(got["streamed"]) == mgr.numStreamedBlocks()
+    keys = np.concatenate([c.keys for c in mgr.m_grid.values()])
+    assert len(np.unique(keys)) == len(keys)
     assert int(got["blocks"]) == g.scene.counters()["num_allocated"]
     assert (int(got["hits"]), int(got["misses"])) == (r["hits"], r["misses"])
     assert abs(float(got["checksum"]) - checksum) <= 1e-9 * abs(checksum)

@@ -243,3 +243,46 @@ def test_room_full_size_round_trip_in_parts(torch_cuda):
     for k in ("depth", "pos", "col"):
         assert np.array_equal(r0[k], r2[k]), k
     assert (r0["hits"], r0["misses"]) == (r2["hits"], r2["misses"])
+
+
+@pytest.mark.parametrize("cap", [100000, 700])
+def test_chunk_grid_keeps_newest_image_after_reintegration(torch_cuda, cap):
+    """Stream out beyond 1.2 m, integrate ten more frames (which re-create some of the
+    streamed-out blocks with new data), stream out again from the same centre, stream
+    everything in: every block comes back with its newest voxels (the re-created
+    blocks' second image, the others' first), and the chunk grid held one image per
+    distinct block coordinate in between.  With cap 700 the stream-in collects chunks
+    in parts over several passes."""
+    wl = synth.WORKLOADS[1]
+    sc = synth.SynthScene(1)
+    intr = sc.intrinsics
+    g = GpuRun(torch_cuda, wl)
+    for i in range(10):
+        g.integrate(sc.render(i), intr)
+    cam = sc.render(9)["pose"][:3, 3].astype(np.float32)
+    state0 = _gpu_by_coord(g)
+    mgr = vsg.VoxelStreamingManager(g.scene, g.params, (1.0, 1.0, 1.0), (64, 64, 64), (-32, -32, -32))
+    mgr.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash = cap
+    n1 = mgr.streamOut(g.scene, cam, 1.2)
+    out1 = set(state0) - set(_gpu_by_coord(g))
+    assert n1 == len(out1) > 1000 and mgr.numStreamedBlocks() == n1
+    for i in range(10, 20):
+        g.integrate(sc.render(i), intr)
+    state1 = _gpu_by_coord(g)
+    again = out1 & set(state1)
+    assert len(again) > 100  # blocks streamed out and re-created by integration
+    n2 = mgr.streamOut(g.scene, cam, 1.2)
+    out2 = set(state1) - set(_gpu_by_coord(g))
+    assert n2 == len(out2) and again <= out2
+    assert mgr.numStreamedBlocks() == len(out1 | out2)  # one image per distinct block
+    st = mgr.streamIn(g.scene, cam, 100.0)
+    assert np.all(st == 0) and len(st) == len(out1 | out2) and mgr.numStreamedBlocks() == 0
+    final = _gpu_by_coord(g)
+    expect = {**{c: state0[c] for c in out1}, **state1}
+    assert set(final) == set(expect)
+    for c, v in expect.items():
+        for a, b in zip(v, final[c]):
+            assert np.array_equal(a, b), c
+    # the re-created blocks really changed, so the check above tells newest from oldest
+    assert sum(not np.array_equal(state0[c][1], state1[c][1]) for c in again) > len(again) // 2
+    check_cell_masks(g)
