@@ -226,11 +226,14 @@ def se3_exp(xi) -> np.ndarray:
     return T
 
 
+GN_SCHEDULE = [0.2] * 3 + [0.08] * 3 + [0.03] * 4  # residual window (m) of the 10 Gauss-Newton iterations
+
+
 def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
     """Differentiable pose refinement (the differentiation configuration): the even frames
     0..126 fused at their poses, the 64 odd frames 1..127 perturbed by seeded uniform
-    rotations <= 2 deg and translations <= 2 cm, 10 Gauss-Newton iterations each (FP64
-    6x6 solve on the device; residual window 20 -> 8 -> 3 cm; smoothed-gradient
+    rotations <= 2 deg and translations <= 2 cm, 10 Gauss-Newton iterations each on the
+    device (vsg_pose_gn_refine; FP64 6x6 solve; residual window 20 -> 8 -> 3 cm; smoothed-gradient
     Jacobian, residuals weighted by the sensor's noise model, standard trilinear
     sampling)."""
     dev = torch.device("cuda:0")
@@ -260,9 +263,7 @@ def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
         mn, mx = synth.depth_range(f["depth"])
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for it in range(10):
-            res = 0.2 if it < 3 else (0.08 if it < 6 else 0.03)
-            T, _ = rr.gn_step(scene, obs, fx, fy, cx, cy, mn, mx, T, max_residual=res, lam=1e-3)
+        T, _ = rr.gn_refine(scene, obs, fx, fy, cx, cy, mn, mx, T, GN_SCHEDULE, lam=1e-3)
         t_gn += time.perf_counter() - t0
         errs.append(_pose_err(Tgt, T))
     scene.close()
@@ -276,18 +277,37 @@ def pose_refinement_bench(torch, vsg, synth, sc, frames, wl) -> dict:
             "ms_per_frame": round(t_gn / 64 * 1e3, 2), "ms_per_iteration": round(t_gn / 640 * 1e3, 3)}
 
 
-def large_run_bench(torch, vsg, synth, nframes: int, stream_radius: float = 0.0) -> dict:
+def _stage(ms) -> dict:
+    a = np.asarray(ms, np.float64)
+    return {"mean": round(float(a.mean()), 3), "p95": round(float(np.quantile(a, 0.95)), 3)} if a.size else {}
+
+
+def _pose_summary(R: np.ndarray) -> dict:
+    ok = (R[:, 0] <= 0.1) & (R[:, 1] <= 1.0)
+    return {"median": {"deg": round(float(np.median(R[:, 0])), 4), "mm": round(float(np.median(R[:, 1])), 3)},
+            "p95": {"deg": round(float(np.quantile(R[:, 0], 0.95)), 4), "mm": round(float(np.quantile(R[:, 1], 0.95)), 3)},
+            "max": {"deg": round(float(R[:, 0].max()), 3), "mm": round(float(R[:, 1].max()), 2)},
+            "final": {"deg": round(float(R[-1, 0]), 4), "mm": round(float(R[-1, 1]), 3)},
+            "within_0.1deg_1mm": int(ok.sum()), "frames": int(len(R))}
+
+
+def large_run_bench(torch, vsg, synth, nframes: int, stream_radius: float = 0.0, trace_path: str = "") -> dict:
     """The large combined configuration: 1024x768 room (8x8x3 m, 60 primitives), voxel 2 mm,
     truncation 1 cm, 2^22 buckets and blocks, FP16 store.  Every frame after the first is
-    tracked from a prior perturbed by a seeded uniform <= 1 deg / 1 cm (10 Gauss-Newton
-    iterations, smoothed Jacobian, noise-weighted residuals, standard trilinear sampling),
-    then integrated at the refined pose and raycast there (the reference's sampling).
-    The tracking runs against a companion model of the same frames (voxel 5 mm,
-    truncation 8 cm: wider than the sensor noise, 1-2.6 cm at 2-4 m), integrated at the
-    same refined poses; the 2 mm / 1 cm model's noisy shell pulls the alignment away from
-    the true pose.  With stream_radius > 0, every 25 frames the 2 mm scene's blocks beyond
-    that distance from the camera go to a VoxelStreamingManager chunk grid (1 m chunks) and
-    the chunks within it come back.  Per-frame times of the stages and the pose errors."""
+    tracked from a prior perturbed by a seeded uniform <= 1 deg / 1 cm: Gauss-Newton on
+    the device over an image pyramid (vsg.PoseTracker: 3 iterations at 1/2 resolution,
+    2 at full; projective window, smoothed Jacobian, noise-weighted residuals, standard
+    trilinear sampling, one pose read-back per level), then integrated at the refined
+    pose and raycast there (the reference's sampling).
+
+    The pose that is integrated comes from tracking against a companion model of the same
+    frames (voxel 5 mm, truncation 8 cm: wider than the sensor noise, 1-2.6 cm at 2-4 m),
+    integrated at the same refined poses.  The same prior is also refined against the
+    2 mm / 1 cm model itself, whose result is reported beside it and not used.  With
+    stream_radius > 0, every 25 frames the 2 mm scene's blocks beyond that distance from
+    the camera go to a VoxelStreamingManager chunk grid (1 m chunks) and the chunks
+    within it come back.  Per-frame times of the stages (a synchronisation after each)
+    and the pose errors; with trace_path, one line per frame goes there."""
     wl = synth.WORKLOADS[4]
     sc = synth.SynthScene(4, nframes=nframes)
     fx, fy, cx, cy = sc.intrinsics
@@ -296,88 +316,105 @@ def large_run_bench(torch, vsg, synth, nframes: int, stream_radius: float = 0.0)
     stream = torch.cuda.current_stream().cuda_stream
     scene.set_stream(stream)
     mgr = None
-    t_stream, max_blocks, n_stream = 0.0, 0, 0
+    t_stream, max_blocks, n_stream = [], 0, 0
     if stream_radius > 0:
         mgr = vsg.VoxelStreamingManager(scene, scene.m_params, (1.0, 1.0, 1.0), (64, 64, 64), (-32, -32, -32))
     track_voxel, track_trunc = 0.005, 0.08
     tscene = vsg.VoxelScene(vsg.Params(1 << 20, 1 << 20, track_voxel, track_trunc), "fp16")
     tscene.set_stream(stream)
-    tracker = vsg.RayCastRender()
-    assert tracker.initialize(sc.W, sc.H)
-    tracker.set_sampling("trilinear")
-    tracker.set_gn_noise(0.0012, 0.0019, 0.4)
+    trackers = [vsg.PoseTracker(sc.W, sc.H) for _ in range(2)]
     rr = vsg.RayCastRender()
     assert rr.initialize(sc.W, sc.H)
     rng = np.random.default_rng(5)
-    t_int = t_trk = t_ray = t_tint = 0.0
-    prior, refined = [], []
+    ms = {k: [] for k in ("track", "track_2mm_model", "integrate", "integrate_tracking_model", "raycast")}
+    prior, refined, refined2 = [], [], []
+    trace = open(trace_path, "w") if trace_path else None
     for i in range(nframes):
         f = sc.render(i)
         Tgt = f["pose"].astype(np.float64)
         p, c, n = (torch.from_numpy(f[k]).to(dev) for k in ("points", "colors", "normals"))
-        obs = torch.from_numpy(f["depth"].reshape(-1)).to(dev)
+        obs = torch.from_numpy(f["depth"]).to(dev)
         mn, mx = synth.depth_range(f["depth"])
         torch.cuda.synchronize()
         T = Tgt
-        t0 = time.perf_counter()
         if i > 0:
             axis = rng.normal(size=3)
             axis /= np.linalg.norm(axis)
             tdir = rng.normal(size=3)
             tdir /= np.linalg.norm(tdir)
-            T = Tgt @ se3_exp(np.concatenate([axis * np.radians(rng.uniform(0, 1.0)), tdir * rng.uniform(0, 0.01)]))
-            prior.append(_pose_err(Tgt, T))
-            for it in range(10):
-                res = 0.2 if it < 3 else (0.08 if it < 6 else 0.03)
-                T, _ = tracker.gn_step(tscene, obs, fx, fy, cx, cy, mn, mx, T, max_residual=res, lam=1e-3)
+            T0 = Tgt @ se3_exp(np.concatenate([axis * np.radians(rng.uniform(0, 1.0)), tdir * rng.uniform(0, 0.01)]))
+            prior.append(_pose_err(Tgt, T0))
+            t0 = time.perf_counter()
+            T, st = trackers[0].refine(tscene, obs, fx, fy, cx, cy, mn, mx, T0, lam=1e-3)
+            t1 = time.perf_counter()
+            T2, st2 = trackers[1].refine(scene, obs, fx, fy, cx, cy, mn, mx, T0, lam=1e-3)
+            t2 = time.perf_counter()
+            ms["track"].append((t1 - t0) * 1e3)
+            ms["track_2mm_model"].append((t2 - t1) * 1e3)
             refined.append(_pose_err(Tgt, T))
-        t1 = time.perf_counter()
+            refined2.append(_pose_err(Tgt, T2))
+            if trace:
+                trace.write(f"{i} prior {prior[-1][0]:.4f} {prior[-1][1]:.3f} track {refined[-1][0]:.4f} "
+                            f"{refined[-1][1]:.3f} n {st['count']} track2mm {refined2[-1][0]:.4f} "
+                            f"{refined2[-1][1]:.3f} n {st2['count']}\n")
         Tf = T.astype(np.float32)
+        t0 = time.perf_counter()
         scene.integrateFromScanData(p, c, n, sc.W, sc.H, wl.trunc, Tf, fx, fy, cx, cy)
         torch.cuda.synchronize()
-        t2 = time.perf_counter()
+        t1 = time.perf_counter()
         tscene.integrateFromScanData(p, c, n, sc.W, sc.H, track_trunc, Tf, fx, fy, cx, cy)
         torch.cuda.synchronize()
-        t_tint += time.perf_counter() - t2
         t2 = time.perf_counter()
         rr.render(scene, Tf[:3, 3], Tf, fx, fy, cx, cy, mn, mx)
         torch.cuda.synchronize()
         t3 = time.perf_counter()
+        ms["integrate"].append((t1 - t0) * 1e3)
+        ms["integrate_tracking_model"].append((t2 - t1) * 1e3)
+        ms["raycast"].append((t3 - t2) * 1e3)
         if mgr is not None and i % 25 == 24:
             max_blocks = max(max_blocks, scene.counters()["num_allocated"])
             ts0 = time.perf_counter()
             mgr.streamOut(scene, Tf[:3, 3], stream_radius)
             mgr.streamIn(scene, Tf[:3, 3], stream_radius)
-            t_stream += time.perf_counter() - ts0
-            n_stream += 1
-        t_trk += t1 - t0
-        t_int += t2 - t1
-        t_ray += t3 - t2
+            t_stream.append((time.perf_counter() - ts0) * 1e3)
+        if i % 100 == 99:
+            log(f"large run: frame {i + 1}/{nframes}, track {np.mean(ms['track']):.2f} ms, "
+                f"refined median {np.median(np.array(refined)[:, 1]):.3f} mm")
+    if trace:
+        trace.close()
     blocks = scene.counters()["num_allocated"]
     tblocks = tscene.counters()["num_allocated"]
+    streaming = None
+    if mgr is not None:
+        keys = [c.keys for c in mgr.m_grid.values()]
+        images = int(mgr.numStreamedBlocks())
+        distinct = int(len(np.unique(np.concatenate(keys)))) if keys else 0
+        streaming = {"radius": stream_radius, "every_frames": 25, "max_blocks_before": int(max_blocks),
+                     "images_in_chunk_grid": images, "distinct_blocks_in_chunk_grid": distinct,
+                     "dropped_outside_grid": int(mgr.droppedOutsideGrid),
+                     "round_trip_ms": _stage(t_stream)}
     scene.close()
     tscene.close()
-    tracker.close()
+    for t in trackers:
+        t.close()
     rr.close()
-    P, R = np.array(prior), np.array(refined)
-    log(f"large run: {nframes} frames, refined pose median {np.median(R[:, 0]):.3f} deg {np.median(R[:, 1]):.2f} mm")
+    P, R, R2 = np.array(prior), np.array(refined), np.array(refined2)
+    per_frame = {k: _stage(v) for k, v in ms.items()}
+    used = np.array(ms["track"][:]) + np.array(ms["integrate"][1:]) + np.array(ms["integrate_tracking_model"][1:]) \
+        + np.array(ms["raycast"][1:])
+    log(f"large run: {nframes} frames, tracking-model pose median {np.median(R[:, 0]):.3f} deg "
+        f"{np.median(R[:, 1]):.2f} mm, 2 mm model {np.median(R2[:, 0]):.3f} deg {np.median(R2[:, 1]):.2f} mm")
     return {"workload": "room 1024x768, voxel 2 mm, trunc 0.01 m, 2^22 buckets, FP16 store", "frames": nframes,
             "blocks": int(blocks),
             "tracking_model": {"voxel": track_voxel, "trunc": track_trunc, "blocks": int(tblocks)},
-            **({"streaming": {"radius": stream_radius, "every_frames": 25, "max_blocks_before": int(max_blocks),
-                              "blocks_in_chunk_grid": int(mgr.numStreamedBlocks()),
-                              "dropped_outside_grid": int(mgr.droppedOutsideGrid),
-                              "ms_per_round_trip": round(t_stream / max(n_stream, 1) * 1e3, 2)}} if mgr else {}),
-            "ms_per_frame": {"track_10_iterations": round(t_trk / max(nframes - 1, 1) * 1e3, 3),
-                             "integrate": round(t_int / nframes * 1e3, 3), "raycast": round(t_ray / nframes * 1e3, 3),
-                             "integrate_tracking_model": round(t_tint / nframes * 1e3, 3)},
-            "prior_error_median": {"deg": round(float(np.median(P[:, 0])), 3), "mm": round(float(np.median(P[:, 1])), 2)},
-            "refined_error_median": {"deg": round(float(np.median(R[:, 0])), 3),
-                                     "mm": round(float(np.median(R[:, 1])), 2)},
-            "final_pose_error": {"deg": round(float(R[-1, 0]), 3), "mm": round(float(R[-1, 1]), 2)},
-            "refined_error_max": {"deg": round(float(R[:, 0].max()), 3), "mm": round(float(R[:, 1].max()), 2)},
-            "within_0.1deg_1mm": int(((R[:, 0] <= 0.1) & (R[:, 1] <= 1.0)).sum()), "tracked_frames": int(len(R)),
-            "note": "wall-clock per stage with a synchronisation after each (host-issued GN iterations)"}
+            **({"streaming": streaming} if streaming else {}),
+            "ms_per_frame": {**per_frame, "track_integrate_raycast": _stage(used)},
+            "gn_iterations": {f"1/{d}" if d > 1 else "full": len(w) for d, w in vsg.PoseTracker.LEVELS},
+            "prior_error": _pose_summary(P),
+            "pose_error_tracking_model": _pose_summary(R),
+            "pose_error_2mm_model": _pose_summary(R2),
+            "note": "wall clock per stage with a synchronisation after each; track_integrate_raycast = track + "
+                    "integrate of both models + raycast (the 2 mm model's tracking is reported, not used)"}
 
 
 def _pose_err(Tgt, T):
@@ -413,8 +450,10 @@ def main() -> int:
                     help="skip timing the depth maps from a point cloud")
     ap.add_argument("--no-pose", action="store_true",
                     help="skip the pose-refinement run (64 held-out frames, 10 iterations each)")
-    ap.add_argument("--large-frames", type=int, default=100,
+    ap.add_argument("--large-frames", type=int, default=1000,
                     help="frames of the 1024x768 tracked run (0: skip; the configuration has 1000)")
+    ap.add_argument("--large-trace", default="",
+                    help="write the large run's per-frame pose errors to this file")
     ap.add_argument("--step-budget", type=int, default=None,
                     help="raycast steps per lane before the cooperative tail pass (library default if unset)")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
@@ -660,7 +699,8 @@ def main() -> int:
         extra["pose_refinement"] = pose_refinement_bench(torch, vsg, synth, sc, frames, wl)
     if args.large_frames > 0:
         log(f"large run: {args.large_frames} frames 1024x768 with tracking")
-        extra["large_run"] = large_run_bench(torch, vsg, synth, args.large_frames, args.large_stream_radius)
+        extra["large_run"] = large_run_bench(torch, vsg, synth, args.large_frames, args.large_stream_radius,
+                                             args.large_trace)
     if args.compare_fp16 and args.store == "fp32":
         r16 = measure("fp16")
         extra["fp16_store"] = {"value": round(frames_done / r16["t_total"], 3),

@@ -431,6 +431,17 @@ vsg_status vsg_render_set_gn_jacobian(vsg_render* render, int32_t mode);
  * reference. */
 vsg_status vsg_render_set_gn_noise(vsg_render* render, float a, float b, float z0);
 
+/* Projective window of the renders of vsg_pose_gn_step / vsg_pose_gn_refine (on != 0;
+ * off by default): each ray marches only over depths within the step's max_residual of
+ * its pixel's observed depth (and within [min_depth, max_depth]), and pixels without an
+ * observed depth are not marched.  A crossing outside the window would fail the
+ * residual test anyway; what changes is that a nearer surface outside the window (an
+ * occluder the full render would return, leaving the pixel out) is not seen, so the
+ * pixel can count with the surface inside the window.  The march then covers a few
+ * centimetres per ray instead of the whole depth range.  No counterpart in the
+ * reference. */
+vsg_status vsg_render_set_gn_window(vsg_render* render, int32_t on);
+
 /* One Gauss-Newton step of frame-to-model pose refinement: renders at pose (camera
  * position = its translation), takes the Jacobian of vsg_render_set_gn_jacobian's mode,
  * accumulates in FP64

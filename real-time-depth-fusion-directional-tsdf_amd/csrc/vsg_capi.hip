@@ -163,6 +163,7 @@ struct vsg_render {
   float* gn_posef = nullptr;  // GN pose for the next render: 16 floats + camera position (3)
   float* cam_buf = nullptr;   // camera position of a render with a device pose (3 floats)
   int32_t gn_jacobian = VSG_GN_SMOOTHED;  // vsg_gn_jacobian of vsg_pose_gn_step
+  bool gn_window = false;                 // vsg_render_set_gn_window
   float gn_noise[3] = {0.0f, 0.0f, 0.0f};  // depth noise model {a, b, z0} of the residual weights (a = 0: none)
   // surface-point extraction: outputs of capacity ext_cap, scratch sized for a scene of
   // ext_buckets buckets and ext_blocks blocks
@@ -1064,6 +1065,12 @@ vsg_status vsg_render_set_gn_jacobian(vsg_render* r, int32_t mode) {
   return VSG_OK;
 }
 
+vsg_status vsg_render_set_gn_window(vsg_render* r, int32_t on) {
+  if (!r) return fail(VSG_ERR_INVALID_ARG, "vsg_render_set_gn_window: null renderer");
+  r->gn_window = on != 0;
+  return VSG_OK;
+}
+
 namespace {
 
 vsg_status gn_scratch(vsg_render* r) {
@@ -1110,12 +1117,18 @@ vsg_status vsg_pose_gn_step(vsg_render* r, vsg_scene* s, const float* observed_d
     if (!std::isfinite(pose[i])) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_step: pose must be finite");
   vsg_status st = gn_scratch(r);
   if (st != VSG_OK) return st;
-  float pf[16], cam[3];
-  for (int i = 0; i < 16; ++i) pf[i] = (float)pose[i];
-  cam[0] = pf[3];
-  cam[1] = pf[7];
-  cam[2] = pf[11];
-  st = vsg_render_frame(r, s, cam, pf, k, min_depth, max_depth, 0.0f);
+  RenderArgs ra;
+  std::memset(&ra, 0, sizeof(ra));
+  for (int i = 0; i < 16; ++i) ra.M[i] = (float)pose[i];
+  ra.cam[0] = ra.M[3];
+  ra.cam[1] = ra.M[7];
+  ra.cam[2] = ra.M[11];
+  if (!all_finite(ra.M, 16)) return fail(VSG_ERR_INVALID_ARG, "vsg_pose_gn_step: pose must be finite in float");
+  if (r->gn_window) {
+    ra.win_obs = observed_depth;
+    ra.win = max_residual;
+  }
+  st = render_launch(r, s, ra, k, min_depth, max_depth, 0.0f);
   if (st == VSG_OK) st = gn_jacobian(r, s);
   if (st != VSG_OK) return st;
   {
@@ -1161,6 +1174,10 @@ vsg_status vsg_pose_gn_refine(vsg_render* r, vsg_scene* s, const float* observed
     } else {  // the previous solve's pose, left on the device
       ra.Mdev = r->gn_posef;
       ra.camdev = r->gn_posef + 16;
+    }
+    if (r->gn_window) {
+      ra.win_obs = observed_depth;
+      ra.win = max_residuals[it];
     }
     st = render_launch(r, s, ra, k, min_depth, max_depth, 0.0f);
     if (st == VSG_OK) st = gn_jacobian(r, s);

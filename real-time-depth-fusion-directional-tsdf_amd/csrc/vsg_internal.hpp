@@ -25,6 +25,8 @@ struct RenderArgs {
   uint32_t budget;        // steps per lane before a ray moves to the cooperative tail pass (0: none)
   void* tail;             // tail queue (TailRay[W*H]) when budget > 0
   uint32_t* tail_ctl;     // tail queue {count, next, workgroups done}, zero between renders
+  const float* win_obs;   // when non-null: observed depth per pixel; a ray marches only over depths within
+  float win;              // win of it (and within [minD, maxD]); a pixel without one (<= 0) is not marched
 };
 
 // A ray whose march continues in the tail pass: its pixel and march state.

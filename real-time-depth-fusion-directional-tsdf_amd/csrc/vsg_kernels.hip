@@ -1078,6 +1078,15 @@ __device__ __forceinline__ bool ray_setup(const RenderArgs& ra, uint32_t x, uint
   sk.inv_v = 1.0f / rs.v;
   const float mag = fmaxf(fmaxf(fabsf(rs.camx), fabsf(rs.camy)), fabsf(rs.camz)) + fabsf(rs.ray_end);
   sk.eps = 0.02f + 8e-6f * mag * sk.inv_v;
+  if (ra.win_obs) {  // projective window of the tracking renders
+    const float o = ra.win_obs[y * (uint32_t)ra.W + x];
+    if (!(o > 0.0f)) {
+      h.marched = false;
+      return false;
+    }
+    rs.ray0 = h.d2r * fmaxf(minD, o - ra.win);
+    rs.ray_end = h.d2r * fminf(maxD, o + ra.win);
+  }
   return true;
 }
 
@@ -1990,17 +1999,41 @@ struct Pose64 {
 // Sums the partials in a fixed order, solves (H + lambda diag(H)) delta = -g by
 // Cholesky and composes pose * exp(delta^), all in FP64.  out: new pose (16),
 // delta (6), cost, count, solved flag.
+// The sum: lane l of the 256 adds rows l, l + 256, ... in turn, then the lanes' sums
+// are added by a fixed butterfly within each wave and the four waves' sums in wave
+// order, so the result does not depend on timing.
 // pose_in (may be null: the pose passed by value) may be `out` itself (the previous
 // iteration's result): lane 0 reads it before writing.  posef (may be null) receives the
 // new pose as 16 floats and the camera position as 3 more, the form a render reads.
-__global__ __launch_bounds__(64) void k_gn_solve(const double* __restrict__ partials, int nwg, double lambda,
-                                                 Pose64 pose, const double* pose_in, double* out,
-                                                 float* __restrict__ posef) {
+constexpr int kGnSolveThreads = 256;
+__global__ __launch_bounds__(kGnSolveThreads) void k_gn_solve(const double* __restrict__ partials, int nwg,
+                                                              double lambda, Pose64 pose, const double* pose_in,
+                                                              double* out, float* __restrict__ posef) {
+  __shared__ double wsum[kGnSolveThreads / 64][kGnTerms];
   __shared__ double sum[kGnTerms];
+  {
+    double acc[kGnTerms];
+#pragma unroll
+    for (int k = 0; k < kGnTerms; ++k) acc[k] = 0.0;
+    for (int b = threadIdx.x; b < nwg; b += kGnSolveThreads) {
+      const double* row = partials + (size_t)b * kGnTerms;
+#pragma unroll
+      for (int k = 0; k < kGnTerms; ++k) acc[k] += row[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kGnTerms; ++k) {
+      double v = acc[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += shfl_xor_d(v, o);
+      if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6][k] = v;
+    }
+  }
+  __syncthreads();
   if (threadIdx.x < kGnTerms) {
-    double s = 0.0;
-    for (int b = 0; b < nwg; ++b) s += partials[(size_t)b * kGnTerms + threadIdx.x];
-    sum[threadIdx.x] = s;
+    double v = wsum[0][threadIdx.x];
+#pragma unroll
+    for (int w = 1; w < kGnSolveThreads / 64; ++w) v += wsum[w][threadIdx.x];
+    sum[threadIdx.x] = v;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
@@ -3111,7 +3144,8 @@ void launch_gn(hipStream_t s, const float* depth, const float* obs, const float*
   hipLaunchKernelGGL(k_gn_accum, dim3(nwg), dim3(256), 0, s, depth, obs, jac, W, H, max_res, nz, partials);
   Pose64 p;
   for (int i = 0; i < 16; ++i) p.T[i] = pose ? pose[i] : 0.0;
-  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(64), 0, s, partials, (int)nwg, lambda, p, pose_in, out, posef);
+  hipLaunchKernelGGL(k_gn_solve, dim3(1), dim3(kGnSolveThreads), 0, s, partials, (int)nwg, lambda, p, pose_in, out,
+                     posef);
 }
 
 uint32_t tail_workgroups(int32_t W, int32_t H) {

@@ -133,6 +133,7 @@ _SIGNATURES = {
     "vsg_render_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "vsg_render_set_gn_jacobian": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsg_render_set_gn_noise": (C.c_int, [C.c_void_p, C.c_float, C.c_float, C.c_float]),
+    "vsg_render_set_gn_window": (C.c_int, [C.c_void_p, C.c_int32]),
     "vsg_pose_gn_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
                                    C.c_float, C.c_float, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "vsg_pose_gn_refine": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(vsg_intrinsics), C.c_float,
@@ -724,6 +725,11 @@ class RayCastRender:
         depth (a = 0: uniform)."""
         _check(lib().vsg_render_set_gn_noise(self._h, float(a), float(b), float(z0)))
 
+    def set_gn_window(self, on: bool = True) -> None:
+        """Projective window of the Gauss-Newton renders: each ray marches only within the
+        step's max_residual of its observed depth (pixels without one are not marched)."""
+        _check(lib().vsg_render_set_gn_window(self._h, 1 if on else 0))
+
     def gn_step(self, voxelScene: VoxelScene, observed_depth, fx, fy, cx, cy, minDepth, maxDepth, pose,
                 max_residual: float = 0.05, lam: float = 1e-4):
         """One FP64 Gauss-Newton step on the device; returns (new pose float64 4x4, stats dict)."""
@@ -795,6 +801,69 @@ class RayCastRender:
 
     def isAllocated(self) -> bool:
         return self.m_isAllocated
+
+
+class PoseTracker:
+    """Frame-to-model pose refinement over an image pyramid (no counterpart in the
+    reference): Gauss-Newton on the device (RayCastRender.gn_refine) at reduced, then
+    full resolution, with the projective window on (RayCastRender.set_gn_window: each
+    ray marches only within the iteration's residual window of its observed depth).
+    Level d renders every d-th row and column: its image is ceil(W/d) x ceil(H/d) with
+    intrinsics fx/d, fy/d, cx/d, cy/d, so its pixel (x, y) is the full image's (d x, d y)
+    exactly, and the observed depth is the same subsample.
+
+    levels: (d, residual windows) pairs run in order, one Gauss-Newton iteration per
+    window; d a power of two, so that the level's rays are bit for bit the full image's
+    ((x' - cx/d) / (fx/d) rounds as (d x' - cx) / fx).  The default (3 at 1/2 with 20, 8
+    and 8 cm, 2 at full with 3 cm) reaches the accuracy of ten full-resolution iterations
+    with windows 20, 20, 20, 8, 8, 8, 3, 3, 3, 3 cm (median and maximum pose error within
+    0.0005 deg / 0.005 mm over 30 frames of the 1024x768 room sequence) in 2.5 ms instead
+    of 6.6 ms (24.5 ms without the window).
+
+    The subsampled depths are made on torch's current stream: use the scene's stream
+    there (VoxelScene.set_stream) or synchronise the two."""
+
+    LEVELS = ((2, (0.2, 0.08, 0.08)), (1, (0.03, 0.03)))
+
+    def __init__(self, width: int, height: int, levels=LEVELS, sampling: str = "trilinear",
+                 jacobian: str = "smoothed", noise=(0.0012, 0.0019, 0.4), window: bool = True):
+        self.width, self.height = int(width), int(height)
+        self.levels = tuple((int(d), tuple(float(r) for r in rs)) for d, rs in levels)
+        self.renders = {}
+        for d, _ in self.levels:
+            if d < 1 or d & (d - 1):
+                raise ValueError("pyramid level divisors must be powers of two")
+            if d in self.renders:
+                continue
+            r = RayCastRender()
+            if not r.initialize(-(-self.width // d), -(-self.height // d)):
+                raise VSGError(2, "PoseTracker: renderer allocation failed")
+            r.set_sampling(sampling)
+            r.set_gn_jacobian(jacobian)
+            if noise is not None:
+                r.set_gn_noise(*noise)
+            r.set_gn_window(window)
+            self.renders[d] = r
+
+    def refine(self, voxelScene: VoxelScene, observed_depth, fx: float, fy: float, cx: float, cy: float,
+               minDepth: float, maxDepth: float, pose, lam: float = 1e-3):
+        """observed_depth: device float32 tensor of H*W (or H x W) depths of the frame.
+        Returns (refined pose float64 4x4, stats of the last iteration)."""
+        obs = observed_depth.reshape(self.height, self.width)
+        T = np.asarray(pose, np.float64)
+        st = {"cost": 0.0, "count": 0, "solved": False, "rot": 0.0, "trans": 0.0}
+        for d, windows in self.levels:
+            if not windows:
+                continue
+            o = obs if d == 1 else obs[::d, ::d].contiguous()
+            T, st = self.renders[d].gn_refine(voxelScene, o.reshape(-1), fx / d, fy / d, cx / d, cy / d, minDepth,
+                                              maxDepth, T, windows, lam=lam)
+        return T, st
+
+    def close(self) -> None:
+        for r in self.renders.values():
+            r.close()
+        self.renders = {}
 
 
 # ---- marching cubes (VoxelSG/include/MarchingCubes.h, VoxelSG/src/MarchingCubes.cu) ------------
