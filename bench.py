@@ -65,36 +65,91 @@ def frame_bytes(st: dict, npix: int, b_sw: int) -> int:
             + st["updated_voxels"] * (2 * b_sw + 8) + 512 * b_sw * st["ray_blocks"])
 
 
-def cpu_baseline(sc, frames: list[dict], wl, nframes: int, stride: int) -> dict:
-    """The oracle (float, literal reference algorithm: every allocated block integrated,
-    10-slot bucket probes, half-voxel ray steps) on the first `nframes` frames: full
-    integrate, raycast of every `stride`-th row and column scaled by stride^2."""
+def host_info() -> dict:
+    """Cores this process may use, the CPU model and its clock (from /proc/cpuinfo)."""
+    info = {"nproc": len(os.sched_getaffinity(0)), "cpu_model": None, "mhz": None}
+    try:
+        for line in open("/proc/cpuinfo"):
+            k, _, v = line.partition(":")
+            k = k.strip()
+            if k == "model name" and info["cpu_model"] is None:
+                info["cpu_model"] = v.strip()
+            elif k == "cpu MHz" and info["mhz"] is None:
+                info["mhz"] = float(v)
+    except OSError:
+        pass
+    return info
+
+
+def cpu_run(synth, config: int, threads: int, frames: int, raycast_frames, progress: bool = False) -> dict:
+    """The oracle (float, the reference's algorithm: allocation in canonical order,
+    compaction by a full table scan, every allocated block integrated, 10-slot bucket
+    probes on every lookup, a ray march every half voxel with no empty-space skipping) on
+    frames 0..frames-1 of a configuration's sequence with `threads` OpenMP threads.
+    Every frame is integrated, its allocation, compaction and integration timed apart;
+    the frames in raycast_frames are then raycast over the whole image at their pose
+    (depth range by the driver's rule, timed with the raycast).  Frame synthesis is
+    outside the timed stages."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
+    used = O.set_threads(threads)
+    wl = synth.WORKLOADS[config]
+    sc = synth.SynthScene(config)
     o = O.OracleScene(wl.buckets, wl.blocks, wl.voxel)
-    W, H = sc.W, sc.H
-    ys, xs = np.meshgrid(np.arange(0, H, stride), np.arange(0, W, stride), indexing="ij")
-    pix = (ys * W + xs).reshape(-1).astype(np.int32)
-    t_int = t_ray = 0.0
-    for i in range(nframes):
-        f = frames[i]
-        t0 = time.perf_counter()
-        o.integrate(f["points"], f["colors"], wl.trunc, f["pose"], sc.intrinsics)
-        t1 = time.perf_counter()
-        mn, mx = f["range"]
-        T = f["pose"].astype(np.float64)
-        o.render(W, H, T[:3, 3], T, sc.intrinsics, mn, mx, pixels=pix)
-        t2 = time.perf_counter()
-        t_int += t1 - t0
-        t_ray += t2 - t1
-    scale = (W * H) / len(pix)
-    per_frame = (t_int + t_ray * scale) / nframes
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(1.0 / per_frame, 4), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle (float) on frames 0-{nframes - 1} of the 300-frame room 640x480 sequence: "
-                       f"integrate in full ({t_int / nframes * 1e3:.0f} ms/frame), raycast of every "
-                       f"{stride}th row and column scaled x{scale:.0f} ({t_ray * scale / nframes * 1e3:.0f} ms/frame)"),
-            "integrate_ms": round(t_int / nframes * 1e3, 2), "raycast_ms": round(t_ray * scale / nframes * 1e3, 2)}
+    st = {"alloc": [], "compact": [], "integrate": [], "raycast": []}
+    ctr = {"num_allocated": [], "visible": [], "updated": [], "hits": [], "misses": []}
+    rset = set(int(i) for i in raycast_frames)
+    for i in range(frames):
+        f = sc.render(i)
+        c = o.integrate(f["points"], f["colors"], wl.trunc, f["pose"], sc.intrinsics)
+        a, k, g = o.stage_times()
+        st["alloc"].append(a * 1e3)
+        st["compact"].append(k * 1e3)
+        st["integrate"].append(g * 1e3)
+        for n in ("num_allocated", "visible", "updated"):
+            ctr[n].append(c[n])
+        if i in rset:
+            T = f["pose"].astype(np.float64)
+            t0 = time.perf_counter()
+            mn, mx = synth.depth_range(f["depth"])
+            r = o.render(sc.W, sc.H, T[:3, 3], T, sc.intrinsics, mn, mx)
+            st["raycast"].append((time.perf_counter() - t0) * 1e3)
+            ctr["hits"].append(r["hits"])
+            ctr["misses"].append(r["misses"])
+        if progress:
+            print(f"[cpu] config {config} threads {used} frame {i}: alloc {a * 1e3:.2f} compact {k * 1e3:.2f} "
+                  f"integrate {g * 1e3:.2f} ms" + (f" raycast {st['raycast'][-1]:.0f} ms" if i in rset else ""),
+                  file=sys.stderr, flush=True)
+    summary = {k: {"mean": round(float(np.mean(v)), 3), "p95": round(float(np.quantile(v, 0.95)), 3)}
+               for k, v in st.items() if v}
+    frame_ms = sum(summary[k]["mean"] for k in ("alloc", "compact", "integrate")) + summary["raycast"]["mean"]
+    return {"config": config, "workload": wl.name, "threads": used, "frames": frames,
+            "raycast_frames": sorted(rset), "frames_per_s": round(1e3 / frame_ms, 5), "ms": summary,
+            "counters_mean": {k: int(np.mean(v)) for k, v in ctr.items() if v},
+            "per_frame_ms": {k: [round(x, 3) for x in v] for k, v in st.items()}}
+
+
+def cpu_baseline(synth) -> dict:
+    """The oracle timed on the host's cores, bounded to about 10-20 s: the 320x240 sphere
+    sequence (30 frames integrated, full-image raycasts after frames 9, 19 and 29) and
+    the first 30 frames of the 640x480 room sequence (integrated, full-image raycasts
+    after frames 14 and 29), all cores.  value = the room sample's frames/s (mean
+    allocation + compaction + integration + raycast per frame).  The full table (both
+    sequences at 1 thread and all cores, a raycast after every frame) comes from
+    bench.py --cpu-plan runs."""
+    host = host_info()
+    n = host["nproc"]
+    c0 = cpu_run(synth, 0, n, 30, (9, 19, 29))
+    c1 = cpu_run(synth, 1, n, 30, (14, 29))
+    for r in (c0, c1):
+        r.pop("per_frame_ms")
+    return {"value": c1["frames_per_s"], "unit": "frames/s", "cores": c1["threads"], "kind": "port",
+            "sample": ("oracle (float, the reference's algorithm) on frames 0-29 of the 640x480 room sequence, "
+                       "every frame integrated, full-image raycasts after frames 14 and 29; per-stage ms below; "
+                       "also the 320x240 sphere sequence (30 frames, raycasts after 9, 19, 29)"),
+            "host": host, "room_640x480": c1, "sphere_320x240": c0,
+            "plan": ("profiles/cpu_baseline.md: both sequences at 1 thread and all cores, a raycast after every "
+                     "frame (bench.py --cpu-plan)")}
 
 
 def error_stats(err: np.ndarray) -> dict:
@@ -433,9 +488,11 @@ def main() -> int:
     ap.add_argument("--compare-fp16", action="store_true", default=True,
                     help="also time the FP16 store (reported beside the FP32 headline)")
     ap.add_argument("--no-compare-fp16", dest="compare_fp16", action="store_false")
-    ap.add_argument("--cpu-frames", type=int, default=40)
-    ap.add_argument("--cpu-stride", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-plan", default="",
+                    help="CONFIG:THREADS:FIRST-LAST (THREADS 0 = all cores): only run the oracle on frames "
+                         "0..LAST of configuration CONFIG (0: 320x240 sphere, 1: 640x480 room), raycasting after "
+                         "each of frames FIRST..LAST, and print its per-stage times as one JSON line (no GPU)")
     ap.add_argument("--precision-frames", type=int, default=30)
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-extract", action="store_true",
@@ -461,6 +518,15 @@ def main() -> int:
                          "its FETCH_SIZE / WRITE_SIZE under rocprofv3 --pmc scale the counters of the "
                          "fusion kernels to bytes")
     args = ap.parse_args()
+    if args.cpu_plan:
+        load_pkg()
+        from vsg_amd import synth
+        cfg, thr, rng = args.cpu_plan.split(":")
+        first, last = (int(v) for v in rng.split("-"))
+        thr = int(thr) or host_info()["nproc"]
+        res = cpu_run(synth, int(cfg), thr, last + 1, range(first, last + 1), progress=True)
+        print(json.dumps({"cpu_plan": args.cpu_plan, "host": host_info(), **res}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path is single-GPU: "
               f"each GPU would own a share of the hash buckets)", file=sys.stderr)
@@ -716,8 +782,8 @@ def main() -> int:
                                                       args.precision_frames, 20000)
     cpu = None
     if not args.no_cpu_baseline:
-        log(f"cpu baseline: oracle on {args.cpu_frames} frames")
-        cpu = cpu_baseline(sc, frames, wl, args.cpu_frames, args.cpu_stride)
+        log("cpu baseline: oracle on 30 room frames and 30 sphere frames, all cores")
+        cpu = cpu_baseline(synth)
     out = {
         "metric": "frames/s 640x480 integrate+raycast, % HBM roofline; fp16 vs fp32 at ≤0.5mm err",
         "value": round(fps, 3),

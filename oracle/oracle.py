@@ -38,6 +38,9 @@ def lib() -> C.CDLL:
         L.vsgo_integrate.argtypes = [vp, vp, vp, vp, C.c_int, C.c_int, C.c_double, vp, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_int, vp]
         L.vsgo_set_directional.argtypes = [vp, C.c_int]
+        L.vsgo_stage_times.argtypes = [vp, vp]
+        L.vsgo_set_threads.restype = C.c_int
+        L.vsgo_set_threads.argtypes = [C.c_int]
         L.vsgo_export_directional.argtypes = [vp, C.c_uint64, vp, vp, vp]
         L.vsgo_alloc.argtypes = [vp, vp, C.c_int, C.c_int, C.c_double, vp]
         L.vsgo_render.restype = C.c_int
@@ -84,6 +87,12 @@ def lib() -> C.CDLL:
         L.vsgo_stream_in.argtypes = [vp, C.c_uint64, vp, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
+
+
+def set_threads(n: int = 0) -> int:
+    """OpenMP threads of the oracle's parallel loops (n <= 0: unchanged); returns the
+    number in effect."""
+    return int(lib().vsgo_set_threads(int(n)))
 
 
 def _f32(a) -> np.ndarray:
@@ -134,6 +143,12 @@ class OracleScene:
                              H, float(trunc), _pose(pose).ctypes.data, fx, fy, cx, cy, int(visible_only),
                              ctr.ctypes.data)
         return dict(zip(COUNTER_NAMES, (int(v) for v in ctr)))
+
+    def stage_times(self) -> tuple[float, float, float]:
+        """Seconds of the last integrate's allocation, compaction and integration."""
+        out = np.zeros(3, np.float64)
+        lib().vsgo_stage_times(self._h, out.ctypes.data)
+        return float(out[0]), float(out[1]), float(out[2])
 
     def export_directional(self, nblocks: int) -> dict:
         """Direction slots of the first nblocks blocks: id, weight, dtsdf, each [nblocks*512, 2]."""

@@ -10,6 +10,8 @@
 #include <utility>
 #include <vector>
 
+#include <omp.h>
+
 #include "vsg_oracle.hpp"
 
 using namespace vsgo;
@@ -116,6 +118,27 @@ void vsgo_integrate(void* p, const float* pts, const uint8_t* rgb, const float* 
   Handle* h = (Handle*)p;
   if (h->fp64) do_integrate(h->sd, pts, rgb, nrm, W, H, trunc, T16, fx, fy, cx, cy, visible_only, counters);
   else do_integrate(h->sf, pts, rgb, nrm, W, H, trunc, T16, fx, fy, cx, cy, visible_only, counters);
+}
+
+// Seconds of the last integrate's allocation, compaction and integration (out3).
+void vsgo_stage_times(void* p, double* out3) {
+  Handle* h = (Handle*)p;
+  if (h->fp64) {
+    out3[0] = h->sd.t_alloc;
+    out3[1] = h->sd.t_compact;
+    out3[2] = h->sd.t_integrate;
+  } else {
+    out3[0] = h->sf.t_alloc;
+    out3[1] = h->sf.t_compact;
+    out3[2] = h->sf.t_integrate;
+  }
+}
+
+// OpenMP threads of the parallel loops (integration over blocks, raycast over pixels);
+// returns the number in effect.
+int vsgo_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
 }
 
 // Directional extension on (before the first frame) or off.

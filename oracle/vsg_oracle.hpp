@@ -40,6 +40,7 @@
 // hold the SDF as FP32 or as FP16 (with FP32/FP64 arithmetic around it).
 #pragma once
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -185,6 +186,8 @@ template <class R> struct Scene {
   // per-frame counters
   // c_heap_fail: buckets whose insertion failed for lack of blocks in the last frame
   uint32_t c_valid_pixels = 0, c_new_blocks = 0, c_heap_fail = 0, c_updated = 0, c_visible = 0;
+  // wall-clock seconds of the last integrate's stages: allocation, compaction, integration
+  double t_alloc = 0.0, t_compact = 0.0, t_integrate = 0.0;
 
   void init(uint32_t B, uint32_t N, R v, int fp16) {
     num_buckets = B;
@@ -612,11 +615,15 @@ inline void integrate_block(Scene<R>& s, I3 bc, int32_t ptr, const float* pts, c
 template <class R>
 inline void integrate_frame(Scene<R>& s, const float* pts, const uint8_t* rgb, int W, int H, R trunc, const R* M,
                             R fx, R fy, R cx, R cy, int visible_only, const float* nrm = nullptr) {
+  using clock = std::chrono::steady_clock;
+  const auto t0 = clock::now();
   alloc_frame(s, pts, W, H, trunc, M);
+  const auto t1 = clock::now();
   // compaction: every live entry, in table order
   std::vector<std::pair<I3, int32_t>> list;
   for (size_t i = 0; i < s.slot_ptr.size(); ++i)
     if (s.slot_ptr[i] != -1) list.push_back({s.slot_pos[i], s.slot_ptr[i]});
+  const auto t2 = clock::now();
   R zmin = R(1e30f), zmax = R(-1e30f);
   bool finite = true;
   if (visible_only) {
@@ -643,6 +650,10 @@ inline void integrate_frame(Scene<R>& s, const float* pts, const uint8_t* rgb, i
   s.c_updated = updated;
   s.c_visible = visible;
   s.frames++;
+  const auto t3 = clock::now();
+  s.t_alloc = std::chrono::duration<double>(t1 - t0).count();
+  s.t_compact = std::chrono::duration<double>(t2 - t1).count();
+  s.t_integrate = std::chrono::duration<double>(t3 - t2).count();
 }
 
 // ---- A13: getVoxel (RayCastRender.cu:55-121): weight and sdf of the voxel at p.

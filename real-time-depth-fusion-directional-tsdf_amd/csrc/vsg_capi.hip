@@ -150,7 +150,7 @@ struct vsg_render {
   size_t rc_march_words = 0;     // the march's slice of rc (the tail pass's follows it)
   uint32_t last_budget = 0;      // step budget of the last render (0: the tail pass did not run)
   void* tail = nullptr;          // tail queue: TailRay per pixel
-  uint32_t* tail_ctl = nullptr;  // {count, next, workgroups done}
+  uint32_t* tail_ctl = nullptr;  // kTailCtlWords queue controls of the tail pass
   uint32_t budget = 0;           // march steps per lane before the tail pass takes the ray over
   uint32_t calls = 0;
   hipEvent_t ev = nullptr;
@@ -741,15 +741,15 @@ vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out) {
   if (st == VSG_OK) st = dmalloc(&r->col, n * sizeof(float4), "colour output");
   if (st == VSG_OK) st = dmalloc(&r->nrm, n * sizeof(float4), "normal output");
   const size_t nwg = (size_t)((width + 7) / 8) * ((height + 7) / 8);
-  r->rc_words = 2 * (nwg + tail_workgroups(width, height));
   r->rc_march_words = 2 * nwg;
+  r->rc_words = 2 * (nwg + tail_workgroups(width, height));
   if (st == VSG_OK) st = dmalloc(&r->rc, r->rc_words * sizeof(uint32_t), "render counters");
   if (st == VSG_OK && hipMemset(r->rc, 0, r->rc_words * sizeof(uint32_t)) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipMemset");
   if (st == VSG_OK) st = dmalloc(&r->tail, n * sizeof(TailRay), "tail queue");
-  if (st == VSG_OK) st = dmalloc(&r->tail_ctl, 4 * sizeof(uint32_t), "tail queue controls");
+  if (st == VSG_OK) st = dmalloc(&r->tail_ctl, kTailCtlWords * sizeof(uint32_t), "tail queue controls");
   if (st == VSG_OK) st = dmalloc(&r->cam_buf, 4 * sizeof(float), "camera position");
-  if (st == VSG_OK && hipMemset(r->tail_ctl, 0, 4 * sizeof(uint32_t)) != hipSuccess)
+  if (st == VSG_OK && hipMemset(r->tail_ctl, 0, kTailCtlWords * sizeof(uint32_t)) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipMemset");
   r->budget = kDefaultStepBudget;
   if (st == VSG_OK && hipEventCreateWithFlags(&r->ev, hipEventDisableTiming) != hipSuccess)

@@ -24,7 +24,7 @@ struct RenderArgs {
   int32_t mode;           // vsg_sampling: 0 the reference's trilinear sampling, 1 standard trilinear
   uint32_t budget;        // steps per lane before a ray moves to the cooperative tail pass (0: none)
   void* tail;             // tail queue (TailRay[W*H]) when budget > 0
-  uint32_t* tail_ctl;     // tail queue {count, next, workgroups done}, zero between renders
+  uint32_t* tail_ctl;     // queue controls (kTailCtlWords), counts zero between renders
   const float* win_obs;   // when non-null: observed depth per pixel; a ray marches only over depths within
   float win;              // win of it (and within [minD, maxD]); a pixel without one (<= 0) is not marched
 };
@@ -37,6 +37,9 @@ struct TailRay {
   uint32_t pad[3];
 };
 
+// Queue control words of a renderer's tail pass: {count, next, workgroups done}.
+constexpr uint32_t kTailCtlWords = 4;
+
 // Counters of the march statistics (diagnostics).
 constexpr uint32_t kMarchStats = 8;
 
@@ -48,6 +51,9 @@ constexpr uint32_t kDefaultStepBudget = 48;
 
 // Workgroups of the tail pass for a W x H render.
 uint32_t tail_workgroups(int32_t W, int32_t H);
+
+// Workgroups of the march (and of the fused raycast) for a W x H render.
+uint32_t render_grid(int32_t W, int32_t H);
 
 void launch_reset_voxels(hipStream_t s, DevState* st, void* vox, uint32_t word_bytes, uint32_t* col,
                          uint32_t* dmeta, float2* dval, uint32_t* pcnt);

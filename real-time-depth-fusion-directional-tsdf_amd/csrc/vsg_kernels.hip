@@ -1439,6 +1439,182 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // for the next render.
 constexpr int kTailGroup = 16;
 
+// March state of a ray in the tail pass, one per group of kTailGroup lanes (the same in
+// every lane of the group).
+struct TailState {
+  RayHit h;
+  RaySetup rs;
+  uint32_t pix;
+  float ray, last_sdf, last_alpha;
+  uint32_t last_w;
+};
+
+// Diagnostic counts of the tail pass (lane gl == 0 of each group counts).
+struct TailStats {
+  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_window = 0, n_tskip = 0;
+};
+
+// One iteration of the tail pass on the group's ray: a skip over samples the occupancy
+// maps prove invalid, or a window of kTailGroup consecutive samples (lane gl takes the
+// sample gl additions of the step ahead) resolved in order; the first sign change that
+// passes the thresholds is refined.  Returns true when the ray is finished (s.h.found:
+// whether it hit).  Every lane of the group takes part.
+template <bool F16, bool DIAG, bool REF, class Cache>
+__device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
+                                               const uint32_t* __restrict__ col, Cache& bc, OccCache& oc,
+                                               TailState& s, int gl, int gbase, TailStats& ts) {
+  constexpr int G = kTailGroup;
+  RayHit& h = s.h;
+  const RaySetup& rs = s.rs;
+  float& ray = s.ray;
+  float& last_sdf = s.last_sdf;
+  float& last_alpha = s.last_alpha;
+  uint32_t& last_w = s.last_w;
+  bool done = false;
+  bool window = true;
+  {
+    float t0, t1;
+    const int occ = skip_test(t, oc, rs.sk, h.wdx, h.wdy, h.wdz, ray, &t0, &t1);
+    const bool inside = t0 <= ray && ray < t1;
+    if (occ != 2 && inside) {
+      if (gl == 0) {
+        if (occ == 0) ts.n_super++;
+        else ts.n_block++;
+        ts.n_tskip++;
+      }
+      last_w = 0u;
+      bool stl;
+      const float from = ray;
+      ray = replay_steps(ray, fminf(t1, rs.ray_end), rs.inc, &stl);
+      if (DIAG && gl == 0) ts.n_replay += (uint32_t)((ray - from) / rs.inc + 0.5f);
+      if (stl || !(ray < rs.ray_end)) done = true;
+      window = false;
+    }
+  }
+  if (window) {
+    if (gl == 0) ts.n_window++;
+    // sample gl of the window: gl more additions of the step
+    float r = ray;
+    bool in = true;
+    for (int k = 0; k < gl; ++k) {
+      const float nx = r + rs.inc;
+      if (nx == r) {
+        in = false;
+        break;
+      }
+      r = nx;
+    }
+    in = in && r < rs.ray_end;
+    float dist = 0.0f;
+    bool valid = false;
+    if (in) {
+      valid = trilinear<F16, false, REF>(t, bc, vox, col, rs.camx + r * h.wdx, rs.camy + r * h.wdy,
+                                         rs.camz + r * h.wdz, rs.v, &dist, nullptr) == kSampleValid;
+      ts.n_eval++;
+    }
+    // the sample before: lane gl - 1's, or the carried state for the first lane
+    float p_sdf = __shfl_up(dist, 1, G), p_r = __shfl_up(r, 1, G);
+    int p_v = __shfl_up(valid ? 1 : 0, 1, G);
+    if (gl == 0) {
+      p_sdf = last_sdf;
+      p_r = last_alpha;
+      p_v = last_w ? 1 : 0;
+    }
+    const bool cand = valid && p_v && p_sdf < 0.0f && dist > 0.0f && fabsf(p_sdf - dist) < rs.th &&
+                      fabsf(dist) < rs.th;
+    const uint32_t gm = (uint32_t)((__ballot(cand) >> gbase) & ((1ull << G) - 1ull));
+    const uint32_t gi = (uint32_t)((__ballot(in) >> gbase) & ((1ull << G) - 1ull));
+    if (gm != 0u) {
+      const int f = __builtin_ctz(gm);
+      const float a0 = __shfl(p_r, f, G), ad0 = __shfl(p_sdf, f, G), b0 = __shfl(r, f, G), bd0 = __shfl(dist, f, G);
+      // findIntersectionBisection (RayCastRender.cu:229-271), the same in every lane of the group
+      float a = a0, ad = ad0, bb = b0, bd = bd0, c = 0.0f;
+      uint32_t rgb = 0;
+      bool ok = true;
+#pragma unroll
+      for (int it = 0; it < 3; ++it) {
+        c = a + (ad / (ad - bd)) * (bb - a);
+        float cd;
+        const float qx = rs.camx + c * h.wdx, qy = rs.camy + c * h.wdy, qz = rs.camz + c * h.wdz;
+        const int stat = it < 2 ? trilinear<F16, false, REF>(t, bc, vox, col, qx, qy, qz, rs.v, &cd, &rgb)
+                                : trilinear<F16, true, REF>(t, bc, vox, col, qx, qy, qz, rs.v, &cd, &rgb);
+        if (stat != kSampleValid) {
+          ok = false;
+          break;
+        }
+        if (ad * cd > 0.0f) {
+          a = c;
+          ad = cd;
+        } else {
+          bb = c;
+          bd = cd;
+        }
+      }
+      if (ok) {
+        h.found = true;
+        h.alpha = c;
+        h.rgb = rgb;
+        h.a = a0;
+        h.ad = ad0;
+        h.b = b0;
+        h.bd = bd0;
+        done = true;
+      } else {
+        last_sdf = bd0;
+        last_alpha = b0;
+        last_w = 1u;
+        const float nx = b0 + rs.inc;
+        if (nx == b0 || !(nx < rs.ray_end)) done = true;
+        ray = nx;
+      }
+    } else {
+      // no accepted crossing: continue after the window's last sample
+      const int L = 31 - __builtin_clz(gi | 1u);
+      const float rL = __shfl(r, L, G), dL = __shfl(dist, L, G);
+      const int vL = __shfl(valid ? 1 : 0, L, G);
+      if (vL) {
+        last_sdf = dL;
+        last_alpha = rL;
+        last_w = 1u;
+      } else {
+        last_w = 0u;
+      }
+      const float nx = rL + rs.inc;
+      if (L < G - 1 || nx == rL || !(nx < rs.ray_end)) done = true;
+      ray = nx;
+    }
+  }
+  return done;
+}
+
+// Outputs of a finished tail ray (called by lane gl == 0 of its group); counts the hit or
+// miss.
+template <bool DEV>
+__device__ __forceinline__ void tail_write(const RenderArgs& ra, const TailState& s, float* __restrict__ out_depth,
+                                           float4* __restrict__ out_pos, float4* __restrict__ out_col,
+                                           float4* __restrict__ out_nrm, uint32_t& hit, uint32_t& miss) {
+  const float MINF = 1e10f;
+  const RayHit& h = s.h;
+  float o_depth = MINF;
+  float4 o_pos = make_float4(MINF, MINF, MINF, MINF), o_col = o_pos;
+  if (h.found) {
+    const float alpha = h.alpha;
+    o_depth = alpha / h.d2r;
+    float cam[3];
+    render_cam<DEV>(ra, cam);
+    o_pos = make_float4(cam[0] + alpha * h.wdx, cam[1] + alpha * h.wdy, cam[2] + alpha * h.wdz, 1.0f);
+    o_col = make_float4((float)(h.rgb & 0xFFu) / 255.0f, (float)((h.rgb >> 8) & 0xFFu) / 255.0f,
+                        (float)((h.rgb >> 16) & 0xFFu) / 255.0f, 1.0f);
+    hit++;
+  } else {
+    miss++;
+  }
+  out_depth[s.pix] = o_depth;
+  out_pos[s.pix] = o_pos;
+  out_col[s.pix] = o_col;
+  out_nrm[s.pix] = make_float4(MINF, MINF, MINF, MINF);
+}
+
 template <bool F16, bool DIAG, bool REF, bool DEV>
 __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, const void* __restrict__ vox,
                                                     const uint32_t* __restrict__ col, float* __restrict__ out_depth,
@@ -1453,19 +1629,17 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
   constexpr uint32_t kGroups = 64u / (uint32_t)G;  // groups per wave
   const uint32_t n_static = gridDim.x * kGroups;     // entries handed out without atomics
   uint32_t next_static = blockIdx.x * kGroups + (uint32_t)(gbase / G);
-  const float MINF = 1e10f;
   uint32_t hit = 0, miss = 0;
-  uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_window = 0, n_tskip = 0;
+  TailStats ts;
   __shared__ int32_t slot_lds[8 * 64];
   BlockCache<DIAG> bc;
   bc.init(slot_lds + threadIdx.x, ra.mark_bkt, ra.mark_blk, ra.march_stats);
   OccCache oc;
   occ_cache_init(oc);
-  RayHit h;
-  RaySetup rs;
-  uint32_t pix = 0;
-  float ray = 0.0f, last_sdf = 0.0f, last_alpha = 0.0f;
-  uint32_t last_w = 0;
+  TailState s;
+  s.pix = 0;
+  s.ray = s.last_sdf = s.last_alpha = 0.0f;
+  s.last_w = 0;
   bool have = false, exhausted = false;
   while (true) {
     // queue entries for the groups that need one: the first statically, then one
@@ -1489,12 +1663,12 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
     if (want) {
       if (e < n) {
         const TailRay q = queue[e];
-        pix = q.pix;
-        (void)ray_setup<DEV>(ra, pix % (uint32_t)ra.W, pix / (uint32_t)ra.W, h, rs);
-        ray = q.ray;
-        last_sdf = q.last_sdf;
-        last_alpha = q.last_alpha;
-        last_w = q.last_w;
+        s.pix = q.pix;
+        (void)ray_setup<DEV>(ra, s.pix % (uint32_t)ra.W, s.pix / (uint32_t)ra.W, s.h, s.rs);
+        s.ray = q.ray;
+        s.last_sdf = q.last_sdf;
+        s.last_alpha = q.last_alpha;
+        s.last_w = q.last_w;
         have = true;
       } else {
         exhausted = true;
@@ -1502,154 +1676,22 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
     }
     if (!__any(have)) break;
     if (!have) continue;
-    bool done = false;
-    bool window = true;
-    {
-      float t0, t1;
-      const int occ = skip_test(t, oc, rs.sk, h.wdx, h.wdy, h.wdz, ray, &t0, &t1);
-      const bool inside = t0 <= ray && ray < t1;
-      if (occ != 2 && inside) {
-        if (gl == 0) {
-          if (occ == 0) n_super++;
-          else n_block++;
-          n_tskip++;
-        }
-        last_w = 0u;
-        bool stl;
-        const float from = ray;
-        ray = replay_steps(ray, fminf(t1, rs.ray_end), rs.inc, &stl);
-        if (DIAG && gl == 0) n_replay += (uint32_t)((ray - from) / rs.inc + 0.5f);
-        if (stl || !(ray < rs.ray_end)) done = true;
-        window = false;
-      }
-    }
-    if (window) {
-      if (gl == 0) n_window++;
-      // sample gl of the window: gl more additions of the step
-      float r = ray;
-      bool in = true;
-      for (int k = 0; k < gl; ++k) {
-        const float nx = r + rs.inc;
-        if (nx == r) {
-          in = false;
-          break;
-        }
-        r = nx;
-      }
-      in = in && r < rs.ray_end;
-      float dist = 0.0f;
-      bool valid = false;
-      if (in) {
-        valid = trilinear<F16, false, REF>(t, bc, vox, col, rs.camx + r * h.wdx, rs.camy + r * h.wdy,
-                                           rs.camz + r * h.wdz, rs.v, &dist, nullptr) == kSampleValid;
-        n_eval++;
-      }
-      // the sample before: lane gl - 1's, or the carried state for the first lane
-      float p_sdf = __shfl_up(dist, 1, G), p_r = __shfl_up(r, 1, G);
-      int p_v = __shfl_up(valid ? 1 : 0, 1, G);
-      if (gl == 0) {
-        p_sdf = last_sdf;
-        p_r = last_alpha;
-        p_v = last_w ? 1 : 0;
-      }
-      const bool cand = valid && p_v && p_sdf < 0.0f && dist > 0.0f && fabsf(p_sdf - dist) < rs.th &&
-                        fabsf(dist) < rs.th;
-      const uint32_t gm = (uint32_t)((__ballot(cand) >> gbase) & ((1ull << G) - 1ull));
-      const uint32_t gi = (uint32_t)((__ballot(in) >> gbase) & ((1ull << G) - 1ull));
-      if (gm != 0u) {
-        const int f = __builtin_ctz(gm);
-        const float a0 = __shfl(p_r, f, G), ad0 = __shfl(p_sdf, f, G), b0 = __shfl(r, f, G), bd0 = __shfl(dist, f, G);
-        // findIntersectionBisection (RayCastRender.cu:229-271), the same in every lane of the group
-        float a = a0, ad = ad0, bb = b0, bd = bd0, c = 0.0f;
-        uint32_t rgb = 0;
-        bool ok = true;
-#pragma unroll
-        for (int it = 0; it < 3; ++it) {
-          c = a + (ad / (ad - bd)) * (bb - a);
-          float cd;
-          const float qx = rs.camx + c * h.wdx, qy = rs.camy + c * h.wdy, qz = rs.camz + c * h.wdz;
-          const int stat = it < 2 ? trilinear<F16, false, REF>(t, bc, vox, col, qx, qy, qz, rs.v, &cd, &rgb)
-                                  : trilinear<F16, true, REF>(t, bc, vox, col, qx, qy, qz, rs.v, &cd, &rgb);
-          if (stat != kSampleValid) {
-            ok = false;
-            break;
-          }
-          if (ad * cd > 0.0f) {
-            a = c;
-            ad = cd;
-          } else {
-            bb = c;
-            bd = cd;
-          }
-        }
-        if (ok) {
-          h.found = true;
-          h.alpha = c;
-          h.rgb = rgb;
-          h.a = a0;
-          h.ad = ad0;
-          h.b = b0;
-          h.bd = bd0;
-          done = true;
-        } else {
-          last_sdf = bd0;
-          last_alpha = b0;
-          last_w = 1u;
-          const float nx = b0 + rs.inc;
-          if (nx == b0 || !(nx < rs.ray_end)) done = true;
-          ray = nx;
-        }
-      } else {
-        // no accepted crossing: continue after the window's last sample
-        const int L = 31 - __builtin_clz(gi | 1u);
-        const float rL = __shfl(r, L, G), dL = __shfl(dist, L, G);
-        const int vL = __shfl(valid ? 1 : 0, L, G);
-        if (vL) {
-          last_sdf = dL;
-          last_alpha = rL;
-          last_w = 1u;
-        } else {
-          last_w = 0u;
-        }
-        const float nx = rL + rs.inc;
-        if (L < G - 1 || nx == rL || !(nx < rs.ray_end)) done = true;
-        ray = nx;
-      }
-    }
-    if (done) {
-      if (gl == 0) {
-        float o_depth = MINF;
-        float4 o_pos = make_float4(MINF, MINF, MINF, MINF), o_col = o_pos;
-        if (h.found) {
-          const float alpha = h.alpha;
-          o_depth = alpha / h.d2r;
-          float cam[3];
-          render_cam<DEV>(ra, cam);
-          o_pos = make_float4(cam[0] + alpha * h.wdx, cam[1] + alpha * h.wdy, cam[2] + alpha * h.wdz, 1.0f);
-          o_col = make_float4((float)(h.rgb & 0xFFu) / 255.0f, (float)((h.rgb >> 8) & 0xFFu) / 255.0f,
-                              (float)((h.rgb >> 16) & 0xFFu) / 255.0f, 1.0f);
-          hit++;
-        } else {
-          miss++;
-        }
-        out_depth[pix] = o_depth;
-        out_pos[pix] = o_pos;
-        out_col[pix] = o_col;
-        out_nrm[pix] = make_float4(MINF, MINF, MINF, MINF);
-      }
+    if (tail_iteration<F16, DIAG, REF>(t, ra, vox, col, bc, oc, s, gl, gbase, ts)) {
+      if (gl == 0) tail_write<DEV>(ra, s, out_depth, out_pos, out_col, out_nrm, hit, miss);
       have = false;
     }
   }
   if (DIAG && ra.march_stats) {
-    const uint32_t e = wave_sum(n_eval), su = wave_sum(n_super), bl = wave_sum(n_block), re = wave_sum(n_replay);
-    const uint32_t wi = wave_sum(n_window), ts = wave_sum(n_tskip);
+    const uint32_t e = wave_sum(ts.n_eval), su = wave_sum(ts.n_super), bl = wave_sum(ts.n_block),
+                   re = wave_sum(ts.n_replay);
+    const uint32_t wi = wave_sum(ts.n_window), tsk = wave_sum(ts.n_tskip);
     if (lane == 0) {
       atomicAdd(&ra.march_stats[0], e);
       atomicAdd(&ra.march_stats[1], su);
       atomicAdd(&ra.march_stats[2], bl);
       atomicAdd(&ra.march_stats[3], re);
       atomicAdd(&ra.march_stats[6], wi);
-      atomicAdd(&ra.march_stats[7], ts);
+      atomicAdd(&ra.march_stats[7], tsk);
     }
   }
   hit = wave_sum(hit);
@@ -3148,6 +3190,12 @@ void launch_gn(hipStream_t s, const float* depth, const float* obs, const float*
                      posef);
 }
 
+uint32_t render_grid(int32_t W, int32_t H) {
+  const uint32_t tiles_x = (uint32_t)((W + 7) / 8), tiles_y = (uint32_t)((H + 7) / 8);
+  const uint32_t nst = ((tiles_x + kSuperTile - 1u) / kSuperTile) * ((tiles_y + kSuperTile - 1u) / kSuperTile);
+  return 8u * ((nst + 7u) / 8u) * kSuperTile * kSuperTile;
+}
+
 uint32_t tail_workgroups(int32_t W, int32_t H) {
   const uint64_t n = ((uint64_t)W * (uint64_t)H + 63u) / 64u;
   return (uint32_t)(n < 1u ? 1u : (n > 4096u ? 4096u : n));
@@ -3263,8 +3311,7 @@ void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState*
                    uint32_t* rc) {
   const uint32_t tiles_x = (uint32_t)((ra.W + 7) / 8), tiles_y = (uint32_t)((ra.H + 7) / 8);
   const uint32_t ntiles = tiles_x * tiles_y;
-  const uint32_t nst = ((tiles_x + kSuperTile - 1u) / kSuperTile) * ((tiles_y + kSuperTile - 1u) / kSuperTile);
-  const dim3 grid(8u * ((nst + 7u) / 8u) * kSuperTile * kSuperTile);
+  const dim3 grid(render_grid(ra.W, ra.H));
   const bool diag = ra.mark_blk != nullptr;
   const bool ref = ra.mode == 0;
   uint32_t* rc_tail = rc + 2 * (size_t)ntiles;
