@@ -19,6 +19,7 @@ import argparse
 import importlib.util
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -264,6 +265,21 @@ def depthgen_bench(torch, vsg, sc, frames, reps: int = 20) -> dict:
             "algorithmic_bytes": int(b), "gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 5)}
 
 
+def shim_bench(nframes: int, passes: int) -> dict | None:
+    """The C++ drop-in classes (include/vsg_voxelscene.hpp) on the same sequence and passes,
+    timed by lib/vsg_shim_example's bench form (frames resident in device memory, pose as
+    a device pointer)."""
+    exe = os.path.join(PKG_DIR, "lib", "vsg_shim_example")
+    if not os.path.exists(exe):
+        return None
+    out = subprocess.run([exe, "bench", str(nframes), str(passes)], capture_output=True, text=True, timeout=600)
+    if out.returncode != 0:
+        raise RuntimeError(f"vsg_shim_example bench failed ({out.returncode}): {out.stderr[-500:]}")
+    tok = out.stdout.split()[1:]  # "bench" then name / value pairs
+    got = {tok[i]: tok[i + 1] for i in range(0, len(tok) - 1, 2)}
+    return {"value": float(got["fps"]), "ms_per_frame": float(got["ms_per_frame"]), "passes": passes}
+
+
 def se3_exp(xi) -> np.ndarray:
     """exp of the twist xi = (omega, upsilon) as a 4x4 (Rodrigues; V = I + B [w] + C [w]^2)."""
     w, u = np.asarray(xi[:3], np.float64), np.asarray(xi[3:], np.float64)
@@ -503,6 +519,8 @@ def main() -> int:
                     help="stream the large run's blocks beyond this distance (m) out every 25 frames (0: off)")
     ap.add_argument("--no-stream", action="store_true",
                     help="skip timing a stream-out / stream-in round trip of the fused scene")
+    ap.add_argument("--no-shim", action="store_true",
+                    help="skip timing the C++ drop-in classes on the same sequence (lib/vsg_shim_example bench)")
     ap.add_argument("--no-depthgen", action="store_true",
                     help="skip timing the depth maps from a point cloud")
     ap.add_argument("--no-pose", action="store_true",
@@ -759,6 +777,12 @@ def main() -> int:
         extra["marching_cubes"] = res["mesh"]
     if res["stream"] is not None:
         extra["streaming"] = res["stream"]
+    if not args.no_shim and nf == 300:
+        log(f"C++ drop-in path: {args.steps} passes over the same sequence")
+        sh = shim_bench(nf, args.steps)
+        if sh is not None:
+            sh["ratio_to_python"] = round(sh["value"] / fps, 4)
+            extra["cpp_shim"] = sh
     if not args.no_depthgen:
         extra["depth_from_points"] = depthgen_bench(torch, vsg, sc, frames)
     if not args.no_pose and nf >= 128:

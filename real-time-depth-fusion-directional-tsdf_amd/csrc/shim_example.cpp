@@ -23,11 +23,21 @@
 //
 // usage: vsg_shim_example <workload 0|1> <frames> [points.ply|-] [gen|-] [mesh.ply|-]
 //                         [stream:<radius>[:<cap>]]
+//        vsg_shim_example bench <frames> <passes>
+//
+// The bench form times the host code path of the drop-in classes on the 640x480 room
+// sequence as bench.py times the Python one: the frames' points, colours, normals and
+// poses resident in device memory before the timed region, then `passes` passes over
+// the sequence from an empty scene (VoxelScene::reset), each frame
+// integrateFromScanData then RayCastRender::render (pose as a device pointer, one
+// renderer for all frames) at the frame's depth range; one untimed pass first.  Prints
+// the frames per second and milliseconds per frame.
 // prints: frames, allocated blocks, hits, misses, a checksum of the last depth map,
 // the number of surface points found, the number of triangles and the number of
 // blocks held in the streaming chunk grid.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -46,7 +56,73 @@ void vsgs_render(void* p, int frame, const double* T16, int noise, float* pts, u
 void vsgs_depth_range(const float* depth, long n, float* out2);
 }
 
+static int bench_main(int nframes, int passes) {
+  void* sc = vsgs_create(1, 0, nframes);
+  int whn[3];
+  double intr[4];
+  vsgs_info(sc, whn, intr);
+  const int W = whn[0], H = whn[1];
+  const float fx = (float)intr[0], fy = (float)intr[1], cx = (float)intr[2], cy = (float)intr[3];
+  const size_t n = (size_t)W * H;
+  Params params(1 << 20, 1 << 19, 0.004f);
+  const float truncation = 0.02f;
+  VoxelScene scene(params);
+  if (!scene.isAllocated()) return 1;
+  float3 *d_pts = nullptr, *d_nrm = nullptr;
+  uchar3* d_rgb = nullptr;
+  float* d_T = nullptr;
+  if (hipMalloc(&d_pts, 12 * n * nframes) != hipSuccess || hipMalloc(&d_nrm, 12 * n * nframes) != hipSuccess ||
+      hipMalloc(&d_rgb, 3 * n * nframes) != hipSuccess || hipMalloc(&d_T, 64 * (size_t)nframes) != hipSuccess)
+    return 2;
+  std::vector<float> pts(3 * n), nrm(3 * n), depth(n), T(16 * (size_t)nframes), cams(3 * (size_t)nframes),
+      ranges(2 * (size_t)nframes);
+  std::vector<uint8_t> rgb(3 * n);
+  for (int i = 0; i < nframes; ++i) {
+    double Td[16];
+    vsgs_pose(sc, i, Td);
+    vsgs_render(sc, i, Td, 1, pts.data(), rgb.data(), nrm.data(), depth.data());
+    for (int k = 0; k < 16; ++k) T[16 * (size_t)i + k] = (float)Td[k];
+    cams[3 * (size_t)i] = T[16 * (size_t)i + 3];
+    cams[3 * (size_t)i + 1] = T[16 * (size_t)i + 7];
+    cams[3 * (size_t)i + 2] = T[16 * (size_t)i + 11];
+    vsgs_depth_range(depth.data(), (long)n, &ranges[2 * (size_t)i]);
+    if (hipMemcpy(d_pts + n * i, pts.data(), 12 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_rgb + n * i, rgb.data(), 3 * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_nrm + n * i, nrm.data(), 12 * n, hipMemcpyHostToDevice) != hipSuccess)
+      return 3;
+  }
+  if (hipMemcpy(d_T, T.data(), 64 * (size_t)nframes, hipMemcpyHostToDevice) != hipSuccess) return 3;
+  RayCastRender renderer;
+  if (!renderer.initialize(W, H)) return 4;
+  auto pass = [&]() {
+    scene.reset();
+    for (int i = 0; i < nframes; ++i) {
+      const float* Ti = &T[16 * (size_t)i];
+      scene.integrateFromScanData(d_pts + n * i, d_rgb + n * i, d_nrm + n * i, W, H, truncation, Ti, fx, fy, cx, cy);
+      const float3 cam = make_float3(cams[3 * (size_t)i], cams[3 * (size_t)i + 1], cams[3 * (size_t)i + 2]);
+      renderer.render(&scene, cam, d_T + 16 * (size_t)i, fx, fy, cx, cy, ranges[2 * (size_t)i],
+                      ranges[2 * (size_t)i + 1]);
+    }
+  };
+  pass();
+  if (hipDeviceSynchronize() != hipSuccess) return 5;
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int p = 0; p < passes; ++p) pass();
+  if (hipDeviceSynchronize() != hipSuccess) return 5;
+  const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  const double frames = (double)nframes * passes;
+  std::printf("bench frames %d passes %d fps %.3f ms_per_frame %.5f\n", nframes, passes, frames / s, s / frames * 1e3);
+  (void)hipFree(d_pts);
+  (void)hipFree(d_nrm);
+  (void)hipFree(d_rgb);
+  (void)hipFree(d_T);
+  vsgs_destroy(sc);
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc > 1 && std::string(argv[1]) == "bench")
+    return bench_main(argc > 2 ? std::atoi(argv[2]) : 300, argc > 3 ? std::atoi(argv[3]) : 3);
   const int cfg = argc > 1 ? std::atoi(argv[1]) : 0;
   const int nframes = argc > 2 ? std::atoi(argv[2]) : 5;
   void* sc = vsgs_create(cfg, 0, 0);
