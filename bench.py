@@ -457,13 +457,18 @@ def large_run_bench(torch, vsg, synth, nframes: int, stream_radius: float = 0.0,
     tblocks = tscene.counters()["num_allocated"]
     streaming = None
     if mgr is not None:
-        keys = [c.keys for c in mgr.m_grid.values()]
+        keys = [c.live_keys() for c in mgr.m_grid.values()]
         images = int(mgr.numStreamedBlocks())
         distinct = int(len(np.unique(np.concatenate(keys)))) if keys else 0
         streaming = {"radius": stream_radius, "every_frames": 25, "max_blocks_before": int(max_blocks),
                      "images_in_chunk_grid": images, "distinct_blocks_in_chunk_grid": distinct,
                      "dropped_outside_grid": int(mgr.droppedOutsideGrid),
-                     "round_trip_ms": _stage(t_stream)}
+                     "round_trip_ms": _stage(t_stream),
+                     "blocks_out_per_round_trip": int(mgr.nStreamedOut / max(len(t_stream), 1)),
+                     "blocks_in_per_round_trip": int(mgr.nStreamedIn / max(len(t_stream), 1)),
+                     "image_bytes": int(scene.block_image_bytes()),
+                     "ms_per_round_trip_by_part": {k: round(v / max(len(t_stream), 1) * 1e3, 1)
+                                                   for k, v in mgr.seconds.items()}}
     scene.close()
     tscene.close()
     for t in trackers:

@@ -17,6 +17,7 @@ from __future__ import annotations
 import ctypes as C
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -1204,44 +1205,75 @@ class _Chunk:
     """One chunk of the streaming grid: one image per block coordinate, in the order the
     coordinates first arrived; a coordinate that arrives again keeps its place and takes
     the newer image.  Coordinates are packed as the scene's keys (21 bits per axis,
-    injective over the coordinates a scene can hold)."""
+    injective over the coordinates a scene can hold).  Rows [n0, n) of the buffers are
+    live; the buffers grow by doubling, so an update copies only what it adds, and taking
+    blocks from the front moves n0 instead of the rows."""
 
     def __init__(self, image_bytes: int):
-        self.keys = np.zeros(0, np.int64)
-        self.coords = np.zeros((0, 3), np.int32)
-        self.images = np.zeros((0, image_bytes), np.uint8)
+        self.n0 = self.n = 0
+        self.keys = np.zeros(16, np.int64)
+        self.coords = np.zeros((16, 3), np.int32)
+        self.images = np.zeros((16, image_bytes), np.uint8)
 
     def __len__(self) -> int:
-        return len(self.keys)
+        return self.n - self.n0
+
+    def live_keys(self) -> np.ndarray:
+        """Packed coordinates of the blocks the chunk holds, in order."""
+        return self.keys[self.n0:self.n]
 
     @staticmethod
     def pack(coords) -> np.ndarray:
         c = np.asarray(coords, np.int64).reshape(-1, 3) & 0x1FFFFF
         return (c[:, 0] << 42) | (c[:, 1] << 21) | c[:, 2]
 
-    def put(self, coords, images) -> None:
-        k = self.pack(coords)
+    def _room(self, m: int) -> None:
+        if self.n + m <= len(self.keys):
+            return
+        live = len(self)
+        cap = max(16, 2 * (live + m))
+        keys, coords = np.empty(cap, np.int64), np.empty((cap, 3), np.int32)
+        images = np.empty((cap, self.images.shape[1]), np.uint8)
+        keys[:live] = self.keys[self.n0:self.n]
+        coords[:live] = self.coords[self.n0:self.n]
+        images[:live] = self.images[self.n0:self.n]
+        self.keys, self.coords, self.images = keys, coords, images
+        self.n0, self.n = 0, live
+
+    def put(self, coords, images, rows) -> None:
+        """Blocks rows of (coords, images): each image is copied once, from `images` into
+        the chunk."""
+        rows = np.asarray(rows, np.int64)
+        k = self.pack(coords[rows])
+        src = rows
         if len(np.unique(k)) < len(k):  # repeated within the batch: first place, last image
             u, first = np.unique(k, return_index=True)
             last = len(k) - 1 - np.unique(k[::-1], return_index=True)[1]
             o = np.argsort(first)
-            k, coords, images = k[first[o]], coords[first[o]], images[last[o]]
-        if len(self.keys):
-            srt = np.argsort(self.keys)
-            j = np.minimum(np.searchsorted(self.keys[srt], k), len(srt) - 1)
-            hit = self.keys[srt[j]] == k
-            self.images[srt[j[hit]]] = images[hit]
-            k, coords, images = k[~hit], coords[~hit], images[~hit]
-        if len(k):
-            self.keys = np.concatenate([self.keys, k])
-            self.coords = np.concatenate([self.coords, coords])
-            self.images = np.concatenate([self.images, images])
+            k, rows, src = k[first[o]], rows[first[o]], rows[last[o]]
+        live = self.keys[self.n0:self.n]
+        if len(live):
+            srt = np.argsort(live)
+            j = np.minimum(np.searchsorted(live[srt], k), len(srt) - 1)
+            hit = live[srt[j]] == k
+            if hit.any():
+                self.images[self.n0 + srt[j[hit]]] = images[src[hit]]
+                k, rows, src = k[~hit], rows[~hit], src[~hit]
+        m = len(k)
+        if m:
+            self._room(m)
+            self.keys[self.n:self.n + m] = k
+            self.coords[self.n:self.n + m] = coords[rows]
+            np.take(images, src, axis=0, out=self.images[self.n:self.n + m])
+            self.n += m
 
-    def take_front(self, n: int):
-        """Removes and returns the first n blocks (coords, images)."""
-        c, im = self.coords[:n], self.images[:n]
-        self.keys, self.coords, self.images = self.keys[n:].copy(), self.coords[n:].copy(), self.images[n:].copy()
-        return c, im
+    def take_front(self, m: int):
+        """Removes the first m blocks; returns views of their (coords, images), valid until
+        the next put."""
+        m = min(m, len(self))
+        sl = slice(self.n0, self.n0 + m)
+        self.n0 += m
+        return self.coords[sl], self.images[sl]
 
 
 class VoxelStreamingManager:
@@ -1286,6 +1318,10 @@ class VoxelStreamingManager:
         self.nStreamedIn = 0
         self._out = (np.zeros((0, 3), np.int32), np.zeros((0, 0), np.uint8))
         self._in = (np.zeros((0, 3), np.int32), np.zeros((0, 0), np.uint8))
+        self._stage = None  # stream-out's host buffer, kept between calls
+        # seconds spent in the scene's stream_out / stream_in calls (kernels and copies) and
+        # in the chunk grid (host)
+        self.seconds = {"scene_out": 0.0, "scene_in": 0.0, "grid": 0.0}
 
     # -- chunk geometry (VoxelStreamingManager.cpp:905-990) ----------------------
     def worldToChunks(self, posWorld) -> np.ndarray:
@@ -1317,15 +1353,24 @@ class VoxelStreamingManager:
                                  multiThreaded: bool = False) -> int:
         """Removes the blocks at distance >= radius from sphereCenter from the scene
         (at most m_maxNumberOfSDFBlocksIntegrateFromGlobalHash per call) and holds
-        them for streamOutCopyToChunkGrid.  Returns how many."""
-        coords, images = self.scene.stream_out(sphereCenter, radius, self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash)
-        self._out = (coords.copy(), images.copy())
+        them for streamOutCopyToChunkGrid, in a host buffer the next call reuses (the
+        reference's single output buffer).  Returns how many."""
+        cap = self.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash
+        nb = self.scene.block_image_bytes()
+        if self._stage is None or self._stage[0].shape[0] < cap or self._stage[1].shape[1] != nb:
+            self._stage = (np.empty((cap, 3), np.int32), np.empty((cap, nb), np.uint8))
+        t0 = time.perf_counter()
+        coords, images = self.scene.stream_out(sphereCenter, radius, cap, self._stage[0], self._stage[1])
+        self.seconds["scene_out"] += time.perf_counter() - t0
+        self._out = (coords, images)  # views of the staging buffer
         self.nStreamedOut += len(coords)
         return len(coords)
 
     def streamOutCopyToChunkGrid(self, multiThreaded: bool = False) -> None:
         coords, images = self._out
+        t0 = time.perf_counter()
         self.updateChunkGrid(coords, images)
+        self.seconds["grid"] += time.perf_counter() - t0
         self._out = (np.zeros((0, 3), np.int32), np.zeros((0, 0), np.uint8))
 
     def updateChunkGrid(self, coords, images) -> None:
@@ -1336,17 +1381,18 @@ class VoxelStreamingManager:
         self.droppedOutsideGrid += int((~valid).sum())
         if not valid.any():
             return
-        coords = np.asarray(coords, np.int32).reshape(-1, 3)[valid]
-        images = np.asarray(images, np.uint8).reshape(len(valid), -1)[valid]
-        idx = self.linearizeChunkPos(chunks[valid])
+        coords = np.asarray(coords, np.int32).reshape(-1, 3)
+        images = np.asarray(images, np.uint8).reshape(len(valid), -1)
+        keep = np.flatnonzero(valid)
+        idx = self.linearizeChunkPos(chunks[keep])
         order = np.argsort(idx, kind="stable")
         cut = np.flatnonzero(np.diff(idx[order])) + 1
-        for rows in np.split(order, cut):
-            ci = int(idx[rows[0]])
+        for part in np.split(order, cut):
+            ci = int(idx[part[0]])
             chunk = self.m_grid.get(ci)
             if chunk is None:
                 chunk = self.m_grid[ci] = _Chunk(images.shape[1])
-            chunk.put(coords[rows], images[rows])
+            chunk.put(coords, images, keep[part])
 
     # -- stream-in ------------------------------------------------------------------
     def streamInFindChunk(self, sphereCenter, radius: float, useParts: bool = False) -> int:
@@ -1387,7 +1433,10 @@ class VoxelStreamingManager:
 
     def streamInCopyToGPUBuffer(self, sphereCenter, radius: float, useParts: bool = False,
                                 multiThreaded: bool = False) -> int:
-        return self.streamInFindChunk(sphereCenter, radius, useParts)
+        t0 = time.perf_counter()
+        n = self.streamInFindChunk(sphereCenter, radius, useParts)
+        self.seconds["grid"] += time.perf_counter() - t0
+        return n
 
     def streamInInsertToHashTable(self, multiThreaded: bool = False) -> np.ndarray:
         """Inserts the collected blocks; those that could not be inserted (bucket full,
@@ -1396,7 +1445,9 @@ class VoxelStreamingManager:
         self._in = (np.zeros((0, 3), np.int32), np.zeros((0, 0), np.uint8))
         if len(coords) == 0:
             return np.zeros(0, np.uint8)
+        t0 = time.perf_counter()
         status = self.scene.stream_in(coords, images)
+        self.seconds["scene_in"] += time.perf_counter() - t0
         self.nStreamedIn += int((status == 0).sum())
         back = (status == 2) | (status == 3)
         if back.any():

@@ -106,7 +106,7 @@ def test_cpp_streaming_manager_matches_python(torch_cuda, cap):
     checksum = float(np.sum(d[ok].astype(np.float64) * ((k[ok] % 97) + 1)))
     assert mgr.numStreamedBlocks() > 0
     assert int(got["streamed"]) == mgr.numStreamedBlocks()
-    keys = np.concatenate([c.keys for c in mgr.m_grid.values()])
+    keys = np.concatenate([c.live_keys() for c in mgr.m_grid.values()])
     assert len(np.unique(keys)) == len(keys)
     assert int(got["blocks"]) == g.scene.counters()["num_allocated"]
     assert (int(got["hits"]), int(got["misses"])) == (r["hits"], r["misses"])

@@ -30,10 +30,14 @@ class FakeScene:
     def block_image_bytes(self):
         return NB
 
-    def stream_out(self, center, radius, cap):
+    def stream_out(self, center, radius, cap, coords_out=None, images_out=None):
         c, im = self.queue.pop(0) if self.queue else (np.zeros((0, 3), np.int32), np.zeros((0, NB), np.uint8))
         assert len(c) <= cap
-        return c, im
+        if coords_out is None:
+            return c, im
+        coords_out[:len(c)] = c  # like VoxelScene.stream_out: views of the caller's buffers
+        images_out[:len(c)] = im
+        return coords_out[:len(c)], images_out[:len(c)]
 
     def stream_in(self, coords, images):
         st = np.zeros(len(coords), np.uint8)
@@ -181,3 +185,22 @@ def test_random_traffic_holds_no_more_images_than_distinct_blocks(seed):
     st = m.streamIn(sc, (0.0, 0.0, 0.0), 10.0)
     assert np.all(st == 0) and m.numStreamedBlocks() == 0
     assert {k: int(v[0]) for k, v in sc.blocks.items()} == newest
+
+
+def test_partial_take_then_put_keeps_order_and_newest():
+    """A chunk taken in part keeps its remaining blocks in order; a later put replaces a
+    remaining block's image in place and appends new ones after the rest (buffers grown
+    by doubling, taken rows skipped, not moved)."""
+    m = _mgr(FakeScene(), cap=4)
+    a = _chunk_blocks(0, 10)
+    m.updateChunkGrid(a, _img(np.arange(10), 10))
+    assert m.streamInFindChunk((0.0, 0.0, 0.0), 5.0) == 4
+    assert [int(v) for v in m._in[1][:, 0]] == [0, 1, 2, 3]
+    extra = np.array([[40, 40, 0], [41, 40, 0]], np.int32)  # 3.2 m: another chunk
+    m.updateChunkGrid(np.concatenate([a[5:6], extra]), _img([55, 60, 61], 3))
+    m.m_maxNumberOfSDFBlocksIntegrateFromGlobalHash = 100
+    assert m.streamInFindChunk((0.0, 0.0, 0.0), 5.0) == 8
+    coords, images = m._in
+    assert [int(v) for v in images[:6, 0]] == [4, 55, 6, 7, 8, 9]
+    assert np.array_equal(coords[:6], a[4:])
+    assert sorted(int(v) for v in images[6:, 0]) == [60, 61]
