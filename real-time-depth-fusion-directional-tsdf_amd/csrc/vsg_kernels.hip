@@ -671,14 +671,16 @@ template <bool DIAG> struct BlockCache {
   }
   __device__ __forceinline__ int32_t get(const TableView& t, int32_t x, int32_t y, int32_t z, uint32_t* mask) {
     const int32_t q = index_lookup(t, x, y, z, mask);
-    if (DIAG) {
-      if (key_in_range(x, y, z)) {
-        const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
-        if (t.cnt[h] > 0) mark_bkt[h] = 1;
-      }
-      if (q >= 0) mark_blk[q] = 1;
-    }
+    if (DIAG) mark(t, x, y, z, q);
     return q;
+  }
+  // Diagnostics of a lookup of block (x, y, z) that found pointer q (-1: absent).
+  __device__ __forceinline__ void mark(const TableView& t, int32_t x, int32_t y, int32_t z, int32_t q) {
+    if (key_in_range(x, y, z)) {
+      const uint32_t h = hash_bucket(x, y, z, t.num_buckets);
+      if (t.cnt[h] > 0) mark_bkt[h] = 1;
+    }
+    if (q >= 0) mark_blk[q] = 1;
   }
   // The window along one axis after a request for blocks [c0, c0 + s]: unchanged
   // when they fit, else moved (by one block: the block that leaves frees its slots;
@@ -704,18 +706,52 @@ template <bool DIAG> struct BlockCache {
     const uint32_t ey = sy ? 0xFFu : ((y0 & 1) ? 0xCCu : 0x33u);
     const uint32_t ez = sz ? 0xFFu : ((z0 & 1) ? 0xF0u : 0x0Fu);
     uint32_t missing = ex & ey & ez & ~known;
+    known |= missing;
+    // Missing blocks are looked up two at a time: both first index probes are issued
+    // before either is used (one memory round trip per pair); a probe that meets
+    // another key continues along the probe sequence.
     while (missing) {
       const int i = __builtin_ctz(missing);
       missing &= missing - 1u;
-      // the window block with slot i's parities
-      const int32_t x = bx + (int32_t)(((uint32_t)i ^ (uint32_t)bx) & 1u);
-      const int32_t y = by + (int32_t)((((uint32_t)i >> 1) ^ (uint32_t)by) & 1u);
-      const int32_t z = bz + (int32_t)((((uint32_t)i >> 2) ^ (uint32_t)bz) & 1u);
-      uint32_t mk;
-      slots[64 * i] = get(t, x, y, z, &mk);
-      pm = (pm & ~(0xFFull << (8 * i))) | ((uint64_t)(mk & 0xFFu) << (8 * i));
-      known |= 1u << i;
+      const int j = missing ? __builtin_ctz(missing) : i;
+      missing &= missing - 1u;
+      const uint32_t hi_ = idx_home(slot_x(i), slot_y(i), slot_z(i), t.idx_mask);
+      const uint32_t hj = idx_home(slot_x(j), slot_y(j), slot_z(j), t.idx_mask);
+      const uint4 ei = t.idx[hi_], ej = t.idx[hj];
+      resolve(t, i, hi_, ei);
+      if (j != i) resolve(t, j, hj, ej);
     }
+  }
+  // Block of slot i from the first probe e (at home slot s) of its index lookup.
+  __device__ __forceinline__ void resolve(const TableView& t, int i, uint32_t s, uint4 e) {
+    const int32_t x = slot_x(i), y = slot_y(i), z = slot_z(i);
+    int32_t q = -1;
+    uint32_t mk = 0u;
+    if (key_in_range(x, y, z)) {
+      const uint64_t key = pack_key(x, y, z);
+      const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
+      for (uint32_t n = 0; n <= t.idx_mask; ++n) {
+        if (e.x == klo && e.y == khi) {
+          q = (int32_t)e.z;
+          mk = e.w;
+          break;
+        }
+        if (e.x == kIdxEmpty && e.y == kIdxEmpty) break;
+        s = (s + 1u) & t.idx_mask;
+        e = t.idx[s];
+      }
+    }
+    if (DIAG) mark(t, x, y, z, q);
+    slots[64 * i] = q;
+    pm = (pm & ~(0xFFull << (8 * i))) | ((uint64_t)(mk & 0xFFu) << (8 * i));
+  }
+  // Coordinates of the window block with slot i's parities.
+  __device__ __forceinline__ int32_t slot_x(int i) const { return bx + (int32_t)(((uint32_t)i ^ (uint32_t)bx) & 1u); }
+  __device__ __forceinline__ int32_t slot_y(int i) const {
+    return by + (int32_t)((((uint32_t)i >> 1) ^ (uint32_t)by) & 1u);
+  }
+  __device__ __forceinline__ int32_t slot_z(int i) const {
+    return bz + (int32_t)((((uint32_t)i >> 2) ^ (uint32_t)bz) & 1u);
   }
   // Pointers of the eight corner blocks of a sample whose corners lie in blocks
   // {x0, x0 + sx} x {y0, y0 + sy} x {z0, z0 + sz}; corner i takes x from bit 0, y from
@@ -796,15 +832,21 @@ __device__ __forceinline__ void corner_setup(float px, float py, float pz, float
   }
 }
 
-template <bool F16, bool COLOR, bool REF, class Cache>
-__device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const void* __restrict__ vox_,
-                                         const uint32_t* __restrict__ col, float px, float py, float pz, float v,
-                                         float* dist, uint32_t* rgb) {
-  using S = Store<F16>;
-  const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
-  int32_t vx[2], vy[2], vz[2];
+// Corners of one sample: the store index of each corner voxel (0 for a corner
+// without an allocated block, flagged in `none`) and the interpolation weights.
+struct Corners {
+  uint32_t vi[8];
+  uint32_t none;  // bit i: corner i has no allocated block
   float wx, wy, wz;
-  corner_setup<REF>(px, py, pz, v, vx, vy, vz, &wx, &wy, &wz);
+};
+
+// Corner setup and block pointers of the sample at (px, py, pz): the block lookups
+// (through the lane's block cache) happen here, the voxel reads in corner_gather.
+template <bool REF, class Cache>
+__device__ __forceinline__ void corner_prep(const TableView& t, Cache& bc, float px, float py, float pz, float v,
+                                            Corners& cn) {
+  int32_t vx[2], vy[2], vz[2];
+  corner_setup<REF>(px, py, pz, v, vx, vy, vz, &cn.wx, &cn.wy, &cn.wz);
   // block (floor division by 8) and offset in the block per axis and corner
   const int32_t bx0 = vx[0] >> 3, bx1 = vx[1] >> 3, by0 = vy[0] >> 3, by1 = vy[1] >> 3;
   const int32_t bz0 = vz[0] >> 3, bz1 = vz[1] >> 3;
@@ -813,18 +855,32 @@ __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const vo
   const uint32_t oz[2] = {((uint32_t)vz[0] & 7u) << 6, ((uint32_t)vz[1] & 7u) << 6};
   int32_t p[8];
   bc.corners(t, bx0, by0, bz0, bx1 != bx0, by1 != by0, bz1 != bz0, p);
-  // Corners without an allocated block or with weight 0 contribute a zero weight:
-  // adding +-0 leaves every sum unchanged (a sum started at +0 never becomes -0),
-  // so this equals skipping them.  Every corner is loaded (a corner without a block
-  // reads voxel 0 of the store and its values are replaced by zeros), so the gather
-  // has no branches.
+  cn.none = 0u;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const bool ok = p[i] >= 0;
+    cn.vi[i] = ok ? (uint32_t)p[i] * (uint32_t)kVox + ox[i & 1] + oy[(i >> 1) & 1] + oz[(i >> 2) & 1] : 0u;
+    cn.none |= ok ? 0u : (1u << i);
+  }
+}
+
+// The interpolation of trilinear() from prepared corners.  Corners without an
+// allocated block or with weight 0 contribute a zero weight: adding +-0 leaves every
+// sum unchanged (a sum started at +0 never becomes -0), so this equals skipping
+// them.  Every corner is loaded (a corner without a block reads voxel 0 of the store
+// and its values are replaced by zeros), so the gather has no branches.
+template <bool F16, bool COLOR>
+__device__ __forceinline__ int corner_gather(const void* __restrict__ vox_, const uint32_t* __restrict__ col,
+                                             const Corners& cn, float* dist, uint32_t* rgb) {
+  using S = Store<F16>;
+  const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
+  const float wx = cn.wx, wy = cn.wy, wz = cn.wz;
   const float ux = 1.0f - wx, uy = 1.0f - wy, uz = 1.0f - wz;
   float d = 0.0f, cr = 0.0f, cg = 0.0f, cbl = 0.0f, wsum = 0.0f;
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
-    const bool ok = p[i] >= 0;
-    const uint32_t vi =
-        ok ? (uint32_t)p[i] * (uint32_t)kVox + ox[i & 1] + oy[(i >> 1) & 1] + oz[(i >> 2) & 1] : 0u;
+    const bool ok = !((cn.none >> i) & 1u);
+    const uint32_t vi = cn.vi[i];
     const typename S::word w = vox[vi];
     const uint32_t c = COLOR ? (ok ? col[vi] : 0u) : 0u;
     const float wi = ((i & 1) ? wx : ux) * ((i & 2) ? wy : uy) * ((i & 4) ? wz : uz);
@@ -850,6 +906,15 @@ __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const vo
     *rgb = ((uint32_t)cr & 0xFFu) | (((uint32_t)cg & 0xFFu) << 8) | (((uint32_t)cbl & 0xFFu) << 16);
   }
   return kSampleValid;
+}
+
+template <bool F16, bool COLOR, bool REF, class Cache>
+__device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const void* __restrict__ vox_,
+                                         const uint32_t* __restrict__ col, float px, float py, float pz, float v,
+                                         float* dist, uint32_t* rgb) {
+  Corners cn;
+  corner_prep<REF>(t, bc, px, py, pz, v, cn);
+  return corner_gather<F16, COLOR>(vox_, col, cn, dist, rgb);
 }
 
 // ---- exact empty-space skipping ---------------------------------------------------
@@ -1145,6 +1210,87 @@ __device__ __forceinline__ int skip_test(const TableView& t, OccCache& oc, const
   return occ;
 }
 
+// Bits lo..hi (0 <= lo <= hi <= 3) of a 4-bit row, and the same rows spread to the
+// y rows (bits 4y) and z planes (bits 16z) of a super-block's 64-bit occupancy word.
+__device__ __forceinline__ uint32_t occ_row(uint32_t lo, uint32_t hi) { return (2u << hi) - (1u << lo); }
+__device__ __forceinline__ uint32_t occ_rows_y(uint32_t lo, uint32_t hi) {
+  return (0x1111u >> (4u * (3u - hi))) & (0x1111u << (4u * lo));
+}
+__device__ __forceinline__ uint64_t occ_planes_z(uint32_t lo, uint32_t hi) {
+  return (0x0001000100010001ull >> (16u * (3u - hi))) & (0x0001000100010001ull << (16u * lo));
+}
+
+// Block range of the samples of a ray with parameter in [ta, tb]: no such sample
+// takes a corner voxel from outside blocks [b0, b1] per axis.  Sample positions are
+// cam + r * wd in float, monotone in r per axis, so every sample of the interval lies
+// between the positions of its ends (computed by the same float operations); its
+// corners lie within 1.125 + eps voxels of its position (as in skip_test).  The
+// interval must span at most 29 voxels per axis: then a range covers at most 5
+// blocks, hence at most two super-blocks per axis (`fits` is false otherwise, and
+// for non-finite positions).
+struct SegRange {
+  int32_t b0[3], b1[3];
+  bool fits;
+};
+
+__device__ __forceinline__ void segment_range(const RaySkip& sk, float ta, float tb, SegRange& sr) {
+  const float mq = 1.125f + sk.eps;
+  const float c[3] = {sk.camx, sk.camy, sk.camz}, d[3] = {sk.wdx, sk.wdy, sk.wdz};
+  sr.fits = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float qa = (c[a] + ta * d[a]) * sk.inv_v, qb = (c[a] + tb * d[a]) * sk.inv_v;
+    sr.b0[a] = cvt_i32(floorf((fminf(qa, qb) - mq) * 0.125f));
+    sr.b1[a] = cvt_i32(floorf((fmaxf(qa, qb) + mq) * 0.125f));
+    sr.fits = sr.fits && __builtin_isfinite(qa) && __builtin_isfinite(qb) &&
+              (uint32_t)sr.b1[a] - (uint32_t)sr.b0[a] <= 4u;
+  }
+}
+
+// Super-block k (0..7) of a range: s0 + (k >> a & 1) per axis, its share [lo, hi] of
+// the block range in local coordinates; false when the range stays in s0 on an axis
+// where k takes the second super-block.
+__device__ __forceinline__ bool segment_part(const SegRange& sr, int k, int32_t s[3], uint32_t lo[3],
+                                             uint32_t hi[3]) {
+  bool in = true;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int32_t s0 = sr.b0[a] >> 2, s1 = sr.b1[a] >> 2;
+    const bool second = (k >> a) & 1;
+    s[a] = s0 + (second ? 1 : 0);
+    in = in && (!second || s1 != s0);
+    lo[a] = second ? 0u : ((uint32_t)sr.b0[a] & 3u);
+    hi[a] = (second || s1 == s0) ? ((uint32_t)sr.b1[a] & 3u) : 3u;
+  }
+  return in;
+}
+
+// Reads the occupancy words of the (up to 8) super-blocks of a range.
+__device__ __forceinline__ void segment_words(const TableView& t, const SegRange& sr, uint64_t w[8]) {
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int32_t s[3];
+    uint32_t lo[3], hi[3];
+    (void)segment_part(sr, k, s, lo, hi);
+    w[k] = t.occ[occ_index(s[0], s[1], s[2], t.occ_mask)];
+  }
+}
+
+// True when the occupancy words w of a fitting range show no allocated block in it.
+__device__ __forceinline__ bool segment_clear(const SegRange& sr, const uint64_t w[8]) {
+  uint64_t hit = 0ull;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    int32_t s[3];
+    uint32_t lo[3], hi[3];
+    const bool in = segment_part(sr, k, s, lo, hi);
+    const uint32_t mxy = occ_row(lo[0], hi[0]) * occ_rows_y(lo[1], hi[1]);
+    const uint64_t m = in ? (uint64_t)mxy * occ_planes_z(lo[2], hi[2]) : 0ull;
+    hit |= w[k] & m;
+  }
+  return sr.fits && hit == 0ull;
+}
+
 // Positive-cell test inside allocated blocks.  The sample's corner voxels lie in the
 // 4^3-voxel cells c0..c1 per axis (Q in [4*c0 + 1, 4*c1 + 3) keeps them there, as for
 // blocks).  A sample can be the upper end of an accepted crossing only if its
@@ -1438,6 +1584,9 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // fetches), until the queue is empty.  The last workgroup clears the queue controls
 // for the next render.
 constexpr int kTailGroup = 16;
+// Look-ahead segment of the tail pass, in voxels along the ray (at most 29: see
+// segment_empty).
+constexpr float kLookSeg = 16.0f;
 
 // March state of a ray in the tail pass, one per group of kTailGroup lanes (the same in
 // every lane of the group).
@@ -1455,14 +1604,15 @@ struct TailStats {
 };
 
 // One iteration of the tail pass on the group's ray: a skip over samples the occupancy
-// maps prove invalid, or a window of kTailGroup consecutive samples (lane gl takes the
-// sample gl additions of the step ahead) resolved in order; the first sign change that
-// passes the thresholds is refined.  Returns true when the ray is finished (s.h.found:
+// map proves invalid (the look-ahead over kTailGroup segments of kLookSeg voxels), or a
+// window of kTailGroup consecutive samples (lane gl takes the sample gl additions of
+// the step ahead) resolved in order; the first sign change that passes the thresholds
+// is refined.  Returns true when the ray is finished (s.h.found:
 // whether it hit).  Every lane of the group takes part.
 template <bool F16, bool DIAG, bool REF, class Cache>
 __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
-                                               const uint32_t* __restrict__ col, Cache& bc, OccCache& oc,
-                                               TailState& s, int gl, int gbase, TailStats& ts) {
+                                               const uint32_t* __restrict__ col, Cache& bc, TailState& s, int gl,
+                                               int gbase, TailStats& ts) {
   constexpr int G = kTailGroup;
   RayHit& h = s.h;
   const RaySetup& rs = s.rs;
@@ -1471,45 +1621,52 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
   float& last_alpha = s.last_alpha;
   uint32_t& last_w = s.last_w;
   bool done = false;
-  bool window = true;
+  // sample gl of the window: gl more additions of the step
+  float r = ray;
+  bool in = true;
+  for (int k = 0; k < gl; ++k) {
+    const float nx = r + rs.inc;
+    if (nx == r) {
+      in = false;
+      break;
+    }
+    r = nx;
+  }
+  in = in && r < rs.ray_end;
+  // Look-ahead: lane gl tests segment [T_gl, T_gl+1] of the ray, T_k = ray + k * S.
+  // Its occupancy words are read together with the block lookups of the lane's
+  // window sample (one memory round trip for both); the voxels are read only when
+  // the window is evaluated.
+  const float S = kLookSeg * rs.v;
+  const float ta = ray + (float)gl * S, tb = ray + (float)(gl + 1) * S;
+  SegRange sr;
+  segment_range(rs.sk, ta, tb, sr);
+  uint64_t ow[8];
+  segment_words(t, sr, ow);
+  Corners cn;
+  if (in) corner_prep<REF>(t, bc, rs.camx + r * h.wdx, rs.camy + r * h.wdy, rs.camz + r * h.wdz, rs.v, cn);
   {
-    float t0, t1;
-    const int occ = skip_test(t, oc, rs.sk, h.wdx, h.wdy, h.wdz, ray, &t0, &t1);
-    const bool inside = t0 <= ray && ray < t1;
-    if (occ != 2 && inside) {
-      if (gl == 0) {
-        if (occ == 0) ts.n_super++;
-        else ts.n_block++;
-        ts.n_tskip++;
-      }
+    // the samples before the first segment that can reach an allocated block are
+    // invalid: the ray moves on to its first sample at or beyond that segment
+    const bool reach = ta < rs.ray_end && !segment_clear(sr, ow);
+    const uint32_t gm = (uint32_t)((__ballot(reach) >> gbase) & ((1ull << G) - 1ull));
+    const int f = gm != 0u ? __builtin_ctz(gm) : G;
+    if (f > 0) {
+      if (gl == 0) ts.n_tskip++;
       last_w = 0u;
       bool stl;
       const float from = ray;
-      ray = replay_steps(ray, fminf(t1, rs.ray_end), rs.inc, &stl);
+      ray = replay_steps(ray, fminf(ray + (float)f * S, rs.ray_end), rs.inc, &stl);
       if (DIAG && gl == 0) ts.n_replay += (uint32_t)((ray - from) / rs.inc + 0.5f);
-      if (stl || !(ray < rs.ray_end)) done = true;
-      window = false;
+      return stl || !(ray < rs.ray_end);
     }
   }
-  if (window) {
+  {
     if (gl == 0) ts.n_window++;
-    // sample gl of the window: gl more additions of the step
-    float r = ray;
-    bool in = true;
-    for (int k = 0; k < gl; ++k) {
-      const float nx = r + rs.inc;
-      if (nx == r) {
-        in = false;
-        break;
-      }
-      r = nx;
-    }
-    in = in && r < rs.ray_end;
     float dist = 0.0f;
     bool valid = false;
     if (in) {
-      valid = trilinear<F16, false, REF>(t, bc, vox, col, rs.camx + r * h.wdx, rs.camy + r * h.wdy,
-                                         rs.camz + r * h.wdz, rs.v, &dist, nullptr) == kSampleValid;
+      valid = corner_gather<F16, false>(vox, col, cn, &dist, nullptr) == kSampleValid;
       ts.n_eval++;
     }
     // the sample before: lane gl - 1's, or the carried state for the first lane
@@ -1616,7 +1773,7 @@ __device__ __forceinline__ void tail_write(const RenderArgs& ra, const TailState
 }
 
 template <bool F16, bool DIAG, bool REF, bool DEV>
-__global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, const void* __restrict__ vox,
+__global__ __launch_bounds__(64, 4) void k_render_tail(TableView t, RenderArgs ra, const void* __restrict__ vox,
                                                     const uint32_t* __restrict__ col, float* __restrict__ out_depth,
                                                     float4* __restrict__ out_pos, float4* __restrict__ out_col,
                                                     float4* __restrict__ out_nrm, uint32_t* __restrict__ rc) {
@@ -1634,8 +1791,6 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
   __shared__ int32_t slot_lds[8 * 64];
   BlockCache<DIAG> bc;
   bc.init(slot_lds + threadIdx.x, ra.mark_bkt, ra.mark_blk, ra.march_stats);
-  OccCache oc;
-  occ_cache_init(oc);
   TailState s;
   s.pix = 0;
   s.ray = s.last_sdf = s.last_alpha = 0.0f;
@@ -1676,7 +1831,7 @@ __global__ __launch_bounds__(64) void k_render_tail(TableView t, RenderArgs ra, 
     }
     if (!__any(have)) break;
     if (!have) continue;
-    if (tail_iteration<F16, DIAG, REF>(t, ra, vox, col, bc, oc, s, gl, gbase, ts)) {
+    if (tail_iteration<F16, DIAG, REF>(t, ra, vox, col, bc, s, gl, gbase, ts)) {
       if (gl == 0) tail_write<DEV>(ra, s, out_depth, out_pos, out_col, out_nrm, hit, miss);
       have = false;
     }
@@ -3321,8 +3476,8 @@ void launch_render(hipStream_t s, bool fp16, const TableView& t, const DevState*
   const bool dev = ra.Mdev != nullptr && !diag;
 #define VSG_RENDER_DEV(F, D, R, DV)                                                                             \
   do {                                                                                                          \
-    hipLaunchKernelGGL((k_render<F, D, R, DV>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr, nrm, \
-                       rc);                                                                                     \
+    hipLaunchKernelGGL((k_render<F, D, R, DV>), grid, dim3(64), 0, s, t, st, ra, vox, col, depth, pos, colr,    \
+                       nrm, rc);                                                                                \
     if (ra.budget != 0u)                                                                                        \
       hipLaunchKernelGGL((k_render_tail<F, D, R, DV>), tgrid, dim3(64), 0, s, t, ra, vox, col, depth, pos, colr,  \
                          nrm, rc_tail);                                                                         \
