@@ -47,7 +47,7 @@ constexpr uint32_t kMarchStats = 8;
 constexpr uint32_t kSuperTile = 4;
 
 // Default march steps per lane before a ray moves to the tail pass.
-constexpr uint32_t kDefaultStepBudget = 48;
+constexpr uint32_t kDefaultStepBudget = 32;
 
 // Workgroups of the tail pass for a W x H render.
 uint32_t tail_workgroups(int32_t W, int32_t H);

@@ -1329,7 +1329,7 @@ __device__ __forceinline__ bool cell_skip(const TableView& t, Cache& bc, const R
 template <bool F16, bool DIAG, bool REF, bool DEV, class Cache>
 __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
                                             const uint32_t* __restrict__ col, uint32_t x, uint32_t y, Cache& bc,
-                                            uint32_t budget) {
+                                            uint32_t budget, float zskip = 0.0f) {
   RayHit h;
   RaySetup rs;
   if (!ray_setup<DEV>(ra, x, y, h, rs)) return h;
@@ -1356,7 +1356,17 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
   uint32_t cell_key = 0u;
   OccCache oc;
   occ_cache_init(oc);
-  bool marching = ray < ray_end, parked = false;
+  if (zskip > 0.0f) {
+    // every sample at a camera depth below zskip is invalid (bundle_start): on to the
+    // first sample at or beyond the parameter of that depth, taken slightly low
+    const float tz = (zskip * h.d2r) * (1.0f - 1.0f / 1048576.0f);
+    if (tz > ray) {
+      const float from = ray;
+      ray = replay_steps(ray, fminf(tz, ray_end), inc, &stalled);
+      if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
+    }
+  }
+  bool marching = ray < ray_end && !stalled, parked = false;
   while (true) {
     if (!__any(marching || parked)) break;
     if (marching && budget != 0u && n_steps >= budget) {
@@ -1487,6 +1497,88 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
   return h;
 }
 
+// Bundle look-ahead of an 8x8 tile (one wave): lane k tests the slab of the tile's
+// frustum between camera depths z + k dz and z + (k + 1) dz against the occupancy map.
+// The samples of every ray of the tile at a camera depth below the first slab that can
+// reach an allocated block are invalid (their corner voxels lie in no allocated block),
+// so each ray may start at its first sample at or beyond that depth.  The points of
+// the tile's rays at camera depth z in [za, zb] lie in the convex hull of the four
+// corner rays' points at za and zb (the pixel direction (u, w, 1) is linear in the
+// pixel, and the pose's rotation is orthonormal: checked to 1e-5, with a matching
+// margin); the hull's bounding box with the corner margins of skip_test gives a block
+// range tested like segment_clear.  Returns 0 when nothing is proven.
+constexpr float kBundleSlab = 8.0f;  // slab thickness in voxels
+constexpr int kBundleRounds = 4;     // look-ahead rounds (64 slabs each)
+
+template <bool DEV>
+__device__ __forceinline__ float bundle_start(const TableView& t, const RenderArgs& ra, uint32_t tx, uint32_t ty) {
+  const float MINF = 1e10f;
+  const float minD = ra.minD, maxD = ra.maxD;
+  if (minD == 0.0f || minD == MINF || maxD == 0.0f || maxD == MINF || !(minD < maxD) || ra.win_obs) return 0.0f;
+  float M[12];
+  render_pose<DEV>(ra, M);
+  float cam[3];
+  render_cam<DEV>(ra, cam);
+  // rotation orthonormal to 1e-5 (else the hull argument does not hold)
+  float dev = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int j = i; j < 3; ++j) {
+      const float d = M[i] * M[j] + M[4 + i] * M[4 + j] + M[8 + i] * M[8 + j] - (i == j ? 1.0f : 0.0f);
+      dev = fmaxf(dev, fabsf(d));
+    }
+  }
+  if (!(dev <= 1e-5f)) return 0.0f;
+  const float v = ra.voxel, inv_v = 1.0f / v;
+  const float u0 = ((float)(8u * tx) - ra.cx) / ra.fx, u1 = ((float)(8u * tx + 7u) - ra.cx) / ra.fx;
+  const float w0 = ((float)(8u * ty) - ra.cy) / ra.fy, w1 = ((float)(8u * ty + 7u) - ra.cy) / ra.fy;
+  // world direction per unit camera depth of the four corner rays
+  float dir[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const float u = (c & 1) ? u1 : u0, w = (c & 2) ? w1 : w0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dir[c][a] = M[4 * a] * u + M[4 * a + 1] * w + M[4 * a + 2];
+  }
+  const float dmax = fmaxf(fmaxf(fabsf(u0), fabsf(u1)), fmaxf(fabsf(w0), fabsf(w1))) + 1.0f;
+  const float mag = fmaxf(fmaxf(fabsf(cam[0]), fabsf(cam[1])), fabsf(cam[2])) + 2.0f * maxD * dmax;
+  const float eps = 0.02f + 4e-5f * mag * inv_v;
+  const float mq = 1.125f + eps;
+  const uint32_t lane = threadIdx.x & 63u;
+  const float dz = kBundleSlab * v;
+  float z = minD;
+  for (int round = 0; round < kBundleRounds; ++round) {
+    const float za = z + (float)lane * dz, zb = z + (float)(lane + 1u) * dz;
+    SegRange sr;
+    sr.fits = true;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      float lo = INFINITY, hi = -INFINITY;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const float pa = cam[a] + dir[c][a] * za, pb = cam[a] + dir[c][a] * zb;
+        lo = fminf(lo, fminf(pa, pb));
+        hi = fmaxf(hi, fmaxf(pa, pb));
+      }
+      const float qa = lo * inv_v, qb = hi * inv_v;
+      sr.b0[a] = cvt_i32(floorf((qa - mq) * 0.125f));
+      sr.b1[a] = cvt_i32(floorf((qb + mq) * 0.125f));
+      sr.fits = sr.fits && __builtin_isfinite(qa) && __builtin_isfinite(qb) &&
+                (uint32_t)sr.b1[a] - (uint32_t)sr.b0[a] <= 4u;
+    }
+    uint64_t ow[8];
+    segment_words(t, sr, ow);
+    const bool reach = za < maxD && !segment_clear(sr, ow);
+    const uint64_t bm = __ballot(reach);
+    const uint32_t f = bm != 0ull ? (uint32_t)__builtin_ctzll(bm) : 64u;
+    if (f == 0u) break;
+    z = z + (float)f * dz;
+    if (!(z < maxD)) break;
+  }
+  return z > minD ? z : 0.0f;
+}
+
 template <bool F16, bool DIAG, bool REF, bool DEV>
 __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* __restrict__ st, RenderArgs ra,
                                                const void* __restrict__ vox, const uint32_t* __restrict__ col,
@@ -1513,6 +1605,8 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
     }
     return;
   }
+  // the whole wave: the tile's rays skip what the bundle look-ahead proves invalid
+  const float zskip = bundle_start<DEV>(t, ra, tx, ty);
   const uint32_t x = tx * 8u + (threadIdx.x & 7u);
   const uint32_t y = ty * 8u + (threadIdx.x >> 3);
   uint32_t hit = 0, miss = 0;
@@ -1524,7 +1618,7 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
     __shared__ int32_t slot_lds[8 * 64];
     BlockCache<DIAG> bc;
     bc.init(slot_lds + threadIdx.x, ra.mark_bkt, ra.mark_blk, ra.march_stats);
-    const RayHit h = march_ray<F16, DIAG, REF, DEV>(t, ra, vox, col, x, y, bc, ra.budget);
+    const RayHit h = march_ray<F16, DIAG, REF, DEV>(t, ra, vox, col, x, y, bc, ra.budget, zskip);
     deferred = h.deferred;
     tr.pix = (uint32_t)pix;
     tr.ray = h.ray;
