@@ -149,7 +149,8 @@ struct vsg_render {
   size_t rc_words = 0;
   size_t rc_march_words = 0;     // the march's slice of rc (the tail pass's follows it)
   uint32_t last_budget = 0;      // step budget of the last render (0: the tail pass did not run)
-  void* tail = nullptr;          // tail queue: TailRay per pixel
+  void* tail = nullptr;          // tail queue: 8 segments (one per XCD) of tail_seg TailRays
+  uint32_t tail_seg = 0;
   uint32_t* tail_ctl = nullptr;  // kTailCtlWords queue controls of the tail pass
   uint32_t budget = 0;           // march steps per lane before the tail pass takes the ray over
   uint32_t calls = 0;
@@ -746,7 +747,8 @@ vsg_status vsg_render_create(int32_t width, int32_t height, vsg_render** out) {
   if (st == VSG_OK) st = dmalloc(&r->rc, r->rc_words * sizeof(uint32_t), "render counters");
   if (st == VSG_OK && hipMemset(r->rc, 0, r->rc_words * sizeof(uint32_t)) != hipSuccess)
     st = fail(VSG_ERR_HIP, "hipMemset");
-  if (st == VSG_OK) st = dmalloc(&r->tail, n * sizeof(TailRay), "tail queue");
+  r->tail_seg = tail_segment(width, height);
+  if (st == VSG_OK) st = dmalloc(&r->tail, (size_t)8 * r->tail_seg * sizeof(TailRay), "tail queue");
   if (st == VSG_OK) st = dmalloc(&r->tail_ctl, kTailCtlWords * sizeof(uint32_t), "tail queue controls");
   if (st == VSG_OK) st = dmalloc(&r->cam_buf, 4 * sizeof(float), "camera position");
   if (st == VSG_OK && hipMemset(r->tail_ctl, 0, kTailCtlWords * sizeof(uint32_t)) != hipSuccess)
@@ -834,6 +836,7 @@ vsg_status render_launch(vsg_render* r, vsg_scene* s, RenderArgs& ra, const vsg_
   ra.budget = r->budget;
   ra.tail = r->tail;
   ra.tail_ctl = r->tail_ctl;
+  ra.tail_seg = r->tail_seg;
   hipStream_t strm = r->has_stream ? r->stream : s->stream;
   if (strm != s->stream) {  // order after the scene's pending work
     VSG_HIP(hipEventRecord(r->ev, s->stream));

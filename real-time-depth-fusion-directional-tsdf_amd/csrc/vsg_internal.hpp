@@ -23,8 +23,9 @@ struct RenderArgs {
                           // skips, deferred rays, tail windows, tail skips
   int32_t mode;           // vsg_sampling: 0 the reference's trilinear sampling, 1 standard trilinear
   uint32_t budget;        // steps per lane before a ray moves to the cooperative tail pass (0: none)
-  void* tail;             // tail queue (TailRay[W*H]) when budget > 0
+  void* tail;             // tail queue (8 segments of tail_seg TailRays, one per XCD) when budget > 0
   uint32_t* tail_ctl;     // queue controls (kTailCtlWords), counts zero between renders
+  uint32_t tail_seg;      // entries per segment of the tail queue
   const float* win_obs;   // when non-null: observed depth per pixel; a ray marches only over depths within
   float win;              // win of it (and within [minD, maxD]); a pixel without one (<= 0) is not marched
 };
@@ -37,8 +38,10 @@ struct TailRay {
   uint32_t pad[3];
 };
 
-// Queue control words of a renderer's tail pass: {count, next, workgroups done}.
-constexpr uint32_t kTailCtlWords = 4;
+// Queue control words of a renderer's tail pass: per XCD x the count of rays the march
+// deferred into segment x (word x) and the next entry handed out dynamically (word
+// 8 + x); word 16 counts the tail pass's finished workgroups.
+constexpr uint32_t kTailCtlWords = 20;
 
 // Counters of the march statistics (diagnostics).
 constexpr uint32_t kMarchStats = 8;
@@ -49,8 +52,12 @@ constexpr uint32_t kSuperTile = 4;
 // Default march steps per lane before a ray moves to the tail pass.
 constexpr uint32_t kDefaultStepBudget = 32;
 
-// Workgroups of the tail pass for a W x H render.
+// Workgroups of the tail pass for a W x H render (a multiple of 8).
 uint32_t tail_workgroups(int32_t W, int32_t H);
+
+// Entries of one XCD's segment of the tail queue for a W x H render: the pixels of the
+// super-tiles dealt to one XCD, at most.
+uint32_t tail_segment(int32_t W, int32_t H);
 
 // Workgroups of the march (and of the fused raycast) for a W x H render.
 uint32_t render_grid(int32_t W, int32_t H);

@@ -1651,11 +1651,15 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
   if (dm != 0ull) {
     uint32_t base = 0;
     if (DIAG && threadIdx.x == 0 && ra.march_stats) atomicAdd(&ra.march_stats[5], (uint32_t)__popcll(dm));
-    if (threadIdx.x == 0) base = atomicAdd(&ra.tail_ctl[0], (uint32_t)__popcll(dm));
+    // the segment of this workgroup's XCD (the tail pass's workgroups on the same XCD
+    // take them, where the march left the blocks in L2)
+    const uint32_t xcd = blockIdx.x & 7u;
+    if (threadIdx.x == 0) base = atomicAdd(&ra.tail_ctl[xcd], (uint32_t)__popcll(dm));
     base = __shfl(base, 0, 64);
     if (deferred) {
       const uint32_t rank = (uint32_t)__popcll(dm & ((1ull << threadIdx.x) - 1ull));
-      if (base + rank < (uint32_t)ra.W * (uint32_t)ra.H) reinterpret_cast<TailRay*>(ra.tail)[base + rank] = tr;
+      if (base + rank < ra.tail_seg)
+        reinterpret_cast<TailRay*>(ra.tail)[(size_t)xcd * ra.tail_seg + base + rank] = tr;
     }
   }
   // one wave per workgroup: one write of its hit / miss counts (summed when read)
@@ -1875,11 +1879,14 @@ __global__ __launch_bounds__(64, 4) void k_render_tail(TableView t, RenderArgs r
   const int lane = threadIdx.x;
   const int gl = lane & (G - 1);
   const int gbase = lane & ~(G - 1);
-  const uint32_t n = min(ra.tail_ctl[0], (uint32_t)ra.W * (uint32_t)ra.H);
-  const TailRay* __restrict__ queue = reinterpret_cast<const TailRay*>(ra.tail);
-  constexpr uint32_t kGroups = 64u / (uint32_t)G;  // groups per wave
-  const uint32_t n_static = gridDim.x * kGroups;     // entries handed out without atomics
-  uint32_t next_static = blockIdx.x * kGroups + (uint32_t)(gbase / G);
+  // workgroup b runs on XCD b % 8 and takes segment b % 8 of the queue (the rays the
+  // march deferred on that XCD)
+  const uint32_t xcd = blockIdx.x & 7u;
+  const uint32_t n = min(ra.tail_ctl[xcd], ra.tail_seg);
+  const TailRay* __restrict__ queue = reinterpret_cast<const TailRay*>(ra.tail) + (size_t)xcd * ra.tail_seg;
+  constexpr uint32_t kGroups = 64u / (uint32_t)G;         // groups per wave
+  const uint32_t n_static = (gridDim.x >> 3) * kGroups;   // entries handed out without atomics
+  uint32_t next_static = (blockIdx.x >> 3) * kGroups + (uint32_t)(gbase / G);
   uint32_t hit = 0, miss = 0;
   TailStats ts;
   __shared__ int32_t slot_lds[8 * 64];
@@ -1904,7 +1911,7 @@ __global__ __launch_bounds__(64, 4) void k_render_tail(TableView t, RenderArgs r
       const uint64_t need = __ballot(want && gl == 0);
       if (need != 0ull) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ra.tail_ctl[1], (uint32_t)__popcll(need));
+        if (lane == 0) base = atomicAdd(&ra.tail_ctl[8 + xcd], (uint32_t)__popcll(need));
         base = __shfl(base, 0, 64);
         if (want) e = n_static + base + (uint32_t)__popcll(need & ((1ull << gbase) - 1ull));
       }
@@ -1950,10 +1957,9 @@ __global__ __launch_bounds__(64, 4) void k_render_tail(TableView t, RenderArgs r
     rc[2 * blockIdx.x + 1] = miss;
     // the last workgroup to finish clears the queue controls for the next render
     __threadfence();
-    if (atomicAdd(&ra.tail_ctl[2], 1u) == gridDim.x - 1u) {
-      ra.tail_ctl[0] = 0u;
-      ra.tail_ctl[1] = 0u;
-      ra.tail_ctl[2] = 0u;
+    if (atomicAdd(&ra.tail_ctl[16], 1u) == gridDim.x - 1u) {
+#pragma unroll
+      for (int i = 0; i < 17; ++i) ra.tail_ctl[i] = 0u;
     }
   }
 }
@@ -3447,7 +3453,14 @@ uint32_t render_grid(int32_t W, int32_t H) {
 
 uint32_t tail_workgroups(int32_t W, int32_t H) {
   const uint64_t n = ((uint64_t)W * (uint64_t)H + 63u) / 64u;
-  return (uint32_t)(n < 1u ? 1u : (n > 4096u ? 4096u : n));
+  const uint32_t c = (uint32_t)(n > 4096u ? 4096u : n);
+  return c < 8u ? 8u : (c + 7u) / 8u * 8u;
+}
+
+uint32_t tail_segment(int32_t W, int32_t H) {
+  const uint32_t tiles_x = (uint32_t)((W + 7) / 8), tiles_y = (uint32_t)((H + 7) / 8);
+  const uint32_t nst = ((tiles_x + kSuperTile - 1u) / kSuperTile) * ((tiles_y + kSuperTile - 1u) / kSuperTile);
+  return (nst + 7u) / 8u * kSuperTile * kSuperTile * 64u;
 }
 
 template <class T> static void launch_scan(hipStream_t s, const T* in, uint32_t n, uint32_t* sums, uint32_t* out) {
