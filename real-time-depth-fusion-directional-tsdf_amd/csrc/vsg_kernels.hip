@@ -908,6 +908,42 @@ __device__ __forceinline__ int corner_gather(const void* __restrict__ vox_, cons
   return kSampleValid;
 }
 
+// The distance of corner_gather<F16, false> from the eight corner words w, already read.
+template <bool F16>
+__device__ __forceinline__ int interp_words(const Corners& cn, const typename Store<F16>::word w[8], float* dist) {
+  using S = Store<F16>;
+  const float wx = cn.wx, wy = cn.wy, wz = cn.wz;
+  const float ux = 1.0f - wx, uy = 1.0f - wy, uz = 1.0f - wz;
+  float d = 0.0f, wsum = 0.0f;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const bool ok = !((cn.none >> i) & 1u);
+    const float wi = ((i & 1) ? wx : ux) * ((i & 2) ? wy : uy) * ((i & 4) ? wz : uz);
+    const float wk = ok && S::weight(w[i]) != 0u ? wi : 0.0f;
+    d += wk * (ok ? S::sdf(w[i]) : 0.0f);
+    wsum += wk;
+  }
+  if (wsum <= 1e-6f) return kSampleInvalid;
+  *dist = d * (1.0f / wsum);
+  return kSampleValid;
+}
+
+// Two samples' distances (corner_gather<F16, false> of each) with all sixteen corner
+// reads issued before either interpolation.
+template <bool F16>
+__device__ __forceinline__ void corner_gather2(const void* __restrict__ vox_, const Corners& ca, const Corners& cb,
+                                               int* sa, float* da, int* sb, float* db) {
+  using S = Store<F16>;
+  const typename S::word* vox = reinterpret_cast<const typename S::word*>(vox_);
+  typename S::word wa[8], wb[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) wa[i] = vox[ca.vi[i]];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) wb[i] = vox[cb.vi[i]];
+  *sa = interp_words<F16>(ca, wa, da);
+  *sb = interp_words<F16>(cb, wb, db);
+}
+
 template <bool F16, bool COLOR, bool REF, class Cache>
 __device__ __forceinline__ int trilinear(const TableView& t, Cache& bc, const void* __restrict__ vox_,
                                          const uint32_t* __restrict__ col, float px, float py, float pz, float v,
@@ -1719,21 +1755,24 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
   float& last_alpha = s.last_alpha;
   uint32_t& last_w = s.last_w;
   bool done = false;
-  // sample gl of the window: gl more additions of the step
-  float r = ray;
-  bool in = true;
-  for (int k = 0; k < gl; ++k) {
-    const float nx = r + rs.inc;
-    if (nx == r) {
-      in = false;
+  // samples 2 gl and 2 gl + 1 of the window (2 kTailGroup consecutive samples): 2 gl
+  // and 2 gl + 1 more additions of the step
+  float r0 = ray;
+  bool in0 = true;
+  for (int k = 0; k < 2 * gl; ++k) {
+    const float nx = r0 + rs.inc;
+    if (nx == r0) {
+      in0 = false;
       break;
     }
-    r = nx;
+    r0 = nx;
   }
-  in = in && r < rs.ray_end;
+  const float r1 = r0 + rs.inc;
+  bool in1 = in0 && r1 != r0 && r1 < rs.ray_end;
+  in0 = in0 && r0 < rs.ray_end;
   // Look-ahead: lane gl tests segment [T_gl, T_gl+1] of the ray, T_k = ray + k * S.
   // Its occupancy words are read together with the block lookups of the lane's
-  // window sample (one memory round trip for both); the voxels are read only when
+  // window samples (one memory round trip for both); the voxels are read only when
   // the window is evaluated.
   const float S = kLookSeg * rs.v;
   const float ta = ray + (float)gl * S, tb = ray + (float)(gl + 1) * S;
@@ -1741,8 +1780,10 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
   segment_range(rs.sk, ta, tb, sr);
   uint64_t ow[8];
   segment_words(t, sr, ow);
-  Corners cn;
-  if (in) corner_prep<REF>(t, bc, rs.camx + r * h.wdx, rs.camy + r * h.wdy, rs.camz + r * h.wdz, rs.v, cn);
+  Corners c0, c1;
+  if (in0) corner_prep<REF>(t, bc, rs.camx + r0 * h.wdx, rs.camy + r0 * h.wdy, rs.camz + r0 * h.wdz, rs.v, c0);
+  c1 = c0;
+  if (in1) corner_prep<REF>(t, bc, rs.camx + r1 * h.wdx, rs.camy + r1 * h.wdy, rs.camz + r1 * h.wdz, rs.v, c1);
   {
     // the samples before the first segment that can reach an allocated block are
     // invalid: the ray moves on to its first sample at or beyond that segment
@@ -1761,27 +1802,36 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
   }
   {
     if (gl == 0) ts.n_window++;
-    float dist = 0.0f;
-    bool valid = false;
-    if (in) {
-      valid = corner_gather<F16, false>(vox, col, cn, &dist, nullptr) == kSampleValid;
-      ts.n_eval++;
+    float d0 = 0.0f, d1 = 0.0f;
+    int st0 = kSampleInvalid, st1 = kSampleInvalid;
+    if (in0) {
+      corner_gather2<F16>(vox, c0, c1, &st0, &d0, &st1, &d1);
+      ts.n_eval += in1 ? 2u : 1u;
     }
-    // the sample before: lane gl - 1's, or the carried state for the first lane
-    float p_sdf = __shfl_up(dist, 1, G), p_r = __shfl_up(r, 1, G);
-    int p_v = __shfl_up(valid ? 1 : 0, 1, G);
+    const bool v0 = in0 && st0 == kSampleValid, v1 = in1 && st1 == kSampleValid;
+    // the sample before sample 2 gl: lane gl - 1's second, or the carried state for the
+    // first lane; before sample 2 gl + 1: the lane's first
+    float p_sdf = __shfl_up(d1, 1, G), p_r = __shfl_up(r1, 1, G);
+    int p_v = __shfl_up(v1 ? 1 : 0, 1, G);
     if (gl == 0) {
       p_sdf = last_sdf;
       p_r = last_alpha;
       p_v = last_w ? 1 : 0;
     }
-    const bool cand = valid && p_v && p_sdf < 0.0f && dist > 0.0f && fabsf(p_sdf - dist) < rs.th &&
-                      fabsf(dist) < rs.th;
-    const uint32_t gm = (uint32_t)((__ballot(cand) >> gbase) & ((1ull << G) - 1ull));
-    const uint32_t gi = (uint32_t)((__ballot(in) >> gbase) & ((1ull << G) - 1ull));
-    if (gm != 0u) {
-      const int f = __builtin_ctz(gm);
-      const float a0 = __shfl(p_r, f, G), ad0 = __shfl(p_sdf, f, G), b0 = __shfl(r, f, G), bd0 = __shfl(dist, f, G);
+    const bool cand0 = v0 && p_v && p_sdf < 0.0f && d0 > 0.0f && fabsf(p_sdf - d0) < rs.th && fabsf(d0) < rs.th;
+    const bool cand1 = v1 && v0 && d0 < 0.0f && d1 > 0.0f && fabsf(d0 - d1) < rs.th && fabsf(d1) < rs.th;
+    const uint32_t g0 = (uint32_t)((__ballot(cand0) >> gbase) & ((1ull << G) - 1ull));
+    const uint32_t g1 = (uint32_t)((__ballot(cand1) >> gbase) & ((1ull << G) - 1ull));
+    const uint32_t gi0 = (uint32_t)((__ballot(in0) >> gbase) & ((1ull << G) - 1ull));
+    const uint32_t gi1 = (uint32_t)((__ballot(in1) >> gbase) & ((1ull << G) - 1ull));
+    if ((g0 | g1) != 0u) {
+      // the first candidate in sample order: lane f's first sample, else its second
+      const int f = __builtin_ctz(g0 | g1);
+      const bool first = (g0 >> f) & 1u;
+      const float a0 = first ? __shfl(p_r, f, G) : __shfl(r0, f, G);
+      const float ad0 = first ? __shfl(p_sdf, f, G) : __shfl(d0, f, G);
+      const float b0 = first ? __shfl(r0, f, G) : __shfl(r1, f, G);
+      const float bd0 = first ? __shfl(d0, f, G) : __shfl(d1, f, G);
       // findIntersectionBisection (RayCastRender.cu:229-271), the same in every lane of the group
       float a = a0, ad = ad0, bb = b0, bd = bd0, c = 0.0f;
       uint32_t rgb = 0;
@@ -1823,10 +1873,13 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
         ray = nx;
       }
     } else {
-      // no accepted crossing: continue after the window's last sample
-      const int L = 31 - __builtin_clz(gi | 1u);
-      const float rL = __shfl(r, L, G), dL = __shfl(dist, L, G);
-      const int vL = __shfl(valid ? 1 : 0, L, G);
+      // no accepted crossing: continue after the window's last sample (lane L's second
+      // when it has one, else its first)
+      const int L = 31 - __builtin_clz(gi0 | 1u);
+      const bool second = (gi1 >> L) & 1u;
+      const float rL = second ? __shfl(r1, L, G) : __shfl(r0, L, G);
+      const float dL = second ? __shfl(d1, L, G) : __shfl(d0, L, G);
+      const int vL = second ? __shfl(v1 ? 1 : 0, L, G) : __shfl(v0 ? 1 : 0, L, G);
       if (vL) {
         last_sdf = dL;
         last_alpha = rL;
@@ -1835,7 +1888,7 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
         last_w = 0u;
       }
       const float nx = rL + rs.inc;
-      if (L < G - 1 || nx == rL || !(nx < rs.ray_end)) done = true;
+      if (L < G - 1 || !second || nx == rL || !(nx < rs.ray_end)) done = true;
       ray = nx;
     }
   }
