@@ -1337,18 +1337,33 @@ __device__ __forceinline__ bool segment_clear(const SegRange& sr, const uint64_t
 // last sample matters for what follows (it is the lower end of the next bracket).
 // *last: key of the last cell range found to hold a positive voxel (consecutive
 // samples mostly share it; a key collision only costs a skipped test).
-template <class Cache>
-__device__ __forceinline__ bool cell_skip(const TableView& t, Cache& bc, const RaySkip& sk, float wdx, float wdy,
-                                          float wdz, float ray, uint32_t* last, float* t0, float* t1) {
+// The cells around the sample at ray parameter `ray` (as in cell_skip) and their key.
+struct CellRange {
+  int32_t cx0, cx1, cy0, cy1, cz0, cz1;
+  uint32_t key;
+};
+
+__device__ __forceinline__ void cell_range(const RaySkip& sk, float wdx, float wdy, float wdz, float ray,
+                                           CellRange& cr) {
   const float mq = 1.125f + sk.eps;
   const float qx = (sk.camx + ray * wdx) * sk.inv_v, qy = (sk.camy + ray * wdy) * sk.inv_v,
               qz = (sk.camz + ray * wdz) * sk.inv_v;
-  const int32_t cx0 = cvt_i32(floorf((qx - mq) * 0.25f)), cx1 = cvt_i32(floorf((qx + mq) * 0.25f));
-  const int32_t cy0 = cvt_i32(floorf((qy - mq) * 0.25f)), cy1 = cvt_i32(floorf((qy + mq) * 0.25f));
-  const int32_t cz0 = cvt_i32(floorf((qz - mq) * 0.25f)), cz1 = cvt_i32(floorf((qz + mq) * 0.25f));
-  const uint32_t key = ((uint32_t)cx0 & 0x1FFu) | (((uint32_t)cy0 & 0x1FFu) << 9) | (((uint32_t)cz0 & 0x1FFu) << 18) |
-                       ((uint32_t)(cx1 - cx0) << 27) | ((uint32_t)(cy1 - cy0) << 28) | ((uint32_t)(cz1 - cz0) << 29) |
-                       0x80000000u;
+  cr.cx0 = cvt_i32(floorf((qx - mq) * 0.25f));
+  cr.cx1 = cvt_i32(floorf((qx + mq) * 0.25f));
+  cr.cy0 = cvt_i32(floorf((qy - mq) * 0.25f));
+  cr.cy1 = cvt_i32(floorf((qy + mq) * 0.25f));
+  cr.cz0 = cvt_i32(floorf((qz - mq) * 0.25f));
+  cr.cz1 = cvt_i32(floorf((qz + mq) * 0.25f));
+  cr.key = ((uint32_t)cr.cx0 & 0x1FFu) | (((uint32_t)cr.cy0 & 0x1FFu) << 9) | (((uint32_t)cr.cz0 & 0x1FFu) << 18) |
+           ((uint32_t)(cr.cx1 - cr.cx0) << 27) | ((uint32_t)(cr.cy1 - cr.cy0) << 28) |
+           ((uint32_t)(cr.cz1 - cr.cz0) << 29) | 0x80000000u;
+}
+
+template <class Cache>
+__device__ __forceinline__ bool cell_skip(const TableView& t, Cache& bc, const RaySkip& sk, const CellRange& cr,
+                                          uint32_t* last, float* t0, float* t1) {
+  const int32_t cx0 = cr.cx0, cx1 = cr.cx1, cy0 = cr.cy0, cy1 = cr.cy1, cz0 = cr.cz0, cz1 = cr.cz1;
+  const uint32_t key = cr.key;
   if (key == *last) return false;
   if (!bc.cells_free(t, cx0, cx1, cy0, cy1, cz0, cz1)) {
     *last = key;
@@ -1416,7 +1431,12 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
     if (marching) {
       n_steps++;
       bool step = true;
-      {
+      CellRange cr;
+      cell_range(sk, wdx, wdy, wdz, ray, cr);
+      // the cells of the last range found to hold a positive voxel (hence in allocated
+      // blocks): no skip applies, the sample is evaluated (a key collision only costs an
+      // evaluation the skips would have avoided)
+      if (cr.key != cell_key) {
         float t0, t1;
         const int occ = skip_test(t, oc, sk, wdx, wdy, wdz, ray, &t0, &t1);
         const bool inside = t0 <= ray && ray < t1;  // false for a NaN interval
@@ -1429,8 +1449,7 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
           if (DIAG) n_replay += (uint32_t)((ray - from) / inc + 0.5f);
           if (stalled || !(ray < ray_end)) marching = false;
           step = false;
-        } else if (occ == 2 && cell_skip(t, bc, sk, wdx, wdy, wdz, ray, &cell_key, &t0, &t1) && t0 <= ray &&
-                   ray < t1) {
+        } else if (occ == 2 && cell_skip(t, bc, sk, cr, &cell_key, &t0, &t1) && t0 <= ray && ray < t1) {
           n_cell++;
           if (!(t1 < ray_end)) {
             // every remaining sample lies in the box: no crossing can be accepted
