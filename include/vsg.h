@@ -232,7 +232,7 @@ vsg_status vsg_render_set_sampling(vsg_render* render, int32_t mode);
 
 /* Steps (skips or sample evaluations) a ray marches with one lane before the
  * cooperative tail pass continues it with several lanes; 0 marches every ray with
- * one lane to the end.  Results do not depend on it (default 32).  No counterpart
+ * one lane to the end.  Results do not depend on it (default 28).  No counterpart
  * in the reference: the tail pass is this library's load balancing. */
 vsg_status vsg_render_set_step_budget(vsg_render* render, uint32_t steps);
 

@@ -50,7 +50,7 @@ constexpr uint32_t kMarchStats = 8;
 constexpr uint32_t kSuperTile = 4;
 
 // Default march steps per lane before a ray moves to the tail pass.
-constexpr uint32_t kDefaultStepBudget = 32;
+constexpr uint32_t kDefaultStepBudget = 28;
 
 // Workgroups of the tail pass for a W x H render (a multiple of 8).
 uint32_t tail_workgroups(int32_t W, int32_t H);

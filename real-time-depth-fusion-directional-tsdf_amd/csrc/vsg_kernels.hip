@@ -1736,7 +1736,7 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // with one atomic per wave (a single queue counter would otherwise serialise the
 // fetches), until the queue is empty.  The last workgroup clears the queue controls
 // for the next render.
-constexpr int kTailGroup = 16;
+constexpr int kTailGroup = 8;
 // Look-ahead segment of the tail pass, in voxels along the ray (at most 29: see
 // segment_empty).
 constexpr float kLookSeg = 16.0f;
