@@ -539,12 +539,15 @@ __global__ __launch_bounds__(512) void k_integrate(TableView t, DevState* st, Fr
       const float3 D = xform_point(M, Px, Py, Pz);
       const float sdx = vwx - D.x, sdy = vwy - D.y, sdz = vwz - D.z;
       float sdf = sqrtf(sdx * sdx + sdy * sdy + sdz * sdz);
+      // the truncation test (sdf < -tr || sdf > tr) depends on |sdf| only: the same as
+      // sdf > tr before the sign is applied (NaN passes both), so the two distances that
+      // decide the sign are computed only for voxels within the band
+      if (sdf > tr) break;
       const float cdx = D.x - camx, cdy = D.y - camy, cdz = D.z - camz;
       const float cam_dist = sqrtf(cdx * cdx + cdy * cdy + cdz * cdz);
       const float vox_dist =
           sqrtf((vwx - camx) * (vwx - camx) + (vwy - camy) * (vwy - camy) + (vwz - camz) * (vwz - camz));
       if (vox_dist < cam_dist) sdf = -sdf;
-      if (sdf < -tr || sdf > tr) break;
       sdf = fmaxf(-tr, fminf(tr, sdf));
       const size_t vi = (size_t)ptr * kVox + (size_t)idx;
       const typename S::word w = vox[vi];
