@@ -1741,7 +1741,7 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // for the next render.
 constexpr int kTailGroup = 8;
 // Look-ahead segment of the tail pass, in voxels along the ray (at most 29: see
-// segment_empty).
+// segment_range).
 constexpr float kLookSeg = 16.0f;
 
 // March state of a ray in the tail pass, one per group of kTailGroup lanes (the same in
@@ -1778,17 +1778,11 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
   uint32_t& last_w = s.last_w;
   bool done = false;
   // samples 2 gl and 2 gl + 1 of the window (2 kTailGroup consecutive samples): 2 gl
-  // and 2 gl + 1 more additions of the step
-  float r0 = ray;
-  bool in0 = true;
-  for (int k = 0; k < 2 * gl; ++k) {
-    const float nx = r0 + rs.inc;
-    if (nx == r0) {
-      in0 = false;
-      break;
-    }
-    r0 = nx;
-  }
+  // and 2 gl + 1 more additions of the step (an addition that leaves the value
+  // unchanged ends the ray there)
+  bool stl0;
+  const float r0 = advance_steps(ray, rs.inc, 2u * (uint32_t)gl, &stl0);
+  bool in0 = !stl0;
   const float r1 = r0 + rs.inc;
   bool in1 = in0 && r1 != r0 && r1 < rs.ray_end;
   in0 = in0 && r0 < rs.ray_end;

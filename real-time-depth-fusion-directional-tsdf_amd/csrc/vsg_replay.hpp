@@ -87,4 +87,40 @@ __host__ __device__ inline float replay_steps(float r, float target, float inc, 
   return replay_steps(r, target, inc, stalled, &prev);
 }
 
+// The value after k additions r += inc (the same closed form, by binade).  *stalled is
+// set, and the value returned, at the first addition that leaves it unchanged.
+__host__ __device__ inline float advance_steps(float r, float inc, uint32_t k, bool* stalled) {
+  *stalled = false;
+  while (k > 0u) {
+    if (r >= 1.17549435e-38f && inc > 0.0f) {  // r a positive normal number
+      int e;
+      (void)frexpf(r, &e);
+      const int sc = 24 - e;
+      const float qi = ldexpf(inc, sc);
+      const float I = floorf(qi);
+      const float phi = qi - I;
+      if (phi != 0.5f && I < 16777216.0f) {
+        const uint32_t S = (uint32_t)I + (phi > 0.5f ? 1u : 0u);
+        const uint32_t R = (uint32_t)ldexpf(r, sc);
+        if (S > 0u && R + S < 16777216u) {
+          // additions that stay in the binade: R + m S <= 2^24 - 1
+          const uint32_t room = floor_div24(16777215u - R, S);
+          const uint32_t m = k < room ? k : room;
+          r = ldexpf((float)(R + m * S), -sc);
+          k -= m;
+          if (k == 0u) break;
+        }
+      }
+    }
+    const float nx = r + inc;
+    if (nx == r) {
+      *stalled = true;
+      return r;
+    }
+    r = nx;
+    --k;
+  }
+  return r;
+}
+
 }  // namespace vsg

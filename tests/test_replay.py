@@ -1,7 +1,8 @@
 """The march's closed-form replay of step additions (csrc/vsg_replay.hpp) lands on the
 same float as the literal loop `while (r < target) r += inc`, and reports the same
 element before it, for random and edge-case inputs, including binade crossings, ties
-(inc a half-integer number of ulps) and stalls.  Host build of the same header the kernels use."""
+(inc a half-integer number of ulps) and stalls; its advance by k additions lands on the
+same float (and stall) as k literal additions.  Host build of the same header the kernels use."""
 import os
 import subprocess
 
@@ -47,6 +48,30 @@ static int check(float r, float t, float inc) {
   return 0;
 }
 
+static float loop_advance(float r, float inc, uint32_t k, bool* stalled) {
+  *stalled = false;
+  for (uint32_t i = 0; i < k; ++i) {
+    float nx = r + inc;
+    if (nx == r) { *stalled = true; return r; }
+    r = nx;
+  }
+  return r;
+}
+
+static int check_adv(float r, float inc, uint32_t k) {
+  bool s1, s2;
+  float a = vsg::advance_steps(r, inc, k, &s1);
+  float b = loop_advance(r, inc, k, &s2);
+  uint32_t ua, ub;
+  std::memcpy(&ua, &a, 4);
+  std::memcpy(&ub, &b, 4);
+  if (ua != ub || s1 != s2) {
+    std::printf("ADVANCE MISMATCH r=%.9g inc=%.9g k=%u got %.9g (%d) want %.9g (%d)\n", r, inc, k, a, s1, b, s2);
+    return 1;
+  }
+  return 0;
+}
+
 int main() {
   std::mt19937_64 rng(12345);
   std::uniform_real_distribution<double> U(0.0, 1.0);
@@ -87,6 +112,33 @@ int main() {
   bad += check(2.0f, 1.0f, 0.002f);
   bad += check(1e-30f, 0.01f, 0.002f);
   n += 4;
+  // k additions: random, across binade boundaries, ties, stalls
+  for (int i = 0; i < 200000; ++i) {
+    float v = (float)(0.001 + 0.05 * U(rng));
+    float r = (float)(0.01 + 6.0 * U(rng));
+    bad += check_adv(r, v * 0.5f, (uint32_t)(300 * U(rng)));
+    ++n;
+  }
+  for (float e : edges) {
+    for (int i = 0; i < 20000; ++i) {
+      float inc = (float)(0.0005 + 0.01 * U(rng));
+      float r = e - (float)(0.3 * U(rng)) * inc * 64.0f;
+      bad += check_adv(r, inc, (uint32_t)(200 * U(rng)));
+      ++n;
+    }
+  }
+  for (int i = 0; i < 20000; ++i) {
+    float r = (float)(1.0 + U(rng));
+    float inc = (float)((1 + (int)(1000 * U(rng))) + 0.5) * 1.1920929e-7f;
+    bad += check_adv(r, inc, (uint32_t)(130 * U(rng)));
+    ++n;
+  }
+  bad += check_adv(3.0e7f, 0.5f, 10);
+  bad += check_adv(1.0f, 1e-9f, 5);
+  bad += check_adv(1e-30f, 0.002f, 7);
+  bad += check_adv(0.0f, 0.002f, 126);
+  bad += check_adv(1.5f, 0.002f, 0);
+  n += 5;
   std::printf("checked %ld bad %d\n", n, bad);
   return bad ? 1 : 0;
 }
