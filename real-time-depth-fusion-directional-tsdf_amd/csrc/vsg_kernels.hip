@@ -1940,7 +1940,7 @@ __device__ __forceinline__ void tail_write(const RenderArgs& ra, const TailState
 }
 
 template <bool F16, bool DIAG, bool REF, bool DEV>
-__global__ __launch_bounds__(64, 4) void k_render_tail(TableView t, RenderArgs ra, const void* __restrict__ vox,
+__global__ __launch_bounds__(64, 3) void k_render_tail(TableView t, RenderArgs ra, const void* __restrict__ vox,
                                                     const uint32_t* __restrict__ col, float* __restrict__ out_depth,
                                                     float4* __restrict__ out_pos, float4* __restrict__ out_col,
                                                     float4* __restrict__ out_nrm, uint32_t* __restrict__ rc) {
@@ -3521,8 +3521,9 @@ uint32_t render_grid(int32_t W, int32_t H) {
 }
 
 uint32_t tail_workgroups(int32_t W, int32_t H) {
+  // three waves per SIMD (the tail kernel's register budget): 256 CUs x 4 SIMDs x 3
   const uint64_t n = ((uint64_t)W * (uint64_t)H + 63u) / 64u;
-  const uint32_t c = (uint32_t)(n > 4096u ? 4096u : n);
+  const uint32_t c = (uint32_t)(n > 3072u ? 3072u : n);
   return c < 8u ? 8u : (c + 7u) / 8u * 8u;
 }
 
