@@ -1565,7 +1565,7 @@ __device__ __forceinline__ RayHit march_ray(const TableView& t, const RenderArgs
 // pixel, and the pose's rotation is orthonormal: checked to 1e-5, with a matching
 // margin); the hull's bounding box with the corner margins of skip_test gives a block
 // range tested like segment_clear.  Returns 0 when nothing is proven.
-constexpr float kBundleSlab = 8.0f;  // slab thickness in voxels
+constexpr float kBundleSlab = 6.0f;  // slab thickness in voxels
 constexpr int kBundleRounds = 4;     // look-ahead rounds (64 slabs each)
 
 template <bool DEV>
