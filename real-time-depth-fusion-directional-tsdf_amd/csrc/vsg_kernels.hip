@@ -1740,6 +1740,8 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // fetches), until the queue is empty.  The last workgroup clears the queue controls
 // for the next render.
 constexpr int kTailGroup = 8;
+// Widest group of a wave's last rays (tail_drain).
+constexpr int kTailDrainGroup = 16;
 // Look-ahead segment of the tail pass, in voxels along the ray (at most 29: see
 // segment_range).
 constexpr float kLookSeg = 16.0f;
@@ -1759,17 +1761,27 @@ struct TailStats {
   uint32_t n_eval = 0, n_super = 0, n_block = 0, n_replay = 0, n_window = 0, n_tskip = 0;
 };
 
-// One iteration of the tail pass on the group's ray: a skip over samples the occupancy
-// map proves invalid (the look-ahead over kTailGroup segments of kLookSeg voxels), or a
-// window of kTailGroup consecutive samples (lane gl takes the sample gl additions of
-// the step ahead) resolved in order; the first sign change that passes the thresholds
-// is refined.  Returns true when the ray is finished (s.h.found:
-// whether it hit).  Every lane of the group takes part.
-template <bool F16, bool DIAG, bool REF, class Cache>
+// Bit i: predicate p of lane gbase + i of this lane's group of G lanes.
+template <int G>
+__device__ __forceinline__ uint64_t group_bits(bool p, int gbase) {
+  const uint64_t b = __ballot(p);
+  if constexpr (G == 64) {
+    return b;
+  } else {
+    return (b >> gbase) & ((1ull << G) - 1ull);
+  }
+}
+
+// One iteration of the tail pass on the ray of a group of G lanes: a skip over samples
+// the occupancy map proves invalid (the look-ahead over G segments of kLookSeg voxels),
+// or a window of 2 G consecutive samples (lane gl takes samples 2 gl and 2 gl + 1)
+// resolved in order; the first sign change that passes the thresholds is refined.
+// Returns true when the ray is finished (s.h.found: whether it hit).  Every lane of the
+// group takes part.
+template <bool F16, bool DIAG, bool REF, int G, class Cache>
 __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
                                                const uint32_t* __restrict__ col, Cache& bc, TailState& s, int gl,
                                                int gbase, TailStats& ts) {
-  constexpr int G = kTailGroup;
   RayHit& h = s.h;
   const RaySetup& rs = s.rs;
   float& ray = s.ray;
@@ -1777,7 +1789,7 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
   float& last_alpha = s.last_alpha;
   uint32_t& last_w = s.last_w;
   bool done = false;
-  // samples 2 gl and 2 gl + 1 of the window (2 kTailGroup consecutive samples): 2 gl
+  // samples 2 gl and 2 gl + 1 of the window (2 G consecutive samples): 2 gl
   // and 2 gl + 1 more additions of the step (an addition that leaves the value
   // unchanged ends the ray there)
   bool stl0;
@@ -1804,8 +1816,8 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
     // the samples before the first segment that can reach an allocated block are
     // invalid: the ray moves on to its first sample at or beyond that segment
     const bool reach = ta < rs.ray_end && !segment_clear(sr, ow);
-    const uint32_t gm = (uint32_t)((__ballot(reach) >> gbase) & ((1ull << G) - 1ull));
-    const int f = gm != 0u ? __builtin_ctz(gm) : G;
+    const uint64_t gm = group_bits<G>(reach, gbase);
+    const int f = gm != 0ull ? __builtin_ctzll(gm) : G;
     if (f > 0) {
       if (gl == 0) ts.n_tskip++;
       last_w = 0u;
@@ -1836,14 +1848,12 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
     }
     const bool cand0 = v0 && p_v && p_sdf < 0.0f && d0 > 0.0f && fabsf(p_sdf - d0) < rs.th && fabsf(d0) < rs.th;
     const bool cand1 = v1 && v0 && d0 < 0.0f && d1 > 0.0f && fabsf(d0 - d1) < rs.th && fabsf(d1) < rs.th;
-    const uint32_t g0 = (uint32_t)((__ballot(cand0) >> gbase) & ((1ull << G) - 1ull));
-    const uint32_t g1 = (uint32_t)((__ballot(cand1) >> gbase) & ((1ull << G) - 1ull));
-    const uint32_t gi0 = (uint32_t)((__ballot(in0) >> gbase) & ((1ull << G) - 1ull));
-    const uint32_t gi1 = (uint32_t)((__ballot(in1) >> gbase) & ((1ull << G) - 1ull));
-    if ((g0 | g1) != 0u) {
+    const uint64_t g0 = group_bits<G>(cand0, gbase), g1 = group_bits<G>(cand1, gbase);
+    const uint64_t gi0 = group_bits<G>(in0, gbase), gi1 = group_bits<G>(in1, gbase);
+    if ((g0 | g1) != 0ull) {
       // the first candidate in sample order: lane f's first sample, else its second
-      const int f = __builtin_ctz(g0 | g1);
-      const bool first = (g0 >> f) & 1u;
+      const int f = __builtin_ctzll(g0 | g1);
+      const bool first = (g0 >> f) & 1ull;
       const float a0 = first ? __shfl(p_r, f, G) : __shfl(r0, f, G);
       const float ad0 = first ? __shfl(p_sdf, f, G) : __shfl(d0, f, G);
       const float b0 = first ? __shfl(r0, f, G) : __shfl(r1, f, G);
@@ -1891,8 +1901,8 @@ __device__ __forceinline__ bool tail_iteration(const TableView& t, const RenderA
     } else {
       // no accepted crossing: continue after the window's last sample (lane L's second
       // when it has one, else its first)
-      const int L = 31 - __builtin_clz(gi0 | 1u);
-      const bool second = (gi1 >> L) & 1u;
+      const int L = 63 - __builtin_clzll(gi0 | 1ull);
+      const bool second = (gi1 >> L) & 1ull;
       const float rL = second ? __shfl(r1, L, G) : __shfl(r0, L, G);
       const float dL = second ? __shfl(d1, L, G) : __shfl(d0, L, G);
       const int vL = second ? __shfl(v1 ? 1 : 0, L, G) : __shfl(v0 ? 1 : 0, L, G);
@@ -1937,6 +1947,65 @@ __device__ __forceinline__ void tail_write(const RenderArgs& ra, const TailState
   out_pos[s.pix] = o_pos;
   out_col[s.pix] = o_col;
   out_nrm[s.pix] = make_float4(MINF, MINF, MINF, MINF);
+}
+
+// The end of a wave's tail work, when no group can take another ray: the rays still
+// running (one per group of G lanes, `have` in its lanes) are finished; once they fit
+// in groups of 2 G lanes (up to kTailDrainGroup), they move to such groups.
+template <bool F16, bool DIAG, bool REF, bool DEV, int G, class Cache>
+__device__ void tail_regroup(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
+                             const uint32_t* __restrict__ col, Cache& bc, const TailState& s_in, uint64_t act,
+                             TailStats& ts, float* __restrict__ out_depth, float4* __restrict__ out_pos,
+                             float4* __restrict__ out_col, float4* __restrict__ out_nrm, uint32_t& hit,
+                             uint32_t& miss);
+
+template <bool F16, bool DIAG, bool REF, bool DEV, int G, class Cache>
+__device__ __forceinline__ void tail_drain(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
+                                           const uint32_t* __restrict__ col, Cache& bc, TailState& s, bool have,
+                                           TailStats& ts, float* __restrict__ out_depth, float4* __restrict__ out_pos,
+                                           float4* __restrict__ out_col, float4* __restrict__ out_nrm, uint32_t& hit,
+                                           uint32_t& miss) {
+  const int lane = threadIdx.x;
+  const int gl = lane & (G - 1), gbase = lane & ~(G - 1);
+  while (true) {
+    const uint64_t act = __ballot(have && gl == 0);
+    if (act == 0ull) return;
+    if constexpr (2 * G <= kTailDrainGroup) {
+      if (__popcll(act) <= 64 / (2 * G)) {
+        tail_regroup<F16, DIAG, REF, DEV, 2 * G>(t, ra, vox, col, bc, s, act, ts, out_depth, out_pos, out_col, out_nrm,
+                                                  hit, miss);
+        return;
+      }
+    }
+    if (have && tail_iteration<F16, DIAG, REF, G>(t, ra, vox, col, bc, s, gl, gbase, ts)) {
+      if (gl == 0) tail_write<DEV>(ra, s, out_depth, out_pos, out_col, out_nrm, hit, miss);
+      have = false;
+    }
+  }
+}
+
+// Moves the rays of the groups in `act` (bit l: lane l leads a group with a ray) to
+// groups of G lanes, the k-th ray to lanes [k G, (k + 1) G), and finishes them.
+template <bool F16, bool DIAG, bool REF, bool DEV, int G, class Cache>
+__device__ __forceinline__ void tail_regroup(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
+                                             const uint32_t* __restrict__ col, Cache& bc, const TailState& s_in,
+                                             uint64_t act, TailStats& ts, float* __restrict__ out_depth,
+                                             float4* __restrict__ out_pos, float4* __restrict__ out_col,
+                                             float4* __restrict__ out_nrm, uint32_t& hit, uint32_t& miss) {
+  const int lane = threadIdx.x;
+  uint64_t m = act;
+  for (int i = 0; i < lane / G; ++i) m &= m - 1ull;
+  const bool have = m != 0ull;
+  const int src = have ? __builtin_ctzll(m) : 0;
+  TailState s;
+  s.pix = __shfl(s_in.pix, src, 64);
+  s.ray = __shfl(s_in.ray, src, 64);
+  s.last_sdf = __shfl(s_in.last_sdf, src, 64);
+  s.last_alpha = __shfl(s_in.last_alpha, src, 64);
+  s.last_w = __shfl(s_in.last_w, src, 64);
+  if (have) (void)ray_setup<DEV>(ra, s.pix % (uint32_t)ra.W, s.pix / (uint32_t)ra.W, s.h, s.rs);
+  tail_drain<F16, DIAG, REF, DEV, G>(t, ra, vox, col, bc, s, have, ts, out_depth, out_pos, out_col, out_nrm, hit,
+                                     miss);
 }
 
 template <bool F16, bool DIAG, bool REF, bool DEV>
@@ -2000,8 +2069,20 @@ __global__ __launch_bounds__(64, 3) void k_render_tail(TableView t, RenderArgs r
       }
     }
     if (!__any(have)) break;
+    // After the fetch every group holds a ray or has found the queue empty (for good:
+    // entries are only taken).  Once the rays left fit in groups twice as wide, the
+    // wave regroups (tail_drain): a wave's last, long rays get more lanes instead of
+    // running many short windows while its other lanes idle (groups of 16 lanes once
+    // at most 4 rays are left: 0.517 ms against 0.533; widening on to 32 and 64 lanes
+    // measured 0.523).
+    const uint64_t act = __ballot(have && gl == 0);
+    if (__popcll(act) <= 64 / (2 * G)) {
+      tail_regroup<F16, DIAG, REF, DEV, 2 * G>(t, ra, vox, col, bc, s, act, ts, out_depth, out_pos, out_col, out_nrm,
+                                                hit, miss);
+      break;
+    }
     if (!have) continue;
-    if (tail_iteration<F16, DIAG, REF>(t, ra, vox, col, bc, s, gl, gbase, ts)) {
+    if (tail_iteration<F16, DIAG, REF, G>(t, ra, vox, col, bc, s, gl, gbase, ts)) {
       if (gl == 0) tail_write<DEV>(ra, s, out_depth, out_pos, out_col, out_nrm, hit, miss);
       have = false;
     }
