@@ -1730,15 +1730,15 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 }
 
 // Tail pass: the rays whose march ran out of steps, one ray per group of kTailGroup
-// lanes.  A group takes the skip test at the window start like a single lane would;
-// otherwise its lanes evaluate kTailGroup consecutive samples at once (the positions
-// the reference's additions reach, computed by the same additions) and the group
-// resolves them in order: the first sign change that passes the thresholds is
-// refined, and the march continues after the window's last sample.  Group g starts
-// with queue entry g; afterwards the groups of a wave that need a ray take them
-// with one atomic per wave (a single queue counter would otherwise serialise the
-// fetches), until the queue is empty.  The last workgroup clears the queue controls
-// for the next render.
+// lanes.  A group looks ahead along the ray against the occupancy map and skips what
+// it proves invalid; otherwise its lanes evaluate 2 kTailGroup consecutive samples at
+// once (the positions the reference's additions reach) and the group resolves them in
+// order: the first sign change that passes the thresholds is refined, and the march
+// continues after the window's last sample.  Group g starts with queue entry g;
+// afterwards the groups of a wave that need a ray take them with one atomic per wave
+// (a single queue counter would otherwise serialise the fetches), until the queue is
+// empty; then a wave's last rays move to wider groups (tail_drain).  The last
+// workgroup clears the queue controls for the next render.
 constexpr int kTailGroup = 8;
 // Widest group of a wave's last rays (tail_drain).
 constexpr int kTailDrainGroup = 16;
@@ -1746,8 +1746,8 @@ constexpr int kTailDrainGroup = 16;
 // segment_range).
 constexpr float kLookSeg = 16.0f;
 
-// March state of a ray in the tail pass, one per group of kTailGroup lanes (the same in
-// every lane of the group).
+// March state of a ray in the tail pass, one per group of lanes (the same in every lane
+// of the group).
 struct TailState {
   RayHit h;
   RaySetup rs;
