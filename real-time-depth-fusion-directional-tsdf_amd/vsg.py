@@ -534,6 +534,14 @@ class RayCastRender:
         self.m_isAllocated = True
         return True
 
+    def set_stream(self, hip_stream: int) -> None:
+        """Stream of this renderer's work (vsg_render_set_stream); by default the scene's.
+        A render on another stream is ordered after the scene's queued work, and the
+        scene's later work after the render."""
+        if not self.m_isAllocated:
+            raise RuntimeError("RayCastRender: not initialized")
+        _check(lib().vsg_render_set_stream(self._h, C.c_void_p(int(hip_stream))))
+
     def close(self) -> None:
         if self._h.value:
             _check(lib().vsg_render_destroy(self._h))
