@@ -837,6 +837,7 @@ vsg_status render_launch(vsg_render* r, vsg_scene* s, RenderArgs& ra, const vsg_
   ra.tail = r->tail;
   ra.tail_ctl = r->tail_ctl;
   ra.tail_seg = r->tail_seg;
+  ra.tail_par = r->calls & 1u;
   hipStream_t strm = r->has_stream ? r->stream : s->stream;
   if (strm != s->stream) {  // order after the scene's pending work
     VSG_HIP(hipEventRecord(r->ev, s->stream));

@@ -24,8 +24,9 @@ struct RenderArgs {
   int32_t mode;           // vsg_sampling: 0 the reference's trilinear sampling, 1 standard trilinear
   uint32_t budget;        // steps per lane before a ray moves to the cooperative tail pass (0: none)
   void* tail;             // tail queue (8 segments of tail_seg TailRays, one per XCD) when budget > 0
-  uint32_t* tail_ctl;     // queue controls (kTailCtlWords), counts zero between renders
+  uint32_t* tail_ctl;     // queue controls (kTailCtlWords): set tail_par for this render
   uint32_t tail_seg;      // entries per segment of the tail queue
+  uint32_t tail_par;      // queue control set of this render (render count parity)
   const float* win_obs;   // when non-null: observed depth per pixel; a ray marches only over depths within
   float win;              // win of it (and within [minD, maxD]); a pixel without one (<= 0) is not marched
 };
@@ -38,10 +39,11 @@ struct TailRay {
   uint32_t pad[3];
 };
 
-// Queue control words of a renderer's tail pass: per XCD x the count of rays the march
-// deferred into segment x (word x) and the next entry handed out dynamically (word
-// 8 + x); word 16 counts the tail pass's finished workgroups.
-constexpr uint32_t kTailCtlWords = 20;
+// Queue control words of a renderer's tail pass, two sets of 16 used by alternate
+// renders: per XCD x the count of rays the march deferred into segment x (word
+// 16 p + x) and the next entry handed out dynamically (word 16 p + 8 + x).  A render's
+// march clears the other set (the previous render's, whose tail pass has finished).
+constexpr uint32_t kTailCtlWords = 32;
 
 // Counters of the march statistics (diagnostics).
 constexpr uint32_t kMarchStats = 8;

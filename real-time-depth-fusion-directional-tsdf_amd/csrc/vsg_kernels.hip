@@ -1643,6 +1643,9 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
                                                float* __restrict__ out_depth, float4* __restrict__ out_pos,
                                                float4* __restrict__ out_col, float4* __restrict__ out_nrm,
                                                uint32_t* __restrict__ rc) {
+  // queue controls: this render uses set tail_par; the other set, the previous render's
+  // (whose tail pass has finished), is cleared for the next render
+  if (blockIdx.x == 0u && threadIdx.x < 16u) ra.tail_ctl[16u * (ra.tail_par ^ 1u) + threadIdx.x] = 0u;
   // Workgroups are dealt to the 8 XCDs in turn (workgroup b runs on XCD b % 8).  The
   // image is cut into super-tiles of kSuperTile x kSuperTile 8x8 tiles, dealt to the
   // XCDs in turn: the rays of a super-tile, which read the same blocks, share one L2,
@@ -1712,7 +1715,7 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
     // the segment of this workgroup's XCD (the tail pass's workgroups on the same XCD
     // take them, where the march left the blocks in L2)
     const uint32_t xcd = blockIdx.x & 7u;
-    if (threadIdx.x == 0) base = atomicAdd(&ra.tail_ctl[xcd], (uint32_t)__popcll(dm));
+    if (threadIdx.x == 0) base = atomicAdd(&ra.tail_ctl[16u * ra.tail_par + xcd], (uint32_t)__popcll(dm));
     base = __shfl(base, 0, 64);
     if (deferred) {
       const uint32_t rank = (uint32_t)__popcll(dm & ((1ull << threadIdx.x) - 1ull));
@@ -1737,8 +1740,9 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // continues after the window's last sample.  Group g starts with queue entry g;
 // afterwards the groups of a wave that need a ray take them with one atomic per wave
 // (a single queue counter would otherwise serialise the fetches), until the queue is
-// empty; then a wave's last rays move to wider groups (tail_drain).  The last
-// workgroup clears the queue controls for the next render.
+// empty; then a wave's last rays move to wider groups (tail_drain).  The queue
+// controls alternate between two sets by render; the march clears the one it does
+// not use (no workgroup of the tail pass needs to know it is the last).
 constexpr int kTailGroup = 8;
 // Widest group of a wave's last rays (tail_drain).
 constexpr int kTailDrainGroup = 16;
@@ -2020,7 +2024,8 @@ __global__ __launch_bounds__(64, 3) void k_render_tail(TableView t, RenderArgs r
   // workgroup b runs on XCD b % 8 and takes segment b % 8 of the queue (the rays the
   // march deferred on that XCD)
   const uint32_t xcd = blockIdx.x & 7u;
-  const uint32_t n = min(ra.tail_ctl[xcd], ra.tail_seg);
+  uint32_t* const ctl = ra.tail_ctl + 16u * ra.tail_par;
+  const uint32_t n = min(ctl[xcd], ra.tail_seg);
   const TailRay* __restrict__ queue = reinterpret_cast<const TailRay*>(ra.tail) + (size_t)xcd * ra.tail_seg;
   constexpr uint32_t kGroups = 64u / (uint32_t)G;         // groups per wave
   const uint32_t n_static = (gridDim.x >> 3) * kGroups;   // entries handed out without atomics
@@ -2049,7 +2054,7 @@ __global__ __launch_bounds__(64, 3) void k_render_tail(TableView t, RenderArgs r
       const uint64_t need = __ballot(want && gl == 0);
       if (need != 0ull) {
         uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&ra.tail_ctl[8 + xcd], (uint32_t)__popcll(need));
+        if (lane == 0) base = atomicAdd(&ctl[8 + xcd], (uint32_t)__popcll(need));
         base = __shfl(base, 0, 64);
         if (want) e = n_static + base + (uint32_t)__popcll(need & ((1ull << gbase) - 1ull));
       }
@@ -2105,12 +2110,6 @@ __global__ __launch_bounds__(64, 3) void k_render_tail(TableView t, RenderArgs r
   if (lane == 0) {
     rc[2 * blockIdx.x] = hit;
     rc[2 * blockIdx.x + 1] = miss;
-    // the last workgroup to finish clears the queue controls for the next render
-    __threadfence();
-    if (atomicAdd(&ra.tail_ctl[16], 1u) == gridDim.x - 1u) {
-#pragma unroll
-      for (int i = 0; i < 17; ++i) ra.tail_ctl[i] = 0u;
-    }
   }
 }
 
