@@ -72,7 +72,9 @@ void launch_cell_masks(hipStream_t s, bool fp16, const TableView& t, const DevSt
 // Positive-cell masks of blocks [first, first + n) into out (device, n bytes).
 void launch_export_cell_masks(hipStream_t s, const TableView& t, uint32_t first, uint32_t n, uint8_t* out);
 void launch_reset_table(hipStream_t s, const TableView& t, DevState* st);
-constexpr uint32_t kIntegrateGrid = 2048;  // workgroups of k_integrate (grid-stride over the list)
+// Workgroups of k_integrate (grid-stride over the list): 4096 measured 60.9 us per room frame
+// against 62.5 at 2048, 66.4 at 1024 and 62.6 at 8192.
+constexpr uint32_t kIntegrateGrid = 4096;
 void launch_alloc_claim(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
                         uint8_t* mark_bkt, uint32_t* part);
 void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
