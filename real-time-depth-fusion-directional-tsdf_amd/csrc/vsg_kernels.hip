@@ -1745,7 +1745,7 @@ __global__ __launch_bounds__(64, 5) void k_render(TableView t, const DevState* _
 // not use (no workgroup of the tail pass needs to know it is the last).
 constexpr int kTailGroup = 8;
 // Widest group of a wave's last rays (tail_drain).
-constexpr int kTailDrainGroup = 16;
+constexpr int kTailDrainGroup = 32;
 // Look-ahead segment of the tail pass, in voxels along the ray (at most 29: see
 // segment_range).
 constexpr float kLookSeg = 16.0f;
@@ -2078,8 +2078,8 @@ __global__ __launch_bounds__(64, 3) void k_render_tail(TableView t, RenderArgs r
     // entries are only taken).  Once the rays left fit in groups twice as wide, the
     // wave regroups (tail_drain): a wave's last, long rays get more lanes instead of
     // running many short windows while its other lanes idle (groups of 16 lanes once
-    // at most 4 rays are left: 0.517 ms against 0.533; widening on to 32 and 64 lanes
-    // measured 0.523).
+    // at most 4 rays are left, of 32 once at most 2 are: 0.495 ms against 0.497 with 16
+    // lanes at most, 0.496 with 64 and 0.533 without the drain).
     const uint64_t act = __ballot(have && gl == 0);
     if (__popcll(act) <= 64 / (2 * G)) {
       tail_regroup<F16, DIAG, REF, DEV, 2 * G>(t, ra, vox, col, bc, s, act, ts, out_depth, out_pos, out_col, out_nrm,
