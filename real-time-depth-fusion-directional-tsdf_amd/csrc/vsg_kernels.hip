@@ -1957,11 +1957,11 @@ __device__ __forceinline__ void tail_write(const RenderArgs& ra, const TailState
 // running (one per group of G lanes, `have` in its lanes) are finished; once they fit
 // in groups of 2 G lanes (up to kTailDrainGroup), they move to such groups.
 template <bool F16, bool DIAG, bool REF, bool DEV, int G, class Cache>
-__device__ void tail_regroup(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
-                             const uint32_t* __restrict__ col, Cache& bc, const TailState& s_in, uint64_t act,
-                             TailStats& ts, float* __restrict__ out_depth, float4* __restrict__ out_pos,
-                             float4* __restrict__ out_col, float4* __restrict__ out_nrm, uint32_t& hit,
-                             uint32_t& miss);
+__device__ __forceinline__ void tail_regroup(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
+                                             const uint32_t* __restrict__ col, Cache& bc, const TailState& s_in,
+                                             uint64_t act, TailStats& ts, float* __restrict__ out_depth,
+                                             float4* __restrict__ out_pos, float4* __restrict__ out_col,
+                                             float4* __restrict__ out_nrm, uint32_t& hit, uint32_t& miss);
 
 template <bool F16, bool DIAG, bool REF, bool DEV, int G, class Cache>
 __device__ __forceinline__ void tail_drain(const TableView& t, const RenderArgs& ra, const void* __restrict__ vox,
