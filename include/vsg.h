@@ -550,7 +550,9 @@ vsg_status vsg_diag_convert(const float* in, int32_t* out, uint32_t n);
 
 /* ---- kernel timing ---------------------------------------------------------- */
 
-/* Kernel ids for vsg_timing_read. */
+/* Kernel ids for vsg_timing_read.  VSG_K_ALLOC_SCAN is no longer launched (the
+ * insertion sums the resolve pass's workgroup counts itself): its slot reads zero
+ * launches and keeps the other ids' values. */
 enum {
   VSG_K_ALLOC_CLAIM = 0,
   VSG_K_ALLOC_RESOLVE = 1,

@@ -429,8 +429,6 @@ vsg_status check_frame(const char* who, vsg_scene* s, const vsg_frame* f, float 
 // of [0, npix) and [0, nwg) is written before it is read).
 vsg_status integrate_frame(vsg_scene* s, const vsg_frame* f, float truncation, const float pose[16],
                            const vsg_intrinsics* k, uint32_t* pixcnt, uint32_t* wgsum, uint32_t* claim_part) {
-  const size_t npix = (size_t)f->width * (size_t)f->height;
-  const uint32_t nwg = (uint32_t)((npix + 255) / 256);
   FrameArgs fa;
   std::memset(&fa, 0, sizeof(fa));
   for (int i = 0; i < 16; ++i) {
@@ -463,13 +461,8 @@ vsg_status integrate_frame(vsg_scene* s, const vsg_frame* f, float truncation, c
   }
   if ((r = check_launch("alloc resolve")) != VSG_OK) return r;
   {
-    TimedLaunch tl(s->stream, VSG_K_ALLOC_SCAN);
-    launch_alloc_scan(s->stream, s->st, wgsum, claim_part, nwg, fa.cset, s->t.num_blocks);
-  }
-  if ((r = check_launch("alloc scan")) != VSG_OK) return r;
-  {
     TimedLaunch tl(s->stream, VSG_K_ALLOC_INSERT);
-    launch_alloc_insert(s->stream, s->t, s->st, fa, pts, pixcnt, wgsum);
+    launch_alloc_insert(s->stream, s->t, s->st, fa, pts, pixcnt, wgsum, claim_part);
   }
   if ((r = check_launch("alloc insert")) != VSG_OK) return r;
   {

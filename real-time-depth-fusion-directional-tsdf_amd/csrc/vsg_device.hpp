@@ -79,7 +79,7 @@ struct DevState {
   uint32_t n_alloc;       // live blocks (block pointers are 0..n_alloc-1)
   uint32_t n_alloc_prev;  // live blocks before the current frame's insertions
   uint32_t frames;
-  uint32_t pad;
+  uint32_t n_alloc_next;  // live blocks after them (made live by the compaction)
   FrameCounters set[2];
 };
 

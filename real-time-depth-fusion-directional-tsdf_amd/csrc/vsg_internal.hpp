@@ -79,10 +79,8 @@ void launch_alloc_claim(hipStream_t s, const TableView& t, const FrameArgs& fa, 
                         uint8_t* mark_bkt, uint32_t* part);
 void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa, const float* pts, uint32_t* pixcnt,
                           uint32_t* wgsum);
-void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, const uint32_t* part, uint32_t nwg, int32_t cset,
-                       uint32_t num_blocks);
 void launch_alloc_insert(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
-                         const uint32_t* pixcnt, const uint32_t* wgoff);
+                         const uint32_t* pixcnt, const uint32_t* wgsum, const uint32_t* part);
 void launch_compact(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, uint32_t* vis,
                     uint32_t grid);
 void launch_integrate(hipStream_t s, bool fp16, const TableView& t, DevState* st, const FrameArgs& fa,

@@ -6,12 +6,13 @@
 //                    (pixel << 10 | DDA step) on the bucket's claim word
 //   k_alloc_resolve  pixels that claimed re-walk their band and count the steps
 //                    whose key won; workgroup sums
-//   k_alloc_scan     one workgroup: exclusive scan of the workgroup sums, new
-//                    block count, heap-exhaustion count, next frame's counters
-//   k_alloc_insert   winners insert their block into the first free slot of the
-//                    bucket with pointer = blocks before + rank in key order
-//   k_compact        one thread per live block: conservative "can this frame
-//                    update it" test, wave ballot + prefix-sum stream compaction
+//   k_alloc_insert   every workgroup sums its predecessors' winners itself; winners
+//                    insert their block into the first free slot of the bucket with
+//                    pointer = blocks before + rank in key order; workgroup 0 sets
+//                    the frame's counters and clears the next frame's
+//   k_compact        makes the new block count live; one thread per live block:
+//                    conservative "can this frame update it" test, wave ballot +
+//                    prefix-sum stream compaction
 //   k_integrate      one 512-thread workgroup per listed block, one voxel per
 //                    thread, grid-stride over the list
 // and for rendering:
@@ -188,36 +189,24 @@ __global__ __launch_bounds__(256) void k_alloc_resolve(TableView t, FrameArgs fa
   if (threadIdx.x == 0) wgsum[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-__global__ __launch_bounds__(1024) void k_alloc_scan(DevState* st, uint32_t* __restrict__ wg,
-                                                     const uint32_t* __restrict__ part, uint32_t nwg, int32_t cset,
-                                                     uint32_t num_blocks) {
-  const uint32_t per = (nwg + 1023u) / 1024u;
-  const uint32_t b0 = threadIdx.x * per;
-  uint32_t local = 0;
+// The frame counters from the claim kernel's per-workgroup partials and the winner
+// total, by workgroup 0 of k_alloc_insert (all of its threads): valid and claiming
+// pixels, out-of-range and non-finite points, the depth range, winners, heap
+// failures; the block count after the insertions (n_alloc_next, published by the
+// compaction) and the next frame's counter set cleared.
+__device__ __forceinline__ void frame_counters(DevState* st, int32_t cset, const uint32_t* __restrict__ part,
+                                               uint32_t nwg, uint32_t prev, uint32_t winners, uint32_t ok) {
   uint32_t cv = 0, cc = 0, co = 0, cn = 0, zmn = 0xFFFFFFFFu, zmx = 0u;
-  for (uint32_t k = 0; k < per; ++k) {
-    if (b0 + k < nwg) {
-      local += wg[b0 + k];
-      const uint32_t* p = part + (size_t)(b0 + k) * kClaimPart;
-      cv += p[0];
-      cc += p[1];
-      co += p[2];
-      cn += p[3];
-      zmn = min(zmn, p[4]);
-      zmx = max(zmx, p[5]);
-    }
+  for (uint32_t k = threadIdx.x; k < nwg; k += 256u) {
+    const uint32_t* p = part + (size_t)k * kClaimPart;
+    cv += p[0];
+    cc += p[1];
+    co += p[2];
+    cn += p[3];
+    zmn = min(zmn, p[4]);
+    zmx = max(zmx, p[5]);
   }
-  uint32_t total = 0;
-  uint32_t run = block_excl_scan<1024>(local, &total);
-  for (uint32_t k = 0; k < per; ++k) {
-    if (b0 + k < nwg) {
-      const uint32_t x = wg[b0 + k];
-      wg[b0 + k] = run;
-      run += x;
-    }
-  }
-  // frame counters: one workgroup-wide reduction of the claim partials
-  __shared__ uint32_t red[16][6];
+  __shared__ uint32_t red[4][6];
   cv = wave_sum(cv);
   cc = wave_sum(cc);
   co = wave_sum(co);
@@ -237,7 +226,7 @@ __global__ __launch_bounds__(1024) void k_alloc_scan(DevState* st, uint32_t* __r
   if (threadIdx.x == 0) {
     FrameCounters& fc = st->set[cset];
     uint32_t a[6] = {0, 0, 0, 0, 0xFFFFFFFFu, 0u};
-    for (int w = 0; w < 16; ++w) {
+    for (int w = 0; w < 4; ++w) {
       for (int k = 0; k < 4; ++k) a[k] += red[w][k];
       a[4] = min(a[4], red[w][4]);
       a[5] = max(a[5], red[w][5]);
@@ -248,15 +237,12 @@ __global__ __launch_bounds__(1024) void k_alloc_scan(DevState* st, uint32_t* __r
     fc.nonfinite = a[3];
     fc.zmin_key = a[4];
     fc.zmax_key = a[5];
-    const uint32_t prev = st->n_alloc;
-    const uint32_t room = num_blocks - prev;
-    const uint32_t ok = total < room ? total : room;
     st->n_alloc_prev = prev;
-    st->n_alloc = prev + ok;
+    st->n_alloc_next = prev + ok;
     st->frames += 1u;
-    fc.winners = total;
-    fc.claimed_buckets = total;
-    fc.heap_fail = total - ok;
+    fc.winners = winners;
+    fc.claimed_buckets = winners;
+    fc.heap_fail = winners - ok;
     FrameCounters& nx = st->set[cset ^ 1];
     nx.valid_pixels = 0;
     nx.claim_pixels = 0;
@@ -272,17 +258,33 @@ __global__ __launch_bounds__(1024) void k_alloc_scan(DevState* st, uint32_t* __r
   }
 }
 
+// Each workgroup sums the resolve's per-workgroup winner counts itself (its
+// predecessors' for its pointer offset, all for the frame's total): no separate scan
+// pass.  The live block count is read here and changes only when the compaction
+// publishes the new one.
 __global__ __launch_bounds__(256) void k_alloc_insert(TableView t, DevState* st, FrameArgs fa,
                                                       const float* __restrict__ pts, const uint32_t* __restrict__ pixcnt,
-                                                      const uint32_t* __restrict__ wgoff) {
+                                                      const uint32_t* __restrict__ wgsum,
+                                                      const uint32_t* __restrict__ part, uint32_t nwg) {
+  uint32_t before = 0, all = 0;
+  for (uint32_t k = threadIdx.x; k < nwg; k += 256u) {
+    const uint32_t w = wgsum[k];
+    all += w;
+    if (k < blockIdx.x) before += w;
+  }
+  uint32_t off, total;
+  (void)block_excl_scan<256>(before, &off);
+  (void)block_excl_scan<256>(all, &total);
+  const uint32_t prev = st->n_alloc;
+  const uint32_t room = t.num_blocks - prev;
+  if (blockIdx.x == 0u) frame_counters(st, fa.cset, part, nwg, prev, total, total < room ? total : room);
   const int pix = blockIdx.x * 256 + threadIdx.x;
   const int npix = fa.W * fa.H;
   const uint32_t n = pix < npix ? pixcnt[pix] : 0u;
   uint32_t tot;
   uint32_t rank = block_excl_scan<256>(n, &tot);
   if (n == 0u) return;
-  rank += wgoff[blockIdx.x];
-  const uint32_t prev = st->n_alloc_prev;
+  rank += off;
   const float Px = pts[3 * pix], Py = pts[3 * pix + 1], Pz = pts[3 * pix + 2];
   float3 lo, hi;
   if (!pixel_band(fa, Px, Py, Pz, &lo, &hi)) return;
@@ -367,7 +369,9 @@ __device__ __forceinline__ bool block_maybe_visible(const FrameArgs& fa, int4 b,
 }
 
 __global__ __launch_bounds__(256) void k_compact(TableView t, DevState* st, FrameArgs fa, uint32_t* __restrict__ vis) {
-  const uint32_t n = st->n_alloc;
+  // the frame's block count after the insertions becomes the live count here
+  const uint32_t n = st->n_alloc_next;
+  if (blockIdx.x == 0u && threadIdx.x == 0u) st->n_alloc = n;
   FrameCounters& fc = st->set[fa.cset];
   const bool any_valid = fc.zmin_key != 0xFFFFFFFFu || fc.nonfinite != 0u;
   const bool cull = fa.cull_ok != 0 && fc.nonfinite == 0u;
@@ -2616,6 +2620,7 @@ __global__ __launch_bounds__(256) void k_reset_table(TableView t, DevState* st) 
   if (tid == 0) {
     st->n_alloc = 0;
     st->n_alloc_prev = 0;
+    st->n_alloc_next = 0;
     st->frames = 0;
     for (int s = 0; s < 2; ++s) {
       FrameCounters& f = st->set[s];
@@ -3505,15 +3510,10 @@ void launch_alloc_resolve(hipStream_t s, const TableView& t, const FrameArgs& fa
   hipLaunchKernelGGL(k_alloc_resolve, dim3(nwg), dim3(256), 0, s, t, fa, pts, pixcnt, wgsum);
 }
 
-void launch_alloc_scan(hipStream_t s, DevState* st, uint32_t* wgsum, const uint32_t* part, uint32_t nwg, int32_t cset,
-                       uint32_t num_blocks) {
-  hipLaunchKernelGGL(k_alloc_scan, dim3(1), dim3(1024), 0, s, st, wgsum, part, nwg, cset, num_blocks);
-}
-
 void launch_alloc_insert(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, const float* pts,
-                         const uint32_t* pixcnt, const uint32_t* wgoff) {
+                         const uint32_t* pixcnt, const uint32_t* wgsum, const uint32_t* part) {
   const uint32_t nwg = (uint32_t)((fa.W * fa.H + 255) / 256);
-  hipLaunchKernelGGL(k_alloc_insert, dim3(nwg), dim3(256), 0, s, t, st, fa, pts, pixcnt, wgoff);
+  hipLaunchKernelGGL(k_alloc_insert, dim3(nwg), dim3(256), 0, s, t, st, fa, pts, pixcnt, wgsum, part, nwg);
 }
 
 void launch_compact(hipStream_t s, const TableView& t, DevState* st, const FrameArgs& fa, uint32_t* vis,
@@ -3953,6 +3953,7 @@ __global__ __launch_bounds__(256) void k_reindex(TableView t, uint32_t n) {
 __global__ void k_set_alloc(DevState* st, uint32_t n) {
   st->n_alloc = n;
   st->n_alloc_prev = n;
+  st->n_alloc_next = n;
 }
 
 // Stream-in status codes (per input block).
