@@ -28,6 +28,8 @@ VSG_OK = 0
 STATUS = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "HIP_ERROR", 4: "NOT_READY"}
 STORE_FP32 = 0
 STORE_FP16 = 1
+# timing ids of vsg_timing_read, in the C enum's order; "alloc_scan" is no longer
+# launched (the insertion sums the workgroup counts itself) and reads zero launches
 KERNELS = ["alloc_claim", "alloc_resolve", "alloc_scan", "alloc_insert", "compact", "integrate", "raycast",
            "reset", "adjoint", "gauss_newton", "extract", "depthgen", "mesh"]
 
